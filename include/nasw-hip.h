@@ -1,0 +1,208 @@
+/*
+ * nasw-hip.h -- C-ABI of the MI355X spliced-DP core.
+ *
+ * Drop-in boundary for the reference's DP layer: every type and constant below has
+ * the same layout and value as the reference header (nasw.h:33-82, nasw.h:157-184),
+ * and every ns_* entry point has the same signature, argument meaning and error
+ * behaviour as the reference function it replaces (cited per declaration). A caller
+ * written against nasw.h (align.c, map.c, hmm.c) links against libnasw_hip.so instead
+ * of nasw-sse.o/nasw-tab.o/nasw-opencl.o/nasw-ext-gpu.o/hmm-gpu.o without source changes.
+ *
+ * All DP work runs on the GPU (hand-written gfx950 kernels); there is no CPU DP
+ * path in this library. Entry points that the reference lets "fail back to the CPU"
+ * (return 0 / NULL / INT32_MIN) only do so when no GPU is usable, and then report why
+ * through ns_gpu_error().
+ *
+ * The nsh_* entry points at the end have no counterpart in the reference: they expose
+ * the device-resident batch engine (inputs already in HBM) used by the benchmarks.
+ */
+#ifndef NASW_HIP_H
+#define NASW_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+#include <string.h>
+
+/* CIGAR operators (nasw.h:33-42) */
+#define NS_CIGAR_M	0
+#define NS_CIGAR_I	1
+#define NS_CIGAR_D	2
+#define NS_CIGAR_N	3  /* phase-0 intron */
+#define NS_CIGAR_F	10 /* frameshift deletion */
+#define NS_CIGAR_G	11 /* frameshift match */
+#define NS_CIGAR_U	12 /* phase-1 intron */
+#define NS_CIGAR_V	13 /* phase-2 intron */
+#define NS_CIGAR_E	14 /* placeholder, unused */
+#define NS_CIGAR_STR   "MIDNSHP=XBFGUVE"
+
+/* option flags and splice models (nasw.h:44-53) */
+#define NS_F_CIGAR      0x1
+#define NS_F_EXT_LEFT   0x2
+#define NS_F_EXT_RIGHT  0x4
+#define NS_F_GPU        0x8
+
+#define NS_S_NONE       0
+#define NS_S_GENERIC    1
+#define NS_S_MAMMAL     2
+
+#define NS_SPSC_OFFSET 64
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* lookup tables (nasw.h:55-58; filled by ns_make_tables) */
+extern char *ns_tab_nt_i2c, *ns_tab_aa_i2c;
+extern uint8_t ns_tab_a2r[22], ns_tab_nt4[256], ns_tab_aa20[256], ns_tab_aa13[256];
+extern uint8_t ns_tab_codon[64], ns_tab_codon13[64];
+extern int8_t ns_mat_blosum62[484];
+
+/* alignment options; field order and types as nasw.h:62-75 */
+typedef struct {
+	int32_t flag;
+	int32_t go, ge, io, fs;     /* gap open, gap extension, intron open, frameshift */
+	int32_t xdrop, end_bonus;   /* extension x-drop; bonus for reaching the protein end */
+	int32_t asize;              /* alphabet size (22) */
+	int32_t gpu_device;
+	int64_t gpu_min_cells;
+	int32_t sp[6];              /* splice penalties: 0 pos3, 1 GC-AC, 2 AT-AC, 3 other, 4 pos0, 5 poly-Y */
+	int32_t sp_null_bonus;      /* negative */
+	float ie_coef;
+	const int8_t *sc;           /* asize x asize substitution matrix */
+	const int8_t *pssm;         /* optional al x asize position-specific scores */
+	uint8_t *nt4, *aa20, *codon;
+} ns_opt_t;
+
+/* result; layout as nasw.h:77-82 */
+typedef struct {
+	int32_t n_cigar, m_cigar;
+	int32_t nt_len, aa_len;
+	int32_t score;
+	uint32_t *cigar;            /* grown with krealloc(km, ...), freed by the caller with kfree(km, ...) */
+} ns_rst_t;
+
+/* ---- tables and options (nasw-tab.c) ---- */
+int ns_make_tables(int codon_type);                                   /* nasw-tab.c:85-107 */
+void ns_opt_init(ns_opt_t *opt);                                      /* nasw-tab.c:131-147 */
+void ns_opt_set_sp(ns_opt_t *opt, int32_t model);                     /* nasw-tab.c:124-129 */
+void ns_set_stop_sc(int32_t asize, int8_t *mat, int8_t score);        /* nasw-tab.c:149-156 */
+
+/* ---- single-problem DP (nasw-sse.c); each call runs one GPU launch ---- */
+void ns_global_gs16(void *km, const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, ns_rst_t *r);  /* nasw-sse.c:774-777 */
+void ns_global_gs32(void *km, const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, ns_rst_t *r);  /* nasw-sse.c:975-978 */
+void ns_global_gs16b(void *km, const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, const uint8_t *ss, ns_rst_t *r); /* nasw-sse.c:763-772 */
+void ns_global_gs32b(void *km, const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, const uint8_t *ss, ns_rst_t *r); /* nasw-sse.c:779-973 */
+void ns_global_gs_auto(void *km, const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, const uint8_t *ss, ns_rst_t *r); /* nasw-sse.c:980-999 */
+
+/* ---- GPU global-DP API (nasw.h:140-152; nasw-cuda.cu:516-611, 1068-1145) ---- */
+typedef struct ns_gpu_request_s ns_gpu_request_t;
+int ns_gpu_init(int32_t device);
+const char *ns_gpu_error(void);
+const char *ns_gpu_name(void);
+uint64_t ns_gpu_dispatch_count(void);
+uint64_t ns_gpu_launch_count(void);
+double ns_gpu_h2d_seconds(void);
+double ns_gpu_kernel_seconds(void);
+double ns_gpu_d2h_seconds(void);
+ns_gpu_request_t *ns_gpu_submit(void *km, const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, const uint8_t *ss);
+int ns_gpu_wait(ns_gpu_request_t *request, ns_rst_t *r);
+int ns_gpu_global(void *km, const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, const uint8_t *ss, ns_rst_t *r);
+
+/* ---- gapless X-drop prefilter (nasw.h:157-184; nasw-cuda.cu:1334-1412) ---- */
+typedef struct {
+	int64_t target_offset, target_length;
+	int64_t query_offset;
+	int32_t query_length;
+	int32_t anchor_offset, anchor_count;
+	int32_t reverse;
+} ns_gpu_gapless_candidate_t;
+typedef struct {
+	int64_t target_block;
+	int32_t query_end;
+	int32_t reserved;
+} ns_gpu_gapless_anchor_t;
+typedef struct {
+	uint64_t batches, candidates, anchors, seed_positions;
+} ns_gpu_gapless_profile_t;
+void ns_gpu_gapless_profile(ns_gpu_gapless_profile_t *profile);
+int ns_gpu_gapless_supported(void);
+void ns_gpu_gapless_reset_target(void);
+int ns_gpu_gapless_score_batch(const uint8_t *packed_nt, int64_t packed_nt_bases,
+		const uint8_t *queries, int64_t query_aa,
+		const ns_gpu_gapless_candidate_t *candidates, int32_t n_candidates,
+		const ns_gpu_gapless_anchor_t *anchors, int64_t n_anchors,
+		int32_t block_width, int32_t kmer, int32_t xdrop,
+		const int8_t *score_matrix, int32_t asize, const uint8_t *codon,
+		int32_t *scores);
+
+/* ---- extension building blocks (nasw-sse.c:744-761) ---- */
+uint8_t *ns_ext_prep(void *km, const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, const uint8_t *ss, uint8_t **aas, int8_t **donor, int8_t **acceptor);
+int32_t ns_ext_stripe_width(const ns_opt_t *opt, int32_t al);
+int32_t ns_ext_ie_penalty(const ns_opt_t *opt, int32_t d);
+
+/* ---- batched extension DP (nasw.h:196-212; nasw-ext-gpu.c:931-1114) ---- */
+typedef struct ns_ext_batch_s ns_ext_batch_t;
+ns_ext_batch_t *ns_ext_batch_init(void *km);
+int32_t ns_ext_batch_add(ns_ext_batch_t *b, const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, const uint8_t *ss);
+int32_t ns_ext_batch_flush(ns_ext_batch_t *b);
+int ns_ext_batch_ready(ns_ext_batch_t *b, int32_t id);
+void ns_ext_batch_wait(ns_ext_batch_t *b);
+const ns_rst_t *ns_ext_batch_get(ns_ext_batch_t *b, int32_t id);
+void ns_ext_batch_destroy(ns_ext_batch_t *b);
+uint64_t ns_ext_gpu_launch_count(void);
+uint64_t ns_ext_gpu_request_count(void);
+
+/* kalloc ABI of the caller (kalloc.h:14-32): km == NULL means libc malloc. The library carries
+ * weak fall-backs for km == NULL; a caller linking the reference's kalloc.c overrides them. */
+void *kmalloc(void *km, size_t size);
+void *krealloc(void *km, void *ptr, size_t size);
+void *kcalloc(void *km, size_t count, size_t size);
+void kfree(void *km, void *ptr);
+
+/* CIGAR append with run merging (F and G never merge); same semantics as nasw.h:219-229 */
+static inline uint32_t *ns_push_cigar(void *km, int32_t *n_cigar, int32_t *m_cigar, uint32_t *cigar, uint32_t op, int32_t len)
+{
+	if (*n_cigar == 0 || op != (cigar[(*n_cigar) - 1]&0xf) || op == NS_CIGAR_F || op == NS_CIGAR_G) {
+		if (*n_cigar == *m_cigar) {
+			(*m_cigar) += ((*m_cigar)>>1) + 8;
+			cigar = (uint32_t*)krealloc(km, cigar, (size_t)(*m_cigar) * sizeof(uint32_t));
+		}
+		cigar[(*n_cigar)++] = (uint32_t)len<<4 | op;
+	} else cigar[(*n_cigar)-1] += (uint32_t)len<<4;
+	return cigar;
+}
+static inline void ns_rst_init(ns_rst_t *r) { memset(r, 0, sizeof(*r)); }
+
+/* ======================================================================================
+ * Device-resident batch engine (no counterpart in the reference).
+ *
+ * A plan owns HBM copies of many problems' raw inputs, sorted into length buckets.
+ * nsh_ext_plan_run() prepares (codon translation, splice signals, query profiles) and
+ * solves every problem with one persistent launch per bucket group; only the 12-byte
+ * results travel back. Results equal ns_global_gs16b(flag = EXT_LEFT|EXT_RIGHT) of the
+ * same request exactly.
+ * ====================================================================================== */
+typedef struct nsh_ext_plan_s nsh_ext_plan_t;
+typedef struct { int32_t nt_len, aa_len, score; } nsh_ext_result_t;
+
+/* All problems of a plan share opt (except the EXT_LEFT/EXT_RIGHT bit, taken per problem
+ * from dirs[i] when dirs != NULL). ns/ss are concatenated raw nt strings (ASCII or 0..4),
+ * as the concatenated proteins; ss may be NULL. Returns NULL and sets ns_gpu_error() on failure. */
+nsh_ext_plan_t *nsh_ext_plan_create(int32_t n, const char *ns, const int64_t *ns_off, const int32_t *nl,
+		const char *as, const int64_t *as_off, const int32_t *al, const uint8_t *ss,
+		const uint8_t *dirs, const ns_opt_t *opt);
+int nsh_ext_plan_run(nsh_ext_plan_t *p);                      /* enqueue on the plan's stream; 1 ok, 0 error */
+int nsh_ext_plan_sync(nsh_ext_plan_t *p);                     /* wait for the last run */
+int nsh_ext_plan_results(nsh_ext_plan_t *p, nsh_ext_result_t *out); /* copy results (input order) */
+double nsh_ext_plan_cells(const nsh_ext_plan_t *p, int32_t kind);   /* 0: sum nl*al, 1: sum (rows done)*al after a run */
+float nsh_ext_plan_last_ms(nsh_ext_plan_t *p, int32_t which);  /* 0: whole run, 1: DP kernels only (HIP events) */
+void nsh_ext_plan_destroy(nsh_ext_plan_t *p);
+
+int nsh_device_synchronize(void);
+const char *nsh_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

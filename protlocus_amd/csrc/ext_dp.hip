@@ -1,0 +1,412 @@
+// Batched spliced extension DP on gfx950 (score-only ns_global_gs16b with NS_F_EXT_LEFT/RIGHT;
+// reference semantics: nasw-sse.c:360-459 and 596-741, recurrence nasw-sse.c:19-26).
+//
+// Execution model
+//   * One wave64 solves one problem; waves are persistent and pull problems from a queue
+//     ordered by decreasing cost, so long problems start first and the tail is short.
+//   * Row-synchronous: each lane owns C consecutive columns in each of two column halves
+//     ([0, 64C) and [64C, 128C)); the two halves are packed into the 16-bit lanes of a
+//     32-bit register and advanced together with VOP3P saturating instructions
+//     (v_pk_add_i16/v_pk_sub_i16 clamp, v_pk_max_i16), i.e. two cells per instruction with
+//     exactly the reference's int16 saturation.
+//   * Within a row every cell except the horizontal insertion term depends only on previous
+//     rows. Each lane runs the insertion chain over its own columns with a zero carry, then
+//     one max-plus scan across lanes (DPP row_shr / row_bcast) yields every lane's exact
+//     incoming insertion, which is folded into the lane's cells (max with a decaying carry).
+//     Saturating subtraction is monotone, so this equals the reference's striped pass plus
+//     lazy-F bit for bit.
+//   * Per row the wave reduces the row maximum (padded columns included, columns beyond the
+//     reference's padding excluded), applies the ie_coef*log2 length penalty from a host
+//     table, updates the best row (snapshot of its H values in registers) and checks x-drop.
+//   * The query profile lives in LDS as int8 (22 residues x 128C columns); the per-row
+//     residue, donor[i+1] and acceptor[i] come from a 4-byte record per row prepared on the
+//     device; records and penalties are streamed 64 rows at a time into one VGPR each.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <type_traits>
+#include <vector>
+#include "nsh_internal.h"
+#include "ext_dp_types.h"
+
+namespace {
+using namespace nsh_ext;
+constexpr int CMAX_SMALL = kCmaxSmall, CMAX = kCmax;
+#define DEV static __device__ __forceinline__
+
+__global__ void fill_i32_kernel(int32_t *p, int32_t n, int32_t v)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------------------------
+// Device-side input preparation (nasw-sse.c:95-214): one work-group per problem writes
+// rec[i] = nas[i] | donor[i+1] << 8 | acceptor[i] << 16 and the DP-order protein codes.
+// ------------------------------------------------------------------------------------
+struct PrepProb {
+	int64_t ns_off, as_off, rec_off, aa_off;
+	int32_t nl, al, left, io;
+	int32_t has_ss, pad_[3];
+};
+struct PrepConst {
+	uint8_t nt4[256], aa20[256], codon[64];
+	int32_t sp[6], sp_null;
+};
+
+DEV int wrap8(int v) { return (int)(int8_t)(uint8_t)(v & 0xff); }
+
+struct PrepView {
+	const uint8_t *ns, *ss; int nl, left, has_ss, max_spsc;
+	const uint8_t *nt4;
+	const int32_t *sp; int sp_null;
+	__device__ int b(int x) const { return nt4[ns[left? nl - 1 - x : x]]; } // nt4 code in DP order
+	__device__ int spsc(int s) const { int v = (int)(int8_t)(s >> 1) - 64; return v > max_spsc? max_spsc : v; }
+	__device__ int donor(int p) const {
+		int d;
+		if (!left) {
+			if (p < nl - 3) {
+				int t = 3;
+				const int b1 = b(p + 1), b2 = b(p + 2);
+				if (b1 == 2 && b2 == 3) t = (p + 3 < nl && (b(p + 3) == 0 || b(p + 3) == 2))? (b(p) == 2? -1 : 4) : 0;
+				else if (b1 == 2 && b2 == 1 && b(p) == 2) t = 1;
+				else if (b1 == 0 && b2 == 3) t = 2;
+				d = wrap8(t < 0? 0 : sp[t]);
+			} else d = wrap8(sp[3]);
+			if (has_ss && p <= nl - 2) {
+				const int s = ss[p + 1];
+				if (s == 0xff) d = wrap8(d - sp_null);
+				else if (!(s & 1)) d = wrap8(d - spsc(s));
+			}
+		} else {
+			if (p < nl - 3) {
+				int t = 3, pen_y = 0;
+				const int b1 = b(p + 1), b2 = b(p + 2);
+				if (b1 == 2 && b2 == 0) {
+					t = (p + 3 < nl && (b(p + 3) == 1 || b(p + 3) == 3))? -1 : 0;
+					for (int j = p + 5; j < nl && j < p + 8; ++j) { const int x = b(j); if (x != 1 && x != 3) pen_y += sp[5]; }
+				} else if (b1 == 1 && b2 == 0) t = 2;
+				d = wrap8(t < 0? 0 : sp[t]);
+				if (t == -1 || t == 0) d = wrap8(d + pen_y);
+			} else d = wrap8(sp[3]);
+			if (has_ss && p <= nl - 1) {
+				const int s = ss[nl - 1 - p];
+				if (s == 0xff) d = wrap8(d - sp_null);
+				else if (s & 1) d = wrap8(d - spsc(s));
+			}
+		}
+		return d;
+	}
+	__device__ int acceptor(int p) const {
+		int a;
+		if (!left) {
+			if (p >= 1 && p < nl) {
+				int t = 3, pen_y = 0;
+				const int bm1 = b(p - 1), b0 = b(p);
+				if (bm1 == 0 && b0 == 2) {
+					t = (p >= 2 && (b(p - 2) == 1 || b(p - 2) == 3))? -1 : 0;
+					for (int j = p - 4; j >= 0 && j > p - 7; --j) { const int x = b(j); if (x != 1 && x != 3) pen_y += sp[5]; }
+				} else if (bm1 == 0 && b0 == 1) t = 2;
+				a = wrap8(t < 0? 0 : sp[t]);
+				if (t == -1 || t == 0) a = wrap8(a + pen_y);
+			} else a = wrap8(sp[3]);
+			if (has_ss && p <= nl - 2) {
+				const int s = ss[p + 1];
+				if (s == 0xff) a = wrap8(a - sp_null);
+				else if (s & 1) a = wrap8(a - spsc(s));
+			}
+		} else {
+			if (p >= 1 && p < nl) {
+				int t = 3;
+				const int bm1 = b(p - 1), b0 = b(p);
+				if (bm1 == 3 && b0 == 2) t = (p >= 2 && (b(p - 2) == 0 || b(p - 2) == 2))? ((p + 1 < nl && b(p + 1) == 2)? -1 : 4) : 0;
+				else if (bm1 == 1 && b0 == 2 && p + 1 < nl && b(p + 1) == 1) t = 1;
+				else if (bm1 == 3 && b0 == 0) t = 2;
+				a = wrap8(t < 0? 0 : sp[t]);
+			} else a = wrap8(sp[3]);
+			if (has_ss && p <= nl - 1) {
+				const int s = ss[nl - 1 - p];
+				if (s == 0xff) a = wrap8(a - sp_null);
+				else if (!(s & 1)) a = wrap8(a - spsc(s));
+			}
+		}
+		return a;
+	}
+};
+
+__global__ void __launch_bounds__(256) ext_prep_kernel(const PrepProb *pp, const uint8_t *raw_ns, const uint8_t *raw_ss,
+		const uint8_t *raw_as, uint32_t *rec, uint8_t *aas, const PrepConst *pcg)
+{
+	__shared__ PrepConst pc;
+	const PrepProb P = pp[blockIdx.x];
+	for (int k = threadIdx.x; k < (int)sizeof(PrepConst); k += blockDim.x) ((uint8_t*)&pc)[k] = ((const uint8_t*)pcg)[k];
+	__syncthreads();
+	PrepView v;
+	v.ns = raw_ns + P.ns_off, v.ss = P.has_ss? raw_ss + P.ns_off : nullptr;
+	v.nl = P.nl, v.left = P.left, v.has_ss = P.has_ss, v.max_spsc = (P.io + 1) / 2 - 1;
+	v.nt4 = pc.nt4, v.sp = pc.sp, v.sp_null = pc.sp_null;
+	const int X = pc.aa20['X'];
+	const uint8_t *as = raw_as + P.as_off;
+	for (int j = threadIdx.x; j < P.al; j += blockDim.x)
+		aas[P.aa_off + j] = pc.aa20[as[P.left? P.al - 1 - j : j]];
+	for (int i = threadIdx.x; i < P.nl; i += blockDim.x) {
+		// residue of the codon ending at DP position i (left: reversed, shifted by two)
+		int res = X;
+		const int k = P.left? P.nl + 1 - i : i;
+		if (i >= 2 && k >= 2) {
+			const int c0 = pc.nt4[v.ns[k - 2]], c1 = pc.nt4[v.ns[k - 1]], c2 = pc.nt4[v.ns[k]];
+			if (c0 < 4 && c1 < 4 && c2 < 4) res = pc.codon[c0 << 4 | c1 << 2 | c2];
+		}
+		const int d = v.donor(i + 1), ac = v.acceptor(i);
+		rec[P.rec_off + i] = (uint32_t)res | ((uint32_t)(d & 0xff) << 8) | ((uint32_t)(ac & 0xff) << 16);
+	}
+}
+
+} // namespace
+
+// ======================================================================================
+// host side: plans
+// ======================================================================================
+#define HIPOK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { nsh_set_error("%s: %s", #x, hipGetErrorString(e_)); goto fail; } } while (0)
+
+struct nsh_ext_plan_s {
+	int32_t n = 0;
+	hipStream_t stream = nullptr;
+	hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+	// device buffers
+	uint8_t *d_ns = nullptr, *d_ss = nullptr, *d_as = nullptr, *d_aas = nullptr;
+	uint32_t *d_rec = nullptr;
+	int8_t *d_mat = nullptr, *d_pssm = nullptr;
+	int32_t *d_pen = nullptr, *d_counter = nullptr, *d_out = nullptr;
+	ExtProb *d_prob = nullptr;
+	PrepProb *d_prep = nullptr;
+	PrepConst *d_pc = nullptr;
+	ExtArgs args[2] = {};        // [0]: C <= 8 problems, [1]: 8 < C <= 16
+	int32_t grid[2] = { 0, 0 };
+	double cells_window = 0;     // sum nl*al
+	std::vector<int32_t> al;     // per input index
+	std::vector<int32_t> out_h;  // 4 per problem
+	int ran = 0;
+};
+
+static void plan_free_dev(nsh_ext_plan_t *p)
+{
+	void *bufs[] = { p->d_ns, p->d_ss, p->d_as, p->d_aas, p->d_rec, p->d_mat, p->d_pssm, p->d_pen, p->d_counter,
+		p->d_out, p->d_prob, p->d_prep, p->d_pc };
+	for (void *b : bufs) if (b) (void)hipFree(b);
+}
+
+template <typename T>
+static hipError_t dalloc(T **p, size_t n) { return hipMalloc((void**)p, n * sizeof(T) + 16); }
+
+nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_opt_t *opt)
+{
+	if (!nsh_ensure_device()) return nullptr;
+	if (n < 1) { nsh_set_error("empty extension plan"); return nullptr; }
+	nsh_ext_plan_t *p = new nsh_ext_plan_s();
+	p->n = n;
+	int64_t l_ns = 0, l_as = 0, l_pssm = 0;
+	int32_t max_nl = 2, any_ss = 0, n_small = 0;
+	std::vector<ExtProb> prob(n);
+	std::vector<PrepProb> prep(n);
+	std::vector<uint8_t> h_ns, h_ss, h_as;
+	std::vector<int8_t> h_pssm;
+	std::vector<int32_t> pen;
+	std::vector<int32_t> order(n);
+	PrepConst pc;
+	p->al.resize(n);
+	const int32_t asize = opt->asize;
+	if (asize != 22) { nsh_set_error("asize %d not supported (22 expected)", asize); delete p; return nullptr; }
+	for (int t = 0; t < 64; ++t)
+		if (opt->codon[t] >= 22) { nsh_set_error("codon table entry %d out of range", t); delete p; return nullptr; }
+	for (int t = 0; t < 256; ++t)
+		if (opt->aa20[t] >= 22 || opt->nt4[t] > 4) { nsh_set_error("alphabet table entry %d out of range", t); delete p; return nullptr; }
+	if (opt->ge < 0 || opt->go < 0 || (int64_t)opt->ge * CMAX * 64 > 32767) {
+		nsh_set_error("gap penalties go=%d ge=%d out of the supported range", opt->go, opt->ge); delete p; return nullptr;
+	}
+	for (int32_t k = 0; k < n; ++k) {
+		const nsh_ext_req &q = req[k];
+		int32_t C = (q.npad + 127) / 128;
+		if (q.al < 1 || q.nl < 0 || q.npad < q.al) { nsh_set_error("request %d: nl=%d al=%d npad=%d", k, q.nl, q.al, q.npad); delete p; return nullptr; }
+		if (C > CMAX) { nsh_set_error("request %d: padded protein length %d exceeds %d", k, q.npad, 128 * CMAX); delete p; return nullptr; }
+		if (C < 1) C = 1;
+		ExtProb &e = prob[k];
+		e.nl = q.nl, e.al = q.al, e.npad = q.npad, e.io = q.io, e.out = k, e.C = C, e.left = q.left;
+		e.rec_off = l_ns, e.aa_off = l_as, e.pssm_off = -1;
+		PrepProb &r = prep[k];
+		r.ns_off = l_ns, r.as_off = l_as, r.rec_off = l_ns, r.aa_off = l_as;
+		r.nl = q.nl, r.al = q.al, r.left = q.left, r.io = q.io, r.has_ss = q.ss != nullptr;
+		if (opt->pssm) e.pssm_off = l_pssm, l_pssm += (int64_t)q.al * asize;
+		l_ns += q.nl + 1, l_as += q.al + 1;
+		if (q.nl > max_nl) max_nl = q.nl;
+		any_ss |= q.ss != nullptr;
+		n_small += C <= CMAX_SMALL;
+		p->al[k] = q.al;
+		p->cells_window += (double)q.nl * q.al;
+	}
+	h_ns.resize(l_ns), h_as.resize(l_as);
+	if (any_ss) h_ss.assign(l_ns, 0xff);
+	if (l_pssm) h_pssm.resize(l_pssm);
+	for (int32_t k = 0; k < n; ++k) {
+		const nsh_ext_req &q = req[k];
+		if (q.nl) std::memcpy(&h_ns[prep[k].ns_off], q.ns, q.nl);
+		std::memcpy(&h_as[prep[k].as_off], q.as, q.al);
+		if (q.ss && q.nl) std::memcpy(&h_ss[prep[k].ns_off], q.ss, q.nl);
+		if (opt->pssm) std::memcpy(&h_pssm[prob[k].pssm_off], opt->pssm, (size_t)q.al * asize);
+	}
+	pen.resize(max_nl + 2);
+	for (int32_t d = 0; d < (int32_t)pen.size(); ++d) pen[d] = nsh_ie_penalty(opt->ie_coef, d);
+	std::memcpy(pc.nt4, opt->nt4, 256), std::memcpy(pc.aa20, opt->aa20, 256), std::memcpy(pc.codon, opt->codon, 64);
+	for (int t = 0; t < 6; ++t) pc.sp[t] = opt->sp[t];
+	pc.sp_null = opt->sp_null_bonus;
+	// queue order: small-C problems first (their launch), then large; longest work first in each
+	for (int32_t k = 0; k < n; ++k) order[k] = k;
+	std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+		const int bx = prob[x].C > CMAX_SMALL, by = prob[y].C > CMAX_SMALL;
+		if (bx != by) return bx < by;
+		return (int64_t)prob[x].nl * prob[x].C > (int64_t)prob[y].nl * prob[y].C; });
+	{
+		std::vector<ExtProb> sorted(n);
+		for (int32_t k = 0; k < n; ++k) sorted[k] = prob[order[k]];
+		prob.swap(sorted);
+	}
+	p->out_h.assign((size_t)4 * n, 0);
+	HIPOK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+	HIPOK(hipEventCreate(&p->ev0)); HIPOK(hipEventCreate(&p->ev1)); HIPOK(hipEventCreate(&p->ev2));
+	HIPOK(dalloc(&p->d_ns, l_ns)); HIPOK(dalloc(&p->d_as, l_as)); HIPOK(dalloc(&p->d_aas, l_as));
+	HIPOK(dalloc(&p->d_rec, l_ns));
+	if (any_ss) HIPOK(dalloc(&p->d_ss, l_ns));
+	HIPOK(dalloc(&p->d_mat, (size_t)asize * asize));
+	if (l_pssm) HIPOK(dalloc(&p->d_pssm, l_pssm));
+	HIPOK(dalloc(&p->d_pen, pen.size())); HIPOK(dalloc(&p->d_counter, 2)); HIPOK(dalloc(&p->d_out, (size_t)4 * n));
+	HIPOK(dalloc(&p->d_prob, n)); HIPOK(dalloc(&p->d_prep, n)); HIPOK(dalloc(&p->d_pc, 1));
+	HIPOK(hipMemcpyAsync(p->d_ns, h_ns.data(), l_ns, hipMemcpyHostToDevice, p->stream));
+	HIPOK(hipMemcpyAsync(p->d_as, h_as.data(), l_as, hipMemcpyHostToDevice, p->stream));
+	if (any_ss) HIPOK(hipMemcpyAsync(p->d_ss, h_ss.data(), l_ns, hipMemcpyHostToDevice, p->stream));
+	HIPOK(hipMemcpyAsync(p->d_mat, opt->sc, (size_t)asize * asize, hipMemcpyHostToDevice, p->stream));
+	if (l_pssm) HIPOK(hipMemcpyAsync(p->d_pssm, h_pssm.data(), l_pssm, hipMemcpyHostToDevice, p->stream));
+	HIPOK(hipMemcpyAsync(p->d_pen, pen.data(), pen.size() * 4, hipMemcpyHostToDevice, p->stream));
+	HIPOK(hipMemcpyAsync(p->d_prob, prob.data(), (size_t)n * sizeof(ExtProb), hipMemcpyHostToDevice, p->stream));
+	HIPOK(hipMemcpyAsync(p->d_prep, prep.data(), (size_t)n * sizeof(PrepProb), hipMemcpyHostToDevice, p->stream));
+	HIPOK(hipMemcpyAsync(p->d_pc, &pc, sizeof(pc), hipMemcpyHostToDevice, p->stream));
+	HIPOK(hipStreamSynchronize(p->stream));
+	{
+		int dev = 0, ncu = 0;
+		(void)hipGetDevice(&dev);
+		(void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+		if (ncu <= 0) ncu = 256;
+		for (int v = 0; v < 2; ++v) {
+			ExtArgs &a = p->args[v];
+			a.prob = p->d_prob + (v? n_small : 0), a.n = v? n - n_small : n_small;
+			a.rec = p->d_rec, a.aas = p->d_aas, a.pssm = p->d_pssm, a.mat = p->d_mat, a.asize = asize;
+			a.pen = p->d_pen, a.pen_n = (int32_t)pen.size();
+			a.go = opt->go, a.ge = opt->ge, a.fs = opt->fs, a.xdrop = opt->xdrop, a.eb = opt->end_bonus;
+			a.counter = p->d_counter + v, a.out = p->d_out;
+			// persistent waves: about as many as the CUs keep resident (LDS-bound: 7 or 3 per CU)
+			p->grid[v] = (int32_t)std::min<int64_t>(a.n, (int64_t)ncu * (v? 3 : 7));
+		}
+	}
+	return p;
+fail:
+	plan_free_dev(p);
+	delete p;
+	return nullptr;
+}
+
+extern "C" nsh_ext_plan_t *nsh_ext_plan_create(int32_t n, const char *ns, const int64_t *ns_off, const int32_t *nl,
+		const char *as, const int64_t *as_off, const int32_t *al, const uint8_t *ss,
+		const uint8_t *dirs, const ns_opt_t *opt)
+{
+	if (n < 1 || !ns || !as || !nl || !al || !opt) { nsh_set_error("nsh_ext_plan_create: bad arguments"); return nullptr; }
+	std::vector<nsh_ext_req> req(n);
+	for (int32_t k = 0; k < n; ++k) {
+		nsh_ext_req &q = req[k];
+		const int32_t flag = dirs? (dirs[k]? NS_F_EXT_LEFT : NS_F_EXT_RIGHT) : (opt->flag & (NS_F_EXT_LEFT | NS_F_EXT_RIGHT));
+		q.ns = ns + (ns_off? ns_off[k] : 0), q.nl = nl[k];
+		q.as = as + (as_off? as_off[k] : 0), q.al = al[k];
+		q.ss = ss? ss + (ns_off? ns_off[k] : 0) : nullptr;
+		q.left = (flag & NS_F_EXT_LEFT) != 0;
+		q.io = opt->io;
+		const int32_t w = nsh_host_stripe_width(flag? flag : NS_F_EXT_RIGHT, q.al);
+		q.npad = (q.al + w - 1) / w * w;
+	}
+	return nsh_ext_plan_build(n, req.data(), opt);
+}
+
+extern "C" int nsh_ext_plan_run(nsh_ext_plan_t *p)
+{
+	if (!p) return 0;
+	HIPOK(hipEventRecord(p->ev0, p->stream));
+	hipLaunchKernelGGL(ext_prep_kernel, dim3(p->n), dim3(256), 0, p->stream, p->d_prep, p->d_ns, p->d_ss, p->d_as,
+			p->d_rec, p->d_aas, p->d_pc);
+	hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, p->stream, p->d_counter, 2, 0);
+	HIPOK(hipGetLastError());
+	HIPOK(hipEventRecord(p->ev1, p->stream));
+	if (p->grid[0] > 0) launch_small(p->args[0], p->grid[0], p->stream);
+	if (p->grid[1] > 0) launch_large(p->args[1], p->grid[1], p->stream);
+	HIPOK(hipGetLastError());
+	HIPOK(hipEventRecord(p->ev2, p->stream));
+	nsh_count_launch(2 + (p->grid[0] > 0) + (p->grid[1] > 0));
+	p->ran = 1;
+	return 1;
+fail:
+	return 0;
+}
+
+extern "C" int nsh_ext_plan_sync(nsh_ext_plan_t *p)
+{
+	if (!p) return 0;
+	HIPOK(hipStreamSynchronize(p->stream));
+	return 1;
+fail:
+	return 0;
+}
+
+int nsh_ext_plan_query(nsh_ext_plan_t *p)
+{
+	return !p || hipStreamQuery(p->stream) != hipErrorNotReady;
+}
+
+extern "C" int nsh_ext_plan_results(nsh_ext_plan_t *p, nsh_ext_result_t *out)
+{
+	if (!p || !p->ran) { nsh_set_error("nsh_ext_plan_results: plan has not run"); return 0; }
+	HIPOK(hipMemcpyAsync(p->out_h.data(), p->d_out, (size_t)16 * p->n, hipMemcpyDeviceToHost, p->stream));
+	HIPOK(hipStreamSynchronize(p->stream));
+	if (out)
+		for (int32_t k = 0; k < p->n; ++k)
+			out[k].nt_len = p->out_h[4 * k], out[k].aa_len = p->out_h[4 * k + 1], out[k].score = p->out_h[4 * k + 2];
+	return 1;
+fail:
+	return 0;
+}
+
+extern "C" double nsh_ext_plan_cells(const nsh_ext_plan_t *p, int32_t kind)
+{
+	if (!p) return 0;
+	if (kind == 0) return p->cells_window;
+	double s = 0;
+	for (int32_t k = 0; k < p->n; ++k) s += (double)p->out_h[4 * k + 3] * p->al[k];
+	return s;
+}
+
+extern "C" float nsh_ext_plan_last_ms(nsh_ext_plan_t *p, int32_t which)
+{
+	float ms = 0;
+	if (!p || !p->ran) return -1.f;
+	if (hipEventElapsedTime(&ms, which == 1? p->ev1 : p->ev0, p->ev2) != hipSuccess) return -1.f;
+	return ms;
+}
+
+extern "C" void nsh_ext_plan_destroy(nsh_ext_plan_t *p)
+{
+	if (!p) return;
+	if (p->stream) (void)hipStreamSynchronize(p->stream);
+	plan_free_dev(p);
+	if (p->ev0) (void)hipEventDestroy(p->ev0);
+	if (p->ev1) (void)hipEventDestroy(p->ev1);
+	if (p->ev2) (void)hipEventDestroy(p->ev2);
+	if (p->stream) (void)hipStreamDestroy(p->stream);
+	delete p;
+}

@@ -1,0 +1,31 @@
+// Problem descriptor and kernel arguments of the batched extension DP (host and device).
+#ifndef NSH_EXT_DP_TYPES_H
+#define NSH_EXT_DP_TYPES_H
+#include <cstdint>
+#include <hip/hip_runtime.h>
+
+namespace nsh_ext {
+struct ExtProb {
+	int64_t rec_off;   // first row record (uint32 units)
+	int64_t aa_off;    // first DP-order residue code of the protein
+	int64_t pssm_off;  // -1: score with the matrix; else byte offset of al x asize PSSM rows (query order)
+	int32_t nl, al, npad, io;
+	int32_t out, C, left, pad_;
+};
+
+struct ExtArgs {
+	const ExtProb *prob; int32_t n;
+	const uint32_t *rec; const uint8_t *aas; const int8_t *pssm; const int8_t *mat; int32_t asize;
+	const int32_t *pen; int32_t pen_n;
+	int32_t go, ge, fs, xdrop, eb;
+	int32_t *counter;
+	int32_t *out;      // 4 ints per problem: nt_len, aa_len, score, rows evaluated
+};
+
+
+// kernel launchers (one per LDS variant): C <= 8 and 8 < C <= 16 columns per lane per half
+void launch_small(const ExtArgs &a, int grid, hipStream_t s);
+void launch_large(const ExtArgs &a, int grid, hipStream_t s);
+constexpr int kCmaxSmall = 8, kCmax = 16;
+} // namespace nsh_ext
+#endif

@@ -1,0 +1,42 @@
+// Internal declarations shared by the host sources of libnasw_hip.
+#ifndef NSH_INTERNAL_H
+#define NSH_INTERNAL_H
+
+#include <cstdint>
+#include <cstddef>
+#include "../../include/nasw-hip.h"
+
+#define NSH_NEG16 (-32768)
+
+// ---- device/runtime state (nasw_api.cpp) ----
+int nsh_ensure_device();               // 1 when a GPU is usable (initialises it once)
+void nsh_set_error(const char *fmt, ...);
+void nsh_count_launch(uint64_t n);
+void nsh_add_times(double h2d, double kernel, double d2h);
+int32_t nsh_host_stripe_width(int32_t flag, int32_t al); // the reference CPU's stripe width on this host
+
+// length penalty of the extension DP: (int)(ie_coef * log2approx(d) + .5f) for d >= 2 (nasw-sse.c:341-349, 758-761)
+int32_t nsh_ie_penalty(float ie_coef, int32_t d);
+
+// ---- extension DP engine (ext_dp.hip) ----
+// one request as the engine sees it; raw strings are borrowed until the plan is created
+struct nsh_ext_req {
+	const char *ns; int32_t nl;
+	const char *as; int32_t al;
+	const uint8_t *ss;
+	int32_t left, io;
+	int32_t npad;     // al rounded up to the reference CPU's stripe width
+};
+nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_opt_t *opt);
+int nsh_ext_plan_query(nsh_ext_plan_t *p);   // 1 when the last run has completed
+
+// host-side input preparation (prep_host.cpp), used by the ns_ext_prep() building block
+void nsh_prep_host(const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt,
+		const uint8_t *ss, int left, uint8_t *nas, uint8_t *aas, int8_t *donor, int8_t *acc);
+
+// ---- global DP (global_dp.hip) ----
+// runs ns_global_gs16b/gs32b semantics for one problem on the GPU; 1 on success
+int nsh_global_run(void *km, const char *ns, int32_t nl, const char *as, int32_t al,
+		const ns_opt_t *opt, const uint8_t *ss, int32_t is32, ns_rst_t *r);
+
+#endif

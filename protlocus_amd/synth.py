@@ -1,0 +1,137 @@
+"""Seeded synthetic workloads for the spliced-DP benchmarks and parity tests.
+
+The generators follow the workload descriptions the benchmarks are quoted on:
+  * one pair: a protein drawn from BLOSUM62 background frequencies, reverse-translated with
+    uniform synonymous codons, substituted along BLOSUM62 conditionals, split by GT..AG
+    introns, with a 1-nt frameshift and random flanks;
+  * extension batches: many such pairs with log-uniform protein lengths.
+Background frequencies are the published BLOSUM62 ones (Henikoff & Henikoff 1992); the
+conditionals P(b|a) are proportional to p_b * 2^(s_ab/2), i.e. the half-bit matrix read back
+as target frequencies, excluding b == a.
+"""
+import numpy as np
+
+AA = "ARNDCQEGHILKMFPSTWYV"   # aa20 order of the C-ABI tables (first 20 codes)
+BG = np.array([.074, .052, .045, .054, .025, .034, .054, .074, .026, .068,
+               .099, .058, .025, .047, .039, .057, .051, .013, .032, .073])
+BG = BG / BG.sum()
+# standard genetic code in AAA, AAC, AAG, AAT, ACA, ... order
+CODE1 = "KNKNTTTTRSRSIIMIQHQHPPPPRRRRLLLLEDEDAAAAGGGGVVVV*Y*YSSSS*CWCLFLF"
+NT = np.frombuffer(b"ACGT", dtype=np.uint8)
+_B62 = [
+    [4, -1, -2, -2, 0, -1, -1, 0, -2, -1, -1, -1, -1, -2, -1, 1, 0, -3, -2, 0],
+    [-1, 5, 0, -2, -3, 1, 0, -2, 0, -3, -2, 2, -1, -3, -2, -1, -1, -3, -2, -3],
+    [-2, 0, 6, 1, -3, 0, 0, 0, 1, -3, -3, 0, -2, -3, -2, 1, 0, -4, -2, -3],
+    [-2, -2, 1, 6, -3, 0, 2, -1, -1, -3, -4, -1, -3, -3, -1, 0, -1, -4, -3, -3],
+    [0, -3, -3, -3, 9, -3, -4, -3, -3, -1, -1, -3, -1, -2, -3, -1, -1, -2, -2, -1],
+    [-1, 1, 0, 0, -3, 5, 2, -2, 0, -3, -2, 1, 0, -3, -1, 0, -1, -2, -1, -2],
+    [-1, 0, 0, 2, -4, 2, 5, -2, 0, -3, -3, 1, -2, -3, -1, 0, -1, -3, -2, -2],
+    [0, -2, 0, -1, -3, -2, -2, 6, -2, -4, -4, -2, -3, -3, -2, 0, -2, -2, -3, -3],
+    [-2, 0, 1, -1, -3, 0, 0, -2, 8, -3, -3, -1, -2, -1, -2, -1, -2, -2, 2, -3],
+    [-1, -3, -3, -3, -1, -3, -3, -4, -3, 4, 2, -3, 1, 0, -3, -2, -1, -3, -1, 3],
+    [-1, -2, -3, -4, -1, -2, -3, -4, -3, 2, 4, -2, 2, 0, -3, -2, -1, -2, -1, 1],
+    [-1, 2, 0, -1, -3, 1, 1, -2, -1, -3, -2, 5, -1, -3, -1, 0, -1, -3, -2, -2],
+    [-1, -1, -2, -3, -1, 0, -2, -3, -2, 1, 2, -1, 5, 0, -2, -1, -1, -1, -1, 1],
+    [-2, -3, -3, -3, -2, -3, -3, -3, -1, 0, 0, -3, 0, 6, -4, -2, -2, 1, 3, -1],
+    [-1, -2, -2, -1, -3, -1, -1, -2, -2, -3, -3, -1, -2, -4, 7, -1, -1, -4, -3, -2],
+    [1, -1, 1, 0, -1, 0, 0, 0, -1, -2, -2, 0, -1, -2, -1, 4, 1, -3, -2, -2],
+    [0, -1, 0, -1, -1, -1, -1, -2, -2, -1, -1, -1, -1, -2, -1, 1, 5, -2, -2, 0],
+    [-3, -3, -4, -4, -2, -2, -3, -2, -2, -3, -2, -3, -1, 1, -4, -3, -2, 11, 2, -3],
+    [-2, -2, -2, -3, -2, -1, -2, -3, 2, -1, -1, -2, -1, 3, -3, -2, -2, 2, 7, -1],
+    [0, -3, -3, -3, -1, -2, -2, -3, -3, 3, 1, -2, 1, -1, -2, -2, 0, -3, -1, 4]]
+B62 = np.array(_B62, dtype=np.float64)
+_COND = BG[None, :] * np.power(2.0, B62 / 2.0)
+np.fill_diagonal(_COND, 0.0)
+COND = _COND / _COND.sum(axis=1, keepdims=True)
+COND_CDF = np.cumsum(COND, axis=1)
+SYN = {a: [i for i, c in enumerate(CODE1) if c == a] for a in AA}
+CODON_STR = ["".join("ACGT"[(i >> s) & 3] for s in (4, 2, 0)) for i in range(64)]
+
+
+def rand_dna(rng, n, gc=0.5):
+    p = np.array([(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2])
+    return NT[rng.choice(4, size=n, p=p)].tobytes()
+
+
+_AAIDX = np.full(256, 0, dtype=np.int64)
+for _i, _c in enumerate(AA):
+    _AAIDX[ord(_c)] = _i
+_SYN_TAB = np.zeros((20, 6), dtype=np.int64)
+_SYN_N = np.zeros(20, dtype=np.int64)
+for _i, _c in enumerate(AA):
+    _SYN_N[_i] = len(SYN[_c])
+    _SYN_TAB[_i, :len(SYN[_c])] = SYN[_c]
+_CODON_NT = np.array([[ord(ch) for ch in c] for c in CODON_STR], dtype=np.uint8)
+_AA_BYTES = np.frombuffer(AA.encode(), dtype=np.uint8)
+
+
+def substitute(rng, prot, rate):
+    codes = _AAIDX[np.frombuffer(prot, dtype=np.uint8)]
+    hit = np.nonzero(rng.random(len(codes)) < rate)[0]
+    if len(hit):
+        u = rng.random(len(hit))
+        cdf = COND_CDF[codes[hit]]
+        codes = codes.copy()
+        codes[hit] = np.minimum((cdf < (u * cdf[:, -1])[:, None]).sum(axis=1), 19)
+    return _AA_BYTES[codes].tobytes()
+
+
+def rand_protein(rng, n):
+    return _AA_BYTES[rng.choice(20, size=n, p=BG)].tobytes()
+
+
+def reverse_translate(rng, prot):
+    codes = _AAIDX[np.frombuffer(prot, dtype=np.uint8)]
+    pick = (rng.random(len(codes)) * _SYN_N[codes]).astype(np.int64)
+    return _CODON_NT[_SYN_TAB[codes, pick]].tobytes()
+
+
+def gene_model(rng, prot_len, sub_rate, n_introns, intron_len_fn, n_fs, flank):
+    """protein + DNA carrying its (mutated) coding sequence; returns (dna, protein)"""
+    prot = rand_protein(rng, prot_len)
+    cds = reverse_translate(rng, substitute(rng, prot, sub_rate))
+    seq = bytearray(cds)
+    cuts = sorted(rng.integers(1, max(2, len(cds) - 1), size=n_introns).tolist(), reverse=True)
+    for c in cuts:
+        L = int(intron_len_fn())
+        seq[c:c] = b"GT" + rand_dna(rng, max(0, L - 4)) + b"AG"
+    for _ in range(n_fs):
+        p = int(rng.integers(0, len(seq)))
+        if rng.random() < 0.5:
+            del seq[p]
+        else:
+            seq[p:p] = rand_dna(rng, 1)
+    return rand_dna(rng, flank) + bytes(seq) + rand_dna(rng, flank), prot
+
+
+def single_pair(seed=0):
+    """one 300-aa protein vs ~2.4-3 kb DNA: 15% substitutions, 2 introns 80-400 nt, one 1-nt indel,
+    500-nt flanks (configuration of the single-pair global DP check)"""
+    rng = np.random.default_rng(seed)
+    return gene_model(rng, 300, 0.15, 2, lambda: rng.integers(80, 401), 1, 500)
+
+
+def extension_batch(n, seed=1, min_len=50, max_len=1000):
+    """n (dna, protein) pairs: lengths log-uniform in [min_len, max_len], 10-30% substitutions,
+    0-3 GT..AG introns (log-normal, median 300 nt, capped at 5 kb), 0-2 frameshifts, 200-nt flanks"""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        L = int(round(np.exp(rng.uniform(np.log(min_len), np.log(max_len)))))
+        rate = rng.uniform(0.10, 0.30)
+        ni = int(rng.integers(0, 4))
+        nfs = int(rng.integers(0, 3))
+        out.append(gene_model(rng, L, rate, ni, lambda: min(5000, max(8, rng.lognormal(np.log(300), 0.8))), nfs, 200))
+    return out
+
+
+def pack_batch(pairs):
+    """concatenate pairs for nsh_ext_plan_create: (ns, ns_off, nl, as, as_off, al)"""
+    nl = np.array([len(d) for d, _ in pairs], dtype=np.int32)
+    al = np.array([len(p) for _, p in pairs], dtype=np.int32)
+    ns_off = np.zeros(len(pairs), dtype=np.int64)
+    as_off = np.zeros(len(pairs), dtype=np.int64)
+    if len(pairs) > 1:
+        ns_off[1:] = np.cumsum(nl.astype(np.int64))[:-1]
+        as_off[1:] = np.cumsum(al.astype(np.int64))[:-1]
+    return b"".join(d for d, _ in pairs), ns_off, nl, b"".join(p for _, p in pairs), as_off, al

@@ -1,0 +1,324 @@
+"""ctypes bindings shared by the tests and bench.py.
+
+Three libraries expose the reference's DP interface (types as nasw.h:62-82):
+  * the product, protlocus_amd/libnasw_hip.so (GPU kernels behind the ns_* C-ABI);
+  * the oracle, oracle/liboracle.so (or_* restatement; checker only);
+  * the compiled reference, oracle/_ref/libnsref.so (built from the reference sources in
+    the build container when they are present; used to pin the oracle).
+"""
+import ctypes as C
+import os
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libnsref.so")
+PRODUCT_SO = os.path.join(ROOT, "protlocus_amd", "libnasw_hip.so")
+
+NS_F_CIGAR, NS_F_EXT_LEFT, NS_F_EXT_RIGHT, NS_F_GPU = 1, 2, 4, 8
+NS_S_NONE, NS_S_GENERIC, NS_S_MAMMAL = 0, 1, 2
+CIGAR_STR = "MIDNSHP=XBFGUVE"
+
+
+class NsOpt(C.Structure):
+    _fields_ = [("flag", C.c_int32), ("go", C.c_int32), ("ge", C.c_int32), ("io", C.c_int32), ("fs", C.c_int32),
+                ("xdrop", C.c_int32), ("end_bonus", C.c_int32), ("asize", C.c_int32), ("gpu_device", C.c_int32),
+                ("gpu_min_cells", C.c_int64), ("sp", C.c_int32 * 6), ("sp_null_bonus", C.c_int32),
+                ("ie_coef", C.c_float), ("sc", C.c_void_p), ("pssm", C.c_void_p),
+                ("nt4", C.c_void_p), ("aa20", C.c_void_p), ("codon", C.c_void_p)]
+
+
+class NsRst(C.Structure):
+    _fields_ = [("n_cigar", C.c_int32), ("m_cigar", C.c_int32), ("nt_len", C.c_int32), ("aa_len", C.c_int32),
+                ("score", C.c_int32), ("cigar", C.POINTER(C.c_uint32))]
+
+
+class OrExt(C.Structure):
+    _fields_ = [("nt_len", C.c_int32), ("aa_len", C.c_int32), ("score", C.c_int32)]
+
+
+class GaplessCand(C.Structure):
+    _fields_ = [("target_offset", C.c_int64), ("target_length", C.c_int64), ("query_offset", C.c_int64),
+                ("query_length", C.c_int32), ("anchor_offset", C.c_int32), ("anchor_count", C.c_int32),
+                ("reverse", C.c_int32)]
+
+
+class GaplessAnchor(C.Structure):
+    _fields_ = [("target_block", C.c_int64), ("query_end", C.c_int32), ("reserved", C.c_int32)]
+
+
+class OrHmm(C.Structure):
+    _fields_ = [("len", C.c_int32), ("emit", C.POINTER(C.c_int16)), ("trans", C.POINTER(C.c_int16)),
+                ("pssm", C.POINTER(C.c_int8))]
+
+
+OPT_FIELDS = ("flag", "go", "ge", "io", "fs", "xdrop", "end_bonus", "asize", "gpu_min_cells", "sp_null_bonus", "ie_coef")
+
+
+def cigar_str(rst):
+    return "".join("%d%s" % (rst.cigar[k] >> 4, CIGAR_STR[rst.cigar[k] & 0xf]) for k in range(rst.n_cigar))
+
+
+def _buf(x):
+    """bytes-like -> (ctypes char pointer, keepalive)"""
+    if x is None:
+        return None, None
+    if isinstance(x, str):
+        x = x.encode()
+    b = C.create_string_buffer(bytes(x), len(x) + 1)
+    return b, b
+
+
+class _Lib:
+    def opt(self, **kw):
+        o = NsOpt()
+        self._opt_init(C.byref(o))
+        if "sp_model" in kw:
+            self._opt_set_sp(C.byref(o), kw.pop("sp_model"))
+        if "sp" in kw:
+            for i, v in enumerate(kw.pop("sp")):
+                o.sp[i] = v
+        mat = kw.pop("mat", None)
+        pssm = kw.pop("pssm", None)
+        for k, v in kw.items():
+            setattr(o, k, v)
+        o._keep = []
+        if mat is not None:
+            m = (C.c_int8 * len(mat))(*[int(v) for v in mat])
+            o.sc = C.cast(m, C.c_void_p).value
+            o._keep.append(m)
+        if pssm is not None:
+            p = np.ascontiguousarray(pssm, dtype=np.int8)
+            o.pssm = p.ctypes.data
+            o._keep.append(p)
+        return o
+
+
+class Oracle(_Lib):
+    def __init__(self, path=ORACLE_SO):
+        self.lib = L = C.CDLL(path)
+        L.or_make_tables.argtypes = [C.c_int]
+        L.or_opt_init.argtypes = [C.c_void_p]
+        L.or_opt_set_sp.argtypes = [C.c_void_p, C.c_int32]
+        L.or_dp16_score.argtypes = [C.c_char_p, C.c_int32, C.c_char_p, C.c_int32, C.c_void_p, C.c_char_p,
+                                    C.c_int32, C.c_void_p, C.c_void_p]
+        for f in (L.or_global16,):
+            f.argtypes = [C.c_char_p, C.c_int32, C.c_char_p, C.c_int32, C.c_void_p, C.c_char_p, C.c_int32, C.c_void_p]
+        for f in (L.or_global32,):
+            f.argtypes = [C.c_char_p, C.c_int32, C.c_char_p, C.c_int32, C.c_void_p, C.c_char_p, C.c_void_p]
+        L.or_global_auto.argtypes = [C.c_char_p, C.c_int32, C.c_char_p, C.c_int32, C.c_void_p, C.c_char_p, C.c_int32, C.c_void_p]
+        L.or_global_trace.argtypes = [C.c_char_p, C.c_int32, C.c_char_p, C.c_int32, C.c_void_p, C.c_char_p, C.c_int32,
+                                      C.c_void_p, C.POINTER(C.c_int32)]
+        L.or_global_trace.restype = C.c_int32
+        L.or_ie_penalty.argtypes = [C.c_float, C.c_int32]
+        L.or_ie_penalty.restype = C.c_int32
+        L.or_hmm_parse.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p]
+        L.or_hmm_viterbi.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
+        L.or_hmm_viterbi.restype = C.c_int32
+        L.or_hmm_free.argtypes = [C.c_void_p]
+        L.or_gapless_score.argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                       C.c_void_p, C.c_int32, C.c_void_p]
+        L.or_gapless_score.restype = C.c_int32
+        L.or_pssm_ungapped.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, C.c_int32,
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.or_pssm_ungapped.restype = C.c_int32
+        assert L.or_make_tables(1) == 0
+        self._opt_init = L.or_opt_init
+        self._opt_set_sp = L.or_opt_set_sp
+        self.nt4 = bytes((C.c_uint8 * 256).in_dll(L, "or_nt4"))
+        self.aa20 = bytes((C.c_uint8 * 256).in_dll(L, "or_aa20"))
+        self.codon = bytes((C.c_uint8 * 64).in_dll(L, "or_codon"))
+        self.blosum62 = list((C.c_int8 * 484).in_dll(L, "or_blosum62"))
+
+    def ext(self, ns, as_, opt, ss=None, npad=None):
+        """16-bit score-only DP (extension when opt.flag has EXT_*); returns (nt_len, aa_len, score)."""
+        al = len(as_)
+        if npad is None:
+            npad = (al + 15) // 16 * 16 if al >= 16 else (al + 7) // 8 * 8
+        e = OrExt()
+        rows = C.c_int32()
+        b_ss, _ = _buf(ss)
+        self.lib.or_dp16_score(bytes(ns), len(ns), bytes(as_.encode() if isinstance(as_, str) else as_), al,
+                               C.byref(opt), b_ss, npad, C.byref(e), C.byref(rows))
+        return (e.nt_len, e.aa_len, e.score), rows.value
+
+    def global_dp(self, ns, as_, opt, ss=None, mode="auto", ext_width=16):
+        r = NsRst()
+        a = as_.encode() if isinstance(as_, str) else bytes(as_)
+        b_ss, _ = _buf(ss)
+        if mode == "auto":
+            self.lib.or_global_auto(bytes(ns), len(ns), a, len(a), C.byref(opt), b_ss, ext_width, C.byref(r))
+        elif mode == 16:
+            self.lib.or_global16(bytes(ns), len(ns), a, len(a), C.byref(opt), b_ss, ext_width, C.byref(r))
+        else:
+            self.lib.or_global32(bytes(ns), len(ns), a, len(a), C.byref(opt), b_ss, C.byref(r))
+        out = (r.nt_len, r.aa_len, r.score, cigar_str(r))
+        return out
+
+    def trace(self, ns, as_, opt, is32=0, ss=None):
+        a = as_.encode() if isinstance(as_, str) else bytes(as_)
+        vs = 4 if is32 else 8
+        npad = (len(a) + vs - 1) // vs * vs
+        buf = np.zeros(len(ns) * npad, dtype=np.uint16)
+        sc = C.c_int32()
+        b_ss, _ = _buf(ss)
+        self.lib.or_global_trace(bytes(ns), len(ns), a, len(a), C.byref(opt), b_ss, is32, buf.ctypes.data, C.byref(sc))
+        return buf.reshape(len(ns), npad), sc.value
+
+    def ie_penalty(self, coef, d):
+        return self.lib.or_ie_penalty(coef, d)
+
+
+class Reference(_Lib):
+    """the reference's own nasw-sse.c/nasw-tab.c/hmm.c compiled by oracle/Makefile"""
+
+    def __init__(self, path=REF_SO):
+        self.lib = L = C.CDLL(path)
+        L.ns_make_tables.argtypes = [C.c_int]
+        L.ns_opt_init.argtypes = [C.c_void_p]
+        L.ns_opt_set_sp.argtypes = [C.c_void_p, C.c_int32]
+        for f in (L.ns_global_gs16b, L.ns_global_gs32b, L.ns_global_gs_auto):
+            f.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, C.c_char_p, C.c_int32, C.c_void_p, C.c_char_p, C.c_void_p]
+        L.ns_ext_stripe_width.argtypes = [C.c_void_p, C.c_int32]
+        L.ns_ext_ie_penalty.argtypes = [C.c_void_p, C.c_int32]
+        L.mp_hmm_open.argtypes = [C.c_char_p]
+        L.mp_hmm_open.restype = C.c_void_p
+        L.mp_hmm_read1.argtypes = [C.c_void_p]
+        L.mp_hmm_read1.restype = C.c_void_p
+        L.mp_hmm_close.argtypes = [C.c_void_p]
+        L.mp_hmm_destroy.argtypes = [C.c_void_p]
+        L.mp_hmm_dp_score.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
+        L.mp_hmm_dp_score.restype = C.c_int32
+        L.mp_hmm_length.argtypes = [C.c_void_p]
+        L.mp_hmm_emissions.argtypes = [C.c_void_p]
+        L.mp_hmm_emissions.restype = C.POINTER(C.c_int16)
+        L.mp_hmm_transitions.argtypes = [C.c_void_p]
+        L.mp_hmm_transitions.restype = C.POINTER(C.c_int16)
+        L.mp_hmm_pssm.argtypes = [C.c_void_p]
+        L.mp_hmm_pssm.restype = C.POINTER(C.c_int8)
+        assert L.ns_make_tables(1) == 0
+        self._opt_init = L.ns_opt_init
+        self._opt_set_sp = L.ns_opt_set_sp
+        self.free = C.CDLL(None).free
+        self.free.argtypes = [C.c_void_p]
+
+    def run(self, fn, ns, as_, opt, ss=None):
+        r = NsRst()
+        a = as_.encode() if isinstance(as_, str) else bytes(as_)
+        b_ss, _ = _buf(ss)
+        getattr(self.lib, fn)(None, bytes(ns), len(ns), a, len(a), C.byref(opt), b_ss, C.byref(r))
+        out = (r.nt_len, r.aa_len, r.score, cigar_str(r))
+        if r.cigar:
+            self.free(C.cast(r.cigar, C.c_void_p))
+        return out
+
+    def stripe_width(self, opt, al):
+        return self.lib.ns_ext_stripe_width(C.byref(opt), al)
+
+
+def have_ref():
+    return os.path.exists(REF_SO)
+
+
+class ExtResult(C.Structure):
+    _fields_ = [("nt_len", C.c_int32), ("aa_len", C.c_int32), ("score", C.c_int32)]
+
+
+class Product(_Lib):
+    """the product library: GPU kernels behind the reference's ns_* C-ABI"""
+
+    def __init__(self, path=PRODUCT_SO):
+        self.lib = L = C.CDLL(path)
+        L.ns_make_tables.argtypes = [C.c_int]
+        L.ns_opt_init.argtypes = [C.c_void_p]
+        L.ns_opt_set_sp.argtypes = [C.c_void_p, C.c_int32]
+        for f in (L.ns_global_gs16b, L.ns_global_gs32b, L.ns_global_gs_auto):
+            f.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, C.c_char_p, C.c_int32, C.c_void_p, C.c_char_p, C.c_void_p]
+        L.ns_gpu_init.argtypes = [C.c_int32]
+        L.ns_gpu_error.restype = C.c_char_p
+        L.ns_gpu_name.restype = C.c_char_p
+        L.ns_ext_stripe_width.argtypes = [C.c_void_p, C.c_int32]
+        L.ns_ext_ie_penalty.argtypes = [C.c_void_p, C.c_int32]
+        L.ns_ext_batch_init.argtypes = [C.c_void_p]
+        L.ns_ext_batch_init.restype = C.c_void_p
+        L.ns_ext_batch_add.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, C.c_char_p, C.c_int32, C.c_void_p, C.c_char_p]
+        L.ns_ext_batch_flush.argtypes = [C.c_void_p]
+        L.ns_ext_batch_ready.argtypes = [C.c_void_p, C.c_int32]
+        L.ns_ext_batch_wait.argtypes = [C.c_void_p]
+        L.ns_ext_batch_get.argtypes = [C.c_void_p, C.c_int32]
+        L.ns_ext_batch_get.restype = C.POINTER(NsRst)
+        L.ns_ext_batch_destroy.argtypes = [C.c_void_p]
+        L.ns_ext_gpu_request_count.restype = C.c_uint64
+        L.ns_ext_gpu_launch_count.restype = C.c_uint64
+        L.ns_gpu_dispatch_count.restype = C.c_uint64
+        L.ns_gpu_launch_count.restype = C.c_uint64
+        L.nsh_ext_plan_create.argtypes = [C.c_int32, C.c_char_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_void_p,
+                                          C.c_void_p, C.c_char_p, C.c_char_p, C.c_void_p]
+        L.nsh_ext_plan_create.restype = C.c_void_p
+        for f in (L.nsh_ext_plan_run, L.nsh_ext_plan_sync, L.nsh_ext_plan_destroy):
+            f.argtypes = [C.c_void_p]
+        L.nsh_ext_plan_results.argtypes = [C.c_void_p, C.c_void_p]
+        L.nsh_ext_plan_cells.argtypes = [C.c_void_p, C.c_int32]
+        L.nsh_ext_plan_cells.restype = C.c_double
+        L.nsh_ext_plan_last_ms.argtypes = [C.c_void_p, C.c_int32]
+        L.nsh_ext_plan_last_ms.restype = C.c_float
+        L.nsh_build_info.restype = C.c_char_p
+        assert L.ns_make_tables(1) == 0
+        self._opt_init = L.ns_opt_init
+        self._opt_set_sp = L.ns_opt_set_sp
+        self.free = C.CDLL(None).free
+        self.free.argtypes = [C.c_void_p]
+
+    def error(self):
+        return self.lib.ns_gpu_error().decode()
+
+    def run(self, fn, ns, as_, opt, ss=None):
+        r = NsRst()
+        a = as_.encode() if isinstance(as_, str) else bytes(as_)
+        b_ss, _ = _buf(ss)
+        getattr(self.lib, fn)(None, bytes(ns), len(ns), a, len(a), C.byref(opt), b_ss, C.byref(r))
+        out = (r.nt_len, r.aa_len, r.score, cigar_str(r))
+        if r.cigar:
+            self.free(C.cast(r.cigar, C.c_void_p))
+        return out
+
+    def ext_plan(self, ns, ns_off, nl, as_, as_off, al, opt, ss=None, dirs=None):
+        nl = np.ascontiguousarray(nl, dtype=np.int32)
+        al = np.ascontiguousarray(al, dtype=np.int32)
+        ns_off = np.ascontiguousarray(ns_off, dtype=np.int64)
+        as_off = np.ascontiguousarray(as_off, dtype=np.int64)
+        d = None if dirs is None else bytes(np.ascontiguousarray(dirs, dtype=np.uint8))
+        p = self.lib.nsh_ext_plan_create(len(nl), ns, ns_off.ctypes.data, nl.ctypes.data, as_, as_off.ctypes.data,
+                                         al.ctypes.data, ss, d, C.byref(opt))
+        if not p:
+            raise RuntimeError("nsh_ext_plan_create: " + self.error())
+        return ExtPlan(self, p, len(nl))
+
+
+class ExtPlan:
+    def __init__(self, prod, p, n):
+        self.prod, self.p, self.n = prod, p, n
+
+    def run(self, sync=True):
+        L = self.prod.lib
+        if not L.nsh_ext_plan_run(self.p):
+            raise RuntimeError("nsh_ext_plan_run: " + self.prod.error())
+        if sync and not L.nsh_ext_plan_sync(self.p):
+            raise RuntimeError("nsh_ext_plan_sync: " + self.prod.error())
+
+    def results(self):
+        out = (ExtResult * self.n)()
+        if not self.prod.lib.nsh_ext_plan_results(self.p, out):
+            raise RuntimeError("nsh_ext_plan_results: " + self.prod.error())
+        return [(r.nt_len, r.aa_len, r.score) for r in out]
+
+    def cells(self, kind):
+        return self.prod.lib.nsh_ext_plan_cells(self.p, kind)
+
+    def ms(self, which):
+        return self.prod.lib.nsh_ext_plan_last_ms(self.p, which)
+
+    def close(self):
+        if self.p:
+            self.prod.lib.nsh_ext_plan_destroy(self.p)
+            self.p = None

@@ -1,0 +1,151 @@
+"""GPU parity of the batched extension DP against the oracle.
+
+Mirrors the reference's extension tests: the closed forms of test/test_dp32.c:33-54 and the
+mixed request set of test/test_gpu.c:292-386 (LCG inputs, AVX2/SSE stripe widths, with and
+without a splice-score track, io 20/29, x-drop 30/100/1e6, end bonus 5/0, left and right,
+several flushes in flight), plus seeded config-1-style batches. Bar: bit-exact
+(nt_len, aa_len, score) on every request.
+"""
+import concurrent.futures as cf
+import numpy as np
+import pytest
+
+import nsffi
+from nsffi import NS_F_EXT_LEFT, NS_F_EXT_RIGHT, NS_S_GENERIC
+
+pytestmark = pytest.mark.gpu
+
+
+def lcg(seed):
+    return (seed * 1103515245 + 12345) & 0xffffffff
+
+
+def test_ext_closed_form(product):
+    # TGG^n vs W^n: score = 11n + end_bonus, nt_len = 3n, aa_len = n (test/test_dp32.c:33-54)
+    for flag in (NS_F_EXT_LEFT, NS_F_EXT_RIGHT):
+        for n in (7, 19, 100, 700):
+            o = product.opt(flag=flag)
+            nt_len, aa_len, score, _ = product.run("ns_global_gs16b", b"TGG" * n, "W" * n, o)
+            assert (nt_len, aa_len, score) == (3 * n, n, 11 * n + 5), (flag, n)
+
+
+def ref_style_requests(seed=11, n_req=42, max_nt=30000, max_aa=700):
+    """the request mix of test/test_gpu.c:300-345, restated"""
+    aa_lens = [1, 3, 7, 15, 16, 17, 31, 33, 64, 100, 129, 257, 300, 511, 640]
+    nt = bytearray(max_nt)
+    aa = bytearray(max_aa)
+    ss = bytearray(max_nt)
+    for i in range(max_nt):
+        seed = lcg(seed); nt[i] = b"ACGT"[(seed >> 16) & 3]
+    for i in range(max_aa):
+        seed = lcg(seed); aa[i] = b"ACDEFGHIKLMNPQRSTVWY"[(seed >> 16) % 20]
+    for i in range(max_nt):
+        seed = lcg(seed)
+        ss[i] = ((((seed >> 20) % 40) + 44) << 1 | ((seed >> 27) & 1)) & 0xff if (seed >> 16) % 5 == 0 else 0xff
+    codon = ["GCT", "TGT", "GAT", "GAA", "TTT", "GGT", "CAT", "ATT", "AAA", "TTA", "ATG", "AAT", "CCT", "CAA", "CGT",
+             "TCT", "ACT", "GTT", "TGG", "TAT"]
+    for k in range(400):
+        c = codon["ACDEFGHIKLMNPQRSTVWY".index(chr(aa[k]))]
+        p = 100 + k * 3 + (90 if k > 150 else 0)
+        seed = lcg(seed)
+        if p + 2 < max_nt and (seed >> 16) % 8:
+            nt[p:p + 3] = c.encode()
+    reqs = []
+    for grp in range(3):
+        for i in range(grp * n_req // 3, (grp + 1) * n_req // 3):
+            seed = lcg(seed)
+            kw = dict(sp_model=NS_S_GENERIC, flag=NS_F_EXT_LEFT if i & 1 else NS_F_EXT_RIGHT,
+                      end_bonus=5 if grp == 0 else 0, xdrop=30 if grp == 1 else (1000000 if grp == 2 else 100))
+            if i % 3 == 0:
+                kw["io"] = 20
+            al = aa_lens[i % len(aa_lens)]
+            nl = 3 + (seed >> 8) % (max_nt - 10)
+            if i % 6 == 0:
+                nl = 3 + i // 6
+            o_ns, o_as = (seed >> 4) % 7, (seed >> 12) % 5
+            s = bytes(ss[o_ns:o_ns + nl]) if i % 3 == 1 else None
+            reqs.append((grp, bytes(nt[o_ns:o_ns + nl]), bytes(aa[o_as:o_as + al]), kw, s))
+    return reqs
+
+
+def _oracle_ext(oracle, product, ns, a, kw, ss):
+    o = oracle.opt(**kw)
+    w = product.lib.ns_ext_stripe_width(nsffi.C.byref(product.opt(**kw)), len(a))
+    npad = (len(a) + w - 1) // w * w
+    return oracle.ext(ns, a, o, ss, npad)[0]
+
+
+def test_ext_single_calls_match_oracle(oracle, product):
+    for grp, ns, a, kw, ss in ref_style_requests():
+        exp = _oracle_ext(oracle, product, ns, a, kw, ss)
+        got = product.run("ns_global_gs16b", ns, a, product.opt(**kw), ss)[:3]
+        assert got == exp, (grp, len(ns), len(a), kw, ss is not None)
+
+
+def test_ext_batch_api_matches_oracle(oracle, product):
+    L = product.lib
+    reqs = ref_style_requests()
+    b = L.ns_ext_batch_init(None)
+    assert b, product.error()
+    keep = []
+    for grp in range(3):
+        ids = []
+        before = L.ns_ext_gpu_request_count()
+        for g, ns, a, kw, ss in reqs:
+            if g != grp:
+                continue
+            o = product.opt(**kw)
+            keep.append((o, ns, a, ss))
+            ids.append((L.ns_ext_batch_add(b, ns, len(ns), a, len(a), nsffi.C.byref(o), ss), ns, a, kw, ss))
+        assert L.ns_ext_batch_flush(b) == len(ids)
+        assert L.ns_ext_gpu_request_count() - before == len(ids)
+        for rid, ns, a, kw, ss in ids:
+            r = L.ns_ext_batch_get(b, rid).contents
+            assert (r.nt_len, r.aa_len, r.score) == _oracle_ext(oracle, product, ns, a, kw, ss), (grp, len(ns), len(a))
+    # several flushes in flight; ready/wait semantics (nasw.h:196-209)
+    flights = []
+    for grp in range(3):
+        ids = []
+        for g, ns, a, kw, ss in reqs:
+            if g == grp:
+                o = product.opt(**kw)
+                keep.append(o)
+                ids.append((L.ns_ext_batch_add(b, ns, len(ns), a, len(a), nsffi.C.byref(o), ss), ns, a, kw, ss))
+        assert L.ns_ext_batch_flush(b) == len(ids)
+        flights.append(ids)
+    assert L.ns_ext_batch_flush(b) == 0
+    for ids in flights:
+        while not L.ns_ext_batch_ready(b, ids[0][0]):
+            L.ns_ext_batch_wait(b)
+        for rid, ns, a, kw, ss in ids:
+            assert L.ns_ext_batch_ready(b, rid)
+            r = L.ns_ext_batch_get(b, rid).contents
+            assert (r.nt_len, r.aa_len, r.score) == _oracle_ext(oracle, product, ns, a, kw, ss)
+    L.ns_ext_batch_wait(b)
+    L.ns_ext_batch_destroy(b)
+
+
+@pytest.mark.parametrize("seed,n,lo,hi", [(1, 384, 50, 1000), (2, 256, 20, 140), (3, 64, 900, 2000)])
+def test_ext_plan_config1_batch(oracle, product, seed, n, lo, hi):
+    from protlocus_amd import synth
+    pairs = synth.extension_batch(n, seed=seed, min_len=lo, max_len=hi)
+    rng = np.random.default_rng(seed)
+    dirs = rng.integers(0, 2, size=n).astype(np.uint8)
+    ns, ns_off, nl, as_, as_off, al = synth.pack_batch(pairs)
+    plan = product.ext_plan(ns, ns_off, nl, as_, as_off, al, product.opt(flag=NS_F_EXT_RIGHT), dirs=dirs)
+    try:
+        plan.run()
+        got = plan.results()
+        got2 = (plan.run(), plan.results())[1]   # a second run of the same plan is identical
+    finally:
+        plan.close()
+    assert got == got2
+
+    def one(k):
+        d, p = pairs[k]
+        kw = dict(flag=NS_F_EXT_LEFT if dirs[k] else NS_F_EXT_RIGHT)
+        return _oracle_ext(oracle, product, d, p, kw, None)
+    with cf.ThreadPoolExecutor(8) as ex:
+        exp = list(ex.map(one, range(n)))
+    bad = [k for k in range(n) if got[k] != exp[k]]
+    assert not bad, [(k, len(pairs[k][0]), len(pairs[k][1]), got[k], exp[k]) for k in bad[:5]]
