@@ -31,9 +31,21 @@
 #include "nsh_internal.h"
 #include "ext_dp_types.h"
 
+namespace nsh_ext {
+void launch_group0(const ExtArgs &a, int grid, hipStream_t s);
+void launch_group1(const ExtArgs &a, int grid, hipStream_t s);
+void launch_group2(const ExtArgs &a, int grid, hipStream_t s);
+void launch_group(int g, const ExtArgs &a, int grid, hipStream_t s)
+{
+	if (g == 0) launch_group0(a, grid, s);
+	else if (g == 1) launch_group1(a, grid, s);
+	else launch_group2(a, grid, s);
+}
+} // namespace nsh_ext
+
 namespace {
 using namespace nsh_ext;
-constexpr int CMAX_SMALL = kCmaxSmall, CMAX = kCmax;
+constexpr int CMAX = kCmax;
 #define DEV static __device__ __forceinline__
 
 __global__ void fill_i32_kernel(int32_t *p, int32_t n, int32_t v)
@@ -175,6 +187,8 @@ struct nsh_ext_plan_s {
 	int32_t n = 0;
 	hipStream_t stream = nullptr;
 	hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+	hipStream_t gs[kGroups] = {};          // one stream per column-count group
+	hipEvent_t gev[kGroups] = {};          // completion of each group's launch
 	// device buffers
 	uint8_t *d_ns = nullptr, *d_ss = nullptr, *d_as = nullptr, *d_aas = nullptr;
 	uint32_t *d_rec = nullptr;
@@ -183,8 +197,8 @@ struct nsh_ext_plan_s {
 	ExtProb *d_prob = nullptr;
 	PrepProb *d_prep = nullptr;
 	PrepConst *d_pc = nullptr;
-	ExtArgs args[2] = {};        // [0]: C <= 8 problems, [1]: 8 < C <= 16
-	int32_t grid[2] = { 0, 0 };
+	ExtArgs args[kGroups] = {};
+	int32_t grid[kGroups] = {};
 	double cells_window = 0;     // sum nl*al
 	std::vector<int32_t> al;     // per input index
 	std::vector<int32_t> out_h;  // 4 per problem
@@ -208,7 +222,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 	nsh_ext_plan_t *p = new nsh_ext_plan_s();
 	p->n = n;
 	int64_t l_ns = 0, l_as = 0, l_pssm = 0;
-	int32_t max_nl = 2, any_ss = 0, n_small = 0;
+	int32_t max_nl = 2, any_ss = 0, n_grp[kGroups] = {};
 	std::vector<ExtProb> prob(n);
 	std::vector<PrepProb> prep(n);
 	std::vector<uint8_t> h_ns, h_ss, h_as;
@@ -242,7 +256,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		l_ns += q.nl + 1, l_as += q.al + 1;
 		if (q.nl > max_nl) max_nl = q.nl;
 		any_ss |= q.ss != nullptr;
-		n_small += C <= CMAX_SMALL;
+		++n_grp[group_of(C)];
 		p->al[k] = q.al;
 		p->cells_window += (double)q.nl * q.al;
 	}
@@ -264,7 +278,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 	// queue order: small-C problems first (their launch), then large; longest work first in each
 	for (int32_t k = 0; k < n; ++k) order[k] = k;
 	std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-		const int bx = prob[x].C > CMAX_SMALL, by = prob[y].C > CMAX_SMALL;
+		const int bx = group_of(prob[x].C), by = group_of(prob[y].C);
 		if (bx != by) return bx < by;
 		return (int64_t)prob[x].nl * prob[x].C > (int64_t)prob[y].nl * prob[y].C; });
 	{
@@ -275,12 +289,16 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 	p->out_h.assign((size_t)4 * n, 0);
 	HIPOK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
 	HIPOK(hipEventCreate(&p->ev0)); HIPOK(hipEventCreate(&p->ev1)); HIPOK(hipEventCreate(&p->ev2));
+	for (int g = 0; g < kGroups; ++g) {
+		HIPOK(hipStreamCreateWithFlags(&p->gs[g], hipStreamNonBlocking));
+		HIPOK(hipEventCreateWithFlags(&p->gev[g], hipEventDisableTiming));
+	}
 	HIPOK(dalloc(&p->d_ns, l_ns)); HIPOK(dalloc(&p->d_as, l_as)); HIPOK(dalloc(&p->d_aas, l_as));
 	HIPOK(dalloc(&p->d_rec, l_ns));
 	if (any_ss) HIPOK(dalloc(&p->d_ss, l_ns));
 	HIPOK(dalloc(&p->d_mat, (size_t)asize * asize));
 	if (l_pssm) HIPOK(dalloc(&p->d_pssm, l_pssm));
-	HIPOK(dalloc(&p->d_pen, pen.size())); HIPOK(dalloc(&p->d_counter, 2)); HIPOK(dalloc(&p->d_out, (size_t)4 * n));
+	HIPOK(dalloc(&p->d_pen, pen.size())); HIPOK(dalloc(&p->d_counter, kGroups)); HIPOK(dalloc(&p->d_out, (size_t)4 * n));
 	HIPOK(dalloc(&p->d_prob, n)); HIPOK(dalloc(&p->d_prep, n)); HIPOK(dalloc(&p->d_pc, 1));
 	HIPOK(hipMemcpyAsync(p->d_ns, h_ns.data(), l_ns, hipMemcpyHostToDevice, p->stream));
 	HIPOK(hipMemcpyAsync(p->d_as, h_as.data(), l_as, hipMemcpyHostToDevice, p->stream));
@@ -297,15 +315,17 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		(void)hipGetDevice(&dev);
 		(void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
 		if (ncu <= 0) ncu = 256;
-		for (int v = 0; v < 2; ++v) {
-			ExtArgs &a = p->args[v];
-			a.prob = p->d_prob + (v? n_small : 0), a.n = v? n - n_small : n_small;
+		int64_t first = 0;
+		for (int g = 0; g < kGroups; ++g) {
+			ExtArgs &a = p->args[g];
+			a.prob = p->d_prob + first, a.n = n_grp[g];
+			first += n_grp[g];
 			a.rec = p->d_rec, a.aas = p->d_aas, a.pssm = p->d_pssm, a.mat = p->d_mat, a.asize = asize;
 			a.pen = p->d_pen, a.pen_n = (int32_t)pen.size();
 			a.go = opt->go, a.ge = opt->ge, a.fs = opt->fs, a.xdrop = opt->xdrop, a.eb = opt->end_bonus;
-			a.counter = p->d_counter + v, a.out = p->d_out;
-			// persistent waves: about as many as the CUs keep resident (LDS-bound: 7 or 3 per CU)
-			p->grid[v] = (int32_t)std::min<int64_t>(a.n, (int64_t)ncu * (v? 3 : 7));
+			a.counter = p->d_counter + g, a.out = p->d_out;
+			// persistent waves: about as many as the CUs keep resident
+			p->grid[g] = (int32_t)std::min<int64_t>(a.n, (int64_t)ncu * kGroupWavesPerCU[g]);
 		}
 	}
 	return p;
@@ -341,14 +361,20 @@ extern "C" int nsh_ext_plan_run(nsh_ext_plan_t *p)
 	HIPOK(hipEventRecord(p->ev0, p->stream));
 	hipLaunchKernelGGL(ext_prep_kernel, dim3(p->n), dim3(256), 0, p->stream, p->d_prep, p->d_ns, p->d_ss, p->d_as,
 			p->d_rec, p->d_aas, p->d_pc);
-	hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, p->stream, p->d_counter, 2, 0);
+	hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, p->stream, p->d_counter, kGroups, 0);
 	HIPOK(hipGetLastError());
 	HIPOK(hipEventRecord(p->ev1, p->stream));
-	if (p->grid[0] > 0) launch_small(p->args[0], p->grid[0], p->stream);
-	if (p->grid[1] > 0) launch_large(p->args[1], p->grid[1], p->stream);
-	HIPOK(hipGetLastError());
+	for (int g = 0; g < kGroups; ++g) {
+		if (p->grid[g] <= 0) continue;
+		HIPOK(hipStreamWaitEvent(p->gs[g], p->ev1, 0));
+		launch_group(g, p->args[g], p->grid[g], p->gs[g]);
+		HIPOK(hipGetLastError());
+		HIPOK(hipEventRecord(p->gev[g], p->gs[g]));
+		HIPOK(hipStreamWaitEvent(p->stream, p->gev[g], 0));
+		nsh_count_launch(1);
+	}
 	HIPOK(hipEventRecord(p->ev2, p->stream));
-	nsh_count_launch(2 + (p->grid[0] > 0) + (p->grid[1] > 0));
+	nsh_count_launch(2);
 	p->ran = 1;
 	return 1;
 fail:
@@ -407,6 +433,10 @@ extern "C" void nsh_ext_plan_destroy(nsh_ext_plan_t *p)
 	if (p->ev0) (void)hipEventDestroy(p->ev0);
 	if (p->ev1) (void)hipEventDestroy(p->ev1);
 	if (p->ev2) (void)hipEventDestroy(p->ev2);
+	for (int g = 0; g < kGroups; ++g) {
+		if (p->gev[g]) (void)hipEventDestroy(p->gev[g]);
+		if (p->gs[g]) (void)hipStreamDestroy(p->gs[g]);
+	}
 	if (p->stream) (void)hipStreamDestroy(p->stream);
 	delete p;
 }

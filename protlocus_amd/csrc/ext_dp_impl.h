@@ -1,5 +1,5 @@
-// Device side of the batched extension DP (see ext_dp.hip for the execution model).
-// Included by ext_dp_k8.hip and ext_dp_k16.hip, which instantiate the two LDS variants.
+// Device side of the batched extension DP (execution model: see ext_dp.hip).
+// Included by the ext_dp_k*.hip translation units, one per column-count group.
 #ifndef NSH_EXT_DP_IMPL_H
 #define NSH_EXT_DP_IMPL_H
 #include <hip/hip_runtime.h>
@@ -11,30 +11,40 @@
 namespace nsh_ext {
 
 typedef short s2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u2 __attribute__((ext_vector_type(2)));
 #define DEV static __device__ __forceinline__
 
+// saturating int16 pairs: v_pk_add_i16/v_pk_sub_i16 with clamp, v_pk_max_i16, v_pk_min_i16
 DEV s2 adds(s2 a, s2 b) { return __builtin_elementwise_add_sat(a, b); }
 DEV s2 subs(s2 a, s2 b) { return __builtin_elementwise_sub_sat(a, b); }
 DEV s2 vmax(s2 a, s2 b) { return __builtin_elementwise_max(a, b); }
 DEV s2 vmin(s2 a, s2 b) { return __builtin_elementwise_min(a, b); }
+// the same values biased by +32768 as unsigned pairs, where 0 stands for -32768 (scan only)
+DEV u2 subu(u2 a, u2 b) { return __builtin_elementwise_sub_sat(a, b); }
+DEV u2 maxu(u2 a, u2 b) { return __builtin_elementwise_max(a, b); }
 DEV int pk(s2 a) { return __builtin_bit_cast(int, a); }
 DEV s2 up(int a) { return __builtin_bit_cast(s2, a); }
+DEV int pku(u2 a) { return __builtin_bit_cast(int, a); }
+DEV u2 upu(int a) { return __builtin_bit_cast(u2, a); }
 DEV s2 spl(int x) { return s2{(short)x, (short)x}; }
+DEV u2 splu(int x) { return u2{(unsigned short)x, (unsigned short)x}; }
 DEV int lo16(int x) { return (int)(short)(x & 0xffff); }
 DEV int hi16(int x) { return (int)(short)((unsigned)x >> 16); }
 
 constexpr int NEG = -32768;
 constexpr int PACKNEG = (int)0x80008000u;
-constexpr int CMAX_SMALL = 8;    // columns per lane per half: npad <= 1024 (22.5 KB LDS profile)
-constexpr int CMAX = 16;         // large variant: npad <= 2048 (45 KB LDS profile)
-constexpr int NRES = 22;         // residue codes of the profile rows
+constexpr int BIAS = (int)0x80008000u;   // xor: signed pair <-> biased unsigned pair
+constexpr int NRES = 22;                 // residue codes of the profile rows
 
 // DPP controls (GFX9 encoding)
 enum : int { ROW_SHR1 = 0x111, ROW_SHR2 = 0x112, ROW_SHR4 = 0x114, ROW_SHR8 = 0x118,
 	WAVE_SHR1 = 0x138, ROW_BCAST15 = 0x142, ROW_BCAST31 = 0x143 };
-// lanes without a source keep their own value (old == src, bound_ctrl off)
-template <int CTRL, int ROWMASK = 0xf>
-DEV int dpp(int x) { return __builtin_amdgcn_update_dpp(x, x, CTRL, ROWMASK, 0xf, false); }
+// lanes without a source read 0 (bound_ctrl); every lane is written
+template <int CTRL>
+DEV int dpp0(int x) { return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xf, 0xf, true); }
+// lanes without a source keep their own value
+template <int CTRL>
+DEV int dppk(int x) { return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xf, 0xf, false); }
 
 template <int C>
 DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L)
@@ -78,12 +88,12 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 		}
 	__syncthreads();
 
-	// ---- scan constants: the insertion carry decays by ge per column, geC per lane ----
+	// ---- scan constants (biased unsigned): the insertion carry decays by ge per column ----
 	const int geC = ge * C;
-	const s2 DEC1 = spl(geC), DEC2 = spl(2 * geC), DEC4 = spl(4 * geC), DEC8 = spl(8 * geC);
-	const s2 DEC15 = spl(geC * ((L & 15) + 1));
-	const s2 DEC31 = spl(L >= 32? geC * (L - 31) : 0);
-	const s2 DECL = s2{0, (short)(geC * L)};
+	const u2 DEC1 = splu(geC), DEC2 = splu(2 * geC), DEC4 = splu(4 * geC), DEC8 = splu(8 * geC);
+	const u2 DEC15 = splu(geC * ((L & 15) + 1));                  // row_bcast:15 source distance
+	const u2 DEC31 = splu(L >= 32? geC * (L - 31) : 0xffff);       // row_bcast:31 (rows 0, 1: no source)
+	const u2 DECL = u2{0, (unsigned short)(geC * L)};              // low half's total into the high half
 	const s2 GO = spl(go), GE = spl(ge), IO = spl(io), FS = spl(fs);
 
 	// ---- DP state: three rows of H and D, intron states, best-row snapshot ----
@@ -112,13 +122,21 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 	load_blk(0, recv, penv);
 	if (nl > 64) load_blk(64, recn, penn);
 	int d0 = 0, dp1 = 0, am1 = 0, a0 = 0, dm1 = 0, am2 = 0;
-	if (nl >= 2) {
+	// profile bytes of the next row, loaded one row ahead
+	int pf_lo[C], pf_hi[C];
+	auto prefetch = [&](int res) {
+		const int8_t *pr = prof + res * ROWB + L;
+#pragma unroll
+		for (int k = 0; k < C; ++k) pf_lo[k] = pr[k * 64], pf_hi[k] = pr[HALF + k * 64];
+	};
+	if (nl >= 3) {
 		const int r0 = __builtin_amdgcn_readlane(recv, 0), r1 = __builtin_amdgcn_readlane(recv, 1);
 		d0 = (int)(int8_t)(r0 >> 8), dp1 = (int)(int8_t)(r1 >> 8);
 		am1 = (int)(int8_t)(r0 >> 16), a0 = (int)(int8_t)(r1 >> 16);
+		if (C <= 8) prefetch(__builtin_amdgcn_readlane(recv, 2) & 0xff);
 	}
 
-	int best = INT_MIN, best_log = INT_MIN, best_i = -1, last_i = 1, i = 2;
+	int best_log = INT_MIN, best_i = -1, last_i = 1, i = 2;
 
 	auto row = [&](auto Rc) -> bool {
 		constexpr int R = decltype(Rc)::value, R2 = (R + 1) % 3, R1 = (R + 2) % 3;
@@ -135,70 +153,78 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 		const s2 GEI = spl(res == 20? fs : ge);
 		const s2 DM1 = spl(dm1), D0 = spl(d0), DP1 = spl(dp1), A0 = spl(a0), AM1 = spl(am1), AM2 = spl(am2);
 
-		// profile of this row's residue
-		const int8_t *pr = prof + res * ROWB + L;
+		// this row's profile values (bytes prefetched during the previous row)
 		s2 s[C];
+		if (C > 8) prefetch(res);
 #pragma unroll
-		for (int k = 0; k < C; ++k) {
-			const int lo = pr[k * 64], hi = pr[HALF + k * 64];
-			s[k] = up((int)__builtin_amdgcn_perm((uint32_t)hi, (uint32_t)lo, sel[k]));
+		for (int k = 0; k < C; ++k) s[k] = up((int)__builtin_amdgcn_perm((uint32_t)pf_hi[k], (uint32_t)pf_lo[k], sel[k]));
+		if (C <= 8 && i + 1 < nl) {
+			const int j = (i + 1) & 63;
+			prefetch((j? __builtin_amdgcn_readlane(recv, j) : __builtin_amdgcn_readlane(recn, 0)) & 0xff);
 		}
 
-		// local pass: all terms, insertion chain with zero carry
+		// phase A: every term but the insertion; independent across columns
 		const s2 n1 = nh[R1], n2 = nh[R2], n3 = nh[R];
-		s2 uprev = subs(n1, IO), mprev = vmax(n1, n2), h3prev = n3;
-		s2 hprev = up(PACKNEG), I = up(PACKNEG), rm = up(PACKNEG);
+		s2 u[C], m[C], nonI[C];
+#pragma unroll
+		for (int k = 0; k < C; ++k) u[k] = subs(H1[k], IO), m[k] = vmax(H1[k], H2[k]);
+		const s2 u_n = subs(n1, IO), m_n = vmax(n1, n2);
 #pragma unroll
 		for (int k = 0; k < C; ++k) {
-			const s2 h3 = H3[k], h1 = H1[k];
-			s2 h = adds(h3prev, s[k]);                                   // H(i-3,j-1) + s
-			const s2 dn = subs(vmax(subs(h3, GO), D3[k]), GEI);           // D
-			h = vmax(h, dn);
-			const s2 u = subs(h1, IO);
-			A[k] = vmax(subs(u, DM1), A[k]);     h = vmax(h, subs(A[k], A0));   // phase-0 intron
-			B[k] = vmax(subs(uprev, D0), B[k]);  h = vmax(h, subs(B[k], AM2));  // phase-1
-			Cc[k] = vmax(subs(uprev, DP1), Cc[k]); h = vmax(h, subs(Cc[k], AM1)); // phase-2
-			const s2 m = vmax(h1, H2[k]);
-			h = vmax(h, subs(vmax(m, mprev), FS));                          // 4 frameshift moves
-			I = subs(vmax(subs(hprev, GO), I), GE);                          // insertion (local)
-			h = vmax(h, I);
-			rm = vmax(rm, vmin(h, lim[k]));
-			H3[k] = h, D3[k] = dn;
-			hprev = h, uprev = u, mprev = m, h3prev = h3;
+			const s2 h3p = k? H3[k - 1] : n3, up_ = k? u[k - 1] : u_n, mp = k? m[k - 1] : m_n;
+			const s2 hm = adds(h3p, s[k]);                                  // H(i-3,j-1) + s
+			const s2 dn = subs(vmax(subs(H3[k], GO), D3[k]), GEI);          // D
+			A[k] = vmax(subs(u[k], DM1), A[k]);                             // phase-0 intron
+			B[k] = vmax(subs(up_, D0), B[k]);                               // phase-1
+			Cc[k] = vmax(subs(up_, DP1), Cc[k]);                            // phase-2
+			const s2 f = subs(vmax(m[k], mp), FS);                          // 4 frameshift moves
+			nonI[k] = vmax(vmax(vmax(hm, dn), vmax(subs(A[k], A0), subs(B[k], AM2))), vmax(subs(Cc[k], AM1), f));
+			D3[k] = dn;
 		}
-
-		// max-plus scan of the lanes' outgoing insertions
-		int S = pk(subs(vmax(subs(hprev, GO), I), GE));
-		S = pk(vmax(up(S), subs(up(dpp<ROW_SHR1>(S)), DEC1)));
-		S = pk(vmax(up(S), subs(up(dpp<ROW_SHR2>(S)), DEC2)));
-		S = pk(vmax(up(S), subs(up(dpp<ROW_SHR4>(S)), DEC4)));
-		S = pk(vmax(up(S), subs(up(dpp<ROW_SHR8>(S)), DEC8)));
-		S = pk(vmax(up(S), subs(up(dpp<ROW_BCAST15, 0xA>(S)), DEC15)));
-		S = pk(vmax(up(S), subs(up(dpp<ROW_BCAST31, 0xC>(S)), DEC31)));
-		const int Tlo = lo16(__builtin_amdgcn_readlane(S, 63));   // insertion leaving the low half
-		int Sx = dpp<WAVE_SHR1>(S);
-		if (L == 0) Sx = PACKNEG;
-		const s2 cin = vmax(up(Sx), subs(s2{(short)NEG, (short)Tlo}, DECL));
-
-		// fold the carry into the lane's cells
-		s2 c = cin;
+		// phase B: insertion chain inside the lane (zero carry); opening from an H that is itself an
+		// insertion is dominated by extending it, so the chain only needs the non-insertion terms
+		s2 I = up(PACKNEG), hh[C];
 #pragma unroll
 		for (int k = 0; k < C; ++k) {
-			H3[k] = vmax(H3[k], c);
-			if (k + 1 < C) c = subs(c, GE);
+			if (k) I = subs(vmax(subs(nonI[k - 1], GO), I), GE);
+			hh[k] = vmax(nonI[k], I);
 		}
-		rm = vmax(rm, vmin(cin, lim[0]));
+		const s2 X = subs(vmax(subs(nonI[C - 1], GO), I), GE);          // insertion leaving the lane
 
-		// row maximum over both halves
-		int m = pk(rm);
-		m = pk(vmax(up(m), up(dpp<ROW_SHR1>(m))));
-		m = pk(vmax(up(m), up(dpp<ROW_SHR2>(m))));
-		m = pk(vmax(up(m), up(dpp<ROW_SHR4>(m))));
-		m = pk(vmax(up(m), up(dpp<ROW_SHR8>(m))));
-		m = pk(vmax(up(m), up(dpp<ROW_BCAST15, 0xA>(m))));
-		m = pk(vmax(up(m), up(dpp<ROW_BCAST31, 0xC>(m))));
-		const int mt = __builtin_amdgcn_readlane(m, 63);
-		const int rowmax = std::max(lo16(mt), hi16(mt));
+		// max-plus scan of the lanes' outgoing insertions (biased unsigned, 0 = -32768)
+		int S = pk(X) ^ BIAS;
+		S = pku(maxu(upu(S), subu(upu(dpp0<ROW_SHR1>(S)), DEC1)));
+		S = pku(maxu(upu(S), subu(upu(dpp0<ROW_SHR2>(S)), DEC2)));
+		S = pku(maxu(upu(S), subu(upu(dpp0<ROW_SHR4>(S)), DEC4)));
+		S = pku(maxu(upu(S), subu(upu(dpp0<ROW_SHR8>(S)), DEC8)));
+		S = pku(maxu(upu(S), subu(upu(dpp0<ROW_BCAST15>(S)), DEC15)));
+		S = pku(maxu(upu(S), subu(upu(dpp0<ROW_BCAST31>(S)), DEC31)));
+		const int Tlo = __builtin_amdgcn_readlane(S, 63) & 0xffff;     // leaving the low half (biased)
+		const u2 cinu = maxu(upu(dpp0<WAVE_SHR1>(S)), subu(u2{0, (unsigned short)Tlo}, DECL));
+		const s2 cin = up(pku(cinu) ^ BIAS);
+
+		// phase C: fold the carry into the cells, masked row maximum
+		s2 mk[C];
+#pragma unroll
+		for (int k = 0; k < C; ++k) {
+			const s2 hk = k? vmax(hh[k], subs(cin, spl(ge * k))) : vmax(hh[0], cin);
+			H3[k] = hk;
+			mk[k] = vmin(hk, lim[k]);
+		}
+#pragma unroll
+		for (int w = 1; w < C; w *= 2)
+#pragma unroll
+			for (int k = 0; k + w < C; k += 2 * w) mk[k] = vmax(mk[k], mk[k + w]);
+		// row maximum over both halves and all lanes
+		// (biased by +32768 so that the zero a source-less lane reads is neutral)
+		const s2 rm = mk[0];
+		int mx = std::max((int)rm.x, (int)rm.y) + 32768;
+		mx = std::max(mx, dpp0<ROW_SHR1>(mx));
+		mx = std::max(mx, dpp0<ROW_SHR2>(mx));
+		mx = std::max(mx, dpp0<ROW_SHR4>(mx));
+		mx = std::max(mx, dpp0<ROW_SHR8>(mx));
+		const int rowmax = std::max(std::max(__builtin_amdgcn_readlane(mx, 15), __builtin_amdgcn_readlane(mx, 31)),
+				std::max(__builtin_amdgcn_readlane(mx, 47), __builtin_amdgcn_readlane(mx, 63))) - 32768;
 
 		// H(i, al-1)
 		int ev = 0;
@@ -216,18 +242,17 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 		const int tmp = rowmax > es? rowmax : es;
 		const int tlog = tmp - pen_i;
 		if (tlog > best_log) {
-			best = tmp, best_log = tlog, best_i = i;
+			best_log = tlog, best_i = i;
 #pragma unroll
 			for (int k = 0; k < C; ++k) Hmax[k] = H3[k];
 		}
 		// left neighbour of the next row: lane L-1's last column; lane 0 gets column -1
 		// (neg_inf for rows > 2) in the low half and the low half's last column in the high half
 		const int V = pk(H3[C - 1]);
-		const int Vs = dpp<WAVE_SHR1>(V);
+		const int Vs = dpp0<WAVE_SHR1>(V);
 		const int V63 = __builtin_amdgcn_readlane(V, 63);
 		nh[R] = up(L == 0? (int)(((unsigned)V63 << 16) | 0x8000u) : Vs);
 		if (i == 2 && L == 0) nh[R2].x = NEG, nh[R1].x = NEG;
-		(void)best;
 		last_i = i;
 		if (best_log - tlog > xdrop) return false;
 		return ++i < nl;
@@ -264,10 +289,11 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 	}
 }
 
-template <int CM>
+// persistent waves over one group's queue; CLO..CHI columns per lane per half
+template <int CLO, int CHI>
 __global__ void __launch_bounds__(64) ext_dp_kernel(ExtArgs a)
 {
-	__shared__ int8_t prof[NRES * 128 * CM];
+	__shared__ int8_t prof[NRES * 128 * CHI];
 	const int L = threadIdx.x;
 	for (;;) {
 		int p = 0;
@@ -275,29 +301,11 @@ __global__ void __launch_bounds__(64) ext_dp_kernel(ExtArgs a)
 		p = __builtin_amdgcn_readlane(p, 0);
 		if (p >= a.n) break;
 		const ExtProb P = a.prob[p];
-		if (CM <= CMAX_SMALL) {
-			switch (P.C) {
-			case 1: ext_solve<1>(a, P, prof, L); break;
-			case 2: ext_solve<2>(a, P, prof, L); break;
-			case 3: ext_solve<3>(a, P, prof, L); break;
-			case 4: ext_solve<4>(a, P, prof, L); break;
-			case 5: ext_solve<5>(a, P, prof, L); break;
-			case 6: ext_solve<6>(a, P, prof, L); break;
-			case 7: ext_solve<7>(a, P, prof, L); break;
-			default: ext_solve<CMAX_SMALL>(a, P, prof, L); break;
-			}
-		} else {
-			switch (P.C) {
-			case 9: ext_solve<9>(a, P, prof, L); break;
-			case 10: ext_solve<10>(a, P, prof, L); break;
-			case 11: ext_solve<11>(a, P, prof, L); break;
-			case 12: ext_solve<12>(a, P, prof, L); break;
-			case 13: ext_solve<13>(a, P, prof, L); break;
-			case 14: ext_solve<14>(a, P, prof, L); break;
-			case 15: ext_solve<15>(a, P, prof, L); break;
-			default: ext_solve<CMAX>(a, P, prof, L); break;
-			}
-		}
+		const int C = P.C < CLO? CLO : P.C > CHI? CHI : P.C;
+#define NSH_C(K) if (K >= CLO && K <= CHI && C == K) { ext_solve<(K >= CLO && K <= CHI)? K : CLO>(a, P, prof, L); continue; }
+		NSH_C(1) NSH_C(2) NSH_C(3) NSH_C(4) NSH_C(5) NSH_C(6) NSH_C(7) NSH_C(8)
+		NSH_C(9) NSH_C(10) NSH_C(11) NSH_C(12) NSH_C(13) NSH_C(14) NSH_C(15) NSH_C(16)
+#undef NSH_C
 	}
 }
 

@@ -23,9 +23,13 @@ struct ExtArgs {
 };
 
 
-// kernel launchers (one per LDS variant): C <= 8 and 8 < C <= 16 columns per lane per half
-void launch_small(const ExtArgs &a, int grid, hipStream_t s);
-void launch_large(const ExtArgs &a, int grid, hipStream_t s);
-constexpr int kCmaxSmall = 8, kCmax = 16;
+// column-count groups, each a kernel with its own LDS size (22 x 128C bytes for the group's
+// largest C) and occupancy; the groups run concurrently on separate streams
+constexpr int kGroups = 3;
+constexpr int kGroupLo[kGroups] = { 1, 5, 9 }, kGroupHi[kGroups] = { 4, 8, 16 };
+constexpr int kGroupWavesPerCU[kGroups] = { 14, 7, 3 };   // LDS-bound residency (160 KB per CU)
+constexpr int kCmax = 16;
+inline int group_of(int C) { return C <= 4? 0 : C <= 8? 1 : 2; }
+void launch_group(int g, const ExtArgs &a, int grid, hipStream_t s);
 } // namespace nsh_ext
 #endif
