@@ -30,6 +30,7 @@
 #include <vector>
 #include "nsh_internal.h"
 #include "ext_dp_types.h"
+#include "prep.h"
 
 namespace nsh_ext {
 void launch_group0(const ExtArgs &a, int grid, hipStream_t s);
@@ -45,6 +46,8 @@ void launch_group(int g, const ExtArgs &a, int grid, hipStream_t s)
 
 namespace {
 using namespace nsh_ext;
+using nsh_prep::PrepProb;
+using nsh_prep::PrepConst;
 constexpr int CMAX = kCmax;
 #define DEV static __device__ __forceinline__
 
@@ -58,16 +61,6 @@ __global__ void fill_i32_kernel(int32_t *p, int32_t n, int32_t v)
 // Device-side input preparation (nasw-sse.c:95-214): one work-group per problem writes
 // rec[i] = nas[i] | donor[i+1] << 8 | acceptor[i] << 16 and the DP-order protein codes.
 // ------------------------------------------------------------------------------------
-struct PrepProb {
-	int64_t ns_off, as_off, rec_off, aa_off;
-	int32_t nl, al, left, io;
-	int32_t has_ss, pad_[3];
-};
-struct PrepConst {
-	uint8_t nt4[256], aa20[256], codon[64];
-	int32_t sp[6], sp_null;
-};
-
 DEV int wrap8(int v) { return (int)(int8_t)(uint8_t)(v & 0xff); }
 
 struct PrepView {
@@ -178,6 +171,12 @@ __global__ void __launch_bounds__(256) ext_prep_kernel(const PrepProb *pp, const
 
 } // namespace
 
+void nsh_prep::launch(const PrepProb *pp, int32_t n, const uint8_t *raw_ns, const uint8_t *raw_ss, const uint8_t *raw_as,
+		uint32_t *rec, uint8_t *aas, const PrepConst *pc, hipStream_t s)
+{
+	hipLaunchKernelGGL(ext_prep_kernel, dim3(n), dim3(256), 0, s, pp, raw_ns, raw_ss, raw_as, rec, aas, pc);
+}
+
 // ======================================================================================
 // host side: plans
 // ======================================================================================
@@ -247,7 +246,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		if (C > CMAX) { nsh_set_error("request %d: padded protein length %d exceeds %d", k, q.npad, 128 * CMAX); delete p; return nullptr; }
 		if (C < 1) C = 1;
 		ExtProb &e = prob[k];
-		e.nl = q.nl, e.al = q.al, e.npad = q.npad, e.io = q.io, e.out = k, e.C = C, e.left = q.left;
+		e.nl = q.nl, e.al = q.al, e.npad = q.npad, e.io = q.io, e.out = k, e.C = C, e.left = q.left, e.mode = q.mode;
 		e.rec_off = l_ns, e.aa_off = l_as, e.pssm_off = -1;
 		PrepProb &r = prep[k];
 		r.ns_off = l_ns, r.as_off = l_as, r.rec_off = l_ns, r.aa_off = l_as;

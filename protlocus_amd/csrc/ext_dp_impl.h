@@ -136,7 +136,7 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 		if (C <= 8) prefetch(__builtin_amdgcn_readlane(recv, 2) & 0xff);
 	}
 
-	int best_log = INT_MIN, best_i = -1, last_i = 1, i = 2;
+	int best_log = INT_MIN, best_i = -1, last_i = 1, i = 2, last_ev = NEG;
 
 	auto row = [&](auto Rc) -> bool {
 		constexpr int R = decltype(Rc)::value, R2 = (R + 1) % 3, R1 = (R + 2) % 3;
@@ -236,6 +236,7 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 		default: break;
 		}
 		ev = he? hi16(ev) : lo16(ev);
+		last_ev = ev;
 
 		// row score, length penalty, best row and x-drop (nasw-sse.c:438-447)
 		const int es = ev + eb;
@@ -285,7 +286,9 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 	}
 	if (L == 0) {
 		int32_t *o = a.out + 4 * (int64_t)P.out;
-		o[0] = best_i + 1, o[1] = bj + 1, o[2] = bv, o[3] = last_i - 1;
+		if (P.mode) o[0] = nl, o[1] = al, o[2] = last_ev;   // nasw-sse.c:559: H(last row, al-1)
+		else o[0] = best_i + 1, o[1] = bj + 1, o[2] = bv;
+		o[3] = last_i - 1;
 	}
 }
 

@@ -10,7 +10,8 @@ struct ExtProb {
 	int64_t aa_off;    // first DP-order residue code of the protein
 	int64_t pssm_off;  // -1: score with the matrix; else byte offset of al x asize PSSM rows (query order)
 	int32_t nl, al, npad, io;
-	int32_t out, C, left, pad_;
+	int32_t out, C, left;
+	int32_t mode;      // 0: extension result; 1: plain score-only global DP, score H(last row, al-1)
 };
 
 struct ExtArgs {

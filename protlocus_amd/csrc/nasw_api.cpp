@@ -167,6 +167,7 @@ static int nsh_ext_one(const char *ns, int32_t nl, const char *as, int32_t al, c
 	nsh_ext_req q;
 	q.ns = ns, q.nl = nl, q.as = as, q.al = al, q.ss = ss;
 	q.left = (opt->flag & NS_F_EXT_LEFT) != 0, q.io = opt->io;
+	q.mode = (opt->flag & (NS_F_EXT_LEFT | NS_F_EXT_RIGHT)) == 0;
 	const int32_t w = nsh_host_stripe_width(opt->flag, al);
 	q.npad = (al + w - 1) / w * w;
 	nsh_ext_plan_t *p = nsh_ext_plan_build(1, &q, opt);
@@ -189,7 +190,9 @@ static void nsh_global_dispatch(void *km, const char *ns, int32_t nl, const char
 		const uint8_t *ss, int32_t is32, ns_rst_t *r)
 {
 	int ok;
-	if (!is32 && (opt->flag & (NS_F_EXT_LEFT | NS_F_EXT_RIGHT))) ok = nsh_ext_one(ns, nl, as, al, opt, ss, r);
+	// int16 score-only (extension or plain global) runs on the extension engine (nasw-sse.c:360-459)
+	const bool ext = (opt->flag & (NS_F_EXT_LEFT | NS_F_EXT_RIGHT)) != 0;
+	if (!is32 && (ext || !(opt->flag & NS_F_CIGAR))) ok = nsh_ext_one(ns, nl, as, al, opt, ss, r);
 	else ok = nsh_global_run(km, ns, nl, as, al, opt, ss, is32, r);
 	if (!ok) {
 		std::fprintf(stderr, "[nasw_hip] GPU DP failed (nl=%d al=%d): %s\n", nl, al, ns_gpu_error());
@@ -225,18 +228,40 @@ extern "C" void ns_global_gs_auto(void *km, const char *ns, int32_t nl, const ch
 	else ns_global_gs16b(km, ns, nl, as, al, opt, ss, r);
 }
 
-// ns_gpu_submit/wait (nasw.h:149-151): each submission is computed on the GPU before submit
-// returns and consumed exactly once by ns_gpu_wait; the CIGAR is allocated from km.
-struct ns_gpu_request_s { ns_rst_t rst; int ok; };
+// ns_gpu_submit/wait/global (nasw.h:140-151, nasw-cuda.cu:1068-1145): the int16 global DP with
+// CIGAR. Like the reference, a request is refused (NULL / 0) when nl < 3, al < 1, asize < 1 or
+// the best possible score al * max(sc) + end_bonus exceeds 30000 (the caller then takes the int32
+// path). The extension flags are ignored and the CIGAR is always produced; the splice-score track
+// and a PSSM are honoured, so the result equals ns_global_gs16b with NS_F_CIGAR. Each submission
+// is computed before submit returns and is consumed exactly once by ns_gpu_wait; the CIGAR is
+// allocated from km.
+struct ns_gpu_request_s { ns_rst_t rst; void *km; int ok; };
+
+static int nsh_gpu_global_accepts(int32_t nl, int32_t al, const ns_opt_t *opt)
+{
+	if (nl < 3 || al < 1 || opt->asize < 1) return 0;
+	int32_t max_sub = 0;
+	for (int32_t i = 0; i < opt->asize * opt->asize; ++i) if (max_sub < opt->sc[i]) max_sub = opt->sc[i];
+	return (int64_t)al * max_sub + opt->end_bonus <= 30000;
+}
+
+static int nsh_gpu_global_run(void *km, const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt,
+		const uint8_t *ss, ns_rst_t *r)
+{
+	if (!nsh_ensure_device() || !nsh_gpu_global_accepts(nl, al, opt)) return 0;
+	ns_opt_t o = *opt;
+	o.flag = (o.flag | NS_F_CIGAR) & ~(NS_F_GPU | NS_F_EXT_LEFT | NS_F_EXT_RIGHT);
+	if (!nsh_global_run(km, ns, nl, as, al, &o, ss, 0, r)) return 0;
+	++g_dispatch;
+	return 1;
+}
 
 extern "C" ns_gpu_request_t *ns_gpu_submit(void *km, const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, const uint8_t *ss)
 {
-	if (!nsh_ensure_device()) return nullptr;
 	ns_gpu_request_t *q = (ns_gpu_request_t*)std::calloc(1, sizeof(ns_gpu_request_t));
-	ns_opt_t o = *opt;
-	o.flag &= ~(NS_F_GPU);
-	const int is32 = 0;
-	q->ok = nsh_global_run(km, ns, nl, as, al, &o, ss, is32, &q->rst);
+	if (!q) return nullptr;
+	q->rst.n_cigar = q->rst.m_cigar = 0, q->rst.cigar = nullptr, q->km = km;
+	q->ok = nsh_gpu_global_run(km, ns, nl, as, al, opt, ss, &q->rst);
 	if (!q->ok) { std::free(q); return nullptr; }
 	return q;
 }
@@ -245,17 +270,21 @@ extern "C" int ns_gpu_wait(ns_gpu_request_t *q, ns_rst_t *r)
 {
 	if (!q) return 0;
 	const int ok = q->ok;
-	if (ok) *r = q->rst;
+	if (ok) {   // the caller's r may already own a CIGAR buffer (ns_rst_init): append into it
+		for (int32_t k = 0; k < q->rst.n_cigar; ++k) {
+			const uint32_t c = q->rst.cigar[k];
+			r->cigar = ns_push_cigar(q->km, &r->n_cigar, &r->m_cigar, r->cigar, c & 0xf, c >> 4);
+		}
+		r->nt_len = q->rst.nt_len, r->aa_len = q->rst.aa_len, r->score = q->rst.score;
+	}
+	if (q->rst.cigar) kfree(q->km, q->rst.cigar);
 	std::free(q);
 	return ok;
 }
 
 extern "C" int ns_gpu_global(void *km, const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, const uint8_t *ss, ns_rst_t *r)
 {
-	if (!nsh_ensure_device()) return 0;
-	ns_opt_t o = *opt;
-	o.flag &= ~NS_F_GPU;
-	return nsh_global_run(km, ns, nl, as, al, &o, ss, 0, r);
+	return nsh_gpu_global_run(km, ns, nl, as, al, opt, ss, r);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -26,6 +26,7 @@ struct nsh_ext_req {
 	const uint8_t *ss;
 	int32_t left, io;
 	int32_t npad;     // al rounded up to the reference CPU's stripe width
+	int32_t mode = 0; // 1: score-only global DP without extension flags (nasw-sse.c:459, 559)
 };
 nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_opt_t *opt);
 int nsh_ext_plan_query(nsh_ext_plan_t *p);   // 1 when the last run has completed

@@ -1,0 +1,22 @@
+// Device-side input preparation shared by the DP engines: problem descriptor, tables, launcher.
+#ifndef NSH_PREP_H
+#define NSH_PREP_H
+#include <cstdint>
+#include <hip/hip_runtime.h>
+
+namespace nsh_prep {
+struct PrepProb {
+	int64_t ns_off, as_off, rec_off, aa_off;
+	int32_t nl, al, left, io;
+	int32_t has_ss, pad_[3];
+};
+struct PrepConst {
+	uint8_t nt4[256], aa20[256], codon[64];
+	int32_t sp[6], sp_null;
+};
+
+// one work-group per problem; rec[i] = nas[i] | donor[i+1] << 8 | acceptor[i] << 16 (bytes)
+void launch(const PrepProb *pp, int32_t n, const uint8_t *raw_ns, const uint8_t *raw_ss, const uint8_t *raw_as,
+		uint32_t *rec, uint8_t *aas, const PrepConst *pc, hipStream_t s);
+} // namespace nsh_prep
+#endif

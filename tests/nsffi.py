@@ -263,6 +263,11 @@ class Product(_Lib):
         L.nsh_ext_plan_last_ms.argtypes = [C.c_void_p, C.c_int32]
         L.nsh_ext_plan_last_ms.restype = C.c_float
         L.nsh_build_info.restype = C.c_char_p
+        L.ns_gpu_global.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, C.c_char_p, C.c_int32, C.c_void_p, C.c_char_p,
+                                    C.c_void_p]
+        L.ns_gpu_submit.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, C.c_char_p, C.c_int32, C.c_void_p, C.c_char_p]
+        L.ns_gpu_submit.restype = C.c_void_p
+        L.ns_gpu_wait.argtypes = [C.c_void_p, C.c_void_p]
         assert L.ns_make_tables(1) == 0
         self._opt_init = L.ns_opt_init
         self._opt_set_sp = L.ns_opt_set_sp
@@ -281,6 +286,17 @@ class Product(_Lib):
         if r.cigar:
             self.free(C.cast(r.cigar, C.c_void_p))
         return out
+
+    def gpu_global(self, ns, as_, opt, ss=None):
+        """ns_gpu_global: (ok, (nt_len, aa_len, score, cigar))"""
+        r = NsRst()
+        a = as_.encode() if isinstance(as_, str) else bytes(as_)
+        b_ss, _ = _buf(ss)
+        ok = self.lib.ns_gpu_global(None, bytes(ns), len(ns), a, len(a), C.byref(opt), b_ss, C.byref(r))
+        out = (r.nt_len, r.aa_len, r.score, cigar_str(r))
+        if r.cigar:
+            self.free(C.cast(r.cigar, C.c_void_p))
+        return ok, out
 
     def ext_plan(self, ns, ns_off, nl, as_, as_off, al, opt, ss=None, dirs=None):
         nl = np.ascontiguousarray(nl, dtype=np.int32)

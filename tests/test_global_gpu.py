@@ -1,0 +1,199 @@
+"""GPU parity of the global spliced DP (ns_global_gs16b / ns_global_gs32b / ns_global_gs_auto /
+ns_gpu_global / ns_gpu_submit+wait) against the oracle.
+
+Mirrors test/test_gpu.c:40-125 and 196-220 (tile boundaries with TGG/W, the int32 fallback,
+several submissions in flight) and adds seeded random cases: planted spliced, mutated and
+frameshifted copies, with and without a splice-score track, random gap/intron/frameshift
+penalties, int16 and int32, protein lengths straddling every lane, wave and stripe-segment
+boundary of both kernels (1..4096 aa). Bar: bit-exact (nt_len, aa_len, score, CIGAR).
+"""
+import concurrent.futures as cf
+import numpy as np
+import pytest
+
+import nsffi
+from nsffi import NS_F_CIGAR, NS_F_EXT_LEFT, NS_F_EXT_RIGHT
+
+pytestmark = pytest.mark.gpu
+
+AA = "ACDEFGHIKLMNPQRSTVWY"
+CODON = ["GCT", "TGT", "GAT", "GAA", "TTT", "GGT", "CAT", "ATT", "AAA", "TTA", "ATG", "AAT", "CCT", "CAA", "CGT",
+         "TCT", "ACT", "GTT", "TGG", "TAT"]
+
+
+def planted_case(rng, al, flank=None):
+    """random protein and a DNA that holds a mutated, maybe spliced and frameshifted copy of it"""
+    a = "".join(rng.choice(list(AA), size=al))
+    flank = int(rng.integers(0, 30)) if flank is None else flank
+    nt = [rng.choice(list("ACGT"), size=flank)]
+    for k, c in enumerate(a):
+        cod = list(CODON[AA.index(c)])
+        if rng.random() < 0.15:
+            cod[int(rng.integers(0, 3))] = "ACGT"[int(rng.integers(0, 4))]
+        nt.append(np.array(cod))
+        if k == al // 2 and rng.random() < 0.4:   # an intron
+            nt.append(np.array(list("GTAAG") + list(rng.choice(list("ACGT"), size=int(rng.integers(30, 120)))) +
+                               list("TTTCAG")))
+        if rng.random() < 0.004:                  # a frameshift
+            nt.append(rng.choice(list("ACGT"), size=int(rng.integers(1, 3))))
+    nt.append(rng.choice(list("ACGT"), size=int(rng.integers(0, 30))))
+    ns = "".join("".join(x) for x in nt).encode()
+    if rng.random() < 0.25:                        # an unrelated target
+        ns = rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), size=len(ns)).tobytes()
+    return ns, a
+
+
+def random_opt_kw(rng):
+    kw = {}
+    if rng.random() < 0.3:
+        kw["go"] = int(rng.integers(3, 13))
+    if rng.random() < 0.25:
+        kw["fs"] = int(rng.integers(5, 25))
+    if rng.random() < 0.25:
+        kw["io"] = int(rng.integers(10, 40))
+    return kw
+
+
+def random_ss(rng, nl):
+    if rng.random() >= 0.3:
+        return None
+    ss = np.full(nl, 0xff, dtype=np.uint8)
+    m = rng.random(nl) < 0.2
+    ss[m] = ((rng.integers(0, 40, size=m.sum()) + 44) << 1 | rng.integers(0, 2, size=m.sum())).astype(np.uint8)
+    return bytes(ss)
+
+
+# lane / wave / segment boundaries: C columns per lane (1..4), 64-lane waves, up to 16 waves,
+# int16 stripe segments (8 lanes) and int32 segments (4 lanes)
+AL_EDGES = [1, 2, 3, 4, 5, 7, 8, 9, 12, 15, 16, 17, 31, 32, 33, 63, 64, 65, 127, 128, 129, 191, 192, 193, 255,
+            256, 257, 300, 511, 512, 513, 640, 767, 768, 769, 1023, 1024, 1025, 1500, 2047, 2048, 2049, 2560]
+
+
+def _check_many(oracle, product, cases):
+    def one(c):
+        ns, a, kw, ss, is32 = c
+        return oracle.global_dp(ns, a, oracle.opt(**kw), ss, mode=32 if is32 else 16)
+    with cf.ThreadPoolExecutor(8) as ex:
+        exp = list(ex.map(one, cases))
+    bad = []
+    for c, e in zip(cases, exp):
+        ns, a, kw, ss, is32 = c
+        got = product.run("ns_global_gs32b" if is32 else "ns_global_gs16b", ns, a, product.opt(**kw), ss)
+        if got != e:
+            bad.append((len(ns), len(a), kw, ss is not None, is32, got[:3], e[:3], got[3][:60], e[3][:60]))
+    assert not bad, bad[:4]
+
+
+def test_gpu_tile_boundaries(oracle, product):
+    # test/test_gpu.c:40-94: TGG^k vs W^m through ns_gpu_global, against the CPU result
+    for nl, al in [(3, 1), (9, 3), (12, 4), (15, 5), (18, 6), (21, 7), (24, 8), (27, 9), (45, 15), (48, 16), (51, 17),
+                   (195, 65), (771, 257), (1539, 513), (3075, 1025)]:
+        ns, a = (b"TGG" * nl)[:nl], "W" * al
+        exp = oracle.global_dp(ns, a, oracle.opt(flag=NS_F_CIGAR), mode=16)
+        ok, got = product.gpu_global(ns, a, product.opt(flag=NS_F_CIGAR))
+        assert ok, product.error()
+        assert got == exp, (nl, al)
+        assert got[3] == "%dM" % al
+
+
+def test_int32_fallback_and_closed_form(oracle, product):
+    # test/test_gpu.c:96-125: W-W scored 127 -> best score 38100 > 30000 -> int32; the GPU entry refuses
+    mat = list(oracle.blosum62)
+    trp = (nsffi.C.c_uint8 * 256).in_dll(oracle.lib, "or_aa20")[ord("W")]   # ns_tab_aa20['W']
+    assert mat[trp * 22 + trp] == 11
+    mat[trp * 22 + trp] = 127
+    ns, a = b"TGG" * 300, "W" * 300
+    o = product.opt(flag=NS_F_CIGAR, mat=mat)
+    ok, _ = product.gpu_global(ns, a, o)
+    assert not ok
+    exp = oracle.global_dp(ns, a, oracle.opt(flag=NS_F_CIGAR, mat=mat), mode=32)
+    assert product.run("ns_global_gs_auto", ns, a, o) == exp == (900, 300, 38100, "300M")
+    # 3000 x W: 33000 > 30000 picks int32 in gs_auto; score 11 per codon
+    ns, a = b"TGG" * 3000, "W" * 3000
+    got = product.run("ns_global_gs_auto", ns, a, product.opt(flag=NS_F_CIGAR))
+    assert got == (9000, 3000, 33000, "3000M")
+    assert product.run("ns_global_gs32b", ns, a, product.opt(flag=NS_F_CIGAR)) == got
+
+
+@pytest.mark.parametrize("is32", [0, 1])
+def test_global_cigar_edges(oracle, product, is32):
+    rng = np.random.default_rng(100 + is32)
+    cases = []
+    for al in AL_EDGES:
+        ns, a = planted_case(rng, al)
+        cases.append((ns, a, dict(flag=NS_F_CIGAR, **random_opt_kw(rng)), random_ss(rng, len(ns)), is32))
+    _check_many(oracle, product, cases)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_global_cigar_random(oracle, product, seed):
+    rng = np.random.default_rng(seed)
+    cases = []
+    for _ in range(40):
+        al = int(rng.integers(1, 400)) if rng.random() < 0.8 else int(rng.integers(400, 1200))
+        ns, a = planted_case(rng, al)
+        if rng.random() < 0.2:   # targets much shorter or longer than 3 al
+            ns = ns[:int(rng.integers(0, len(ns) + 1))] if rng.random() < 0.5 else ns + ns
+        cases.append((ns, a, dict(flag=NS_F_CIGAR, **random_opt_kw(rng)), random_ss(rng, len(ns)),
+                      int(rng.integers(0, 2))))
+    _check_many(oracle, product, cases)
+
+
+def test_global_pssm(oracle, product):
+    rng = np.random.default_rng(7)
+    cases = []
+    for al in (5, 64, 130, 257, 700):
+        ns, a = planted_case(rng, al)
+        pssm = rng.integers(-6, 9, size=(al, 22)).astype(np.int8)
+        for is32 in (0, 1):
+            cases.append((ns, a, dict(flag=NS_F_CIGAR, pssm=pssm), None, is32))
+    _check_many(oracle, product, cases)
+
+
+def test_global_score_only(oracle, product):
+    # int16 score-only global (x-drop and length penalty apply, nasw-sse.c:360-459) and int32
+    # score-only (no x-drop); EXT_LEFT keeps the left preparation in the int32 score-only path
+    rng = np.random.default_rng(11)
+    for k in range(30):
+        al = int(rng.integers(1, 900))
+        ns, a = planted_case(rng, al)
+        kw = random_opt_kw(rng)
+        if k % 3 == 0:
+            kw["xdrop"] = 30
+        ss = random_ss(rng, len(ns))
+        for is32, flag in ((0, 0), (1, 0), (1, NS_F_EXT_LEFT)):
+            exp = oracle.global_dp(ns, a, oracle.opt(flag=flag, **kw), ss, mode=32 if is32 else 16)
+            got = product.run("ns_global_gs32b" if is32 else "ns_global_gs16b", ns, a, product.opt(flag=flag, **kw), ss)
+            assert got == exp, (k, is32, flag, len(ns), al)
+
+
+def test_gpu_submit_wait(oracle, product):
+    # test/test_gpu.c:196-220: several submissions in flight, each consumed once
+    L = product.lib
+    rng = np.random.default_rng(5)
+    reqs = []
+    for _ in range(8):
+        ns, a = planted_case(rng, int(rng.integers(3, 300)))
+        o = product.opt(flag=NS_F_CIGAR)
+        q = L.ns_gpu_submit(None, ns, len(ns), a.encode(), len(a), nsffi.C.byref(o), None)
+        assert q, product.error()
+        reqs.append((q, ns, a, o))
+    for q, ns, a, o in reversed(reqs):
+        r = nsffi.NsRst()
+        assert L.ns_gpu_wait(q, nsffi.C.byref(r))
+        got = (r.nt_len, r.aa_len, r.score, nsffi.cigar_str(r))
+        if r.cigar:
+            product.free(nsffi.C.cast(r.cigar, nsffi.C.c_void_p))
+        assert got == oracle.global_dp(ns, a, oracle.opt(flag=NS_F_CIGAR), mode=16)
+    assert not L.ns_gpu_submit(None, b"AC", 2, b"W", 1, nsffi.C.byref(product.opt(flag=NS_F_CIGAR)), None)
+
+
+def test_ext_flags_with_cigar_are_score_only(oracle, product):
+    # ns_global_gs16b with EXT_* and CIGAR computes the extension score without a CIGAR
+    rng = np.random.default_rng(9)
+    for flag in (NS_F_EXT_LEFT, NS_F_EXT_RIGHT):
+        ns, a = planted_case(rng, 150)
+        kw = dict(flag=flag | NS_F_CIGAR)
+        w = product.lib.ns_ext_stripe_width(nsffi.C.byref(product.opt(**kw)), len(a))
+        exp = oracle.global_dp(ns, a, oracle.opt(**kw), mode=16, ext_width=w)
+        assert product.run("ns_global_gs16b", ns, a, product.opt(**kw)) == exp
