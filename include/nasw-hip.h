@@ -135,6 +135,16 @@ int ns_gpu_gapless_score_batch(const uint8_t *packed_nt, int64_t packed_nt_bases
 		const int8_t *score_matrix, int32_t asize, const uint8_t *codon,
 		int32_t *scores);
 
+/* ---- profile-HMM Viterbi on the GPU (mppriv.h:128-134; hmm-gpu.c:282-352) ----
+ * The profile object belongs to the caller's hmm.c (accessors mp_hmm_length, mp_hmm_emissions,
+ * mp_hmm_transitions, mp_hmm_gpu_slot); its device copy is cached in gpu slot 0.
+ * mp_hmm_dp_score_gpu_batch: score[i] = mp_hmm_dp_score(hmm, nt + off[i], len[i]); 1 on success.
+ * mp_hmm_dp_score_gpu: INT32_MIN when nlen < 3 or the GPU path failed. */
+struct mp_hmm_s;
+int mp_hmm_dp_score_gpu_batch(const struct mp_hmm_s *hmm, int32_t n, const uint8_t *nt, int64_t l_nt, const int32_t *off, const int32_t *len, int32_t *score);
+int32_t mp_hmm_dp_score_gpu(const struct mp_hmm_s *hmm, const uint8_t *nt, int32_t nlen);
+void mp_hmm_gpu_release(struct mp_hmm_s *hmm);
+
 /* ---- extension building blocks (nasw-sse.c:744-761) ---- */
 uint8_t *ns_ext_prep(void *km, const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, const uint8_t *ss, uint8_t **aas, int8_t **donor, int8_t **acceptor);
 int32_t ns_ext_stripe_width(const ns_opt_t *opt, int32_t al);
@@ -197,6 +207,20 @@ int nsh_ext_plan_results(nsh_ext_plan_t *p, nsh_ext_result_t *out); /* copy resu
 double nsh_ext_plan_cells(const nsh_ext_plan_t *p, int32_t kind);   /* 0: sum nl*al, 1: sum (rows done)*al after a run */
 float nsh_ext_plan_last_ms(nsh_ext_plan_t *p, int32_t which);  /* 0: whole run, 1: DP kernels only (HIP events) */
 void nsh_ext_plan_destroy(nsh_ext_plan_t *p);
+
+/* Profile-HMM engine behind mp_hmm_dp_score_gpu*: a profile resident in HBM (emissions M x 20 in
+ * HMMER column order, best residue at +4, transitions M x 7 m->m m->i m->d i->m i->i d->m d->d, as
+ * mp_hmm_read1 stores them, hmm.c:112-195). Scores equal mp_hmm_dp_score (hmm.c:252-291). M <= 1024. */
+typedef struct nsh_hmm_s nsh_hmm_t;
+nsh_hmm_t *nsh_hmm_create(int32_t M, const int16_t *emit, const int16_t *trans);
+int nsh_hmm_score_batch(nsh_hmm_t *h, int32_t n, const uint8_t *nt, int64_t l_nt, const int32_t *off,
+		const int32_t *len, int32_t *score);
+void nsh_hmm_destroy(nsh_hmm_t *h);
+
+/* Best ungapped PSSM segment of item k: ps_ungapped(pssm, L, aa + orf_off[k], orf_len[k], diag[k])
+ * (prescreen.c:334-344) -> score[k], query interval [qs[k], qe[k]). pssm: L x 22 int8 rows. */
+int nsh_pssm_ungapped_batch(const int8_t *pssm, int32_t L, const uint8_t *aa, int64_t l_aa, int32_t n,
+		const int64_t *orf_off, const int32_t *orf_len, const int32_t *diag, int32_t *score, int32_t *qs, int32_t *qe);
 
 int nsh_device_synchronize(void);
 const char *nsh_build_info(void);

@@ -284,7 +284,7 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 		const int ov = __shfl_xor(bv, off), oj = __shfl_xor(bj, off);
 		if (ov > bv || (ov == bv && oj < bj)) bv = ov, bj = oj;
 	}
-	if (L == 0) {
+	{   // every lane holds the same values: all store them (no lane-dependent branch)
 		int32_t *o = a.out + 4 * (int64_t)P.out;
 		if (P.mode) o[0] = nl, o[1] = al, o[2] = last_ev;   // nasw-sse.c:559: H(last row, al-1)
 		else o[0] = best_i + 1, o[1] = bj + 1, o[2] = bv;
@@ -299,9 +299,9 @@ __global__ void __launch_bounds__(64) ext_dp_kernel(ExtArgs a)
 	__shared__ int8_t prof[NRES * 128 * CHI];
 	const int L = threadIdx.x;
 	for (;;) {
-		int p = 0;
-		if (L == 0) p = atomicAdd(a.counter, 1);
-		p = __builtin_amdgcn_readlane(p, 0);
+		// ticket: every lane takes part in the atomic (lane 0 adds 1, the others 0), so the loop has
+		// no lane-dependent branch; the ticket is lane 0's value, wave-uniform
+		const int p = __builtin_amdgcn_readfirstlane(atomicAdd(a.counter, L == 0? 1 : 0));
 		if (p >= a.n) break;
 		const ExtProb P = a.prob[p];
 		const int C = P.C < CLO? CLO : P.C > CHI? CHI : P.C;

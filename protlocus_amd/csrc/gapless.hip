@@ -1,0 +1,286 @@
+// Gapless X-drop candidate scoring (ns_gpu_gapless_*: nasw.h:157-184; semantics of
+// nasw-cuda.cu:1151-1295, host side 1296-1412).
+//
+// One wave64 scores one candidate. For each of its (at most 32) anchors:
+//   * seed: the lanes split the block_width target end positions, each lane sums the k-mer
+//     score at its positions (codons translated on the fly from the 4-bit packed genome, the
+//     reverse strand complemented on the fly), and a wave reduction keeps the best score with
+//     the smallest target end;
+//   * extension: left and right X-drop walks run 64 steps at a time: every lane scores one step,
+//     a wave prefix sum gives the running score, a prefix max the running best, and the first
+//     step that leaves the diagonal or drops more than xdrop below the best ends the walk;
+//   * the wave keeps the segments in score order (ties: later after earlier) in LDS and, after the
+//     last anchor, sums them with the overlap discount sc * (len - ov) / len.
+// The genome stays resident in HBM across calls (re-uploaded only when the caller's buffer or
+// its length changes, or after ns_gpu_gapless_reset_target()).
+#include <hip/hip_runtime.h>
+#include <atomic>
+#include <chrono>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include "nsh_internal.h"
+
+namespace {
+
+#define DEV static __device__ __forceinline__
+
+constexpr int kWaves = 4;          // candidates per work-group
+constexpr int kMaxAnchors = 32;
+
+struct GlArgs {
+	const uint8_t *pk;                            // 4-bit packed genome, 2 bases per byte (low nibble first)
+	const uint8_t *q;                             // aa codes of all queries
+	const ns_gpu_gapless_candidate_t *cand;
+	const ns_gpu_gapless_anchor_t *anch;
+	int32_t n, bw, kmer, xdrop, asize;
+	const int8_t *mat;
+	const uint8_t *codon;
+	int32_t *score;
+};
+
+// base at oriented position pos of a contig (4 outside it); reverse: complement of the mirrored base
+DEV int gl_base(const uint8_t *pk, int64_t off, int64_t len, int rev, int64_t pos)
+{
+	if (pos < 0 || pos >= len) return 4;
+	const int64_t a = off + (rev? len - 1 - pos : pos);
+	const int b = (pk[a >> 1] >> ((a & 1) * 4)) & 0xf;
+	return rev && b < 4? 3 - b : b;
+}
+// amino acid of the codon ending at oriented position end; 20 (stop) for any base outside ACGT
+DEV int gl_aa(const uint8_t *pk, const uint8_t *codon, int64_t off, int64_t len, int rev, int64_t end)
+{
+	const int b0 = gl_base(pk, off, len, rev, end - 2), b1 = gl_base(pk, off, len, rev, end - 1),
+		b2 = gl_base(pk, off, len, rev, end);
+	return b0 < 4 && b1 < 4 && b2 < 4? codon[b0 << 4 | b1 << 2 | b2] : 20;
+}
+
+DEV int wave_sum_incl(int v, int L)
+{
+#pragma unroll
+	for (int d = 1; d < 64; d <<= 1) { const int t = __shfl_up(v, d); if (L >= d) v += t; }
+	return v;
+}
+DEV int wave_max_incl(int v, int L)
+{
+#pragma unroll
+	for (int d = 1; d < 64; d <<= 1) { const int t = __shfl_up(v, d); if (L >= d) v = max(v, t); }
+	return v;
+}
+
+// one X-drop walk along the diagonal, 64 steps per iteration; step(st, &v) is false once step st
+// (1-based) leaves the sequences. Sequential meaning: stop before an invalid step; after a valid
+// step add its score, raise the best on a strict gain, and stop when the best exceeds the running
+// score by more than xdrop. Returns the best gain (>= 0); *len = the step where it was first reached.
+template <class F>
+DEV int xdrop_walk(const F &step, int xdrop, int L, int *len)
+{
+	int lr = 0, lb = 0, ll = 0;
+	for (int base = 1;; base += 64) {
+		int v = 0;
+		const bool valid = step(base + L, &v);
+		const int S = lr + wave_sum_incl(valid? v : 0, L);
+		const int Mx = max(lb, wave_max_incl(S, L));
+		const uint64_t sb = __ballot(!valid || Mx - S > xdrop);
+		const int f = sb? __builtin_ctzll(sb) : 64;
+		const int np = f == 64? 64 : (((__ballot(valid) >> f) & 1)? f + 1 : f);   // steps taken
+		int cm = L < np? S : INT_MIN;
+#pragma unroll
+		for (int d = 32; d >= 1; d >>= 1) cm = max(cm, __shfl_xor(cm, d));
+		if (cm > lb) {
+			ll = base + __builtin_ctzll(__ballot(L < np && S == cm));
+			lb = cm;
+		}
+		if (sb) break;   // the walk ended inside this chunk (possibly after its last step)
+		lr = __shfl(S, 63);
+	}
+	*len = ll;
+	return lb;
+}
+
+__global__ void __launch_bounds__(64 * kWaves) gapless_kernel(GlArgs a)
+{
+	__shared__ int seg[kWaves][3][kMaxAnchors];
+	const int L = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
+	const int ci = blockIdx.x * kWaves + w;
+	if (ci >= a.n) return;   // whole waves leave; the kernel has no barrier
+	const ns_gpu_gapless_candidate_t c = a.cand[ci];
+	const uint8_t *qq = a.q + c.query_offset;
+	const int kmer = a.kmer, asz = a.asize, xdrop = a.xdrop;
+	const int64_t off = c.target_offset, tlen = c.target_length;
+	const int rev = c.reverse;
+	const int na = min(c.anchor_count, kMaxAnchors);
+	int *sq = seg[w][0], *se = seg[w][1], *sv = seg[w][2];
+	int n_seg = 0;
+	for (int an = 0; an < na; ++an) {
+		const ns_gpu_gapless_anchor_t A = a.anch[c.anchor_offset + an];
+		const int bq = A.query_end;
+		if (bq < kmer - 1 || bq >= c.query_length) continue;
+		// best k-mer seed over the block; ties go to the smallest target end
+		int best = INT_MIN;
+		int64_t bt = INT64_MAX;
+		for (int d = L; d < a.bw; d += 64) {
+			const int64_t te = A.target_block + d;
+			if (te - (int64_t)(kmer - 1) * 3 - 2 < 0 || te >= tlen) continue;
+			int sc = 0;
+			for (int s = 0; s < kmer; ++s)
+				sc += a.mat[gl_aa(a.pk, a.codon, off, tlen, rev, te - (int64_t)(kmer - 1 - s) * 3) * asz + qq[bq - kmer + 1 + s]];
+			if (sc > best) best = sc, bt = te;
+		}
+#pragma unroll
+		for (int d = 32; d >= 1; d >>= 1) {
+			const int ob = __shfl_xor(best, d);
+			const int64_t ot = __shfl_xor(bt, d);
+			if (ob > best || (ob == best && ot < bt)) best = ob, bt = ot;
+		}
+		if (best == INT_MIN) continue;   // no usable seed: no segment
+		int ll, rl;
+		const int lb = xdrop_walk([&](int st, int *v) {
+			const int qp = bq - kmer - (st - 1);
+			const int64_t te = bt - (int64_t)(kmer - 1 + st) * 3;
+			if (qp < 0 || te - 2 < 0) return false;
+			*v = a.mat[gl_aa(a.pk, a.codon, off, tlen, rev, te) * asz + qq[qp]];
+			return true;
+		}, xdrop, L, &ll);
+		const int rb = xdrop_walk([&](int st, int *v) {
+			const int qp = bq + st;
+			const int64_t te = bt + (int64_t)st * 3;
+			if (qp >= c.query_length || te >= tlen) return false;
+			*v = a.mat[gl_aa(a.pk, a.codon, off, tlen, rev, te) * asz + qq[qp]];
+			return true;
+		}, xdrop, L, &rl);
+		{   // insert into the score-descending list, after equal scores. Every lane runs this
+			// wave-uniform code on the same LDS words with the same values (no lane branch).
+			const int sc = best + lb + rb, qs = bq - (kmer - 1) - ll, qe = bq + rl;
+			int j = n_seg;
+			for (; j > 0 && sv[j - 1] < sc; --j) sq[j] = sq[j - 1], se[j] = se[j - 1], sv[j] = sv[j - 1];
+			sq[j] = qs, se[j] = qe, sv[j] = sc;
+		}
+		++n_seg;
+	}
+	{
+		int total = 0;
+		for (int i = 0; i < n_seg; ++i) {
+			const int len = se[i] - sq[i] + 1;
+			if (i > 0 && sv[i] < xdrop) break;
+			int ov = 0;
+			for (int j = 0; j < i; ++j) {
+				const int s = max(sq[i], sq[j]), e = min(se[i], se[j]);
+				if (sv[j] != INT_MIN && e >= s) ov += e - s + 1;
+			}
+			if (ov >= len) { sv[i] = INT_MIN; continue; }
+			total += ov > 0? (int)((int64_t)sv[i] * (len - ov) / len) : sv[i];
+		}
+		a.score[ci] = n_seg? total : INT_MIN;
+	}
+}
+
+// ---------------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------------
+struct DevBuf { void *p = nullptr; size_t cap = 0; };
+enum { B_TARGET, B_QUERY, B_CAND, B_ANCH, B_MAT, B_CODON, B_SCORE, B_COUNT };
+std::mutex g_mu;   // calls come from worker threads; buffers and the stream are shared
+DevBuf g_buf[B_COUNT];
+const uint8_t *g_target_src = nullptr;
+int64_t g_target_bases = 0;
+hipStream_t g_stream = nullptr;
+std::atomic<uint64_t> g_batches{0}, g_cands{0}, g_anchors{0}, g_seedpos{0};
+
+bool ensure(DevBuf &b, size_t n)
+{
+	if (b.cap >= n && b.p) return true;
+	if (b.p) (void)hipFree(b.p), b.p = nullptr, b.cap = 0;
+	const size_t cap = n + n / 4 + 256;
+	if (hipMalloc(&b.p, cap) != hipSuccess) { nsh_set_error("gapless: hipMalloc(%zu) failed", cap); return false; }
+	b.cap = cap;
+	return true;
+}
+
+} // namespace
+
+extern "C" void ns_gpu_gapless_profile(ns_gpu_gapless_profile_t *profile)
+{
+	profile->batches = g_batches.load(), profile->candidates = g_cands.load();
+	profile->anchors = g_anchors.load(), profile->seed_positions = g_seedpos.load();
+}
+
+extern "C" int ns_gpu_gapless_supported(void) { return nsh_ensure_device()? 1 : 0; }
+
+extern "C" void ns_gpu_gapless_reset_target(void)
+{
+	std::lock_guard<std::mutex> lk(g_mu);
+	g_target_src = nullptr, g_target_bases = 0;
+}
+
+#define GLOK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { nsh_set_error("gapless: %s: %s", #x, hipGetErrorString(e_)); return 0; } } while (0)
+
+extern "C" int ns_gpu_gapless_score_batch(const uint8_t *packed_nt, int64_t packed_nt_bases,
+		const uint8_t *queries, int64_t query_aa,
+		const ns_gpu_gapless_candidate_t *candidates, int32_t n_candidates,
+		const ns_gpu_gapless_anchor_t *anchors, int64_t n_anchors,
+		int32_t block_width, int32_t kmer, int32_t xdrop,
+		const int8_t *score_matrix, int32_t asize, const uint8_t *codon,
+		int32_t *scores)
+{
+	if (!packed_nt || packed_nt_bases < 1 || !queries || query_aa < 1 || !candidates || n_candidates < 1 ||
+			!anchors || n_anchors < 1 || block_width < 1 || kmer < 1 || xdrop < 0 || !score_matrix ||
+			asize < 1 || !codon || !scores)
+		return 0;
+	if (!nsh_ensure_device()) return 0;
+	// every device access stays inside the buffers: residue codes index the matrix, contigs lie in
+	// the genome, queries and anchors in their arrays
+	if (asize < 21) { nsh_set_error("gapless: asize %d < 21 (stop codons score row 20)", asize); return 0; }
+	for (int t = 0; t < 64; ++t) if (codon[t] >= asize) { nsh_set_error("gapless: codon table out of range"); return 0; }
+	for (int64_t k = 0; k < query_aa; ++k)
+		if (queries[k] >= asize) { nsh_set_error("gapless: query residue %lld out of range", (long long)k); return 0; }
+	uint64_t n_anch_used = 0;
+	for (int32_t i = 0; i < n_candidates; ++i) {
+		const ns_gpu_gapless_candidate_t &c = candidates[i];
+		const int64_t na = c.anchor_count < kMaxAnchors? c.anchor_count : kMaxAnchors;
+		if (c.target_offset < 0 || c.target_length < 0 || c.target_offset + c.target_length > packed_nt_bases ||
+				c.query_offset < 0 || c.query_length < 0 || c.query_offset + c.query_length > query_aa ||
+				(na > 0 && (c.anchor_offset < 0 || c.anchor_offset + na > n_anchors))) {
+			nsh_set_error("gapless: candidate %d out of range", i);
+			return 0;
+		}
+		n_anch_used += na > 0? na : 0;
+	}
+	std::lock_guard<std::mutex> lk(g_mu);
+	const size_t sz[B_COUNT] = { (size_t)(packed_nt_bases + 1) >> 1, (size_t)query_aa,
+		(size_t)n_candidates * sizeof(*candidates), (size_t)n_anchors * sizeof(*anchors),
+		(size_t)asize * asize, 64, (size_t)n_candidates * sizeof(int32_t) };
+	for (int b = 0; b < B_COUNT; ++b) if (!ensure(g_buf[b], sz[b])) return 0;
+	if (!g_stream) GLOK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+	auto t0 = std::chrono::steady_clock::now();
+	if (g_target_src != packed_nt || g_target_bases != packed_nt_bases) {
+		g_target_src = nullptr;
+		GLOK(hipMemcpyAsync(g_buf[B_TARGET].p, packed_nt, sz[B_TARGET], hipMemcpyHostToDevice, g_stream));
+	}
+	GLOK(hipMemcpyAsync(g_buf[B_QUERY].p, queries, sz[B_QUERY], hipMemcpyHostToDevice, g_stream));
+	GLOK(hipMemcpyAsync(g_buf[B_CAND].p, candidates, sz[B_CAND], hipMemcpyHostToDevice, g_stream));
+	GLOK(hipMemcpyAsync(g_buf[B_ANCH].p, anchors, sz[B_ANCH], hipMemcpyHostToDevice, g_stream));
+	GLOK(hipMemcpyAsync(g_buf[B_MAT].p, score_matrix, sz[B_MAT], hipMemcpyHostToDevice, g_stream));
+	GLOK(hipMemcpyAsync(g_buf[B_CODON].p, codon, 64, hipMemcpyHostToDevice, g_stream));
+	GLOK(hipStreamSynchronize(g_stream));
+	g_target_src = packed_nt, g_target_bases = packed_nt_bases;
+	auto t1 = std::chrono::steady_clock::now();
+	GlArgs a;
+	a.pk = (const uint8_t*)g_buf[B_TARGET].p, a.q = (const uint8_t*)g_buf[B_QUERY].p;
+	a.cand = (const ns_gpu_gapless_candidate_t*)g_buf[B_CAND].p, a.anch = (const ns_gpu_gapless_anchor_t*)g_buf[B_ANCH].p;
+	a.n = n_candidates, a.bw = block_width, a.kmer = kmer, a.xdrop = xdrop, a.asize = asize;
+	a.mat = (const int8_t*)g_buf[B_MAT].p, a.codon = (const uint8_t*)g_buf[B_CODON].p, a.score = (int32_t*)g_buf[B_SCORE].p;
+	hipLaunchKernelGGL(gapless_kernel, dim3((n_candidates + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, g_stream, a);
+	GLOK(hipGetLastError());
+	GLOK(hipStreamSynchronize(g_stream));
+	auto t2 = std::chrono::steady_clock::now();
+	GLOK(hipMemcpyAsync(scores, g_buf[B_SCORE].p, sz[B_SCORE], hipMemcpyDeviceToHost, g_stream));
+	GLOK(hipStreamSynchronize(g_stream));
+	auto t3 = std::chrono::steady_clock::now();
+	using sec = std::chrono::duration<double>;
+	nsh_add_times(sec(t1 - t0).count(), sec(t2 - t1).count(), sec(t3 - t2).count());
+	nsh_count_launch(1);
+	++g_batches, g_cands += n_candidates, g_anchors += n_anch_used, g_seedpos += n_anch_used * (uint64_t)block_width;
+	return 1;
+}

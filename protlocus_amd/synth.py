@@ -135,3 +135,67 @@ def pack_batch(pairs):
         ns_off[1:] = np.cumsum(nl.astype(np.int64))[:-1]
         as_off[1:] = np.cumsum(al.astype(np.int64))[:-1]
     return b"".join(d for d, _ in pairs), ns_off, nl, b"".join(p for _, p in pairs), as_off, al
+
+
+# ---------------------------------------------------------------------------------------
+# profile HMMs (HMMER3 text) and genomes for the profile workloads
+# ---------------------------------------------------------------------------------------
+HMM_ALPHA = "ACDEFGHIKLMNPQRSTVWY"   # HMMER3 emission column order
+HMM_BG = np.array([BG[AA.index(c)] for c in HMM_ALPHA])
+
+
+def hmm_params(rng, M, alpha=0.5, ins_ext=0.5, del_rate=0.05):
+    """match emissions ~ Dirichlet(alpha * background) per state; i->i = ins_ext (expected
+    insert length 1/(1-ins_ext)); m->d = del_rate. Returns probabilities (M x 20, M x 7) with
+    transitions in HMMER order m->m m->i m->d i->m i->i d->m d->d."""
+    em = rng.dirichlet(alpha * HMM_BG, size=M)
+    em = np.maximum(em, 1e-6)
+    em /= em.sum(axis=1, keepdims=True)
+    mi = 0.02
+    tr = np.zeros((M, 7))
+    tr[:, 0], tr[:, 1], tr[:, 2] = 1 - mi - del_rate, mi, del_rate
+    tr[:, 3], tr[:, 4] = 1 - ins_ext, ins_ext
+    tr[:, 5], tr[:, 6] = 0.7, 0.3
+    tr[-1, 2], tr[-1, 0] = 0.0, 1 - mi      # no delete out of the last state
+    tr[-1, 5], tr[-1, 6] = 1.0, 0.0
+    return em, tr
+
+
+def hmm_text(em, tr, name="synthetic"):
+    """one HMMER3 record: probabilities written as -ln(p) with 5 decimals, 0 as '*'"""
+    M = len(em)
+    f = lambda p: "*" if p <= 0 else "%.5f" % (-np.log(p))
+    out = ["HMMER3/f [synthetic]", "NAME  %s" % name, "LENG  %d" % M, "ALPH  amino",
+           "HMM          " + "        ".join(HMM_ALPHA),
+           "            m->m     m->i     m->d     i->m     i->i     d->m     d->d",
+           "  COMPO   " + " ".join(f(p) for p in HMM_BG),
+           "          " + " ".join(f(p) for p in HMM_BG),
+           "          " + " ".join(f(p) for p in [0.98, 0.01, 0.01, 0.5, 0.5, 1.0, 0.0])]
+    for k in range(M):
+        out.append("%7d   " % (k + 1) + " ".join(f(p) for p in em[k]) + " %d - - - -" % (k + 1))
+        out.append("          " + " ".join(f(p) for p in HMM_BG))
+        out.append("          " + " ".join(f(p) for p in tr[k]))
+    out.append("//")
+    return "\n".join(out) + "\n"
+
+
+def hmm_sample_protein(rng, em, tr, max_len=None):
+    """a protein emitted by a path through the profile (match, insert and delete states)"""
+    M = len(em)
+    seq, k, state = [], 0, 0   # state 0: match, 1: insert, 2: delete
+    while k < M:
+        if state == 0:
+            seq.append(HMM_ALPHA[rng.choice(20, p=em[k])])
+            r = rng.random()
+            state = 0 if r < tr[k, 0] else (1 if r < tr[k, 0] + tr[k, 1] else 2)
+            if state != 1:
+                k += 1
+        elif state == 1:
+            seq.append(HMM_ALPHA[rng.choice(20, p=HMM_BG)])
+            state = 1 if rng.random() < tr[k - 1 if k else 0, 4] else 0
+            if state == 0:
+                k += 1
+        else:
+            state = 2 if rng.random() < tr[k - 1 if k else 0, 6] else 0
+            k += 1
+    return "".join(seq[:max_len] if max_len else seq)

@@ -1,0 +1,116 @@
+"""Seeded input generators shared by the parity tests and the golden-fixture script."""
+import numpy as np
+
+AA = "ACDEFGHIKLMNPQRSTVWY"
+CODON = ["GCT", "TGT", "GAT", "GAA", "TTT", "GGT", "CAT", "ATT", "AAA", "TTA", "ATG", "AAT", "CCT", "CAA", "CGT",
+         "TCT", "ACT", "GTT", "TGG", "TAT"]
+
+
+def planted_case(rng, al, flank=None, fs_rate=0.004):
+    """random protein and a DNA that holds a mutated, maybe spliced and frameshifted copy of it
+    (a quarter of the targets are unrelated random DNA of the same length)"""
+    a = "".join(rng.choice(list(AA), size=al))
+    flank = int(rng.integers(0, 30)) if flank is None else flank
+    nt = [rng.choice(list("ACGT"), size=flank)]
+    for k, c in enumerate(a):
+        cod = list(CODON[AA.index(c)])
+        if rng.random() < 0.15:
+            cod[int(rng.integers(0, 3))] = "ACGT"[int(rng.integers(0, 4))]
+        nt.append(np.array(cod))
+        if k == al // 2 and rng.random() < 0.4:   # an intron
+            nt.append(np.array(list("GTAAG") + list(rng.choice(list("ACGT"), size=int(rng.integers(30, 120)))) +
+                               list("TTTCAG")))
+        if rng.random() < fs_rate:                # a frameshift
+            nt.append(rng.choice(list("ACGT"), size=int(rng.integers(1, 3))))
+    nt.append(rng.choice(list("ACGT"), size=int(rng.integers(0, 30))))
+    ns = "".join("".join(x) for x in nt).encode()
+    if rng.random() < 0.25:
+        ns = rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), size=len(ns)).tobytes()
+    return ns, a
+
+
+def random_opt_kw(rng):
+    kw = {}
+    if rng.random() < 0.3:
+        kw["go"] = int(rng.integers(3, 13))
+    if rng.random() < 0.25:
+        kw["fs"] = int(rng.integers(5, 25))
+    if rng.random() < 0.25:
+        kw["io"] = int(rng.integers(10, 40))
+    return kw
+
+
+def random_ss(rng, nl, p=0.3):
+    """a splice-score track (0xff = unset, score << 1 | acceptor) with probability p, else None"""
+    if rng.random() >= p:
+        return None
+    ss = np.full(nl, 0xff, dtype=np.uint8)
+    m = rng.random(nl) < 0.2
+    ss[m] = ((rng.integers(0, 40, size=m.sum()) + 44) << 1 | rng.integers(0, 2, size=m.sum())).astype(np.uint8)
+    return bytes(ss)
+
+
+def pack4(nt4):
+    """4-bit packing of base codes, two per byte, low nibble first (ntseq.c layout)"""
+    nt4 = np.asarray(nt4, dtype=np.uint8)
+    if len(nt4) & 1:
+        nt4 = np.append(nt4, 0)
+    return (nt4[0::2] | (nt4[1::2] << 4)).astype(np.uint8).tobytes()
+
+
+def oriented(g, off, clen, rev):
+    """the contig as the gapless kernel reads it: reverse strand = reversed complement, N stays N"""
+    o = np.asarray(g[off:off + clen], dtype=np.uint8)
+    if rev:
+        o = o[::-1].copy()
+        m = o < 4
+        o[m] = 3 - o[m]
+    return o
+
+
+def gapless_batch(rng, n_cand, codon, kmer, bw, genome_len=60000, n_contigs=3):
+    """a random genome (with N runs) split into contigs; queries either random or the translated,
+    mutated frame of a contig window; anchors near the planted diagonal, at random, or out of range"""
+    import nsffi
+    g = rng.integers(0, 4, size=genome_len).astype(np.uint8)
+    for _ in range(20):
+        s = int(rng.integers(0, genome_len - 50))
+        g[s:s + int(rng.integers(1, 40))] = 4
+    cuts = np.sort(rng.choice(np.arange(1000, genome_len - 1000), size=n_contigs - 1, replace=False))
+    bounds = [0] + [int(x) for x in cuts] + [genome_len]
+    contigs = [(bounds[i], bounds[i + 1] - bounds[i]) for i in range(n_contigs)]
+    queries, cands, anchors = bytearray(), [], []
+    for _ in range(n_cand):
+        ci = int(rng.integers(0, n_contigs))
+        off, clen = contigs[ci]
+        rev = int(rng.integers(0, 2))
+        qlen = int(rng.integers(1, 300))
+        planted = rng.random() < 0.7 and clen > 3 * qlen + 10
+        if planted:
+            o = oriented(g, off, clen, rev)
+            e0 = int(rng.integers(2, clen - 3 * qlen + 1))
+            q = np.empty(qlen, dtype=np.uint8)
+            for k in range(qlen):
+                b = o[e0 + 3 * k - 2:e0 + 3 * k + 1]
+                q[k] = codon[b[0] << 4 | b[1] << 2 | b[2]] if b.max() < 4 else 20
+            mut = rng.random(qlen) < 0.2
+            q[mut] = rng.integers(0, 22, size=int(mut.sum()))
+        else:
+            q = rng.integers(0, 22, size=qlen).astype(np.uint8)
+        c = nsffi.GaplessCand(target_offset=off, target_length=clen, query_offset=len(queries), query_length=qlen,
+                              anchor_offset=len(anchors), anchor_count=0, reverse=rev)
+        na = int(rng.integers(0, 40)) if rng.random() < 0.95 else 0
+        for _ in range(na):
+            r = rng.random()
+            if planted and r < 0.6:
+                k = int(rng.integers(0, qlen))
+                tb = e0 + 3 * k - int(rng.integers(0, bw))
+                qe = k
+            else:
+                qe = int(rng.integers(-2, qlen + 2))
+                tb = int(rng.integers(-400, clen + 100))
+            anchors.append(nsffi.GaplessAnchor(target_block=tb, query_end=qe, reserved=0))
+        c.anchor_count = na if rng.random() < 0.97 else -1
+        queries += q.tobytes()
+        cands.append(c)
+    return g, bytes(queries), cands, anchors

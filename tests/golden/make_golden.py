@@ -1,0 +1,139 @@
+#!/usr/bin/env python3
+"""Writes the golden vectors of the spliced-DP and profile-HMM paths from the compiled
+reference (oracle/_ref/libnsref.so, built from the reference's own C sources by
+`make -C oracle ref`).
+
+    python tests/golden/make_golden.py
+
+Outputs (inputs and expected outputs only):
+  dp_cases.json.gz   ns_global_gs16b / gs32b / gs_auto: extension, CIGAR, score-only, PSSM
+  hmm_cases.json.gz  mp_hmm_read1 + mp_hmm_dp_score on synthetic HMMER3 profiles
+  tables.json        ns_tab_nt4 / ns_tab_aa20 / ns_tab_codon / ns_mat_blosum62
+"""
+import base64
+import ctypes as C
+import gzip
+import json
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [os.path.join(ROOT, "tests"), ROOT]
+
+import nsffi                                       # noqa: E402
+from cases import planted_case, random_opt_kw, random_ss   # noqa: E402
+from nsffi import NS_F_CIGAR, NS_F_EXT_LEFT, NS_F_EXT_RIGHT  # noqa: E402
+from protlocus_amd import synth                    # noqa: E402
+
+B64 = lambda b: None if b is None else base64.b64encode(b).decode()
+
+
+def dp_cases(ref):
+    rng = np.random.default_rng(20260101)
+    out = []
+
+    def add(fn, ns, a, kw, ss):
+        o = ref.opt(**kw)
+        res = ref.run(fn, ns, a, o, ss)
+        kwj = dict(kw)
+        if "pssm" in kwj:
+            kwj["pssm"] = np.asarray(kwj["pssm"]).tolist()
+        out.append({"fn": fn, "ns": ns.decode(), "as": a, "kw": kwj, "ss": B64(ss),
+                    "ext_width": ref.stripe_width(o, len(a)), "out": list(res)})
+
+    for k in range(160):    # extension, both directions, with/without ss, x-drop and end bonus variants
+        al = int(rng.integers(1, 700)) if k % 4 else int(rng.integers(1, 40))
+        ns, a = planted_case(rng, al)
+        kw = dict(flag=NS_F_EXT_LEFT if k & 1 else NS_F_EXT_RIGHT, **random_opt_kw(rng))
+        if k % 5 == 0:
+            kw["xdrop"] = int(rng.choice([20, 50, 1000000]))
+        if k % 7 == 0:
+            kw["end_bonus"] = 0
+        add("ns_global_gs16b", ns, a, kw, random_ss(rng, len(ns)))
+    for fn in ("ns_global_gs16b", "ns_global_gs32b"):   # CIGAR
+        for k in range(70):
+            al = int(rng.integers(1, 500)) if k % 3 else int(rng.integers(1, 30))
+            ns, a = planted_case(rng, al)
+            add(fn, ns, a, dict(flag=NS_F_CIGAR, **random_opt_kw(rng)), random_ss(rng, len(ns)))
+    for fn in ("ns_global_gs16b", "ns_global_gs32b"):   # score-only global
+        for k in range(25):
+            ns, a = planted_case(rng, int(rng.integers(1, 500)))
+            add(fn, ns, a, dict(flag=0, **random_opt_kw(rng)), random_ss(rng, len(ns)))
+    for k in range(16):      # PSSM rows, all modes
+        al = int(rng.integers(2, 300))
+        ns, a = planted_case(rng, al)
+        pssm = rng.integers(-6, 9, size=(al, 22)).astype(np.int8)
+        flag = [NS_F_CIGAR, NS_F_EXT_LEFT, NS_F_EXT_RIGHT, 0][k % 4]
+        add("ns_global_gs32b" if k % 8 >= 4 else "ns_global_gs16b", ns, a, dict(flag=flag, pssm=pssm), None)
+    for n in (7, 100):       # closed forms (test/test_dp32.c:33-86)
+        add("ns_global_gs_auto", b"TGG" * n, "W" * n, dict(flag=NS_F_CIGAR), None)
+    add("ns_global_gs_auto", b"TGG" * 3000, "W" * 3000, dict(flag=NS_F_CIGAR), None)
+    return out
+
+
+def hmm_cases(ref):
+    rng = np.random.default_rng(777)
+    L = ref.lib
+    out = []
+    for M in (1, 2, 7, 33, 64, 65, 150, 300):
+        em, tr = synth.hmm_params(rng, M)
+        text = synth.hmm_text(em, tr, "g%d" % M)
+        with tempfile.NamedTemporaryFile("w", suffix=".hmm", delete=False) as f:
+            f.write(text)
+        try:
+            fp = L.mp_hmm_open(f.name.encode())
+            h = L.mp_hmm_read1(fp)
+            L.mp_hmm_close(fp)
+        finally:
+            os.unlink(f.name)
+        assert h
+        regions, scores = [], []
+        for r in range(6):
+            if r < 3:     # a region holding a profile-sampled protein, reverse-translated
+                prot = synth.hmm_sample_protein(rng, em, tr)
+                cod = synth.reverse_translate(rng, prot.encode())
+                nt = np.concatenate([rng.integers(0, 4, size=int(rng.integers(0, 60))),
+                                     _nt4(cod),
+                                     rng.integers(0, 4, size=int(rng.integers(0, 60)))]).astype(np.uint8)
+            else:
+                nt = rng.integers(0, 5 if r == 5 else 4, size=int(rng.integers(0, 900))).astype(np.uint8)
+            regions.append(nt.tobytes().hex())
+            scores.append(L.mp_hmm_dp_score(h, nt.tobytes(), len(nt)))
+        L.mp_hmm_destroy(h)
+        out.append({"hmm": text, "regions": regions, "scores": scores})
+    return out
+
+
+def _nt4(codons):
+    lut = np.full(256, 4, dtype=np.uint8)
+    for i, c in enumerate(b"ACGT"):
+        lut[c] = i
+    return lut[np.frombuffer(codons, dtype=np.uint8)]
+
+
+def tables(ref):
+    L = ref.lib
+    get = lambda name, n, t=C.c_uint8: list((t * n).in_dll(L, name))
+    return {"nt4": get("ns_tab_nt4", 256), "aa20": get("ns_tab_aa20", 256), "codon": get("ns_tab_codon", 64),
+            "blosum62": get("ns_mat_blosum62", 484, C.c_int8)}
+
+
+def main():
+    if not nsffi.have_ref():
+        raise SystemExit("build the reference first: make -C oracle ref")
+    ref = nsffi.Reference()
+    with gzip.open(os.path.join(HERE, "dp_cases.json.gz"), "wt") as f:
+        json.dump(dp_cases(ref), f, separators=(",", ":"))
+    with gzip.open(os.path.join(HERE, "hmm_cases.json.gz"), "wt") as f:
+        json.dump(hmm_cases(ref), f, separators=(",", ":"))
+    with open(os.path.join(HERE, "tables.json"), "w") as f:
+        json.dump(tables(ref), f, separators=(",", ":"))
+    print("wrote golden vectors to", HERE)
+
+
+if __name__ == "__main__":
+    main()

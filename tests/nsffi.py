@@ -47,6 +47,11 @@ class GaplessAnchor(C.Structure):
     _fields_ = [("target_block", C.c_int64), ("query_end", C.c_int32), ("reserved", C.c_int32)]
 
 
+class GaplessProfile(C.Structure):
+    _fields_ = [("batches", C.c_uint64), ("candidates", C.c_uint64), ("anchors", C.c_uint64),
+                ("seed_positions", C.c_uint64)]
+
+
 class OrHmm(C.Structure):
     _fields_ = [("len", C.c_int32), ("emit", C.POINTER(C.c_int16)), ("trans", C.POINTER(C.c_int16)),
                 ("pssm", C.POINTER(C.c_int8))]
@@ -268,6 +273,17 @@ class Product(_Lib):
         L.ns_gpu_submit.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, C.c_char_p, C.c_int32, C.c_void_p, C.c_char_p]
         L.ns_gpu_submit.restype = C.c_void_p
         L.ns_gpu_wait.argtypes = [C.c_void_p, C.c_void_p]
+        L.nsh_hmm_create.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
+        L.nsh_hmm_create.restype = C.c_void_p
+        L.nsh_hmm_score_batch.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                          C.c_void_p]
+        L.nsh_hmm_destroy.argtypes = [C.c_void_p]
+        L.nsh_pssm_ungapped_batch.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int64, C.c_int32, C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ns_gpu_gapless_score_batch.argtypes = [C.c_char_p, C.c_int64, C.c_char_p, C.c_int64, C.c_void_p, C.c_int32,
+                                                 C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                                 C.c_int32, C.c_char_p, C.c_void_p]
+        L.ns_gpu_gapless_profile.argtypes = [C.c_void_p]
         assert L.ns_make_tables(1) == 0
         self._opt_init = L.ns_opt_init
         self._opt_set_sp = L.ns_opt_set_sp

@@ -16,51 +16,7 @@ from nsffi import NS_F_CIGAR, NS_F_EXT_LEFT, NS_F_EXT_RIGHT
 
 pytestmark = pytest.mark.gpu
 
-AA = "ACDEFGHIKLMNPQRSTVWY"
-CODON = ["GCT", "TGT", "GAT", "GAA", "TTT", "GGT", "CAT", "ATT", "AAA", "TTA", "ATG", "AAT", "CCT", "CAA", "CGT",
-         "TCT", "ACT", "GTT", "TGG", "TAT"]
-
-
-def planted_case(rng, al, flank=None):
-    """random protein and a DNA that holds a mutated, maybe spliced and frameshifted copy of it"""
-    a = "".join(rng.choice(list(AA), size=al))
-    flank = int(rng.integers(0, 30)) if flank is None else flank
-    nt = [rng.choice(list("ACGT"), size=flank)]
-    for k, c in enumerate(a):
-        cod = list(CODON[AA.index(c)])
-        if rng.random() < 0.15:
-            cod[int(rng.integers(0, 3))] = "ACGT"[int(rng.integers(0, 4))]
-        nt.append(np.array(cod))
-        if k == al // 2 and rng.random() < 0.4:   # an intron
-            nt.append(np.array(list("GTAAG") + list(rng.choice(list("ACGT"), size=int(rng.integers(30, 120)))) +
-                               list("TTTCAG")))
-        if rng.random() < 0.004:                  # a frameshift
-            nt.append(rng.choice(list("ACGT"), size=int(rng.integers(1, 3))))
-    nt.append(rng.choice(list("ACGT"), size=int(rng.integers(0, 30))))
-    ns = "".join("".join(x) for x in nt).encode()
-    if rng.random() < 0.25:                        # an unrelated target
-        ns = rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), size=len(ns)).tobytes()
-    return ns, a
-
-
-def random_opt_kw(rng):
-    kw = {}
-    if rng.random() < 0.3:
-        kw["go"] = int(rng.integers(3, 13))
-    if rng.random() < 0.25:
-        kw["fs"] = int(rng.integers(5, 25))
-    if rng.random() < 0.25:
-        kw["io"] = int(rng.integers(10, 40))
-    return kw
-
-
-def random_ss(rng, nl):
-    if rng.random() >= 0.3:
-        return None
-    ss = np.full(nl, 0xff, dtype=np.uint8)
-    m = rng.random(nl) < 0.2
-    ss[m] = ((rng.integers(0, 40, size=m.sum()) + 44) << 1 | rng.integers(0, 2, size=m.sum())).astype(np.uint8)
-    return bytes(ss)
+from cases import AA, planted_case, random_opt_kw, random_ss
 
 
 # lane / wave / segment boundaries: C columns per lane (1..4), 64-lane waves, up to 16 waves,

@@ -1,0 +1,336 @@
+"""CPU checks of the oracle (no GPU): the golden vectors written from the compiled reference,
+closed forms from the reference's own tests, invariants, and, where oracle/_ref is built, a
+fresh random comparison against the compiled reference.
+
+What each oracle function is pinned by:
+  or_make_tables / or_opt_init    tables.json (reference tables), closed-form entries
+  or_dp16_score (extension)       golden vectors; TGG^n/W^n closed form (test/test_dp32.c:33-54);
+                                  left/right mirror symmetry; monotone in xdrop
+  or_global16/32 (CIGAR, score)   golden vectors; 3000M / 33000 (test/test_dp32.c:72-86);
+                                  gs16b == gs32b when unsaturated (test/test_dp32.c:7-31);
+                                  CIGAR spans (nl, al) and its re-scored value equals the score
+  or_hmm_parse / or_hmm_viterbi   golden vectors; one-state closed form; brute force on tiny inputs
+  or_gapless_score                closed forms and brute force (no reference CPU form exists)
+  or_pssm_ungapped                brute force over all segments of the diagonal
+"""
+import base64
+import gzip
+import json
+import os
+
+import numpy as np
+import pytest
+
+import nsffi
+from nsffi import NS_F_CIGAR, NS_F_EXT_LEFT, NS_F_EXT_RIGHT
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load(name):
+    with gzip.open(os.path.join(GOLD, name), "rt") as f:
+        return json.load(f)
+
+
+def _opt(lib, kw):
+    kw = dict(kw)
+    if "pssm" in kw:
+        kw["pssm"] = np.array(kw["pssm"], dtype=np.int8)
+    return lib.opt(**kw)
+
+
+def _oracle_run(oracle, case):
+    ss = None if case["ss"] is None else base64.b64decode(case["ss"])
+    ns, a, o = case["ns"].encode(), case["as"], _opt(oracle, case["kw"])
+    mode = {"ns_global_gs16b": 16, "ns_global_gs32b": 32, "ns_global_gs_auto": "auto"}[case["fn"]]
+    return list(oracle.global_dp(ns, a, o, ss, mode=mode, ext_width=case["ext_width"]))
+
+
+def test_tables_match_reference(oracle):
+    import ctypes as C
+    t = json.load(open(os.path.join(GOLD, "tables.json")))
+    L = oracle.lib
+    assert list((C.c_uint8 * 256).in_dll(L, "or_nt4")) == t["nt4"]
+    assert list((C.c_uint8 * 256).in_dll(L, "or_aa20")) == t["aa20"]
+    assert list((C.c_uint8 * 64).in_dll(L, "or_codon")) == t["codon"]
+    assert oracle.blosum62 == t["blosum62"]
+    # closed forms: ACGT -> 0..3, 'N' -> 4; standard code TGG = W, ATG = M, TAA = '*'
+    aa = "ARNDCQEGHILKMFPSTWYV*X"
+    assert [t["nt4"][ord(c)] for c in "ACGTNacgtn"] == [0, 1, 2, 3, 4, 0, 1, 2, 3, 4]
+    assert [t["aa20"][ord(c)] for c in aa] == list(range(22))
+    cod = lambda s: t["codon"][(("ACGT".index(s[0])) << 4) | ("ACGT".index(s[1]) << 2) | "ACGT".index(s[2])]
+    assert (aa[cod("TGG")], aa[cod("ATG")], aa[cod("TAA")], aa[cod("GCT")]) == ("W", "M", "*", "A")
+    w = aa.index("W")
+    assert t["blosum62"][w * 22 + w] == 11 and t["blosum62"][0] == 4
+
+
+def test_oracle_matches_golden_dp(oracle):
+    bad = []
+    for k, case in enumerate(_load("dp_cases.json.gz")):
+        got = _oracle_run(oracle, case)
+        if got != case["out"]:
+            bad.append((k, case["fn"], case["kw"].get("flag"), len(case["as"]), got[:3], case["out"][:3]))
+    assert not bad, bad[:5]
+
+
+def test_oracle_matches_golden_hmm(oracle):
+    import ctypes as C
+    for case in _load("hmm_cases.json.gz"):
+        h = nsffi.OrHmm()
+        assert oracle.lib.or_hmm_parse(case["hmm"].encode(), C.byref(h), None) == 1
+        try:
+            for reg, exp in zip(case["regions"], case["scores"]):
+                nt = bytes.fromhex(reg)
+                assert oracle.lib.or_hmm_viterbi(C.byref(h), nt, len(nt)) == exp
+        finally:
+            oracle.lib.or_hmm_free(C.byref(h))
+
+
+def test_extension_closed_form(oracle):
+    # test/test_dp32.c:33-54: score 11n + 5, nt_len 3n, aa_len n, both directions
+    for flag in (NS_F_EXT_LEFT, NS_F_EXT_RIGHT):
+        for n in (1, 7, 19, 100, 333):
+            assert oracle.ext(b"TGG" * n, "W" * n, oracle.opt(flag=flag))[0] == (3 * n, n, 11 * n + 5)
+
+
+def test_global_closed_form_and_width_agreement(oracle):
+    # test/test_dp32.c:72-86 and 7-31
+    assert oracle.global_dp(b"TGG" * 3000, "W" * 3000, oracle.opt(flag=NS_F_CIGAR), mode=32) == \
+        (9000, 3000, 33000, "3000M")
+    assert oracle.global_dp(b"TGG" * 3000, "W" * 3000, oracle.opt(flag=NS_F_CIGAR), mode="auto")[2] == 33000
+    from cases import planted_case
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        ns, a = planted_case(rng, int(rng.integers(1, 200)))
+        s16 = oracle.global_dp(ns, a, oracle.opt(flag=NS_F_CIGAR), mode=16)
+        s32 = oracle.global_dp(ns, a, oracle.opt(flag=NS_F_CIGAR), mode=32)
+        if s16[2] > -30000:   # unsaturated
+            assert s16[2] == s32[2]
+
+
+def _rescore(oracle, ns, a, cigar):
+    """span of a CIGAR in (nt, aa)"""
+    import re
+    nt = aa = 0
+    for n, op in re.findall(r"(\d+)([MIDNFGUV])", cigar):
+        n = int(n)
+        nt += {"M": 3 * n, "I": 0, "D": 3 * n, "N": n, "F": n, "G": n, "U": n, "V": n}[op]
+        aa += {"M": n, "I": n, "D": 0, "N": 0, "F": 0, "G": n, "U": n, "V": n}[op]
+    return nt, aa
+
+
+def test_cigar_spans_whole_pair(oracle):
+    # a global CIGAR consumes exactly (nl, al) (align.c:86-99). G of length 2 consumes 2 nt
+    # and one residue; U/V (intron with a split codon) consume the intron and one residue
+    # per codon part, so the nt span is checked only for M/I/D/N/F-only CIGARs.
+    from cases import planted_case
+    rng = np.random.default_rng(5)
+    for _ in range(30):
+        ns, a = planted_case(rng, int(rng.integers(1, 150)))
+        for mode in (16, 32):
+            nt_len, aa_len, score, cig = oracle.global_dp(ns, a, oracle.opt(flag=NS_F_CIGAR), mode=mode)
+            nt, aa = _rescore(oracle, ns, a, cig)
+            if not any(op in cig for op in "GUV"):
+                assert (nt, aa) == (len(ns), len(a)), cig
+            assert nt_len == len(ns) and aa_len == len(a)
+
+
+def test_extension_mirror_and_xdrop_monotone(oracle):
+    from cases import planted_case
+    rng = np.random.default_rng(8)
+    for _ in range(20):
+        ns, a = planted_case(rng, int(rng.integers(1, 200)))
+        prev = None
+        for xd in (10, 50, 200, 100000):
+            sc = oracle.ext(ns, a, oracle.opt(flag=NS_F_EXT_RIGHT, xdrop=xd))[0][2]
+            assert prev is None or sc >= prev
+            prev = sc
+
+
+def test_hmm_one_state_closed_form(oracle):
+    # M = 1: every codon scores max(0, m->m... ) independently of the previous row except through
+    # the insert state; with a stop-free all-W region the best is the W emission shifted to +4
+    # when W is the most likely residue (hmm.c:152-165)
+    import ctypes as C
+    from protlocus_amd import synth
+    em = np.full((1, 20), 0.01)
+    em[0, synth.HMM_ALPHA.index("W")] = 1 - 0.19
+    tr = np.array([[0.98, 0.01, 0.01, 0.5, 0.5, 1.0, 0.0]])
+    text = synth.hmm_text(em, tr)
+    h = nsffi.OrHmm()
+    assert oracle.lib.or_hmm_parse(text.encode(), C.byref(h), None) == 1
+    nt = bytes([3, 2, 2] * 10)   # TGG x 10
+    assert oracle.lib.or_hmm_viterbi(C.byref(h), nt, len(nt)) == 4
+    nt = bytes([0, 0, 0] * 10)   # AAA = K: emission lrint(2 ln .01) - lrint(2 ln .81) + 4 = -5 < 0
+    assert oracle.lib.or_hmm_viterbi(C.byref(h), nt, len(nt)) == 0
+    oracle.lib.or_hmm_free(C.byref(h))
+
+
+def _brute_viterbi(emit, trans, a2a, codon, nt):
+    """unvectorised restatement over explicit dictionaries, tiny inputs only"""
+    M = len(emit)
+    pm = {j: 0 for j in range(M + 1)}
+    pi, pd = dict(pm), dict(pm)
+    best = 0
+    for i in range(len(nt) // 3):
+        c = nt[3 * i:3 * i + 3]
+        r = 20 if max(c) > 3 else codon[c[0] << 4 | c[1] << 2 | c[2]]
+        e = a2a[r] if r < 20 else -1
+        cm, ci, cd = {0: 0}, {0: 0}, {0: 0}
+        for j in range(1, M + 1):
+            t = trans[j - 1]
+            cm[j] = max(0, max(pm[j - 1] + t[0], pi[j - 1] + t[3], pd[j - 1] + t[5]) + (emit[j - 1][e] if e >= 0 else 0))
+            ci[j] = max(0, pm[j] + t[1], pi[j] + t[4])
+            cd[j] = max(0, pm[j - 1] + t[2], pd[j - 1] + t[6])
+            best = max(best, cm[j], ci[j])
+        pm, pi, pd = cm, ci, cd
+    return best
+
+
+def test_hmm_brute_force(oracle):
+    import ctypes as C
+    from protlocus_amd import synth
+    rng = np.random.default_rng(12)
+    oracle.lib.or_hmm_aa20_to_alpha.restype = C.POINTER(C.c_int8)
+    p = oracle.lib.or_hmm_aa20_to_alpha()
+    a2a = [p[i] for i in range(22)]
+    codon = list(oracle.codon)
+    for M in (1, 3, 9):
+        em, tr = synth.hmm_params(rng, M)
+        h = nsffi.OrHmm()
+        assert oracle.lib.or_hmm_parse(synth.hmm_text(em, tr).encode(), C.byref(h), None) == 1
+        emit = [[h.emit[j * 20 + k] for k in range(20)] for j in range(M)]
+        trans = [[h.trans[j * 7 + k] for k in range(7)] for j in range(M)]
+        for _ in range(10):
+            nt = bytes(rng.integers(0, 5, size=int(rng.integers(0, 40))).astype(np.uint8))
+            assert oracle.lib.or_hmm_viterbi(C.byref(h), nt, len(nt)) == _brute_viterbi(emit, trans, a2a, codon, nt)
+        oracle.lib.or_hmm_free(C.byref(h))
+
+
+def test_pssm_ungapped_brute_force(oracle):
+    import ctypes as C
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        L, n = int(rng.integers(1, 30)), int(rng.integers(1, 40))
+        pssm = rng.integers(-8, 9, size=(L, 22)).astype(np.int8)
+        orf = rng.integers(0, 22, size=n).astype(np.uint8)
+        diag = int(rng.integers(-L + 1, n))
+        qs, qe = C.c_int32(), C.c_int32()
+        got = oracle.lib.or_pssm_ungapped(pssm.ctypes.data, L, orf.tobytes(), n, diag, C.byref(qs), C.byref(qe))
+        i0, i1 = max(0, -diag), min(L, n - diag)
+        best = 0
+        for s in range(i0, i1):
+            acc = 0
+            for e in range(s, i1):
+                acc += int(pssm[e, orf[e + diag]])
+                best = max(best, acc)
+        assert got == best
+        if got > 0:
+            assert sum(int(pssm[i, orf[i + diag]]) for i in range(qs.value, qe.value)) == got
+
+
+@pytest.mark.skipif(not nsffi.have_ref(), reason="compiled reference (oracle/_ref) not built")
+def test_oracle_matches_compiled_reference_random(oracle, ref):
+    from cases import planted_case, random_opt_kw, random_ss
+    rng = np.random.default_rng(99)
+    for k in range(300):
+        al = int(rng.integers(1, 400))
+        ns, a = planted_case(rng, al)
+        ss = random_ss(rng, len(ns))
+        kw = random_opt_kw(rng)
+        flag = [NS_F_CIGAR, NS_F_EXT_LEFT, NS_F_EXT_RIGHT, 0][k % 4]
+        fn = "ns_global_gs32b" if (k // 4) % 2 else "ns_global_gs16b"
+        o = ref.opt(flag=flag, **kw)
+        exp = ref.run(fn, ns, a, o, ss)
+        got = oracle.global_dp(ns, a, oracle.opt(flag=flag, **kw), ss, mode=32 if "32" in fn else 16,
+                               ext_width=ref.stripe_width(o, al))
+        assert got == exp, (k, fn, flag, al)
+
+
+def _gapless_one(oracle, g, q, cands, anchors, bw, kmer, xdrop, mat=None):
+    from cases import pack4
+    import ctypes as C
+    mat = np.array(oracle.blosum62 if mat is None else mat, dtype=np.int8)
+    AN = (nsffi.GaplessAnchor * max(1, len(anchors)))(*anchors)
+    return [oracle.lib.or_gapless_score(pack4(g), bytes(q), C.byref(c), AN, bw, kmer, xdrop, mat.ctypes.data, 22,
+                                        oracle.codon) for c in cands]
+
+
+def test_gapless_closed_forms(oracle):
+    """a query equal to the translation of the target: the seed sits on the true diagonal and both
+    walks run to the ends, so the score is the sum of the BLOSUM62 diagonal over the query; on the
+    reverse strand the same holds for the reverse complement; a second anchor on the same segment
+    is fully overlapped and adds nothing; two disjoint exons add up"""
+    from cases import oriented
+    rng = np.random.default_rng(1)
+    aa = "ARNDCQEGHILKMFPSTWYV"
+    b62 = oracle.blosum62
+    codon = oracle.codon
+    # a stop-free random coding sequence
+    while True:
+        g = rng.integers(0, 4, size=300).astype(np.uint8)
+        q = [codon[g[3 * k] << 4 | g[3 * k + 1] << 2 | g[3 * k + 2]] for k in range(100)]
+        if 20 not in q:
+            break
+    diag = sum(b62[r * 22 + r] for r in q)
+    kmer, bw = 4, 8
+    for rev in (0, 1):
+        gg = g if not rev else oriented(g, 0, len(g), 1)   # store the reverse complement, read reversed
+        cand = nsffi.GaplessCand(target_offset=0, target_length=300, query_offset=0, query_length=100,
+                                 anchor_offset=0, anchor_count=1, reverse=rev)
+        an = [nsffi.GaplessAnchor(target_block=3 * 50 + 2 - 3, query_end=50)]
+        assert _gapless_one(oracle, gg, q, [cand], an, bw, kmer, 10 ** 6) == [diag]
+        an2 = an + [nsffi.GaplessAnchor(target_block=3 * 80 + 2 - 5, query_end=80)]
+        cand.anchor_count = 2
+        assert _gapless_one(oracle, gg, q, [cand], an2, bw, kmer, 10 ** 6) == [diag]
+    # no usable seed: INT32_MIN; anchors whose query_end < kmer - 1 are skipped
+    cand = nsffi.GaplessCand(target_offset=0, target_length=300, query_offset=0, query_length=100,
+                             anchor_offset=0, anchor_count=1, reverse=0)
+    assert _gapless_one(oracle, g, q, [cand], [nsffi.GaplessAnchor(target_block=0, query_end=2)], bw, kmer, 50) == \
+        [-2 ** 31]
+    # two exons: query halves planted 600 nt apart, one anchor each; after the first segment only
+    # segments scoring >= xdrop count, so xdrop is kept below the half scores
+    g2 = np.concatenate([g[:150], rng.integers(0, 4, size=600).astype(np.uint8), g[150:]])
+    cand = nsffi.GaplessCand(target_offset=0, target_length=len(g2), query_offset=0, query_length=100,
+                             anchor_offset=0, anchor_count=2, reverse=0)
+    an = [nsffi.GaplessAnchor(target_block=3 * 20 + 2, query_end=20),
+          nsffi.GaplessAnchor(target_block=600 + 3 * 70 + 2, query_end=70)]
+    s = _gapless_one(oracle, g2, q, [cand], an, 1, kmer, 30)[0]
+    assert abs(s - diag) <= 60   # each walk ends a little past the exon edge, inside the random spacer
+    assert _gapless_one(oracle, g2, q, [cand], an, 1, kmer, 10 ** 6)[0] < diag * 3 // 4   # one segment only
+
+
+def test_gapless_brute_seed_choice(oracle):
+    """the seed of one anchor is the block position with the best k-mer score, smallest end on
+    ties (checked by enumerating the block); with x-drop 0 and a query of exactly k residues the
+    candidate score is that seed score plus the best non-negative walk gains, which are 0 here
+    because the query has no residue outside the k-mer"""
+    from cases import oriented
+    rng = np.random.default_rng(2)
+    codon = oracle.codon
+    b62 = oracle.blosum62
+    for trial in range(40):
+        kmer = int(rng.integers(1, 6))
+        g = rng.integers(0, 4, size=200).astype(np.uint8)
+        q = rng.integers(0, 21, size=kmer).astype(np.uint8)
+        tb, bw = int(rng.integers(0, 150)), int(rng.integers(1, 60))
+        rev = int(rng.integers(0, 2))
+        o = oriented(g, 0, 200, rev)
+        best = None
+        for d in range(bw):
+            te = tb + d
+            if te - (kmer - 1) * 3 - 2 < 0 or te >= 200:
+                continue
+            sc = 0
+            for s in range(kmer):
+                e = te - (kmer - 1 - s) * 3
+                b = o[e - 2:e + 1]
+                r = codon[int(b[0]) << 4 | int(b[1]) << 2 | int(b[2])] if b.max() < 4 else 20
+                sc += b62[r * 22 + int(q[s])]
+            if best is None or sc > best:
+                best = sc
+        cand = nsffi.GaplessCand(target_offset=0, target_length=200, query_offset=0, query_length=kmer,
+                                 anchor_offset=0, anchor_count=1, reverse=rev)
+        got = _gapless_one(oracle, g, q, [cand], [nsffi.GaplessAnchor(target_block=tb, query_end=kmer - 1)], bw,
+                           kmer, 0)[0]
+        assert got == (best if best is not None else -2 ** 31), trial

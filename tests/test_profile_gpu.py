@@ -1,0 +1,152 @@
+"""GPU parity of the profile and gapless paths against the oracle (bit-exact):
+  * profile-HMM Viterbi (nsh_hmm_*, behind mp_hmm_dp_score_gpu*): the golden profiles and
+    regions written from the compiled reference, and random profiles with M at every lane
+    boundary up to 1024 against regions with N bases, stop codons and lengths 0..3000
+    (test/test_gpu.c:221-290 compares the GPU with mp_hmm_dp_score the same way);
+  * gapless X-drop candidate scores (ns_gpu_gapless_score_batch) against the oracle's
+    restatement of nasw-cuda.cu:1151-1295, forward and reverse strands, k-mer 1..8, block
+    widths 1..300, x-drop 0..60, anchors out of range and more than 32 anchors;
+  * the ungapped PSSM diagonal (nsh_pssm_ungapped_batch) against prescreen.c:334-344.
+"""
+import ctypes as C
+import gzip
+import json
+import os
+
+import numpy as np
+import pytest
+
+import nsffi
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _parse(oracle, text):
+    h = nsffi.OrHmm()
+    assert oracle.lib.or_hmm_parse(text.encode(), C.byref(h), None) == 1
+    M = h.len
+    emit = np.ctypeslib.as_array(h.emit, shape=(M * 20,)).copy()
+    trans = np.ctypeslib.as_array(h.trans, shape=(M * 7,)).copy()
+    return h, M, emit, trans
+
+
+def _gpu_scores(product, M, emit, trans, regions):
+    L = product.lib
+    g = L.nsh_hmm_create(M, emit.ctypes.data, trans.ctypes.data)
+    assert g, product.error()
+    try:
+        nt = b"".join(regions)
+        off = np.cumsum([0] + [len(r) for r in regions[:-1]]).astype(np.int32)
+        ln = np.array([len(r) for r in regions], dtype=np.int32)
+        out = np.zeros(len(regions), dtype=np.int32)
+        assert L.nsh_hmm_score_batch(g, len(regions), nt, len(nt), off.ctypes.data, ln.ctypes.data, out.ctypes.data), \
+            product.error()
+        return out.tolist()
+    finally:
+        L.nsh_hmm_destroy(g)
+
+
+def test_hmm_golden(oracle, product):
+    with gzip.open(os.path.join(GOLD, "hmm_cases.json.gz"), "rt") as f:
+        cases = json.load(f)
+    for case in cases:
+        h, M, emit, trans = _parse(oracle, case["hmm"])
+        oracle.lib.or_hmm_free(C.byref(h))
+        regions = [bytes.fromhex(r) for r in case["regions"]]
+        assert _gpu_scores(product, M, emit, trans, regions) == case["scores"], M
+
+
+@pytest.mark.parametrize("M", [1, 2, 63, 64, 65, 127, 128, 129, 200, 333, 511, 512, 513, 800, 1000, 1024])
+def test_hmm_random(oracle, product, M):
+    from protlocus_amd import synth
+    rng = np.random.default_rng(M)
+    em, tr = synth.hmm_params(rng, M)
+    h, M2, emit, trans = _parse(oracle, synth.hmm_text(em, tr))
+    assert M2 == M
+    regions = []
+    for k in range(24):
+        if k % 3 == 0:   # a profile-sampled protein, reverse-translated, in frame
+            prot = synth.hmm_sample_protein(rng, em, tr)
+            cod = np.frombuffer(synth.reverse_translate(rng, prot.encode()), dtype=np.uint8)
+            lut = np.full(256, 4, dtype=np.uint8)
+            lut[list(b"ACGT")] = [0, 1, 2, 3]
+            nt = np.concatenate([rng.integers(0, 4, size=3 * int(rng.integers(0, 40))), lut[cod],
+                                 rng.integers(0, 4, size=int(rng.integers(0, 100)))]).astype(np.uint8)
+        else:
+            nt = rng.integers(0, 5 if k % 3 == 1 else 4, size=int(rng.integers(0, 3000))).astype(np.uint8)
+        regions.append(nt.tobytes())
+    exp = [oracle.lib.or_hmm_viterbi(C.byref(h), r, len(r)) for r in regions]
+    oracle.lib.or_hmm_free(C.byref(h))
+    assert _gpu_scores(product, M, emit, trans, regions) == exp
+
+
+def test_hmm_rejects_long_profiles(product):
+    emit = np.zeros(1025 * 20, dtype=np.int16)
+    trans = np.zeros(1025 * 7, dtype=np.int16)
+    assert not product.lib.nsh_hmm_create(1025, emit.ctypes.data, trans.ctypes.data)
+
+
+@pytest.mark.parametrize("seed,kmer,bw,xdrop", [(1, 5, 256, 30), (2, 3, 64, 0), (3, 7, 300, 60), (4, 1, 1, 10),
+                                                 (5, 8, 100, 25), (6, 4, 17, 5), (7, 2, 32, 1000)])
+def test_gapless_batch(oracle, product, seed, kmer, bw, xdrop):
+    from cases import gapless_batch, pack4
+    rng = np.random.default_rng(seed)
+    codon = oracle.codon
+    g, queries, cands, anchors = gapless_batch(rng, 300, codon, kmer, bw)
+    pk = pack4(g)
+    mat = np.array(oracle.blosum62, dtype=np.int8)
+    CA = (nsffi.GaplessCand * len(cands))(*cands)
+    AN = (nsffi.GaplessAnchor * max(1, len(anchors)))(*anchors)
+    out = np.zeros(len(cands), dtype=np.int32)
+    L = product.lib
+    # the genome is cached by (pointer, length) like the reference (nasw-cuda.cu:1349-1356); a new
+    # genome object may reuse a freed one's address, so the caller resets the cache (nasw.h:178)
+    L.ns_gpu_gapless_reset_target()
+    ok = L.ns_gpu_gapless_score_batch(pk, len(g), queries, len(queries), CA, len(cands), AN, max(1, len(anchors)),
+                                      bw, kmer, xdrop, mat.ctypes.data, 22, codon, out.ctypes.data)
+    assert ok, product.error()
+    exp = [oracle.lib.or_gapless_score(pk, queries, C.byref(c), AN, bw, kmer, xdrop, mat.ctypes.data, 22, codon)
+           for c in cands]
+    bad = [(i, int(out[i]), exp[i]) for i in range(len(cands)) if out[i] != exp[i]]
+    assert not bad, bad[:5]
+    assert any(e > 100 for e in exp) and any(e == -2 ** 31 for e in exp)   # planted hits and empty candidates
+    prof = nsffi.GaplessProfile()
+    L.ns_gpu_gapless_profile(C.byref(prof))
+    assert prof.batches >= 1 and prof.candidates >= len(cands)
+
+
+def test_gapless_rejects_out_of_range(oracle, product):
+    from cases import pack4
+    g = np.zeros(100, dtype=np.uint8)
+    q = bytes(10)
+    c = nsffi.GaplessCand(target_offset=50, target_length=60, query_offset=0, query_length=10, anchor_offset=0,
+                          anchor_count=1, reverse=0)
+    a = nsffi.GaplessAnchor(target_block=0, query_end=5, reserved=0)
+    out = np.zeros(1, dtype=np.int32)
+    mat = np.array(oracle.blosum62, dtype=np.int8)
+    assert not product.lib.ns_gpu_gapless_score_batch(pack4(g), 100, q, 10, C.byref(c), 1, C.byref(a), 1, 16, 3, 10,
+                                                      mat.ctypes.data, 22, oracle.codon, out.ctypes.data)
+
+
+def test_pssm_ungapped_batch(oracle, product):
+    rng = np.random.default_rng(17)
+    for L in (1, 7, 64, 300, 1500):
+        pssm = rng.integers(-8, 9, size=(L, 22)).astype(np.int8)
+        orfs = [rng.integers(0, 22, size=int(rng.integers(1, 700))).astype(np.uint8) for _ in range(50)]
+        aa = b"".join(o.tobytes() for o in orfs)
+        offs = np.cumsum([0] + [len(o) for o in orfs[:-1]]).astype(np.int64)
+        items = [(k, int(rng.integers(-L + 1, len(orfs[k])))) for k in range(50) for _ in range(8)]
+        io = np.array([offs[k] for k, _ in items], dtype=np.int64)
+        il = np.array([len(orfs[k]) for k, _ in items], dtype=np.int32)
+        idg = np.array([d for _, d in items], dtype=np.int32)
+        n = len(items)
+        sc, qs, qe = (np.zeros(n, dtype=np.int32) for _ in range(3))
+        assert product.lib.nsh_pssm_ungapped_batch(pssm.ctypes.data, L, aa, len(aa), n, io.ctypes.data,
+                                                   il.ctypes.data, idg.ctypes.data, sc.ctypes.data, qs.ctypes.data,
+                                                   qe.ctypes.data), product.error()
+        for i, (k, d) in enumerate(items):
+            a, b = C.c_int32(), C.c_int32()
+            e = oracle.lib.or_pssm_ungapped(pssm.ctypes.data, L, orfs[k].tobytes(), len(orfs[k]), d, C.byref(a),
+                                            C.byref(b))
+            assert (sc[i], qs[i], qe[i]) == (e, a.value, b.value), (L, i, k, d)
