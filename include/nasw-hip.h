@@ -196,13 +196,21 @@ typedef struct nsh_ext_plan_s nsh_ext_plan_t;
 typedef struct { int32_t nt_len, aa_len, score; } nsh_ext_result_t;
 
 /* All problems of a plan share opt (except the EXT_LEFT/EXT_RIGHT bit, taken per problem
- * from dirs[i] when dirs != NULL). ns/ss are concatenated raw nt strings (ASCII or 0..4),
+ * from dirs[i] when dirs != NULL). With dirs == NULL and no EXT bit in opt->flag the plan runs
+ * the plain score-only global DP instead (results equal ns_global_gs16b without CIGAR). ns/ss are concatenated raw nt strings (ASCII or 0..4),
  * as the concatenated proteins; ss may be NULL. Returns NULL and sets ns_gpu_error() on failure. */
 nsh_ext_plan_t *nsh_ext_plan_create(int32_t n, const char *ns, const int64_t *ns_off, const int32_t *nl,
 		const char *as, const int64_t *as_off, const int32_t *al, const uint8_t *ss,
 		const uint8_t *dirs, const ns_opt_t *opt);
 int nsh_ext_plan_run(nsh_ext_plan_t *p);                      /* enqueue on the plan's stream; 1 ok, 0 error */
 int nsh_ext_plan_sync(nsh_ext_plan_t *p);                     /* wait for the last run */
+/* run only the problems with keep[i] != 0 (input order); the others report nt_len = aa_len = 0,
+ * score INT32_MIN and no evaluated rows */
+int nsh_ext_plan_run_subset(nsh_ext_plan_t *p, const uint8_t *keep);
+/* the 64 prescreen features of every problem (input order), computed on the device from the
+ * plan's resident inputs (also copied to host_out, n x 64, when it is not NULL); returns a device
+ * pointer to n x 64 floats owned by the plan, or NULL (protein > 4096 aa or DNA > 24000 nt) */
+const float *nsh_ext_plan_features(nsh_ext_plan_t *p, float *host_out);
 int nsh_ext_plan_results(nsh_ext_plan_t *p, nsh_ext_result_t *out); /* copy results (input order) */
 double nsh_ext_plan_cells(const nsh_ext_plan_t *p, int32_t kind);   /* 0: sum nl*al, 1: sum (rows done)*al after a run */
 float nsh_ext_plan_last_ms(nsh_ext_plan_t *p, int32_t which);  /* 0: whole run, 1: DP kernels only (HIP events) */
@@ -221,6 +229,24 @@ void nsh_hmm_destroy(nsh_hmm_t *h);
  * (prescreen.c:334-344) -> score[k], query interval [qs[k], qe[k]). pssm: L x 22 int8 rows. */
 int nsh_pssm_ungapped_batch(const int8_t *pssm, int32_t L, const uint8_t *aa, int64_t l_aa, int32_t n,
 		const int64_t *orf_off, const int32_t *orf_len, const int32_t *diag, int32_t *score, int32_t *qs, int32_t *qe);
+
+/* Learned chain-candidate prescreen: MLP 64 -> 256 -> 256 -> 1 (ReLU, logistic output), BF16
+ * MFMA kernels with FP32 accumulation, FP32 Adam masters. Parameter vector (FP32, in order):
+ * W1[64][256], b1[256], W2[256][256], b2[256], w3[256], b3. No reference counterpart. */
+typedef struct nsh_mlp_s nsh_mlp_t;
+int32_t nsh_mlp_param_count(void);
+nsh_mlp_t *nsh_mlp_create(const float *params);
+/* one step on B samples (B % 32 == 0): forward, gradients (mean BCE), Adam when lr > 0 */
+int nsh_mlp_train_step(nsh_mlp_t *h, const float *X, const float *t, int32_t B, float lr, float *loss);
+int nsh_mlp_get_params(nsh_mlp_t *h, float *params, float *grads);
+/* logits of n candidates; X: n x 64 FP32 features on the host (x_on_device = 0) or device */
+int nsh_mlp_infer(nsh_mlp_t *h, const float *X, int64_t n, int32_t x_on_device, float *logit);
+float nsh_mlp_last_ms(nsh_mlp_t *h);   /* device time of the last inference (ms) */
+void nsh_mlp_destroy(nsh_mlp_t *h);
+
+/* the same features from raw host strings (proteins ASCII, DNA ASCII or 0..4); out: n x 64 */
+int nsh_features(int32_t n, const char *as, const int64_t *as_off, const int32_t *al, int64_t l_as,
+		const char *ns, const int64_t *ns_off, const int32_t *nl, int64_t l_ns, float *out);
 
 int nsh_device_synchronize(void);
 const char *nsh_build_info(void);

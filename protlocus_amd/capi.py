@@ -1,0 +1,218 @@
+"""ctypes view of libnasw_hip.so for Python drivers (the prescreen trainer, the benchmark).
+
+Types follow nasw.h:62-82; the engine entry points are declared in include/nasw-hip.h. Loading
+fails loudly when the library is missing; every call that fails raises with ns_gpu_error().
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import LIB_PATH
+
+NS_F_CIGAR, NS_F_EXT_LEFT, NS_F_EXT_RIGHT, NS_F_GPU = 1, 2, 4, 8
+N_FEATURES = 64
+
+
+class NsOpt(C.Structure):
+    _fields_ = [("flag", C.c_int32), ("go", C.c_int32), ("ge", C.c_int32), ("io", C.c_int32), ("fs", C.c_int32),
+                ("xdrop", C.c_int32), ("end_bonus", C.c_int32), ("asize", C.c_int32), ("gpu_device", C.c_int32),
+                ("gpu_min_cells", C.c_int64), ("sp", C.c_int32 * 6), ("sp_null_bonus", C.c_int32),
+                ("ie_coef", C.c_float), ("sc", C.c_void_p), ("pssm", C.c_void_p),
+                ("nt4", C.c_void_p), ("aa20", C.c_void_p), ("codon", C.c_void_p)]
+
+
+class NsRst(C.Structure):
+    _fields_ = [("n_cigar", C.c_int32), ("m_cigar", C.c_int32), ("nt_len", C.c_int32), ("aa_len", C.c_int32),
+                ("score", C.c_int32), ("cigar", C.POINTER(C.c_uint32))]
+
+
+class ExtResult(C.Structure):
+    _fields_ = [("nt_len", C.c_int32), ("aa_len", C.c_int32), ("score", C.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    """the product library, loaded once (raises when it is not built)"""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libnasw_hip.so is not built: run python -m protlocus_amd.build")
+        L = C.CDLL(LIB_PATH)
+        L.ns_make_tables.argtypes = [C.c_int]
+        L.ns_opt_init.argtypes = [C.c_void_p]
+        L.ns_gpu_init.argtypes = [C.c_int32]
+        L.ns_gpu_error.restype = C.c_char_p
+        L.ns_gpu_name.restype = C.c_char_p
+        L.nsh_ext_plan_create.argtypes = [C.c_int32, C.c_char_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_void_p,
+                                          C.c_void_p, C.c_char_p, C.c_char_p, C.c_void_p]
+        L.nsh_ext_plan_create.restype = C.c_void_p
+        for f in (L.nsh_ext_plan_run, L.nsh_ext_plan_sync, L.nsh_ext_plan_destroy):
+            f.argtypes = [C.c_void_p]
+        L.nsh_ext_plan_run_subset.argtypes = [C.c_void_p, C.c_char_p]
+        L.nsh_ext_plan_features.argtypes = [C.c_void_p, C.c_void_p]
+        L.nsh_ext_plan_features.restype = C.c_void_p
+        L.nsh_ext_plan_results.argtypes = [C.c_void_p, C.c_void_p]
+        L.nsh_ext_plan_cells.argtypes = [C.c_void_p, C.c_int32]
+        L.nsh_ext_plan_cells.restype = C.c_double
+        L.nsh_ext_plan_last_ms.argtypes = [C.c_void_p, C.c_int32]
+        L.nsh_ext_plan_last_ms.restype = C.c_float
+        L.nsh_mlp_param_count.restype = C.c_int32
+        L.nsh_mlp_create.argtypes = [C.c_void_p]
+        L.nsh_mlp_create.restype = C.c_void_p
+        L.nsh_mlp_train_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_void_p]
+        L.nsh_mlp_get_params.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.nsh_mlp_infer.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]
+        L.nsh_mlp_last_ms.argtypes = [C.c_void_p]
+        L.nsh_mlp_last_ms.restype = C.c_float
+        L.nsh_mlp_destroy.argtypes = [C.c_void_p]
+        L.nsh_features.argtypes = [C.c_int32, C.c_char_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_char_p, C.c_void_p,
+                                   C.c_void_p, C.c_int64, C.c_void_p]
+        L.nsh_device_synchronize.restype = C.c_int
+        L.nsh_build_info.restype = C.c_char_p
+        if L.ns_make_tables(1) != 0:
+            raise RuntimeError("ns_make_tables failed")
+        _lib = L
+    return _lib
+
+
+def error():
+    return lib().ns_gpu_error().decode()
+
+
+def init_gpu(device=0):
+    if not lib().ns_gpu_init(device):
+        raise RuntimeError("GPU not usable: " + error())
+
+
+def opt(**kw):
+    o = NsOpt()
+    lib().ns_opt_init(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def pack(seqs):
+    """concatenate byte strings: (buffer, int64 offsets, int32 lengths)"""
+    ln = np.array([len(x) for x in seqs], dtype=np.int32)
+    off = np.zeros(len(seqs), dtype=np.int64)
+    if len(seqs) > 1:
+        off[1:] = np.cumsum(ln[:-1], dtype=np.int64)
+    return b"".join(seqs), off, ln
+
+
+class Plan:
+    """a device-resident batch of DP problems (nsh_ext_plan_*)"""
+
+    def __init__(self, dnas, prots, o, dirs=None):
+        L = lib()
+        self.ns, self.ns_off, self.nl = pack(dnas)
+        self.as_, self.as_off, self.al = pack(prots)
+        self.n = len(dnas)
+        self._opt = o
+        d = None if dirs is None else bytes(np.ascontiguousarray(dirs, dtype=np.uint8))
+        self.p = L.nsh_ext_plan_create(self.n, self.ns, self.ns_off.ctypes.data, self.nl.ctypes.data, self.as_,
+                                       self.as_off.ctypes.data, self.al.ctypes.data, None, d, C.byref(o))
+        if not self.p:
+            raise RuntimeError("nsh_ext_plan_create: " + error())
+
+    def run(self, keep=None, sync=True):
+        L = lib()
+        ok = L.nsh_ext_plan_run(self.p) if keep is None else \
+            L.nsh_ext_plan_run_subset(self.p, bytes(np.ascontiguousarray(keep, dtype=np.uint8)))
+        if not ok:
+            raise RuntimeError("plan run: " + error())
+        if sync and not L.nsh_ext_plan_sync(self.p):
+            raise RuntimeError("plan sync: " + error())
+
+    def sync(self):
+        if not lib().nsh_ext_plan_sync(self.p):
+            raise RuntimeError("plan sync: " + error())
+
+    def scores(self):
+        out = (ExtResult * self.n)()
+        if not lib().nsh_ext_plan_results(self.p, out):
+            raise RuntimeError("plan results: " + error())
+        return np.array([r.score for r in out], dtype=np.int64)
+
+    def cells(self, kind=1):
+        return lib().nsh_ext_plan_cells(self.p, kind)
+
+    def ms(self, which=1):
+        return lib().nsh_ext_plan_last_ms(self.p, which)
+
+    def features_device(self):
+        ptr = lib().nsh_ext_plan_features(self.p, None)
+        if not ptr:
+            raise RuntimeError("plan features: " + error())
+        return ptr
+
+    def features_host(self):
+        X = np.zeros((self.n, N_FEATURES), dtype=np.float32)
+        if not lib().nsh_ext_plan_features(self.p, X.ctypes.data):
+            raise RuntimeError("plan features: " + error())
+        return X
+
+    def close(self):
+        if self.p:
+            lib().nsh_ext_plan_destroy(self.p)
+            self.p = None
+
+
+def features(prots, dnas):
+    """the 64 prescreen features of (protein, DNA window) pairs, host arrays in, n x 64 out"""
+    as_, as_off, al = pack(prots)
+    ns, ns_off, nl = pack(dnas)
+    out = np.zeros((len(prots), N_FEATURES), dtype=np.float32)
+    if not lib().nsh_features(len(prots), as_, as_off.ctypes.data, al.ctypes.data, len(as_), ns,
+                              ns_off.ctypes.data, nl.ctypes.data, len(ns), out.ctypes.data):
+        raise RuntimeError("nsh_features: " + error())
+    return out
+
+
+class MLP:
+    """the prescreen network (nsh_mlp_*): FP32 parameters in, BF16 MFMA kernels on the device"""
+
+    def __init__(self, params):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        assert params.size == lib().nsh_mlp_param_count()
+        self.h = lib().nsh_mlp_create(params.ctypes.data)
+        if not self.h:
+            raise RuntimeError("nsh_mlp_create: " + error())
+
+    def train_step(self, X, t, lr):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = C.c_float()
+        if not lib().nsh_mlp_train_step(self.h, X.ctypes.data, t.ctypes.data, len(t), lr, C.byref(loss)):
+            raise RuntimeError("nsh_mlp_train_step: " + error())
+        return loss.value
+
+    def params(self, grads=False):
+        n = lib().nsh_mlp_param_count()
+        p, g = np.zeros(n, dtype=np.float32), np.zeros(n, dtype=np.float32)
+        if not lib().nsh_mlp_get_params(self.h, p.ctypes.data, g.ctypes.data):
+            raise RuntimeError("nsh_mlp_get_params: " + error())
+        return (p, g) if grads else p
+
+    def infer(self, X=None, n=None, device_ptr=None):
+        if device_ptr is None:
+            X = np.ascontiguousarray(X, dtype=np.float32)
+            n = len(X)
+        out = np.zeros(n, dtype=np.float32)
+        ok = lib().nsh_mlp_infer(self.h, X.ctypes.data if device_ptr is None else device_ptr, n,
+                                 0 if device_ptr is None else 1, out.ctypes.data)
+        if not ok:
+            raise RuntimeError("nsh_mlp_infer: " + error())
+        return out
+
+    def last_ms(self):
+        return lib().nsh_mlp_last_ms(self.h)
+
+    def close(self):
+        if self.h:
+            lib().nsh_mlp_destroy(self.h)
+            self.h = None

@@ -202,12 +202,20 @@ struct nsh_ext_plan_s {
 	std::vector<int32_t> al;     // per input index
 	std::vector<int32_t> out_h;  // 4 per problem
 	int ran = 0;
+	int ncu = 256;
+	int32_t max_al = 0, max_nl = 0;
+	std::vector<ExtProb> prob_h;             // descriptors in queue order (host copy)
+	int32_t grp_first[kGroups] = {}, grp_n[kGroups] = {};
+	// subset runs and features
+	ExtProb *d_prob_sub = nullptr;
+	std::vector<ExtProb> sub_h;
+	float *d_feat = nullptr;
 };
 
 static void plan_free_dev(nsh_ext_plan_t *p)
 {
 	void *bufs[] = { p->d_ns, p->d_ss, p->d_as, p->d_aas, p->d_rec, p->d_mat, p->d_pssm, p->d_pen, p->d_counter,
-		p->d_out, p->d_prob, p->d_prep, p->d_pc };
+		p->d_out, p->d_prob, p->d_prep, p->d_pc, p->d_prob_sub, p->d_feat };
 	for (void *b : bufs) if (b) (void)hipFree(b);
 }
 
@@ -254,6 +262,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		if (opt->pssm) e.pssm_off = l_pssm, l_pssm += (int64_t)q.al * asize;
 		l_ns += q.nl + 1, l_as += q.al + 1;
 		if (q.nl > max_nl) max_nl = q.nl;
+		p->max_nl = std::max(p->max_nl, q.nl), p->max_al = std::max(p->max_al, q.al);
 		any_ss |= q.ss != nullptr;
 		++n_grp[group_of(C)];
 		p->al[k] = q.al;
@@ -285,6 +294,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		for (int32_t k = 0; k < n; ++k) sorted[k] = prob[order[k]];
 		prob.swap(sorted);
 	}
+	p->prob_h = prob;
 	p->out_h.assign((size_t)4 * n, 0);
 	HIPOK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
 	HIPOK(hipEventCreate(&p->ev0)); HIPOK(hipEventCreate(&p->ev1)); HIPOK(hipEventCreate(&p->ev2));
@@ -314,9 +324,11 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		(void)hipGetDevice(&dev);
 		(void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
 		if (ncu <= 0) ncu = 256;
+		p->ncu = ncu;
 		int64_t first = 0;
 		for (int g = 0; g < kGroups; ++g) {
 			ExtArgs &a = p->args[g];
+			p->grp_first[g] = (int32_t)first, p->grp_n[g] = n_grp[g];
 			a.prob = p->d_prob + first, a.n = n_grp[g];
 			first += n_grp[g];
 			a.rec = p->d_rec, a.aas = p->d_aas, a.pssm = p->d_pssm, a.mat = p->d_mat, a.asize = asize;
@@ -340,6 +352,8 @@ extern "C" nsh_ext_plan_t *nsh_ext_plan_create(int32_t n, const char *ns, const 
 {
 	if (n < 1 || !ns || !as || !nl || !al || !opt) { nsh_set_error("nsh_ext_plan_create: bad arguments"); return nullptr; }
 	std::vector<nsh_ext_req> req(n);
+	// no direction at all: plain score-only global DP (ns_global_gs16b without CIGAR or EXT flags)
+	const bool global = !dirs && !(opt->flag & (NS_F_EXT_LEFT | NS_F_EXT_RIGHT));
 	for (int32_t k = 0; k < n; ++k) {
 		nsh_ext_req &q = req[k];
 		const int32_t flag = dirs? (dirs[k]? NS_F_EXT_LEFT : NS_F_EXT_RIGHT) : (opt->flag & (NS_F_EXT_LEFT | NS_F_EXT_RIGHT));
@@ -348,25 +362,33 @@ extern "C" nsh_ext_plan_t *nsh_ext_plan_create(int32_t n, const char *ns, const 
 		q.ss = ss? ss + (ns_off? ns_off[k] : 0) : nullptr;
 		q.left = (flag & NS_F_EXT_LEFT) != 0;
 		q.io = opt->io;
-		const int32_t w = nsh_host_stripe_width(flag? flag : NS_F_EXT_RIGHT, q.al);
+		q.mode = global;
+		const int32_t w = nsh_host_stripe_width(global? 0 : flag, q.al);
 		q.npad = (q.al + w - 1) / w * w;
 	}
 	return nsh_ext_plan_build(n, req.data(), opt);
 }
 
-extern "C" int nsh_ext_plan_run(nsh_ext_plan_t *p)
+// results of problems a subset run skips: nt_len = aa_len = rows = 0, score = INT32_MIN
+__global__ void clear_out_kernel(int32_t *out, int32_t n)
 {
-	if (!p) return 0;
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) out[4 * i] = 0, out[4 * i + 1] = 0, out[4 * i + 2] = INT_MIN, out[4 * i + 3] = 0;
+}
+
+static int plan_launch(nsh_ext_plan_t *p, const ExtArgs *args, const int32_t *grid, bool clear_out)
+{
 	HIPOK(hipEventRecord(p->ev0, p->stream));
 	hipLaunchKernelGGL(ext_prep_kernel, dim3(p->n), dim3(256), 0, p->stream, p->d_prep, p->d_ns, p->d_ss, p->d_as,
 			p->d_rec, p->d_aas, p->d_pc);
 	hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, p->stream, p->d_counter, kGroups, 0);
+	if (clear_out) hipLaunchKernelGGL(clear_out_kernel, dim3((p->n + 255) / 256), dim3(256), 0, p->stream, p->d_out, p->n);
 	HIPOK(hipGetLastError());
 	HIPOK(hipEventRecord(p->ev1, p->stream));
 	for (int g = 0; g < kGroups; ++g) {
-		if (p->grid[g] <= 0) continue;
+		if (grid[g] <= 0) continue;
 		HIPOK(hipStreamWaitEvent(p->gs[g], p->ev1, 0));
-		launch_group(g, p->args[g], p->grid[g], p->gs[g]);
+		launch_group(g, args[g], grid[g], p->gs[g]);
 		HIPOK(hipGetLastError());
 		HIPOK(hipEventRecord(p->gev[g], p->gs[g]));
 		HIPOK(hipStreamWaitEvent(p->stream, p->gev[g], 0));
@@ -378,6 +400,52 @@ extern "C" int nsh_ext_plan_run(nsh_ext_plan_t *p)
 	return 1;
 fail:
 	return 0;
+}
+
+extern "C" int nsh_ext_plan_run(nsh_ext_plan_t *p)
+{
+	return p? plan_launch(p, p->args, p->grid, false) : 0;
+}
+
+extern "C" int nsh_ext_plan_run_subset(nsh_ext_plan_t *p, const uint8_t *keep)
+{
+	if (!p || !keep) { nsh_set_error("nsh_ext_plan_run_subset: bad arguments"); return 0; }
+	ExtArgs args[kGroups];
+	int32_t grid[kGroups] = {};
+	if (!p->d_prob_sub) HIPOK(dalloc(&p->d_prob_sub, p->n));
+	HIPOK(hipStreamSynchronize(p->stream));   // the previous subset list may still be in flight
+	p->sub_h.clear();
+	for (int g = 0; g < kGroups; ++g) {
+		const int32_t first = (int32_t)p->sub_h.size();
+		for (int32_t k = p->grp_first[g]; k < p->grp_first[g] + p->grp_n[g]; ++k)
+			if (keep[p->prob_h[k].out]) p->sub_h.push_back(p->prob_h[k]);
+		args[g] = p->args[g];
+		args[g].prob = p->d_prob_sub + first, args[g].n = (int32_t)p->sub_h.size() - first;
+		grid[g] = std::min(args[g].n, p->ncu * kGroupWavesPerCU[g]);
+	}
+	if (!p->sub_h.empty())
+		HIPOK(hipMemcpyAsync(p->d_prob_sub, p->sub_h.data(), p->sub_h.size() * sizeof(ExtProb), hipMemcpyHostToDevice, p->stream));
+	return plan_launch(p, args, grid, true);
+fail:
+	return 0;
+}
+
+extern "C" const float *nsh_ext_plan_features(nsh_ext_plan_t *p, float *host_out)
+{
+	if (!p) return nullptr;
+	if (p->max_al > 4096 || p->max_nl > 24000) {
+		nsh_set_error("nsh_ext_plan_features: candidates limited to al <= 4096, nl <= 24000 (plan has %d, %d)", p->max_al, p->max_nl);
+		return nullptr;
+	}
+	if (!p->d_feat) HIPOK(dalloc(&p->d_feat, (size_t)p->n * 64));
+	nsh_feat::launch(p->d_prep, p->n, p->d_ns, p->d_as, p->d_pc, p->d_feat, p->stream);
+	HIPOK(hipGetLastError());
+	if (host_out) HIPOK(hipMemcpyAsync(host_out, p->d_feat, (size_t)p->n * 64 * 4, hipMemcpyDeviceToHost, p->stream));
+	HIPOK(hipStreamSynchronize(p->stream));
+	nsh_count_launch(1);
+	return p->d_feat;
+fail:
+	return nullptr;
 }
 
 extern "C" int nsh_ext_plan_sync(nsh_ext_plan_t *p)

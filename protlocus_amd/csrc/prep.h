@@ -19,4 +19,10 @@ struct PrepConst {
 void launch(const PrepProb *pp, int32_t n, const uint8_t *raw_ns, const uint8_t *raw_ss, const uint8_t *raw_as,
 		uint32_t *rec, uint8_t *aas, const PrepConst *pc, hipStream_t s);
 } // namespace nsh_prep
+
+namespace nsh_feat {
+// the 64 prescreen features of n candidates (features.hip); out: n x 64 floats, device
+void launch(const nsh_prep::PrepProb *pp, int32_t n, const uint8_t *ns, const uint8_t *as,
+		const nsh_prep::PrepConst *pc, float *out, hipStream_t s);
+} // namespace nsh_feat
 #endif

@@ -89,6 +89,11 @@ def reverse_translate(rng, prot):
 def gene_model(rng, prot_len, sub_rate, n_introns, intron_len_fn, n_fs, flank):
     """protein + DNA carrying its (mutated) coding sequence; returns (dna, protein)"""
     prot = rand_protein(rng, prot_len)
+    return gene_of(rng, prot, sub_rate, n_introns, intron_len_fn, n_fs, flank), prot
+
+
+def gene_of(rng, prot, sub_rate, n_introns, intron_len_fn, n_fs, flank):
+    """DNA carrying a mutated, spliced, frameshifted coding sequence of protein `prot`"""
     cds = reverse_translate(rng, substitute(rng, prot, sub_rate))
     seq = bytearray(cds)
     cuts = sorted(rng.integers(1, max(2, len(cds) - 1), size=n_introns).tolist(), reverse=True)
@@ -101,7 +106,7 @@ def gene_model(rng, prot_len, sub_rate, n_introns, intron_len_fn, n_fs, flank):
             del seq[p]
         else:
             seq[p:p] = rand_dna(rng, 1)
-    return rand_dna(rng, flank) + bytes(seq) + rand_dna(rng, flank), prot
+    return rand_dna(rng, flank) + bytes(seq) + rand_dna(rng, flank)
 
 
 def single_pair(seed=0):
