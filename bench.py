@@ -1,17 +1,21 @@
 #!/usr/bin/env python3
-"""Spliced-DP benchmark: batched extension DP, 65,536 protein/DNA pairs on one MI355X.
+"""Spliced-DP benchmark on one MI355X.
 
-Workload: protein lengths log-uniform 50-1000 aa drawn from BLOSUM62 background frequencies,
-reverse-translated, 10-30% substitutions, 0-3 GT..AG introns (log-normal, median 300 nt,
-max 5 kb), 0-2 frameshifts, 200-nt random flanks; one right-extension request per pair over
-the whole target (int16 saturating scores, BLOSUM62, library default penalties).
+Headline (value): batched extension DP, 65,536 protein/DNA pairs. Protein lengths log-uniform
+50-1000 aa drawn from BLOSUM62 background frequencies, reverse-translated, 10-30%
+substitutions, 0-3 GT..AG introns (log-normal, median 300 nt, max 5 kb), 0-2 frameshifts,
+200-nt random flanks; one right-extension request per pair over the whole target (int16
+saturating scores, BLOSUM62, library default penalties). A step is one pass of the hot path
+over the whole batch with the raw inputs already resident in HBM: device-side preparation
+(codon translation, splice signals) plus the DP of every pair. GCUPS counts evaluated cells,
+(rows evaluated) x al per pair, rows starting at 2 and ending at the x-drop row; the
+reference's nl x al window cells are reported beside it.
 
-A step is one pass of the hot path over the whole batch with the raw inputs already resident
-in HBM: device-side preparation (codon translation, splice signals) plus the DP of every pair.
-GCUPS counts evaluated cells, (rows evaluated) x al per pair, rows starting at 2 and ending at
-the x-drop row; the reference's nl x al window cells are reported beside it.
-
-Output: one JSON line (rank 0).
+Beside it, in the same JSON line:
+  * prescreen: the learned chain-candidate prescreen trained on the GPU, then the plain and the
+    prescreened DP pipelines timed on fresh candidates (recall against the exact DP);
+  * hmm: profile-HMM Viterbi scan, 64 profiles (M 100-800) x a 10-Mb genome in 6 frames.
+Output: one JSON line.
 """
 import argparse
 import concurrent.futures as cf
@@ -19,17 +23,29 @@ import ctypes as C
 import json
 import os
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, ROOT)
 
-VALU_PEAK_TOPS = 256 * 64 * 2 * 2.4e9 / 1e12   # packed-int16 VALU ops/s of MI355X (256 CUs, 2.4 GHz)
-OPS_PER_CELL = 32                               # algorithmic int ops per extension cell (31 + row max)
+VALU_I32_TOPS = 256 * 64 * 2.4e9 / 1e12          # int32 VALU ops/s of MI355X (256 CUs, 2.4 GHz)
+VALU_PEAK_TOPS = 2 * VALU_I32_TOPS                # packed int16 (two 16-bit ops per lane)
+MFMA_BF16_TFLOPS = 2500.0                         # dense BF16 MFMA
+OPS_PER_CELL = 32                                 # extension DP: 31 recurrence ops + row max
+HMM_OPS_PER_CELL = 17                             # Viterbi: 8 adds, 4 max, 3 clamps, 2 running-best max
+MLP_FLOP_PER_SAMPLE = 2 * (64 * 256 + 256 * 256 + 256)
 
 
-def cpu_baseline(pairs, kw, seconds=15.0, max_threads=16):
+def threads():
+    try:
+        return min(len(os.sched_getaffinity(0)), 16)
+    except AttributeError:
+        return min(os.cpu_count() or 1, 16)
+
+
+def cpu_baseline(pairs, kw, seconds=15.0):
     """the reference CPU path (nasw-sse.c via oracle/_ref when built, else the oracle port) on a
     bounded sample of the same workload, threaded like kt_for"""
     import nsffi
@@ -39,22 +55,86 @@ def cpu_baseline(pairs, kw, seconds=15.0, max_threads=16):
     else:
         lib, kind = nsffi.Oracle(), "port"
         fn = lambda d, p: lib.ext(d, p, lib.opt(**kw))[0]
-    try:
-        threads = min(len(os.sched_getaffinity(0)), max_threads)
-    except AttributeError:
-        threads = min(os.cpu_count() or 1, max_threads)
-    # chunks of pairs until about `seconds` of wall time have been spent
+    nt = threads()
     n, t0 = 0, time.perf_counter()
-    with cf.ThreadPoolExecutor(threads) as ex:
+    with cf.ThreadPoolExecutor(nt) as ex:
         while n < len(pairs) and time.perf_counter() - t0 < seconds:
-            chunk = pairs[n:n + 4 * threads]
+            chunk = pairs[n:n + 4 * nt]
             list(ex.map(lambda x: fn(*x), chunk))
             n += len(chunk)
     dt = time.perf_counter() - t0
-    sample = pairs[:n]
-    win = sum(len(d) * len(p) for d, p in sample)
-    return {"value": win / dt / 1e9, "unit": "GCUPS (nl*al window cells)", "cores": threads, "kind": kind,
-            "sample": "%d of the pairs, %.1f s wall" % (n, dt)}, n
+    win = sum(len(d) * len(p) for d, p in pairs[:n])
+    return {"value": win / dt / 1e9, "unit": "GCUPS (nl*al window cells)", "cores": nt, "kind": kind,
+            "sample": "%d of the pairs, %.1f s wall" % (n, dt)}
+
+
+def bench_hmm(seconds_cpu=4.0, log=None):
+    """64 profiles x a 10-Mb genome in 6 frames: Viterbi cells/s on the GPU (kernel time) and the
+    reference's mp_hmm_dp_score (or the oracle) on a bounded sample of the same regions"""
+    import numpy as np
+    import nsffi
+    from protlocus_amd import capi, synth
+    t0 = time.time()
+    profs, nt, off, ln = synth.profile_scan()
+    rows = (ln // 3).astype(np.int64)
+    if log:
+        log("[bench] hmm workload: %d profiles, %d regions, %.1f Mb in %.1f s" % (len(profs), len(ln), len(nt) / 1e6,
+                                                                              time.time() - t0))
+    cells = 0.0
+    kern_ms = 0.0
+    t0 = time.perf_counter()
+    best = []
+    for em, tr in profs:
+        e, t = synth.hmm_tables(em, tr)
+        h = capi.HMM(e, t)
+        try:
+            s = h.score(nt, off, ln)
+            kern_ms += h.last_ms()
+        finally:
+            h.close()
+        cells += float(rows.sum()) * len(e)
+        best.append(int(s.max()))
+    wall = time.perf_counter() - t0
+    gcups = cells / (kern_ms * 1e-3) / 1e9
+    out = {"workload": "64 profile HMMs (M 100-800, log-uniform) x 10 Mb genome (GC 0.45, 200 planted hits), "
+                       "3 kb windows in 6 frames", "profiles": len(profs), "regions": int(len(ln)),
+           "cells": cells, "kernel_ms": round(kern_ms, 2), "gcups": round(gcups, 1),
+           "wall_s_with_uploads": round(wall, 2),
+           "roofline": {"bound": "valu", "achieved": round(gcups * HMM_OPS_PER_CELL / 1e3, 3),
+                        "peak": round(VALU_I32_TOPS, 2), "unit": "TOP/s",
+                        "frac": round(gcups * HMM_OPS_PER_CELL / 1e3 / VALU_I32_TOPS, 4),
+                        "ops_per_cell": HMM_OPS_PER_CELL}}
+    # CPU: one profile (the first) over a sample of regions, threaded
+    e, t = synth.hmm_tables(*profs[0])
+    text = synth.hmm_text(*profs[0])
+    if nsffi.have_ref():
+        ref = nsffi.Reference()
+        with tempfile.NamedTemporaryFile("w", suffix=".hmm", delete=False) as f:
+            f.write(text)
+        fp = ref.lib.mp_hmm_open(f.name.encode())
+        hh = ref.lib.mp_hmm_read1(fp)
+        ref.lib.mp_hmm_close(fp)
+        os.unlink(f.name)
+        fn = lambda k: ref.lib.mp_hmm_dp_score(hh, nt[off[k]:off[k] + ln[k]], int(ln[k]))
+        kind = "reference"
+    else:
+        ora = nsffi.Oracle()
+        hh = nsffi.OrHmm()
+        ora.lib.or_hmm_parse(text.encode(), C.byref(hh), None)
+        fn = lambda k: ora.lib.or_hmm_viterbi(C.byref(hh), nt[off[k]:off[k] + ln[k]], int(ln[k]))
+        kind = "port"
+    ntr = threads()
+    n, t0 = 0, time.perf_counter()
+    with cf.ThreadPoolExecutor(ntr) as ex:
+        while n < len(ln) and time.perf_counter() - t0 < seconds_cpu:
+            ks = list(range(n, min(len(ln), n + 8 * ntr)))
+            list(ex.map(fn, ks))
+            n += len(ks)
+    dt = time.perf_counter() - t0
+    cpu_cells = float(rows[:n].sum()) * len(e)
+    out["cpu_baseline"] = {"value": round(cpu_cells / dt / 1e9, 3), "unit": "GCUPS", "cores": ntr, "kind": kind,
+                           "sample": "profile 0 (M=%d) x %d regions, %.1f s wall" % (len(e), n, dt)}
+    return out
 
 
 def main():
@@ -65,18 +145,20 @@ def main():
     ap.add_argument("--pairs", type=int, default=65536)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="headline only (no prescreen / hmm sections)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X; the DP problems are "
               "independent, so N GPUs would shard pairs by cost (see DESIGN.md) - run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    log = lambda s: print(s, file=sys.stderr, flush=True)
 
     import nsffi
     from protlocus_amd import synth
     t0 = time.time()
     pairs = synth.extension_batch(args.pairs, seed=args.seed)
-    print("[bench] generated %d pairs in %.1f s" % (len(pairs), time.time() - t0), file=sys.stderr, flush=True)
+    log("[bench] generated %d pairs in %.1f s" % (len(pairs), time.time() - t0))
     prod = nsffi.Product()
     if not prod.lib.ns_gpu_init(0):
         raise SystemExit("GPU not usable: " + prod.error())
@@ -85,7 +167,7 @@ def main():
     plan = prod.ext_plan(ns, ns_off, nl, as_, as_off, al, prod.opt(**kw))
     for _ in range(args.warmup):
         plan.run()
-    print("[bench] warmup done", file=sys.stderr, flush=True)
+    log("[bench] warmup done")
     prod.lib.nsh_device_synchronize()
     dp_ms = []
     t0 = time.perf_counter()
@@ -95,7 +177,7 @@ def main():
         dp_ms.append(plan.ms(1))
     prod.lib.nsh_device_synchronize()
     wall = time.perf_counter() - t0
-    res = plan.results()
+    plan.results()
     cells = plan.cells(1)          # evaluated cells of the last run
     win = plan.cells(0)
     plan.close()
@@ -121,11 +203,29 @@ def main():
                      "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
                      "ops_per_cell": OPS_PER_CELL},
     }
-    print("[bench] gpu: %.2f GCUPS, %.1f ms/step, dp %.1f ms" % (gcups, ms, dp_avg), file=sys.stderr, flush=True)
+    log("[bench] gpu: %.2f GCUPS, %.1f ms/step, dp %.1f ms" % (gcups, ms, dp_avg))
     if not args.no_cpu:
-        cb, n = cpu_baseline(pairs, kw, seconds=args.cpu_seconds)
+        cb = cpu_baseline(pairs, kw, seconds=args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["gpu_window_vs_cpu"] = round(out["window_gcups"] / cb["value"], 1) if cb["value"] else None
+        log("[bench] cpu baseline: %.2f GCUPS (%s, %d threads)" % (cb["value"], cb["kind"], cb["cores"]))
+    if not args.no_extras:
+        from protlocus_amd import prescreen
+        del pairs
+        t0 = time.time()
+        ps = prescreen.evaluate(n_train=16384, n_val=4096, n_eval=16384, epochs=12, seed=11, log=log)
+        ps["workload"] = ("16,384 proteins x (true locus + decoy window), protein 50-1000 aa; trained on 16,384 "
+                          "other proteins' candidates; DP = plain global spliced DP over the window")
+        n = ps["candidates"]
+        if ps["mlp_infer_ms"] > 0:
+            tf = n * MLP_FLOP_PER_SAMPLE / (ps["mlp_infer_ms"] * 1e-3) / 1e12
+            ps["mfma"] = {"achieved": round(tf, 2), "peak": MFMA_BF16_TFLOPS, "unit": "TFLOP/s",
+                          "frac": round(tf / MFMA_BF16_TFLOPS, 5), "flop_per_candidate": MLP_FLOP_PER_SAMPLE}
+        out["prescreen"] = ps
+        log("[bench] prescreen: recall %.4f, kept %.3f, speedup %.2fx (%.1f s)" % (ps["recall"], ps["kept_frac"],
+                                                                                  ps["speedup"], time.time() - t0))
+        out["hmm"] = bench_hmm(log=log)
+        log("[bench] hmm: %.1f GCUPS" % out["hmm"]["gcups"])
     print(json.dumps(out), flush=True)
 
 

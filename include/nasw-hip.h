@@ -223,6 +223,7 @@ typedef struct nsh_hmm_s nsh_hmm_t;
 nsh_hmm_t *nsh_hmm_create(int32_t M, const int16_t *emit, const int16_t *trans);
 int nsh_hmm_score_batch(nsh_hmm_t *h, int32_t n, const uint8_t *nt, int64_t l_nt, const int32_t *off,
 		const int32_t *len, int32_t *score);
+float nsh_hmm_last_ms(nsh_hmm_t *h);   /* device time of the last batch's Viterbi kernel (ms) */
 void nsh_hmm_destroy(nsh_hmm_t *h);
 
 /* Best ungapped PSSM segment of item k: ps_ungapped(pssm, L, aa + orf_off[k], orf_len[k], diag[k])

@@ -70,6 +70,13 @@ def lib():
         L.nsh_mlp_destroy.argtypes = [C.c_void_p]
         L.nsh_features.argtypes = [C.c_int32, C.c_char_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_char_p, C.c_void_p,
                                    C.c_void_p, C.c_int64, C.c_void_p]
+        L.nsh_hmm_create.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
+        L.nsh_hmm_create.restype = C.c_void_p
+        L.nsh_hmm_score_batch.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                          C.c_void_p]
+        L.nsh_hmm_last_ms.argtypes = [C.c_void_p]
+        L.nsh_hmm_last_ms.restype = C.c_float
+        L.nsh_hmm_destroy.argtypes = [C.c_void_p]
         L.nsh_device_synchronize.restype = C.c_int
         L.nsh_build_info.restype = C.c_char_p
         if L.ns_make_tables(1) != 0:
@@ -215,4 +222,34 @@ class MLP:
     def close(self):
         if self.h:
             lib().nsh_mlp_destroy(self.h)
+            self.h = None
+
+
+class HMM:
+    """a profile resident on the device (nsh_hmm_*): int16 emissions M x 20, transitions M x 7"""
+
+    def __init__(self, emit, trans):
+        self.emit = np.ascontiguousarray(emit, dtype=np.int16)
+        self.trans = np.ascontiguousarray(trans, dtype=np.int16)
+        self.M = len(self.emit)
+        self.h = lib().nsh_hmm_create(self.M, self.emit.ctypes.data, self.trans.ctypes.data)
+        if not self.h:
+            raise RuntimeError("nsh_hmm_create: " + error())
+
+    def score(self, nt, off, ln):
+        """Viterbi scores of regions nt[off[i]:off[i]+ln[i]] (base codes)"""
+        off32 = np.ascontiguousarray(off, dtype=np.int32)
+        ln = np.ascontiguousarray(ln, dtype=np.int32)
+        out = np.zeros(len(ln), dtype=np.int32)
+        if not lib().nsh_hmm_score_batch(self.h, len(ln), nt, len(nt), off32.ctypes.data, ln.ctypes.data,
+                                         out.ctypes.data):
+            raise RuntimeError("nsh_hmm_score_batch: " + error())
+        return out
+
+    def last_ms(self):
+        return lib().nsh_hmm_last_ms(self.h)
+
+    def close(self):
+        if self.h:
+            lib().nsh_hmm_destroy(self.h)
             self.h = None

@@ -144,6 +144,8 @@ struct nsh_hmm_s {
 	uint8_t *d_nt = nullptr; size_t cap_nt = 0;
 	int64_t *d_off = nullptr; int32_t *d_len = nullptr, *d_order = nullptr, *d_score = nullptr; size_t cap_n = 0;
 	int ncu = 0;
+	hipEvent_t e0 = nullptr, e1 = nullptr;   // around the last Viterbi launch
+	float last_ms = 0.f;
 };
 
 #define HMMOK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { nsh_set_error("hmm: %s: %s", #x, hipGetErrorString(e_)); goto fail; } } while (0)
@@ -179,6 +181,8 @@ extern "C" nsh_hmm_t *nsh_hmm_create(int32_t M, const int16_t *emit, const int16
 	HMMOK(hipGetDevice(&dev));
 	HMMOK(hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, dev));
 	HMMOK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+	HMMOK(hipEventCreate(&h->e0));
+	HMMOK(hipEventCreate(&h->e1));
 	HMMOK(hipMalloc((void**)&h->d_emitT, et.size() * 2));
 	HMMOK(hipMalloc((void**)&h->d_trans, tr.size() * 2));
 	HMMOK(hipMalloc((void**)&h->d_calpha, 64));
@@ -192,11 +196,15 @@ fail:
 	return nullptr;
 }
 
+extern "C" float nsh_hmm_last_ms(nsh_hmm_t *h) { return h? h->last_ms : 0.f; }
+
 extern "C" void nsh_hmm_destroy(nsh_hmm_t *h)
 {
 	if (!h) return;
 	void *bufs[] = { h->d_emitT, h->d_trans, h->d_calpha, h->d_counter, h->d_nt, h->d_off, h->d_len, h->d_order, h->d_score };
 	for (void *b : bufs) if (b) (void)hipFree(b);
+	if (h->e0) (void)hipEventDestroy(h->e0);
+	if (h->e1) (void)hipEventDestroy(h->e1);
 	if (h->stream) (void)hipStreamDestroy(h->stream);
 	delete h;
 }
@@ -242,10 +250,13 @@ extern "C" int nsh_hmm_score_batch(nsh_hmm_t *h, int32_t n, const uint8_t *nt, i
 		a.counter = h->d_counter, a.score = h->d_score;
 		const int grid = std::max(1, std::min((n + kWaves - 1) / kWaves, h->ncu * 4));
 		const size_t lds = (size_t)21 * 64 * h->C * 2;
+		HMMOK(hipEventRecord(h->e0, h->stream));
 		hipLaunchKernelGGL(KTab::fn[h->C - 1], dim3(grid), dim3(64 * kWaves), lds, h->stream, a);
 		HMMOK(hipGetLastError());
+		HMMOK(hipEventRecord(h->e1, h->stream));
 		HMMOK(hipMemcpyAsync(score, h->d_score, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream));
 		HMMOK(hipStreamSynchronize(h->stream));
+		HMMOK(hipEventElapsedTime(&h->last_ms, h->e0, h->e1));
 		nsh_count_launch(2);
 		ok = 1;
 	}

@@ -204,3 +204,54 @@ def hmm_sample_protein(rng, em, tr, max_len=None):
             state = 2 if rng.random() < tr[k - 1 if k else 0, 6] else 0
             k += 1
     return "".join(seq[:max_len] if max_len else seq)
+
+
+def hmm_tables(em, tr):
+    """the int16 emission (M x 20, best residue at +4) and transition (M x 7) tables that the
+    HMMER3 text of hmm_text() reads back as (-ln p printed with 5 decimals, score lrint(-2x))"""
+    def score(p):
+        p = np.asarray(p, dtype=np.float64)
+        x = np.where(p > 0, np.round(-np.log(np.where(p > 0, p, 1.0)), 5), 0.0)
+        v = np.clip(np.rint(-2.0 * x), -32767, 32767)
+        return np.where(p > 0, v, -32768).astype(np.int64)
+    e = score(em)
+    e = e - e.max(axis=1, keepdims=True) + 4
+    return e.astype(np.int16), score(tr).astype(np.int16)
+
+
+def profile_scan(n_prof=64, genome_len=10_000_000, win=3000, step=2700, n_plant=200, seed=5,
+                 min_m=100, max_m=800):
+    """profile-HMM scan workload: n_prof profiles (M log-uniform), a random genome (GC 0.45) with
+    n_plant profile-sampled hits (reverse-translated, either strand), cut into windows; each window
+    is scored in its six frames (forward offsets 0-2, reverse complement offsets 0-2), one region
+    per frame. Returns (profiles [(em, tr)], regions (bytes of base codes 0..3), offsets, lengths)."""
+    rng = np.random.default_rng(seed)
+    profs = []
+    for _ in range(n_prof):
+        M = int(round(np.exp(rng.uniform(np.log(min_m), np.log(max_m)))))
+        profs.append(hmm_params(rng, M))
+    p = np.array([0.275, 0.225, 0.225, 0.275])
+    g = rng.choice(4, size=genome_len, p=p).astype(np.uint8)
+    lut = np.full(256, 4, dtype=np.uint8)
+    lut[list(b"ACGT")] = [0, 1, 2, 3]
+    for _ in range(n_plant):
+        em, tr = profs[int(rng.integers(0, n_prof))]
+        cds = lut[np.frombuffer(reverse_translate(rng, hmm_sample_protein(rng, em, tr).encode()), dtype=np.uint8)]
+        if rng.random() < 0.5:
+            cds = (3 - cds)[::-1]
+        pos = int(rng.integers(0, genome_len - len(cds)))
+        g[pos:pos + len(cds)] = cds
+    starts = np.arange(0, max(1, genome_len - win + 1), step)
+    rc = (3 - g)[::-1]
+    regs = []
+    for s in starts:
+        e = min(genome_len, s + win)
+        fw = g[s:e]
+        rv = rc[genome_len - e:genome_len - s]
+        for f in range(3):
+            regs.append(fw[f:])
+            regs.append(rv[f:])
+    lens = np.array([len(r) for r in regs], dtype=np.int32)
+    offs = np.zeros(len(regs), dtype=np.int64)
+    offs[1:] = np.cumsum(lens[:-1], dtype=np.int64)
+    return profs, np.concatenate(regs).tobytes(), offs, lens

@@ -334,3 +334,18 @@ def test_gapless_brute_seed_choice(oracle):
         got = _gapless_one(oracle, g, q, [cand], [nsffi.GaplessAnchor(target_block=tb, query_end=kmer - 1)], bw,
                            kmer, 0)[0]
         assert got == (best if best is not None else -2 ** 31), trial
+
+
+def test_hmm_tables_equal_parser(oracle):
+    """the benchmark's int16 profile tables equal what the HMMER3 reader makes of the same text"""
+    import ctypes as C
+    from protlocus_amd import synth
+    rng = np.random.default_rng(31)
+    for M in (1, 17, 120):
+        em, tr = synth.hmm_params(rng, M)
+        e, t = synth.hmm_tables(em, tr)
+        h = nsffi.OrHmm()
+        assert oracle.lib.or_hmm_parse(synth.hmm_text(em, tr).encode(), C.byref(h), None) == 1
+        assert np.array_equal(np.ctypeslib.as_array(h.emit, shape=(M * 20,)), e.ravel())
+        assert np.array_equal(np.ctypeslib.as_array(h.trans, shape=(M * 7,)), t.ravel())
+        oracle.lib.or_hmm_free(C.byref(h))
