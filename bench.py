@@ -147,7 +147,28 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline only (no prescreen / hmm sections)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device three times (a copy kernel of known traffic for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        # HIP runtime device-to-device copies (the runtime's copy kernel): n bytes read, n written
+        hip = C.CDLL("libamdhip64.so")
+        hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        hip.hipFree.argtypes = [C.c_void_p]
+        n = args.copy_calibration << 20
+        src, dst = C.c_void_p(), C.c_void_p()
+        if hip.hipMalloc(C.byref(src), n) or hip.hipMalloc(C.byref(dst), n) or hip.hipMemset(src, 1, n):
+            raise SystemExit("hipMalloc/hipMemset failed")
+        for _ in range(3):
+            if hip.hipMemcpy(dst, src, n, 3):   # hipMemcpyDeviceToDevice
+                raise SystemExit("hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+        hip.hipFree(src), hip.hipFree(dst)
+        print(json.dumps({"copy_calibration_bytes": n, "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X; the DP problems are "
               "independent, so N GPUs would shard pairs by cost (see DESIGN.md) - run with --gpus 1", file=sys.stderr)

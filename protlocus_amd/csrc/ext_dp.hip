@@ -18,7 +18,9 @@
 //   * Per row the wave reduces the row maximum (padded columns included, columns beyond the
 //     reference's padding excluded), applies the ie_coef*log2 length penalty from a host
 //     table, updates the best row (snapshot of its H values in registers) and checks x-drop.
-//   * The query profile lives in LDS as int8 (22 residues x 128C columns); the per-row
+//   * Scores: the 22 x 22 matrix is staged once per work-group in LDS as a table indexed by
+//     column residue x 22 + row residue (a PSSM is staged per problem as an int8 profile of
+//     22 residues x 128C columns), so occupancy is bound by registers, not LDS; the per-row
 //     residue, donor[i+1] and acceptor[i] come from a 4-byte record per row prepared on the
 //     device; records and penalties are streamed 64 rows at a time into one VGPR each.
 #include <hip/hip_runtime.h>
@@ -336,7 +338,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 			a.go = opt->go, a.ge = opt->ge, a.fs = opt->fs, a.xdrop = opt->xdrop, a.eb = opt->end_bonus;
 			a.counter = p->d_counter + g, a.out = p->d_out;
 			// persistent waves: about as many as the CUs keep resident
-			p->grid[g] = (int32_t)std::min<int64_t>(a.n, (int64_t)ncu * kGroupWavesPerCU[g]);
+			p->grid[g] = (int32_t)std::min<int64_t>(a.n, (int64_t)ncu * (opt->pssm? kGroupWavesPerCUPssm[g] : kGroupWavesPerCU[g]));
 		}
 	}
 	return p;
@@ -421,7 +423,7 @@ extern "C" int nsh_ext_plan_run_subset(nsh_ext_plan_t *p, const uint8_t *keep)
 			if (keep[p->prob_h[k].out]) p->sub_h.push_back(p->prob_h[k]);
 		args[g] = p->args[g];
 		args[g].prob = p->d_prob_sub + first, args[g].n = (int32_t)p->sub_h.size() - first;
-		grid[g] = std::min(args[g].n, p->ncu * kGroupWavesPerCU[g]);
+		grid[g] = std::min(args[g].n, p->ncu * (args[g].pssm? kGroupWavesPerCUPssm[g] : kGroupWavesPerCU[g]));
 	}
 	if (!p->sub_h.empty())
 		HIPOK(hipMemcpyAsync(p->d_prob_sub, p->sub_h.data(), p->sub_h.size() * sizeof(ExtProb), hipMemcpyHostToDevice, p->stream));

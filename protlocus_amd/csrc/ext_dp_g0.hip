@@ -5,6 +5,7 @@
 namespace nsh_ext {
 void launch_group0(const ExtArgs &a, int grid, hipStream_t s)
 {
-	hipLaunchKernelGGL((ext_dp_kernel<1, 4>), dim3(grid), dim3(64), 0, s, a);
+	if (a.pssm) hipLaunchKernelGGL((ext_dp_kernel<1, 4, true>), dim3(grid), dim3(64), 0, s, a);
+	else hipLaunchKernelGGL((ext_dp_kernel<1, 4, false>), dim3(grid), dim3(64), 0, s, a);
 }
 } // namespace nsh_ext

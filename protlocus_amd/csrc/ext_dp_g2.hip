@@ -5,6 +5,7 @@
 namespace nsh_ext {
 void launch_group2(const ExtArgs &a, int grid, hipStream_t s)
 {
-	hipLaunchKernelGGL((ext_dp_kernel<9, 16>), dim3(grid), dim3(64), 0, s, a);
+	// the table path's per-column offsets would spill at C > 8: always the LDS profile here
+	hipLaunchKernelGGL((ext_dp_kernel<9, 16, true>), dim3(grid), dim3(64), 0, s, a);
 }
 } // namespace nsh_ext

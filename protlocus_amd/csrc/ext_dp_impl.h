@@ -46,8 +46,15 @@ DEV int dpp0(int x) { return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xf, 0xf, t
 template <int CTRL>
 DEV int dppk(int x) { return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xf, 0xf, false); }
 
-template <int C>
-DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L)
+// PROF: scores are staged per problem as an int8 profile (22 residues x 128C columns) in LDS,
+// from the PSSM rows or the matrix; otherwise (matrix only) the 22 x 22 matrix is staged once per
+// work-group as an int16 23 x 22 table (row 22 = padding, -32768) indexed by column residue * 22
+// + row residue,
+// which leaves the LDS small enough for register-bound occupancy.
+constexpr int TAB_PAD = 22;    // table row of the padded columns (all -128)
+
+template <int C, bool PROF>
+DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, int *dring, const int L)
 {
 	constexpr int HALF = 64 * C, ROWB = 128 * C;
 	const int nl = P.nl, al = P.al, npad = P.npad, asz = a.asize;
@@ -58,35 +65,43 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 	const int pen_len = 3 * al;
 
 	// ---- per-lane column constants and the int8 query profile in LDS ----
-	// sel[k]: v_perm selector packing the two sign-extended profile bytes of column k into an
+	// sel[k] (profile path): v_perm selector packing the two sign-extended profile bytes of column k into an
 	// int16 pair; padded columns (al <= c < npad) become -32768 like the reference profile.
 	// lim[k]: columns at or beyond npad (not part of the reference's row maximum) are masked.
-	uint32_t sel[C];
+	uint32_t sel[PROF? C : 1];
 	s2 lim[C];
 #pragma unroll
 	for (int k = 0; k < C; ++k) {
 		const int clo = L * C + k, chi = HALF + L * C + k;
 		const bool plo = clo >= al && clo < npad, phi = chi >= al && chi < npad;
-		sel[k] = (plo? 0x000Cu : 0x0100u) | (phi? 0x040C0000u : 0x05040000u);
+		if (PROF) sel[k] = (plo? 0x000Cu : 0x0100u) | (phi? 0x040C0000u : 0x05040000u);
 		lim[k] = s2{(short)(clo < npad? 32767 : NEG), (short)(chi < npad? 32767 : NEG)};
 	}
-	__syncthreads();
+	// table offsets of the lane's columns (matrix scores): column residue * 22 (padding row past
+	// al), low half in bits 0-15, high half in bits 16-31
+	int ta[C];
+	if (PROF) {
+		__syncthreads();
 #pragma unroll
-	for (int h = 0; h < 2; ++h)
+		for (int h = 0; h < 2; ++h)
+#pragma unroll
+			for (int k = 0; k < C; ++k) {
+				const int c = h * HALF + L * C + k;
+				const bool real = c < al;
+				const int q = P.left? al - 1 - c : c;
+				int8_t *dst = prof + h * HALF + k * 64 + L;
+				const int aa = real? aas[c] : 0;
+				for (int r = 0; r < NRES; ++r)
+					dst[r * ROWB] = (int8_t)(real? (pssm? pssm[q * asz + r] : a.mat[r * asz + aa]) : -128);
+			}
+		__syncthreads();
+	} else {
 #pragma unroll
 		for (int k = 0; k < C; ++k) {
-			const int c = h * HALF + L * C + k;
-			const bool real = c < al;
-			const int q = P.left? al - 1 - c : c;
-			const int aa = real? aas[c] : 0;
-			int8_t *dst = prof + h * HALF + k * 64 + L;
-			for (int r = 0; r < NRES; ++r) {
-				int v = -128;
-				if (real) v = pssm? pssm[q * asz + r] : a.mat[r * asz + aa];
-				dst[r * ROWB] = (int8_t)v;
-			}
+			const int clo = L * C + k, chi = HALF + L * C + k;
+			ta[k] = (clo < al? aas[clo] : TAB_PAD) * NRES | (chi < al? aas[chi] : TAB_PAD) * NRES << 16;
 		}
-	__syncthreads();
+	}
 
 	// ---- scan constants (biased unsigned): the insertion carry decays by ge per column ----
 	const int geC = ge * C;
@@ -97,11 +112,15 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 	const s2 GO = spl(go), GE = spl(ge), IO = spl(io), FS = spl(fs);
 
 	// ---- DP state: three rows of H and D, intron states, best-row snapshot ----
-	s2 Hs[3][C], Ds[3][C], A[C], B[C], Cc[C], Hmax[C], nh[3];
+	// (the best row's H values are kept in LDS: snap[k * 64 + L], written on each new best row)
+	// (D of rows i-1..i-3 lives in LDS: dring[(row % 3) * C + k][L])
+	s2 Hs[3][C], A[C], B[C], Cc[C], nh[3];
 #pragma unroll
 	for (int k = 0; k < C; ++k) {
-		Hs[0][k] = Hs[1][k] = Hs[2][k] = Ds[0][k] = Ds[1][k] = Ds[2][k] = up(PACKNEG);
-		A[k] = B[k] = Cc[k] = Hmax[k] = up(PACKNEG);
+		Hs[0][k] = Hs[1][k] = Hs[2][k] = up(PACKNEG);
+		dring[(0 * C + k) * 64 + L] = dring[(1 * C + k) * 64 + L] = dring[(2 * C + k) * 64 + L] = PACKNEG;
+		A[k] = B[k] = Cc[k] = up(PACKNEG);
+		snap[k * 64 + L] = PACKNEG;
 	}
 	// left neighbour column of each lane's first column, rows i-3 (nh[0]), i-2 (nh[1]), i-1 (nh[2])
 	// at the first row; column -1 holds H(-1,-1)=0 and H(0,-1)=H(1,-1)=-fs only for row 2.
@@ -125,9 +144,15 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 	// profile bytes of the next row, loaded one row ahead
 	int pf_lo[C], pf_hi[C];
 	auto prefetch = [&](int res) {
-		const int8_t *pr = prof + res * ROWB + L;
+		if (PROF) {
+			const int8_t *pr = prof + res * ROWB + L;
 #pragma unroll
-		for (int k = 0; k < C; ++k) pf_lo[k] = pr[k * 64], pf_hi[k] = pr[HALF + k * 64];
+			for (int k = 0; k < C; ++k) pf_lo[k] = pr[k * 64], pf_hi[k] = pr[HALF + k * 64];
+		} else {
+			const int16_t *tab = (const int16_t*)prof;
+#pragma unroll
+			for (int k = 0; k < C; ++k) pf_lo[k] = tab[(ta[k] & 0xffff) + res], pf_hi[k] = tab[(ta[k] >> 16) + res];
+		}
 	};
 	if (nl >= 3) {
 		const int r0 = __builtin_amdgcn_readlane(recv, 0), r1 = __builtin_amdgcn_readlane(recv, 1);
@@ -140,7 +165,8 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 
 	auto row = [&](auto Rc) -> bool {
 		constexpr int R = decltype(Rc)::value, R2 = (R + 1) % 3, R1 = (R + 2) % 3;
-		s2 *H3 = Hs[R], *H2 = Hs[R2], *H1 = Hs[R1], *D3 = Ds[R];
+		s2 *H3 = Hs[R], *H2 = Hs[R2], *H1 = Hs[R1];
+		int *D3 = dring + R * C * 64 + L;
 		if ((i & 63) == 0) { // next block of records
 			recv = recn, penv = penn;
 			if (i + 64 < nl) load_blk(i + 64, recn, penn);
@@ -157,7 +183,8 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 		s2 s[C];
 		if (C > 8) prefetch(res);
 #pragma unroll
-		for (int k = 0; k < C; ++k) s[k] = up((int)__builtin_amdgcn_perm((uint32_t)pf_hi[k], (uint32_t)pf_lo[k], sel[k]));
+		for (int k = 0; k < C; ++k)   // table path: int16 values, the low halves of the two words
+			s[k] = up((int)__builtin_amdgcn_perm((uint32_t)pf_hi[k], (uint32_t)pf_lo[k], PROF? sel[k] : 0x05040100u));
 		if (C <= 8 && i + 1 < nl) {
 			const int j = (i + 1) & 63;
 			prefetch((j? __builtin_amdgcn_readlane(recv, j) : __builtin_amdgcn_readlane(recn, 0)) & 0xff);
@@ -173,13 +200,13 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 		for (int k = 0; k < C; ++k) {
 			const s2 h3p = k? H3[k - 1] : n3, up_ = k? u[k - 1] : u_n, mp = k? m[k - 1] : m_n;
 			const s2 hm = adds(h3p, s[k]);                                  // H(i-3,j-1) + s
-			const s2 dn = subs(vmax(subs(H3[k], GO), D3[k]), GEI);          // D
+			const s2 dn = subs(vmax(subs(H3[k], GO), up(D3[k * 64])), GEI);   // D
 			A[k] = vmax(subs(u[k], DM1), A[k]);                             // phase-0 intron
 			B[k] = vmax(subs(up_, D0), B[k]);                               // phase-1
 			Cc[k] = vmax(subs(up_, DP1), Cc[k]);                            // phase-2
 			const s2 f = subs(vmax(m[k], mp), FS);                          // 4 frameshift moves
 			nonI[k] = vmax(vmax(vmax(hm, dn), vmax(subs(A[k], A0), subs(B[k], AM2))), vmax(subs(Cc[k], AM1), f));
-			D3[k] = dn;
+			D3[k * 64] = pk(dn);
 		}
 		// phase B: insertion chain inside the lane (zero carry); opening from an H that is itself an
 		// insertion is dominated by extending it, so the chain only needs the non-insertion terms
@@ -245,7 +272,7 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 		if (tlog > best_log) {
 			best_log = tlog, best_i = i;
 #pragma unroll
-			for (int k = 0; k < C; ++k) Hmax[k] = H3[k];
+			for (int k = 0; k < C; ++k) snap[k * 64 + L] = pk(H3[k]);
 		}
 		// left neighbour of the next row: lane L-1's last column; lane 0 gets column -1
 		// (neg_inf for rows > 2) in the low half and the low half's last column in the high half
@@ -275,7 +302,8 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 		for (int k = 0; k < C; ++k) {
 			const int j = h * HALF + L * C + k;
 			if (j < al) {
-				const int v = (h? (int)Hmax[k].y : (int)Hmax[k].x) + (j == al - 1? eb : 0);
+				const s2 hm = up(snap[k * 64 + L]);
+				const int v = (h? (int)hm.y : (int)hm.x) + (j == al - 1? eb : 0);
 				if (v > bv) bv = v, bj = j;
 			}
 		}
@@ -292,12 +320,23 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, const int L
 	}
 }
 
-// persistent waves over one group's queue; CLO..CHI columns per lane per half
-template <int CLO, int CHI>
-__global__ void __launch_bounds__(64) ext_dp_kernel(ExtArgs a)
+// persistent waves over one group's queue; CLO..CHI columns per lane per half. The waves-per-SIMD
+// target trades registers for latency hiding (measured: 6 / 3 / 1 for C <= 4 / <= 8 / <= 16).
+template <int CLO, int CHI, bool PROF>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CHI <= 4? 6 : CHI <= 8? 3 : 1)))
+ext_dp_kernel(ExtArgs a)
 {
-	__shared__ int8_t prof[NRES * 128 * CHI];
+	__shared__ __attribute__((aligned(16))) int8_t prof[PROF? NRES * 128 * CHI : (TAB_PAD + 1) * NRES * 2];
+	__shared__ int snap[64 * CHI], dring[3 * 64 * CHI];
 	const int L = threadIdx.x;
+	if (!PROF) {   // the matrix as an int16 [column residue][row residue] table, padding row -32768
+		int16_t *tab = (int16_t*)prof;
+		for (int t = L; t < (TAB_PAD + 1) * NRES; t += 64) {
+			const int aa = t / NRES, r = t % NRES;
+			tab[t] = (int16_t)(aa < TAB_PAD? a.mat[r * a.asize + aa] : NEG);
+		}
+		__syncthreads();
+	}
 	for (;;) {
 		// ticket: every lane takes part in the atomic (lane 0 adds 1, the others 0), so the loop has
 		// no lane-dependent branch; the ticket is lane 0's value, wave-uniform
@@ -305,7 +344,7 @@ __global__ void __launch_bounds__(64) ext_dp_kernel(ExtArgs a)
 		if (p >= a.n) break;
 		const ExtProb P = a.prob[p];
 		const int C = P.C < CLO? CLO : P.C > CHI? CHI : P.C;
-#define NSH_C(K) if (K >= CLO && K <= CHI && C == K) { ext_solve<(K >= CLO && K <= CHI)? K : CLO>(a, P, prof, L); continue; }
+#define NSH_C(K) if (K >= CLO && K <= CHI && C == K) { ext_solve<(K >= CLO && K <= CHI)? K : CLO, PROF>(a, P, prof, snap, dring, L); continue; }
 		NSH_C(1) NSH_C(2) NSH_C(3) NSH_C(4) NSH_C(5) NSH_C(6) NSH_C(7) NSH_C(8)
 		NSH_C(9) NSH_C(10) NSH_C(11) NSH_C(12) NSH_C(13) NSH_C(14) NSH_C(15) NSH_C(16)
 #undef NSH_C

@@ -24,11 +24,16 @@ struct ExtArgs {
 };
 
 
-// column-count groups, each a kernel with its own LDS size (22 x 128C bytes for the group's
-// largest C) and occupancy; the groups run concurrently on separate streams
+// column-count groups, each a kernel with its own register and LDS footprint and occupancy; the
+// groups run concurrently on separate streams
 constexpr int kGroups = 3;
 constexpr int kGroupLo[kGroups] = { 1, 5, 9 }, kGroupHi[kGroups] = { 4, 8, 16 };
-constexpr int kGroupWavesPerCU[kGroups] = { 14, 7, 3 };   // LDS-bound residency (160 KB per CU)
+// persistent waves per CU. With matrix scores the kernels are compiled for 6 / 3 / 1 waves per SIMD
+// (the register budget that measured fastest: more waves hide the per-row dependency chain better
+// than the few spilled registers cost); with per-problem profiles LDS (22 x 128C bytes per wave)
+// also bounds residency.
+constexpr int kGroupWavesPerCU[kGroups] = { 24, 12, 3 };
+constexpr int kGroupWavesPerCUPssm[kGroups] = { 12, 7, 3 };
 constexpr int kCmax = 16;
 inline int group_of(int C) { return C <= 4? 0 : C <= 8? 1 : 2; }
 void launch_group(int g, const ExtArgs &a, int grid, hipStream_t s);

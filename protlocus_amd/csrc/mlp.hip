@@ -187,28 +187,36 @@ __global__ void head_kernel(const u16 *H2, const float *w3, const float *b3, con
 	}
 }
 
-// dW3[c] = sum_s H2[s][c] dy[s]; db3 = sum_s dy[s]; dH2[s][c] = dy[s] w3[c] [H2 > 0] (BF16)
-__global__ void head_back_kernel(const u16 *H2, const float *dy, const float *w3, int B, float *dW3, u16 *dH2)
+constexpr int ROWS_PER_BLOCK = 16;
+
+// dH2[s][c] = dy[s] w3[c] [H2 > 0] (BF16); dW3[c] += sum_s H2[s][c] dy[s] (zeroed beforehand).
+// 16 samples per work-group, one column per thread.
+__global__ void __launch_bounds__(256) head_back_kernel(const u16 *H2, const float *dy, const float *w3, int B,
+		float *dW3, u16 *dH2)
 {
-	const int c = blockIdx.x * blockDim.x + threadIdx.x;
-	if (c >= HID) return;
+	const int c = threadIdx.x, s0 = blockIdx.x * ROWS_PER_BLOCK;
 	float acc = 0.f;
-	for (int s = 0; s < B; ++s) {
+	for (int s = s0; s < s0 + ROWS_PER_BLOCK && s < B; ++s) {
 		const float h = bf2f(H2[(long)s * HID + c]);
 		acc += h * dy[s];
 		dH2[(long)s * HID + c] = h > 0.f? f2bf(dy[s] * w3[c]) : (u16)0;
 	}
-	dW3[c] = acc;
+	atomicAdd(&dW3[c], acc);
 }
 
-// column sums of a BF16 [B][N] matrix (bias gradients); also sum of a FP32 vector when N == 1
-__global__ void colsum_kernel(const u16 *D, int B, int N, float *out)
+// out[c] += column sums of a BF16 [B][N] matrix over 16 rows per work-group (bias gradients)
+__global__ void __launch_bounds__(256) colsum_kernel(const u16 *D, int B, int N, float *out)
 {
-	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	const int c = threadIdx.x, s0 = blockIdx.x * ROWS_PER_BLOCK;
 	if (c >= N) return;
 	float acc = 0.f;
-	for (int s = 0; s < B; ++s) acc += bf2f(D[(long)s * N + c]);
-	out[c] = acc;
+	for (int s = s0; s < s0 + ROWS_PER_BLOCK && s < B; ++s) acc += bf2f(D[(long)s * N + c]);
+	atomicAdd(&out[c], acc);
+}
+__global__ void fill_f32_kernel(float *p, long n, float v)
+{
+	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = v;
 }
 __global__ void vecsum_kernel(const float *v, int n, float *out)
 {
@@ -383,6 +391,8 @@ extern "C" int nsh_mlp_train_step(nsh_mlp_t *h, const float *X, const float *t, 
 		MLPOK(hipMemcpyAsync(xf, X, (size_t)B * IN * 4, hipMemcpyHostToDevice, s));
 		MLPOK(hipMemcpyAsync(h->t, t, (size_t)B * 4, hipMemcpyHostToDevice, s));
 		hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(((long)B * IN + 255) / 256), dim3(256), 0, s, xf, (long)B * IN, h->X);
+		hipLaunchKernelGGL(fill_f32_kernel, dim3((P_N + 255) / 256), dim3(256), 0, s, h->g, (long)P_N, 0.f);   // atomics below
+		const int nrb = (B + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
 		GemmArgs g = {};
 		// H1 = relu(X W1 + b1): A = X [B][64], B(k,n) = W1[k][n] = W1T[n][k]
 		g.A = h->X, g.sam = IN, g.sak = 1, g.B = h->W1Tb, g.sbn = IN, g.sbk = 1, g.M = B, g.N = HID, g.K = IN;
@@ -392,14 +402,14 @@ extern "C" int nsh_mlp_train_step(nsh_mlp_t *h, const float *X, const float *t, 
 		g.A = h->H1, g.sam = HID, g.B = h->W2Tb, g.sbn = HID, g.K = HID, g.Cb = h->H2, g.bias = h->p + P_B2;
 		gemm(s, g);
 		hipLaunchKernelGGL(head_kernel, dim3((B + 3) / 4), dim3(256), 0, s, h->H2, h->p + P_W3, h->p + P_B3, h->t, B, h->y, h->dy, h->loss);
-		hipLaunchKernelGGL(head_back_kernel, dim3(1), dim3(HID), 0, s, h->H2, h->dy, h->p + P_W3, B, h->g + P_W3, h->dH2);
+		hipLaunchKernelGGL(head_back_kernel, dim3(nrb), dim3(HID), 0, s, h->H2, h->dy, h->p + P_W3, B, h->g + P_W3, h->dH2);
 		hipLaunchKernelGGL(vecsum_kernel, dim3(1), dim3(256), 0, s, h->dy, B, h->g + P_B3);
 		// dW2 = H1^T dH2: A(m,k) = H1[k][m], B(k,n) = dH2[k][n]
 		g = {};
 		g.A = h->H1, g.sam = 1, g.sak = HID, g.B = h->dH2, g.sbn = 1, g.sbk = HID, g.M = HID, g.N = HID, g.K = B;
 		g.epi = 0, g.Cf = h->g + P_W2, g.scm = HID;
 		gemm(s, g);
-		hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(HID), 0, s, h->dH2, B, HID, h->g + P_B2);
+		hipLaunchKernelGGL(colsum_kernel, dim3(nrb), dim3(HID), 0, s, h->dH2, B, HID, h->g + P_B2);
 		// dH1 = (dH2 W2^T) [H1 > 0]: B(k,n) = W2[n][k] (row n of W2 has unit k stride)
 		g = {};
 		g.A = h->dH2, g.sam = HID, g.sak = 1, g.B = h->W2b, g.sbn = HID, g.sbk = 1, g.M = B, g.N = HID, g.K = HID;
@@ -410,7 +420,7 @@ extern "C" int nsh_mlp_train_step(nsh_mlp_t *h, const float *X, const float *t, 
 		g.A = h->X, g.sam = 1, g.sak = IN, g.B = h->dH1, g.sbn = 1, g.sbk = HID, g.M = IN, g.N = HID, g.K = B;
 		g.epi = 0, g.Cf = h->g + P_W1, g.scm = HID;
 		gemm(s, g);
-		hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(HID), 0, s, h->dH1, B, HID, h->g + P_B1);
+		hipLaunchKernelGGL(colsum_kernel, dim3(nrb), dim3(HID), 0, s, h->dH1, B, HID, h->g + P_B1);
 		hipLaunchKernelGGL(vecsum_kernel, dim3(1), dim3(256), 0, s, h->loss, B, h->lsum);
 		if (lr > 0.f) {
 			++h->step;
