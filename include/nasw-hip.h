@@ -243,9 +243,12 @@ int nsh_mlp_get_params(nsh_mlp_t *h, float *params, float *grads);
 /* logits of n candidates; X: n x 64 FP32 features on the host (x_on_device = 0) or device */
 int nsh_mlp_infer(nsh_mlp_t *h, const float *X, int64_t n, int32_t x_on_device, float *logit);
 float nsh_mlp_last_ms(nsh_mlp_t *h);   /* device time of the last inference (ms) */
+/* standardise inputs on the device, (x - mu) / sd per feature, in training and inference */
+int nsh_mlp_set_norm(nsh_mlp_t *h, const float *mu, const float *sd);
 void nsh_mlp_destroy(nsh_mlp_t *h);
 
-/* the same features from raw host strings (proteins ASCII, DNA ASCII or 0..4); out: n x 64 */
+/* the same features from raw host strings (proteins ASCII, DNA ASCII or 0..4), segment scores
+ * with BLOSUM62 (plans use their own matrix); out: n x 64 */
 int nsh_features(int32_t n, const char *as, const int64_t *as_off, const int32_t *al, int64_t l_as,
 		const char *ns, const int64_t *ns_off, const int32_t *nl, int64_t l_ns, float *out);
 

@@ -65,6 +65,7 @@ def lib():
         L.nsh_mlp_train_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_void_p]
         L.nsh_mlp_get_params.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.nsh_mlp_infer.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]
+        L.nsh_mlp_set_norm.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.nsh_mlp_last_ms.argtypes = [C.c_void_p]
         L.nsh_mlp_last_ms.restype = C.c_float
         L.nsh_mlp_destroy.argtypes = [C.c_void_p]
@@ -183,12 +184,17 @@ def features(prots, dnas):
 class MLP:
     """the prescreen network (nsh_mlp_*): FP32 parameters in, BF16 MFMA kernels on the device"""
 
-    def __init__(self, params):
+    def __init__(self, params, mu=None, sd=None):
         params = np.ascontiguousarray(params, dtype=np.float32)
         assert params.size == lib().nsh_mlp_param_count()
         self.h = lib().nsh_mlp_create(params.ctypes.data)
         if not self.h:
             raise RuntimeError("nsh_mlp_create: " + error())
+        if mu is not None:
+            mu = np.ascontiguousarray(mu, dtype=np.float32)
+            sd = np.ascontiguousarray(sd, dtype=np.float32)
+            if not lib().nsh_mlp_set_norm(self.h, mu.ctypes.data, sd.ctypes.data):
+                raise RuntimeError("nsh_mlp_set_norm: " + error())
 
     def train_step(self, X, t, lr):
         X = np.ascontiguousarray(X, dtype=np.float32)

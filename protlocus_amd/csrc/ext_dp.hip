@@ -440,7 +440,7 @@ extern "C" const float *nsh_ext_plan_features(nsh_ext_plan_t *p, float *host_out
 		return nullptr;
 	}
 	if (!p->d_feat) HIPOK(dalloc(&p->d_feat, (size_t)p->n * 64));
-	nsh_feat::launch(p->d_prep, p->n, p->d_ns, p->d_as, p->d_pc, p->d_feat, p->stream);
+	nsh_feat::launch(p->d_prep, p->n, p->d_ns, p->d_as, p->d_pc, p->d_mat, p->d_feat, p->stream);
 	HIPOK(hipGetLastError());
 	if (host_out) HIPOK(hipMemcpyAsync(host_out, p->d_feat, (size_t)p->n * 64 * 4, hipMemcpyDeviceToHost, p->stream));
 	HIPOK(hipStreamSynchronize(p->stream));

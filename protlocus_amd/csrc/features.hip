@@ -13,7 +13,9 @@
 // 11 GC / nl | 12 GT / nl * 16 | 13 AG / nl * 16 | 14 anchors of the dominant frame / anchors |
 // 15 largest diagonal bin (8 codons wide, per frame) / al | 16 bins with >= 3 anchors / 10 |
 // 17 protein positions hit / al | 18 codon positions hit / (nl/3) | 19 sd(diagonal) / al |
-// 20-39 protein composition | 40-59 dominant-frame composition | 60-63 anchors per query quarter / al
+// 20-39 protein composition | 40-43 best ungapped segment scores (score matrix) of the four
+// fullest diagonal bins, descending, / al | 44 their sum / al | 45 how many reach 20, / 4 |
+// 46-59 dominant-frame composition of residues 0-13 | 60-63 anchors per query quarter / al
 #include <hip/hip_runtime.h>
 #include <chrono>
 #include <cmath>
@@ -34,6 +36,7 @@ struct FeatArgs {
 	const nsh_prep::PrepProb *pp;   // per candidate: ns_off, as_off, nl, al (raw strings)
 	const uint8_t *ns, *as;         // raw DNA (ASCII or 0..4) and protein (ASCII) bytes
 	const nsh_prep::PrepConst *pc;  // nt4 / aa20 / codon tables
+	const int8_t *mat;              // 22 x 22 scores, [target residue][protein residue]
 	int32_t n;
 	float *out;                     // n x 64
 };
@@ -128,6 +131,71 @@ __global__ void __launch_bounds__(256) feature_kernel(FeatArgs a)
 	__syncthreads();
 	atomicAdd(&cnt[56], hq);
 	atomicAdd(&cnt[57], ht);
+	// the four fullest diagonal bins (ties: smaller frame * nb + bin), one at a time
+	__shared__ int topbin[4], redv[4], redi[4], segsc[4][8];
+	for (int r = 0; r < 4; ++r) {
+		int bv = 0, bi = INT_MAX;
+		for (int i = t; i < 3 * nb; i += 256) {
+			const int f = i / nb, bin = i % nb;
+			const int v = (hist[f][bin >> 1] >> ((bin & 1) * 16)) & 0xffff;
+			if (v > bv) bv = v, bi = i;
+		}
+#pragma unroll
+		for (int d = 32; d >= 1; d >>= 1) {
+			const int ov = __shfl_xor(bv, d), oi = __shfl_xor(bi, d);
+			if (ov > bv || (ov == bv && oi < bi)) bv = ov, bi = oi;
+		}
+		if ((t & 63) == 0) redv[t >> 6] = bv, redi[t >> 6] = bi;
+		__syncthreads();
+		if (t == 0) {
+			int v = 0, ix = INT_MAX;
+			for (int k = 0; k < 4; ++k)
+				if (redv[k] > v || (redv[k] == v && redi[k] < ix)) v = redv[k], ix = redi[k];
+			topbin[r] = v > 0? ix : -1;
+			if (v > 0) {
+				const int f = ix / nb, bin = ix % nb;
+				hist[f][bin >> 1] &= ~(0xffffu << ((bin & 1) * 16));
+			}
+		}
+		__syncthreads();
+	}
+	// best ungapped segment (max subarray sum, empty allowed) on each of a bin's 8 diagonals: every
+	// thread takes one eighth of one diagonal of one bin (4 x 8 x 8 = 256), summarises its chunk
+	// (total, best prefix, best suffix, best inside) and 32 threads chain the eight summaries
+	__shared__ int csum[4][8][8][4];
+	{
+		const int r = t >> 6, dd = (t >> 3) & 7, ch = t & 7, tb = topbin[r];
+		int tot = 0, pre = 0, suf = 0, ins = 0;
+		if (tb >= 0) {
+			const int f = tb / nb, d = (tb % nb) * 8 - al + dd;
+			const int q0 = d < 0? -d : 0, q1 = min(al, ncodf[f] - d), len = q1 > q0? q1 - q0 : 0;
+			const int sz = (len + 7) / 8, qa = q0 + ch * sz, qb = min(q1, qa + sz);
+			int run = 0;
+			for (int q = qa; q < qb; ++q) {
+				const int o = f + 3 * (q + d), b0 = w(o), b1 = w(o + 1), b2 = w(o + 2);
+				const int rr = b0 < 4 && b1 < 4 && b2 < 4? codon[b0 << 4 | b1 << 2 | b2] : 21;
+				const int v = a.mat[rr * 22 + p(q)];
+				tot += v;
+				pre = max(pre, tot);
+				run = max(0, run + v);
+				ins = max(ins, run);
+			}
+			suf = run;   // the best sum ending at the chunk's last position (clamped at zero)
+		}
+		csum[r][dd][ch][0] = tot, csum[r][dd][ch][1] = pre, csum[r][dd][ch][2] = suf, csum[r][dd][ch][3] = ins;
+	}
+	__syncthreads();
+	if (t < 32) {
+		const int r = t >> 3, dd = t & 7;
+		int best = 0, s = 0;   // s: best sum ending at the previous chunk's end (>= 0)
+		for (int ch = 0; ch < 8; ++ch) {
+			const int *cs = csum[r][dd][ch];
+			best = max(best, max(cs[3], s + cs[1]));
+			s = max(cs[2], s + cs[0]);
+			s = max(s, 0);
+		}
+		segsc[r][dd] = best;
+	}
 	__syncthreads();
 	if (t == 0) {
 		float *o = a.out + (int64_t)c * NF;
@@ -154,7 +222,17 @@ __global__ void __launch_bounds__(256) feature_kernel(FeatArgs a)
 		}
 		o[19] = (float)(sd / (double)al);
 		for (int r = 0; r < 20; ++r) o[20 + r] = (float)cnt[14 + r] / fal;
-		for (int r = 0; r < 20; ++r) o[40 + r] = (float)cnt[34 + r] / (float)(ncodf[dom] > 0? ncodf[dom] : 1);
+		int sg[4];
+		for (int r = 0; r < 4; ++r) {
+			sg[r] = 0;
+			for (int k = 0; k < 8; ++k) sg[r] = max(sg[r], segsc[r][k]);
+		}
+		for (int x = 1; x < 4; ++x)   // descending
+			for (int y = x; y > 0 && sg[y] > sg[y - 1]; --y) { const int tmp = sg[y]; sg[y] = sg[y - 1], sg[y - 1] = tmp; }
+		for (int r = 0; r < 4; ++r) o[40 + r] = (float)sg[r] / fal;
+		o[44] = (float)(sg[0] + sg[1] + sg[2] + sg[3]) / fal;
+		o[45] = (float)((sg[0] >= 20) + (sg[1] >= 20) + (sg[2] >= 20) + (sg[3] >= 20)) / 4.f;
+		for (int r = 0; r < 14; ++r) o[46 + r] = (float)cnt[34 + r] / (float)(ncodf[dom] > 0? ncodf[dom] : 1);
 		for (int k = 0; k < 4; ++k) o[60 + k] = (float)cnt[10 + k] / fal;
 	}
 #undef p
@@ -164,10 +242,10 @@ __global__ void __launch_bounds__(256) feature_kernel(FeatArgs a)
 } // namespace
 
 void nsh_feat::launch(const nsh_prep::PrepProb *pp, int32_t n, const uint8_t *ns, const uint8_t *as,
-		const nsh_prep::PrepConst *pc, float *out, hipStream_t s)
+		const nsh_prep::PrepConst *pc, const int8_t *mat, float *out, hipStream_t s)
 {
 	FeatArgs fa;
-	fa.pp = pp, fa.ns = ns, fa.as = as, fa.pc = pc, fa.n = n, fa.out = out;
+	fa.pp = pp, fa.ns = ns, fa.as = as, fa.pc = pc, fa.mat = mat, fa.n = n, fa.out = out;
 	hipLaunchKernelGGL(feature_kernel, dim3(n), dim3(256), 0, s, fa);
 }
 
@@ -195,6 +273,7 @@ extern "C" int nsh_features(int32_t n, const char *as, const int64_t *as_off, co
 	uint8_t *d_as = nullptr, *d_ns = nullptr;
 	nsh_prep::PrepProb *d_pp = nullptr;
 	nsh_prep::PrepConst *d_pc = nullptr;
+	int8_t *d_mat = nullptr;
 	float *d_out = nullptr;
 	hipStream_t s = nullptr;
 	int ok = 0;
@@ -206,11 +285,13 @@ extern "C" int nsh_features(int32_t n, const char *as, const int64_t *as_off, co
 		FOK(hipMalloc((void**)&d_pp, (size_t)n * sizeof(nsh_prep::PrepProb)));
 		FOK(hipMalloc((void**)&d_pc, sizeof(pc)));
 		FOK(hipMalloc((void**)&d_out, (size_t)n * NF * 4));
+		FOK(hipMalloc((void**)&d_mat, 22 * 22));
+		FOK(hipMemcpyAsync(d_mat, ns_mat_blosum62, 22 * 22, hipMemcpyHostToDevice, s));
 		FOK(hipMemcpyAsync(d_as, as, (size_t)l_as, hipMemcpyHostToDevice, s));
 		FOK(hipMemcpyAsync(d_ns, ns, (size_t)l_ns, hipMemcpyHostToDevice, s));
 		FOK(hipMemcpyAsync(d_pp, pp.data(), (size_t)n * sizeof(nsh_prep::PrepProb), hipMemcpyHostToDevice, s));
 		FOK(hipMemcpyAsync(d_pc, &pc, sizeof(pc), hipMemcpyHostToDevice, s));
-		nsh_feat::launch(d_pp, n, d_ns, d_as, d_pc, d_out, s);
+		nsh_feat::launch(d_pp, n, d_ns, d_as, d_pc, d_mat, d_out, s);
 		FOK(hipGetLastError());
 		FOK(hipMemcpyAsync(out, d_out, (size_t)n * NF * 4, hipMemcpyDeviceToHost, s));
 		FOK(hipStreamSynchronize(s));
@@ -219,7 +300,7 @@ extern "C" int nsh_features(int32_t n, const char *as, const int64_t *as_off, co
 	}
 fail:
 	{
-		void *bufs[] = { d_as, d_ns, d_pp, d_pc, d_out };
+		void *bufs[] = { d_as, d_ns, d_pp, d_pc, d_mat, d_out };
 		for (void *b : bufs) if (b) (void)hipFree(b);
 	}
 	if (s) (void)hipStreamDestroy(s);

@@ -256,10 +256,11 @@ __global__ void to_bf16_kernel(const float *W, int in, int out, u16 *Wb, u16 *WT
 	Wb[i] = b;
 	WTb[(long)c * in + r] = b;
 }
-__global__ void f32_to_bf16_kernel(const float *x, long n, u16 *y)
+// feature standardisation fused with the BF16 conversion: y = bf16((x - mu[f]) * isd[f]), f = i % 64
+__global__ void f32_to_bf16_kernel(const float *x, long n, const float *mu, const float *isd, u16 *y)
 {
 	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) y[i] = f2bf(x[i]);
+	if (i < n) { const int f = (int)(i % IN); y[i] = f2bf((x[i] - mu[f]) * isd[f]); }
 }
 
 } // namespace
@@ -284,6 +285,7 @@ struct nsh_mlp_s {
 	// inference scratch
 	long icap = 0;
 	u16 *Xi = nullptr; float *Xf = nullptr, *li = nullptr;
+	float *norm = nullptr;   // mu[64], 1/sd[64] of the input standardisation (0 and 1 by default)
 	std::mutex mu;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	float last_ms = 0.f;
@@ -318,6 +320,12 @@ extern "C" nsh_mlp_t *nsh_mlp_create(const float *params)
 	MLPOK(hipMalloc((void**)&h->W1Tb, IN * HID * 2));
 	MLPOK(hipMalloc((void**)&h->W2b, HID * HID * 2));
 	MLPOK(hipMalloc((void**)&h->W2Tb, HID * HID * 2));
+	MLPOK(hipMalloc((void**)&h->norm, 2 * IN * 4));
+	{
+		float nm[2 * IN];
+		for (int f = 0; f < IN; ++f) nm[f] = 0.f, nm[IN + f] = 1.f;
+		MLPOK(hipMemcpy(h->norm, nm, sizeof(nm), hipMemcpyHostToDevice));
+	}
 	MLPOK(hipMemcpy(h->p, params, P_N * 4, hipMemcpyHostToDevice));
 	MLPOK(hipMemset(h->m, 0, P_N * 4));
 	MLPOK(hipMemset(h->v, 0, P_N * 4));
@@ -334,7 +342,7 @@ extern "C" void nsh_mlp_destroy(nsh_mlp_t *h)
 {
 	if (!h) return;
 	void *bufs[] = { h->p, h->g, h->m, h->v, h->W1b, h->W1Tb, h->W2b, h->W2Tb, h->X, h->H1, h->H2, h->dH1, h->dH2,
-		h->Xt, h->t, h->y, h->dy, h->loss, h->lsum, h->Xi, h->Xf, h->li };
+		h->norm, h->Xt, h->t, h->y, h->dy, h->loss, h->lsum, h->Xi, h->Xf, h->li };
 	for (void *b : bufs) if (b) (void)hipFree(b);
 	if (h->e0) (void)hipEventDestroy(h->e0);
 	if (h->e1) (void)hipEventDestroy(h->e1);
@@ -390,7 +398,8 @@ extern "C" int nsh_mlp_train_step(nsh_mlp_t *h, const float *X, const float *t, 
 	{
 		MLPOK(hipMemcpyAsync(xf, X, (size_t)B * IN * 4, hipMemcpyHostToDevice, s));
 		MLPOK(hipMemcpyAsync(h->t, t, (size_t)B * 4, hipMemcpyHostToDevice, s));
-		hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(((long)B * IN + 255) / 256), dim3(256), 0, s, xf, (long)B * IN, h->X);
+		hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(((long)B * IN + 255) / 256), dim3(256), 0, s, xf, (long)B * IN,
+			h->norm, h->norm + IN, h->X);
 		hipLaunchKernelGGL(fill_f32_kernel, dim3((P_N + 255) / 256), dim3(256), 0, s, h->g, (long)P_N, 0.f);   // atomics below
 		const int nrb = (B + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
 		GemmArgs g = {};
@@ -465,7 +474,8 @@ extern "C" int nsh_mlp_infer(nsh_mlp_t *h, const float *X, int64_t n, int32_t x_
 			xd = h->Xf;
 		}
 		MLPOK(hipEventRecord(h->e0, s));
-		hipLaunchKernelGGL(f32_to_bf16_kernel, dim3((n * IN + 255) / 256), dim3(256), 0, s, xd, n * IN, h->Xi);
+		hipLaunchKernelGGL(f32_to_bf16_kernel, dim3((n * IN + 255) / 256), dim3(256), 0, s, xd, n * IN, h->norm,
+			h->norm + IN, h->Xi);
 		hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, s, h->Xi, (int)n, h->W1Tb,
 			h->p + P_B1, h->W2Tb, h->p + P_B2, h->p + P_W3, h->p + P_B3, h->li);
 		MLPOK(hipEventRecord(h->e1, s));
@@ -481,3 +491,17 @@ fail:
 }
 
 extern "C" float nsh_mlp_last_ms(nsh_mlp_t *h) { return h? h->last_ms : 0.f; }
+
+// input standardisation applied on the device before the first layer: (x - mu) / sd per feature
+extern "C" int nsh_mlp_set_norm(nsh_mlp_t *h, const float *mu, const float *sd)
+{
+	if (!h || !mu || !sd) { nsh_set_error("mlp: bad arguments"); return 0; }
+	float nm[2 * IN];
+	for (int f = 0; f < IN; ++f) {
+		if (!(sd[f] > 0.f)) { nsh_set_error("mlp: sd[%d] must be positive", f); return 0; }
+		nm[f] = mu[f], nm[IN + f] = 1.f / sd[f];
+	}
+	std::lock_guard<std::mutex> lk(h->mu);
+	if (hipMemcpy(h->norm, nm, sizeof(nm), hipMemcpyHostToDevice) != hipSuccess) { nsh_set_error("mlp: copy failed"); return 0; }
+	return 1;
+}

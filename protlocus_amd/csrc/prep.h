@@ -23,6 +23,6 @@ void launch(const PrepProb *pp, int32_t n, const uint8_t *raw_ns, const uint8_t 
 namespace nsh_feat {
 // the 64 prescreen features of n candidates (features.hip); out: n x 64 floats, device
 void launch(const nsh_prep::PrepProb *pp, int32_t n, const uint8_t *ns, const uint8_t *as,
-		const nsh_prep::PrepConst *pc, float *out, hipStream_t s);
+		const nsh_prep::PrepConst *pc, const int8_t *mat, float *out, hipStream_t s);
 } // namespace nsh_feat
 #endif

@@ -75,29 +75,20 @@ def init_params(rng):
     return np.concatenate([W1.ravel(), np.zeros(256), W2.ravel(), np.zeros(256), w3, [0.0]]).astype(np.float32)
 
 
-def fold_standardisation(params, mu, sd):
-    """a network on standardised features -> the same network on raw features"""
-    p = params.astype(np.float64).copy()
-    W1 = p[:64 * 256].reshape(64, 256)
-    b1 = p[64 * 256:64 * 256 + 256]
-    W1n = W1 / sd[:, None]
-    b1n = b1 - (mu / sd) @ W1
-    p[:64 * 256] = W1n.ravel()
-    p[64 * 256:64 * 256 + 256] = b1n
-    return p.astype(np.float32)
-
-
-def train(X, y, epochs=12, batch=1024, lr=2e-3, seed=0, log=None):
-    """train on the GPU; returns FP32 parameters for raw features"""
+def train(X, y, epochs=12, batch=1024, lr=2e-3, seed=0, pos_weight=1.0, log=None):
+    """train on the GPU; returns (FP32 parameters, feature means, feature sds). pos_weight > 1 draws accepted
+    candidates that much more often (the recall target makes the low tail of their logits matter)"""
     rng = np.random.default_rng(seed)
-    mu = X.mean(axis=0)
-    sd = X.std(axis=0) + 1e-3
-    Xs = ((X - mu) / sd).astype(np.float32)
-    net = capi.MLP(init_params(rng))
+    mu = X.mean(axis=0).astype(np.float32)
+    sd = (X.std(axis=0) + 1e-3).astype(np.float32)
+    Xs = np.ascontiguousarray(X, dtype=np.float32)      # standardised on the device
+    net = capi.MLP(init_params(rng), mu, sd)
     n = len(y) // batch * batch
+    w = np.where(y > 0, pos_weight, 1.0)
+    w = w / w.sum()
     try:
         for ep in range(epochs):
-            idx = rng.permutation(len(y))[:n]
+            idx = rng.choice(len(y), size=n, replace=True, p=w) if pos_weight != 1.0 else rng.permutation(len(y))[:n]
             tot = 0.0
             step_lr = lr * (0.5 * (1 + np.cos(np.pi * ep / epochs)))
             for s in range(0, n, batch):
@@ -108,7 +99,7 @@ def train(X, y, epochs=12, batch=1024, lr=2e-3, seed=0, log=None):
         params = net.params()
     finally:
         net.close()
-    return fold_standardisation(params, mu.astype(np.float64), sd.astype(np.float64))
+    return params, mu, sd
 
 
 def calibrate(logits, labels, target=TARGET_RECALL):
@@ -125,19 +116,28 @@ def host_features(plan, n):
     return plan.features_host()
 
 
-def run_prescreened(plan, net, tau, n, rng):
-    """features -> logits -> keep (plus the audit) -> DP on the kept candidates; returns keep mask"""
+def run_prescreened(plan, net, tau, n, rng, stages=None):
+    """features -> logits -> keep (plus the audit) -> DP on the kept candidates; returns keep mask.
+    stages (a dict) receives the wall time of each stage in ms."""
+    t0 = time.perf_counter()
     ptr = plan.features_device()
+    t1 = time.perf_counter()
     logit = net.infer(device_ptr=ptr, n=n)
+    t2 = time.perf_counter()
     keep = logit >= tau
     rejected = np.nonzero(~keep)[0]
     audit = rng.choice(rejected, size=int(round(AUDIT_FRAC * len(rejected))), replace=False) if len(rejected) else []
     keep[audit] = True
+    t3 = time.perf_counter()
     plan.run(keep=keep.astype(np.uint8))
+    t4 = time.perf_counter()
+    if stages is not None:
+        for k, a, b in (("features", t0, t1), ("mlp", t1, t2), ("select", t2, t3), ("dp", t3, t4)):
+            stages[k] = stages.get(k, 0.0) + (b - a) * 1e3
     return keep, logit
 
 
-def evaluate(n_train=16384, n_val=4096, n_eval=16384, seed=11, epochs=12, log=print):
+def evaluate(n_train=16384, n_val=4096, n_eval=16384, seed=11, epochs=12, pos_weight=1.0, lr=2e-3, log=print):
     """end-to-end: train, calibrate, then time the plain and the prescreened pipelines on a
     fresh candidate set; returns a dict of measurements"""
     t0 = time.time()
@@ -150,8 +150,8 @@ def evaluate(n_train=16384, n_val=4096, n_eval=16384, seed=11, epochs=12, log=pr
     if log:
         log("[prescreen] data %.1f s: train %d (%.1f%% accepted), val %d" % (time.time() - t0, len(ytr),
                                                                            100 * ytr.mean(), len(va.prots)))
-    params = train(Xtr, ytr, epochs=epochs, seed=seed, log=log)
-    net = capi.MLP(params)
+    params, mu, sd = train(Xtr, ytr, epochs=epochs, seed=seed, pos_weight=pos_weight, lr=lr, log=log)
+    net = capi.MLP(params, mu, sd)
     lv = net.infer(Xva)
     tau = calibrate(lv, va.labels)
     # hand-feature baseline: the best single feature threshold at the same calibration target
@@ -165,20 +165,22 @@ def evaluate(n_train=16384, n_val=4096, n_eval=16384, seed=11, epochs=12, log=pr
     ev = Batch(*candidates(n_eval, seed + 2)[:2])
     n = len(ev.prots)
     rng = np.random.default_rng(seed + 3)
-    # plain pipeline: the DP of every candidate
-    for _ in range(2):
-        ev.plan.run()
-    t = time.perf_counter()
+    # plain pipeline: the DP of every candidate (mean of 3 timed runs after a warm-up)
     ev.plan.run()
-    plain_s = time.perf_counter() - t
+    t = time.perf_counter()
+    for _ in range(3):
+        ev.plan.run()
+    plain_s = (time.perf_counter() - t) / 3
     accepted = accept(ev.plan.scores(), ev.al)    # refreshes the plan's per-problem results
     assert np.array_equal(accepted, ev.labels)
     cells_plain = ev.plan.cells(1)
     # prescreened pipeline
     run_prescreened(ev.plan, net, tau, n, rng)
+    stages = {}
     t = time.perf_counter()
-    keep, logit = run_prescreened(ev.plan, net, tau, n, rng)
-    pre_s = time.perf_counter() - t
+    for _ in range(3):
+        keep, logit = run_prescreened(ev.plan, net, tau, n, rng, stages)
+    pre_s = (time.perf_counter() - t) / 3
     got = ev.plan.scores()
     cells_pre = ev.plan.cells(1)
     kept_pass = keep & accept(got, ev.al)
@@ -195,6 +197,7 @@ def evaluate(n_train=16384, n_val=4096, n_eval=16384, seed=11, epochs=12, log=pr
         "plain_proteins_per_s": round((n // 2) / plain_s, 1),
         "prescreened_proteins_per_s": round((n // 2) / pre_s, 1),
         "mlp_infer_ms": round(float(net.last_ms()), 4),
+        "prescreened_stage_ms": {k: round(v / 3, 3) for k, v in stages.items()},
         "baseline_feature": int(best_f), "baseline_kept_frac": round(float(base_keep.mean()), 4),
         "baseline_recall": round(float(base_keep[accepted].mean()), 5),
         "train_candidates": int(len(ytr)),

@@ -5,8 +5,9 @@ import math
 import numpy as np
 
 
-def features(prot, dna, nt4, aa20, codon):
-    """prot: protein bytes (ASCII), dna: DNA bytes (ASCII or 0..4); tables as lists"""
+def features(prot, dna, nt4, aa20, codon, mat):
+    """prot: protein bytes (ASCII), dna: DNA bytes (ASCII or 0..4); tables and the 22 x 22 matrix
+    ([target residue][protein residue]) as flat lists"""
     p = [aa20[b] for b in prot]
     w = [nt4[b] for b in dna]
     al, nl = len(p), len(w)
@@ -65,6 +66,30 @@ def features(prot, dna, nt4, aa20, codon):
     for r in p:
         if r < 20:
             pc[r] += 1
+    # the four fullest diagonal bins (ties: smaller frame * nb + bin), scored by the best ungapped
+    # segment over their 8 diagonals
+    nb = (al + ncod) // 8 + 2
+    counts = {}
+    for (f, b), v in hist.items():
+        counts[f * nb + b] = v
+    seg = []
+    for _ in range(4):
+        if not counts or max(counts.values()) <= 0:
+            seg.append(0)
+            continue
+        top = max(counts.values())
+        ix = min(i for i, v in counts.items() if v == top)
+        del counts[ix]
+        f, b = ix // nb, ix % nb
+        best = 0
+        for dd in range(8):
+            d = b * 8 - al + dd
+            sc = 0
+            for q in range(max(0, -d), min(al, ncodf[f] - d)):
+                sc = max(0, sc + mat[aa_at(f, q + d) * 22 + p[q]])
+                best = max(best, sc)
+        seg.append(best)
+    seg.sort(reverse=True)
     f32 = np.float32
     o = np.zeros(64, dtype=np.float32)
     fal, fnl = f32(al), f32(nl)
@@ -93,7 +118,12 @@ def features(prot, dna, nt4, aa20, codon):
     o[19] = sd / al
     for r in range(20):
         o[20 + r] = f32(pc[r]) / fal
-        o[40 + r] = f32(domc[r]) / f32(max(1, ncodf[dom]))
+    for r in range(4):
+        o[40 + r] = f32(seg[r]) / fal
+    o[44] = f32(sum(seg)) / fal
+    o[45] = f32(sum(1 for v in seg if v >= 20)) / f32(4)
+    for r in range(14):
+        o[46 + r] = f32(domc[r]) / f32(max(1, ncodf[dom]))
     for k in range(4):
         o[60 + k] = f32(quart[k]) / fal
     return o

@@ -47,7 +47,7 @@ def test_features_match_cpu(oracle, gpu):
     prots, dnas = _cands(1, 40)
     got = gpu.features(prots, dnas)
     for k, (p, d) in enumerate(zip(prots, dnas)):
-        exp = ref_features(p, d, nt4, aa20, codon)
+        exp = ref_features(p, d, nt4, aa20, codon, oracle.blosum62)
         assert np.allclose(got[k], exp, rtol=1e-5, atol=1e-6), (k, np.nonzero(~np.isclose(got[k], exp, 1e-5, 1e-6)))
 
 
@@ -142,8 +142,8 @@ def test_training_separates_classes(gpu):
     X = rng.normal(0, 1, size=(8192, 64)).astype(np.float32)
     w = rng.normal(0, 1, size=64)
     y = ((X @ w + 0.5 * np.sin(3 * X[:, 0])) > 0).astype(np.float32)
-    params = prescreen.train(X, y, epochs=6, batch=512, lr=3e-3)
-    net = gpu.MLP(params)
+    params, mu, sd = prescreen.train(X, y, epochs=6, batch=512, lr=3e-3)
+    net = gpu.MLP(params, mu, sd)
     try:
         acc = ((net.infer(X) > 0) == (y > 0)).mean()
     finally:
