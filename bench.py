@@ -234,8 +234,8 @@ def main():
         from protlocus_amd import prescreen
         del pairs
         t0 = time.time()
-        ps = prescreen.evaluate(n_train=16384, n_val=4096, n_eval=16384, epochs=12, seed=11, log=log)
-        ps["workload"] = ("16,384 proteins x (true locus + decoy window), protein 50-1000 aa; trained on 16,384 "
+        ps = prescreen.evaluate(n_train=32768, n_val=8192, n_eval=16384, epochs=20, seed=11, log=log)
+        ps["workload"] = ("16,384 proteins x (true locus + decoy window), protein 50-1000 aa; trained on 32,768 "
                           "other proteins' candidates; DP = plain global spliced DP over the window")
         n = ps["candidates"]
         if ps["mlp_infer_ms"] > 0:

@@ -143,11 +143,12 @@ struct PrepView {
 	}
 };
 
-__global__ void __launch_bounds__(256) ext_prep_kernel(const PrepProb *pp, const uint8_t *raw_ns, const uint8_t *raw_ss,
-		const uint8_t *raw_as, uint32_t *rec, uint8_t *aas, const PrepConst *pcg)
+// idx (optional): the problems to prepare, one per work-group (subset runs)
+__global__ void __launch_bounds__(256) ext_prep_kernel(const PrepProb *pp, const int32_t *idx, const uint8_t *raw_ns,
+		const uint8_t *raw_ss, const uint8_t *raw_as, uint32_t *rec, uint8_t *aas, const PrepConst *pcg)
 {
 	__shared__ PrepConst pc;
-	const PrepProb P = pp[blockIdx.x];
+	const PrepProb P = pp[idx? idx[blockIdx.x] : (int32_t)blockIdx.x];
 	for (int k = threadIdx.x; k < (int)sizeof(PrepConst); k += blockDim.x) ((uint8_t*)&pc)[k] = ((const uint8_t*)pcg)[k];
 	__syncthreads();
 	PrepView v;
@@ -176,7 +177,7 @@ __global__ void __launch_bounds__(256) ext_prep_kernel(const PrepProb *pp, const
 void nsh_prep::launch(const PrepProb *pp, int32_t n, const uint8_t *raw_ns, const uint8_t *raw_ss, const uint8_t *raw_as,
 		uint32_t *rec, uint8_t *aas, const PrepConst *pc, hipStream_t s)
 {
-	hipLaunchKernelGGL(ext_prep_kernel, dim3(n), dim3(256), 0, s, pp, raw_ns, raw_ss, raw_as, rec, aas, pc);
+	hipLaunchKernelGGL(ext_prep_kernel, dim3(n), dim3(256), 0, s, pp, (const int32_t*)nullptr, raw_ns, raw_ss, raw_as, rec, aas, pc);
 }
 
 // ======================================================================================
@@ -211,13 +212,15 @@ struct nsh_ext_plan_s {
 	// subset runs and features
 	ExtProb *d_prob_sub = nullptr;
 	std::vector<ExtProb> sub_h;
+	int32_t *d_idx_sub = nullptr;            // kept input indices (their inputs are prepared)
+	std::vector<int32_t> idx_h;
 	float *d_feat = nullptr;
 };
 
 static void plan_free_dev(nsh_ext_plan_t *p)
 {
 	void *bufs[] = { p->d_ns, p->d_ss, p->d_as, p->d_aas, p->d_rec, p->d_mat, p->d_pssm, p->d_pen, p->d_counter,
-		p->d_out, p->d_prob, p->d_prep, p->d_pc, p->d_prob_sub, p->d_feat };
+		p->d_out, p->d_prob, p->d_prep, p->d_pc, p->d_prob_sub, p->d_idx_sub, p->d_feat };
 	for (void *b : bufs) if (b) (void)hipFree(b);
 }
 
@@ -378,11 +381,14 @@ __global__ void clear_out_kernel(int32_t *out, int32_t n)
 	if (i < n) out[4 * i] = 0, out[4 * i + 1] = 0, out[4 * i + 2] = INT_MIN, out[4 * i + 3] = 0;
 }
 
-static int plan_launch(nsh_ext_plan_t *p, const ExtArgs *args, const int32_t *grid, bool clear_out)
+// idx/n_idx: prepare only these inputs (subset runs); nullptr prepares all p->n
+static int plan_launch(nsh_ext_plan_t *p, const ExtArgs *args, const int32_t *grid, bool clear_out, const int32_t *idx,
+		int32_t n_idx)
 {
 	HIPOK(hipEventRecord(p->ev0, p->stream));
-	hipLaunchKernelGGL(ext_prep_kernel, dim3(p->n), dim3(256), 0, p->stream, p->d_prep, p->d_ns, p->d_ss, p->d_as,
-			p->d_rec, p->d_aas, p->d_pc);
+	if (n_idx > 0)
+		hipLaunchKernelGGL(ext_prep_kernel, dim3(n_idx), dim3(256), 0, p->stream, p->d_prep, idx, p->d_ns, p->d_ss, p->d_as,
+				p->d_rec, p->d_aas, p->d_pc);
 	hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, p->stream, p->d_counter, kGroups, 0);
 	if (clear_out) hipLaunchKernelGGL(clear_out_kernel, dim3((p->n + 255) / 256), dim3(256), 0, p->stream, p->d_out, p->n);
 	HIPOK(hipGetLastError());
@@ -406,7 +412,7 @@ fail:
 
 extern "C" int nsh_ext_plan_run(nsh_ext_plan_t *p)
 {
-	return p? plan_launch(p, p->args, p->grid, false) : 0;
+	return p? plan_launch(p, p->args, p->grid, false, nullptr, p->n) : 0;
 }
 
 extern "C" int nsh_ext_plan_run_subset(nsh_ext_plan_t *p, const uint8_t *keep)
@@ -415,8 +421,11 @@ extern "C" int nsh_ext_plan_run_subset(nsh_ext_plan_t *p, const uint8_t *keep)
 	ExtArgs args[kGroups];
 	int32_t grid[kGroups] = {};
 	if (!p->d_prob_sub) HIPOK(dalloc(&p->d_prob_sub, p->n));
-	HIPOK(hipStreamSynchronize(p->stream));   // the previous subset list may still be in flight
-	p->sub_h.clear();
+	if (!p->d_idx_sub) HIPOK(dalloc(&p->d_idx_sub, p->n));
+	HIPOK(hipStreamSynchronize(p->stream));   // the previous subset lists may still be in flight
+	p->sub_h.clear(), p->idx_h.clear();
+	for (int32_t k = 0; k < p->n; ++k)
+		if (keep[k]) p->idx_h.push_back(k);
 	for (int g = 0; g < kGroups; ++g) {
 		const int32_t first = (int32_t)p->sub_h.size();
 		for (int32_t k = p->grp_first[g]; k < p->grp_first[g] + p->grp_n[g]; ++k)
@@ -425,9 +434,11 @@ extern "C" int nsh_ext_plan_run_subset(nsh_ext_plan_t *p, const uint8_t *keep)
 		args[g].prob = p->d_prob_sub + first, args[g].n = (int32_t)p->sub_h.size() - first;
 		grid[g] = std::min(args[g].n, p->ncu * (args[g].pssm? kGroupWavesPerCUPssm[g] : kGroupWavesPerCU[g]));
 	}
-	if (!p->sub_h.empty())
+	if (!p->sub_h.empty()) {
 		HIPOK(hipMemcpyAsync(p->d_prob_sub, p->sub_h.data(), p->sub_h.size() * sizeof(ExtProb), hipMemcpyHostToDevice, p->stream));
-	return plan_launch(p, args, grid, true);
+		HIPOK(hipMemcpyAsync(p->d_idx_sub, p->idx_h.data(), p->idx_h.size() * sizeof(int32_t), hipMemcpyHostToDevice, p->stream));
+	}
+	return plan_launch(p, args, grid, true, p->d_idx_sub, (int32_t)p->idx_h.size());
 fail:
 	return 0;
 }

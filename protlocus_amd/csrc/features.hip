@@ -1,12 +1,16 @@
 // Chain-candidate features for the learned prescreen (no reference counterpart).
 //
-// One work-group of 256 threads per candidate (protein aa codes, DNA window as base codes).
-// Anchors are exact amino-acid 3-mer matches between the protein and the three forward
+// One work-group of 512 threads per candidate (protein aa codes, DNA window as base codes).
+// Each string is read from HBM once, coalesced: the window becomes one residue code per base
+// position (the codon starting there) and the protein its residue codes, both staged in LDS
+// with the score matrix, so every later pass (anchors, composition, segment scores) runs from
+// LDS. Anchors are exact amino-acid 3-mer matches between the protein and the three forward
 // translation frames of the window; the protein's 3-mers are indexed in LDS by code
-// (20^3 = 8000 entries, first position), so each window codon is one LDS lookup. Counts go to
-// LDS with atomics; one thread turns them into the 64 features (layout below). Every count is
-// an integer, every feature one float expression of them, so a CPU restatement reproduces the
-// features exactly (up to the final float rounding).
+// (20^3 = 8000 entries, first position), so each window codon is one LDS lookup. Scalar counts
+// are kept per thread and summed per wave; histograms use LDS atomics; one thread turns the
+// counts into the 64 features (layout below). Every count is an integer, every feature one
+// float expression of them, so a CPU restatement reproduces the features exactly (up to the
+// final float rounding).
 //
 // Features (float32): 0 log2(1+al) | 1 log2(1+nl) | 2 nl/(3al) | 3-5 anchors per frame / al |
 // 6 anchors / al | 7-9 stop codons per frame / codons of the frame | 10 codons with N / codons |
@@ -28,6 +32,7 @@
 namespace {
 
 constexpr int NF = 64;
+constexpr int NT = 512;               // threads per candidate (eight waves)
 constexpr int KMAX = 8000;            // 20^3
 constexpr int AL_MAX = 4096, NL_MAX = 24000;
 constexpr int NBIN = (AL_MAX + NL_MAX / 3) / 8 + 2;   // diagonal bins per frame
@@ -41,7 +46,29 @@ struct FeatArgs {
 	float *out;                     // n x 64
 };
 
-__global__ void __launch_bounds__(256) feature_kernel(FeatArgs a)
+// wave-wide reductions with DPP: quad swaps, half-row and row mirrors leave every lane of a
+// 16-lane row with the row's result; the four rows are combined from lanes 0, 16, 32, 48.
+// Every lane of the wave must be active.
+template <int CTRL> static __device__ __forceinline__ int dpp(int v)
+{
+	return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+static __device__ __forceinline__ int wave_sum(int v)
+{
+	v += dpp<0xb1>(v), v += dpp<0x4e>(v), v += dpp<0x141>(v), v += dpp<0x140>(v);
+	return __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16) + __builtin_amdgcn_readlane(v, 32) +
+		__builtin_amdgcn_readlane(v, 48);
+}
+static __device__ __forceinline__ unsigned wave_umax(unsigned v)
+{
+	v = max(v, (unsigned)dpp<0xb1>((int)v)), v = max(v, (unsigned)dpp<0x4e>((int)v));
+	v = max(v, (unsigned)dpp<0x141>((int)v)), v = max(v, (unsigned)dpp<0x140>((int)v));
+	const unsigned a = __builtin_amdgcn_readlane((int)v, 0), b = __builtin_amdgcn_readlane((int)v, 16);
+	const unsigned c = __builtin_amdgcn_readlane((int)v, 32), d = __builtin_amdgcn_readlane((int)v, 48);
+	return max(max(a, b), max(c, d));
+}
+
+__global__ void __launch_bounds__(NT) feature_kernel(FeatArgs a)
 {
 	__shared__ int first[KMAX];
 	__shared__ unsigned hist[3][(NBIN + 1) / 2];   // two 16-bit bins per word
@@ -50,59 +77,86 @@ __global__ void __launch_bounds__(256) feature_kernel(FeatArgs a)
 	                          // 14-33 protein comp, 34-53 dominant comp, 54 bins>=3, 55 max bin
 	__shared__ long long dsum[2];
 	__shared__ uint8_t nt4[256], aa20[256], codon[64];
-	const int t = threadIdx.x, c = blockIdx.x;
+	__shared__ uint8_t cod[NL_MAX];     // cod[i]: residue of the codon at bases i..i+2 (21 if any is not ACGT)
+	__shared__ uint8_t pa[AL_MAX];      // protein residue codes
+	__shared__ int8_t smat[22 * 22];
+	const int t = threadIdx.x, c = blockIdx.x, lane = t & 63;
 	const nsh_prep::PrepProb P = a.pp[c];
 	const uint8_t *praw = a.as + P.as_off, *wraw = a.ns + P.ns_off;
 	const int al = P.al, nl = P.nl;
-	nt4[t] = a.pc->nt4[t], aa20[t] = a.pc->aa20[t];
+	if (t < 256) nt4[t] = a.pc->nt4[t], aa20[t] = a.pc->aa20[t];
 	if (t < 64) codon[t] = a.pc->codon[t];
-	__syncthreads();
-#define p(i) aa20[praw[i]]
-#define w(i) nt4[wraw[i]]
+	for (int i = t; i < 22 * 22; i += NT) smat[i] = a.mat[i];
 	const int ncod = nl / 3, ncodf[3] = { nl / 3, (nl - 1) / 3 > 0? (nl - 1) / 3 : 0, (nl - 2) / 3 > 0? (nl - 2) / 3 : 0 };
 	const int nb = (al + ncod) / 8 + 2;
-	for (int i = t; i < KMAX; i += 256) first[i] = INT_MAX;
-	for (int i = t; i < 3 * ((NBIN + 1) / 2); i += 256) (&hist[0][0])[i] = 0;
-	for (int i = t; i < AL_MAX / 32; i += 256) qhit[i] = 0;
-	for (int i = t; i < NL_MAX / 3 / 32 + 1; i += 256) thit[i] = 0;
+	for (int i = t; i < KMAX; i += NT) first[i] = INT_MAX;
+	for (int i = t; i < 3 * ((NBIN + 1) / 2); i += NT) (&hist[0][0])[i] = 0;
+	for (int i = t; i < AL_MAX / 32; i += NT) qhit[i] = 0;
+	for (int i = t; i < NL_MAX / 3 / 32 + 1; i += NT) thit[i] = 0;
 	if (t < 96) cnt[t] = 0;
 	if (t < 2) dsum[t] = 0;
 	__syncthreads();
-	for (int i = t; i + 2 < al; i += 256) {
-		const int x = p(i), y = p(i + 1), z = p(i + 2);
-		if (x < 20 && y < 20 && z < 20) atomicMin(&first[(x * 20 + y) * 20 + z], i);
-	}
-	for (int i = t; i < al; i += 256) if (p(i) < 20) atomicAdd(&cnt[14 + p(i)], 1);
-	for (int i = t; i < nl; i += 256) {
-		const int b = w(i);
-		if (b == 1 || b == 2) atomicAdd(&cnt[7], 1);
-		if (i + 1 < nl) {
-			if (b == 2 && w(i + 1) == 3) atomicAdd(&cnt[8], 1);
-			if (b == 0 && w(i + 1) == 2) atomicAdd(&cnt[9], 1);
+	// one coalesced pass over each string: protein codes and 3-mer index, window codon codes and
+	// base counts (per-thread counters, one LDS update per wave at the end)
+	// four strides per iteration with every load issued before the first use (the loops are
+	// bound by memory latency, not bandwidth)
+	for (int i0 = t; i0 < al; i0 += 4 * NT) {
+		int raw[4];
+#pragma unroll
+		for (int u = 0; u < 4; ++u) raw[u] = i0 + NT * u < al? praw[i0 + NT * u] : 0;
+#pragma unroll
+		for (int u = 0; u < 4; ++u) {
+			const int i = i0 + NT * u;
+			if (i < al) {
+				const int x = aa20[raw[u]];
+				pa[i] = (uint8_t)x;
+				if (x < 20) atomicAdd(&cnt[14 + x], 1);
+			}
 		}
+	}
+	int n_gc = 0, n_gt = 0, n_ag = 0;
+	for (int i0 = t; i0 < nl; i0 += 4 * NT) {
+		int raw[4][3];
+#pragma unroll
+		for (int u = 0; u < 4; ++u)
+#pragma unroll
+			for (int e = 0; e < 3; ++e) raw[u][e] = i0 + NT * u + e < nl? wraw[i0 + NT * u + e] : 'N';
+#pragma unroll
+		for (int u = 0; u < 4; ++u) {
+			const int i = i0 + NT * u;
+			if (i < nl) {
+				const int b0 = nt4[raw[u][0]], b1 = i + 1 < nl? nt4[raw[u][1]] : 4, b2 = i + 2 < nl? nt4[raw[u][2]] : 4;
+				n_gc += b0 == 1 || b0 == 2;
+				n_gt += b0 == 2 && b1 == 3;
+				n_ag += b0 == 0 && b1 == 2;
+				if (i + 2 < nl) cod[i] = b0 < 4 && b1 < 4 && b2 < 4? codon[b0 << 4 | b1 << 2 | b2] : 21;
+			}
+		}
+	}
+	__syncthreads();
+	for (int i = t; i + 2 < al; i += NT) {
+		const int x = pa[i], y = pa[i + 1], z = pa[i + 2];
+		if (x < 20 && y < 20 && z < 20) atomicMin(&first[(x * 20 + y) * 20 + z], i);
 	}
 	__syncthreads();
 	// anchors: codon j of frame f covers bases f+3j .. f+3j+2; a 3-mer spans codons j..j+2
 	long long ds = 0, ds2 = 0;
-	for (int f = 0; f < 3; ++f) {
-		for (int j = t; j < ncodf[f]; j += 256) {
-			int r[3];
+	int n_anch[3] = {}, n_stop[3] = {}, n_x = 0, n_quart[4] = {};
 #pragma unroll
-			for (int e = 0; e < 3; ++e) {
-				const int jj = j + e;
-				if (jj >= ncodf[f]) { r[e] = 21; continue; }
-				const int o = f + 3 * jj, b0 = w(o), b1 = w(o + 1), b2 = w(o + 2);
-				r[e] = b0 < 4 && b1 < 4 && b2 < 4? codon[b0 << 4 | b1 << 2 | b2] : 21;
-			}
-			if (r[0] == 20) atomicAdd(&cnt[3 + f], 1);
-			if (r[0] == 21) atomicAdd(&cnt[6], 1);
-			if (r[0] < 20 && r[1] < 20 && r[2] < 20) {
-				const int q = first[(r[0] * 20 + r[1]) * 20 + r[2]];
+	for (int f = 0; f < 3; ++f) {
+		const int nc = ncodf[f];
+		for (int j = t; j < nc; j += NT) {
+			const int r0 = cod[f + 3 * j], r1 = j + 1 < nc? cod[f + 3 * j + 3] : 21, r2 = j + 2 < nc? cod[f + 3 * j + 6] : 21;
+			n_stop[f] += r0 == 20;
+			n_x += r0 == 21;
+			if (r0 < 20 && r1 < 20 && r2 < 20) {
+				const int q = first[(r0 * 20 + r1) * 20 + r2];
 				if (q != INT_MAX) {
-					atomicAdd(&cnt[f], 1);
+					++n_anch[f];
 					atomicOr(&qhit[q >> 5], 1u << (q & 31));
 					atomicOr(&thit[j >> 5], 1u << (j & 31));
-					atomicAdd(&cnt[10 + (4 * q) / al], 1);
+					const int qt = (4 * q) / al;
+					n_quart[0] += qt == 0, n_quart[1] += qt == 1, n_quart[2] += qt == 2, n_quart[3] += qt == 3;
 					const int d = j - q, bin = (d + al) >> 3;
 					atomicAdd(&hist[f][bin >> 1], 1u << ((bin & 1) * 16));
 					ds += d, ds2 += (long long)d * d;
@@ -110,71 +164,94 @@ __global__ void __launch_bounds__(256) feature_kernel(FeatArgs a)
 			}
 		}
 	}
-	atomicAdd((unsigned long long*)&dsum[0], (unsigned long long)ds);
-	atomicAdd((unsigned long long*)&dsum[1], (unsigned long long)ds2);
+	{
+		// per thread every count is at most 48 (16 codons in each of three frames, or 47 bases), so a
+		// wave's sum fits 16 bits: two counts per 32-bit sum. Per thread |ds| < 2^19 and ds2 < 2^32,
+		// summed as its 16-bit low and high parts.
+		const int v[14] = { n_anch[0], n_anch[1], n_anch[2], n_stop[0], n_stop[1], n_stop[2], n_x, n_gc, n_gt, n_ag,
+			n_quart[0], n_quart[1], n_quart[2], n_quart[3] };
+#pragma unroll
+		for (int k = 0; k < 14; k += 2) {
+			const int sv = wave_sum(v[k] | v[k + 1] << 16);
+			if (lane == 0) atomicAdd(&cnt[k], sv & 0xffff), atomicAdd(&cnt[k + 1], (int)((unsigned)sv >> 16));
+		}
+		const int s1 = wave_sum((int)ds), s2l = wave_sum((int)(ds2 & 0xffff)), s2h = wave_sum((int)(ds2 >> 16));
+		if (lane == 0) {
+			atomicAdd((unsigned long long*)&dsum[0], (unsigned long long)(long long)s1);
+			atomicAdd((unsigned long long*)&dsum[1], (unsigned long long)((long long)s2h << 16) + (unsigned long long)s2l);
+		}
+	}
 	__syncthreads();
 	const int dom = cnt[0] >= cnt[1] && cnt[0] >= cnt[2]? 0 : (cnt[1] >= cnt[2]? 1 : 2);
-	for (int j = t; j < ncodf[dom]; j += 256) {
-		const int o = dom + 3 * j, b0 = w(o), b1 = w(o + 1), b2 = w(o + 2);
-		const int r = b0 < 4 && b1 < 4 && b2 < 4? codon[b0 << 4 | b1 << 2 | b2] : 21;
+	for (int j = t; j < ncodf[dom]; j += NT) {
+		const int r = cod[dom + 3 * j];
 		if (r < 20) atomicAdd(&cnt[34 + r], 1);
 	}
-	for (int i = t; i < 3 * nb; i += 256) {
-		const int f = i / nb, bin = i % nb;
-		const int v = (hist[f][bin >> 1] >> ((bin & 1) * 16)) & 0xffff;
-		if (v >= 3) atomicAdd(&cnt[54], 1);
-		atomicMax(&cnt[55], v);
-	}
+	// bin statistics and the four fullest bins. A bin's key is count << 16 | (65535 - frame * nb - bin):
+	// keys are distinct, and the largest key is the fullest bin with the smallest index. Round r
+	// takes the largest key below round r-1's, so no bin is cleared and one barrier ends a round.
+	int n3 = 0;
+	unsigned vmax = 0;
 	int hq = 0, ht = 0;
-	for (int i = t; i < AL_MAX / 32; i += 256) hq += __popc(qhit[i]);
-	for (int i = t; i < NL_MAX / 3 / 32 + 1; i += 256) ht += __popc(thit[i]);
-	__syncthreads();
-	atomicAdd(&cnt[56], hq);
-	atomicAdd(&cnt[57], ht);
-	// the four fullest diagonal bins (ties: smaller frame * nb + bin), one at a time
-	__shared__ int topbin[4], redv[4], redi[4], segsc[4][8];
+	for (int i = t; i < AL_MAX / 32; i += NT) hq += __popc(qhit[i]);
+	for (int i = t; i < NL_MAX / 3 / 32 + 1; i += NT) ht += __popc(thit[i]);
+	__shared__ unsigned redk[4][NT / 64];
+	__shared__ int topbin[4], segsc[4][8];
+	unsigned prev = 0xffffffffu;
 	for (int r = 0; r < 4; ++r) {
-		int bv = 0, bi = INT_MAX;
-		for (int i = t; i < 3 * nb; i += 256) {
-			const int f = i / nb, bin = i % nb;
-			const int v = (hist[f][bin >> 1] >> ((bin & 1) * 16)) & 0xffff;
-			if (v > bv) bv = v, bi = i;
-		}
-#pragma unroll
-		for (int d = 32; d >= 1; d >>= 1) {
-			const int ov = __shfl_xor(bv, d), oi = __shfl_xor(bi, d);
-			if (ov > bv || (ov == bv && oi < bi)) bv = ov, bi = oi;
-		}
-		if ((t & 63) == 0) redv[t >> 6] = bv, redi[t >> 6] = bi;
-		__syncthreads();
-		if (t == 0) {
-			int v = 0, ix = INT_MAX;
-			for (int k = 0; k < 4; ++k)
-				if (redv[k] > v || (redv[k] == v && redi[k] < ix)) v = redv[k], ix = redi[k];
-			topbin[r] = v > 0? ix : -1;
-			if (v > 0) {
-				const int f = ix / nb, bin = ix % nb;
-				hist[f][bin >> 1] &= ~(0xffffu << ((bin & 1) * 16));
+		unsigned best = 0;
+		for (int f = 0; f < 3; ++f)
+			for (int bin = t; bin < nb; bin += NT) {
+				const unsigned v = (hist[f][bin >> 1] >> ((bin & 1) * 16)) & 0xffff;
+				const unsigned key = v << 16 | (unsigned)(65535 - (f * nb + bin));
+				if (r == 0) n3 += v >= 3, vmax = max(vmax, v);
+				if (v > 0 && key < prev) best = max(best, key);
 			}
+		best = wave_umax(best);
+		if (lane == 0) redk[r][t >> 6] = best;
+		if (r == 0) {
+			const int s3 = wave_sum(n3), sq = wave_sum(hq), st = wave_sum(ht);
+			const unsigned m = wave_umax(vmax);
+			if (lane == 0) atomicAdd(&cnt[54], s3), atomicMax(&cnt[55], (int)m), atomicAdd(&cnt[56], sq), atomicAdd(&cnt[57], st);
 		}
 		__syncthreads();
+		best = 0;
+#pragma unroll
+		for (int w = 0; w < NT / 64; ++w) best = max(best, redk[r][w]);
+		if (t == 0) topbin[r] = best? 65535 - (int)(best & 0xffff) : -1;
+		prev = best? best : 0;
 	}
+	__syncthreads();
 	// best ungapped segment (max subarray sum, empty allowed) on each of a bin's 8 diagonals: every
-	// thread takes one eighth of one diagonal of one bin (4 x 8 x 8 = 256), summarises its chunk
-	// (total, best prefix, best suffix, best inside) and 32 threads chain the eight summaries
-	__shared__ int csum[4][8][8][4];
+	// thread takes one of NCH chunks of one diagonal of one bin (4 x 8 x NCH = NT), summarises it
+	// (total, best prefix, best suffix, best inside) and 32 threads chain the NCH summaries
+	constexpr int NCH = NT / 32;
+	__shared__ int csum[4][8][NCH][4];
 	{
-		const int r = t >> 6, dd = (t >> 3) & 7, ch = t & 7, tb = topbin[r];
+		const int r = t / (8 * NCH), dd = (t / NCH) & 7, ch = t % NCH, tb = topbin[r];
 		int tot = 0, pre = 0, suf = 0, ins = 0;
 		if (tb >= 0) {
 			const int f = tb / nb, d = (tb % nb) * 8 - al + dd;
 			const int q0 = d < 0? -d : 0, q1 = min(al, ncodf[f] - d), len = q1 > q0? q1 - q0 : 0;
-			const int sz = (len + 7) / 8, qa = q0 + ch * sz, qb = min(q1, qa + sz);
-			int run = 0;
-			for (int q = qa; q < qb; ++q) {
-				const int o = f + 3 * (q + d), b0 = w(o), b1 = w(o + 1), b2 = w(o + 2);
-				const int rr = b0 < 4 && b1 < 4 && b2 < 4? codon[b0 << 4 | b1 << 2 | b2] : 21;
-				const int v = a.mat[rr * 22 + p(q)];
+			const int sz = (len + NCH - 1) / NCH, qa = q0 + ch * sz, qb = min(q1, qa + sz);
+			const int cb = f + 3 * d;   // cod[cb + 3q]: the window residue facing protein position q
+			int run = 0, q = qa;
+			for (; q + 4 <= qb; q += 4) {          // four positions per step, all loads first
+				int cc[4], pp[4], v[4];
+#pragma unroll
+				for (int u = 0; u < 4; ++u) cc[u] = cod[cb + 3 * (q + u)], pp[u] = pa[q + u];
+#pragma unroll
+				for (int u = 0; u < 4; ++u) v[u] = smat[cc[u] * 22 + pp[u]];
+#pragma unroll
+				for (int u = 0; u < 4; ++u) {
+					tot += v[u];
+					pre = max(pre, tot);
+					run = max(0, run + v[u]);
+					ins = max(ins, run);
+				}
+			}
+			for (; q < qb; ++q) {
+				const int v = smat[cod[cb + 3 * q] * 22 + pa[q]];
 				tot += v;
 				pre = max(pre, tot);
 				run = max(0, run + v);
@@ -188,7 +265,7 @@ __global__ void __launch_bounds__(256) feature_kernel(FeatArgs a)
 	if (t < 32) {
 		const int r = t >> 3, dd = t & 7;
 		int best = 0, s = 0;   // s: best sum ending at the previous chunk's end (>= 0)
-		for (int ch = 0; ch < 8; ++ch) {
+		for (int ch = 0; ch < NCH; ++ch) {
 			const int *cs = csum[r][dd][ch];
 			best = max(best, max(cs[3], s + cs[1]));
 			s = max(cs[2], s + cs[0]);
@@ -197,46 +274,57 @@ __global__ void __launch_bounds__(256) feature_kernel(FeatArgs a)
 		segsc[r][dd] = best;
 	}
 	__syncthreads();
-	if (t == 0) {
-		float *o = a.out + (int64_t)c * NF;
+	// the 64 features, one per lane of the first wave: numerator and denominator are exact
+	// integers in float, one division per lane
+	if (t < 64) {
+		const int f = t;
 		const int nanch = cnt[0] + cnt[1] + cnt[2];
 		const float fal = (float)al, fnl = (float)nl;
-		o[0] = log2f(1.f + fal), o[1] = log2f(1.f + fnl), o[2] = fnl / (3.f * fal);
-		for (int f = 0; f < 3; ++f) o[3 + f] = (float)cnt[f] / fal;
-		o[6] = (float)nanch / fal;
-		for (int f = 0; f < 3; ++f) o[7 + f] = (float)cnt[3 + f] / (float)(ncodf[f] > 0? ncodf[f] : 1);
-		const int tot = ncodf[0] + ncodf[1] + ncodf[2];
-		o[10] = (float)cnt[6] / (float)(tot > 0? tot : 1);
-		o[11] = (float)cnt[7] / (nl > 0? fnl : 1.f);
-		o[12] = 16.f * (float)cnt[8] / (nl > 0? fnl : 1.f);
-		o[13] = 16.f * (float)cnt[9] / (nl > 0? fnl : 1.f);
-		o[14] = (float)cnt[dom] / (float)(nanch > 0? nanch : 1);
-		o[15] = (float)cnt[55] / fal;
-		o[16] = (float)cnt[54] / 10.f;
-		o[17] = (float)cnt[56] / fal;
-		o[18] = (float)cnt[57] / (float)(ncod > 0? ncod : 1);
-		double sd = 0.;
-		if (nanch > 0) {
-			const double m = (double)dsum[0] / nanch, m2 = (double)dsum[1] / nanch;
-			sd = m2 - m * m > 0.? sqrt(m2 - m * m) : 0.;
-		}
-		o[19] = (float)(sd / (double)al);
-		for (int r = 0; r < 20; ++r) o[20 + r] = (float)cnt[14 + r] / fal;
 		int sg[4];
+#pragma unroll
 		for (int r = 0; r < 4; ++r) {
 			sg[r] = 0;
 			for (int k = 0; k < 8; ++k) sg[r] = max(sg[r], segsc[r][k]);
 		}
+#pragma unroll
 		for (int x = 1; x < 4; ++x)   // descending
-			for (int y = x; y > 0 && sg[y] > sg[y - 1]; --y) { const int tmp = sg[y]; sg[y] = sg[y - 1], sg[y - 1] = tmp; }
-		for (int r = 0; r < 4; ++r) o[40 + r] = (float)sg[r] / fal;
-		o[44] = (float)(sg[0] + sg[1] + sg[2] + sg[3]) / fal;
-		o[45] = (float)((sg[0] >= 20) + (sg[1] >= 20) + (sg[2] >= 20) + (sg[3] >= 20)) / 4.f;
-		for (int r = 0; r < 14; ++r) o[46 + r] = (float)cnt[34 + r] / (float)(ncodf[dom] > 0? ncodf[dom] : 1);
-		for (int k = 0; k < 4; ++k) o[60 + k] = (float)cnt[10 + k] / fal;
+#pragma unroll
+			for (int y = x; y > 0; --y)
+				if (sg[y] > sg[y - 1]) { const int tmp = sg[y]; sg[y] = sg[y - 1], sg[y - 1] = tmp; }
+		const int tot = ncodf[0] + ncodf[1] + ncodf[2];
+		const float nl1 = nl > 0? fnl : 1.f;
+		float num = 0.f, den = fal;
+		if (f == 2) num = fnl, den = 3.f * fal;
+		else if (f >= 3 && f < 6) num = (float)cnt[f - 3];
+		else if (f == 6) num = (float)nanch;
+		else if (f >= 7 && f < 10) num = (float)cnt[f - 4], den = (float)(ncodf[f - 7] > 0? ncodf[f - 7] : 1);
+		else if (f == 10) num = (float)cnt[6], den = (float)(tot > 0? tot : 1);
+		else if (f == 11) num = (float)cnt[7], den = nl1;
+		else if (f == 12 || f == 13) num = 16.f * (float)cnt[f - 4], den = nl1;
+		else if (f == 14) num = (float)cnt[dom], den = (float)(nanch > 0? nanch : 1);
+		else if (f == 15) num = (float)cnt[55];
+		else if (f == 16) num = (float)cnt[54], den = 10.f;
+		else if (f == 17) num = (float)cnt[56];
+		else if (f == 18) num = (float)cnt[57], den = (float)(ncod > 0? ncod : 1);
+		else if (f >= 20 && f < 40) num = (float)cnt[f - 6];
+		else if (f >= 40 && f < 44) num = (float)sg[0] * (f == 40) + (float)sg[1] * (f == 41) + (float)sg[2] * (f == 42) + (float)sg[3] * (f == 43);
+		else if (f == 44) num = (float)(sg[0] + sg[1] + sg[2] + sg[3]);
+		else if (f == 45) num = (float)((sg[0] >= 20) + (sg[1] >= 20) + (sg[2] >= 20) + (sg[3] >= 20)), den = 4.f;
+		else if (f >= 46 && f < 60) num = (float)cnt[f - 12], den = (float)(ncodf[dom] > 0? ncodf[dom] : 1);
+		else if (f >= 60) num = (float)cnt[f - 50];
+		float v = num / den;
+		const float lg = log2f(1.f + (f == 0? fal : fnl));
+		if (f < 2) v = lg;
+		if (f == 19) {
+			double sd = 0.;
+			if (nanch > 0) {
+				const double m = (double)dsum[0] / nanch, m2 = (double)dsum[1] / nanch;
+				sd = m2 - m * m > 0.? sqrt(m2 - m * m) : 0.;
+			}
+			v = (float)(sd / (double)al);
+		}
+		a.out[(int64_t)c * NF + f] = v;
 	}
-#undef p
-#undef w
 }
 
 } // namespace
@@ -246,7 +334,7 @@ void nsh_feat::launch(const nsh_prep::PrepProb *pp, int32_t n, const uint8_t *ns
 {
 	FeatArgs fa;
 	fa.pp = pp, fa.ns = ns, fa.as = as, fa.pc = pc, fa.mat = mat, fa.n = n, fa.out = out;
-	hipLaunchKernelGGL(feature_kernel, dim3(n), dim3(256), 0, s, fa);
+	hipLaunchKernelGGL(feature_kernel, dim3(n), dim3(NT), 0, s, fa);
 }
 
 #define FOK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { nsh_set_error("features: %s: %s", #x, hipGetErrorString(e_)); goto fail; } } while (0)
