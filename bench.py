@@ -36,6 +36,11 @@ MFMA_BF16_TFLOPS = 2500.0                         # dense BF16 MFMA
 OPS_PER_CELL = 32                                 # extension DP: 31 recurrence ops + row max
 HMM_OPS_PER_CELL = 17                             # Viterbi: 8 adds, 4 max, 3 clamps, 2 running-best max
 MLP_FLOP_PER_SAMPLE = 2 * (64 * 256 + 256 * 256 + 256)
+# HBM bytes moved by one DP phase (both column-group kernels) of the default workload (65,536
+# pairs, seed 1): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of one step, each scaled by what the
+# counter reads for a 1-GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); the summary and the
+# commands are in profiles/ext_dp_kernel_65536pairs.md
+EXT_DP_TRAFFIC_BYTES = 889.9e6
 
 
 def threads():
@@ -171,7 +176,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X; the DP problems are "
-              "independent, so N GPUs would shard pairs by cost (see DESIGN.md) - run with --gpus 1", file=sys.stderr)
+              "independent, so N GPUs would shard pairs by cost (longest-processing-time bins) - run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     log = lambda s: print(s, file=sys.stderr, flush=True)
 
@@ -221,7 +226,8 @@ def main():
         "window_gcups": round(win / (ms * 1e-3) / 1e9, 2),
         "dp_kernel_ms": round(dp_avg, 3),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
-                     "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+                     "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
+                     "traffic": EXT_DP_TRAFFIC_BYTES if (args.pairs, args.seed) == (65536, 1) else None,
                      "ops_per_cell": OPS_PER_CELL},
     }
     log("[bench] gpu: %.2f GCUPS, %.1f ms/step, dp %.1f ms" % (gcups, ms, dp_avg))

@@ -252,6 +252,57 @@ void nsh_mlp_destroy(nsh_mlp_t *h);
 int nsh_features(int32_t n, const char *as, const int64_t *as_off, const int32_t *al, int64_t l_as,
 		const char *ns, const int64_t *ns_off, const int32_t *nl, int64_t l_ns, float *out);
 
+/* ======================================================================================
+ * Alignment output: statistics, CDS features and the PAF / GFF3 / GTF records of one
+ * alignment, byte-identical to the reference's writers. Host code.
+ * ====================================================================================== */
+typedef struct {                /* one alignment: mp_reg1_t with its mp_extra_t (protlocus.h:205-254) */
+	const char *qname, *aa;     /* protein name and sequence (ASCII) */
+	int32_t qlen, qs, qe;       /* protein length, aligned interval [qs, qe) */
+	const char *ctg_name;       /* contig name and length */
+	int64_t ctg_len;
+	int32_t rev;                /* strand: 0 forward, 1 reverse (mp_reg1_t.vid & 1) */
+	int64_t vs, ve;             /* aligned target interval in oriented coordinates */
+	int32_t dp_score;           /* DP score (mp_extra_t.dp_score) */
+	int32_t profile_score;      /* HMM Viterbi score, INT32_MIN when none (PAF ps:i:) */
+	int32_t rank_sc;            /* rerank score (PAF ra:i: with MP_F_ACCURACY and MP_ACC_RERANK) */
+	int32_t chn_sc, chn_sc_ungap, cnt;   /* chain fields, printed when the details were dropped */
+	const uint32_t *cigar;      /* len << 4 | op, as ns_rst_t.cigar */
+	int32_t n_cigar;
+} nsh_aln_t;
+
+typedef struct {                /* what mp_extra_stop / mp_extra_start / mp_extra_cal fill into mp_extra_t */
+	int32_t dp_max, blen, n_fs, n_stop, n_iden, n_plus, dist_stop, dist_start;
+	int32_t n_exon, n_feat;
+} nsh_aln_stats_t;
+
+typedef struct {                /* one CDS (type 0) or stop_codon (type 1) feature; layout of mp_feat_t (protlocus.h:230-237) */
+	int64_t vs, ve;
+	int32_t qs, qe;
+	int16_t type, phase;
+	int32_t n_fs, n_stop;
+	int32_t score, n_iden, blen;
+	char donor[2], acceptor[2];
+} nsh_feat_t;
+
+/* Statistics and features of an alignment: mp_extra_stop, mp_extra_start and mp_extra_cal
+ * (align.c:148-306) as align.c:608-612 calls them. nt: oriented target bases (codes 0..4) of
+ * [as, ae), covering [vs, ve), searched for the nearest stop and start codons. pssm: the
+ * protein's rows (qlen x opt->asize, HMM mode) or NULL to score with opt->sc. Returns the number
+ * of features (>= 1); 0 when the CIGAR does not span [qs, qe) x [vs, ve), where the reference
+ * drops the alignment's details (align.c:258-268); -1 on bad arguments or m_feat too small. */
+int32_t nsh_aln_stats(const nsh_aln_t *a, const uint8_t *nt, int64_t as, int64_t ae, const ns_opt_t *opt,
+		const int8_t *pssm, nsh_aln_stats_t *st, nsh_feat_t *feat, int32_t m_feat);
+/* The records mp_write_output (format.c:457-477) writes for the alignment: flag and acc_flags
+ * are mp_mapopt_t.flag and .acc_flags (MP_F_GFF 0x8, MP_F_NO_PAF 0x10, MP_F_GTF 0x20,
+ * MP_F_NO_CS 0x200, MP_F_ACCURACY 0x400; MP_ACC_RERANK 0x8), gff_prefix (NULL: "PL"), id and
+ * gff_delim as the reference passes them, hit_idx the hit's rank. st and feat from
+ * nsh_aln_stats, st == NULL when it returned 0. Writes at most size - 1 bytes and a NUL; returns
+ * the full length, or -1 (MP_F_SHOW_RESIDUE and MP_F_SHOW_TRANS are not supported). */
+int64_t nsh_write_output(char *out, int64_t size, const nsh_aln_t *a, const nsh_aln_stats_t *st,
+		const nsh_feat_t *feat, const uint8_t *nt, int64_t as, int32_t flag, int32_t acc_flags,
+		const char *gff_prefix, int64_t id, int32_t gff_delim, int32_t hit_idx);
+
 int nsh_device_synchronize(void);
 const char *nsh_build_info(void);
 

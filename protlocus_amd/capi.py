@@ -27,6 +27,30 @@ class NsRst(C.Structure):
                 ("score", C.c_int32), ("cigar", C.POINTER(C.c_uint32))]
 
 
+class Aln(C.Structure):
+    """nsh_aln_t: one alignment as the output writers see it"""
+    _fields_ = [("qname", C.c_char_p), ("aa", C.c_char_p), ("qlen", C.c_int32), ("qs", C.c_int32), ("qe", C.c_int32),
+                ("ctg_name", C.c_char_p), ("ctg_len", C.c_int64), ("rev", C.c_int32), ("vs", C.c_int64),
+                ("ve", C.c_int64), ("dp_score", C.c_int32), ("profile_score", C.c_int32), ("rank_sc", C.c_int32),
+                ("chn_sc", C.c_int32), ("chn_sc_ungap", C.c_int32), ("cnt", C.c_int32),
+                ("cigar", C.c_void_p), ("n_cigar", C.c_int32)]
+
+
+class AlnStats(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("dp_max", "blen", "n_fs", "n_stop", "n_iden", "n_plus", "dist_stop",
+                                          "dist_start", "n_exon", "n_feat")]
+
+
+class Feat(C.Structure):
+    _fields_ = [("vs", C.c_int64), ("ve", C.c_int64), ("qs", C.c_int32), ("qe", C.c_int32), ("type", C.c_int16),
+                ("phase", C.c_int16), ("n_fs", C.c_int32), ("n_stop", C.c_int32), ("score", C.c_int32),
+                ("n_iden", C.c_int32), ("blen", C.c_int32), ("donor", C.c_char * 2), ("acceptor", C.c_char * 2)]
+
+
+MP_F_GFF, MP_F_NO_PAF, MP_F_GTF, MP_F_NO_CS, MP_F_ACCURACY, MP_ACC_RERANK = 0x8, 0x10, 0x20, 0x200, 0x400, 0x8
+INT32_MIN = -2 ** 31
+
+
 class ExtResult(C.Structure):
     _fields_ = [("nt_len", C.c_int32), ("aa_len", C.c_int32), ("score", C.c_int32)]
 
@@ -78,6 +102,12 @@ def lib():
         L.nsh_hmm_last_ms.argtypes = [C.c_void_p]
         L.nsh_hmm_last_ms.restype = C.c_float
         L.nsh_hmm_destroy.argtypes = [C.c_void_p]
+        L.nsh_aln_stats.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_int32]
+        L.nsh_aln_stats.restype = C.c_int32
+        L.nsh_write_output.argtypes = [C.c_char_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_int64, C.c_int32, C.c_int32, C.c_char_p, C.c_int64, C.c_int32, C.c_int32]
+        L.nsh_write_output.restype = C.c_int64
         L.nsh_device_synchronize.restype = C.c_int
         L.nsh_build_info.restype = C.c_char_p
         if L.ns_make_tables(1) != 0:
@@ -259,3 +289,33 @@ class HMM:
         if self.h:
             lib().nsh_hmm_destroy(self.h)
             self.h = None
+
+
+def alignment_output(qname, aa, qs, qe, ctg_name, ctg_len, rev, vs, ve, cigar, dp_score, nt, as_, o,
+                     pssm_row=None, profile_score=INT32_MIN, rank_sc=0, chn_sc=0, flag=0, acc_flags=0,
+                     gff_prefix=b"PL", id=0, gff_delim=-1, hit_idx=0):
+    """statistics, features and records of one alignment (nsh_aln_stats + nsh_write_output).
+    nt: oriented target base codes (0..4) of [as_, ae); cigar: sequence of len << 4 | op.
+    Returns (n_feat, AlnStats or None, [Feat], text bytes)."""
+    L = lib()
+    cig = np.ascontiguousarray(cigar, dtype=np.uint32)
+    ntb = np.ascontiguousarray(nt, dtype=np.uint8)
+    a = Aln(qname=qname, aa=aa, qlen=len(aa), qs=qs, qe=qe, ctg_name=ctg_name, ctg_len=ctg_len, rev=rev, vs=vs, ve=ve,
+            dp_score=dp_score, profile_score=profile_score, rank_sc=rank_sc, chn_sc=chn_sc, chn_sc_ungap=0, cnt=0,
+            cigar=cig.ctypes.data if len(cig) else None, n_cigar=len(cig))
+    st = AlnStats()
+    feats = (Feat * 64)()
+    prow = None if pssm_row is None else np.ascontiguousarray(pssm_row, dtype=np.int8)
+    nf = L.nsh_aln_stats(C.byref(a), ntb.ctypes.data, as_, as_ + len(ntb), C.byref(o),
+                         None if prow is None else prow.ctypes.data, C.byref(st), feats, 64)
+    if nf < 0:
+        raise RuntimeError("nsh_aln_stats: " + error())
+    stp = C.byref(st) if nf > 0 else None
+    n = L.nsh_write_output(None, 0, C.byref(a), stp, feats, ntb.ctypes.data, as_, flag, acc_flags, gff_prefix, id,
+                           gff_delim, hit_idx)
+    if n < 0:
+        raise RuntimeError("nsh_write_output: " + error())
+    buf = C.create_string_buffer(n + 1)
+    L.nsh_write_output(buf, n + 1, C.byref(a), stp, feats, ntb.ctypes.data, as_, flag, acc_flags, gff_prefix, id,
+                       gff_delim, hit_idx)
+    return nf, (st if nf > 0 else None), list(feats[:max(nf, 0)]), buf.raw[:n]

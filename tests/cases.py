@@ -114,3 +114,57 @@ def gapless_batch(rng, n_cand, codon, kmer, bw, genome_len=60000, n_contigs=3):
         queries += q.tobytes()
         cands.append(c)
     return g, bytes(queries), cands, anchors
+
+
+CIGAR_OPS = "MIDNSHP=XBFGUVE"
+NT4 = {ord(c): i for i, c in enumerate("ACGT")}
+NT4.update({ord(c): i for i, c in enumerate("acgt")})
+
+
+def parse_cigar(s):
+    """'12M3N4M' -> [len << 4 | op]"""
+    import re
+    return [int(n) << 4 | CIGAR_OPS.index(op) for n, op in re.findall(r"(\d+)([A-Z=])", s)]
+
+
+def nt4_codes(dna):
+    return np.array([NT4.get(b, 4) for b in dna], dtype=np.uint8)
+
+
+def output_case(rng, k):
+    """one alignment for the output writers: a protein (a slice [qs, qe) of it planted as a
+    spliced, mutated gene), a contig holding the gene between random flanks on strand `rev`,
+    the DP window [vs, ve) in oriented coordinates and a surrounding search region [as_, ae)."""
+    from protlocus_amd import synth
+    qlen = int(rng.integers(12, 360))
+    prot = synth.rand_protein(rng, qlen)
+    qs = 0 if k % 3 else int(rng.integers(0, qlen // 4))
+    qe = qlen if k % 4 else int(rng.integers(qs + (qlen - qs) // 2, qlen + 1))
+    qe = max(qe, qs + 4)
+    gene = synth.gene_of(rng, prot[qs:qe], float(rng.uniform(0.0, 0.35)), int(rng.integers(0, 4)),
+                         lambda: int(rng.integers(30, 300)), int(rng.integers(0, 2)), 0)
+    stop = [b"TAA", b"TAG", b"TGA"][k % 3] if k % 2 == 0 else b""
+    left = synth.rand_dna(rng, int(rng.integers(3, 240)))
+    right = synth.rand_dna(rng, int(rng.integers(6, 240)))
+    if k % 5 == 0:   # a few N bases in the flanks and the gene
+        gene = bytearray(gene)
+        for p in rng.integers(0, len(gene), size=2):
+            gene[p] = ord("N")
+        gene = bytes(gene)
+        left = b"N" + left[1:]
+    if k % 6 == 0:   # lower-case protein letters (upper-cased in cs:Z:)
+        prot = prot.lower()
+    ori = left + gene + stop + right
+    vs, ve = len(left), len(left) + len(gene)
+    as_ = int(rng.integers(0, vs + 1)) if k % 4 else 0
+    ae = int(rng.integers(ve, len(ori) + 1)) if k % 4 else len(ori)
+    rev = k % 2
+    comp = bytes.maketrans(b"ACGTNacgtn", b"TGCANtgcan")
+    contig = ori[::-1].translate(comp) if rev else ori
+    return dict(prot=prot, qs=qs, qe=qe, contig=contig, rev=rev, vs=vs, ve=ve, as_=as_, ae=ae,
+                window=ori[vs:ve], oriented=ori)
+
+
+# (flag, acc_flags, gff_delim) variants the output tests run on every case
+OUTPUT_VARIANTS = [(0, 0, -1), (0x200, 0, -1), (0x8, 0, -1), (0x8 | 0x10, 0, ord("|")), (0x20, 0, -1),
+                   (0x400, 0x8, -1)]

@@ -8,6 +8,7 @@ reference (oracle/_ref/libnsref.so, built from the reference's own C sources by
 Outputs (inputs and expected outputs only):
   dp_cases.json.gz   ns_global_gs16b / gs32b / gs_auto: extension, CIGAR, score-only, PSSM
   hmm_cases.json.gz  mp_hmm_read1 + mp_hmm_dp_score on synthetic HMMER3 profiles
+  output_cases.json.gz  mp_extra_* (align.c) + mp_write_output (format.c): PAF / GFF3 / GTF records
   tables.json        ns_tab_nt4 / ns_tab_aa20 / ns_tab_codon / ns_mat_blosum62
 """
 import base64
@@ -25,7 +26,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [os.path.join(ROOT, "tests"), ROOT]
 
 import nsffi                                       # noqa: E402
-from cases import planted_case, random_opt_kw, random_ss   # noqa: E402
+from cases import (OUTPUT_VARIANTS, nt4_codes, output_case, parse_cigar, planted_case,   # noqa: E402
+                   random_opt_kw, random_ss)
 from nsffi import NS_F_CIGAR, NS_F_EXT_LEFT, NS_F_EXT_RIGHT  # noqa: E402
 from protlocus_amd import synth                    # noqa: E402
 
@@ -115,6 +117,33 @@ def _nt4(codons):
     return lut[np.frombuffer(codons, dtype=np.uint8)]
 
 
+def output_cases(ref):
+    """alignments through the reference's mp_extra_* + mp_write_output (oracle/aln_ref.c) under
+    every flag variant of OUTPUT_VARIANTS; CIGARs from ns_global_gs16b(NS_F_CIGAR)"""
+    rng = np.random.default_rng(20260303)
+    o = ref.opt()
+    ro = ref.opt(flag=NS_F_CIGAR)
+    out = []
+    for k in range(48):
+        c = output_case(rng, k)
+        nl, al, score, cg = ref.run("ns_global_gs16b", c["window"], c["prot"][c["qs"]:c["qe"]], ro)
+        assert (nl, al) == (len(c["window"]), c["qe"] - c["qs"])
+        cigar = parse_cigar(cg)
+        ve_aln = c["ve"] + (3 if k % 11 == 5 else 0)      # a span mismatch drops the details
+        pssm = rng.integers(-12, 9, size=(len(c["prot"]) - c["qs"], 22)).astype(np.int8) if k % 7 == 3 else None
+        extra = dict(profile_score=int(rng.integers(-50, 400)) if k % 3 == 0 else -2 ** 31,
+                     rank_sc=int(rng.integers(0, 999)), chn_sc=int(rng.integers(0, 999)), id=k, hit_idx=k % 4)
+        g = nt4_codes(c["contig"])
+        exp = [B64(ref.output(b"chr1", g, b"q1", c["prot"], c["rev"], c["qs"], c["qe"], c["vs"], ve_aln, c["as_"],
+                              c["ae"], cigar, score, o, pssm_row=pssm, flag=f, acc_flags=a, gff_delim=d, **extra))
+               for f, a, d in OUTPUT_VARIANTS]
+        out.append({"k": k, "prot": B64(c["prot"]), "contig": B64(c["contig"]), "oriented": B64(c["oriented"]),
+                    "window": B64(c["window"]), "qs": c["qs"], "qe": c["qe"], "rev": c["rev"], "vs": c["vs"],
+                    "ve": c["ve"], "ve_aln": ve_aln, "as_": c["as_"], "ae": c["ae"], "cigar": cigar, "score": score,
+                    "pssm": None if pssm is None else pssm.tolist(), "extra": extra, "expected": exp})
+    return out
+
+
 def tables(ref):
     L = ref.lib
     get = lambda name, n, t=C.c_uint8: list((t * n).in_dll(L, name))
@@ -128,6 +157,8 @@ def main():
     ref = nsffi.Reference()
     with gzip.open(os.path.join(HERE, "dp_cases.json.gz"), "wt") as f:
         json.dump(dp_cases(ref), f, separators=(",", ":"))
+    with gzip.open(os.path.join(HERE, "output_cases.json.gz"), "wt") as f:
+        json.dump(output_cases(ref), f, separators=(",", ":"))
     with gzip.open(os.path.join(HERE, "hmm_cases.json.gz"), "wt") as f:
         json.dump(hmm_cases(ref), f, separators=(",", ":"))
     with open(os.path.join(HERE, "tables.json"), "w") as f:

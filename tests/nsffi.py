@@ -201,6 +201,12 @@ class Reference(_Lib):
         L.mp_hmm_transitions.restype = C.POINTER(C.c_int16)
         L.mp_hmm_pssm.argtypes = [C.c_void_p]
         L.mp_hmm_pssm.restype = C.POINTER(C.c_int8)
+        L.or_ref_output.argtypes = [C.c_char_p, C.c_void_p, C.c_int64, C.c_char_p, C.c_char_p, C.c_int32, C.c_int32,
+                                    C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_void_p,
+                                    C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                    C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, C.c_char_p, C.c_int64,
+                                    C.c_int32, C.c_int32, C.c_char_p, C.c_int64]
+        L.or_ref_output.restype = C.c_int64
         assert L.ns_make_tables(1) == 0
         self._opt_init = L.ns_opt_init
         self._opt_set_sp = L.ns_opt_set_sp
@@ -219,6 +225,24 @@ class Reference(_Lib):
 
     def stripe_width(self, opt, al):
         return self.lib.ns_ext_stripe_width(C.byref(opt), al)
+
+    def output(self, ctg_name, ctg_nt4, qname, aa, rev, qs, qe, vs, ve, as_, ae, cigar, score, o, pssm_row=None,
+               profile_score=-2 ** 31, rank_sc=0, chn_sc=0, flag=0, acc_flags=0, gff_prefix=b"PL", id=0,
+               gff_delim=-1, hit_idx=0):
+        """the reference's mp_extra_gen/stop/start/cal + mp_write_output for one alignment (oracle/aln_ref.c)"""
+        g = np.ascontiguousarray(ctg_nt4, dtype=np.uint8)
+        cig = np.ascontiguousarray(cigar, dtype=np.uint32)
+        mat = (C.c_int8 * 484).from_address(o.sc)
+        prow = None if pssm_row is None else np.ascontiguousarray(pssm_row, dtype=np.int8)
+        args = (ctg_name, g.ctypes.data, len(g), qname, aa, len(aa), rev, qs, qe, vs, ve, as_, ae,
+                cig.ctypes.data, len(cig), score, profile_score, o.go, o.ge, o.fs, C.addressof(mat),
+                None if prow is None else prow.ctypes.data, flag, acc_flags, rank_sc, chn_sc, gff_prefix, id,
+                gff_delim, hit_idx)
+        n = self.lib.or_ref_output(*args, None, 0)
+        assert n >= 0
+        buf = C.create_string_buffer(n + 1)
+        self.lib.or_ref_output(*args, buf, n + 1)
+        return buf.raw[:n]
 
 
 def have_ref():

@@ -16,7 +16,7 @@ from nsffi import NS_F_CIGAR, NS_F_EXT_LEFT, NS_F_EXT_RIGHT
 
 pytestmark = pytest.mark.gpu
 
-from cases import AA, planted_case, random_opt_kw, random_ss
+from cases import AA, parse_cigar, planted_case, random_opt_kw, random_ss
 
 
 # lane / wave / segment boundaries: C columns per lane (1..4), 64-lane waves, up to 16 waves,
@@ -153,3 +153,31 @@ def test_ext_flags_with_cigar_are_score_only(oracle, product):
         w = product.lib.ns_ext_stripe_width(nsffi.C.byref(product.opt(**kw)), len(a))
         exp = oracle.global_dp(ns, a, oracle.opt(**kw), mode=16, ext_width=w)
         assert product.run("ns_global_gs16b", ns, a, product.opt(**kw)) == exp
+
+
+def test_gpu_cigars_give_reference_output_records(product):
+    """the output path end to end: the GPU global DP's CIGAR of every golden output case
+    (ns_gpu_global), then the library's statistics and PAF / GFF3 / GTF records, byte for byte
+    against the records the reference's align.c / format.c wrote for the same alignment"""
+    import base64
+    import gzip
+    import json
+    import os
+    from cases import OUTPUT_VARIANTS, nt4_codes
+    from protlocus_amd import capi
+    with gzip.open(os.path.join(os.path.dirname(__file__), "golden", "output_cases.json.gz"), "rt") as f:
+        cases = json.load(f)
+    o = capi.opt()
+    for c in cases:
+        prot, window = base64.b64decode(c["prot"]), base64.b64decode(c["window"])
+        ok, (nl, al, score, cg) = product.gpu_global(window, prot[c["qs"]:c["qe"]], product.opt(flag=NS_F_CIGAR))
+        assert ok, (c["k"], product.error())
+        cigar = parse_cigar(cg)
+        assert (cigar, score) == (c["cigar"], c["score"]), c["k"]
+        nt = nt4_codes(base64.b64decode(c["oriented"]))[c["as_"]:c["ae"]]
+        pssm = None if c["pssm"] is None else np.array(c["pssm"], dtype=np.int8)
+        for (flag, acc, delim), exp in zip(OUTPUT_VARIANTS, c["expected"]):
+            text = capi.alignment_output(b"q1", prot, c["qs"], c["qe"], b"chr1", len(base64.b64decode(c["contig"])),
+                                         c["rev"], c["vs"], c["ve_aln"], cigar, score, nt, c["as_"], o, pssm_row=pssm,
+                                         flag=flag, acc_flags=acc, gff_delim=delim, **c["extra"])[3]
+            assert text == base64.b64decode(exp), (c["k"], flag)
