@@ -287,8 +287,9 @@ typedef struct {                /* one CDS (type 0) or stop_codon (type 1) featu
 
 /* Statistics and features of an alignment: mp_extra_stop, mp_extra_start and mp_extra_cal
  * (align.c:148-306) as align.c:608-612 calls them. nt: oriented target bases (codes 0..4) of
- * [as, ae), covering [vs, ve), searched for the nearest stop and start codons. pssm: the
- * protein's rows (qlen x opt->asize, HMM mode) or NULL to score with opt->sc. Returns the number
+ * [as, ae), covering [vs, ve), searched for the nearest stop and start codons. pssm: the PSSM
+ * row of protein position qs, rows of opt->asize (HMM mode, mp_pssm_at(c, qs)), or NULL to score
+ * with opt->sc. Returns the number
  * of features (>= 1); 0 when the CIGAR does not span [qs, qe) x [vs, ve), where the reference
  * drops the alignment's details (align.c:258-268); -1 on bad arguments or m_feat too small. */
 int32_t nsh_aln_stats(const nsh_aln_t *a, const uint8_t *nt, int64_t as, int64_t ae, const ns_opt_t *opt,
