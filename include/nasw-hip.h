@@ -231,6 +231,26 @@ void nsh_hmm_destroy(nsh_hmm_t *h);
 int nsh_pssm_ungapped_batch(const int8_t *pssm, int32_t L, const uint8_t *aa, int64_t l_aa, int32_t n,
 		const int64_t *orf_off, const int32_t *orf_len, const int32_t *diag, int32_t *score, int32_t *qs, int32_t *qe);
 
+/* PSSM diagonal evidence of refine chains: ev[c] = mp_chain_pssm_ev(nt + nt_off[c], l_nt[c],
+ * pssm + pssm_off[c], qlen[c], kmer, n_a[c], a + a_off[c], min_seg, xdrop) (map.c:123-139, with
+ * mp_diag_seg, map.c:97-120). nt: region bases (codes 0..4); pssm: qlen x 22 int8 rows; anchors
+ * x << 32 | q (x the k-mer's last base in the region, q its last residue; q + 1 - kmer must lie in
+ * [0, qlen]). seg_sc / seg_e (both or neither, one per anchor): the anchor's segment score and
+ * protein end, INT32_MIN / -1 when it repeats a recorded segment. 1 on success, 0 on error. */
+int nsh_chain_pssm_ev_batch(int32_t n, const uint8_t *nt, int64_t l_nt_all, const int64_t *nt_off,
+		const int64_t *l_nt, const int8_t *pssm, int64_t l_pssm, const int64_t *pssm_off, const int32_t *qlen,
+		const uint64_t *a, int64_t n_a_all, const int64_t *a_off, const int32_t *n_a, int32_t kmer,
+		int32_t min_seg, int32_t xdrop, int32_t *ev, int32_t *seg_sc, int32_t *seg_e);
+
+/* Six-frame ORF walk of ps_orf_worker + ps_add_orf (prescreen.c:87-132) over one oriented
+ * sequence (base codes 0..4; the reverse strand is passed reverse-complemented, as mp_ntseq_get
+ * returns it): every stop- and N-free run of >= min_orf (>= 1) codons in each frame, in the
+ * reference's order. orf_st[k] is the run's first base, orf_len[k] its codon count, and aa holds
+ * the runs' residues back to back (*n_aa_total of them). At most max_orf runs and max_aa
+ * residues are written. Returns the number of runs, or -1 on error. */
+int64_t nsh_orf_scan(const uint8_t *nt, int64_t len, int32_t min_orf, int64_t max_orf, int64_t *orf_st,
+		int32_t *orf_len, int64_t max_aa, uint8_t *aa, int64_t *n_aa_total);
+
 /* Learned chain-candidate prescreen: MLP 64 -> 256 -> 256 -> 1 (ReLU, logistic output), BF16
  * MFMA kernels with FP32 accumulation, FP32 Adam masters. Parameter vector (FP32, in order):
  * W1[64][256], b1[256], W2[256][256], b2[256], w3[256], b3. No reference counterpart. */

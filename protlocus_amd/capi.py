@@ -108,6 +108,13 @@ def lib():
         L.nsh_write_output.argtypes = [C.c_char_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_int64, C.c_int32, C.c_int32, C.c_char_p, C.c_int64, C.c_int32, C.c_int32]
         L.nsh_write_output.restype = C.c_int64
+        L.nsh_chain_pssm_ev_batch.argtypes = [C.c_int32, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                              C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                              C.c_void_p]
+        L.nsh_orf_scan.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
+                                   C.c_void_p, C.c_void_p]
+        L.nsh_orf_scan.restype = C.c_int64
         L.nsh_device_synchronize.restype = C.c_int
         L.nsh_build_info.restype = C.c_char_p
         if L.ns_make_tables(1) != 0:
@@ -319,3 +326,50 @@ def alignment_output(qname, aa, qs, qe, ctg_name, ctg_len, rev, vs, ve, cigar, d
     L.nsh_write_output(buf, n + 1, C.byref(a), stp, feats, ntb.ctypes.data, as_, flag, acc_flags, gff_prefix, id,
                        gff_delim, hit_idx)
     return nf, (st if nf > 0 else None), list(feats[:max(nf, 0)]), buf.raw[:n]
+
+
+def chain_pssm_ev(chains, kmer, min_seg, xdrop):
+    """PSSM diagonal evidence of refine chains (nsh_chain_pssm_ev_batch). chains: list of
+    (region base codes 0..4, qlen x 22 int8 PSSM, anchors x << 32 | q). Returns (ev per chain,
+    [per-anchor (score, aa_end) or None])."""
+    L = lib()
+    n = len(chains)
+    nts = [np.ascontiguousarray(c[0], dtype=np.uint8) for c in chains]
+    pss = [np.ascontiguousarray(c[1], dtype=np.int8).ravel() for c in chains]
+    ans = [np.ascontiguousarray(c[2], dtype=np.uint64) for c in chains]
+    cat = lambda xs, t: np.concatenate(xs).astype(t) if xs and sum(len(x) for x in xs) else np.zeros(1, t)
+    offs = lambda xs: np.concatenate([[0], np.cumsum([len(x) for x in xs])[:-1]]).astype(np.int64) if xs else np.zeros(0, np.int64)
+    nt, ps, an = cat(nts, np.uint8), cat(pss, np.int8), cat(ans, np.uint64)
+    nt_off, ps_off, a_off = offs(nts), offs(pss), offs(ans)
+    l_nt = np.array([len(x) for x in nts], dtype=np.int64)
+    qlen = np.array([len(x) // 22 for x in pss], dtype=np.int32)
+    n_a = np.array([len(x) for x in ans], dtype=np.int32)
+    n_a_all = int(n_a.sum())
+    ev = np.zeros(max(n, 1), dtype=np.int32)
+    sc, se = np.zeros(max(n_a_all, 1), dtype=np.int32), np.zeros(max(n_a_all, 1), dtype=np.int32)
+    ok = L.nsh_chain_pssm_ev_batch(n, nt.ctypes.data, int(l_nt.sum()), nt_off.ctypes.data, l_nt.ctypes.data,
+                                   ps.ctypes.data, int(sum(len(x) for x in pss)), ps_off.ctypes.data, qlen.ctypes.data,
+                                   an.ctypes.data, n_a_all, a_off.ctypes.data, n_a.ctypes.data, kmer, min_seg, xdrop,
+                                   ev.ctypes.data, sc.ctypes.data, se.ctypes.data)
+    if not ok:
+        raise RuntimeError("nsh_chain_pssm_ev_batch: " + error())
+    per = [None if s == INT32_MIN else (int(s), int(e)) for s, e in zip(sc[:n_a_all], se[:n_a_all])]
+    out, o = [], 0
+    for k in range(n):
+        out.append(per[o:o + n_a[k]])
+        o += n_a[k]
+    return ev[:n].tolist(), out
+
+
+def orf_scan(seq, min_orf):
+    """six-frame ORF walk of one oriented sequence (nsh_orf_scan): [(start, codons)], residues"""
+    L = lib()
+    s = np.ascontiguousarray(seq, dtype=np.uint8)
+    cap = len(s) + 8          # at most len / 3 runs and residues per frame
+    st, ln, aa = np.zeros(cap, np.int64), np.zeros(cap, np.int32), np.zeros(cap, np.uint8)
+    tot = C.c_int64()
+    n = L.nsh_orf_scan(s.ctypes.data, len(s), min_orf, cap, st.ctypes.data, ln.ctypes.data, len(aa), aa.ctypes.data,
+                       C.byref(tot))
+    if n < 0:
+        raise RuntimeError("nsh_orf_scan: " + error())
+    return [(int(st[i]), int(ln[i])) for i in range(n)], aa[:tot.value].tolist()

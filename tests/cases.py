@@ -168,3 +168,64 @@ def output_case(rng, k):
 # (flag, acc_flags, gff_delim) variants the output tests run on every case
 OUTPUT_VARIANTS = [(0, 0, -1), (0x200, 0, -1), (0x8, 0, -1), (0x8 | 0x10, 0, ord("|")), (0x20, 0, -1),
                    (0x400, 0x8, -1)]
+
+
+def revcomp_codes(codes):
+    """reverse complement of base codes 0..4 as mp_ntseq_get returns the reverse strand"""
+    return np.array([c if c >= 4 else 3 - c for c in codes[::-1]], dtype=np.uint8)
+
+
+def chain_case(rng, k):
+    """one refine chain for the PSSM evidence: a region carrying a mutated copy of a protein
+    stretch between random flanks (a few N bases), the protein's PSSM (its residues favoured), and
+    anchors on the true diagonal, repeated on it, random, and at the region's edges"""
+    from protlocus_amd import synth
+    kmer = [3, 4, 5][k % 3]
+    qlen = int(rng.integers(kmer + 20, 300))
+    prot = synth.rand_protein(rng, qlen)
+    qa = int(rng.integers(0, qlen // 3))
+    qb = int(rng.integers(qa + (qlen - qa) // 2, qlen + 1))
+    gene = synth.reverse_translate(rng, synth.substitute(rng, prot[qa:qb], float(rng.uniform(0.0, 0.4))))
+    left = synth.rand_dna(rng, int(rng.integers(0, 90)))
+    dna = bytearray(left + gene + synth.rand_dna(rng, int(rng.integers(0, 90))))
+    for p in rng.integers(0, len(dna), size=int(rng.integers(0, 3))):
+        dna[p] = ord("N")
+    nt = nt4_codes(bytes(dna))
+    aa20 = {c: i for i, c in enumerate(b"ARNDCQEGHILKMFPSTWYV*X")}
+    pssm = rng.integers(-9, 3, size=(qlen, 22)).astype(np.int8)
+    for j, c in enumerate(prot):
+        pssm[j, aa20[c]] = int(rng.integers(2, 9))
+    anchors = []
+    for _ in range(int(rng.integers(1, 24))):
+        r = rng.random()
+        if r < 0.5:     # on the planted diagonal (x = last base of the k-mer, q = its last residue)
+            q = int(rng.integers(max(qa, kmer - 1), qb))
+            x = len(left) + 3 * (q - qa) + 2
+        elif r < 0.8:   # anywhere
+            q = int(rng.integers(kmer - 1, qlen))
+            x = int(rng.integers(0, len(nt) + 6))
+        else:           # near the region's ends, including k-mers that start before it
+            q = int(rng.integers(kmer - 1, qlen))
+            x = int(rng.choice([0, 1, 2, 3 * kmer - 2, len(nt) - 1, len(nt) + 2]))
+        anchors.append(max(x, 0) << 32 | q)
+        if rng.random() < 0.3:   # a second anchor on the same diagonal
+            anchors.append((max(x, 0) + 3) << 32 | min(q + 1, qlen - 1))
+    return dict(nt=nt, pssm=pssm, qlen=qlen, kmer=kmer, anchors=anchors,
+                min_seg=[0, 15, 30, -1000][k % 4], xdrop=[10, 20, 40, 3][k % 4])
+
+
+def orf_case(rng, k):
+    """one sequence (codes 0..4) for the six-frame ORF walk: random DNA with stop-free coding
+    stretches and N runs"""
+    from protlocus_amd import synth
+    parts = []
+    for _ in range(int(rng.integers(1, 6))):
+        r = rng.random()
+        if r < 0.4:
+            parts.append(synth.rand_dna(rng, int(rng.integers(1, 900))))
+        elif r < 0.8:
+            parts.append(synth.reverse_translate(rng, synth.rand_protein(rng, int(rng.integers(5, 400)))))
+        else:
+            parts.append(b"N" * int(rng.integers(1, 4)))
+    seq = nt4_codes(b"".join(parts))
+    return dict(seq=seq, min_orf=[30, 1, 5, 12][k % 4])

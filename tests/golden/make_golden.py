@@ -9,6 +9,8 @@ Outputs (inputs and expected outputs only):
   dp_cases.json.gz   ns_global_gs16b / gs32b / gs_auto: extension, CIGAR, score-only, PSSM
   hmm_cases.json.gz  mp_hmm_read1 + mp_hmm_dp_score on synthetic HMMER3 profiles
   output_cases.json.gz  mp_extra_* (align.c) + mp_write_output (format.c): PAF / GFF3 / GTF records
+  pssm_chain_cases.json.gz  mp_chain_pssm_ev / mp_diag_seg (map.c) on refine chains
+  orf_cases.json.gz  ps_orf_worker (prescreen.c): six-frame ORFs of both strands
   tables.json        ns_tab_nt4 / ns_tab_aa20 / ns_tab_codon / ns_mat_blosum62
 """
 import base64
@@ -26,8 +28,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [os.path.join(ROOT, "tests"), ROOT]
 
 import nsffi                                       # noqa: E402
-from cases import (OUTPUT_VARIANTS, nt4_codes, output_case, parse_cigar, planted_case,   # noqa: E402
-                   random_opt_kw, random_ss)
+from cases import (OUTPUT_VARIANTS, chain_case, nt4_codes, orf_case, output_case, parse_cigar,   # noqa: E402
+                   planted_case, random_opt_kw, random_ss)
 from nsffi import NS_F_CIGAR, NS_F_EXT_LEFT, NS_F_EXT_RIGHT  # noqa: E402
 from protlocus_amd import synth                    # noqa: E402
 
@@ -144,6 +146,35 @@ def output_cases(ref):
     return out
 
 
+def pssm_chain_cases(ref):
+    """mp_chain_pssm_ev (map.c:123-139) and mp_diag_seg (97-120) of the compiled reference on
+    refine chains from cases.chain_case"""
+    rng = np.random.default_rng(20260404)
+    out = []
+    for k in range(60):
+        c = chain_case(rng, k)
+        ev = ref.chain_pssm_ev(c["nt"], c["pssm"], c["qlen"], c["kmer"], c["anchors"], c["min_seg"], c["xdrop"])
+        segs = [list(ref.diag_seg(c["nt"], c["pssm"], c["qlen"], c["kmer"], av >> 32, av & 0xffffffff, c["xdrop"]))
+                for av in c["anchors"]]
+        out.append({"nt": B64(c["nt"].tobytes()), "pssm": B64(c["pssm"].tobytes()), "qlen": c["qlen"],
+                    "kmer": c["kmer"], "anchors": [int(a) for a in c["anchors"]], "min_seg": c["min_seg"],
+                    "xdrop": c["xdrop"], "ev": ev, "diag_seg": segs})
+    return out
+
+
+def orf_cases(ref):
+    """ps_orf_worker + ps_add_orf (prescreen.c:87-132) of the compiled reference on both strands of
+    sequences from cases.orf_case"""
+    rng = np.random.default_rng(20260505)
+    out = []
+    for k in range(40):
+        o = orf_case(rng, k)
+        res = [ref.orfs(o["seq"], rev, o["min_orf"]) for rev in (0, 1)]
+        out.append({"seq": B64(o["seq"].tobytes()), "min_orf": o["min_orf"],
+                    "orfs": [[list(x) for x in r[0]] for r in res], "aa": [B64(bytes(r[1])) for r in res]})
+    return out
+
+
 def tables(ref):
     L = ref.lib
     get = lambda name, n, t=C.c_uint8: list((t * n).in_dll(L, name))
@@ -157,6 +188,10 @@ def main():
     ref = nsffi.Reference()
     with gzip.open(os.path.join(HERE, "dp_cases.json.gz"), "wt") as f:
         json.dump(dp_cases(ref), f, separators=(",", ":"))
+    with gzip.open(os.path.join(HERE, "pssm_chain_cases.json.gz"), "wt") as f:
+        json.dump(pssm_chain_cases(ref), f, separators=(",", ":"))
+    with gzip.open(os.path.join(HERE, "orf_cases.json.gz"), "wt") as f:
+        json.dump(orf_cases(ref), f, separators=(",", ":"))
     with gzip.open(os.path.join(HERE, "output_cases.json.gz"), "wt") as f:
         json.dump(output_cases(ref), f, separators=(",", ":"))
     with gzip.open(os.path.join(HERE, "hmm_cases.json.gz"), "wt") as f:

@@ -207,6 +207,13 @@ class Reference(_Lib):
                                     C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, C.c_char_p, C.c_int64,
                                     C.c_int32, C.c_int32, C.c_char_p, C.c_int64]
         L.or_ref_output.restype = C.c_int64
+        L.or_ref_diag_seg.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.c_int32, C.c_int64, C.c_int32,
+                                      C.c_int32, C.c_void_p]
+        L.or_ref_chain_pssm_ev.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                           C.c_void_p, C.c_int32, C.c_int32]
+        L.or_ref_orfs.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
+                                  C.c_int64, C.c_void_p, C.c_void_p]
+        L.or_ref_orfs.restype = C.c_int64
         assert L.ns_make_tables(1) == 0
         self._opt_init = L.ns_opt_init
         self._opt_set_sp = L.ns_opt_set_sp
@@ -225,6 +232,30 @@ class Reference(_Lib):
 
     def stripe_width(self, opt, al):
         return self.lib.ns_ext_stripe_width(C.byref(opt), al)
+
+    def diag_seg(self, nt, pssm, qlen, kmer, x, q, xdrop):
+        nt = np.ascontiguousarray(nt, dtype=np.uint8)
+        ps = np.ascontiguousarray(pssm, dtype=np.int8)
+        e = C.c_int32()
+        sc = self.lib.or_ref_diag_seg(nt.ctypes.data, len(nt), ps.ctypes.data, qlen, kmer, x, q, xdrop, C.byref(e))
+        return sc, e.value
+
+    def chain_pssm_ev(self, nt, pssm, qlen, kmer, anchors, min_seg, xdrop):
+        nt = np.ascontiguousarray(nt, dtype=np.uint8)
+        ps = np.ascontiguousarray(pssm, dtype=np.int8)
+        a = np.ascontiguousarray(anchors, dtype=np.uint64)
+        return self.lib.or_ref_chain_pssm_ev(nt.ctypes.data, len(nt), ps.ctypes.data, qlen, kmer, len(a),
+                                             a.ctypes.data, min_seg, xdrop)
+
+    def orfs(self, ctg_nt4, rev, min_orf):
+        """ps_orf_worker on strand rev of a contig: [(start, codons)], residues"""
+        g = np.ascontiguousarray(ctg_nt4, dtype=np.uint8)
+        cap = len(g) + 8
+        st, ln, aa = np.zeros(cap, np.int64), np.zeros(cap, np.int32), np.zeros(cap, np.uint8)
+        tot = C.c_int64()
+        n = self.lib.or_ref_orfs(g.ctypes.data, len(g), rev, min_orf, cap, st.ctypes.data, ln.ctypes.data, cap,
+                                 aa.ctypes.data, C.byref(tot))
+        return [(int(st[i]), int(ln[i])) for i in range(n)], aa[:tot.value].tolist()
 
     def output(self, ctg_name, ctg_nt4, qname, aa, rev, qs, qe, vs, ve, as_, ae, cigar, score, o, pssm_row=None,
                profile_score=-2 ** 31, rank_sc=0, chn_sc=0, flag=0, acc_flags=0, gff_prefix=b"PL", id=0,

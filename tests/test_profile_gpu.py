@@ -19,6 +19,7 @@ import pytest
 import nsffi
 
 pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
@@ -150,3 +151,87 @@ def test_pssm_ungapped_batch(oracle, product):
             e = oracle.lib.or_pssm_ungapped(pssm.ctypes.data, L, orfs[k].tobytes(), len(orfs[k]), d, C.byref(a),
                                             C.byref(b))
             assert (sc[i], qs[i], qe[i]) == (e, a.value, b.value), (L, i, k, d)
+
+
+def _tabs(oracle):
+    L = oracle.lib
+    return list((C.c_uint8 * 64).in_dll(L, "or_codon")), list((C.c_uint8 * 256).in_dll(L, "or_aa20"))[ord("X")]
+
+
+def test_chain_pssm_evidence_golden(oracle, product):
+    """nsh_chain_pssm_ev_batch against mp_chain_pssm_ev / mp_diag_seg of the compiled reference
+    (golden records), all chains of a parameter set in one batch"""
+    import base64
+    import pssm_ref
+    from protlocus_amd import capi
+    codon, ax = _tabs(oracle)
+    with gzip.open(os.path.join(HERE, "golden", "pssm_chain_cases.json.gz"), "rt") as f:
+        cases = json.load(f)
+    groups = {}
+    for c in cases:
+        groups.setdefault((c["kmer"], c["min_seg"], c["xdrop"]), []).append(c)
+    for (kmer, min_seg, xdrop), cs in groups.items():
+        chains = [(np.frombuffer(base64.b64decode(c["nt"]), dtype=np.uint8),
+                   np.frombuffer(base64.b64decode(c["pssm"]), dtype=np.int8).reshape(-1, 22), c["anchors"]) for c in cs]
+        ev, per = capi.chain_pssm_ev(chains, kmer, min_seg, xdrop)
+        assert ev == [c["ev"] for c in cs]
+        for c, ch, pa in zip(cs, chains, per):
+            _, exp_per = pssm_ref.chain_pssm_ev(ch[0].tolist(), ch[1].ravel().tolist(), c["qlen"], kmer, c["anchors"],
+                                                min_seg, xdrop, codon, ax)
+            assert [p is None for p in pa] == [p is None for p in exp_per]
+            for p, seg in zip(pa, c["diag_seg"]):
+                if p is not None:
+                    assert list(p) == seg
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_chain_pssm_evidence_random(oracle, product, seed):
+    import pssm_ref
+    from cases import chain_case
+    from protlocus_amd import capi
+    codon, ax = _tabs(oracle)
+    rng = np.random.default_rng(seed)
+    for kmer, min_seg, xdrop in ((3, 15, 20), (5, 0, 3), (4, -1000, 40)):
+        cs = [chain_case(rng, 3 * i + kmer - 3) for i in range(200)]    # chain_case(k) has kmer 3 + k % 3
+        assert all(c["kmer"] == kmer for c in cs)
+        chains = [(c["nt"], c["pssm"], c["anchors"]) for c in cs]
+        chains.append((np.zeros(0, np.uint8), np.zeros((8, 22), np.int8), [2 << 32 | 4]))   # empty region
+        chains.append((cs[0]["nt"], cs[0]["pssm"], []))                                      # no anchors
+        ev, per = capi.chain_pssm_ev(chains, kmer, min_seg, xdrop)
+        for k, (nt, pssm, an) in enumerate(chains):
+            exp, exp_per = pssm_ref.chain_pssm_ev(nt.tolist(), pssm.ravel().tolist(), len(pssm), kmer, an, min_seg,
+                                                  xdrop, codon, ax)
+            assert (ev[k], per[k]) == (exp, exp_per), k
+
+
+def test_orf_scan_golden(oracle, product):
+    import base64
+    from cases import revcomp_codes
+    from protlocus_amd import capi
+    with gzip.open(os.path.join(HERE, "golden", "orf_cases.json.gz"), "rt") as f:
+        cases = json.load(f)
+    for c in cases:
+        seq = np.frombuffer(base64.b64decode(c["seq"]), dtype=np.uint8)
+        for rev in (0, 1):
+            orfs, aa = capi.orf_scan(seq if rev == 0 else revcomp_codes(seq), c["min_orf"])
+            assert [list(x) for x in orfs] == c["orfs"][rev]
+            assert bytes(aa) == base64.b64decode(c["aa"][rev])
+
+
+def test_orf_scan_random_and_edges(oracle, product):
+    """long sequences (several chunks per frame), N runs, stop-free stretches, tiny inputs"""
+    import pssm_ref
+    from cases import nt4_codes, orf_case
+    from protlocus_amd import capi, synth
+    codon, _ = _tabs(oracle)
+    rng = np.random.default_rng(17)
+    seqs = [np.zeros(0, np.uint8), np.array([0, 1], np.uint8), np.array([0, 3, 2], np.uint8),
+            np.full(50, 4, np.uint8)]
+    prot = synth.rand_protein(rng, 9000)                        # one stop-free frame across chunks
+    seqs.append(nt4_codes(synth.reverse_translate(rng, prot)))
+    for _ in range(3):
+        seqs.append(np.concatenate([orf_case(rng, int(rng.integers(0, 8)))["seq"] for _ in range(120)]))
+    for s in seqs:
+        for m in (1, 30):
+            exp = pssm_ref.orf_walk(s.tolist(), m, codon)
+            assert capi.orf_scan(s, m) == exp, (len(s), m)
