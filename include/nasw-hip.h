@@ -251,6 +251,15 @@ int nsh_chain_pssm_ev_batch(int32_t n, const uint8_t *nt, int64_t l_nt_all, cons
 int64_t nsh_orf_scan(const uint8_t *nt, int64_t len, int32_t min_orf, int64_t max_orf, int64_t *orf_st,
 		int32_t *orf_len, int64_t max_aa, uint8_t *aa, int64_t *n_aa_total);
 
+/* Banded gapped stage of the prescreen search: sw[k] = ps_gapped(pssm, L, aa + orf_off[k],
+ * orf_len[k], diag[k], band, go, ge) (prescreen.c:346-382), the best local affine score of the
+ * profile (L x 22 int8 rows) against ORF k (residue codes < 22) within |t - i - diag| <= band, or
+ * the full matrix when band <= 0. Requires 0 <= ge <= go <= 65535 and 2 band + 1 (or, for
+ * band <= 0, the ORF length) <= 1024. 1 on success, 0 on error. */
+int nsh_ps_gapped_batch(const int8_t *pssm, int32_t L, const uint8_t *aa, int64_t l_aa, int32_t n,
+		const int64_t *orf_off, const int32_t *orf_len, const int32_t *diag, int32_t band, int32_t go, int32_t ge,
+		int32_t *sw);
+
 /* Learned chain-candidate prescreen: MLP 64 -> 256 -> 256 -> 1 (ReLU, logistic output), BF16
  * MFMA kernels with FP32 accumulation, FP32 Adam masters. Parameter vector (FP32, in order):
  * W1[64][256], b1[256], W2[256][256], b2[256], w3[256], b3. No reference counterpart. */

@@ -30,3 +30,15 @@ int64_t or_ref_orfs(const uint8_t *ctg_nt4, int64_t len, int32_t rev, int32_t mi
 	free(orf[rev & 1].a), free(av[rev & 1].a), free(nt.seq);
 	return n;
 }
+
+/* ps_gapped (prescreen.c:346-382): banded local affine score of a profile against one ORF */
+int32_t or_ref_ps_gapped(const int8_t *pssm, int32_t L, const uint8_t *orf, int32_t n, int32_t diag, int32_t band,
+		int32_t go, int32_t ge)
+{
+	ps_scratch_t s;
+	int32_t r;
+	memset(&s, 0, sizeof(s));
+	r = ps_gapped(&s, pssm, L, orf, n, diag, band, go, ge);
+	free(s.H), free(s.F);
+	return r;
+}

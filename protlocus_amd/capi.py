@@ -115,6 +115,8 @@ def lib():
         L.nsh_orf_scan.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
                                    C.c_void_p, C.c_void_p]
         L.nsh_orf_scan.restype = C.c_int64
+        L.nsh_ps_gapped_batch.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
         L.nsh_device_synchronize.restype = C.c_int
         L.nsh_build_info.restype = C.c_char_p
         if L.ns_make_tables(1) != 0:
@@ -373,3 +375,18 @@ def orf_scan(seq, min_orf):
     if n < 0:
         raise RuntimeError("nsh_orf_scan: " + error())
     return [(int(st[i]), int(ln[i])) for i in range(n)], aa[:tot.value].tolist()
+
+
+def ps_gapped_batch(pssm, orfs, diags, band, go, ge):
+    """banded local affine scores of one profile (L x 22 int8) against ORFs (residue code arrays)
+    on their diagonals (nsh_ps_gapped_batch)"""
+    L = lib()
+    ps = np.ascontiguousarray(pssm, dtype=np.int8)
+    buf, off, ln = pack([bytes(np.asarray(o, dtype=np.uint8)) for o in orfs])
+    dg = np.ascontiguousarray(diags, dtype=np.int32)
+    sw = np.zeros(max(len(orfs), 1), dtype=np.int32)
+    b = np.frombuffer(buf, dtype=np.uint8) if buf else np.zeros(1, np.uint8)
+    if not L.nsh_ps_gapped_batch(ps.ctypes.data, ps.shape[0], b.ctypes.data, len(buf), len(orfs), off.ctypes.data,
+                                 ln.ctypes.data, dg.ctypes.data, band, go, ge, sw.ctypes.data):
+        raise RuntimeError("nsh_ps_gapped_batch: " + error())
+    return sw[:len(orfs)].tolist()

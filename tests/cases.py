@@ -229,3 +229,33 @@ def orf_case(rng, k):
             parts.append(b"N" * int(rng.integers(1, 4)))
     seq = nt4_codes(b"".join(parts))
     return dict(seq=seq, min_orf=[30, 1, 5, 12][k % 4])
+
+
+def gapped_case(rng, k):
+    """one profile and its (ORF, diagonal) items for the banded gapped stage: ORFs carry mutated,
+    gapped copies of the profile's favoured residues on a diagonal, plus random ORFs"""
+    L = int(rng.integers(1, 160))
+    best = rng.integers(0, 20, size=L)
+    pssm = rng.integers(-9, 3, size=(L, 22)).astype(np.int8)
+    pssm[np.arange(L), best] = rng.integers(3, 10, size=L)
+    orfs, diags = [], []
+    for _ in range(int(rng.integers(1, 12))):
+        if rng.random() < 0.6:
+            core = [int(b) if rng.random() > 0.25 else int(rng.integers(0, 20)) for b in best]
+            for _ in range(int(rng.integers(0, 4))):      # indels
+                p = int(rng.integers(0, len(core) + 1))
+                if rng.random() < 0.5 and p < len(core):
+                    del core[p]
+                else:
+                    core[p:p] = rng.integers(0, 20, size=int(rng.integers(1, 4))).tolist()
+            pre = rng.integers(0, 20, size=int(rng.integers(0, 60))).tolist()
+            orf = pre + core + rng.integers(0, 20, size=int(rng.integers(0, 60))).tolist()
+            dg = len(pre) + int(rng.integers(-3, 4))
+        else:
+            orf = rng.integers(0, 20, size=int(rng.integers(0, 300))).tolist()
+            dg = int(rng.integers(-L - 10, len(orf) + 10))
+        orfs.append(np.array(orf, dtype=np.uint8))
+        diags.append(dg)
+    band = [32, 5, 1, 0, -1, 63][k % 6]
+    go, ge = [(11, 1), (5, 5), (20, 0), (8, 2)][k % 4]
+    return dict(pssm=pssm, orfs=orfs, diags=diags, band=band, go=go, ge=ge)

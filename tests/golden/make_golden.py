@@ -11,6 +11,7 @@ Outputs (inputs and expected outputs only):
   output_cases.json.gz  mp_extra_* (align.c) + mp_write_output (format.c): PAF / GFF3 / GTF records
   pssm_chain_cases.json.gz  mp_chain_pssm_ev / mp_diag_seg (map.c) on refine chains
   orf_cases.json.gz  ps_orf_worker (prescreen.c): six-frame ORFs of both strands
+  ps_gapped_cases.json.gz  ps_gapped (prescreen.c): banded local affine scores
   tables.json        ns_tab_nt4 / ns_tab_aa20 / ns_tab_codon / ns_mat_blosum62
 """
 import base64
@@ -28,7 +29,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [os.path.join(ROOT, "tests"), ROOT]
 
 import nsffi                                       # noqa: E402
-from cases import (OUTPUT_VARIANTS, chain_case, nt4_codes, orf_case, output_case, parse_cigar,   # noqa: E402
+from cases import (OUTPUT_VARIANTS, chain_case, gapped_case, nt4_codes, orf_case, output_case,   # noqa: E402
+                   parse_cigar,
                    planted_case, random_opt_kw, random_ss)
 from nsffi import NS_F_CIGAR, NS_F_EXT_LEFT, NS_F_EXT_RIGHT  # noqa: E402
 from protlocus_amd import synth                    # noqa: E402
@@ -175,6 +177,18 @@ def orf_cases(ref):
     return out
 
 
+def ps_gapped_cases(ref):
+    """ps_gapped (prescreen.c:346-382) of the compiled reference on cases.gapped_case"""
+    rng = np.random.default_rng(20260606)
+    out = []
+    for k in range(48):
+        c = gapped_case(rng, k)
+        sw = [ref.ps_gapped(c["pssm"], o, d, c["band"], c["go"], c["ge"]) for o, d in zip(c["orfs"], c["diags"])]
+        out.append({"pssm": B64(c["pssm"].tobytes()), "orfs": [B64(o.tobytes()) for o in c["orfs"]],
+                    "diags": c["diags"], "band": c["band"], "go": c["go"], "ge": c["ge"], "sw": sw})
+    return out
+
+
 def tables(ref):
     L = ref.lib
     get = lambda name, n, t=C.c_uint8: list((t * n).in_dll(L, name))
@@ -192,6 +206,8 @@ def main():
         json.dump(pssm_chain_cases(ref), f, separators=(",", ":"))
     with gzip.open(os.path.join(HERE, "orf_cases.json.gz"), "wt") as f:
         json.dump(orf_cases(ref), f, separators=(",", ":"))
+    with gzip.open(os.path.join(HERE, "ps_gapped_cases.json.gz"), "wt") as f:
+        json.dump(ps_gapped_cases(ref), f, separators=(",", ":"))
     with gzip.open(os.path.join(HERE, "output_cases.json.gz"), "wt") as f:
         json.dump(output_cases(ref), f, separators=(",", ":"))
     with gzip.open(os.path.join(HERE, "hmm_cases.json.gz"), "wt") as f:

@@ -214,6 +214,8 @@ class Reference(_Lib):
         L.or_ref_orfs.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
                                   C.c_int64, C.c_void_p, C.c_void_p]
         L.or_ref_orfs.restype = C.c_int64
+        L.or_ref_ps_gapped.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                       C.c_int32]
         assert L.ns_make_tables(1) == 0
         self._opt_init = L.ns_opt_init
         self._opt_set_sp = L.ns_opt_set_sp
@@ -246,6 +248,12 @@ class Reference(_Lib):
         a = np.ascontiguousarray(anchors, dtype=np.uint64)
         return self.lib.or_ref_chain_pssm_ev(nt.ctypes.data, len(nt), ps.ctypes.data, qlen, kmer, len(a),
                                              a.ctypes.data, min_seg, xdrop)
+
+    def ps_gapped(self, pssm, orf, diag, band, go, ge):
+        ps = np.ascontiguousarray(pssm, dtype=np.int8)
+        o = np.ascontiguousarray(orf, dtype=np.uint8)
+        return self.lib.or_ref_ps_gapped(ps.ctypes.data, len(ps) // 22 if ps.ndim == 1 else ps.shape[0],
+                                         o.ctypes.data, len(o), diag, band, go, ge)
 
     def orfs(self, ctg_nt4, rev, min_orf):
         """ps_orf_worker on strand rev of a contig: [(start, codons)], residues"""

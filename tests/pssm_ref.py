@@ -103,3 +103,30 @@ def orf_walk(seq, min_orf, codon):
         if k[f] >= min_orf:
             emit(e[f], k[f])
     return orfs, aa
+
+
+def ps_gapped(pssm, L, orf, diag, band, go, ge):
+    """banded local affine score of a profile (flat L x 22) against an ORF (prescreen.c:346-382)"""
+    NEG = -0x3fffffff
+    n = len(orf)
+    H, F = [0] * (n + 1), [NEG] * (n + 1)
+    best = 0
+    for i in range(L):
+        lo = i + diag - band if band > 0 else 0
+        hi = i + diag + band if band > 0 else n - 1
+        lo, hi = max(lo, 0), min(hi, n - 1)
+        if lo > hi:
+            continue
+        e = NEG
+        h_diag = H[lo - 1] if lo > 0 else 0
+        for t in range(lo, hi + 1):
+            h_up = H[t]
+            h = h_diag + pssm[i * 22 + orf[t]]
+            f = max(F[t] - ge, h_up - go)
+            F[t] = f
+            h = max(h, f, e, 0)
+            best = max(best, h)
+            e = max(e - ge, h - go)
+            h_diag = h_up
+            H[t] = h
+    return best

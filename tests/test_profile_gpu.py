@@ -235,3 +235,35 @@ def test_orf_scan_random_and_edges(oracle, product):
         for m in (1, 30):
             exp = pssm_ref.orf_walk(s.tolist(), m, codon)
             assert capi.orf_scan(s, m) == exp, (len(s), m)
+
+
+def test_ps_gapped_golden(product):
+    """nsh_ps_gapped_batch against ps_gapped of the compiled reference (golden records)"""
+    import base64
+    from protlocus_amd import capi
+    with gzip.open(os.path.join(HERE, "golden", "ps_gapped_cases.json.gz"), "rt") as f:
+        cases = json.load(f)
+    for c in cases:
+        pssm = np.frombuffer(base64.b64decode(c["pssm"]), dtype=np.int8).reshape(-1, 22)
+        orfs = [np.frombuffer(base64.b64decode(o), dtype=np.uint8) for o in c["orfs"]]
+        assert capi.ps_gapped_batch(pssm, orfs, c["diags"], c["band"], c["go"], c["ge"]) == c["sw"]
+
+
+def test_ps_gapped_random(product):
+    """larger batches: every band width up to the 1024-column limit, full matrices up to 1024 aa"""
+    import pssm_ref
+    from cases import gapped_case
+    from protlocus_amd import capi
+    rng = np.random.default_rng(23)
+    for k in range(12):
+        c = gapped_case(rng, k)
+        orfs, diags = list(c["orfs"]), list(c["diags"])
+        for _ in range(40):
+            n = int(rng.integers(0, 1025 if c["band"] <= 0 else 700))
+            orfs.append(rng.integers(0, 20, size=n).astype(np.uint8))
+            diags.append(int(rng.integers(-len(c["pssm"]) - 40, n + 40)))
+        band = [32, 100, 511, 0, 7, 64][k % 6]
+        got = capi.ps_gapped_batch(c["pssm"], orfs, diags, band, c["go"], c["ge"])
+        flat = c["pssm"].ravel().tolist()
+        for o, d, g in zip(orfs, diags, got):
+            assert g == pssm_ref.ps_gapped(flat, len(c["pssm"]), o.tolist(), d, band, c["go"], c["ge"]), (k, len(o), d)

@@ -119,3 +119,62 @@ def test_closed_forms(tabs):
                     if len(run) >= m:
                         assert (run[0], len(run)) in found
                     run = []
+
+
+def _gapped_cases():
+    for c in _load("ps_gapped_cases.json.gz"):
+        yield (np.frombuffer(base64.b64decode(c["pssm"]), dtype=np.int8).reshape(-1, 22),
+               [np.frombuffer(base64.b64decode(o), dtype=np.uint8) for o in c["orfs"]], c)
+
+
+def test_ps_gapped_golden():
+    for pssm, orfs, c in _gapped_cases():
+        for o, d, exp in zip(orfs, c["diags"], c["sw"]):
+            assert pssm_ref.ps_gapped(pssm.ravel().tolist(), len(pssm), o.tolist(), d, c["band"], c["go"], c["ge"]) == exp
+
+
+def test_ps_gapped_closed_forms():
+    """a band that covers the whole matrix gives the full-matrix score; a profile that rewards
+    exactly one residue per row scores its gap-free run on an ORF that spells it out; brute force
+    over all local alignments on tiny inputs (no gaps cheaper than matches)"""
+    from itertools import product
+    rng = np.random.default_rng(5)
+    for _ in range(60):
+        L, n = int(rng.integers(1, 30)), int(rng.integers(1, 40))
+        pssm = rng.integers(-6, 7, size=L * 22).tolist()
+        orf = rng.integers(0, 20, size=n).tolist()
+        full = pssm_ref.ps_gapped(pssm, L, orf, 0, 0, 11, 1)
+        assert pssm_ref.ps_gapped(pssm, L, orf, int(rng.integers(-5, 5)), L + n + 5, 11, 1) == full
+    for _ in range(20):
+        L = int(rng.integers(1, 40))
+        res = rng.integers(0, 20, size=L)
+        pssm = np.full((L, 22), -4, dtype=np.int64)
+        pssm[np.arange(L), res] = 5
+        pre = int(rng.integers(0, 10))
+        orf = rng.integers(0, 20, size=pre).tolist() + res.tolist()
+        assert pssm_ref.ps_gapped(pssm.ravel().tolist(), L, orf, pre, 32, 11, 1) >= 5 * L
+        assert pssm_ref.ps_gapped(pssm.ravel().tolist(), L, orf, pre, 32, 11, 1) <= 5 * L   # no better local path
+    # gap-free local alignment is exact when gaps can never pay: go larger than any positive run
+    for _ in range(30):
+        L, n = int(rng.integers(1, 6)), int(rng.integers(1, 7))
+        pssm = rng.integers(-5, 6, size=L * 22).tolist()
+        orf = rng.integers(0, 20, size=n).tolist()
+        brute = 0
+        for i, t in product(range(L), range(n)):
+            s = 0
+            for d in range(min(L - i, n - t)):
+                s += pssm[(i + d) * 22 + orf[t + d]]
+                brute = max(brute, s)
+        assert pssm_ref.ps_gapped(pssm, L, orf, 0, 0, 1000, 1000) == brute
+
+
+@pytest.mark.skipif(not nsffi.have_ref(), reason="the compiled reference (make -C oracle ref) is absent")
+def test_ps_gapped_matches_reference_live():
+    from cases import gapped_case
+    ref = nsffi.Reference()
+    rng = np.random.default_rng(77)
+    for k in range(60):
+        c = gapped_case(rng, k)
+        for o, d in zip(c["orfs"], c["diags"]):
+            assert pssm_ref.ps_gapped(c["pssm"].ravel().tolist(), len(c["pssm"]), o.tolist(), d, c["band"], c["go"],
+                                      c["ge"]) == ref.ps_gapped(c["pssm"], o, d, c["band"], c["go"], c["ge"])
