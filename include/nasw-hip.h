@@ -260,6 +260,31 @@ int nsh_ps_gapped_batch(const int8_t *pssm, int32_t L, const uint8_t *aa, int64_
 		const int64_t *orf_off, const int32_t *orf_len, const int32_t *diag, int32_t band, int32_t go, int32_t ge,
 		int32_t *sw);
 
+/* The profile prescreen search (prescreen.c:133-486): an ORF database and spaced k-mer index of a
+ * genome built once on the device (mp_prescreen_build: contigs of base codes 0..4, forward
+ * strand, at nt + ctg_off[c]; kmer 5..7; ORFs of >= max(min_orf, span) aa on both strands), then
+ * per profile the reference worker's candidates (ps_query_worker): pssm L x 22 int8 rows, thr the
+ * k-mer score threshold (mp_ps_kmer_thr), kmer_max, min_ungapped, max_seqs, band, go, ge and
+ * max_evalue as mp_psopt_t holds them. nsh_ps_cand_t has mp_cand1_t's layout (protlocus.h:73-80);
+ * the candidates come in the worker's order, before mp_cand_tab_set merges them. The search
+ * returns how many there are (only max_out are written), or -1 on error.
+ * nsh_prescreen_size: 0 ORFs, 1 residues, 2 index entries. */
+typedef struct nsh_prescreen_s nsh_prescreen_t;
+typedef struct {
+	uint32_t vid;               /* contig << 1 | reverse */
+	int64_t vs, ve;             /* the ORF's oriented span */
+	double ev;                  /* E-value of its banded gapped score */
+	int32_t aqs, aqe, asc;      /* best ungapped diagonal segment: query interval and score */
+	int64_t avs;                /* oriented start of the codon facing query position aqs */
+} nsh_ps_cand_t;
+nsh_prescreen_t *nsh_prescreen_build(int32_t n_ctg, const uint8_t *nt, const int64_t *ctg_off,
+		const int64_t *ctg_len, int32_t kmer, int32_t min_orf);
+int64_t nsh_prescreen_size(const nsh_prescreen_t *ps, int32_t what);
+int32_t nsh_prescreen_search(nsh_prescreen_t *ps, const int8_t *pssm, int32_t L, int32_t thr, int32_t kmer_max,
+		int32_t min_ungapped, int32_t max_seqs, int32_t band, int32_t go, int32_t ge, double max_evalue,
+		nsh_ps_cand_t *out, int32_t max_out);
+void nsh_prescreen_destroy(nsh_prescreen_t *ps);
+
 /* Learned chain-candidate prescreen: MLP 64 -> 256 -> 256 -> 1 (ReLU, logistic output), BF16
  * MFMA kernels with FP32 accumulation, FP32 Adam masters. Parameter vector (FP32, in order):
  * W1[64][256], b1[256], W2[256][256], b2[256], w3[256], b3. No reference counterpart. */

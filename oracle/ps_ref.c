@@ -42,3 +42,56 @@ int32_t or_ref_ps_gapped(const int8_t *pssm, int32_t L, const uint8_t *orf, int3
 	free(s.H), free(s.F);
 	return r;
 }
+
+/* The whole per-profile prescreen search of the reference: mp_prescreen_build over a genome of
+ * contigs (base codes 0..4, forward strand, concatenated at ctg_off), then ps_query_worker
+ * (prescreen.c:402-486) for one profile h (an mp_hmm_t from mp_hmm_read1). Writes up to max
+ * candidates as mp_cand1_t (the worker's order, before mp_cand_tab_set merges them) and returns
+ * how many there were; *n_orf / *n_aa receive the ORF database's size. */
+int32_t or_ref_ps_search(int32_t n_ctg, const uint8_t *nt4, const int64_t *ctg_off, const int64_t *ctg_len,
+		const mp_hmm_t *h, int32_t kmer, int32_t min_orf, float sens, int32_t kmer_thr, int32_t kmer_max,
+		int32_t min_ungapped, int32_t max_seqs, int32_t band, int32_t go, int32_t ge, double max_evalue,
+		int32_t max, mp_cand1_t *out, int64_t *n_orf, int64_t *n_aa)
+{
+	mp_ntdb_t nt;
+	mp_psopt_t po;
+	mp_prescreen_t *ps;
+	ps_batch_aux_t aux;
+	int64_t i, tot = 0;
+	int32_t c, n;
+	memset(&nt, 0, sizeof(nt));
+	for (c = 0; c < n_ctg; ++c) tot = ctg_off[c] + ctg_len[c] > tot? ctg_off[c] + ctg_len[c] : tot;
+	nt.seq = (uint8_t*)calloc(tot / 2 + 2, 1);
+	for (i = 0; i < tot; ++i) nt.seq[i >> 1] |= (nt4[i] & 0xf) << ((i & 1) * 4);
+	nt.n_ctg = nt.m_ctg = n_ctg, nt.l_seq = tot;
+	nt.ctg = (mp_ctg_t*)calloc(n_ctg, sizeof(mp_ctg_t));
+	for (c = 0; c < n_ctg; ++c) nt.ctg[c].off = ctg_off[c], nt.ctg[c].len = ctg_len[c], nt.ctg[c].name = (char*)"c";
+	mp_psopt_init(&po);
+	po.kmer = kmer, po.min_orf = min_orf, po.sens = sens, po.kmer_thr = kmer_thr, po.kmer_max = kmer_max;
+	po.min_ungapped = min_ungapped, po.max_seqs = max_seqs, po.band = band, po.go = go, po.ge = ge;
+	ps = mp_prescreen_build(&nt, &po, 1);
+	*n_orf = ps->n_orf, *n_aa = ps->n_aa;
+	memset(&aux, 0, sizeof(aux));
+	aux.ps = ps, aux.po = &po, aux.max_evalue = max_evalue, aux.thr = mp_ps_kmer_thr(&po);
+	aux.hmm = (mp_hmm_t**)&h;
+	aux.scr = (ps_scratch_t*)calloc(1, sizeof(ps_scratch_t));
+	aux.res = (mp_cand1_t**)calloc(1, sizeof(mp_cand1_t*));
+	aux.n_res = (int32_t*)calloc(1, sizeof(int32_t));
+	ps_query_worker(&aux, 0, 0);
+	n = aux.n_res[0];
+	memcpy(out, aux.res[0], (size_t)(n < max? n : max) * sizeof(mp_cand1_t));
+	free(aux.res[0]);
+	ps_scratch_free(aux.scr);
+	free(aux.scr), free(aux.res), free(aux.n_res);
+	mp_prescreen_destroy(ps);
+	free(nt.ctg), free(nt.seq);
+	return n;
+}
+
+int32_t or_ref_ps_kmer_thr(int32_t kmer, float sens, int32_t kmer_thr)
+{
+	mp_psopt_t po;
+	mp_psopt_init(&po);
+	po.kmer = kmer, po.sens = sens, po.kmer_thr = kmer_thr;
+	return mp_ps_kmer_thr(&po);
+}

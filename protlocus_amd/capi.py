@@ -117,6 +117,15 @@ def lib():
         L.nsh_orf_scan.restype = C.c_int64
         L.nsh_ps_gapped_batch.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
                                           C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
+        L.nsh_prescreen_build.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]
+        L.nsh_prescreen_build.restype = C.c_void_p
+        L.nsh_prescreen_size.argtypes = [C.c_void_p, C.c_int32]
+        L.nsh_prescreen_size.restype = C.c_int64
+        L.nsh_prescreen_search.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                           C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_void_p,
+                                           C.c_int32]
+        L.nsh_prescreen_search.restype = C.c_int32
+        L.nsh_prescreen_destroy.argtypes = [C.c_void_p]
         L.nsh_device_synchronize.restype = C.c_int
         L.nsh_build_info.restype = C.c_char_p
         if L.ns_make_tables(1) != 0:
@@ -390,3 +399,46 @@ def ps_gapped_batch(pssm, orfs, diags, band, go, ge):
                                  ln.ctypes.data, dg.ctypes.data, band, go, ge, sw.ctypes.data):
         raise RuntimeError("nsh_ps_gapped_batch: " + error())
     return sw[:len(orfs)].tolist()
+
+
+class PsCand(C.Structure):
+    """nsh_ps_cand_t (the layout of the reference's mp_cand1_t)"""
+    _fields_ = [("vid", C.c_uint32), ("vs", C.c_int64), ("ve", C.c_int64), ("ev", C.c_double), ("aqs", C.c_int32),
+                ("aqe", C.c_int32), ("asc", C.c_int32), ("avs", C.c_int64)]
+
+    def tuple(self):
+        return (self.vid, self.vs, self.ve, self.ev, self.aqs, self.aqe, self.asc, self.avs)
+
+
+class ProfileSearch:
+    """the profile prescreen search over one genome (nsh_prescreen_*): contigs of base codes 0..4"""
+
+    def __init__(self, genome, ctg_off, ctg_len, kmer=6, min_orf=30):
+        L = lib()
+        g = np.ascontiguousarray(genome, dtype=np.uint8)
+        off = np.ascontiguousarray(ctg_off, dtype=np.int64)
+        ln = np.ascontiguousarray(ctg_len, dtype=np.int64)
+        self.p = L.nsh_prescreen_build(len(off), g.ctypes.data, off.ctypes.data, ln.ctypes.data, kmer, min_orf)
+        if not self.p:
+            raise RuntimeError("nsh_prescreen_build: " + error())
+
+    def size(self, what):
+        return lib().nsh_prescreen_size(self.p, what)
+
+    def search(self, pssm, thr, kmer_max=65536, min_ungapped=15, max_seqs=4000, band=32, go=11, ge=1, max_evalue=0.0):
+        ps = np.ascontiguousarray(pssm, dtype=np.int8).reshape(-1, 22)
+        cap = 4096
+        while True:
+            out = (PsCand * cap)()
+            n = lib().nsh_prescreen_search(self.p, ps.ctypes.data, ps.shape[0], thr, kmer_max, min_ungapped, max_seqs,
+                                           band, go, ge, max_evalue, out, cap)
+            if n < 0:
+                raise RuntimeError("nsh_prescreen_search: " + error())
+            if n <= cap:
+                return [out[i].tuple() for i in range(n)]
+            cap = n
+
+    def close(self):
+        if self.p:
+            lib().nsh_prescreen_destroy(self.p)
+            self.p = None

@@ -40,4 +40,12 @@ void nsh_prep_host(const char *ns, int32_t nl, const char *as, int32_t al, const
 int nsh_global_run(void *km, const char *ns, int32_t nl, const char *as, int32_t al,
 		const ns_opt_t *opt, const uint8_t *ss, int32_t is32, ns_rst_t *r);
 
+// ---- banded gapped stage of the prescreen search (ps_gapped.hip), device-resident items ----
+#include <hip/hip_runtime.h>
+namespace nsh_psg {
+void launch(int C, bool full, const int8_t *d_pssm, int32_t L, const uint8_t *d_aa, const int64_t *d_off,
+		const int32_t *d_len, const int32_t *d_diag, int32_t band, int32_t go, int32_t ge, const int32_t *d_items,
+		int32_t n_items, int32_t *d_out, hipStream_t s);
+}
+
 #endif

@@ -128,6 +128,17 @@ std::mutex g_mu;
 
 } // namespace
 
+// device-resident items (the prescreen search): one launch for the items of one column count
+void nsh_psg::launch(int C, bool full, const int8_t *d_pssm, int32_t L, const uint8_t *d_aa, const int64_t *d_off,
+		const int32_t *d_len, const int32_t *d_diag, int32_t band, int32_t go, int32_t ge, const int32_t *d_items,
+		int32_t n_items, int32_t *d_out, hipStream_t s)
+{
+	GapArgs ga;
+	ga.pssm = d_pssm, ga.L = L, ga.aa = d_aa, ga.off = d_off, ga.len = d_len, ga.diag = d_diag;
+	ga.band = band, ga.go = go, ga.ge = ge, ga.items = d_items, ga.n_items = n_items, ga.out = d_out;
+	::launch(C, ga, full, s);
+}
+
 #define PGOK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { nsh_set_error("ps_gapped: %s: %s", #x, hipGetErrorString(e_)); goto fail; } } while (0)
 
 extern "C" int nsh_ps_gapped_batch(const int8_t *pssm, int32_t L, const uint8_t *aa, int64_t l_aa, int32_t n,

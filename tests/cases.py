@@ -259,3 +259,31 @@ def gapped_case(rng, k):
     band = [32, 5, 1, 0, -1, 63][k % 6]
     go, ge = [(11, 1), (5, 5), (20, 0), (8, 2)][k % 4]
     return dict(pssm=pssm, orfs=orfs, diags=diags, band=band, go=go, ge=ge)
+
+
+def prescreen_case(rng, k):
+    """a genome of 1-3 contigs (random DNA, a few N runs) carrying sequences sampled from a profile
+    HMM on both strands, the profile's HMMER3 text, and search parameters (variants by k)"""
+    from protlocus_amd import synth
+    M = int(rng.integers(40, 300))
+    em, tr = synth.hmm_params(rng, M)
+    comp = bytes.maketrans(b"ACGTN", b"TGCAN")
+    contigs = []
+    for _ in range(int(rng.integers(1, 4))):
+        parts = []
+        for _ in range(int(rng.integers(1, 4)) + (6 if k % 4 == 3 else 0)):
+            parts.append(synth.rand_dna(rng, int(rng.integers(2000, 30000))))
+            gene = synth.reverse_translate(rng, synth.hmm_sample_protein(rng, em, tr).encode())
+            parts.append(gene if rng.random() < 0.5 else gene[::-1].translate(comp))
+            if rng.random() < 0.3:
+                parts.append(b"N" * int(rng.integers(1, 30)))
+        parts.append(synth.rand_dna(rng, int(rng.integers(100, 5000))))
+        contigs.append(nt4_codes(b"".join(parts)))
+    ctg_len = np.array([len(c) for c in contigs], dtype=np.int64)
+    ctg_off = np.concatenate([[0], np.cumsum(ctg_len)[:-1]]).astype(np.int64)
+    params = dict(kmer=[6, 5, 7, 6][k % 4], min_orf=[30, 20, 30, 45][k % 4], sens=[7.5, 5.0, 7.5, 8.5][k % 4],
+                  kmer_max=[65536, 65536, 40, 65536][k % 4], min_ungapped=[15, 15, 30, 5][k % 4],
+                  max_seqs=[4000, 4000, 4000, 3][k % 4], band=[32, 0, 10, 32][k % 4],
+                  max_evalue=[1e-3, 0.0, 10.0, 0.0][k % 4])
+    return dict(genome=np.concatenate(contigs), ctg_off=ctg_off, ctg_len=ctg_len, text=synth.hmm_text(em, tr),
+                params=params)

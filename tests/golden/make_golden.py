@@ -12,6 +12,7 @@ Outputs (inputs and expected outputs only):
   pssm_chain_cases.json.gz  mp_chain_pssm_ev / mp_diag_seg (map.c) on refine chains
   orf_cases.json.gz  ps_orf_worker (prescreen.c): six-frame ORFs of both strands
   ps_gapped_cases.json.gz  ps_gapped (prescreen.c): banded local affine scores
+  prescreen_cases.json.gz  mp_prescreen_build + ps_query_worker (prescreen.c): profile search candidates
   tables.json        ns_tab_nt4 / ns_tab_aa20 / ns_tab_codon / ns_mat_blosum62
 """
 import base64
@@ -30,7 +31,7 @@ sys.path[:0] = [os.path.join(ROOT, "tests"), ROOT]
 
 import nsffi                                       # noqa: E402
 from cases import (OUTPUT_VARIANTS, chain_case, gapped_case, nt4_codes, orf_case, output_case,   # noqa: E402
-                   parse_cigar,
+                   parse_cigar, prescreen_case,
                    planted_case, random_opt_kw, random_ss)
 from nsffi import NS_F_CIGAR, NS_F_EXT_LEFT, NS_F_EXT_RIGHT  # noqa: E402
 from protlocus_amd import synth                    # noqa: E402
@@ -189,6 +190,27 @@ def ps_gapped_cases(ref):
     return out
 
 
+def prescreen_cases(ref):
+    """the reference's profile prescreen search (mp_prescreen_build + ps_query_worker,
+    prescreen.c:133-486) on cases.prescreen_case genomes and profiles"""
+    import zlib
+    rng = np.random.default_rng(20260707)
+    out = []
+    for k in range(8):
+        c = prescreen_case(rng, k)
+        h = ref.hmm(c["text"])
+        pssm = ref.pssm(h)
+        p = c["params"]
+        thr = ref.lib.or_ref_ps_kmer_thr(p["kmer"], p["sens"], 0)
+        kw = {x: p[x] for x in ("kmer", "min_orf", "sens", "kmer_max", "min_ungapped", "max_seqs", "band", "max_evalue")}
+        cands, size = ref.ps_search(c["genome"], c["ctg_off"], c["ctg_len"], h, **kw)
+        ref.lib.mp_hmm_destroy(h)
+        out.append({"genome": B64(zlib.compress(c["genome"].tobytes(), 9)), "ctg_off": c["ctg_off"].tolist(),
+                    "ctg_len": c["ctg_len"].tolist(), "pssm": B64(pssm.tobytes()), "thr": thr, "params": p,
+                    "size": list(size), "cands": [list(x) for x in cands]})
+    return out
+
+
 def tables(ref):
     L = ref.lib
     get = lambda name, n, t=C.c_uint8: list((t * n).in_dll(L, name))
@@ -208,6 +230,8 @@ def main():
         json.dump(orf_cases(ref), f, separators=(",", ":"))
     with gzip.open(os.path.join(HERE, "ps_gapped_cases.json.gz"), "wt") as f:
         json.dump(ps_gapped_cases(ref), f, separators=(",", ":"))
+    with gzip.open(os.path.join(HERE, "prescreen_cases.json.gz"), "wt") as f:
+        json.dump(prescreen_cases(ref), f, separators=(",", ":"))
     with gzip.open(os.path.join(HERE, "output_cases.json.gz"), "wt") as f:
         json.dump(output_cases(ref), f, separators=(",", ":"))
     with gzip.open(os.path.join(HERE, "hmm_cases.json.gz"), "wt") as f:

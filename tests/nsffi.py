@@ -216,6 +216,10 @@ class Reference(_Lib):
         L.or_ref_orfs.restype = C.c_int64
         L.or_ref_ps_gapped.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                        C.c_int32]
+        L.or_ref_ps_search.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                                       C.c_float, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                       C.c_int32, C.c_double, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_ref_ps_kmer_thr.argtypes = [C.c_int32, C.c_float, C.c_int32]
         assert L.ns_make_tables(1) == 0
         self._opt_init = L.ns_opt_init
         self._opt_set_sp = L.ns_opt_set_sp
@@ -254,6 +258,37 @@ class Reference(_Lib):
         o = np.ascontiguousarray(orf, dtype=np.uint8)
         return self.lib.or_ref_ps_gapped(ps.ctypes.data, len(ps) // 22 if ps.ndim == 1 else ps.shape[0],
                                          o.ctypes.data, len(o), diag, band, go, ge)
+
+    def hmm(self, text):
+        """an mp_hmm_t parsed by the reference's mp_hmm_read1 from HMMER3 text"""
+        import tempfile
+        with tempfile.NamedTemporaryFile("w", suffix=".hmm", delete=False) as f:
+            f.write(text)
+        fp = self.lib.mp_hmm_open(f.name.encode())
+        h = self.lib.mp_hmm_read1(fp)
+        self.lib.mp_hmm_close(fp)
+        os.unlink(f.name)
+        return h
+
+    def pssm(self, h):
+        n = self.lib.mp_hmm_length(h)
+        return np.ctypeslib.as_array(self.lib.mp_hmm_pssm(h), shape=(n * 22,)).reshape(n, 22).copy()
+
+    def ps_search(self, genome, ctg_off, ctg_len, h, kmer=6, min_orf=30, sens=7.5, kmer_thr=0, kmer_max=65536,
+                  min_ungapped=15, max_seqs=4000, band=32, go=11, ge=1, max_evalue=0.0):
+        """the reference's mp_prescreen_build + ps_query_worker: candidate tuples, (ORFs, residues)"""
+        from protlocus_amd.capi import PsCand
+        g = np.ascontiguousarray(genome, dtype=np.uint8)
+        off = np.ascontiguousarray(ctg_off, dtype=np.int64)
+        ln = np.ascontiguousarray(ctg_len, dtype=np.int64)
+        cap = 200000
+        out = (PsCand * cap)()
+        n_orf, n_aa = C.c_int64(), C.c_int64()
+        n = self.lib.or_ref_ps_search(len(off), g.ctypes.data, off.ctypes.data, ln.ctypes.data, h, kmer, min_orf, sens,
+                                      kmer_thr, kmer_max, min_ungapped, max_seqs, band, go, ge, max_evalue, cap, out,
+                                      C.byref(n_orf), C.byref(n_aa))
+        assert n <= cap
+        return [out[i].tuple() for i in range(n)], (n_orf.value, n_aa.value)
 
     def orfs(self, ctg_nt4, rev, min_orf):
         """ps_orf_worker on strand rev of a contig: [(start, codons)], residues"""

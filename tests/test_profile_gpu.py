@@ -267,3 +267,28 @@ def test_ps_gapped_random(product):
         flat = c["pssm"].ravel().tolist()
         for o, d, g in zip(orfs, diags, got):
             assert g == pssm_ref.ps_gapped(flat, len(c["pssm"]), o.tolist(), d, band, c["go"], c["ge"]), (k, len(o), d)
+
+
+def test_prescreen_search_golden(product):
+    """nsh_prescreen_build + nsh_prescreen_search against the reference's mp_prescreen_build +
+    ps_query_worker (golden records): the ORF database size and the candidate list, field by
+    field (E-values included) and in order; spaced 5-, 6- and 7-mers, banded and full gapped
+    stages, E-value cuts, k-mer caps and max_seqs truncation"""
+    import base64
+    import zlib
+    from protlocus_amd import capi
+    with gzip.open(os.path.join(HERE, "golden", "prescreen_cases.json.gz"), "rt") as f:
+        cases = json.load(f)
+    for k, c in enumerate(cases):
+        g = np.frombuffer(zlib.decompress(base64.b64decode(c["genome"])), dtype=np.uint8)
+        p = c["params"]
+        print("case", k, "genome", len(g), "kmer", p["kmer"], flush=True)
+        ps = capi.ProfileSearch(g, c["ctg_off"], c["ctg_len"], kmer=p["kmer"], min_orf=p["min_orf"])
+        try:
+            assert [ps.size(0), ps.size(1)] == c["size"], k
+            pssm = np.frombuffer(base64.b64decode(c["pssm"]), dtype=np.int8).reshape(-1, 22)
+            got = ps.search(pssm, c["thr"], kmer_max=p["kmer_max"], min_ungapped=p["min_ungapped"],
+                            max_seqs=p["max_seqs"], band=p["band"], max_evalue=p["max_evalue"])
+            assert [list(x) for x in got] == c["cands"], (k, got, c["cands"])
+        finally:
+            ps.close()
