@@ -14,7 +14,8 @@ reference's nl x al window cells are reported beside it.
 Beside it, in the same JSON line:
   * prescreen: the learned chain-candidate prescreen trained on the GPU, then the plain and the
     prescreened DP pipelines timed on fresh candidates (recall against the exact DP);
-  * hmm: profile-HMM Viterbi scan, 64 profiles (M 100-800) x a 10-Mb genome in 6 frames.
+  * hmm: profile-HMM Viterbi scan, 64 profiles (M 100-800) x a 10-Mb genome in 6 frames;
+  * profile_search: the profile prescreen search of the same profiles over the same genome.
 Output: one JSON line.
 """
 import argparse
@@ -142,6 +143,74 @@ def bench_hmm(seconds_cpu=4.0, log=None):
     return out
 
 
+def ps_kmer_thr(kmer, sens):
+    """mp_ps_kmer_thr (prescreen.c:63-71) in its float32 arithmetic"""
+    import numpy as np
+    f = np.float32
+    a, b = {5: (108.8, 4.7), 7: (149.15, 6.85)}.get(kmer, (134.35, 6.15))
+    t = f(a) - f(sens) * f(b)
+    return int(np.floor(f(int(t)) / f(4.0) + f(0.5)))
+
+
+def bench_profile_search(cpu=True, seconds_cpu=10.0, log=None):
+    """the profile prescreen search (mp_prescreen_build, then ps_query_worker per profile) over the
+    HMM workload's 10-Mb genome and 64 profiles: the GPU's index build and search times and
+    candidates, and the reference's own search timed on a bounded sample of the profiles (threads
+    over profiles), whose candidates are compared with the GPU's"""
+    import numpy as np
+    import nsffi
+    from protlocus_amd import capi, synth
+    kmer, min_orf, sens, max_ev = 6, 30, 7.5, 1e-3
+    thr = ps_kmer_thr(kmer, sens)
+    profs, g = synth.profile_genome()
+    pssms = [synth.hmm_pssm(em) for em, _ in profs]
+    t0 = time.perf_counter()
+    ps = capi.ProfileSearch(g, [0], [len(g)], kmer=kmer, min_orf=min_orf)
+    build_s = time.perf_counter() - t0
+    try:
+        sizes = (ps.size(0), ps.size(1), ps.size(2))
+        ps.search(pssms[0], thr, max_evalue=max_ev)   # warm-up
+        t0 = time.perf_counter()
+        res = [ps.search(P, thr, max_evalue=max_ev) for P in pssms]
+        search_s = time.perf_counter() - t0
+    finally:
+        ps.close()
+    out = {"workload": "64 profile HMMs (M 100-800) x 10 Mb genome (GC 0.45, 200 planted hits), both strands: "
+                       "ORF database + spaced 6-mer index, then per profile k-mer hits, double-hit diagonals, "
+                       "ungapped and banded gapped scores, E <= 1e-3",
+           "orfs": sizes[0], "orf_residues": sizes[1], "index_entries": sizes[2],
+           "gpu_build_ms": round(build_s * 1e3, 1), "gpu_search_ms_per_profile": round(search_s * 1e3 / len(pssms), 2),
+           "candidates": int(sum(len(r) for r in res))}
+    if log:
+        log("[bench] profile search: build %.0f ms, %.2f ms per profile, %d candidates" % (
+            build_s * 1e3, search_s * 1e3 / len(pssms), out["candidates"]))
+    if cpu and nsffi.have_ref():
+        ref = nsffi.Reference()
+        t0 = time.perf_counter()
+        hnd = ref.ps_build(g, [0], [len(g)], kmer=kmer, min_orf=min_orf)
+        cpu_build = time.perf_counter() - t0
+        hmms = [ref.hmm(synth.hmm_text(em, tr)) for em, tr in profs]
+
+        def one(k):
+            return ref.ps_query(hnd, hmms[k], kmer=kmer, sens=sens, max_evalue=max_ev)
+        ntr = threads()
+        done, same, t0 = [], 0, time.perf_counter()
+        with cf.ThreadPoolExecutor(ntr) as ex:
+            while len(done) < len(profs) and time.perf_counter() - t0 < seconds_cpu:
+                ks = list(range(len(done), min(len(profs), len(done) + ntr)))
+                done += list(ex.map(one, ks))
+        dt = time.perf_counter() - t0
+        same = sum(a == b for a, b in zip(done, res))
+        ref.ps_free(hnd)
+        for h in hmms:
+            ref.lib.mp_hmm_destroy(h)
+        out["cpu_baseline"] = {"value": round(dt * 1e3 / len(done), 2), "unit": "ms per profile (wall, %d threads)" % ntr,
+                               "cores": ntr, "kind": "reference", "build_ms": round(cpu_build * 1e3, 1),
+                               "sample": "%d of 64 profiles, %.1f s wall" % (len(done), dt)}
+        out["identical_candidates"] = "%d of %d sampled profiles" % (same, len(done))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +328,7 @@ def main():
                                                                                   ps["speedup"], time.time() - t0))
         out["hmm"] = bench_hmm(log=log)
         log("[bench] hmm: %.1f GCUPS" % out["hmm"]["gcups"])
+        out["profile_search"] = bench_profile_search(cpu=not args.no_cpu, log=log)
     print(json.dumps(out), flush=True)
 
 

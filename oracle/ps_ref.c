@@ -95,3 +95,58 @@ int32_t or_ref_ps_kmer_thr(int32_t kmer, float sens, int32_t kmer_thr)
 	po.kmer = kmer, po.sens = sens, po.kmer_thr = kmer_thr;
 	return mp_ps_kmer_thr(&po);
 }
+
+/* build once, search many (timing the reference's search on a large genome) */
+typedef struct { mp_ntdb_t nt; mp_prescreen_t *ps; } or_ps_t;
+
+void *or_ref_ps_build(int32_t n_ctg, const uint8_t *nt4, const int64_t *ctg_off, const int64_t *ctg_len, int32_t kmer,
+		int32_t min_orf)
+{
+	or_ps_t *r = (or_ps_t*)calloc(1, sizeof(or_ps_t));
+	mp_psopt_t po;
+	int64_t i, tot = 0;
+	int32_t c;
+	for (c = 0; c < n_ctg; ++c) tot = ctg_off[c] + ctg_len[c] > tot? ctg_off[c] + ctg_len[c] : tot;
+	r->nt.seq = (uint8_t*)calloc(tot / 2 + 2, 1);
+	for (i = 0; i < tot; ++i) r->nt.seq[i >> 1] |= (nt4[i] & 0xf) << ((i & 1) * 4);
+	r->nt.n_ctg = r->nt.m_ctg = n_ctg, r->nt.l_seq = tot;
+	r->nt.ctg = (mp_ctg_t*)calloc(n_ctg, sizeof(mp_ctg_t));
+	for (c = 0; c < n_ctg; ++c) r->nt.ctg[c].off = ctg_off[c], r->nt.ctg[c].len = ctg_len[c], r->nt.ctg[c].name = (char*)"c";
+	mp_psopt_init(&po);
+	po.kmer = kmer, po.min_orf = min_orf;
+	r->ps = mp_prescreen_build(&r->nt, &po, 1);
+	return r;
+}
+
+int32_t or_ref_ps_query(void *handle, const mp_hmm_t *h, int32_t kmer, float sens, int32_t kmer_max, int32_t min_ungapped,
+		int32_t max_seqs, int32_t band, int32_t go, int32_t ge, double max_evalue, int32_t max, mp_cand1_t *out)
+{
+	or_ps_t *r = (or_ps_t*)handle;
+	mp_psopt_t po;
+	ps_batch_aux_t aux;
+	int32_t n;
+	mp_psopt_init(&po);
+	po.kmer = kmer, po.sens = sens, po.kmer_max = kmer_max, po.min_ungapped = min_ungapped, po.max_seqs = max_seqs;
+	po.band = band, po.go = go, po.ge = ge;
+	memset(&aux, 0, sizeof(aux));
+	aux.ps = r->ps, aux.po = &po, aux.max_evalue = max_evalue, aux.thr = mp_ps_kmer_thr(&po);
+	aux.hmm = (mp_hmm_t**)&h;
+	aux.scr = (ps_scratch_t*)calloc(1, sizeof(ps_scratch_t));
+	aux.res = (mp_cand1_t**)calloc(1, sizeof(mp_cand1_t*));
+	aux.n_res = (int32_t*)calloc(1, sizeof(int32_t));
+	ps_query_worker(&aux, 0, 0);
+	n = aux.n_res[0];
+	memcpy(out, aux.res[0], (size_t)(n < max? n : max) * sizeof(mp_cand1_t));
+	free(aux.res[0]);
+	ps_scratch_free(aux.scr);
+	free(aux.scr), free(aux.res), free(aux.n_res);
+	return n;
+}
+
+void or_ref_ps_free(void *handle)
+{
+	or_ps_t *r = (or_ps_t*)handle;
+	if (!r) return;
+	mp_prescreen_destroy(r->ps);
+	free(r->nt.ctg), free(r->nt.seq), free(r);
+}

@@ -9,6 +9,8 @@ Background frequencies are the published BLOSUM62 ones (Henikoff & Henikoff 1992
 conditionals P(b|a) are proportional to p_b * 2^(s_ab/2), i.e. the half-bit matrix read back
 as target frequencies, excluding b == a.
 """
+import math
+
 import numpy as np
 
 AA = "ARNDCQEGHILKMFPSTWYV"   # aa20 order of the C-ABI tables (first 20 codes)
@@ -219,12 +221,36 @@ def hmm_tables(em, tr):
     return e.astype(np.int16), score(tr).astype(np.int16)
 
 
-def profile_scan(n_prof=64, genome_len=10_000_000, win=3000, step=2700, n_plant=200, seed=5,
-                 min_m=100, max_m=800):
-    """profile-HMM scan workload: n_prof profiles (M log-uniform), a random genome (GC 0.45) with
-    n_plant profile-sampled hits (reverse-translated, either strand), cut into windows; each window
-    is scored in its six frames (forward offsets 0-2, reverse complement offsets 0-2), one region
-    per frame. Returns (profiles [(em, tr)], regions (bytes of base codes 0..3), offsets, lengths)."""
+# the log-odds background of the profile PSSM (HMMER's amino-acid frequencies, in HMMER column
+# order), as the reference's hmm.c:28-31 holds it
+HMMER_BG = [0.0787945, 0.0151600, 0.0535222, 0.0668298, 0.0397062, 0.0695071, 0.0229198, 0.0590092, 0.0594422,
+            0.0963728, 0.0237718, 0.0414386, 0.0482904, 0.0395639, 0.0540978, 0.0683364, 0.0540687, 0.0673417,
+            0.0114135, 0.0304133]
+
+
+def hmm_pssm(em, stop_pen=23):
+    """the L x 22 int8 PSSM that mp_hmm_read1 derives from hmm_text()'s record (hmm.c:69-81,
+    152-161): lrint of 2 (-x - ln bg) / ln 2 from each printed -ln p = x, clamped to +-64 (-64 for
+    '*'), in ns_tab_aa20 columns; stop column -stop_pen, X column -1"""
+    aa20 = "ARNDCQEGHILKMFPSTWYV*X"
+    P = np.zeros((len(em), 22), dtype=np.int8)
+    for k, row in enumerate(em):
+        for i, c in enumerate(HMM_ALPHA):
+            p = row[i]
+            if p <= 0:
+                v = -64
+            else:
+                x = float("%.5f" % (-np.log(p)))
+                lo = 2.0 * (-x - math.log(HMMER_BG[i])) / math.log(2.0)
+                v = int(np.rint(min(max(lo, -64.0), 64.0)))
+            P[k, aa20.index(c)] = v
+        P[k, 20], P[k, 21] = -stop_pen, -1
+    return P
+
+
+def profile_genome(n_prof=64, genome_len=10_000_000, n_plant=200, seed=5, min_m=100, max_m=800):
+    """n_prof profiles (M log-uniform) and a random genome (GC 0.45, base codes 0..3) carrying
+    n_plant profile-sampled hits, reverse-translated, on either strand"""
     rng = np.random.default_rng(seed)
     profs = []
     for _ in range(n_prof):
@@ -241,6 +267,16 @@ def profile_scan(n_prof=64, genome_len=10_000_000, win=3000, step=2700, n_plant=
             cds = (3 - cds)[::-1]
         pos = int(rng.integers(0, genome_len - len(cds)))
         g[pos:pos + len(cds)] = cds
+    return profs, g
+
+
+def profile_scan(n_prof=64, genome_len=10_000_000, win=3000, step=2700, n_plant=200, seed=5,
+                 min_m=100, max_m=800):
+    """profile-HMM scan workload: n_prof profiles (M log-uniform), a random genome (GC 0.45) with
+    n_plant profile-sampled hits (reverse-translated, either strand), cut into windows; each window
+    is scored in its six frames (forward offsets 0-2, reverse complement offsets 0-2), one region
+    per frame. Returns (profiles [(em, tr)], regions (bytes of base codes 0..3), offsets, lengths)."""
+    profs, g = profile_genome(n_prof, genome_len, n_plant, seed, min_m, max_m)
     starts = np.arange(0, max(1, genome_len - win + 1), step)
     rc = (3 - g)[::-1]
     regs = []

@@ -220,6 +220,11 @@ class Reference(_Lib):
                                        C.c_float, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                        C.c_int32, C.c_double, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_ref_ps_kmer_thr.argtypes = [C.c_int32, C.c_float, C.c_int32]
+        L.or_ref_ps_build.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]
+        L.or_ref_ps_build.restype = C.c_void_p
+        L.or_ref_ps_query.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_int32, C.c_int32, C.c_int32,
+                                      C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_void_p]
+        L.or_ref_ps_free.argtypes = [C.c_void_p]
         assert L.ns_make_tables(1) == 0
         self._opt_init = L.ns_opt_init
         self._opt_set_sp = L.ns_opt_set_sp
@@ -289,6 +294,26 @@ class Reference(_Lib):
                                       C.byref(n_orf), C.byref(n_aa))
         assert n <= cap
         return [out[i].tuple() for i in range(n)], (n_orf.value, n_aa.value)
+
+    def ps_build(self, genome, ctg_off, ctg_len, kmer=6, min_orf=30):
+        """the reference's mp_prescreen_build over one genome, kept for ps_query; free with ps_free"""
+        g = np.ascontiguousarray(genome, dtype=np.uint8)
+        off = np.ascontiguousarray(ctg_off, dtype=np.int64)
+        ln = np.ascontiguousarray(ctg_len, dtype=np.int64)
+        return self.lib.or_ref_ps_build(len(off), g.ctypes.data, off.ctypes.data, ln.ctypes.data, kmer, min_orf)
+
+    def ps_query(self, handle, h, kmer=6, sens=7.5, kmer_max=65536, min_ungapped=15, max_seqs=4000, band=32, go=11,
+                 ge=1, max_evalue=0.0, cap=65536):
+        """the reference's ps_query_worker for one profile over a ps_build index: candidate tuples"""
+        from protlocus_amd.capi import PsCand
+        out = (PsCand * cap)()
+        n = self.lib.or_ref_ps_query(handle, h, kmer, sens, kmer_max, min_ungapped, max_seqs, band, go, ge, max_evalue,
+                                     cap, out)
+        assert n <= cap
+        return [out[i].tuple() for i in range(n)]
+
+    def ps_free(self, handle):
+        self.lib.or_ref_ps_free(handle)
 
     def orfs(self, ctg_nt4, rev, min_orf):
         """ps_orf_worker on strand rev of a contig: [(start, codons)], residues"""

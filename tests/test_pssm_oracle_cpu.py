@@ -178,3 +178,40 @@ def test_ps_gapped_matches_reference_live():
         for o, d in zip(c["orfs"], c["diags"]):
             assert pssm_ref.ps_gapped(c["pssm"].ravel().tolist(), len(c["pssm"]), o.tolist(), d, c["band"], c["go"],
                                       c["ge"]) == ref.ps_gapped(c["pssm"], o, d, c["band"], c["go"], c["ge"])
+
+
+@pytest.mark.skipif(not nsffi.have_ref(), reason="the compiled reference (make -C oracle ref) is absent")
+def test_hmm_pssm_matches_reference():
+    """synth.hmm_pssm (the PSSM the GPU search is fed) equals mp_hmm_pssm of the parsed record"""
+    from protlocus_amd import synth
+    ref = nsffi.Reference()
+    rng = np.random.default_rng(404)
+    for M in (1, 7, 60, 333):
+        em, tr = synth.hmm_params(rng, M)
+        em[0][3] = 0.0                                  # a '*' emission (-64)
+        h = ref.hmm(synth.hmm_text(em, tr))
+        try:
+            assert (ref.pssm(h) == synth.hmm_pssm(em)).all()
+        finally:
+            ref.lib.mp_hmm_destroy(h)
+
+
+@pytest.mark.skipif(not nsffi.have_ref(), reason="the compiled reference (make -C oracle ref) is absent")
+def test_ps_build_query_matches_one_shot_search():
+    """the kept-index adaptor (one build, many queries) returns what the one-shot adaptor does"""
+    from cases import prescreen_case
+    ref = nsffi.Reference()
+    rng = np.random.default_rng(405)
+    for k in range(4):
+        c = prescreen_case(rng, k)
+        p = c["params"]
+        h = ref.hmm(c["text"])
+        hnd = ref.ps_build(c["genome"], c["ctg_off"], c["ctg_len"], kmer=p["kmer"], min_orf=p["min_orf"])
+        try:
+            q = {kk: v for kk, v in p.items() if kk != "min_orf"}
+            want, _ = ref.ps_search(c["genome"], c["ctg_off"], c["ctg_len"], h, **p)
+            assert ref.ps_query(hnd, h, **q) == want
+            assert ref.ps_query(hnd, h, **q) == want          # the index is read-only across queries
+        finally:
+            ref.ps_free(hnd)
+            ref.lib.mp_hmm_destroy(h)
