@@ -283,6 +283,13 @@ int64_t nsh_prescreen_size(const nsh_prescreen_t *ps, int32_t what);
 int32_t nsh_prescreen_search(nsh_prescreen_t *ps, const int8_t *pssm, int32_t L, int32_t thr, int32_t kmer_max,
 		int32_t min_ungapped, int32_t max_seqs, int32_t band, int32_t go, int32_t ge, double max_evalue,
 		nsh_ps_cand_t *out, int32_t max_out);
+/* The search of n_prof profiles at once (mp_prescreen_batch, prescreen.c:488-501, which runs
+ * ps_query_worker for every profile): pssm holds the profiles' L[i] x 22 rows one after another.
+ * The candidates come profile after profile, each profile's list as nsh_prescreen_search returns
+ * it; n_res[i] receives profile i's count. Returns the total (only max_out are written), or -1. */
+int64_t nsh_prescreen_search_batch(nsh_prescreen_t *ps, int32_t n_prof, const int8_t *pssm, const int32_t *L,
+		int32_t thr, int32_t kmer_max, int32_t min_ungapped, int32_t max_seqs, int32_t band, int32_t go, int32_t ge,
+		double max_evalue, nsh_ps_cand_t *out, int64_t max_out, int32_t *n_res);
 void nsh_prescreen_destroy(nsh_prescreen_t *ps);
 
 /* Learned chain-candidate prescreen: MLP 64 -> 256 -> 256 -> 1 (ReLU, logistic output), BF16

@@ -125,6 +125,10 @@ def lib():
                                            C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_void_p,
                                            C.c_int32]
         L.nsh_prescreen_search.restype = C.c_int32
+        L.nsh_prescreen_search_batch.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                                                 C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                                 C.c_void_p, C.c_int64, C.c_void_p]
+        L.nsh_prescreen_search_batch.restype = C.c_int64
         L.nsh_prescreen_destroy.argtypes = [C.c_void_p]
         L.nsh_device_synchronize.restype = C.c_int
         L.nsh_build_info.restype = C.c_char_p
@@ -437,6 +441,30 @@ class ProfileSearch:
             if n <= cap:
                 return [out[i].tuple() for i in range(n)]
             cap = n
+
+    def search_batch(self, pssms, thr, kmer_max=65536, min_ungapped=15, max_seqs=4000, band=32, go=11, ge=1,
+                     max_evalue=0.0):
+        """the candidates of every profile (a list of L x 22 int8 PSSMs) in one batched search"""
+        mats = [np.ascontiguousarray(m, dtype=np.int8).reshape(-1, 22) for m in pssms]
+        lens = np.array([m.shape[0] for m in mats], dtype=np.int32)
+        flat = np.ascontiguousarray(np.concatenate(mats) if mats else np.zeros((0, 22), np.int8))
+        n_res = np.zeros(max(1, len(mats)), dtype=np.int32)
+        cap = 4096
+        while True:
+            out = (PsCand * cap)()
+            n = lib().nsh_prescreen_search_batch(self.p, len(mats), flat.ctypes.data, lens.ctypes.data, thr, kmer_max,
+                                                 min_ungapped, max_seqs, band, go, ge, max_evalue, out, cap,
+                                                 n_res.ctypes.data)
+            if n < 0:
+                raise RuntimeError("nsh_prescreen_search_batch: " + error())
+            if n <= cap:
+                break
+            cap = n
+        res, k = [], 0
+        for c in n_res[:len(mats)]:
+            res.append([out[i].tuple() for i in range(k, k + int(c))])
+            k += int(c)
+        return res
 
     def close(self):
         if self.p:

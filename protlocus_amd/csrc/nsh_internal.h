@@ -43,9 +43,10 @@ int nsh_global_run(void *km, const char *ns, int32_t nl, const char *as, int32_t
 // ---- banded gapped stage of the prescreen search (ps_gapped.hip), device-resident items ----
 #include <hip/hip_runtime.h>
 namespace nsh_psg {
-void launch(int C, bool full, const int8_t *d_pssm, int32_t L, const uint8_t *d_aa, const int64_t *d_off,
-		const int32_t *d_len, const int32_t *d_diag, int32_t band, int32_t go, int32_t ge, const int32_t *d_items,
-		int32_t n_items, int32_t *d_out, hipStream_t s);
+// d_prow/d_plen (both null, or both set): per item, the first row of its PSSM in d_pssm and its length
+void launch(int C, bool full, const int8_t *d_pssm, int32_t L, const int64_t *d_prow, const int32_t *d_plen,
+		const uint8_t *d_aa, const int64_t *d_off, const int32_t *d_len, const int32_t *d_diag, int32_t band, int32_t go,
+		int32_t ge, const int32_t *d_items, int32_t n_items, int32_t *d_out, hipStream_t s);
 }
 
 #endif

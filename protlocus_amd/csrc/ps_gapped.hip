@@ -24,8 +24,19 @@ namespace {
 constexpr int NEG = -0x3fffffff;
 constexpr int CMAX = 16;
 
+// a DPP lane move: lanes whose source is outside the pattern (or whose row is masked off) keep old
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ int dpp(int old, int src)
+{
+	return __builtin_amdgcn_update_dpp(old, src, CTRL, ROWS, 0xf, false);
+}
+constexpr int kWaveShl1 = 0x130, kWaveShr1 = 0x138;        // lane i takes lane i + 1 / i - 1
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143;    // lane 15 / 31 of the row below, to rows 1,3 / 2,3
+
 struct GapArgs {
 	const int8_t *pssm; int32_t L;
+	const int64_t *prow; const int32_t *plen;   // per item: first PSSM row and length (null: pssm, L for all)
 	const uint8_t *aa; const int64_t *off; const int32_t *len, *diag;
 	int32_t band, go, ge;
 	const int32_t *items; int32_t n_items;   // item indices of this launch (one column count)
@@ -35,33 +46,63 @@ struct GapArgs {
 template <int C, bool FULL>
 __global__ void __launch_bounds__(256) ps_gapped_kernel(GapArgs a)
 {
+	__shared__ int8_t stage[4][64 * 22];   // per wave: 64 consecutive PSSM rows
 	const int Ln = threadIdx.x & 63;
-	const int w = blockIdx.x * 4 + (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const int wv = (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const int w = blockIdx.x * 4 + wv;
 	if (w >= a.n_items) return;   // wave-uniform
 	const int item = a.items[w];
 	const uint8_t *orf = a.aa + a.off[item];
 	const int n = a.len[item], dg = a.diag[item], band = a.band, go = a.go, ge = a.ge;
+	const int8_t *pssm = a.prow? a.pssm + a.prow[item] * 22 : a.pssm;
+	const int L = a.prow? a.plen[item] : a.L;
 	const int W = FULL? n : 2 * band + 1;   // columns held
-	int H[C], F[C];
-#pragma unroll
-	for (int k = 0; k < C; ++k) H[k] = 0, F[k] = NEG;
-	int best = 0;
+	int8_t *tab = stage[wv];
+	int H[C], F[C], res[C];
 	// rows whose band meets [0, n): all of them in full mode
-	const int i0 = FULL? 0 : std::max(0, -dg - band), i1 = FULL? a.L : std::min(a.L, n - dg + band);
+	const int i0 = FULL? 0 : std::max(0, -dg - band), i1 = FULL? L : std::min(L, n - dg + band);
+	{   // the residue under each slot at row i0 (0 off the ORF); banded slots then move one column per row
+		const int base = FULL? 0 : i0 + dg - band;
+#pragma unroll
+		for (int k = 0; k < C; ++k) {
+			const int t = base + Ln * C + k;
+			H[k] = 0, F[k] = NEG, res[k] = t >= 0 && t < n? orf[t] : 0;
+		}
+	}
+	int best = 0, inq = 0;
 	for (int i = i0; i < i1; ++i) {
-		const int8_t *row = a.pssm + (int64_t)i * 22;
+		const int j = (i - i0) & 63;
+		if (j == 0) {   // stage rows i..i+63; banded: the residue entering the last slot after each of them
+			const int r = i + Ln;
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_wave_barrier();
+			if (r < i1) {
+				const uint16_t *src = (const uint16_t*)(pssm + (int64_t)r * 22);
+				uint16_t *dst = (uint16_t*)(tab + Ln * 22);
+#pragma unroll
+				for (int b = 0; b < 11; ++b) dst[b] = src[b];
+			}
+			if (!FULL) {
+				const int tn = r + 1 + dg - band + 64 * C - 1;
+				inq = tn >= 0 && tn < n? orf[tn] : 0;
+			}
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_wave_barrier();
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		}
+		const int8_t *row = tab + j * 22;
 		const int base = FULL? 0 : i + dg - band;   // ORF column of slot 0
 		int hup[C], fup[C], hdg[C];
 		if (FULL) {   // vertical: same slot; diagonal: the slot to the left (column -1 holds 0)
-			const int left = __shfl_up(H[C - 1], 1);
+			const int left = dpp<kWaveShr1>(0, H[C - 1]);
 #pragma unroll
-			for (int k = 0; k < C; ++k) hup[k] = H[k], fup[k] = F[k], hdg[k] = k? H[k - 1] : (Ln? left : 0);
+			for (int k = 0; k < C; ++k) hup[k] = H[k], fup[k] = F[k], hdg[k] = k? H[k - 1] : left;
 		} else {      // vertical: the next slot (beyond the band: H = 0, F = -inf); diagonal: same slot
-			const int hn = __shfl_down(H[0], 1), fn = __shfl_down(F[0], 1);
+			const int hn = dpp<kWaveShl1>(0, H[0]), fn = dpp<kWaveShl1>(NEG, F[0]);
 #pragma unroll
 			for (int k = 0; k < C; ++k) {
-				hup[k] = k + 1 < C? H[k + 1] : (Ln < 63? hn : 0);
-				fup[k] = k + 1 < C? F[k + 1] : (Ln < 63? fn : NEG);
+				hup[k] = k + 1 < C? H[k + 1] : hn;
+				fup[k] = k + 1 < C? F[k + 1] : fn;
 				hdg[k] = H[k];
 			}
 		}
@@ -72,7 +113,7 @@ __global__ void __launch_bounds__(256) ps_gapped_kernel(GapArgs a)
 		for (int k = 0; k < C; ++k) {
 			const int u = Ln * C + k, t = base + u;
 			val[k] = u < W && t >= 0 && t < n;
-			const int s = val[k]? row[orf[t]] : 0;
+			const int s = val[k]? row[res[k]] : 0;
 			const int f = std::max(fup[k] - ge, hup[k] - go);
 			hp[k] = val[k]? std::max(std::max(hdg[k] + s, f), 0) : NEG;
 			fn[k] = val[k]? f : NEG;
@@ -80,21 +121,28 @@ __global__ void __launch_bounds__(256) ps_gapped_kernel(GapArgs a)
 			e = std::max(e - ge, hp[k] - go);
 		}
 		// incoming E of each lane: max over lanes m < L of the E leaving m, decayed by ge·C per lane
+		// (shifts 1-8 inside each row of 16 lanes, then lane 15 and lane 31 broadcast onwards)
 		int V = e;
 		const int geC = ge * C;
-#pragma unroll
-		for (int d = 1; d < 64; d <<= 1) {
-			const int o = __shfl_up(V, d);
-			if (Ln >= d) V = std::max(V, o - geC * d);
-		}
-		int inc = __shfl_up(V, 1);
-		if (Ln == 0) inc = NEG;
+		V = std::max(V, dpp<kRowShr1>(NEG, V) - geC);
+		V = std::max(V, dpp<kRowShr2>(NEG, V) - 2 * geC);
+		V = std::max(V, dpp<kRowShr4>(NEG, V) - 4 * geC);
+		V = std::max(V, dpp<kRowShr8>(NEG, V) - 8 * geC);
+		V = std::max(V, dpp<kRowBcast15, 0xa>(NEG, V) - geC * ((Ln & 15) + 1));
+		V = std::max(V, dpp<kRowBcast31, 0xc>(NEG, V) - geC * ((Ln & 31) + 1));
+		const int inc = dpp<kWaveShr1>(NEG, V);
 #pragma unroll
 		for (int k = 0; k < C; ++k) {
 			const int E = std::max(eloc[k], inc - ge * k);
 			const int h = val[k]? std::max(hp[k], E) : 0;
 			H[k] = h, F[k] = fn[k];
 			best = std::max(best, h);
+		}
+		if (!FULL) {   // next row: every slot takes its right neighbour's residue
+			const int nx = dpp<kWaveShl1>(__builtin_amdgcn_readlane(inq, j), res[0]);
+#pragma unroll
+			for (int k = 0; k + 1 < C; ++k) res[k] = res[k + 1];
+			res[C - 1] = nx;
 		}
 	}
 #pragma unroll
@@ -129,12 +177,12 @@ std::mutex g_mu;
 } // namespace
 
 // device-resident items (the prescreen search): one launch for the items of one column count
-void nsh_psg::launch(int C, bool full, const int8_t *d_pssm, int32_t L, const uint8_t *d_aa, const int64_t *d_off,
-		const int32_t *d_len, const int32_t *d_diag, int32_t band, int32_t go, int32_t ge, const int32_t *d_items,
-		int32_t n_items, int32_t *d_out, hipStream_t s)
+void nsh_psg::launch(int C, bool full, const int8_t *d_pssm, int32_t L, const int64_t *d_prow, const int32_t *d_plen,
+		const uint8_t *d_aa, const int64_t *d_off, const int32_t *d_len, const int32_t *d_diag, int32_t band, int32_t go,
+		int32_t ge, const int32_t *d_items, int32_t n_items, int32_t *d_out, hipStream_t s)
 {
 	GapArgs ga;
-	ga.pssm = d_pssm, ga.L = L, ga.aa = d_aa, ga.off = d_off, ga.len = d_len, ga.diag = d_diag;
+	ga.pssm = d_pssm, ga.L = L, ga.prow = d_prow, ga.plen = d_plen, ga.aa = d_aa, ga.off = d_off, ga.len = d_len, ga.diag = d_diag;
 	ga.band = band, ga.go = go, ga.ge = ge, ga.items = d_items, ga.n_items = n_items, ga.out = d_out;
 	::launch(C, ga, full, s);
 }
@@ -189,7 +237,7 @@ extern "C" int nsh_ps_gapped_batch(const int8_t *pssm, int32_t L, const uint8_t 
 		for (int c = 1; c <= CMAX; ++c) {
 			if (by_c[c].empty()) continue;
 			GapArgs ga;
-			ga.pssm = d_pssm, ga.L = L, ga.aa = d_aa, ga.off = d_off, ga.len = d_len, ga.diag = d_diag;
+			ga.pssm = d_pssm, ga.L = L, ga.prow = nullptr, ga.plen = nullptr, ga.aa = d_aa, ga.off = d_off, ga.len = d_len, ga.diag = d_diag;
 			ga.band = band, ga.go = go, ga.ge = ge;
 			ga.items = d_items + first, ga.n_items = (int32_t)by_c[c].size(), ga.out = d_out;
 			launch(c, ga, full, stream);

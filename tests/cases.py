@@ -287,3 +287,36 @@ def prescreen_case(rng, k):
                   max_evalue=[1e-3, 0.0, 10.0, 0.0][k % 4])
     return dict(genome=np.concatenate(contigs), ctg_off=ctg_off, ctg_len=ctg_len, text=synth.hmm_text(em, tr),
                 params=params)
+
+
+def prescreen_batch_case(rng, k):
+    """one genome (1-3 contigs) and 2-6 profiles for a batched search: most profiles planted on
+    either strand, one not planted, one shorter than every pattern span when k is odd; search
+    parameters as prescreen_case varies them"""
+    from protlocus_amd import synth
+    n_prof = int(rng.integers(2, 7))
+    profs = [synth.hmm_params(rng, int(rng.integers(40, 260))) for _ in range(n_prof)]
+    if k % 2:
+        profs.insert(int(rng.integers(0, n_prof + 1)), synth.hmm_params(rng, int(rng.integers(1, 9))))
+    comp = bytes.maketrans(b"ACGTN", b"TGCAN")
+    contigs = []
+    for _ in range(int(rng.integers(1, 4))):
+        parts = []
+        for _ in range(int(rng.integers(2, 6))):
+            parts.append(synth.rand_dna(rng, int(rng.integers(2000, 20000))))
+            em, tr = profs[int(rng.integers(0, max(1, len(profs) - 1)))]
+            if len(em) >= 20:
+                gene = synth.reverse_translate(rng, synth.hmm_sample_protein(rng, em, tr).encode())
+                parts.append(gene if rng.random() < 0.5 else gene[::-1].translate(comp))
+            if rng.random() < 0.3:
+                parts.append(b"N" * int(rng.integers(1, 30)))
+        parts.append(synth.rand_dna(rng, int(rng.integers(100, 5000))))
+        contigs.append(nt4_codes(b"".join(parts)))
+    ctg_len = np.array([len(c) for c in contigs], dtype=np.int64)
+    ctg_off = np.concatenate([[0], np.cumsum(ctg_len)[:-1]]).astype(np.int64)
+    params = dict(kmer=[6, 5, 7, 6][k % 4], min_orf=[30, 20, 30, 45][k % 4], sens=[7.5, 5.0, 7.5, 8.5][k % 4],
+                  kmer_max=[65536, 65536, 40, 65536][k % 4], min_ungapped=[15, 15, 30, 5][k % 4],
+                  max_seqs=[4000, 4000, 4000, 3][k % 4], band=[32, 0, 10, 32][k % 4],
+                  max_evalue=[1e-3, 0.0, 10.0, 0.0][k % 4])
+    return dict(genome=np.concatenate(contigs), ctg_off=ctg_off, ctg_len=ctg_len,
+                texts=[synth.hmm_text(em, tr) for em, tr in profs], params=params)

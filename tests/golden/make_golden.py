@@ -3,7 +3,7 @@
 reference (oracle/_ref/libnsref.so, built from the reference's own C sources by
 `make -C oracle ref`).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fixture ...]   (all of them by default)
 
 Outputs (inputs and expected outputs only):
   dp_cases.json.gz   ns_global_gs16b / gs32b / gs_auto: extension, CIGAR, score-only, PSSM
@@ -13,6 +13,7 @@ Outputs (inputs and expected outputs only):
   orf_cases.json.gz  ps_orf_worker (prescreen.c): six-frame ORFs of both strands
   ps_gapped_cases.json.gz  ps_gapped (prescreen.c): banded local affine scores
   prescreen_cases.json.gz  mp_prescreen_build + ps_query_worker (prescreen.c): profile search candidates
+  prescreen_batch_cases.json.gz  one mp_prescreen_build, ps_query_worker per profile of a batch
   tables.json        ns_tab_nt4 / ns_tab_aa20 / ns_tab_codon / ns_mat_blosum62
 """
 import base64
@@ -31,7 +32,7 @@ sys.path[:0] = [os.path.join(ROOT, "tests"), ROOT]
 
 import nsffi                                       # noqa: E402
 from cases import (OUTPUT_VARIANTS, chain_case, gapped_case, nt4_codes, orf_case, output_case,   # noqa: E402
-                   parse_cigar, prescreen_case,
+                   parse_cigar, prescreen_batch_case, prescreen_case,
                    planted_case, random_opt_kw, random_ss)
 from nsffi import NS_F_CIGAR, NS_F_EXT_LEFT, NS_F_EXT_RIGHT  # noqa: E402
 from protlocus_amd import synth                    # noqa: E402
@@ -211,6 +212,30 @@ def prescreen_cases(ref):
     return out
 
 
+def prescreen_batch_cases(ref):
+    """the reference's search of several profiles over one index (mp_prescreen_batch's per-profile
+    ps_query_worker lists) on cases.prescreen_batch_case genomes and profiles"""
+    import zlib
+    rng = np.random.default_rng(20260811)
+    out = []
+    for k in range(6):
+        c = prescreen_batch_case(rng, k)
+        p = c["params"]
+        thr = ref.lib.or_ref_ps_kmer_thr(p["kmer"], p["sens"], 0)
+        hnd = ref.ps_build(c["genome"], c["ctg_off"], c["ctg_len"], kmer=p["kmer"], min_orf=p["min_orf"])
+        pssms, cands = [], []
+        for text in c["texts"]:
+            h = ref.hmm(text)
+            pssms.append(B64(ref.pssm(h).tobytes()))
+            q = {x: p[x] for x in ("kmer", "sens", "kmer_max", "min_ungapped", "max_seqs", "band", "max_evalue")}
+            cands.append([list(x) for x in ref.ps_query(hnd, h, **q)])
+            ref.lib.mp_hmm_destroy(h)
+        ref.ps_free(hnd)
+        out.append({"genome": B64(zlib.compress(c["genome"].tobytes(), 9)), "ctg_off": c["ctg_off"].tolist(),
+                    "ctg_len": c["ctg_len"].tolist(), "pssms": pssms, "thr": thr, "params": p, "cands": cands})
+    return out
+
+
 def tables(ref):
     L = ref.lib
     get = lambda name, n, t=C.c_uint8: list((t * n).in_dll(L, name))
@@ -222,22 +247,19 @@ def main():
     if not nsffi.have_ref():
         raise SystemExit("build the reference first: make -C oracle ref")
     ref = nsffi.Reference()
-    with gzip.open(os.path.join(HERE, "dp_cases.json.gz"), "wt") as f:
-        json.dump(dp_cases(ref), f, separators=(",", ":"))
-    with gzip.open(os.path.join(HERE, "pssm_chain_cases.json.gz"), "wt") as f:
-        json.dump(pssm_chain_cases(ref), f, separators=(",", ":"))
-    with gzip.open(os.path.join(HERE, "orf_cases.json.gz"), "wt") as f:
-        json.dump(orf_cases(ref), f, separators=(",", ":"))
-    with gzip.open(os.path.join(HERE, "ps_gapped_cases.json.gz"), "wt") as f:
-        json.dump(ps_gapped_cases(ref), f, separators=(",", ":"))
-    with gzip.open(os.path.join(HERE, "prescreen_cases.json.gz"), "wt") as f:
-        json.dump(prescreen_cases(ref), f, separators=(",", ":"))
-    with gzip.open(os.path.join(HERE, "output_cases.json.gz"), "wt") as f:
-        json.dump(output_cases(ref), f, separators=(",", ":"))
-    with gzip.open(os.path.join(HERE, "hmm_cases.json.gz"), "wt") as f:
-        json.dump(hmm_cases(ref), f, separators=(",", ":"))
-    with open(os.path.join(HERE, "tables.json"), "w") as f:
-        json.dump(tables(ref), f, separators=(",", ":"))
+    gens = [("dp_cases", dp_cases), ("pssm_chain_cases", pssm_chain_cases), ("orf_cases", orf_cases),
+            ("ps_gapped_cases", ps_gapped_cases), ("prescreen_cases", prescreen_cases),
+            ("prescreen_batch_cases", prescreen_batch_cases), ("output_cases", output_cases),
+            ("hmm_cases", hmm_cases)]
+    want = set(sys.argv[1:])
+    for name, fn in gens:
+        if want and name not in want:
+            continue
+        with gzip.GzipFile(os.path.join(HERE, name + ".json.gz"), "wb", mtime=0) as f:
+            f.write(json.dumps(fn(ref), separators=(",", ":")).encode())
+    if not want or "tables" in want:
+        with open(os.path.join(HERE, "tables.json"), "w") as f:
+            json.dump(tables(ref), f, separators=(",", ":"))
     print("wrote golden vectors to", HERE)
 
 

@@ -292,3 +292,30 @@ def test_prescreen_search_golden(product):
             assert [list(x) for x in got] == c["cands"], (k, got, c["cands"])
         finally:
             ps.close()
+
+
+def test_prescreen_search_batch_golden(product):
+    """nsh_prescreen_search_batch against the reference's per-profile ps_query_worker lists over one
+    index (golden records): every profile's list in order, profiles shorter than the pattern span,
+    profiles without candidates, the k-mer cap; each profile searched alone agrees too"""
+    import base64
+    import zlib
+    from protlocus_amd import capi
+    with gzip.open(os.path.join(HERE, "golden", "prescreen_batch_cases.json.gz"), "rt") as f:
+        cases = json.load(f)
+    for k, c in enumerate(cases):
+        g = np.frombuffer(zlib.decompress(base64.b64decode(c["genome"])), dtype=np.uint8)
+        p = c["params"]
+        print("case", k, "genome", len(g), "profiles", len(c["pssms"]), "kmer", p["kmer"], flush=True)
+        pssms = [np.frombuffer(base64.b64decode(x), dtype=np.int8).reshape(-1, 22) for x in c["pssms"]]
+        kw = dict(kmer_max=p["kmer_max"], min_ungapped=p["min_ungapped"], max_seqs=p["max_seqs"], band=p["band"],
+                  max_evalue=p["max_evalue"])
+        ps = capi.ProfileSearch(g, c["ctg_off"], c["ctg_len"], kmer=p["kmer"], min_orf=p["min_orf"])
+        try:
+            got = ps.search_batch(pssms, c["thr"], **kw)
+            assert [[list(x) for x in r] for r in got] == c["cands"], k
+            for j, m in enumerate(pssms):
+                assert [list(x) for x in ps.search(m, c["thr"], **kw)] == c["cands"][j], (k, j)
+            assert ps.search_batch([], c["thr"], **kw) == []
+        finally:
+            ps.close()
