@@ -173,6 +173,7 @@ def bench_profile_search(cpu=True, seconds_cpu=10.0, log=None):
         t0 = time.perf_counter()
         res = ps.search_batch(pssms, thr, max_evalue=max_ev)
         batch_s = time.perf_counter() - t0
+        counts = ps.stats()
         t0 = time.perf_counter()
         single = [ps.search(P, thr, max_evalue=max_ev) for P in pssms]
         search_s = time.perf_counter() - t0
@@ -186,7 +187,7 @@ def bench_profile_search(cpu=True, seconds_cpu=10.0, log=None):
            "gpu_build_ms": round(build_s * 1e3, 1),
            "gpu_batch_ms_per_profile": round(batch_s * 1e3 / len(pssms), 3),
            "gpu_single_ms_per_profile": round(search_s * 1e3 / len(pssms), 3),
-           "candidates": int(sum(len(r) for r in res))}
+           "candidates": int(sum(len(r) for r in res)), "search_counts": counts}
     if log:
         log("[bench] profile search: build %.0f ms, %.3f ms per profile batched (%.3f one by one), %d candidates" % (
             build_s * 1e3, batch_s * 1e3 / len(pssms), search_s * 1e3 / len(pssms), out["candidates"]))

@@ -290,6 +290,10 @@ int32_t nsh_prescreen_search(nsh_prescreen_t *ps, const int8_t *pssm, int32_t L,
 int64_t nsh_prescreen_search_batch(nsh_prescreen_t *ps, int32_t n_prof, const int8_t *pssm, const int32_t *L,
 		int32_t thr, int32_t kmer_max, int32_t min_ungapped, int32_t max_seqs, int32_t band, int32_t go, int32_t ge,
 		double max_evalue, nsh_ps_cand_t *out, int64_t max_out, int32_t *n_res);
+/* Counters of the last search (the figures mp_prescreen_batch logs, prescreen.c:503-525): st[0]
+ * query windows, st[1] k-mers generated, st[2] index hits, st[3] double-hit diagonals, st[4] ORFs
+ * whose best diagonal reaches min_ungapped, st[5] aligned (banded gapped), st[6] kept. 1 or 0. */
+int nsh_prescreen_stats(nsh_prescreen_t *ps, int64_t *st);
 void nsh_prescreen_destroy(nsh_prescreen_t *ps);
 
 /* Learned chain-candidate prescreen: MLP 64 -> 256 -> 256 -> 1 (ReLU, logistic output), BF16

@@ -119,7 +119,8 @@ void *or_ref_ps_build(int32_t n_ctg, const uint8_t *nt4, const int64_t *ctg_off,
 }
 
 int32_t or_ref_ps_query(void *handle, const mp_hmm_t *h, int32_t kmer, float sens, int32_t kmer_max, int32_t min_ungapped,
-		int32_t max_seqs, int32_t band, int32_t go, int32_t ge, double max_evalue, int32_t max, mp_cand1_t *out)
+		int32_t max_seqs, int32_t band, int32_t go, int32_t ge, double max_evalue, int32_t max, mp_cand1_t *out,
+		int64_t *stats)
 {
 	or_ps_t *r = (or_ps_t*)handle;
 	mp_psopt_t po;
@@ -137,6 +138,11 @@ int32_t or_ref_ps_query(void *handle, const mp_hmm_t *h, int32_t kmer, float sen
 	ps_query_worker(&aux, 0, 0);
 	n = aux.n_res[0];
 	memcpy(out, aux.res[0], (size_t)(n < max? n : max) * sizeof(mp_cand1_t));
+	if (stats) {
+		const ps_scratch_t *c = aux.scr;
+		stats[0] = c->n_win, stats[1] = c->n_kmer, stats[2] = c->n_lookup, stats[3] = c->n_dbl;
+		stats[4] = c->n_orf_scored, stats[5] = c->n_aligned, stats[6] = c->n_kept;
+	}
 	free(aux.res[0]);
 	ps_scratch_free(aux.scr);
 	free(aux.scr), free(aux.res), free(aux.n_res);

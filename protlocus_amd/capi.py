@@ -129,6 +129,7 @@ def lib():
                                                  C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                                  C.c_void_p, C.c_int64, C.c_void_p]
         L.nsh_prescreen_search_batch.restype = C.c_int64
+        L.nsh_prescreen_stats.argtypes = [C.c_void_p, C.c_void_p]
         L.nsh_prescreen_destroy.argtypes = [C.c_void_p]
         L.nsh_device_synchronize.restype = C.c_int
         L.nsh_build_info.restype = C.c_char_p
@@ -465,6 +466,15 @@ class ProfileSearch:
             res.append([out[i].tuple() for i in range(k, k + int(c))])
             k += int(c)
         return res
+
+    def stats(self):
+        """counters of the last search: windows, k-mers, index hits, double-hit diagonals, ORFs >=
+        min_ungapped, aligned, kept"""
+        st = np.zeros(7, dtype=np.int64)
+        if not lib().nsh_prescreen_stats(self.p, st.ctypes.data):
+            raise RuntimeError("nsh_prescreen_stats: " + error())
+        keys = ("windows", "kmers", "index_hits", "double_hit_diagonals", "orfs_scored", "aligned", "kept")
+        return dict(zip(keys, (int(v) for v in st)))
 
     def close(self):
         if self.p:

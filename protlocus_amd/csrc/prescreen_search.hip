@@ -352,6 +352,8 @@ struct nsh_prescreen_s {
 	size_t ws_bytes[32] = {};
 	void *pin[4] = {};          // pinned host staging of the search's transfers
 	size_t pin_bytes[4] = {};
+	int64_t stats[7] = {};      // last search: windows, k-mers, index hits, double-hit diagonals,
+	                            // ORFs >= min_ungapped, aligned, kept
 };
 
 #define PSOK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { nsh_set_error("prescreen: %s: %s", #x, hipGetErrorString(e_)); goto fail; } } while (0)
@@ -555,6 +557,11 @@ int search_chunk(nsh_prescreen_t *ps, const SearchIn &in, int32_t p0, int32_t p1
 	const int64_t W = (int64_t)wrow.size();
 	if (W == 0) return 1;
 	if (W > kMaxItems) return 2;
+	int64_t st[7] = { W, 0, 0, 0, 0, 0, 0 };   // added to the index's statistics when the chunk completes
+	struct Commit {
+		nsh_prescreen_t *ps; int64_t *st; bool on = true;
+		~Commit() { if (on) for (int i = 0; i < 7; ++i) ps->stats[i] += st[i]; }
+	} commit = { ps, st };
 	// sort-key layout: profile | ORF | diagonal + maxL (the diagonal lies in (-maxL, max_orf_len))
 	const int D = std::max(1, bits_for((uint64_t)maxL + (uint64_t)ps->max_orf_len));
 	const int O = std::max(1, bits_for((uint64_t)(ps->n_orf - 1)));
@@ -621,9 +628,10 @@ int search_chunk(nsh_prescreen_t *ps, const SearchIn &in, int32_t p0, int32_t p1
 		int64_t tot = 0;
 		WSOK(read_i64(ps, d_E + n, &tot));
 		nsh_count_launch(4);
+		if (t + 1 == k) st[1] = tot;
 		if (tot == 0) return 1;
 		if (t + 1 < k) {   // the next frontier
-			if (tot > kMaxItems) return 2;
+			if (tot > kMaxItems) { commit.on = false; return 2; }
 			const int nx = cur ^ 1;
 			WSOK(wsget(ps, fw[nx], tot, &la.nwin));
 			WSOK(wsget(ps, fc[nx], tot, &la.ncode));
@@ -645,8 +653,9 @@ int search_chunk(nsh_prescreen_t *ps, const SearchIn &in, int32_t p0, int32_t p1
 		WSOK(hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp_bytes, d_hc, d_ho, (int)(n + 1), s));
 		WSOK(read_i64(ps, d_ho + n, &n_hit));
 		nsh_count_launch(2);
+		st[2] = n_hit;
 		if (n_hit == 0) return 1;
-		if (n_hit > kMaxItems) return 2;
+		if (n_hit > kMaxItems) { commit.on = false; return 2; }
 		uint64_t *d_hit;
 		WSOK(wsget(ps, W_HIT, n_hit, &d_hit));
 		la.E = d_ho, la.hit = d_hit;
@@ -677,6 +686,7 @@ int search_chunk(nsh_prescreen_t *ps, const SearchIn &in, int32_t p0, int32_t p1
 	WSOK(hipcub::DeviceSelect::Flagged(d_tmp, tmp_bytes, d_hit2, d_flag, d_dg, d_nsel, (int)n_hit, s));
 	WSOK(read_i64(ps, d_nsel, &n_dg));
 	nsh_count_launch(3);
+	st[3] = n_dg;
 	if (n_dg == 0) return 1;
 	// ungapped scores, the best diagonal per (profile, ORF), the kept ones packed for the host
 	int32_t *d_ung, *d_qs, *d_qe, *d_sel;
@@ -697,6 +707,7 @@ int search_chunk(nsh_prescreen_t *ps, const SearchIn &in, int32_t p0, int32_t p1
 			d_nsel, (int)n_dg, s));
 	WSOK(read_i64(ps, d_nsel, &n_keep));
 	nsh_count_launch(3);
+	st[4] = n_keep;
 	if (n_keep == 0) return 1;
 	WSOK(wsget(ps, W_KEPT, n_keep, &d_kept));
 	hipLaunchKernelGGL(gather_kernel, dim3(blocks(n_keep)), dim3(256), 0, s, d_sel, n_keep, d_dg, d_ung, d_qs, d_qe,
@@ -726,6 +737,7 @@ int search_chunk(nsh_prescreen_t *ps, const SearchIn &in, int32_t p0, int32_t p1
 		}
 	}
 	const int32_t nc = (int32_t)cand.size();
+	st[5] = nc;
 	if (nc == 0) return 1;
 	// banded gapped scores, grouped by the kernel's column count; one pinned block and one copy:
 	// ORF residue offsets, PSSM rows (int64) | ORF lengths, diagonals, profile lengths, items (int32)
@@ -780,6 +792,7 @@ int search_chunk(nsh_prescreen_t *ps, const SearchIn &in, int32_t p0, int32_t p1
 		r.vid = ps->vid[c.orf], r.vs = vs, r.ve = vs + 3 * (int64_t)ps->aa_len[c.orf], r.ev = ev;
 		r.aqs = c.qs, r.aqe = c.qe, r.asc = c.ung, r.avs = vs + 3 * (int64_t)(c.qs + c.diag);
 		res[c.prof].push_back(r);
+		++st[6];
 	}
 	return 1;
 }
@@ -802,8 +815,9 @@ extern "C" int64_t nsh_prescreen_search_batch(nsh_prescreen_t *ps, int32_t n_pro
 	}
 	if (prow[n_prof] > 0 && !pssm) { nsh_set_error("prescreen: bad search arguments"); return -1; }
 	for (int32_t p = 0; p < n_prof; ++p) n_res[p] = 0;
-	if (ps->n_orf == 0 || n_prof == 0) return 0;
 	std::lock_guard<std::mutex> lk(ps->mu);
+	for (int64_t &v : ps->stats) v = 0;
+	if (ps->n_orf == 0 || n_prof == 0) return 0;
 	SearchIn in = { pssm, L, prow.data(), thr, kmer_max, min_ungapped, max_seqs, band, go, ge, max_evalue };
 	int64_t total = 0;
 	int32_t p0 = 0, chunk = std::min<int32_t>(n_prof, 1024);
@@ -828,6 +842,14 @@ extern "C" int64_t nsh_prescreen_search_batch(nsh_prescreen_t *ps, int32_t n_pro
 		p0 = p1;
 	}
 	return total;
+}
+
+extern "C" int nsh_prescreen_stats(nsh_prescreen_t *ps, int64_t *st)
+{
+	if (!ps || !st) { nsh_set_error("prescreen: bad arguments"); return 0; }
+	std::lock_guard<std::mutex> lk(ps->mu);
+	for (int i = 0; i < 7; ++i) st[i] = ps->stats[i];
+	return 1;
 }
 
 extern "C" int32_t nsh_prescreen_search(nsh_prescreen_t *ps, const int8_t *pssm, int32_t L, int32_t thr, int32_t kmer_max,

@@ -223,16 +223,19 @@ def prescreen_batch_cases(ref):
         p = c["params"]
         thr = ref.lib.or_ref_ps_kmer_thr(p["kmer"], p["sens"], 0)
         hnd = ref.ps_build(c["genome"], c["ctg_off"], c["ctg_len"], kmer=p["kmer"], min_orf=p["min_orf"])
-        pssms, cands = [], []
+        pssms, cands, stats = [], [], []
         for text in c["texts"]:
             h = ref.hmm(text)
             pssms.append(B64(ref.pssm(h).tobytes()))
             q = {x: p[x] for x in ("kmer", "sens", "kmer_max", "min_ungapped", "max_seqs", "band", "max_evalue")}
-            cands.append([list(x) for x in ref.ps_query(hnd, h, **q)])
+            st = []
+            cands.append([list(x) for x in ref.ps_query(hnd, h, stats=st, **q)])
+            stats.append(st)
             ref.lib.mp_hmm_destroy(h)
         ref.ps_free(hnd)
         out.append({"genome": B64(zlib.compress(c["genome"].tobytes(), 9)), "ctg_off": c["ctg_off"].tolist(),
-                    "ctg_len": c["ctg_len"].tolist(), "pssms": pssms, "thr": thr, "params": p, "cands": cands})
+                    "ctg_len": c["ctg_len"].tolist(), "pssms": pssms, "thr": thr, "params": p, "cands": cands,
+                    "stats": stats})
     return out
 
 

@@ -223,7 +223,7 @@ class Reference(_Lib):
         L.or_ref_ps_build.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]
         L.or_ref_ps_build.restype = C.c_void_p
         L.or_ref_ps_query.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_int32, C.c_int32, C.c_int32,
-                                      C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_void_p]
+                                      C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_void_p, C.c_void_p]
         L.or_ref_ps_free.argtypes = [C.c_void_p]
         assert L.ns_make_tables(1) == 0
         self._opt_init = L.ns_opt_init
@@ -303,13 +303,17 @@ class Reference(_Lib):
         return self.lib.or_ref_ps_build(len(off), g.ctypes.data, off.ctypes.data, ln.ctypes.data, kmer, min_orf)
 
     def ps_query(self, handle, h, kmer=6, sens=7.5, kmer_max=65536, min_ungapped=15, max_seqs=4000, band=32, go=11,
-                 ge=1, max_evalue=0.0, cap=65536):
-        """the reference's ps_query_worker for one profile over a ps_build index: candidate tuples"""
+                 ge=1, max_evalue=0.0, cap=65536, stats=None):
+        """the reference's ps_query_worker for one profile over a ps_build index: candidate tuples;
+        stats (a list) receives the worker's 7 counters when given"""
         from protlocus_amd.capi import PsCand
         out = (PsCand * cap)()
+        st = (C.c_int64 * 7)()
         n = self.lib.or_ref_ps_query(handle, h, kmer, sens, kmer_max, min_ungapped, max_seqs, band, go, ge, max_evalue,
-                                     cap, out)
+                                     cap, out, st)
         assert n <= cap
+        if stats is not None:
+            stats[:] = list(st)
         return [out[i].tuple() for i in range(n)]
 
     def ps_free(self, handle):

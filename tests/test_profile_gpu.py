@@ -297,7 +297,8 @@ def test_prescreen_search_golden(product):
 def test_prescreen_search_batch_golden(product):
     """nsh_prescreen_search_batch against the reference's per-profile ps_query_worker lists over one
     index (golden records): every profile's list in order, profiles shorter than the pattern span,
-    profiles without candidates, the k-mer cap; each profile searched alone agrees too"""
+    profiles without candidates, the k-mer cap, the search counters; each profile searched alone
+    agrees too"""
     import base64
     import zlib
     from protlocus_amd import capi
@@ -314,6 +315,8 @@ def test_prescreen_search_batch_golden(product):
         try:
             got = ps.search_batch(pssms, c["thr"], **kw)
             assert [[list(x) for x in r] for r in got] == c["cands"], k
+            # the worker's counters, summed over the batch (prescreen.c:503-525)
+            assert list(ps.stats().values()) == [sum(s[i] for s in c["stats"]) for i in range(7)], k
             for j, m in enumerate(pssms):
                 assert [list(x) for x in ps.search(m, c["thr"], **kw)] == c["cands"][j], (k, j)
             assert ps.search_batch([], c["thr"], **kw) == []
