@@ -135,6 +135,35 @@ int ns_gpu_gapless_score_batch(const uint8_t *packed_nt, int64_t packed_nt_bases
 		const int8_t *score_matrix, int32_t asize, const uint8_t *codon,
 		int32_t *scores);
 
+/* ---- --gpu-prefilter candidate selection on the device ----
+ * nsh_gapless_calib_t has mp_gapless_calib_t's layout (mppriv.h:97-101).
+ * nsh_gapless_select_batch: what mp_gpu_prefilter_batch does with the scores of one mini-batch
+ * (map.c:1075-1083): with adaptive != 0, mp_gapless_calibrate over the batch (hit.c:149-189) and,
+ * per query, K = mp_gapless_adaptive_k(n, scores, qlen, min(top_k, 4), cal) (hit.c:195-215);
+ * otherwise K = top_k. Then mp_select_gapless(K, min_per_vid) (hit.c:217-263) per query, as
+ * mp_gpu_prefilter_select runs it (map.c:507-521). Query q owns candidates
+ * cand_off[q]..cand_off[q+1]-1 (at most 8192), with their gapless scores, chain scores (chn_sc),
+ * vid and vs. keep[i] = 1 for the survivors. The caller compacts its regions in order, sets
+ * chn_sc_ungap = scores[i] on all of them, then runs mp_sort_reg and mp_set_parent as map.c:523-524
+ * does. k_out[q] (optional) receives K, 0 for a query without candidates; cal (optional) receives
+ * the calibration, zero unless adaptive. Requires top_k >= 1 and 0 <= min_per_vid <= top_k
+ * (options.c:174-181). Returns 1 on success, 0 on error.
+ * nsh_gapless_prefilter_batch: ns_gpu_gapless_score_batch and the selection in one call. The
+ * scores stay on the device between the two; they are also returned in scores. */
+typedef struct { int32_t noise, n_query; float dup_frac; } nsh_gapless_calib_t;
+int nsh_gapless_select_batch(int32_t n_query, const int32_t *cand_off, const int32_t *qlen, const int32_t *scores,
+		const int32_t *chn_sc, const uint32_t *vid, const int64_t *vs, int32_t top_k, int32_t min_per_vid,
+		int32_t adaptive, uint8_t *keep, int32_t *k_out, nsh_gapless_calib_t *cal);
+int nsh_gapless_prefilter_batch(const uint8_t *packed_nt, int64_t packed_nt_bases,
+		const uint8_t *queries, int64_t query_aa,
+		const ns_gpu_gapless_candidate_t *candidates, int32_t n_candidates,
+		const ns_gpu_gapless_anchor_t *anchors, int64_t n_anchors,
+		int32_t block_width, int32_t kmer, int32_t xdrop,
+		const int8_t *score_matrix, int32_t asize, const uint8_t *codon,
+		int32_t n_query, const int32_t *cand_off, const int32_t *qlen, const int32_t *chn_sc, const uint32_t *vid,
+		const int64_t *vs, int32_t top_k, int32_t min_per_vid, int32_t adaptive,
+		int32_t *scores, uint8_t *keep, int32_t *k_out, nsh_gapless_calib_t *cal);
+
 /* ---- profile-HMM Viterbi on the GPU (mppriv.h:128-134; hmm-gpu.c:282-352) ----
  * The profile object belongs to the caller's hmm.c (accessors mp_hmm_length, mp_hmm_emissions,
  * mp_hmm_transitions, mp_hmm_gpu_slot); its device copy is cached in gpu slot 0.

@@ -130,6 +130,11 @@ def lib():
                                                  C.c_void_p, C.c_int64, C.c_void_p]
         L.nsh_prescreen_search_batch.restype = C.c_int64
         L.nsh_prescreen_stats.argtypes = [C.c_void_p, C.c_void_p]
+        L.nsh_gapless_select_batch.argtypes = [C.c_int32] + [C.c_void_p] * 6 + [C.c_int32] * 3 + [C.c_void_p] * 3
+        L.nsh_gapless_prefilter_batch.argtypes = ([C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32,
+                                                   C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                                   C.c_int32, C.c_void_p, C.c_int32] + [C.c_void_p] * 5 +
+                                                  [C.c_int32] * 3 + [C.c_void_p] * 4)
         L.nsh_prescreen_destroy.argtypes = [C.c_void_p]
         L.nsh_device_synchronize.restype = C.c_int
         L.nsh_build_info.restype = C.c_char_p
@@ -404,6 +409,52 @@ def ps_gapped_batch(pssm, orfs, diags, band, go, ge):
                                  ln.ctypes.data, dg.ctypes.data, band, go, ge, sw.ctypes.data):
         raise RuntimeError("nsh_ps_gapped_batch: " + error())
     return sw[:len(orfs)].tolist()
+
+
+class GaplessCalib(C.Structure):
+    """nsh_gapless_calib_t (mp_gapless_calib_t's layout)"""
+    _fields_ = [("noise", C.c_int32), ("n_query", C.c_int32), ("dup_frac", C.c_float)]
+
+
+def _sel_arrays(cand_off, qlen, chn, vid, vs):
+    return [np.ascontiguousarray(x, dtype=t) for x, t in ((cand_off, np.int32), (qlen, np.int32), (chn, np.int32),
+                                                           (vid, np.uint32), (vs, np.int64))]
+
+
+def gapless_select(cand_off, qlen, scores, chn, vid, vs, top_k, min_per_vid, adaptive):
+    """the --gpu-prefilter selection of a mini-batch (nsh_gapless_select_batch): keep flags, K per
+    query, (noise, n_query, dup_frac)"""
+    off, ql, ch, vi, v = _sel_arrays(cand_off, qlen, chn, vid, vs)
+    sc = np.ascontiguousarray(scores, dtype=np.int32)
+    n_q, n = len(off) - 1, int(off[-1])
+    keep, ks = np.zeros(max(1, n), np.uint8), np.zeros(max(1, n_q), np.int32)
+    cal = GaplessCalib()
+    if not lib().nsh_gapless_select_batch(n_q, off.ctypes.data, ql.ctypes.data, sc.ctypes.data, ch.ctypes.data,
+                                          vi.ctypes.data, v.ctypes.data, top_k, min_per_vid, adaptive,
+                                          keep.ctypes.data, ks.ctypes.data, C.byref(cal)):
+        raise RuntimeError("nsh_gapless_select_batch: " + error())
+    return keep[:n].tolist(), ks[:n_q].tolist(), (cal.noise, cal.n_query, cal.dup_frac)
+
+
+def gapless_prefilter(packed_nt, n_bases, queries, cands, anchors, block_width, kmer, xdrop, mat, codon,
+                      cand_off, qlen, chn, vid, vs, top_k, min_per_vid, adaptive):
+    """gapless scores and the selection in one call (nsh_gapless_prefilter_batch): scores, keep
+    flags, K per query, calibration. cands / anchors: ctypes arrays of the ns_gpu_gapless_* structs"""
+    off, ql, ch, vi, v = _sel_arrays(cand_off, qlen, chn, vid, vs)
+    m = np.ascontiguousarray(mat, dtype=np.int8)
+    n_q, n = len(off) - 1, len(cands)
+    scores, keep, ks = np.zeros(n, np.int32), np.zeros(n, np.uint8), np.zeros(n_q, np.int32)
+    cal = GaplessCalib()
+    q = bytes(queries)
+    cod = bytes(codon)
+    asize = int(round(len(m) ** 0.5))
+    if not lib().nsh_gapless_prefilter_batch(packed_nt, n_bases, q, len(q), cands, n, anchors, len(anchors),
+                                             block_width, kmer, xdrop, m.ctypes.data, asize, cod, n_q,
+                                             off.ctypes.data, ql.ctypes.data, ch.ctypes.data, vi.ctypes.data,
+                                             v.ctypes.data, top_k, min_per_vid, adaptive, scores.ctypes.data,
+                                             keep.ctypes.data, ks.ctypes.data, C.byref(cal)):
+        raise RuntimeError("nsh_gapless_prefilter_batch: " + error())
+    return scores.tolist(), keep.tolist(), ks.tolist(), (cal.noise, cal.n_query, cal.dup_frac)
 
 
 class PsCand(C.Structure):

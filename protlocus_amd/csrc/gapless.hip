@@ -216,6 +216,89 @@ extern "C" void ns_gpu_gapless_reset_target(void)
 
 #define GLOK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { nsh_set_error("gapless: %s: %s", #x, hipGetErrorString(e_)); return 0; } } while (0)
 
+namespace {
+
+// the arguments of one scoring batch; every device access stays inside the buffers: residue
+// codes index the matrix, contigs lie in the genome, queries and anchors in their arrays
+struct ScoreIn {
+	const uint8_t *packed_nt; int64_t packed_nt_bases;
+	const uint8_t *queries; int64_t query_aa;
+	const ns_gpu_gapless_candidate_t *candidates; int32_t n_candidates;
+	const ns_gpu_gapless_anchor_t *anchors; int64_t n_anchors;
+	int32_t block_width, kmer, xdrop;
+	const int8_t *score_matrix; int32_t asize; const uint8_t *codon;
+	uint64_t n_anch_used = 0;
+};
+
+int check_inputs(ScoreIn &in)
+{
+	if (!in.packed_nt || in.packed_nt_bases < 1 || !in.queries || in.query_aa < 1 || !in.candidates ||
+			in.n_candidates < 1 || !in.anchors || in.n_anchors < 1 || in.block_width < 1 || in.kmer < 1 ||
+			in.xdrop < 0 || !in.score_matrix || in.asize < 1 || !in.codon)
+		return 0;
+	if (!nsh_ensure_device()) return 0;
+	if (in.asize < 21) { nsh_set_error("gapless: asize %d < 21 (stop codons score row 20)", in.asize); return 0; }
+	for (int t = 0; t < 64; ++t) if (in.codon[t] >= in.asize) { nsh_set_error("gapless: codon table out of range"); return 0; }
+	for (int64_t k = 0; k < in.query_aa; ++k)
+		if (in.queries[k] >= in.asize) { nsh_set_error("gapless: query residue %lld out of range", (long long)k); return 0; }
+	in.n_anch_used = 0;
+	for (int32_t i = 0; i < in.n_candidates; ++i) {
+		const ns_gpu_gapless_candidate_t &c = in.candidates[i];
+		const int64_t na = c.anchor_count < kMaxAnchors? c.anchor_count : kMaxAnchors;
+		if (c.target_offset < 0 || c.target_length < 0 || c.target_offset + c.target_length > in.packed_nt_bases ||
+				c.query_offset < 0 || c.query_length < 0 || c.query_offset + c.query_length > in.query_aa ||
+				(na > 0 && (c.anchor_offset < 0 || c.anchor_offset + na > in.n_anchors))) {
+			nsh_set_error("gapless: candidate %d out of range", i);
+			return 0;
+		}
+		in.n_anch_used += na > 0? na : 0;
+	}
+	return 1;
+}
+
+// uploads the batch and queues the scoring kernel on g_stream; the scores stay in g_buf[B_SCORE].
+// The caller holds g_mu. Returns the upload's seconds, or a negative value on error.
+double queue_scores(const ScoreIn &in)
+{
+	const size_t sz[B_COUNT] = { (size_t)(in.packed_nt_bases + 1) >> 1, (size_t)in.query_aa,
+		(size_t)in.n_candidates * sizeof(*in.candidates), (size_t)in.n_anchors * sizeof(*in.anchors),
+		(size_t)in.asize * in.asize, 64, (size_t)in.n_candidates * sizeof(int32_t) };
+	for (int b = 0; b < B_COUNT; ++b) if (!ensure(g_buf[b], sz[b])) return -1.0;
+#define QSOK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { nsh_set_error("gapless: %s: %s", #x, hipGetErrorString(e_)); return -1.0; } } while (0)
+	if (!g_stream) QSOK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+	const auto t0 = std::chrono::steady_clock::now();
+	if (g_target_src != in.packed_nt || g_target_bases != in.packed_nt_bases) {
+		g_target_src = nullptr;
+		QSOK(hipMemcpyAsync(g_buf[B_TARGET].p, in.packed_nt, sz[B_TARGET], hipMemcpyHostToDevice, g_stream));
+	}
+	QSOK(hipMemcpyAsync(g_buf[B_QUERY].p, in.queries, sz[B_QUERY], hipMemcpyHostToDevice, g_stream));
+	QSOK(hipMemcpyAsync(g_buf[B_CAND].p, in.candidates, sz[B_CAND], hipMemcpyHostToDevice, g_stream));
+	QSOK(hipMemcpyAsync(g_buf[B_ANCH].p, in.anchors, sz[B_ANCH], hipMemcpyHostToDevice, g_stream));
+	QSOK(hipMemcpyAsync(g_buf[B_MAT].p, in.score_matrix, sz[B_MAT], hipMemcpyHostToDevice, g_stream));
+	QSOK(hipMemcpyAsync(g_buf[B_CODON].p, in.codon, 64, hipMemcpyHostToDevice, g_stream));
+	QSOK(hipStreamSynchronize(g_stream));
+	g_target_src = in.packed_nt, g_target_bases = in.packed_nt_bases;
+	const double up = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+	GlArgs a;
+	a.pk = (const uint8_t*)g_buf[B_TARGET].p, a.q = (const uint8_t*)g_buf[B_QUERY].p;
+	a.cand = (const ns_gpu_gapless_candidate_t*)g_buf[B_CAND].p, a.anch = (const ns_gpu_gapless_anchor_t*)g_buf[B_ANCH].p;
+	a.n = in.n_candidates, a.bw = in.block_width, a.kmer = in.kmer, a.xdrop = in.xdrop, a.asize = in.asize;
+	a.mat = (const int8_t*)g_buf[B_MAT].p, a.codon = (const uint8_t*)g_buf[B_CODON].p, a.score = (int32_t*)g_buf[B_SCORE].p;
+	hipLaunchKernelGGL(gapless_kernel, dim3((in.n_candidates + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, g_stream, a);
+	QSOK(hipGetLastError());
+#undef QSOK
+	nsh_count_launch(1);
+	++g_batches, g_cands += in.n_candidates, g_anchors += in.n_anch_used;
+	g_seedpos += in.n_anch_used * (uint64_t)in.block_width;
+	return up;
+}
+
+// the selection inputs of the fused call
+enum { S_OFF, S_QLEN, S_CHN, S_VID, S_VS, S_KEEP, S_K, S_COUNT };
+DevBuf g_sel[S_COUNT];
+
+} // namespace
+
 extern "C" int ns_gpu_gapless_score_batch(const uint8_t *packed_nt, int64_t packed_nt_bases,
 		const uint8_t *queries, int64_t query_aa,
 		const ns_gpu_gapless_candidate_t *candidates, int32_t n_candidates,
@@ -224,63 +307,71 @@ extern "C" int ns_gpu_gapless_score_batch(const uint8_t *packed_nt, int64_t pack
 		const int8_t *score_matrix, int32_t asize, const uint8_t *codon,
 		int32_t *scores)
 {
-	if (!packed_nt || packed_nt_bases < 1 || !queries || query_aa < 1 || !candidates || n_candidates < 1 ||
-			!anchors || n_anchors < 1 || block_width < 1 || kmer < 1 || xdrop < 0 || !score_matrix ||
-			asize < 1 || !codon || !scores)
-		return 0;
-	if (!nsh_ensure_device()) return 0;
-	// every device access stays inside the buffers: residue codes index the matrix, contigs lie in
-	// the genome, queries and anchors in their arrays
-	if (asize < 21) { nsh_set_error("gapless: asize %d < 21 (stop codons score row 20)", asize); return 0; }
-	for (int t = 0; t < 64; ++t) if (codon[t] >= asize) { nsh_set_error("gapless: codon table out of range"); return 0; }
-	for (int64_t k = 0; k < query_aa; ++k)
-		if (queries[k] >= asize) { nsh_set_error("gapless: query residue %lld out of range", (long long)k); return 0; }
-	uint64_t n_anch_used = 0;
-	for (int32_t i = 0; i < n_candidates; ++i) {
-		const ns_gpu_gapless_candidate_t &c = candidates[i];
-		const int64_t na = c.anchor_count < kMaxAnchors? c.anchor_count : kMaxAnchors;
-		if (c.target_offset < 0 || c.target_length < 0 || c.target_offset + c.target_length > packed_nt_bases ||
-				c.query_offset < 0 || c.query_length < 0 || c.query_offset + c.query_length > query_aa ||
-				(na > 0 && (c.anchor_offset < 0 || c.anchor_offset + na > n_anchors))) {
-			nsh_set_error("gapless: candidate %d out of range", i);
-			return 0;
-		}
-		n_anch_used += na > 0? na : 0;
-	}
+	ScoreIn in = { packed_nt, packed_nt_bases, queries, query_aa, candidates, n_candidates, anchors, n_anchors,
+		block_width, kmer, xdrop, score_matrix, asize, codon };
+	if (!scores || !check_inputs(in)) return 0;
 	std::lock_guard<std::mutex> lk(g_mu);
-	const size_t sz[B_COUNT] = { (size_t)(packed_nt_bases + 1) >> 1, (size_t)query_aa,
-		(size_t)n_candidates * sizeof(*candidates), (size_t)n_anchors * sizeof(*anchors),
-		(size_t)asize * asize, 64, (size_t)n_candidates * sizeof(int32_t) };
-	for (int b = 0; b < B_COUNT; ++b) if (!ensure(g_buf[b], sz[b])) return 0;
-	if (!g_stream) GLOK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
-	auto t0 = std::chrono::steady_clock::now();
-	if (g_target_src != packed_nt || g_target_bases != packed_nt_bases) {
-		g_target_src = nullptr;
-		GLOK(hipMemcpyAsync(g_buf[B_TARGET].p, packed_nt, sz[B_TARGET], hipMemcpyHostToDevice, g_stream));
-	}
-	GLOK(hipMemcpyAsync(g_buf[B_QUERY].p, queries, sz[B_QUERY], hipMemcpyHostToDevice, g_stream));
-	GLOK(hipMemcpyAsync(g_buf[B_CAND].p, candidates, sz[B_CAND], hipMemcpyHostToDevice, g_stream));
-	GLOK(hipMemcpyAsync(g_buf[B_ANCH].p, anchors, sz[B_ANCH], hipMemcpyHostToDevice, g_stream));
-	GLOK(hipMemcpyAsync(g_buf[B_MAT].p, score_matrix, sz[B_MAT], hipMemcpyHostToDevice, g_stream));
-	GLOK(hipMemcpyAsync(g_buf[B_CODON].p, codon, 64, hipMemcpyHostToDevice, g_stream));
-	GLOK(hipStreamSynchronize(g_stream));
-	g_target_src = packed_nt, g_target_bases = packed_nt_bases;
+	const double up = queue_scores(in);
+	if (up < 0) return 0;
 	auto t1 = std::chrono::steady_clock::now();
-	GlArgs a;
-	a.pk = (const uint8_t*)g_buf[B_TARGET].p, a.q = (const uint8_t*)g_buf[B_QUERY].p;
-	a.cand = (const ns_gpu_gapless_candidate_t*)g_buf[B_CAND].p, a.anch = (const ns_gpu_gapless_anchor_t*)g_buf[B_ANCH].p;
-	a.n = n_candidates, a.bw = block_width, a.kmer = kmer, a.xdrop = xdrop, a.asize = asize;
-	a.mat = (const int8_t*)g_buf[B_MAT].p, a.codon = (const uint8_t*)g_buf[B_CODON].p, a.score = (int32_t*)g_buf[B_SCORE].p;
-	hipLaunchKernelGGL(gapless_kernel, dim3((n_candidates + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, g_stream, a);
-	GLOK(hipGetLastError());
 	GLOK(hipStreamSynchronize(g_stream));
 	auto t2 = std::chrono::steady_clock::now();
-	GLOK(hipMemcpyAsync(scores, g_buf[B_SCORE].p, sz[B_SCORE], hipMemcpyDeviceToHost, g_stream));
+	GLOK(hipMemcpyAsync(scores, g_buf[B_SCORE].p, (size_t)n_candidates * 4, hipMemcpyDeviceToHost, g_stream));
 	GLOK(hipStreamSynchronize(g_stream));
 	auto t3 = std::chrono::steady_clock::now();
 	using sec = std::chrono::duration<double>;
-	nsh_add_times(sec(t1 - t0).count(), sec(t2 - t1).count(), sec(t3 - t2).count());
-	nsh_count_launch(1);
-	++g_batches, g_cands += n_candidates, g_anchors += n_anch_used, g_seedpos += n_anch_used * (uint64_t)block_width;
+	nsh_add_times(up, sec(t2 - t1).count(), sec(t3 - t2).count());
+	return 1;
+}
+
+extern "C" int nsh_gapless_prefilter_batch(const uint8_t *packed_nt, int64_t packed_nt_bases,
+		const uint8_t *queries, int64_t query_aa,
+		const ns_gpu_gapless_candidate_t *candidates, int32_t n_candidates,
+		const ns_gpu_gapless_anchor_t *anchors, int64_t n_anchors,
+		int32_t block_width, int32_t kmer, int32_t xdrop,
+		const int8_t *score_matrix, int32_t asize, const uint8_t *codon,
+		int32_t n_query, const int32_t *cand_off, const int32_t *qlen, const int32_t *chn_sc, const uint32_t *vid,
+		const int64_t *vs, int32_t top_k, int32_t min_per_vid, int32_t adaptive,
+		int32_t *scores, uint8_t *keep, int32_t *k_out, nsh_gapless_calib_t *cal)
+{
+	ScoreIn in = { packed_nt, packed_nt_bases, queries, query_aa, candidates, n_candidates, anchors, n_anchors,
+		block_width, kmer, xdrop, score_matrix, asize, codon };
+	if (!scores || !keep || n_query < 1 || !cand_off || !qlen || !chn_sc || !vid || !vs || top_k < 1 ||
+			min_per_vid < 0 || min_per_vid > top_k) {
+		nsh_set_error("gapless prefilter: bad arguments (need top_k >= 1, 0 <= min_per_vid <= top_k)");
+		return 0;
+	}
+	if (cand_off[0] != 0 || cand_off[n_query] != n_candidates) {
+		nsh_set_error("gapless prefilter: cand_off must run from 0 to n_candidates");
+		return 0;
+	}
+	int32_t maxn = 0;
+	for (int32_t q = 0; q < n_query; ++q) {
+		const int32_t n = cand_off[q + 1] - cand_off[q];
+		if (n < 0 || n > 8192) { nsh_set_error("gapless prefilter: query %d has %d candidates (0..8192)", q, n); return 0; }
+		maxn = std::max(maxn, n);
+	}
+	if (!check_inputs(in)) return 0;
+	std::lock_guard<std::mutex> lk(g_mu);
+	if (queue_scores(in) < 0) return 0;
+	const size_t nc = (size_t)n_candidates;
+	const size_t sz[S_COUNT] = { (size_t)(n_query + 1) * 4, (size_t)n_query * 4, nc * 4, nc * 4, nc * 8, nc,
+		(size_t)n_query * 4 };
+	GLOK(hipStreamSynchronize(g_stream));   // the selection buffers may be reallocated below
+	for (int b = 0; b < S_COUNT; ++b) if (!ensure(g_sel[b], sz[b])) return 0;
+	GLOK(hipMemcpyAsync(g_sel[S_OFF].p, cand_off, sz[S_OFF], hipMemcpyHostToDevice, g_stream));
+	GLOK(hipMemcpyAsync(g_sel[S_QLEN].p, qlen, sz[S_QLEN], hipMemcpyHostToDevice, g_stream));
+	GLOK(hipMemcpyAsync(g_sel[S_CHN].p, chn_sc, sz[S_CHN], hipMemcpyHostToDevice, g_stream));
+	GLOK(hipMemcpyAsync(g_sel[S_VID].p, vid, sz[S_VID], hipMemcpyHostToDevice, g_stream));
+	GLOK(hipMemcpyAsync(g_sel[S_VS].p, vs, sz[S_VS], hipMemcpyHostToDevice, g_stream));
+	if (!nsh_gsel::run(g_stream, n_query, (const int32_t*)g_sel[S_OFF].p, (const int32_t*)g_sel[S_QLEN].p,
+			(const int32_t*)g_buf[B_SCORE].p, (const int32_t*)g_sel[S_CHN].p, (const uint32_t*)g_sel[S_VID].p,
+			(const int64_t*)g_sel[S_VS].p, n_candidates, maxn, top_k, min_per_vid, adaptive,
+			(uint8_t*)g_sel[S_KEEP].p, (int32_t*)g_sel[S_K].p, cal))
+		return 0;
+	GLOK(hipMemcpyAsync(scores, g_buf[B_SCORE].p, nc * 4, hipMemcpyDeviceToHost, g_stream));
+	GLOK(hipMemcpyAsync(keep, g_sel[S_KEEP].p, nc, hipMemcpyDeviceToHost, g_stream));
+	if (k_out) GLOK(hipMemcpyAsync(k_out, g_sel[S_K].p, sz[S_K], hipMemcpyDeviceToHost, g_stream));
+	GLOK(hipStreamSynchronize(g_stream));
 	return 1;
 }

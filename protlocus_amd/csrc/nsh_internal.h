@@ -49,4 +49,13 @@ void launch(int C, bool full, const int8_t *d_pssm, int32_t L, const int64_t *d_
 		int32_t ge, const int32_t *d_items, int32_t n_items, int32_t *d_out, hipStream_t s);
 }
 
+// ---- --gpu-prefilter candidate selection (gapless_select.hip), device-resident inputs ----
+// queues the selection of a mini-batch on stream s (the calibration waits on s); keep and the K
+// per query are written by a kernel still in flight when it returns. 1 on success.
+namespace nsh_gsel {
+int run(hipStream_t s, int32_t n_query, const int32_t *d_off, const int32_t *d_qlen, const int32_t *d_sc,
+		const int32_t *d_chn, const uint32_t *d_vid, const int64_t *d_vs, int32_t n_cand, int32_t max_per_query,
+		int32_t top_k, int32_t min_per_vid, int32_t adaptive, uint8_t *d_keep, int32_t *d_k, nsh_gapless_calib_t *cal);
+}
+
 #endif

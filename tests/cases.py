@@ -320,3 +320,39 @@ def prescreen_batch_case(rng, k):
                   max_evalue=[1e-3, 0.0, 10.0, 0.0][k % 4])
     return dict(genome=np.concatenate(contigs), ctg_off=ctg_off, ctg_len=ctg_len,
                 texts=[synth.hmm_text(em, tr) for em, tr in profs], params=params)
+
+
+def gapless_select_case(rng, k):
+    """a --gpu-prefilter mini-batch: per query its coarse candidates' gapless scores (ties,
+    zeros, a few negatives), chain scores, vid (few contig/strand groups) and vs; queries
+    without candidates, single candidates and long lists; options varied by k"""
+    n_query = int(rng.integers(1, 300))
+    sizes = []
+    for _ in range(n_query):
+        r = rng.random()
+        sizes.append(0 if r < 0.08 else 1 if r < 0.15 else int(rng.integers(2, 13)) if r < 0.85
+                     else int(rng.integers(9, 41)) if r < 0.98 else int(rng.integers(41, 300)))
+    cand_off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    n = int(cand_off[-1])
+    hi = int(rng.choice([20, 200, 5000]))
+    scores = rng.integers(0, hi, size=n).astype(np.int32)
+    scores[rng.random(n) < 0.05] = 0
+    scores[rng.random(n) < 0.02] = -rng.integers(1, 50, size=int((rng.random(n) < 0.02).sum()) or 1)[:1]
+    if k % 3 == 2:     # clear winners, some with a near-tie runner-up (duplication pressure)
+        for q in range(n_query):
+            a, b = int(cand_off[q]), int(cand_off[q + 1])
+            if b - a >= 2 and rng.random() < 0.6:
+                w = a + int(rng.integers(0, b - a))
+                scores[w] = hi * 6 + int(rng.integers(0, hi))
+                if rng.random() < 0.4:
+                    r = a + (w - a + 1) % (b - a)
+                    scores[r] = scores[w] * int(rng.integers(78, 101)) // 100
+    chn = rng.integers(0, int(rng.choice([5, 300])), size=n).astype(np.int32)
+    vid = rng.integers(0, int(rng.choice([2, 6, 40])), size=n).astype(np.uint32)
+    vs = rng.integers(0, int(rng.choice([3, 10 ** 7])), size=n).astype(np.int64)
+    qlen = rng.integers(30, 1200, size=n_query).astype(np.int32)
+    top_k = [1, 2, 3, 4, 8, 9, 16, 32][k % 8]
+    min_per_vid = min(top_k, [0, 1, 2, 1, 3][k % 5])
+    adaptive = (k // 8) % 2
+    return dict(cand_off=cand_off, qlen=qlen, scores=scores, chn=chn, vid=vid, vs=vs, top_k=top_k,
+                min_per_vid=min_per_vid, adaptive=adaptive)

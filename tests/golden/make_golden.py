@@ -14,6 +14,7 @@ Outputs (inputs and expected outputs only):
   ps_gapped_cases.json.gz  ps_gapped (prescreen.c): banded local affine scores
   prescreen_cases.json.gz  mp_prescreen_build + ps_query_worker (prescreen.c): profile search candidates
   prescreen_batch_cases.json.gz  one mp_prescreen_build, ps_query_worker per profile of a batch
+  gapless_select_cases.json.gz  mp_gapless_calibrate / _adaptive_k / mp_select_gapless (hit.c) on mini-batches
   tables.json        ns_tab_nt4 / ns_tab_aa20 / ns_tab_codon / ns_mat_blosum62
 """
 import base64
@@ -31,7 +32,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [os.path.join(ROOT, "tests"), ROOT]
 
 import nsffi                                       # noqa: E402
-from cases import (OUTPUT_VARIANTS, chain_case, gapped_case, nt4_codes, orf_case, output_case,   # noqa: E402
+from cases import (OUTPUT_VARIANTS, chain_case, gapless_select_case, gapped_case, nt4_codes, orf_case,  # noqa: E402
+                   output_case,
                    parse_cigar, prescreen_batch_case, prescreen_case,
                    planted_case, random_opt_kw, random_ss)
 from nsffi import NS_F_CIGAR, NS_F_EXT_LEFT, NS_F_EXT_RIGHT  # noqa: E402
@@ -239,6 +241,22 @@ def prescreen_batch_cases(ref):
     return out
 
 
+def gapless_select_cases(ref):
+    """the reference's --gpu-prefilter selection (hit.c:149-263 as map.c:1075-1083 and 507-521
+    drive it) on cases.gapless_select_case mini-batches"""
+    rng = np.random.default_rng(20260903)
+    out = []
+    for k in range(32):
+        c = gapless_select_case(rng, k)
+        keep, ks, cal = ref.gapless_select(c["cand_off"], c["qlen"], c["scores"], c["chn"], c["vid"], c["vs"],
+                                           c["top_k"], c["min_per_vid"], c["adaptive"])
+        rec = {x: c[x].tolist() for x in ("cand_off", "qlen", "scores", "chn", "vid", "vs")}
+        rec.update(top_k=c["top_k"], min_per_vid=c["min_per_vid"], adaptive=c["adaptive"], keep=keep, k=ks,
+                   cal=list(cal))
+        out.append(rec)
+    return out
+
+
 def tables(ref):
     L = ref.lib
     get = lambda name, n, t=C.c_uint8: list((t * n).in_dll(L, name))
@@ -252,7 +270,8 @@ def main():
     ref = nsffi.Reference()
     gens = [("dp_cases", dp_cases), ("pssm_chain_cases", pssm_chain_cases), ("orf_cases", orf_cases),
             ("ps_gapped_cases", ps_gapped_cases), ("prescreen_cases", prescreen_cases),
-            ("prescreen_batch_cases", prescreen_batch_cases), ("output_cases", output_cases),
+            ("prescreen_batch_cases", prescreen_batch_cases), ("gapless_select_cases", gapless_select_cases),
+            ("output_cases", output_cases),
             ("hmm_cases", hmm_cases)]
     want = set(sys.argv[1:])
     for name, fn in gens:

@@ -225,6 +225,7 @@ class Reference(_Lib):
         L.or_ref_ps_query.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_int32, C.c_int32, C.c_int32,
                                       C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_void_p, C.c_void_p]
         L.or_ref_ps_free.argtypes = [C.c_void_p]
+        L.or_ref_gapless_select.argtypes = [C.c_int32] + [C.c_void_p] * 6 + [C.c_int32] * 3 + [C.c_void_p] * 3
         assert L.ns_make_tables(1) == 0
         self._opt_init = L.ns_opt_init
         self._opt_set_sp = L.ns_opt_set_sp
@@ -318,6 +319,20 @@ class Reference(_Lib):
 
     def ps_free(self, handle):
         self.lib.or_ref_ps_free(handle)
+
+    def gapless_select(self, cand_off, qlen, scores, chn, vid, vs, top_k, min_per_vid, adaptive):
+        """the reference's --gpu-prefilter selection (oracle/hit_ref.c): keep flags, K per query,
+        (noise, n_query, dup_frac)"""
+        off = np.ascontiguousarray(cand_off, dtype=np.int32)
+        arrs = [np.ascontiguousarray(x, dtype=t) for x, t in ((qlen, np.int32), (scores, np.int32), (chn, np.int32),
+                                                               (vid, np.uint32), (vs, np.int64))]
+        n_q, n = len(off) - 1, int(off[-1])
+        keep, ks = np.zeros(max(1, n), np.uint8), np.zeros(max(1, n_q), np.int32)
+        cal = (C.c_int32 * 3)()
+        self.lib.or_ref_gapless_select(n_q, off.ctypes.data, *[a.ctypes.data for a in arrs], top_k, min_per_vid,
+                                       adaptive, keep.ctypes.data, ks.ctypes.data, cal)
+        dup = C.cast(C.addressof(cal) + 8, C.POINTER(C.c_float)).contents.value
+        return keep[:n].tolist(), ks[:n_q].tolist(), (cal[0], cal[1], dup)
 
     def orfs(self, ctg_nt4, rev, min_orf):
         """ps_orf_worker on strand rev of a contig: [(start, codons)], residues"""
