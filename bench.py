@@ -15,7 +15,8 @@ Beside it, in the same JSON line:
   * prescreen: the learned chain-candidate prescreen trained on the GPU, then the plain and the
     prescreened DP pipelines timed on fresh candidates (recall against the exact DP);
   * hmm: profile-HMM Viterbi scan, 64 profiles (M 100-800) x a 10-Mb genome in 6 frames;
-  * profile_search: the profile prescreen search of the same profiles over the same genome.
+  * profile_search: the profile prescreen search of the same profiles over the same genome;
+  * gapless_prefilter: gapless scores and top-K selection of one --gpu-prefilter mini-batch.
 Output: one JSON line.
 """
 import argparse
@@ -218,6 +219,91 @@ def bench_profile_search(cpu=True, seconds_cpu=10.0, log=None):
     return out
 
 
+def bench_gapless_prefilter(cpu=True, reps=5, log=None):
+    """the --gpu-prefilter stage of one mini-batch (mp_gpu_prefilter_batch, map.c:1014-1089): GPU
+    gapless scores of every coarse candidate, then the batch calibration, adaptive K and top-K
+    selection (hit.c:149-263). Times the fused call, the scoring alone and the selection alone on
+    the GPU, and the reference's selection (oracle/_ref, one host thread, as the reference runs it
+    on its pipeline thread) on the same scores, whose keep flags must be identical."""
+    import numpy as np
+    import nsffi
+    from protlocus_amd import capi, synth
+    b = synth.gapless_minibatch()
+    prod = nsffi.Product()
+    L = prod.lib
+    g = b["genome"]
+    pk = (np.append(g, 0) if len(g) & 1 else g)
+    pk = (pk[0::2] | (pk[1::2] << 4)).astype(np.uint8).tobytes()
+    n, n_q = len(b["vid"]), len(b["qlen"])
+    ca = np.zeros(n, dtype=[("target_offset", "<i8"), ("target_length", "<i8"), ("query_offset", "<i8"),
+                            ("query_length", "<i4"), ("anchor_offset", "<i4"), ("anchor_count", "<i4"),
+                            ("reverse", "<i4")])
+    for f in ca.dtype.names:
+        ca[f] = b[f]
+    an = np.zeros(len(b["anchor_block"]), dtype=[("target_block", "<i8"), ("query_end", "<i4"), ("reserved", "<i4")])
+    an["target_block"], an["query_end"] = b["anchor_block"], b["anchor_qend"]
+    CA = (nsffi.GaplessCand * n).from_buffer(ca)
+    AN = (nsffi.GaplessAnchor * len(an)).from_buffer(an)
+    mat = np.frombuffer(bytes((C.c_int8 * 484).in_dll(L, "ns_mat_blosum62")), dtype=np.int8)
+    codon = bytes((C.c_uint8 * 64).in_dll(L, "ns_tab_codon"))
+    queries = b["queries"].tobytes()
+    bw, kmer, xdrop, top_k, mpv = 256, 6, 40, 32, 1
+    sa = [np.ascontiguousarray(x, dtype=t) for x, t in ((b["cand_off"], np.int32), (b["qlen"], np.int32),
+                                                        (b["chn"], np.int32), (b["vid"], np.uint32), (b["vs"], np.int64))]
+    sel_args = tuple(sa)
+    scores = np.zeros(n, dtype=np.int32)
+    L.ns_gpu_gapless_reset_target()
+
+    def score():
+        ok = L.ns_gpu_gapless_score_batch(pk, len(g), queries, len(queries), CA, n, AN, len(an), bw, kmer, xdrop,
+                                          mat.ctypes.data, 22, codon, scores.ctypes.data)
+        assert ok, prod.error()
+    out = {"workload": "one --gpu-prefilter mini-batch: %d queries (50-1000 aa, %d residues), %d coarse candidates "
+                       "(2-48 per query, one true locus each, 20%% substitutions), %d anchors, 20 Mb genome in 24 "
+                       "contigs; block 256, k-mer 6, X-drop 40, top-K 32, per-contig floor 1" % (
+                           n_q, len(queries), n, len(an)),
+           "candidates": n}
+    for adaptive in (1, 0):
+        fused = capi.gapless_prefilter(pk, len(g), queries, CA, AN, bw, kmer, xdrop, mat, codon, *sel_args, top_k,
+                                       mpv, adaptive, as_numpy=True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            capi.gapless_prefilter(pk, len(g), queries, CA, AN, bw, kmer, xdrop, mat, codon, *sel_args, top_k, mpv,
+                                   adaptive, as_numpy=True)
+        fused_ms = (time.perf_counter() - t0) * 1e3 / reps
+        score()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            score()
+        score_ms = (time.perf_counter() - t0) * 1e3 / reps
+        assert np.array_equal(scores, fused[0]), "fused and separate gapless scores disagree"
+        capi.gapless_select(sa[0], sa[1], scores, *sa[2:], top_k, mpv, adaptive, as_numpy=True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            sel = capi.gapless_select(sa[0], sa[1], scores, *sa[2:], top_k, mpv, adaptive, as_numpy=True)
+        sel_ms = (time.perf_counter() - t0) * 1e3 / reps
+        assert np.array_equal(sel[0], fused[1]) and np.array_equal(sel[1], fused[2])
+        key = "adaptive" if adaptive else "fixed_k"
+        r = {"fused_ms": round(fused_ms, 3), "score_ms": round(score_ms, 3), "select_ms": round(sel_ms, 3),
+             "kept": int(sel[0].sum()), "calibration": {"noise": sel[2][0], "n_query": sel[2][1],
+                                                      "dup_frac": round(sel[2][2], 4)}}
+        if cpu and nsffi.have_ref():
+            ref = nsffi.Reference()
+            t0 = time.perf_counter()
+            want = ref.gapless_select(b["cand_off"], b["qlen"], scores, b["chn"], b["vid"], b["vs"], top_k, mpv,
+                                      adaptive)
+            cpu_ms = (time.perf_counter() - t0) * 1e3
+            r["cpu_baseline"] = {"value": round(cpu_ms, 3), "unit": "ms per mini-batch (selection only)", "cores": 1,
+                                 "kind": "reference", "sample": "the whole mini-batch, one host thread"}
+            r["identical_to_reference"] = bool(want[0] == sel[0].tolist() and want[1] == sel[1].tolist() and
+                                               want[2] == sel[2])
+        out[key] = r
+        if log:
+            log("[bench] gapless prefilter (%s): fused %.2f ms, score %.2f ms, select %.3f ms, kept %d of %d" % (
+                key, fused_ms, score_ms, sel_ms, r["kept"], n))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,6 +316,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--only-profile-search", action="store_true",
                     help="run only the profile prescreen search section and print its JSON object")
+    ap.add_argument("--only-gapless-prefilter", action="store_true",
+                    help="run only the gapless prefilter section and print its JSON object")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device three times (a copy kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
@@ -259,6 +347,9 @@ def main():
     log = lambda s: print(s, file=sys.stderr, flush=True)
     if args.only_profile_search:
         print(json.dumps(bench_profile_search(cpu=not args.no_cpu, log=log)), flush=True)
+        return
+    if args.only_gapless_prefilter:
+        print(json.dumps(bench_gapless_prefilter(cpu=not args.no_cpu, log=log)), flush=True)
         return
 
     import nsffi
@@ -341,6 +432,7 @@ def main():
         out["hmm"] = bench_hmm(log=log)
         log("[bench] hmm: %.1f GCUPS" % out["hmm"]["gcups"])
         out["profile_search"] = bench_profile_search(cpu=not args.no_cpu, log=log)
+        out["gapless_prefilter"] = bench_gapless_prefilter(cpu=not args.no_cpu, log=log)
     print(json.dumps(out), flush=True)
 
 

@@ -421,9 +421,9 @@ def _sel_arrays(cand_off, qlen, chn, vid, vs):
                                                            (vid, np.uint32), (vs, np.int64))]
 
 
-def gapless_select(cand_off, qlen, scores, chn, vid, vs, top_k, min_per_vid, adaptive):
+def gapless_select(cand_off, qlen, scores, chn, vid, vs, top_k, min_per_vid, adaptive, as_numpy=False):
     """the --gpu-prefilter selection of a mini-batch (nsh_gapless_select_batch): keep flags, K per
-    query, (noise, n_query, dup_frac)"""
+    query, (noise, n_query, dup_frac); lists, or numpy arrays with as_numpy"""
     off, ql, ch, vi, v = _sel_arrays(cand_off, qlen, chn, vid, vs)
     sc = np.ascontiguousarray(scores, dtype=np.int32)
     n_q, n = len(off) - 1, int(off[-1])
@@ -433,13 +433,16 @@ def gapless_select(cand_off, qlen, scores, chn, vid, vs, top_k, min_per_vid, ada
                                           vi.ctypes.data, v.ctypes.data, top_k, min_per_vid, adaptive,
                                           keep.ctypes.data, ks.ctypes.data, C.byref(cal)):
         raise RuntimeError("nsh_gapless_select_batch: " + error())
+    if as_numpy:
+        return keep[:n], ks[:n_q], (cal.noise, cal.n_query, cal.dup_frac)
     return keep[:n].tolist(), ks[:n_q].tolist(), (cal.noise, cal.n_query, cal.dup_frac)
 
 
 def gapless_prefilter(packed_nt, n_bases, queries, cands, anchors, block_width, kmer, xdrop, mat, codon,
-                      cand_off, qlen, chn, vid, vs, top_k, min_per_vid, adaptive):
+                      cand_off, qlen, chn, vid, vs, top_k, min_per_vid, adaptive, as_numpy=False):
     """gapless scores and the selection in one call (nsh_gapless_prefilter_batch): scores, keep
-    flags, K per query, calibration. cands / anchors: ctypes arrays of the ns_gpu_gapless_* structs"""
+    flags, K per query, calibration (lists, or numpy arrays with as_numpy). cands / anchors: ctypes
+    arrays of the ns_gpu_gapless_* structs"""
     off, ql, ch, vi, v = _sel_arrays(cand_off, qlen, chn, vid, vs)
     m = np.ascontiguousarray(mat, dtype=np.int8)
     n_q, n = len(off) - 1, len(cands)
@@ -454,6 +457,8 @@ def gapless_prefilter(packed_nt, n_bases, queries, cands, anchors, block_width, 
                                              v.ctypes.data, top_k, min_per_vid, adaptive, scores.ctypes.data,
                                              keep.ctypes.data, ks.ctypes.data, C.byref(cal)):
         raise RuntimeError("nsh_gapless_prefilter_batch: " + error())
+    if as_numpy:
+        return scores, keep, ks, (cal.noise, cal.n_query, cal.dup_frac)
     return scores.tolist(), keep.tolist(), ks.tolist(), (cal.noise, cal.n_query, cal.dup_frac)
 
 

@@ -291,3 +291,72 @@ def profile_scan(n_prof=64, genome_len=10_000_000, win=3000, step=2700, n_plant=
     offs = np.zeros(len(regs), dtype=np.int64)
     offs[1:] = np.cumsum(lens[:-1], dtype=np.int64)
     return profs, np.concatenate(regs).tobytes(), offs, lens
+
+
+_CODE_AA = np.array([AA.index(c) if c != "*" else 20 for c in CODE1], dtype=np.uint8)
+
+
+def gapless_minibatch(n_query=5000, genome_len=20_000_000, n_contigs=24, seed=9, bw=256, n_anchor=8, kmer=6,
+                      min_len=50, max_len=1000, min_cand=2, max_cand=48, sub_rate=0.2):
+    """a --gpu-prefilter mini-batch (about 2 M query residues, the reference's default mini-batch):
+    a random genome (GC 0.45) cut into contigs; per query (length log-uniform) its true locus, a
+    frame of either strand translated with sub_rate substitutions, among min_cand..max_cand coarse
+    candidates, the rest decoys at random; n_anchor anchors per candidate, spread along the true
+    diagonal or at random for decoys, block-aligned as map.c:474-500 picks them. Returns a dict of
+    numpy arrays: genome codes, queries (aa codes, concatenated), the candidate and anchor fields,
+    and per candidate the chain score, vid and vs; per query cand_off and qlen."""
+    rng = np.random.default_rng(seed)
+    g = rng.choice(4, size=genome_len, p=[0.275, 0.225, 0.225, 0.275]).astype(np.uint8)
+    cuts = np.sort(rng.choice(np.arange(100_000, genome_len - 100_000), n_contigs - 1, replace=False))
+    ctg_off = np.concatenate([[0], cuts]).astype(np.int64)
+    ctg_len = np.diff(np.concatenate([ctg_off, [genome_len]])).astype(np.int64)
+    orient = {}
+
+    def oriented(c, rev):
+        if (c, rev) not in orient:
+            o = g[ctg_off[c]:ctg_off[c] + ctg_len[c]]
+            orient[(c, rev)] = (3 - o[::-1]) if rev else o
+        return orient[(c, rev)]
+    qlen = np.exp(rng.uniform(np.log(min_len), np.log(max_len), n_query)).astype(np.int32)
+    p_ctg = ctg_len / ctg_len.sum()
+    queries, q_off = [], [0]
+    cand = {x: [] for x in ("target_offset", "target_length", "query_offset", "query_length", "anchor_offset",
+                            "anchor_count", "reverse", "chn", "vid", "vs")}
+    anch_block, anch_qend, cand_off = [], [], [0]
+    for q in range(n_query):
+        L = int(qlen[q])
+        c, rev = int(rng.choice(n_contigs, p=p_ctg)), int(rng.integers(0, 2))
+        o = oriented(c, rev)
+        e0 = int(rng.integers(2, len(o) - 3 * L))
+        cod = o[e0 - 2:e0 - 2 + 3 * L].reshape(L, 3)
+        prot = _CODE_AA[cod[:, 0].astype(np.int64) << 4 | cod[:, 1] << 2 | cod[:, 2]]
+        mut = rng.random(L) < sub_rate
+        prot[mut] = rng.integers(0, 20, size=int(mut.sum()))
+        queries.append(prot)
+        n_c = int(rng.integers(min_cand, max_cand + 1))
+        true_at = int(rng.integers(0, n_c))
+        for j in range(n_c):
+            if j == true_at:
+                cc, rr, s0, chn = c, rev, e0, int(L * rng.uniform(1.0, 3.0))
+            else:
+                cc, rr = int(rng.choice(n_contigs, p=p_ctg)), int(rng.integers(0, 2))
+                s0, chn = int(rng.integers(2, ctg_len[cc] - 2)), int(rng.integers(kmer, max(kmer + 1, L)))
+            cand["target_offset"].append(ctg_off[cc]), cand["target_length"].append(ctg_len[cc])
+            cand["query_offset"].append(q_off[-1]), cand["query_length"].append(L)
+            cand["anchor_offset"].append(len(anch_block)), cand["anchor_count"].append(n_anchor)
+            cand["reverse"].append(rr), cand["chn"].append(chn), cand["vid"].append(cc * 2 + rr)
+            cand["vs"].append(s0 - 2)
+            for t in range(n_anchor):
+                k = kmer - 1 + (L - kmer) * t // max(1, n_anchor - 1)
+                if j != true_at:
+                    k = int(rng.integers(kmer - 1, L))
+                pos = s0 + 3 * k if j == true_at else int(rng.integers(0, ctg_len[cc]))
+                anch_block.append(max(0, min(pos, int(ctg_len[cc]) - 1)) // bw * bw), anch_qend.append(k)
+        q_off.append(q_off[-1] + L)
+        cand_off.append(len(cand["vid"]))
+    out = {"genome": g, "queries": np.concatenate(queries).astype(np.uint8), "qlen": qlen,
+           "cand_off": np.array(cand_off, dtype=np.int32), "anchor_block": np.array(anch_block, dtype=np.int64),
+           "anchor_qend": np.array(anch_qend, dtype=np.int32)}
+    for k2, v in cand.items():
+        out[k2] = np.array(v, dtype=np.uint32 if k2 == "vid" else np.int64)
+    return out
