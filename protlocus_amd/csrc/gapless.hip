@@ -28,6 +28,10 @@ namespace {
 
 constexpr int kWaves = 4;          // candidates per work-group
 constexpr int kMaxAnchors = 32;
+constexpr int kMaxKmer = 16;
+constexpr int kMaxAsize = 32;
+constexpr int kChunk = 256;        // seed positions staged per pass
+constexpr int kWin = kChunk + 3 * (kMaxKmer - 1);
 
 struct GlArgs {
 	const uint8_t *pk;                            // 4-bit packed genome, 2 bases per byte (low nibble first)
@@ -56,17 +60,36 @@ DEV int gl_aa(const uint8_t *pk, const uint8_t *codon, int64_t off, int64_t len,
 	return b0 < 4 && b1 < 4 && b2 < 4? codon[b0 << 4 | b1 << 2 | b2] : 20;
 }
 
-DEV int wave_sum_incl(int v, int L)
+template <int CTRL, int ROWS = 0xf>
+DEV int dpp(int old, int src) { return __builtin_amdgcn_update_dpp(old, src, CTRL, ROWS, 0xf, false); }
+
+// inclusive wave scans: shifts 1-8 inside each row of 16 lanes, then lane 15 and lane 31 onwards
+DEV int wave_sum_incl(int v)
 {
-#pragma unroll
-	for (int d = 1; d < 64; d <<= 1) { const int t = __shfl_up(v, d); if (L >= d) v += t; }
+	v += dpp<0x111>(0, v);
+	v += dpp<0x112>(0, v);
+	v += dpp<0x114>(0, v);
+	v += dpp<0x118>(0, v);
+	v += dpp<0x142, 0xa>(0, v);
+	v += dpp<0x143, 0xc>(0, v);
 	return v;
 }
-DEV int wave_max_incl(int v, int L)
+DEV int wave_max_incl(int v)
 {
-#pragma unroll
-	for (int d = 1; d < 64; d <<= 1) { const int t = __shfl_up(v, d); if (L >= d) v = max(v, t); }
+	v = max(v, dpp<0x111>(INT_MIN, v));
+	v = max(v, dpp<0x112>(INT_MIN, v));
+	v = max(v, dpp<0x114>(INT_MIN, v));
+	v = max(v, dpp<0x118>(INT_MIN, v));
+	v = max(v, dpp<0x142, 0xa>(INT_MIN, v));
+	v = max(v, dpp<0x143, 0xc>(INT_MIN, v));
 	return v;
+}
+
+DEV void wave_sync_lds()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // one X-drop walk along the diagonal, 64 steps per iteration; step(st, &v) is false once step st
@@ -80,20 +103,18 @@ DEV int xdrop_walk(const F &step, int xdrop, int L, int *len)
 	for (int base = 1;; base += 64) {
 		int v = 0;
 		const bool valid = step(base + L, &v);
-		const int S = lr + wave_sum_incl(valid? v : 0, L);
-		const int Mx = max(lb, wave_max_incl(S, L));
-		const uint64_t sb = __ballot(!valid || Mx - S > xdrop);
+		const int S = lr + wave_sum_incl(valid? v : 0);
+		const int M = wave_max_incl(S);                     // best running score over steps <= this one
+		const uint64_t sb = __ballot(!valid || max(lb, M) - S > xdrop);
 		const int f = sb? __builtin_ctzll(sb) : 64;
 		const int np = f == 64? 64 : (((__ballot(valid) >> f) & 1)? f + 1 : f);   // steps taken
-		int cm = L < np? S : INT_MIN;
-#pragma unroll
-		for (int d = 32; d >= 1; d >>= 1) cm = max(cm, __shfl_xor(cm, d));
+		const int cm = np > 0? __builtin_amdgcn_readlane(M, np - 1) : INT_MIN;
 		if (cm > lb) {
 			ll = base + __builtin_ctzll(__ballot(L < np && S == cm));
 			lb = cm;
 		}
 		if (sb) break;   // the walk ended inside this chunk (possibly after its last step)
-		lr = __shfl(S, 63);
+		lr = __builtin_amdgcn_readlane(S, 63);
 	}
 	*len = ll;
 	return lb;
@@ -101,32 +122,54 @@ DEV int xdrop_walk(const F &step, int xdrop, int L, int *len)
 
 __global__ void __launch_bounds__(64 * kWaves) gapless_kernel(GlArgs a)
 {
+	__shared__ int8_t smat[kMaxAsize * kMaxAsize];
+	__shared__ uint8_t scod[64];
+	__shared__ uint8_t swin[kWaves][kWin];           // amino acids of the staged seed window
+	__shared__ int8_t scol[kWaves][kMaxKmer][kMaxAsize];   // per seed position: mat[aa][query residue]
 	__shared__ int seg[kWaves][3][kMaxAnchors];
+	const int asz = a.asize;
+	for (int t = threadIdx.x; t < asz * asz; t += blockDim.x) smat[t] = a.mat[t];
+	for (int t = threadIdx.x; t < 64; t += blockDim.x) scod[t] = a.codon[t];
+	__syncthreads();
 	const int L = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
 	const int ci = blockIdx.x * kWaves + w;
-	if (ci >= a.n) return;   // whole waves leave; the kernel has no barrier
+	if (ci >= a.n) return;   // whole waves leave; no barrier follows
 	const ns_gpu_gapless_candidate_t c = a.cand[ci];
 	const uint8_t *qq = a.q + c.query_offset;
-	const int kmer = a.kmer, asz = a.asize, xdrop = a.xdrop;
+	const int kmer = a.kmer, xdrop = a.xdrop, ext = 3 * (kmer - 1);
 	const int64_t off = c.target_offset, tlen = c.target_length;
 	const int rev = c.reverse;
 	const int na = min(c.anchor_count, kMaxAnchors);
 	int *sq = seg[w][0], *se = seg[w][1], *sv = seg[w][2];
+	uint8_t *win = swin[w];
+	int8_t (*col)[kMaxAsize] = scol[w];
 	int n_seg = 0;
 	for (int an = 0; an < na; ++an) {
 		const ns_gpu_gapless_anchor_t A = a.anch[c.anchor_offset + an];
 		const int bq = A.query_end;
 		if (bq < kmer - 1 || bq >= c.query_length) continue;
+		// this anchor's k-mer of the query, as matrix columns
+		wave_sync_lds();
+		for (int t = L; t < kmer * asz; t += 64) {
+			const int s = t / asz, r = t - s * asz;
+			col[s][r] = smat[r * asz + qq[bq - kmer + 1 + s]];
+		}
 		// best k-mer seed over the block; ties go to the smallest target end
 		int best = INT_MIN;
 		int64_t bt = INT64_MAX;
-		for (int d = L; d < a.bw; d += 64) {
-			const int64_t te = A.target_block + d;
-			if (te - (int64_t)(kmer - 1) * 3 - 2 < 0 || te >= tlen) continue;
-			int sc = 0;
-			for (int s = 0; s < kmer; ++s)
-				sc += a.mat[gl_aa(a.pk, a.codon, off, tlen, rev, te - (int64_t)(kmer - 1 - s) * 3) * asz + qq[bq - kmer + 1 + s]];
-			if (sc > best) best = sc, bt = te;
+		for (int d0 = 0; d0 < a.bw; d0 += kChunk) {
+			const int nd = min(kChunk, a.bw - d0);
+			const int64_t p0 = A.target_block + d0 - ext;   // codon end of window slot 0
+			wave_sync_lds();
+			for (int t = L; t < nd + ext; t += 64) win[t] = (uint8_t)gl_aa(a.pk, scod, off, tlen, rev, p0 + t);
+			wave_sync_lds();
+			for (int d = L; d < nd; d += 64) {
+				const int64_t te = A.target_block + d0 + d;
+				if (te - ext - 2 < 0 || te >= tlen) continue;
+				int sc = 0;
+				for (int s = 0; s < kmer; ++s) sc += col[s][win[d + 3 * s]];
+				if (sc > best) best = sc, bt = te;
+			}
 		}
 #pragma unroll
 		for (int d = 32; d >= 1; d >>= 1) {
@@ -140,14 +183,14 @@ __global__ void __launch_bounds__(64 * kWaves) gapless_kernel(GlArgs a)
 			const int qp = bq - kmer - (st - 1);
 			const int64_t te = bt - (int64_t)(kmer - 1 + st) * 3;
 			if (qp < 0 || te - 2 < 0) return false;
-			*v = a.mat[gl_aa(a.pk, a.codon, off, tlen, rev, te) * asz + qq[qp]];
+			*v = smat[gl_aa(a.pk, scod, off, tlen, rev, te) * asz + qq[qp]];
 			return true;
 		}, xdrop, L, &ll);
 		const int rb = xdrop_walk([&](int st, int *v) {
 			const int qp = bq + st;
 			const int64_t te = bt + (int64_t)st * 3;
 			if (qp >= c.query_length || te >= tlen) return false;
-			*v = a.mat[gl_aa(a.pk, a.codon, off, tlen, rev, te) * asz + qq[qp]];
+			*v = smat[gl_aa(a.pk, scod, off, tlen, rev, te) * asz + qq[qp]];
 			return true;
 		}, xdrop, L, &rl);
 		{   // insert into the score-descending list, after equal scores. Every lane runs this
@@ -237,7 +280,11 @@ int check_inputs(ScoreIn &in)
 			in.xdrop < 0 || !in.score_matrix || in.asize < 1 || !in.codon)
 		return 0;
 	if (!nsh_ensure_device()) return 0;
-	if (in.asize < 21) { nsh_set_error("gapless: asize %d < 21 (stop codons score row 20)", in.asize); return 0; }
+	if (in.asize < 21 || in.asize > kMaxAsize) {
+		nsh_set_error("gapless: asize %d outside 21..%d (stop codons score row 20)", in.asize, kMaxAsize);
+		return 0;
+	}
+	if (in.kmer > kMaxKmer) { nsh_set_error("gapless: kmer %d > %d", in.kmer, kMaxKmer); return 0; }
 	for (int t = 0; t < 64; ++t) if (in.codon[t] >= in.asize) { nsh_set_error("gapless: codon table out of range"); return 0; }
 	for (int64_t k = 0; k < in.query_aa; ++k)
 		if (in.queries[k] >= in.asize) { nsh_set_error("gapless: query residue %lld out of range", (long long)k); return 0; }
