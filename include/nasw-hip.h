@@ -325,6 +325,21 @@ int64_t nsh_prescreen_search_batch(nsh_prescreen_t *ps, int32_t n_prof, const in
 int nsh_prescreen_stats(nsh_prescreen_t *ps, int64_t *st);
 void nsh_prescreen_destroy(nsh_prescreen_t *ps);
 
+/* The k-mer block index of a genome (mp_idx_build without the file read: mp_idx_boff,
+ * mp_sketch_nt4 per contig strand and build_bidx, index.c:14-29, 52-97, sketch.c:108-169), built on
+ * the device: contigs of base codes 0..4 at nt + ctg_off[c]; k-mers of 1..7 residues in the 13-letter
+ * alphabet (ns_tab_codon13) over clean ORFs of >= min_aa_len codons, kept when their low mod_bit
+ * hash bits are 0; blocks of 2^bbit bases. nsh_index_size: 0 entries (n_kb), 1 buckets
+ * (2^(4 kmer - mod_bit)), 2 blocks (n_block). nsh_index_copy: the bucket starts (mp_idx_t::ki, one
+ * per bucket), the entries (kb) and the block offsets (bo, 2 n_ctg + 1) into the arrays that are not
+ * NULL; 1 on success. The tables and codes must be set up (ns_make_tables) before the build. */
+typedef struct nsh_index_s nsh_index_t;
+nsh_index_t *nsh_index_build(int32_t n_ctg, const uint8_t *nt, const int64_t *ctg_off, const int64_t *ctg_len,
+		int32_t min_aa_len, int32_t kmer, int32_t mod_bit, int32_t bbit);
+int64_t nsh_index_size(const nsh_index_t *ix, int32_t what);
+int nsh_index_copy(const nsh_index_t *ix, int64_t *ki, uint32_t *kb, uint32_t *bo);
+void nsh_index_destroy(nsh_index_t *ix);
+
 /* Learned chain-candidate prescreen: MLP 64 -> 256 -> 256 -> 1 (ReLU, logistic output), BF16
  * MFMA kernels with FP32 accumulation, FP32 Adam masters. Parameter vector (FP32, in order):
  * W1[64][256], b1[256], W2[256][256], b2[256], w3[256], b3. No reference counterpart. */

@@ -130,6 +130,12 @@ def lib():
                                                  C.c_void_p, C.c_int64, C.c_void_p]
         L.nsh_prescreen_search_batch.restype = C.c_int64
         L.nsh_prescreen_stats.argtypes = [C.c_void_p, C.c_void_p]
+        L.nsh_index_build.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 4
+        L.nsh_index_build.restype = C.c_void_p
+        L.nsh_index_size.argtypes = [C.c_void_p, C.c_int32]
+        L.nsh_index_size.restype = C.c_int64
+        L.nsh_index_copy.argtypes = [C.c_void_p] * 4
+        L.nsh_index_destroy.argtypes = [C.c_void_p]
         L.nsh_gapless_select_batch.argtypes = [C.c_int32] + [C.c_void_p] * 6 + [C.c_int32] * 3 + [C.c_void_p] * 3
         L.nsh_gapless_prefilter_batch.argtypes = ([C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32,
                                                    C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
@@ -409,6 +415,26 @@ def ps_gapped_batch(pssm, orfs, diags, band, go, ge):
                                  ln.ctypes.data, dg.ctypes.data, band, go, ge, sw.ctypes.data):
         raise RuntimeError("nsh_ps_gapped_batch: " + error())
     return sw[:len(orfs)].tolist()
+
+
+def index_build(genome, ctg_off, ctg_len, min_aa_len=30, kmer=6, mod_bit=1, bbit=8):
+    """the k-mer block index built on the device (nsh_index_*): (ki, kb, bo) as numpy arrays"""
+    L = lib()
+    g = np.ascontiguousarray(genome, dtype=np.uint8)
+    off = np.ascontiguousarray(ctg_off, dtype=np.int64)
+    ln = np.ascontiguousarray(ctg_len, dtype=np.int64)
+    h = L.nsh_index_build(len(off), g.ctypes.data, off.ctypes.data, ln.ctypes.data, min_aa_len, kmer, mod_bit, bbit)
+    if not h:
+        raise RuntimeError("nsh_index_build: " + error())
+    try:
+        n_kb, n_bucket = L.nsh_index_size(h, 0), L.nsh_index_size(h, 1)
+        ki, kb = np.zeros(n_bucket, np.int64), np.zeros(max(1, n_kb), np.uint32)
+        bo = np.zeros(2 * len(off) + 1, np.uint32)
+        if not L.nsh_index_copy(h, ki.ctypes.data, kb.ctypes.data, bo.ctypes.data):
+            raise RuntimeError("nsh_index_copy: " + error())
+    finally:
+        L.nsh_index_destroy(h)
+    return ki, kb[:n_kb], bo
 
 
 class GaplessCalib(C.Structure):

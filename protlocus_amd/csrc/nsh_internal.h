@@ -49,6 +49,21 @@ void launch(int C, bool full, const int8_t *d_pssm, int32_t L, const int64_t *d_
 		int32_t ge, const int32_t *d_items, int32_t n_items, int32_t *d_out, hipStream_t s);
 }
 
+// ---- ORF runs of a sequence on the device (orf_scan.hip) ----
+namespace nsh_orfs {
+struct Work {                      // device buffers, grown on demand; release() frees them
+	void *p[6] = {};
+	size_t n[6] = {};
+	int64_t *st = nullptr;         // after find(): first base of each run
+	int32_t *len = nullptr;        // after find(): codons of each run
+	void release();
+};
+// the runs of at least min_orf good codons (ps_orf_worker's walk, prescreen.c:101-132) of an
+// oriented sequence already on the device, unsorted; waits for stream s. 1 on success.
+int find(hipStream_t s, const uint8_t *d_nt, int64_t len, const uint8_t *d_codon, int32_t min_orf, Work &w,
+		int64_t *n_run);
+}
+
 // ---- --gpu-prefilter candidate selection (gapless_select.hip), device-resident inputs ----
 // queues the selection of a mini-batch on stream s (the calibration waits on s); keep and the K
 // per query are written by a kernel still in flight when it returns. 1 on success.

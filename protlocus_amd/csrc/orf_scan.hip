@@ -156,7 +156,70 @@ __global__ void __launch_bounds__(256) orf_gather_kernel(const OrfArgs a, const 
 
 std::mutex g_mu;
 
+__global__ void zero_count_kernel(unsigned long long *p)
+{
+	if (threadIdx.x == 0) *p = 0;
+}
+
+// the frame layout of a sequence of len bases: codons, chunks and first chunk of each frame
+int64_t layout(OrfArgs &a, int64_t len, int32_t min_orf)
+{
+	int64_t nblk = 0;
+	for (int p = 0; p < 3; ++p) {
+		const int64_t r = (p + 2) % 3;
+		a.ncod[p] = len > r? (len - r + 2) / 3 : 0;
+		a.nchunk[p] = (a.ncod[p] + CH - 1) / CH;
+		a.chunk0[p] = nblk;
+		nblk += a.nchunk[p];
+	}
+	a.len = len, a.min_orf = min_orf;
+	a.cap = 3 * (len / 3 / min_orf + 2);
+	return nblk;
+}
+
 } // namespace
+
+void nsh_orfs::Work::release()
+{
+	for (int i = 0; i < 6; ++i) if (p[i]) (void)hipFree(p[i]), p[i] = nullptr, n[i] = 0;
+	st = nullptr, len = nullptr;
+}
+
+int nsh_orfs::find(hipStream_t s, const uint8_t *d_nt, int64_t len, const uint8_t *d_codon, int32_t min_orf, Work &w,
+		int64_t *n_run)
+{
+	*n_run = 0;
+	if (len < 3) return 1;
+	OrfArgs a = {};
+	const int64_t nblk = layout(a, len, min_orf);
+	const size_t need[6] = { (size_t)nblk * 16 + 16, 16, (size_t)a.cap * 8 + 8, (size_t)a.cap * 4 + 8,
+		(size_t)a.cap * 8 + 8, (size_t)a.cap * 4 + 8 };
+	for (int i = 0; i < 6; ++i) {
+		if (w.n[i] >= need[i]) continue;
+		hipError_t e = hipStreamSynchronize(s);
+		if (w.p[i]) (void)hipFree(w.p[i]), w.p[i] = nullptr, w.n[i] = 0;
+		if (e == hipSuccess) e = hipMalloc(&w.p[i], need[i]);
+		if (e != hipSuccess) { nsh_set_error("orf runs: %s", hipGetErrorString(e)); return 0; }
+		w.n[i] = need[i];
+	}
+	a.nt = d_nt, a.codon = d_codon;
+	a.lastbad = (int64_t*)w.p[0], a.carry = (int64_t*)w.p[0] + nblk, a.count = (unsigned long long*)w.p[1];
+	a.st = (int64_t*)w.p[2], a.n_aa = (int32_t*)w.p[3], a.key = (uint64_t*)w.p[4], a.idx = (uint32_t*)w.p[5];
+	w.st = a.st, w.len = a.n_aa;
+	hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(64), 0, s, a.count);
+	hipLaunchKernelGGL(orf_summary_kernel, dim3((unsigned)nblk), dim3(NT), 0, s, a);
+	hipLaunchKernelGGL(orf_carry_kernel, dim3(1), dim3(64), 0, s, a);
+	hipLaunchKernelGGL(orf_emit_kernel, dim3((unsigned)nblk), dim3(NT), 0, s, a);
+	unsigned long long count = 0;
+	hipError_t e = hipGetLastError();
+	if (e == hipSuccess) e = hipMemcpyAsync(&count, a.count, 8, hipMemcpyDeviceToHost, s);
+	if (e == hipSuccess) e = hipStreamSynchronize(s);
+	if (e != hipSuccess) { nsh_set_error("orf runs: %s", hipGetErrorString(e)); return 0; }
+	if ((int64_t)count > a.cap) { nsh_set_error("orf runs: %llu runs exceed the bound %lld", count, (long long)a.cap); return 0; }
+	nsh_count_launch(4);
+	*n_run = (int64_t)count;
+	return 1;
+}
 
 #define OSOK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { nsh_set_error("orf_scan: %s: %s", #x, hipGetErrorString(e_)); goto fail; } } while (0)
 
@@ -172,16 +235,7 @@ extern "C" int64_t nsh_orf_scan(const uint8_t *nt, int64_t len, int32_t min_orf,
 	*n_aa_total = 0;
 	if (len < 3) return 0;
 	OrfArgs a = {};
-	int64_t nblk = 0;
-	for (int p = 0; p < 3; ++p) {
-		const int64_t r = (p + 2) % 3;
-		a.ncod[p] = len > r? (len - r + 2) / 3 : 0;
-		a.nchunk[p] = (a.ncod[p] + CH - 1) / CH;
-		a.chunk0[p] = nblk;
-		nblk += a.nchunk[p];
-	}
-	a.len = len, a.min_orf = min_orf;
-	a.cap = 3 * (len / 3 / min_orf + 2);
+	const int64_t nblk = layout(a, len, min_orf);
 	std::lock_guard<std::mutex> lk(g_mu);
 	static hipStream_t stream = nullptr;
 	uint8_t *d_nt = nullptr, *d_codon = nullptr, *d_aa = nullptr;

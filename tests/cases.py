@@ -356,3 +356,54 @@ def gapless_select_case(rng, k):
     adaptive = (k // 8) % 2
     return dict(cand_off=cand_off, qlen=qlen, scores=scores, chn=chn, vid=vid, vs=vs, top_k=top_k,
                 min_per_vid=min_per_vid, adaptive=adaptive)
+
+
+INDEX_VARIANTS = [(6, 1, 8, 30), (5, 0, 4, 10), (7, 2, 0, 20), (3, 1, 8, 5), (1, 0, 2, 1), (6, 3, 10, 45)]
+
+
+def index_case(rng, k, scale=1):
+    """a genome for the k-mer block index: 1-5 contigs (a tiny one and an all-N one sometimes), N
+    runs, GC varied; (kmer, mod_bit, bbit, min_aa_len) from INDEX_VARIANTS by k"""
+    n_ctg = int(rng.integers(1, 6))
+    lens = [int(rng.integers(1000, 12000 * scale)) for _ in range(n_ctg)]
+    if k % 3 == 1:
+        lens[int(rng.integers(0, n_ctg))] = int(rng.integers(0, 3))
+    parts = []
+    for i, ln in enumerate(lens):
+        gc = float(rng.uniform(0.35, 0.6))
+        c = rng.choice(4, size=ln, p=[(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2]).astype(np.uint8)
+        for _ in range(int(rng.integers(0, 4))):
+            if ln > 50:
+                s0 = int(rng.integers(0, ln - 40))
+                c[s0:s0 + int(rng.integers(1, 40))] = 4
+        if k % 3 == 2 and i == n_ctg - 1:
+            c[:] = 4
+        parts.append(c)
+    g = np.concatenate(parts) if parts else np.zeros(0, np.uint8)
+    off = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+    kmer, mod_bit, bbit, mal = INDEX_VARIANTS[k % len(INDEX_VARIANTS)]
+    return dict(genome=g, ctg_off=off, ctg_len=np.array(lens, np.int64), kmer=kmer, mod_bit=mod_bit, bbit=bbit,
+                min_aa_len=mal)
+
+
+def index_keys(ki, kb):
+    """the sorted unique entries bucket << 32 | block from the bucket starts and entries"""
+    ki = np.asarray(ki, dtype=np.int64)
+    cnt = np.diff(np.append(ki, len(kb)))
+    return (np.repeat(np.arange(len(ki), dtype=np.uint64), cnt) << np.uint64(32)) | np.asarray(kb, np.uint64)
+
+
+def pack_keys(keys):
+    """sorted uint64 keys as base64 of zlib-compressed deltas"""
+    import base64
+    import zlib
+    k = np.asarray(keys, dtype=np.uint64)
+    d = np.diff(np.concatenate([[np.uint64(0)], k])) if len(k) else k
+    return base64.b64encode(zlib.compress(d.astype(np.uint64).tobytes(), 9)).decode()
+
+
+def unpack_keys(text):
+    import base64
+    import zlib
+    d = np.frombuffer(zlib.decompress(base64.b64decode(text)), dtype=np.uint64)
+    return np.cumsum(d, dtype=np.uint64)

@@ -15,6 +15,7 @@ Outputs (inputs and expected outputs only):
   prescreen_cases.json.gz  mp_prescreen_build + ps_query_worker (prescreen.c): profile search candidates
   prescreen_batch_cases.json.gz  one mp_prescreen_build, ps_query_worker per profile of a batch
   gapless_select_cases.json.gz  mp_gapless_calibrate / _adaptive_k / mp_select_gapless (hit.c) on mini-batches
+  index_cases.json.gz  mp_idx_boff + mp_sketch_nt4 + build_bidx (index.c, sketch.c): k-mer block indexes
   tables.json        ns_tab_nt4 / ns_tab_aa20 / ns_tab_codon / ns_mat_blosum62
 """
 import base64
@@ -32,8 +33,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [os.path.join(ROOT, "tests"), ROOT]
 
 import nsffi                                       # noqa: E402
-from cases import (OUTPUT_VARIANTS, chain_case, gapless_select_case, gapped_case, nt4_codes, orf_case,  # noqa: E402
-                   output_case,
+from cases import (OUTPUT_VARIANTS, chain_case, gapless_select_case, gapped_case, index_case, index_keys,  # noqa: E402
+                   nt4_codes, orf_case, output_case, pack_keys,
                    parse_cigar, prescreen_batch_case, prescreen_case,
                    planted_case, random_opt_kw, random_ss)
 from nsffi import NS_F_CIGAR, NS_F_EXT_LEFT, NS_F_EXT_RIGHT  # noqa: E402
@@ -257,11 +258,28 @@ def gapless_select_cases(ref):
     return out
 
 
+def index_cases(ref):
+    """the reference's k-mer block index (oracle/idx_ref.c) on cases.index_case genomes: the
+    block offsets, the entry count and the entries as sorted bucket << 32 | block keys"""
+    import zlib
+    rng = np.random.default_rng(20261001)
+    out = []
+    for k in range(12):
+        c = index_case(rng, k)
+        ki, kb, bo = ref.index(c["genome"], c["ctg_off"], c["ctg_len"], c["min_aa_len"], c["kmer"], c["mod_bit"],
+                               c["bbit"])
+        out.append({"genome": B64(zlib.compress(c["genome"].tobytes(), 9)), "ctg_off": c["ctg_off"].tolist(),
+                    "ctg_len": c["ctg_len"].tolist(), "kmer": c["kmer"], "mod_bit": c["mod_bit"], "bbit": c["bbit"],
+                    "min_aa_len": c["min_aa_len"], "bo": bo.tolist(), "n_kb": int(len(kb)),
+                    "keys": pack_keys(index_keys(ki, kb))})
+    return out
+
+
 def tables(ref):
     L = ref.lib
     get = lambda name, n, t=C.c_uint8: list((t * n).in_dll(L, name))
     return {"nt4": get("ns_tab_nt4", 256), "aa20": get("ns_tab_aa20", 256), "codon": get("ns_tab_codon", 64),
-            "blosum62": get("ns_mat_blosum62", 484, C.c_int8)}
+            "blosum62": get("ns_mat_blosum62", 484, C.c_int8), "codon13": get("ns_tab_codon13", 64)}
 
 
 def main():
@@ -271,6 +289,7 @@ def main():
     gens = [("dp_cases", dp_cases), ("pssm_chain_cases", pssm_chain_cases), ("orf_cases", orf_cases),
             ("ps_gapped_cases", ps_gapped_cases), ("prescreen_cases", prescreen_cases),
             ("prescreen_batch_cases", prescreen_batch_cases), ("gapless_select_cases", gapless_select_cases),
+            ("index_cases", index_cases),
             ("output_cases", output_cases),
             ("hmm_cases", hmm_cases)]
     want = set(sys.argv[1:])

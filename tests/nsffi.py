@@ -226,6 +226,11 @@ class Reference(_Lib):
                                       C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_void_p, C.c_void_p]
         L.or_ref_ps_free.argtypes = [C.c_void_p]
         L.or_ref_gapless_select.argtypes = [C.c_int32] + [C.c_void_p] * 6 + [C.c_int32] * 3 + [C.c_void_p] * 3
+        L.or_ref_index_build.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 4
+        L.or_ref_index_build.restype = C.c_void_p
+        L.or_ref_index_get.argtypes = [C.c_void_p] * 4
+        L.or_ref_index_get.restype = C.c_int64
+        L.or_ref_index_free.argtypes = [C.c_void_p]
         assert L.ns_make_tables(1) == 0
         self._opt_init = L.ns_opt_init
         self._opt_set_sp = L.ns_opt_set_sp
@@ -333,6 +338,23 @@ class Reference(_Lib):
                                        adaptive, keep.ctypes.data, ks.ctypes.data, cal)
         dup = C.cast(C.addressof(cal) + 8, C.POINTER(C.c_float)).contents.value
         return keep[:n].tolist(), ks[:n_q].tolist(), (cal[0], cal[1], dup)
+
+    def index(self, genome, ctg_off, ctg_len, min_aa_len=30, kmer=6, mod_bit=1, bbit=8):
+        """the reference's k-mer block index (oracle/idx_ref.c): (ki, kb, bo) as numpy arrays"""
+        g = np.ascontiguousarray(genome, dtype=np.uint8)
+        off = np.ascontiguousarray(ctg_off, dtype=np.int64)
+        ln = np.ascontiguousarray(ctg_len, dtype=np.int64)
+        h = self.lib.or_ref_index_build(len(off), g.ctypes.data, off.ctypes.data, ln.ctypes.data, min_aa_len, kmer,
+                                        mod_bit, bbit)
+        try:
+            n_kb = self.lib.or_ref_index_get(h, None, None, None)
+            ki = np.zeros(1 << (4 * kmer - mod_bit), np.int64)
+            kb = np.zeros(max(1, n_kb), np.uint32)
+            bo = np.zeros(2 * len(off) + 1, np.uint32)
+            self.lib.or_ref_index_get(h, ki.ctypes.data, kb.ctypes.data, bo.ctypes.data)
+        finally:
+            self.lib.or_ref_index_free(h)
+        return ki, kb[:n_kb], bo
 
     def orfs(self, ctg_nt4, rev, min_orf):
         """ps_orf_worker on strand rev of a contig: [(start, codons)], residues"""

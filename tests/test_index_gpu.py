@@ -1,0 +1,55 @@
+"""GPU parity of the k-mer block index (nsh_index_build) against the reference's mp_idx_boff +
+mp_sketch_nt4 + build_bidx (index.c:14-29, 52-97; sketch.c:108-169): bucket starts, entries and
+block offsets identical, on the golden records of the compiled reference and on random genomes
+against the pinned restatement tests/index_ref.py."""
+import base64
+import gzip
+import json
+import os
+import zlib
+
+import numpy as np
+import pytest
+
+import index_ref
+from cases import index_case, index_keys, unpack_keys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+pytestmark = pytest.mark.gpu
+
+
+def test_index_golden(product):
+    from protlocus_amd import capi
+    with gzip.open(os.path.join(HERE, "golden", "index_cases.json.gz"), "rt") as f:
+        cases = json.load(f)
+    for k, c in enumerate(cases):
+        g = np.frombuffer(zlib.decompress(base64.b64decode(c["genome"])), dtype=np.uint8)
+        print("case", k, "genome", len(g), "contigs", len(c["ctg_len"]), "kmer", c["kmer"], flush=True)
+        ki, kb, bo = capi.index_build(g, c["ctg_off"], c["ctg_len"], c["min_aa_len"], c["kmer"], c["mod_bit"],
+                                      c["bbit"])
+        assert bo.tolist() == c["bo"] and len(kb) == c["n_kb"], k
+        assert np.array_equal(index_keys(ki, kb), unpack_keys(c["keys"])), k
+
+
+def test_index_random(product):
+    from protlocus_amd import capi
+    with open(os.path.join(HERE, "golden", "tables.json")) as f:
+        t = json.load(f)
+    rng = np.random.default_rng(99)
+    for k in range(6):
+        c = index_case(rng, k, scale=4)
+        print("case", k, "genome", len(c["genome"]), flush=True)
+        ki, kb, bo = capi.index_build(c["genome"], c["ctg_off"], c["ctg_len"], c["min_aa_len"], c["kmer"],
+                                      c["mod_bit"], c["bbit"])
+        wki, wkb, wbo, _ = index_ref.index_build(c["genome"], c["ctg_off"].tolist(), c["ctg_len"].tolist(),
+                                                 c["min_aa_len"], c["kmer"], c["mod_bit"], c["bbit"], t["codon"],
+                                                 t["codon13"])
+        assert np.array_equal(ki, wki) and kb.tolist() == list(wkb) and bo.tolist() == wbo, k
+
+
+def test_index_rejects_bad_arguments(product):
+    from protlocus_amd.capi import lib
+    g = np.zeros(10, np.uint8)
+    off, ln = np.zeros(1, np.int64), np.full(1, 10, np.int64)
+    for kmer, mod_bit, bbit, mal in ((8, 1, 8, 30), (6, 24, 8, 30), (6, 1, 31, 30), (6, 1, 8, 0)):
+        assert not lib().nsh_index_build(1, g.ctypes.data, off.ctypes.data, ln.ctypes.data, mal, kmer, mod_bit, bbit)
