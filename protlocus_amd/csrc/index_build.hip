@@ -160,7 +160,7 @@ extern "C" nsh_index_t *nsh_index_build(int32_t n_ctg, const uint8_t *nt, const 
 	void *d_tmp = nullptr;
 	nsh_orfs::Work ow;
 	int64_t cap = 0, n_raw = 0;
-	for (int32_t c = 0; c < n_ctg; ++c) cap += 2 * (ctg_len[c] / 3 + 1);
+	for (int32_t c = 0; c < n_ctg; ++c) cap += 2 * (ctg_len[c] + 1);   // at most one k-mer per codon end and strand
 	{
 		IXOK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
 		IXOK(hipMalloc((void**)&d_g, (size_t)tot + 8));
