@@ -16,7 +16,8 @@ Beside it, in the same JSON line:
     prescreened DP pipelines timed on fresh candidates (recall against the exact DP);
   * hmm: profile-HMM Viterbi scan, 64 profiles (M 100-800) x a 10-Mb genome in 6 frames;
   * profile_search: the profile prescreen search of the same profiles over the same genome;
-  * gapless_prefilter: gapless scores and top-K selection of one --gpu-prefilter mini-batch.
+  * gapless_prefilter: gapless scores and top-K selection of one --gpu-prefilter mini-batch;
+  * index: the k-mer block index of a 100-Mb genome.
 Output: one JSON line.
 """
 import argparse
@@ -304,6 +305,54 @@ def bench_gapless_prefilter(cpu=True, reps=5, log=None):
     return out
 
 
+def bench_index(cpu=True, genome_len=100_000_000, n_ctg=24, log=None):
+    """the k-mer block index of a 100-Mb genome (mp_idx_build's mp_idx_boff, mp_sketch_nt4 per
+    contig strand and build_bidx; index options kmer 6, mod_bit 1, bbit 8, min_aa_len 30): the GPU
+    build (genome upload included), and the reference's code (oracle/_ref) on one host thread, whose
+    bucket starts and entries must be identical"""
+    import numpy as np
+    import nsffi
+    from protlocus_amd import capi
+    rng = np.random.default_rng(17)
+    g = rng.choice(4, size=genome_len, p=[0.275, 0.225, 0.225, 0.275]).astype(np.uint8)
+    for s0 in rng.integers(0, genome_len - 1000, size=2000):
+        g[s0:s0 + int(rng.integers(1, 500))] = 4
+    cuts = np.sort(rng.choice(np.arange(10_000, genome_len - 10_000), n_ctg - 1, replace=False))
+    off = np.concatenate([[0], cuts]).astype(np.int64)
+    ln = np.diff(np.concatenate([off, [genome_len]])).astype(np.int64)
+    nsffi.Product()   # tables
+    capi.index_build(g[:100000], [0], [100000])   # warm-up
+    L = capi.lib()
+    t0 = time.perf_counter()
+    h = L.nsh_index_build(len(off), g.ctypes.data, off.ctypes.data, ln.ctypes.data, 30, 6, 1, 8)
+    gpu_s = time.perf_counter() - t0
+    assert h, capi.error()
+    L.nsh_index_destroy(h)
+    t0 = time.perf_counter()
+    ki, kb, bo = capi.index_build(g, off, ln)
+    copy_s = time.perf_counter() - t0
+    out = {"workload": "k-mer block index of a %d-Mb genome in %d contigs (GC 0.45, 2,000 N runs), both strands: "
+                       "6-mers of the 13-letter alphabet over ORFs >= 30 codons, mod_bit 1, 256-base blocks" % (
+                           genome_len // 1_000_000, n_ctg),
+           "entries": int(len(kb)), "buckets": int(len(ki)), "blocks": int(bo[-1]),
+           "gpu_ms": round(gpu_s * 1e3, 1), "gpu_mbases_per_s": round(genome_len / gpu_s / 1e6, 1),
+           "gpu_with_copy_out_ms": round(copy_s * 1e3, 1)}
+    if log:
+        log("[bench] index: %d entries, GPU %.1f ms" % (len(kb), gpu_s * 1e3))
+    if cpu and nsffi.have_ref():
+        ref = nsffi.Reference()
+        t0 = time.perf_counter()
+        wki, wkb, wbo = ref.index(g, off, ln)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(cpu_s * 1e3, 1), "unit": "ms per index (reference code, one host thread)",
+                               "cores": 1, "kind": "reference", "sample": "the whole genome"}
+        out["identical_to_reference"] = bool(np.array_equal(ki, wki) and np.array_equal(kb, wkb) and
+                                             np.array_equal(bo, wbo))
+        if log:
+            log("[bench] index: reference %.1f ms, identical %s" % (cpu_s * 1e3, out["identical_to_reference"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -318,6 +367,7 @@ def main():
                     help="run only the profile prescreen search section and print its JSON object")
     ap.add_argument("--only-gapless-prefilter", action="store_true",
                     help="run only the gapless prefilter section and print its JSON object")
+    ap.add_argument("--only-index", action="store_true", help="run only the k-mer index section and print its JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device three times (a copy kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
@@ -350,6 +400,9 @@ def main():
         return
     if args.only_gapless_prefilter:
         print(json.dumps(bench_gapless_prefilter(cpu=not args.no_cpu, log=log)), flush=True)
+        return
+    if args.only_index:
+        print(json.dumps(bench_index(cpu=not args.no_cpu, log=log)), flush=True)
         return
 
     import nsffi
@@ -433,6 +486,7 @@ def main():
         log("[bench] hmm: %.1f GCUPS" % out["hmm"]["gcups"])
         out["profile_search"] = bench_profile_search(cpu=not args.no_cpu, log=log)
         out["gapless_prefilter"] = bench_gapless_prefilter(cpu=not args.no_cpu, log=log)
+        out["index"] = bench_index(cpu=not args.no_cpu, log=log)
     print(json.dumps(out), flush=True)
 
 

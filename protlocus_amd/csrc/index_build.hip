@@ -9,11 +9,11 @@
 // strand. The buckets are then filled strand by strand in that order.
 //
 // Here the whole genome goes to the device once; a reverse strand is complemented there. The runs
-// come from the ORF walk kernels (orf_scan.hip, unsorted). One wave per run emits its k-mers
-// with a ballot-packed append, every strand into one buffer. Block numbers are disjoint between
-// strands and increase with the strand index, so one global radix sort and one unique pass give
-// every bucket's entries in the reference's order, and a histogram and a scan give the bucket
-// starts.
+// come from the ORF walk kernels (orf_scan.hip, unsorted). One wave per run emits its k-mers, in
+// a counting pass, a scan over the runs and a writing pass (no shared counter), every strand into
+// one buffer. Block numbers are disjoint between strands and increase with the strand index, so
+// one global radix sort and one unique pass give every bucket's entries in the reference's order,
+// and a histogram and a scan give the bucket starts.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 #include <algorithm>
@@ -50,43 +50,62 @@ struct SkArgs {
 	int32_t kmer, mod_bit, bbit;
 	uint32_t mask_k, mask_mod;
 	int64_t boff;
-	unsigned long long *count; int64_t cap;
+	int64_t *cnt;                 // per run: kept k-mers (counting pass, PASS 0)
+	const int64_t *off;           // per run: first output slot (writing pass, PASS 1)
 	uint64_t *out;
 };
 
-// one wave per run (mp_sketch_clean_orf, sketch.c:108-128): codon t (t >= kmer - 1) closes the
-// k-mer of codons t-kmer+1..t, 4 bits each, the newest lowest
+// one wave per run (mp_sketch_clean_orf, sketch.c:108-128), 64 codons per step: each lane
+// translates one codon to its 13-letter code, and the k-mer closing at codon t (t >= kmer - 1),
+// codons t-kmer+1..t at 4 bits each, the newest lowest, gathers the older codes from the lanes
+// below by DPP shifts (lane 0 takes the previous step's lane 63). PASS 0 counts the kept k-mers
+// of the run, PASS 1 writes them at the run's scanned offset.
+template <int PASS>
 __global__ void __launch_bounds__(256) sketch_kernel(SkArgs a)
 {
+	__shared__ uint8_t c13[64];
+	if (threadIdx.x < 64) c13[threadIdx.x] = a.codon13[threadIdx.x];
+	__syncthreads();
 	const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-	if (r >= a.n_run) return;   // whole waves
+	if (r >= a.n_run) return;   // whole waves; no barrier follows
 	const int L = threadIdx.x & 63;
 	const int64_t st = a.st[r];
 	const int32_t k = a.len[r];
+	uint32_t last[6] = { 0, 0, 0, 0, 0, 0 };   // lane 63 of the previous step, per shift
+	int64_t n = 0, o = PASS? a.off[r] : 0;
 	for (int32_t t0 = 0; t0 < k; t0 += 64) {
 		const int32_t t = t0 + L;
+		uint32_t code = 0;
+		if (t < k) {
+			const uint8_t *c = a.nt + st + 3 * (int64_t)t;
+			code = c13[c[0] << 4 | c[1] << 2 | c[2]];
+		}
+		uint32_t x = code, cur = code;
+#pragma unroll
+		for (int s = 1; s < 7; ++s) {
+			if (s >= a.kmer) break;
+			const uint32_t nx = (uint32_t)__builtin_amdgcn_update_dpp((int)last[s - 1], (int)cur, 0x138, 0xf, 0xf, false);
+			last[s - 1] = (uint32_t)__builtin_amdgcn_readlane((int)cur, 63);
+			cur = nx;
+			x |= cur << (4 * s);
+		}
+		x &= a.mask_k;
 		bool emit = false;
-		uint64_t v = 0;
+		uint32_t y = 0;
 		if (t < k && t >= a.kmer - 1) {
-			uint32_t x = 0;
-			for (int s = 0; s < a.kmer; ++s) {
-				const uint8_t *c = a.nt + st + 3 * (int64_t)(t - a.kmer + 1 + s);
-				x = (x << 4 | a.codon13[c[0] << 4 | c[1] << 2 | c[2]]) & a.mask_k;
-			}
-			const uint32_t y = hash32_mask(x, a.mask_k);
-			if ((y & a.mask_mod) == 0) {
-				emit = true;
-				v = (uint64_t)(y >> a.mod_bit) << 32 | (uint32_t)(((st + 3 * (int64_t)t + 2) >> a.bbit) + a.boff);
-			}
+			y = hash32_mask(x, a.mask_k);
+			emit = (y & a.mask_mod) == 0;
 		}
 		const uint64_t bal = __ballot(emit);
-		if (!bal) continue;
-		unsigned long long base = 0;
-		if (L == 0) base = atomicAdd(a.count, (unsigned long long)__popcll(bal));
-		base = __shfl(base, 0);
-		const int64_t slot = (int64_t)base + __popcll(bal & ((1ull << L) - 1));
-		if (emit && slot < a.cap) a.out[slot] = v;
+		if (PASS == 0) {
+			n += __popcll(bal);
+		} else if (emit) {
+			a.out[o + __popcll(bal & ((1ull << L) - 1))] =
+				(uint64_t)(y >> a.mod_bit) << 32 | (uint32_t)(((st + 3 * (int64_t)t + 2) >> a.bbit) + a.boff);
+		}
+		if (PASS) o += __popcll(bal);
 	}
+	if (PASS == 0 && L == 0) a.cnt[r] = n;
 }
 
 __global__ void zero_u64_kernel(unsigned long long *p, int64_t n)
@@ -155,11 +174,13 @@ extern "C" nsh_index_t *nsh_index_build(int32_t n_ctg, const uint8_t *nt, const 
 	hipStream_t s = nullptr;
 	uint8_t *d_g = nullptr, *d_rc = nullptr, *d_cod = nullptr, *d_cod13 = nullptr;
 	uint64_t *d_raw = nullptr, *d_sorted = nullptr;
-	unsigned long long *d_count = nullptr, *d_cnt = nullptr;
+	unsigned long long *d_cnt = nullptr;
 	int64_t *d_nuniq = nullptr;
 	void *d_tmp = nullptr;
 	nsh_orfs::Work ow;
-	int64_t cap = 0, n_raw = 0;
+	int64_t cap = 0, n_raw = 0, run_cap = 0, *d_rcnt = nullptr, *d_roff = nullptr;
+	void *d_scan = nullptr;
+	size_t scan_bytes = 0;
 	for (int32_t c = 0; c < n_ctg; ++c) cap += 2 * (ctg_len[c] + 1);   // at most one k-mer per codon end and strand
 	{
 		IXOK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -167,12 +188,10 @@ extern "C" nsh_index_t *nsh_index_build(int32_t n_ctg, const uint8_t *nt, const 
 		IXOK(hipMalloc((void**)&d_rc, (size_t)max_len + 8));
 		IXOK(hipMalloc((void**)&d_cod, 64));
 		IXOK(hipMalloc((void**)&d_cod13, 64));
-		IXOK(hipMalloc((void**)&d_count, 8));
 		IXOK(hipMalloc((void**)&d_raw, (size_t)cap * 8 + 8));
 		if (tot) IXOK(hipMemcpyAsync(d_g, nt, (size_t)tot, hipMemcpyHostToDevice, s));
 		IXOK(hipMemcpyAsync(d_cod, ns_tab_codon, 64, hipMemcpyHostToDevice, s));
 		IXOK(hipMemcpyAsync(d_cod13, ns_tab_codon13, 64, hipMemcpyHostToDevice, s));
-		hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, d_count, (int64_t)1);
 		for (int32_t j = 0; j < 2 * n_ctg; ++j) {   // contig strands (mp_ntseq_get orientation)
 			const int64_t len = ctg_len[j >> 1];
 			const uint8_t *seq = d_g + ctg_off[j >> 1];
@@ -188,16 +207,41 @@ extern "C" nsh_index_t *nsh_index_build(int32_t n_ctg, const uint8_t *nt, const 
 			a.nt = seq, a.codon13 = d_cod13, a.st = ow.st, a.len = ow.len, a.n_run = n_run;
 			a.kmer = kmer, a.mod_bit = mod_bit, a.bbit = bbit;
 			a.mask_k = (uint32_t)(((uint64_t)1 << (4 * kmer)) - 1), a.mask_mod = (1u << mod_bit) - 1;
-			a.boff = ix->bo[j], a.count = d_count, a.cap = cap, a.out = d_raw;
-			hipLaunchKernelGGL(sketch_kernel, dim3((unsigned)((n_run + 3) / 4)), dim3(256), 0, s, a);
+			a.boff = ix->bo[j];
+			// kept k-mers per run, their offsets after what the earlier strands wrote, then the k-mers
+			if (n_run + 1 > run_cap) {
+				IXOK(hipStreamSynchronize(s));
+				if (d_rcnt) (void)hipFree(d_rcnt);
+				if (d_roff) (void)hipFree(d_roff);
+				d_rcnt = d_roff = nullptr;
+				run_cap = n_run + 1 + n_run / 2;
+				IXOK(hipMalloc((void**)&d_rcnt, (size_t)run_cap * 8));
+				IXOK(hipMalloc((void**)&d_roff, (size_t)run_cap * 8));
+			}
+			a.cnt = d_rcnt, a.off = d_roff, a.out = d_raw + n_raw;
+			hipLaunchKernelGGL(sketch_kernel<0>, dim3((unsigned)((n_run + 3) / 4)), dim3(256), 0, s, a);
+			hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, (unsigned long long*)(d_rcnt + n_run), (int64_t)1);
+			size_t sb = 0;
+			IXOK(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, d_rcnt, d_roff, (int)(n_run + 1), s));
+			if (sb > scan_bytes) {
+				IXOK(hipStreamSynchronize(s));
+				if (d_scan) (void)hipFree(d_scan);
+				d_scan = nullptr;
+				scan_bytes = sb + sb / 2;
+				IXOK(hipMalloc(&d_scan, scan_bytes));
+			}
+			sb = scan_bytes;
+			IXOK(hipcub::DeviceScan::ExclusiveSum(d_scan, sb, d_rcnt, d_roff, (int)(n_run + 1), s));
+			int64_t n_strand = 0;
+			IXOK(hipMemcpyAsync(&n_strand, d_roff + n_run, 8, hipMemcpyDeviceToHost, s));
+			IXOK(hipStreamSynchronize(s));
+			if (n_raw + n_strand > cap) { nsh_set_error("index: entries exceed the bound %lld", (long long)cap); goto fail; }
+			hipLaunchKernelGGL(sketch_kernel<1>, dim3((unsigned)((n_run + 3) / 4)), dim3(256), 0, s, a);
 			IXOK(hipGetLastError());
-			nsh_count_launch(1 + (j & 1));
+			n_raw += n_strand;
+			nsh_count_launch(4 + (j & 1));
 		}
-		unsigned long long cnt_h = 0;
-		IXOK(hipMemcpyAsync(&cnt_h, d_count, 8, hipMemcpyDeviceToHost, s));
 		IXOK(hipStreamSynchronize(s));
-		n_raw = (int64_t)cnt_h;
-		if (n_raw > cap) { nsh_set_error("index: %lld entries exceed the bound %lld", (long long)n_raw, (long long)cap); goto fail; }
 		if (n_raw > INT_MAX) { nsh_set_error("index: %lld entries exceed one sort", (long long)n_raw); goto fail; }
 		// one sort over every strand, unique entries, bucket sizes, bucket starts
 		const int end_bit = 32 + 4 * kmer - mod_bit;
@@ -231,7 +275,8 @@ extern "C" nsh_index_t *nsh_index_build(int32_t n_ctg, const uint8_t *nt, const 
 		nsh_count_launch(6);
 	}
 	{
-		void *bufs[] = { d_g, d_rc, d_cod, d_cod13, d_raw, d_sorted, d_count, d_cnt, d_nuniq, d_tmp };
+		void *bufs[] = { d_g, d_rc, d_cod, d_cod13, d_raw, d_sorted, d_cnt, d_nuniq, d_tmp, d_rcnt, d_roff,
+			d_scan };
 		for (void *b : bufs) if (b) (void)hipFree(b);
 		ow.release();
 		(void)hipStreamDestroy(s);
@@ -240,7 +285,8 @@ extern "C" nsh_index_t *nsh_index_build(int32_t n_ctg, const uint8_t *nt, const 
 fail:
 	{
 		if (s) (void)hipStreamSynchronize(s);
-		void *bufs[] = { d_g, d_rc, d_cod, d_cod13, d_raw, d_sorted, d_count, d_cnt, d_nuniq, d_tmp };
+		void *bufs[] = { d_g, d_rc, d_cod, d_cod13, d_raw, d_sorted, d_cnt, d_nuniq, d_tmp, d_rcnt, d_roff,
+			d_scan };
 		for (void *b : bufs) if (b) (void)hipFree(b);
 		ow.release();
 		if (s) (void)hipStreamDestroy(s);

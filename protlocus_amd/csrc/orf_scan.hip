@@ -77,16 +77,22 @@ __global__ void __launch_bounds__(NT) orf_summary_kernel(OrfArgs a)
 	if (threadIdx.x == 0) a.lastbad[blockIdx.x] = m;
 }
 
-// one thread per frame: exclusive running maximum over the frame's chunks (a few thousand at most)
-__global__ void orf_carry_kernel(OrfArgs a)
+// one block per frame: exclusive running maximum over the frame's chunks, a tile of NT chunks
+// at a time with the maximum of the earlier tiles carried along
+__global__ void __launch_bounds__(NT) orf_carry_kernel(OrfArgs a)
 {
-	const int p = threadIdx.x;
-	if (p >= 3) return;
+	const int p = blockIdx.x;
+	typedef hipcub::BlockScan<int64_t, NT> Scan;
+	__shared__ typename Scan::TempStorage tmp;
 	int64_t run = -1;
-	for (int64_t ch = 0; ch < a.nchunk[p]; ++ch) {
-		a.carry[a.chunk0[p] + ch] = run;
-		const int64_t v = a.lastbad[a.chunk0[p] + ch];
-		run = v > run? v : run;
+	for (int64_t t0 = 0; t0 < a.nchunk[p]; t0 += NT) {
+		const int64_t ch = t0 + threadIdx.x;
+		const int64_t v = ch < a.nchunk[p]? a.lastbad[a.chunk0[p] + ch] : -1;
+		int64_t before, tile;
+		Scan(tmp).ExclusiveScan(v, before, (int64_t)-1, hipcub::Max(), tile);
+		if (ch < a.nchunk[p]) a.carry[a.chunk0[p] + ch] = before > run? before : run;
+		run = tile > run? tile : run;
+		__syncthreads();   // the scan storage is reused by the next tile
 	}
 }
 
@@ -105,11 +111,33 @@ __global__ void __launch_bounds__(NT) orf_emit_kernel(OrfArgs a)
 		if (u < PER && s[u] != 0 && c < a.ncod[p]) last = c;
 	}
 	typedef hipcub::BlockScan<int64_t, NT> Scan;
+	typedef hipcub::BlockScan<int32_t, NT> Count;
 	__shared__ typename Scan::TempStorage tmp;
+	__shared__ typename Count::TempStorage tmp2;
+	__shared__ unsigned long long base;
 	int64_t before;
 	Scan(tmp).ExclusiveScan(last, before, (int64_t)-1, hipcub::Max());
 	const int64_t carry = a.carry[blockIdx.x];
-	int64_t bad = before > carry? before : carry;   // last non-good codon before cb
+	const int64_t bad0 = before > carry? before : carry;   // last non-good codon before cb
+	// runs ending in this thread's codons: counted, then one atomic per block places them
+	int32_t n_emit = 0;
+	{
+		int64_t bad = bad0;
+#pragma unroll
+		for (int u = 0; u < PER; ++u) {
+			const int64_t c = cb + u;
+			if (c >= a.ncod[p]) break;
+			if (s[u] != 0) { bad = c; continue; }
+			if (s[u + 1] == 0) continue;
+			if (c - bad >= a.min_orf) ++n_emit;
+		}
+	}
+	int32_t off, total;
+	Count(tmp2).ExclusiveSum(n_emit, off, total);
+	if (threadIdx.x == 0) base = total? atomicAdd(a.count, (unsigned long long)total) : 0ull;
+	__syncthreads();
+	unsigned long long slot = base + (unsigned long long)off;
+	int64_t bad = bad0;
 #pragma unroll
 	for (int u = 0; u < PER; ++u) {
 		const int64_t c = cb + u;
@@ -127,13 +155,13 @@ __global__ void __launch_bounds__(NT) orf_emit_kernel(OrfArgs a)
 			for (int64_t x = e + 1; x <= lim; ++x)
 				if (a.nt[x] > 3) { ev = x; break; }
 		}
-		const unsigned long long slot = atomicAdd(a.count, 1ull);
 		if ((int64_t)slot < a.cap) {
 			a.key[slot] = (uint64_t)ev * 4 + (uint64_t)p;
 			a.idx[slot] = (uint32_t)slot;
 			a.st[slot] = e + 1 - 3 * k;
 			a.n_aa[slot] = (int32_t)k;
 		}
+		++slot;
 	}
 }
 
@@ -208,7 +236,7 @@ int nsh_orfs::find(hipStream_t s, const uint8_t *d_nt, int64_t len, const uint8_
 	w.st = a.st, w.len = a.n_aa;
 	hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(64), 0, s, a.count);
 	hipLaunchKernelGGL(orf_summary_kernel, dim3((unsigned)nblk), dim3(NT), 0, s, a);
-	hipLaunchKernelGGL(orf_carry_kernel, dim3(1), dim3(64), 0, s, a);
+	hipLaunchKernelGGL(orf_carry_kernel, dim3(3), dim3(NT), 0, s, a);
 	hipLaunchKernelGGL(orf_emit_kernel, dim3((unsigned)nblk), dim3(NT), 0, s, a);
 	unsigned long long count = 0;
 	hipError_t e = hipGetLastError();
@@ -266,7 +294,7 @@ extern "C" int64_t nsh_orf_scan(const uint8_t *nt, int64_t len, int32_t min_orf,
 		a.nt = d_nt, a.codon = d_codon, a.lastbad = d_chunk, a.carry = d_chunk + nblk, a.count = d_count;
 		a.key = d_key, a.idx = d_idx, a.st = d_st, a.n_aa = d_len;
 		hipLaunchKernelGGL(orf_summary_kernel, dim3((unsigned)nblk), dim3(NT), 0, stream, a);
-		hipLaunchKernelGGL(orf_carry_kernel, dim3(1), dim3(64), 0, stream, a);
+		hipLaunchKernelGGL(orf_carry_kernel, dim3(3), dim3(NT), 0, stream, a);
 		hipLaunchKernelGGL(orf_emit_kernel, dim3((unsigned)nblk), dim3(NT), 0, stream, a);
 		OSOK(hipGetLastError());
 		OSOK(hipMemcpyAsync(&count, d_count, 8, hipMemcpyDeviceToHost, stream));
