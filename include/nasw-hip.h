@@ -339,6 +339,15 @@ nsh_index_t *nsh_index_build(int32_t n_ctg, const uint8_t *nt, const int64_t *ct
 int64_t nsh_index_size(const nsh_index_t *ix, int32_t what);
 int nsh_index_copy(const nsh_index_t *ix, int64_t *ki, uint32_t *kb, uint32_t *bo);
 void nsh_index_destroy(nsh_index_t *ix);
+/* The anchors of n_q queries (ASCII residues at seq + q_off[q], q_len[q] of them) against the
+ * index, as mp_map_coarse collects them with the default options (map.c:583-597): the protein
+ * sketch (mp_sketch_prot, sketch.c:18-38), the repeat cutoff (mp_cal_max_occ, map.c:304-318)
+ * capped at max_occ, the bucket entries of the seeds under it (mp_collect_anchors, map.c:336-395,
+ * without the --accuracy repeat rescue), each query's anchors sorted. An anchor is
+ * block << 32 | query position. Query q's anchors are anchors[a_off[q] .. a_off[q+1]) (a_off has
+ * n_q + 1 entries); returns the total (only max_anchors are written), or -1 on error. */
+int64_t nsh_index_anchors(const nsh_index_t *ix, int32_t n_q, const char *seq, const int64_t *q_off,
+		const int32_t *q_len, int32_t max_occ, int64_t *a_off, uint64_t *anchors, int64_t max_anchors);
 
 /* Learned chain-candidate prescreen: MLP 64 -> 256 -> 256 -> 1 (ReLU, logistic output), BF16
  * MFMA kernels with FP32 accumulation, FP32 Adam masters. Parameter vector (FP32, in order):

@@ -56,3 +56,9 @@ void or_ref_index_free(void *handle)
 	free(r->mi.ki), free(r->mi.kb), free(r->mi.bo);
 	free(r->nt.ctg), free(r->nt.seq), free(r);
 }
+
+/* the index inside a handle, for the anchor adaptor in map_ref.c */
+const mp_idx_t *or_ref_index_mi(void *handle)
+{
+	return &((or_idx_t*)handle)->mi;
+}

@@ -136,6 +136,9 @@ def lib():
         L.nsh_index_size.restype = C.c_int64
         L.nsh_index_copy.argtypes = [C.c_void_p] * 4
         L.nsh_index_destroy.argtypes = [C.c_void_p]
+        L.nsh_index_anchors.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                        C.c_void_p, C.c_void_p, C.c_int64]
+        L.nsh_index_anchors.restype = C.c_int64
         L.nsh_gapless_select_batch.argtypes = [C.c_int32] + [C.c_void_p] * 6 + [C.c_int32] * 3 + [C.c_void_p] * 3
         L.nsh_gapless_prefilter_batch.argtypes = ([C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32,
                                                    C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
@@ -435,6 +438,39 @@ def index_build(genome, ctg_off, ctg_len, min_aa_len=30, kmer=6, mod_bit=1, bbit
     finally:
         L.nsh_index_destroy(h)
     return ki, kb[:n_kb], bo
+
+
+class Index:
+    """a k-mer block index resident on the device (nsh_index_*)"""
+
+    def __init__(self, genome, ctg_off, ctg_len, min_aa_len=30, kmer=6, mod_bit=1, bbit=8):
+        g = np.ascontiguousarray(genome, dtype=np.uint8)
+        off = np.ascontiguousarray(ctg_off, dtype=np.int64)
+        ln = np.ascontiguousarray(ctg_len, dtype=np.int64)
+        self.p = lib().nsh_index_build(len(off), g.ctypes.data, off.ctypes.data, ln.ctypes.data, min_aa_len, kmer,
+                                       mod_bit, bbit)
+        if not self.p:
+            raise RuntimeError("nsh_index_build: " + error())
+
+    def anchors(self, queries, max_occ=20000):
+        """the sorted anchors (block << 32 | query position) of each query (ASCII bytes)"""
+        seq = b"".join(bytes(q) for q in queries)
+        ln = np.array([len(q) for q in queries], dtype=np.int32)
+        off = np.concatenate([[0], np.cumsum(ln)[:-1]]).astype(np.int64) if len(ln) else np.zeros(0, np.int64)
+        aoff = np.zeros(len(ln) + 1, np.int64)
+        n = lib().nsh_index_anchors(self.p, len(ln), seq, off.ctypes.data, ln.ctypes.data, max_occ, aoff.ctypes.data,
+                                    None, 0)
+        if n < 0:
+            raise RuntimeError("nsh_index_anchors: " + error())
+        a = np.zeros(max(1, n), np.uint64)
+        n = lib().nsh_index_anchors(self.p, len(ln), seq, off.ctypes.data, ln.ctypes.data, max_occ, aoff.ctypes.data,
+                                    a.ctypes.data, n)
+        return [a[aoff[q]:aoff[q + 1]] for q in range(len(ln))]
+
+    def close(self):
+        if self.p:
+            lib().nsh_index_destroy(self.p)
+            self.p = None
 
 
 class GaplessCalib(C.Structure):

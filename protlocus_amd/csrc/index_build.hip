@@ -126,6 +126,114 @@ __global__ void bucket_kernel(const uint64_t *key, const int64_t *n_key, unsigne
 
 unsigned blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
 
+// ---- anchors of a batch of queries (map.c:583-597, default options) ----
+
+struct AnArgs {
+	const uint8_t *seq; const int64_t *qoff; const int32_t *qlen; int32_t n_q;
+	const int64_t *sbase;       // per query: first slot of its seed region (q_len slots)
+	const uint8_t *aa13;
+	int32_t kmer, mod_bit;
+	uint32_t mask_k, mask_mod;
+	const int64_t *ki; const uint32_t *kb; int64_t n_bucket;
+	uint64_t *seed;             // per query, at sbase: hash >> mod_bit << 32 | position, compacted
+	uint64_t *cnt;              // per seed: its bucket size (then sorted per query)
+	int32_t *n_seed, *max_occ;
+	int32_t occ_cap;
+	int64_t *n_anchor;          // per query: anchors kept
+	const int64_t *aoff;        // per query: first anchor
+	uint64_t *anchor;
+};
+
+// one wave per query: the protein sketch (mp_sketch_prot, sketch.c:18-38). Position i closes a
+// k-mer when its last kmer residues are all in the 13-letter alphabet; the kept ones (low
+// mod_bit hash bits zero) are compacted in order, with their bucket sizes.
+__global__ void __launch_bounds__(64) qsketch_kernel(AnArgs a)
+{
+	const int q = blockIdx.x, L = threadIdx.x;
+	const uint8_t *s = a.seq + a.qoff[q];
+	const int32_t n = a.qlen[q];
+	int32_t o = 0;
+	for (int32_t i0 = 0; i0 < n; i0 += 64) {
+		const int32_t i = i0 + L;
+		bool emit = false;
+		uint32_t y = 0;
+		if (i < n && i >= a.kmer - 1) {
+			uint32_t x = 0;
+			bool ok = true;
+			for (int t = 0; t < a.kmer; ++t) {
+				const uint32_t c = a.aa13[s[i - a.kmer + 1 + t]];
+				ok = ok && c < 14;
+				x = (x << 4 | c) & a.mask_k;
+			}
+			if (ok) {
+				y = hash32_mask(x, a.mask_k);
+				emit = (y & a.mask_mod) == 0;
+			}
+		}
+		const uint64_t bal = __ballot(emit);
+		if (emit) {
+			const int64_t k = a.sbase[q] + o + __popcll(bal & ((1ull << L) - 1));
+			const uint64_t b = y >> a.mod_bit;
+			a.seed[k] = b << 32 | (uint32_t)i;
+			a.cnt[k] = (uint64_t)(a.ki[b + 1] - a.ki[b]);
+		}
+		o += __popcll(bal);
+	}
+	if (L == 0) a.n_seed[q] = o;
+}
+
+// per query: the repeat cutoff (mp_cal_max_occ, map.c:304-318) from its sorted bucket sizes,
+// capped at the option's max_occ; fewer than 8 seeds keep the option
+__global__ void occ_kernel(AnArgs a)
+{
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= a.n_q) return;
+	const int32_t n = a.n_seed[q];
+	int32_t occ = a.occ_cap;
+	if (n >= 8) {
+		const uint64_t *c = a.cnt + a.sbase[q];
+		const uint64_t q25 = c[(int64_t)(n * .25 + .499)], q75 = c[(int64_t)(n * .75 + .499)];
+		const int32_t v = (int32_t)(q75 + (q75 - q25) * 1.5 + 10.);
+		occ = v < occ? v : occ;
+	}
+	a.max_occ[q] = occ;
+}
+
+// one wave per query: bucket entries of the seeds under the cutoff (mp_collect_anchors,
+// map.c:336-395, no repeat rescue); PASS 0 counts them, PASS 1 writes
+// kb << 32 | position at the query's offset
+template <int PASS>
+__global__ void __launch_bounds__(64) collect_kernel(AnArgs a)
+{
+	const int q = blockIdx.x, L = threadIdx.x;
+	const int32_t n = a.n_seed[q], occ = a.max_occ[q];
+	const uint64_t *sd = a.seed + a.sbase[q];
+	int64_t o = PASS? a.aoff[q] : 0;
+	for (int32_t i0 = 0; i0 < n; i0 += 64) {
+		const int32_t i = i0 + L;
+		int64_t st = 0, take = 0;
+		if (i < n) {
+			const uint64_t b = sd[i] >> 32;
+			st = a.ki[b];
+			take = a.ki[b + 1] - st;
+			if (take > occ) take = 0;
+		}
+		// inclusive scan of the takes across the wave
+		int64_t inc = take;
+		for (int d = 1; d < 64; d <<= 1) {
+			const int64_t t = __shfl_up(inc, d);
+			if (L >= d) inc += t;
+		}
+		if (PASS) {
+			const uint32_t pos = i < n? (uint32_t)sd[i] : 0;
+			for (int64_t j = 0; j < take; ++j)
+				a.anchor[o + inc - take + j] = (uint64_t)a.kb[st + j] << 32 | pos;
+		}
+		o += __shfl(inc, 63);
+	}
+	if (!PASS && L == 0) a.n_anchor[q] = o;
+}
+
 } // namespace
 
 struct nsh_index_s {
@@ -314,4 +422,120 @@ extern "C" int nsh_index_copy(const nsh_index_t *ix, int64_t *ki, uint32_t *kb, 
 	}
 	if (bo) std::copy(ix->bo.begin(), ix->bo.end(), bo);
 	return 1;
+}
+
+#define ANOK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { nsh_set_error("anchors: %s: %s", #x, hipGetErrorString(e_)); goto done; } } while (0)
+
+extern "C" int64_t nsh_index_anchors(const nsh_index_t *ix, int32_t n_q, const char *seq, const int64_t *q_off,
+		const int32_t *q_len, int32_t max_occ, int64_t *a_off, uint64_t *anchors, int64_t max_anchors)
+{
+	if (!ix || n_q < 0 || (n_q > 0 && (!seq || !q_off || !q_len || !a_off)) || max_anchors < 0 ||
+			(max_anchors > 0 && !anchors)) {
+		nsh_set_error("anchors: bad arguments");
+		return -1;
+	}
+	int64_t l_seq = 0, l_seed = 0;
+	std::vector<int64_t> sbase(n_q + 1, 0);
+	for (int32_t q = 0; q < n_q; ++q) {
+		if (q_off[q] < 0 || q_len[q] < 0) { nsh_set_error("anchors: query %d out of range", q); return -1; }
+		l_seq = std::max(l_seq, q_off[q] + q_len[q]);
+		sbase[q + 1] = sbase[q] + q_len[q];
+	}
+	l_seed = sbase[n_q];
+	if (l_seed > INT_MAX) { nsh_set_error("anchors: %lld query residues exceed one sort", (long long)l_seed); return -1; }
+	if (n_q == 0) { if (a_off) a_off[0] = 0; return 0; }
+	int64_t total = -1;
+	hipStream_t s = nullptr;
+	uint8_t *d_seq = nullptr, *d_aa13 = nullptr;
+	int64_t *d_qoff = nullptr, *d_nanch = nullptr, *d_aoff = nullptr, *d_soff = nullptr, *d_send = nullptr,
+		*d_sbase = nullptr;
+	int32_t *d_qlen = nullptr, *d_nseed = nullptr, *d_occ = nullptr;
+	uint64_t *d_seed = nullptr, *d_cnt = nullptr, *d_cnt2 = nullptr, *d_anch = nullptr, *d_anch2 = nullptr;
+	void *d_tmp = nullptr;
+	size_t tb = 0, t2 = 0;
+	std::vector<int64_t> aoff(n_q + 1, 0), nan(n_q), soff(n_q), send(n_q);
+	std::vector<int32_t> nseed(n_q);
+	AnArgs a = {};
+	{
+		ANOK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+		ANOK(hipMalloc((void**)&d_seq, (size_t)l_seq + 8));
+		ANOK(hipMalloc((void**)&d_aa13, 256));
+		ANOK(hipMalloc((void**)&d_qoff, (size_t)n_q * 8));
+		ANOK(hipMalloc((void**)&d_qlen, (size_t)n_q * 4));
+		ANOK(hipMalloc((void**)&d_nseed, (size_t)n_q * 4));
+		ANOK(hipMalloc((void**)&d_occ, (size_t)n_q * 4));
+		ANOK(hipMalloc((void**)&d_nanch, (size_t)n_q * 8));
+		ANOK(hipMalloc((void**)&d_aoff, (size_t)(n_q + 1) * 8));
+		ANOK(hipMalloc((void**)&d_soff, (size_t)n_q * 8));
+		ANOK(hipMalloc((void**)&d_send, (size_t)n_q * 8));
+		ANOK(hipMalloc((void**)&d_sbase, (size_t)(n_q + 1) * 8));
+		ANOK(hipMalloc((void**)&d_seed, (size_t)l_seed * 8 + 8));
+		ANOK(hipMalloc((void**)&d_cnt, (size_t)l_seed * 8 + 8));
+		ANOK(hipMalloc((void**)&d_cnt2, (size_t)l_seed * 8 + 8));
+		ANOK(hipMemcpyAsync(d_sbase, sbase.data(), (size_t)(n_q + 1) * 8, hipMemcpyHostToDevice, s));
+		if (l_seq) ANOK(hipMemcpyAsync(d_seq, seq, (size_t)l_seq, hipMemcpyHostToDevice, s));
+		ANOK(hipMemcpyAsync(d_aa13, ns_tab_aa13, 256, hipMemcpyHostToDevice, s));
+		ANOK(hipMemcpyAsync(d_qoff, q_off, (size_t)n_q * 8, hipMemcpyHostToDevice, s));
+		ANOK(hipMemcpyAsync(d_qlen, q_len, (size_t)n_q * 4, hipMemcpyHostToDevice, s));
+		a.seq = d_seq, a.qoff = d_qoff, a.qlen = d_qlen, a.n_q = n_q, a.sbase = d_sbase, a.aa13 = d_aa13;
+		a.kmer = ix->kmer, a.mod_bit = ix->mod_bit;
+		a.mask_k = (uint32_t)(((uint64_t)1 << (4 * ix->kmer)) - 1), a.mask_mod = (1u << ix->mod_bit) - 1;
+		a.ki = ix->d_ki, a.kb = ix->d_kb, a.n_bucket = ix->n_bucket;
+		a.seed = d_seed, a.cnt = d_cnt, a.n_seed = d_nseed, a.max_occ = d_occ, a.occ_cap = max_occ;
+		a.n_anchor = d_nanch, a.aoff = d_aoff;
+		hipLaunchKernelGGL(qsketch_kernel, dim3(n_q), dim3(64), 0, s, a);
+		ANOK(hipGetLastError());
+		ANOK(hipMemcpyAsync(nseed.data(), d_nseed, (size_t)n_q * 4, hipMemcpyDeviceToHost, s));
+		ANOK(hipStreamSynchronize(s));
+		// the bucket sizes of each query's seeds, sorted (segments of the per-query regions)
+		for (int32_t q = 0; q < n_q; ++q) soff[q] = sbase[q], send[q] = sbase[q] + nseed[q];
+		ANOK(hipMemcpyAsync(d_soff, soff.data(), (size_t)n_q * 8, hipMemcpyHostToDevice, s));
+		ANOK(hipMemcpyAsync(d_send, send.data(), (size_t)n_q * 8, hipMemcpyHostToDevice, s));
+		ANOK(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, d_cnt, d_cnt2, (int)l_seed, n_q, d_soff, d_send,
+				0, 64, s));
+		ANOK(hipMalloc(&d_tmp, tb + 8));
+		ANOK(hipcub::DeviceSegmentedRadixSort::SortKeys(d_tmp, tb, d_cnt, d_cnt2, (int)l_seed, n_q, d_soff, d_send,
+				0, 64, s));
+		a.cnt = d_cnt2;
+		hipLaunchKernelGGL(occ_kernel, dim3((n_q + 255) / 256), dim3(256), 0, s, a);
+		hipLaunchKernelGGL(collect_kernel<0>, dim3(n_q), dim3(64), 0, s, a);
+		ANOK(hipGetLastError());
+		ANOK(hipMemcpyAsync(nan.data(), d_nanch, (size_t)n_q * 8, hipMemcpyDeviceToHost, s));
+		ANOK(hipStreamSynchronize(s));
+		for (int32_t q = 0; q < n_q; ++q) aoff[q + 1] = aoff[q] + nan[q];
+		const int64_t n_a = aoff[n_q];
+		if (n_a > INT_MAX) { nsh_set_error("anchors: %lld anchors exceed one sort", (long long)n_a); goto done; }
+		ANOK(hipMemcpyAsync(d_aoff, aoff.data(), (size_t)(n_q + 1) * 8, hipMemcpyHostToDevice, s));
+		ANOK(hipMalloc((void**)&d_anch, (size_t)n_a * 8 + 8));
+		ANOK(hipMalloc((void**)&d_anch2, (size_t)n_a * 8 + 8));
+		a.anchor = d_anch;
+		hipLaunchKernelGGL(collect_kernel<1>, dim3(n_q), dim3(64), 0, s, a);
+		ANOK(hipGetLastError());
+		// each query's anchors, sorted (radix_sort_mp64 in map.c:596)
+		ANOK(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, t2, d_anch, d_anch2, (int)n_a, n_q, d_aoff,
+				d_aoff + 1, 0, 64, s));
+		if (t2 > tb) {
+			ANOK(hipStreamSynchronize(s));
+			(void)hipFree(d_tmp), d_tmp = nullptr;
+			ANOK(hipMalloc(&d_tmp, t2));
+			tb = t2;
+		}
+		ANOK(hipcub::DeviceSegmentedRadixSort::SortKeys(d_tmp, tb, d_anch, d_anch2, (int)n_a, n_q, d_aoff, d_aoff + 1,
+				0, 64, s));
+		if (max_anchors > 0 && n_a > 0)
+			ANOK(hipMemcpyAsync(anchors, d_anch2, (size_t)std::min(n_a, max_anchors) * 8, hipMemcpyDeviceToHost, s));
+		ANOK(hipStreamSynchronize(s));
+		std::copy(aoff.begin(), aoff.end(), a_off);
+		nsh_count_launch(6);
+		total = n_a;
+	}
+done:
+	if (s) (void)hipStreamSynchronize(s);
+	{
+		void *bufs[] = { d_seq, d_aa13, d_qoff, d_qlen, d_nseed, d_occ, d_nanch, d_aoff, d_soff, d_send, d_seed, d_cnt,
+			d_cnt2, d_anch, d_anch2, d_tmp, d_sbase };
+		for (void *b : bufs) if (b) (void)hipFree(b);
+	}
+	if (s) (void)hipStreamDestroy(s);
+	return total;
 }

@@ -407,3 +407,40 @@ def unpack_keys(text):
     import zlib
     d = np.frombuffer(zlib.decompress(base64.b64decode(text)), dtype=np.uint64)
     return np.cumsum(d, dtype=np.uint64)
+
+
+def anchor_case(rng, k, n_query=12, scale=3):
+    """an index_case genome and queries (ASCII) to collect anchors for: translated genome
+    windows on either strand with 0-25% substitutions (some with X and * at N bases or stops),
+    random proteins, empty and very short queries; a repeat-rich genome when k % 4 == 3"""
+    from protlocus_amd import synth
+    c = index_case(rng, k, scale=scale)
+    g = c["genome"]
+    if k % 4 == 3 and len(g) > 3000:   # repeats: one window copied many times raises the bucket sizes
+        w = g[:600].copy()
+        for _ in range(30):
+            s0 = int(rng.integers(0, len(g) - 600))
+            g[s0:s0 + 600] = w
+    qs = []
+    for _ in range(n_query):
+        r = rng.random()
+        if r < 0.6 and len(g) > 1200:
+            L = int(rng.integers(20, 300))
+            s0 = int(rng.integers(0, len(g) - 3 * L))
+            w = g[s0:s0 + 3 * L]
+            if rng.random() < 0.5:
+                w = np.where(w < 4, 3 - w, w)[::-1]
+            aa = [synth.CODE1[int(w[3 * i]) << 4 | int(w[3 * i + 1]) << 2 | int(w[3 * i + 2])]
+                  if max(w[3 * i:3 * i + 3]) < 4 else "X" for i in range(L)]
+            sub = rng.random(L) < rng.uniform(0.0, 0.25)
+            for i in np.nonzero(sub)[0]:
+                aa[i] = synth.AA[int(rng.integers(0, 20))]
+            qs.append("".join(aa).encode())
+        elif r < 0.9:
+            qs.append(synth.rand_protein(rng, int(rng.integers(1, 250))))
+        else:
+            qs.append(b"" if rng.random() < 0.5 else b"MKV")
+    c["genome"] = g
+    c["queries"] = qs
+    c["max_occ"] = [20000, 50, 3][k % 3]
+    return c

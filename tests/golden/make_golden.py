@@ -16,6 +16,7 @@ Outputs (inputs and expected outputs only):
   prescreen_batch_cases.json.gz  one mp_prescreen_build, ps_query_worker per profile of a batch
   gapless_select_cases.json.gz  mp_gapless_calibrate / _adaptive_k / mp_select_gapless (hit.c) on mini-batches
   index_cases.json.gz  mp_idx_boff + mp_sketch_nt4 + build_bidx (index.c, sketch.c): k-mer block indexes
+  anchor_cases.json.gz  mp_sketch_query + mp_cal_max_occ + mp_collect_anchors (map.c): query anchors
   tables.json        ns_tab_nt4 / ns_tab_aa20 / ns_tab_codon / ns_mat_blosum62
 """
 import base64
@@ -33,7 +34,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [os.path.join(ROOT, "tests"), ROOT]
 
 import nsffi                                       # noqa: E402
-from cases import (OUTPUT_VARIANTS, chain_case, gapless_select_case, gapped_case, index_case, index_keys,  # noqa: E402
+from cases import (OUTPUT_VARIANTS, anchor_case, chain_case, gapless_select_case, gapped_case, index_case,  # noqa: E402
+                   index_keys,
                    nt4_codes, orf_case, output_case, pack_keys,
                    parse_cigar, prescreen_batch_case, prescreen_case,
                    planted_case, random_opt_kw, random_ss)
@@ -275,11 +277,29 @@ def index_cases(ref):
     return out
 
 
+def anchor_cases(ref):
+    """the reference's anchors (map.c:583-597, default options) of cases.anchor_case queries
+    against their genome's index"""
+    import zlib
+    rng = np.random.default_rng(20261015)
+    out = []
+    for k in range(8):
+        c = anchor_case(rng, k)
+        a = ref.anchors(c["genome"], c["ctg_off"], c["ctg_len"], c["queries"], c["min_aa_len"], c["kmer"],
+                        c["mod_bit"], c["bbit"], c["max_occ"])
+        out.append({"genome": B64(zlib.compress(c["genome"].tobytes(), 9)), "ctg_off": c["ctg_off"].tolist(),
+                    "ctg_len": c["ctg_len"].tolist(), "kmer": c["kmer"], "mod_bit": c["mod_bit"], "bbit": c["bbit"],
+                    "min_aa_len": c["min_aa_len"], "max_occ": c["max_occ"],
+                    "queries": [q.decode() for q in c["queries"]], "anchors": [pack_keys(x) for x in a]})
+    return out
+
+
 def tables(ref):
     L = ref.lib
     get = lambda name, n, t=C.c_uint8: list((t * n).in_dll(L, name))
     return {"nt4": get("ns_tab_nt4", 256), "aa20": get("ns_tab_aa20", 256), "codon": get("ns_tab_codon", 64),
-            "blosum62": get("ns_mat_blosum62", 484, C.c_int8), "codon13": get("ns_tab_codon13", 64)}
+            "blosum62": get("ns_mat_blosum62", 484, C.c_int8), "codon13": get("ns_tab_codon13", 64),
+            "aa13": get("ns_tab_aa13", 256)}
 
 
 def main():
@@ -289,7 +309,7 @@ def main():
     gens = [("dp_cases", dp_cases), ("pssm_chain_cases", pssm_chain_cases), ("orf_cases", orf_cases),
             ("ps_gapped_cases", ps_gapped_cases), ("prescreen_cases", prescreen_cases),
             ("prescreen_batch_cases", prescreen_batch_cases), ("gapless_select_cases", gapless_select_cases),
-            ("index_cases", index_cases),
+            ("index_cases", index_cases), ("anchor_cases", anchor_cases),
             ("output_cases", output_cases),
             ("hmm_cases", hmm_cases)]
     want = set(sys.argv[1:])

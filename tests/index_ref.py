@@ -64,3 +64,43 @@ def index_build(genome, ctg_off, ctg_len, min_aa_len, kmer, mod_bit, bbit, codon
     ki = np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(np.int64)
     kb = [k & M32 for k in keys]
     return ki, kb, bo, keys
+
+
+def sketch_prot(seq, kmer, mod_bit, aa13):
+    """mp_sketch_prot (sketch.c:18-38): (hash >> mod_bit) << 32 | position of the kept k-mers"""
+    mask_k, mask_mod = (1 << 4 * kmer) - 1, (1 << mod_bit) - 1
+    out, x, l = [], 0, 0
+    for i, ch in enumerate(seq):
+        c = aa13[ch]
+        if c < 14:
+            x = ((x << 4) | c) & mask_k
+            l += 1
+            if l >= kmer:
+                y = hash32_mask(x, mask_k)
+                if y & mask_mod == 0:
+                    out.append((y >> mod_bit) << 32 | i)
+        else:
+            x = l = 0
+    return sorted(out)
+
+
+def anchors(ki, kb, seq, kmer, mod_bit, max_occ, aa13):
+    """the sorted anchors of one query (ASCII bytes), as map.c:583-597 collects them with the
+    default options: the sketch, the boxplot repeat cutoff (mp_cal_max_occ, map.c:304-318) capped at
+    max_occ, every bucket entry of the seeds under it"""
+    sd = sketch_prot(seq, kmer, mod_bit, aa13)
+    n_kb = len(kb)
+
+    def size(b):
+        return (ki[b + 1] if b + 1 < len(ki) else n_kb) - ki[b]
+    occ = max_occ
+    if len(sd) >= 8:
+        cnt = sorted(size(s >> 32) for s in sd)
+        q25, q75 = cnt[int(len(sd) * .25 + .499)], cnt[int(len(sd) * .75 + .499)]
+        occ = min(int(q75 + (q75 - q25) * 1.5 + 10.), max_occ)
+    out = []
+    for s in sd:
+        b, pos = s >> 32, s & M32
+        if size(b) <= occ:
+            out += [int(kb[ki[b] + j]) << 32 | pos for j in range(size(b))]
+    return sorted(out)

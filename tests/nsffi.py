@@ -231,6 +231,8 @@ class Reference(_Lib):
         L.or_ref_index_get.argtypes = [C.c_void_p] * 4
         L.or_ref_index_get.restype = C.c_int64
         L.or_ref_index_free.argtypes = [C.c_void_p]
+        L.or_ref_anchors.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_int64, C.c_void_p]
+        L.or_ref_anchors.restype = C.c_int64
         assert L.ns_make_tables(1) == 0
         self._opt_init = L.ns_opt_init
         self._opt_set_sp = L.ns_opt_set_sp
@@ -355,6 +357,25 @@ class Reference(_Lib):
         finally:
             self.lib.or_ref_index_free(h)
         return ki, kb[:n_kb], bo
+
+    def anchors(self, genome, ctg_off, ctg_len, queries, min_aa_len=30, kmer=6, mod_bit=1, bbit=8, max_occ=20000):
+        """the reference's anchors of each query (ASCII bytes) against the genome's index
+        (oracle/map_ref.c, oracle/idx_ref.c): one sorted numpy uint64 array per query"""
+        g = np.ascontiguousarray(genome, dtype=np.uint8)
+        off = np.ascontiguousarray(ctg_off, dtype=np.int64)
+        ln = np.ascontiguousarray(ctg_len, dtype=np.int64)
+        h = self.lib.or_ref_index_build(len(off), g.ctypes.data, off.ctypes.data, ln.ctypes.data, min_aa_len, kmer,
+                                        mod_bit, bbit)
+        out = []
+        try:
+            for q in queries:
+                n = self.lib.or_ref_anchors(h, bytes(q), len(q), max_occ, 0, None)
+                a = np.zeros(max(1, n), np.uint64)
+                self.lib.or_ref_anchors(h, bytes(q), len(q), max_occ, n, a.ctypes.data)
+                out.append(a[:n])
+        finally:
+            self.lib.or_ref_index_free(h)
+        return out
 
     def orfs(self, ctg_nt4, rev, min_orf):
         """ps_orf_worker on strand rev of a contig: [(start, codons)], residues"""

@@ -1,7 +1,8 @@
-"""The k-mer block index restatement (tests/index_ref.py) pinned three ways: the golden records of
-the compiled reference (index.c:14-29, 52-97; sketch.c:7-16, 108-169), fresh genomes against the
-live reference when it is built, and closed forms: the hash is a bijection inside its mask, a
-hand-built ORF yields exactly its k-mers, and every bucket's entries rise."""
+"""The k-mer block index and anchor restatements (tests/index_ref.py) pinned three ways: the golden
+records of the compiled reference (index.c:14-29, 52-97; sketch.c:7-38, 108-169; map.c:304-395,
+583-597), fresh genomes and queries against the live reference when it is built, and closed forms:
+the hash is a bijection inside its mask, a hand-built ORF yields exactly its k-mers, every
+bucket's entries rise, and a query made of an indexed k-mer finds that k-mer's blocks."""
 import base64
 import gzip
 import json
@@ -13,7 +14,7 @@ import pytest
 
 import index_ref
 import nsffi
-from cases import index_case, index_keys, unpack_keys
+from cases import anchor_case, index_case, index_keys, unpack_keys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -85,3 +86,56 @@ def test_closed_forms():
     for c in _golden()[:4]:
         keys = unpack_keys(c["keys"])
         assert np.all(keys[1:] > keys[:-1])
+
+
+def _anchor_golden():
+    with gzip.open(os.path.join(HERE, "golden", "anchor_cases.json.gz"), "rt") as f:
+        return json.load(f)
+
+
+def test_anchor_restatement_matches_golden():
+    with open(os.path.join(HERE, "golden", "tables.json")) as f:
+        t = json.load(f)
+    for k, c in enumerate(_anchor_golden()):
+        g = np.frombuffer(zlib.decompress(base64.b64decode(c["genome"])), dtype=np.uint8)
+        ki, kb, _, _ = index_ref.index_build(g, c["ctg_off"], c["ctg_len"], c["min_aa_len"], c["kmer"], c["mod_bit"],
+                                             c["bbit"], t["codon"], t["codon13"])
+        for j, (q, want) in enumerate(zip(c["queries"], c["anchors"])):
+            got = index_ref.anchors(ki, kb, q.encode(), c["kmer"], c["mod_bit"], c["max_occ"], t["aa13"])
+            assert np.array_equal(np.array(got, dtype=np.uint64), unpack_keys(want)), (k, j)
+
+
+@pytest.mark.skipif(not nsffi.have_ref(), reason="the compiled reference (make -C oracle ref) is absent")
+def test_anchor_restatement_matches_reference_live():
+    with open(os.path.join(HERE, "golden", "tables.json")) as f:
+        t = json.load(f)
+    ref = nsffi.Reference()
+    rng = np.random.default_rng(41)
+    for k in range(4):
+        c = anchor_case(rng, k)
+        want = ref.anchors(c["genome"], c["ctg_off"], c["ctg_len"], c["queries"], c["min_aa_len"], c["kmer"],
+                           c["mod_bit"], c["bbit"], c["max_occ"])
+        ki, kb, _, _ = index_ref.index_build(c["genome"], c["ctg_off"].tolist(), c["ctg_len"].tolist(),
+                                             c["min_aa_len"], c["kmer"], c["mod_bit"], c["bbit"], t["codon"],
+                                             t["codon13"])
+        for q, w in zip(c["queries"], want):
+            assert index_ref.anchors(ki, kb, q, c["kmer"], c["mod_bit"], c["max_occ"], t["aa13"]) == [int(x) for x in w]
+
+
+def test_anchor_closed_forms():
+    """a query equal to an ORF's translation finds, at its last position, the blocks of its
+    final k-mer; residues outside the 13-letter alphabet break the k-mers around them"""
+    with open(os.path.join(HERE, "golden", "tables.json")) as f:
+        t = json.load(f)
+    from protlocus_amd import synth
+    rng = np.random.default_rng(8)
+    g = np.array([2, 1, 3] * 60, dtype=np.uint8)   # GCT x 60: an alanine run in frame 0
+    ki, kb, bo, keys = index_ref.index_build(g, [0], [len(g)], 30, 3, 0, 4, t["codon"], t["codon13"])
+    q = b"A" * 10
+    a = index_ref.anchors(ki, kb, q, 3, 0, 20000, t["aa13"])
+    y = index_ref.hash32_mask((t["codon13"][2 << 4 | 1 << 2 | 3] * 0x111) & 0xfff, 0xfff)
+    blocks = [int(kb[j]) for j in range(ki[y], ki[y + 1])]
+    assert blocks and a == sorted(b << 32 | p for p in range(2, 10) for b in blocks)
+    assert index_ref.sketch_prot(b"AA*AA", 3, 0, t["aa13"]) == []
+    assert index_ref.sketch_prot(b"", 3, 0, t["aa13"]) == []
+    assert len(index_ref.sketch_prot(synth.rand_protein(rng, 50), 3, 0, t["aa13"])) == 48
