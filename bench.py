@@ -17,7 +17,7 @@ Beside it, in the same JSON line:
   * hmm: profile-HMM Viterbi scan, 64 profiles (M 100-800) x a 10-Mb genome in 6 frames;
   * profile_search: the profile prescreen search of the same profiles over the same genome;
   * gapless_prefilter: gapless scores and top-K selection of one --gpu-prefilter mini-batch;
-  * index: the k-mer block index of a 100-Mb genome.
+  * index: the k-mer block index of a 100-Mb genome, and the anchors of 20,000 proteins against it.
 Output: one JSON line.
 """
 import argparse
@@ -350,6 +350,58 @@ def bench_index(cpu=True, genome_len=100_000_000, n_ctg=24, log=None):
                                              np.array_equal(bo, wbo))
         if log:
             log("[bench] index: reference %.1f ms, identical %s" % (cpu_s * 1e3, out["identical_to_reference"]))
+    # the anchors of 20,000 proteins (50-1000 aa): half translated genome windows with 20% substitutions
+    from protlocus_amd import synth
+    code1 = np.frombuffer(synth.CODE1.encode(), dtype=np.uint8)
+    qs = []
+    for i in range(20000):
+        L = int(np.exp(rng.uniform(np.log(50), np.log(1000))))
+        if i % 2 == 0:
+            s0 = int(rng.integers(0, genome_len - 3 * L))
+            w = g[s0:s0 + 3 * L].astype(np.int64)
+            bad = (w.reshape(L, 3) > 3).any(axis=1)
+            idx = np.minimum(w[0::3], 3) << 4 | np.minimum(w[1::3], 3) << 2 | np.minimum(w[2::3], 3)
+            p = code1[idx].copy()
+            p[bad] = ord("X")
+            sub = rng.random(L) < 0.2
+            p[sub] = np.frombuffer(synth.rand_protein(rng, int(sub.sum())), dtype=np.uint8)
+            qs.append(p.tobytes())
+        else:
+            qs.append(synth.rand_protein(rng, L))
+    ix = capi.Index(g, off, ln)
+    try:
+        ix.anchors(qs[:100])
+        t0 = time.perf_counter()
+        got = ix.anchors(qs)
+        an_s = time.perf_counter() - t0
+    finally:
+        ix.close()
+    n_anchor = int(sum(len(a) for a in got))
+    out["anchors"] = {"queries": len(qs), "residues": int(sum(len(q) for q in qs)), "anchors": n_anchor,
+                      "gpu_ms": round(an_s * 1e3, 1)}
+    if log:
+        log("[bench] anchors: %d queries, %d anchors, GPU %.1f ms" % (len(qs), n_anchor, an_s * 1e3))
+    if cpu and nsffi.have_ref():
+        ref = nsffi.Reference()
+        L = ref.lib
+        h = L.or_ref_index_build(len(off), g.ctypes.data, off.ctypes.data, ln.ctypes.data, 30, 6, 1, 8)
+        try:
+            same, done, t0 = 0, 0, time.perf_counter()
+            buf = np.zeros(1 << 22, np.uint64)
+            for k, q in enumerate(qs):
+                n = L.or_ref_anchors(h, q, len(q), 20000, len(buf), buf.ctypes.data)
+                same += n == len(got[k]) and np.array_equal(buf[:n], got[k])
+                done += 1
+                if time.perf_counter() - t0 > 10.0:
+                    break
+            cpu_s = time.perf_counter() - t0
+        finally:
+            L.or_ref_index_free(h)
+        out["anchors"]["cpu_baseline"] = {"value": round(cpu_s * 1e3 / done * len(qs), 1),
+                                          "unit": "ms per 20,000 queries (reference code, one host thread, scaled)",
+                                          "cores": 1, "kind": "reference",
+                                          "sample": "the first %d queries, %.1f s" % (done, cpu_s)}
+        out["anchors"]["identical_to_reference"] = "%d of %d sampled queries" % (same, done)
     return out
 
 

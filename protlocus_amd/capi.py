@@ -458,14 +458,16 @@ class Index:
         ln = np.array([len(q) for q in queries], dtype=np.int32)
         off = np.concatenate([[0], np.cumsum(ln)[:-1]]).astype(np.int64) if len(ln) else np.zeros(0, np.int64)
         aoff = np.zeros(len(ln) + 1, np.int64)
-        n = lib().nsh_index_anchors(self.p, len(ln), seq, off.ctypes.data, ln.ctypes.data, max_occ, aoff.ctypes.data,
-                                    None, 0)
-        if n < 0:
-            raise RuntimeError("nsh_index_anchors: " + error())
-        a = np.zeros(max(1, n), np.uint64)
-        n = lib().nsh_index_anchors(self.p, len(ln), seq, off.ctypes.data, ln.ctypes.data, max_occ, aoff.ctypes.data,
-                                    a.ctypes.data, n)
-        return [a[aoff[q]:aoff[q + 1]] for q in range(len(ln))]
+        cap = 24 * len(seq) + 1024   # a first guess; a second call when the anchors do not fit
+        while True:
+            a = np.empty(cap, np.uint64)
+            n = lib().nsh_index_anchors(self.p, len(ln), seq, off.ctypes.data, ln.ctypes.data, max_occ,
+                                        aoff.ctypes.data, a.ctypes.data, cap)
+            if n < 0:
+                raise RuntimeError("nsh_index_anchors: " + error())
+            if n <= cap:
+                return [a[aoff[q]:aoff[q + 1]] for q in range(len(ln))]
+            cap = n
 
     def close(self):
         if self.p:
