@@ -349,6 +349,15 @@ void nsh_index_destroy(nsh_index_t *ix);
 int64_t nsh_index_anchors(const nsh_index_t *ix, int32_t n_q, const char *seq, const int64_t *q_off,
 		const int32_t *q_len, int32_t max_occ, int64_t *a_off, uint64_t *anchors, int64_t max_anchors);
 
+/* Anchor chaining of n_q queries (mp_chain, chain.c:7-219, each query's call as mp_map_coarse
+ * makes it, map.c:611-632): query q's sorted anchors are anchors[a_off[q] .. a_off[q+1]); the
+ * parameters are mp_chain's. n_u[q] receives its chain count; its chains (score << 32 | anchors)
+ * go to u + a_off[q] and their anchors, chain after chain in mp_chain's order, to
+ * a_out + a_off[q] (u and a_out have a_off[n_q] slots). Returns the total chain count, or -1. */
+int64_t nsh_chain_batch(int32_t n_q, const int64_t *a_off, const uint64_t *anchors, int32_t max_dist_x,
+		int32_t max_dist_y, int32_t bw, int32_t max_skip, int32_t max_iter, int32_t min_cnt, int32_t min_sc,
+		float chn_coef_log, int32_t is_spliced, int32_t kmer, int32_t bbit, int32_t *n_u, uint64_t *u, uint64_t *a_out);
+
 /* Learned chain-candidate prescreen: MLP 64 -> 256 -> 256 -> 1 (ReLU, logistic output), BF16
  * MFMA kernels with FP32 accumulation, FP32 Adam masters. Parameter vector (FP32, in order):
  * W1[64][256], b1[256], W2[256][256], b2[256], w3[256], b3. No reference counterpart. */

@@ -2,7 +2,8 @@
  * reference tree is present: the reference's k-mer block index (mp_idx_build without the file
  * read: mp_idx_boff, mp_sketch_nt4 per contig strand as build_worker calls it, build_bidx;
  * index.c:14-29, 52-97, sketch.c:108-169) on a genome given as arrays, by compiling its
- * index.c where it lies (REF is on the include path). */
+ * index.c where it lies (REF is on the include path); and its chaining (mp_chain, chain.c,
+ * compiled with the reference sources) on a copy of a query's anchors. */
 #include "index.c"
 
 typedef struct { mp_ntdb_t nt; mp_idx_t mi; } or_idx_t;
@@ -61,4 +62,27 @@ void or_ref_index_free(void *handle)
 const mp_idx_t *or_ref_index_mi(void *handle)
 {
 	return &((or_idx_t*)handle)->mi;
+}
+
+/* mp_chain (chain.c:160-219) on a copy of n anchors: returns the number of chains and writes
+ * their u[] (score << 32 | anchors, up to max_u) and their anchors (up to max_a), in the order
+ * mp_chain returns them */
+int32_t or_ref_chain(int32_t max_dist_x, int32_t max_dist_y, int32_t bw, int32_t max_skip, int32_t max_iter,
+		int32_t min_cnt, int32_t min_sc, float chn_coef_log, int32_t is_spliced, int32_t kmer, int32_t bbit, int64_t n,
+		const uint64_t *a, int32_t max_u, uint64_t *u_out, int64_t max_a, uint64_t *a_out)
+{
+	uint64_t *b = (uint64_t*)malloc((size_t)(n > 0? n : 1) * 8), *u = 0, *r;
+	int32_t n_u = 0, i;
+	int64_t k = 0;
+	if (n > 0) memcpy(b, a, (size_t)n * 8);
+	r = mp_chain(max_dist_x, max_dist_y, bw, max_skip, max_iter, min_cnt, min_sc, chn_coef_log, is_spliced, kmer, bbit,
+			n, n > 0? b : 0, &n_u, &u, 0);
+	if (n == 0) free(b);
+	for (i = 0; i < n_u; ++i) {
+		if (i < max_u) u_out[i] = u[i];
+		k += (uint32_t)u[i];
+	}
+	for (i = 0; i < k && i < max_a; ++i) a_out[i] = r[i];
+	free(u), free(r);
+	return n_u;
 }

@@ -139,6 +139,9 @@ def lib():
         L.nsh_index_anchors.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
                                         C.c_void_p, C.c_void_p, C.c_int64]
         L.nsh_index_anchors.restype = C.c_int64
+        L.nsh_chain_batch.argtypes = ([C.c_int32, C.c_void_p, C.c_void_p] + [C.c_int32] * 7 +
+                                      [C.c_float, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p])
+        L.nsh_chain_batch.restype = C.c_int64
         L.nsh_gapless_select_batch.argtypes = [C.c_int32] + [C.c_void_p] * 6 + [C.c_int32] * 3 + [C.c_void_p] * 3
         L.nsh_gapless_prefilter_batch.argtypes = ([C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32,
                                                    C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
@@ -473,6 +476,31 @@ class Index:
         if self.p:
             lib().nsh_index_destroy(self.p)
             self.p = None
+
+
+def chain_batch(anchor_lists, max_dist_x, max_dist_y, bw, max_skip, max_iter, min_cnt, min_sc, coef, is_spliced,
+                kmer, bbit):
+    """mp_chain of every query's sorted anchors on the device (nsh_chain_batch): per query the
+    chains (score << 32 | count) and their anchors, as numpy uint64 arrays"""
+    lens = np.array([len(a) for a in anchor_lists], dtype=np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    a = (np.concatenate([np.asarray(x, dtype=np.uint64) for x in anchor_lists]) if len(anchor_lists)
+         else np.zeros(0, np.uint64))
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    n_q = len(anchor_lists)
+    n_u = np.zeros(max(1, n_q), np.int32)
+    u, ao = np.zeros(max(1, len(a)), np.uint64), np.zeros(max(1, len(a)), np.uint64)
+    r = lib().nsh_chain_batch(n_q, off.ctypes.data, a.ctypes.data, max_dist_x, max_dist_y, bw, max_skip, max_iter,
+                              min_cnt, min_sc, coef, is_spliced, kmer, bbit, n_u.ctypes.data, u.ctypes.data,
+                              ao.ctypes.data)
+    if r < 0:
+        raise RuntimeError("nsh_chain_batch: " + error())
+    out = []
+    for q in range(n_q):
+        uq = u[off[q]:off[q] + n_u[q]]
+        m = int((uq & np.uint64(0xffffffff)).sum()) if len(uq) else 0
+        out.append((uq, ao[off[q]:off[q] + m]))
+    return out
 
 
 class GaplessCalib(C.Structure):

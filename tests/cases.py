@@ -444,3 +444,49 @@ def anchor_case(rng, k, n_query=12, scale=3):
     c["queries"] = qs
     c["max_occ"] = [20000, 50, 3][k % 3]
     return c
+
+
+
+CHAIN_VARIANTS = [   # mp_chain arguments after (kmer, bbit): the pre-chain and the main chain of
+    # mp_map_coarse (map.c:611-627 with the default options), and others that reach the other branches
+    lambda k, b: (1 << b, 1 << b, 1 << b, 25, 1000000, 2, 0, 0.75, 1, k, b),
+    lambda k, b: (200000, 1000, 200000, 25, 1000000, 3, 0, 0.75, 1, k, b),
+    lambda k, b: (5000, 300, 500, 3, 50, 1, -5, 0.5, 0, k, b),
+    lambda k, b: (20000, 2000, 1000, 0, 1000000, 2, 10, 1.0, 1, k, b),
+    lambda k, b: (3000, 3000, 40, 1, 7, 1, 0, 2.0, 0, k, b),
+]
+
+
+def anchor_chain_case(rng, k, n_max=3000):
+    """sorted anchors (target block << 32 | query position) for mp_chain: co-linear runs along a
+    diagonal with jitter, some spliced (a jump in the target), repeats of a run elsewhere, and
+    noise; exact duplicates sometimes; (kmer, bbit) cycle over 3..6 and 8, 6, 0, with a
+    CHAIN_VARIANTS parameter set by k"""
+    kmer = 3 + k % 4
+    bbit = (8, 6, 0)[k % 3]
+    n = int(rng.integers(0, n_max)) if k % 7 else int(rng.integers(0, 4))
+    xs = []
+    while len(xs) < n:
+        r = rng.random()
+        if r < 0.55:
+            t0 = int(rng.integers(0, 400000 >> (bbit // 2)))
+            q0 = int(rng.integers(0, 2000))
+            m = int(rng.integers(2, 60))
+            copies = 1 if rng.random() < 0.7 else int(rng.integers(2, 6))
+            for c in range(copies):
+                t, q = t0 + (0 if c == 0 else int(rng.integers(-50000, 50000))), q0
+                for _ in range(m):
+                    q += int(rng.integers(1, 8))
+                    step = int(rng.integers(1, 8)) * 3
+                    t += max(0, step // (1 << bbit) + int(rng.integers(-1, 2))) if bbit else step + int(
+                        rng.integers(-4, 5))
+                    if rng.random() < 0.03:
+                        t += int(rng.integers(10, 4000 >> (bbit // 2)))
+                    if t >= 0:
+                        xs.append(t << 32 | q)
+        else:
+            xs.append(int(rng.integers(0, 400000 >> (bbit // 2))) << 32 | int(rng.integers(0, 2500)))
+        if rng.random() < 0.02 and xs:
+            xs.append(xs[int(rng.integers(0, len(xs)))])
+    a = np.sort(np.array(xs[:n], dtype=np.uint64))
+    return {"anchors": a, "params": CHAIN_VARIANTS[k % len(CHAIN_VARIANTS)](kmer, bbit)}

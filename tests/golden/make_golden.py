@@ -17,6 +17,7 @@ Outputs (inputs and expected outputs only):
   gapless_select_cases.json.gz  mp_gapless_calibrate / _adaptive_k / mp_select_gapless (hit.c) on mini-batches
   index_cases.json.gz  mp_idx_boff + mp_sketch_nt4 + build_bidx (index.c, sketch.c): k-mer block indexes
   anchor_cases.json.gz  mp_sketch_query + mp_cal_max_occ + mp_collect_anchors (map.c): query anchors
+  anchor_chain_cases.json.gz  mp_chain (chain.c) on sorted anchors: chains and their anchors
   tables.json        ns_tab_nt4 / ns_tab_aa20 / ns_tab_codon / ns_mat_blosum62
 """
 import base64
@@ -34,7 +35,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [os.path.join(ROOT, "tests"), ROOT]
 
 import nsffi                                       # noqa: E402
-from cases import (OUTPUT_VARIANTS, anchor_case, chain_case, gapless_select_case, gapped_case, index_case,  # noqa: E402
+from cases import (CHAIN_VARIANTS, OUTPUT_VARIANTS, anchor_case, anchor_chain_case, chain_case, gapless_select_case, gapped_case, index_case,  # noqa: E402
                    index_keys,
                    nt4_codes, orf_case, output_case, pack_keys,
                    parse_cigar, prescreen_batch_case, prescreen_case,
@@ -294,6 +295,29 @@ def anchor_cases(ref):
     return out
 
 
+def anchor_chain_cases(ref):
+    """the reference's mp_chain (chain.c:7-219) on cases.anchor_chain_case anchors, and on the
+    anchors of cases.anchor_case queries with the pre-chain and main-chain parameters"""
+    rng = np.random.default_rng(20261203)
+    out = []
+    for k in range(24):
+        c = anchor_chain_case(rng, k, n_max=1500)
+        u, a = ref.chain(c["anchors"], *c["params"])
+        out.append({"params": list(c["params"]), "anchors": pack_keys(c["anchors"]), "u": pack_keys(u),
+                    "a": pack_keys(a)})
+    for k in range(4):
+        c = anchor_case(rng, k)
+        for q in ref.anchors(c["genome"], c["ctg_off"], c["ctg_len"], c["queries"], c["min_aa_len"], c["kmer"],
+                             c["mod_bit"], c["bbit"], c["max_occ"]):
+            if len(q) < 2:
+                continue
+            for v in CHAIN_VARIANTS[:2]:
+                p = v(c["kmer"], c["bbit"])
+                u, a = ref.chain(q, *p)
+                out.append({"params": list(p), "anchors": pack_keys(q), "u": pack_keys(u), "a": pack_keys(a)})
+    return out
+
+
 def tables(ref):
     L = ref.lib
     get = lambda name, n, t=C.c_uint8: list((t * n).in_dll(L, name))
@@ -310,6 +334,7 @@ def main():
             ("ps_gapped_cases", ps_gapped_cases), ("prescreen_cases", prescreen_cases),
             ("prescreen_batch_cases", prescreen_batch_cases), ("gapless_select_cases", gapless_select_cases),
             ("index_cases", index_cases), ("anchor_cases", anchor_cases),
+            ("anchor_chain_cases", anchor_chain_cases),
             ("output_cases", output_cases),
             ("hmm_cases", hmm_cases)]
     want = set(sys.argv[1:])

@@ -233,6 +233,9 @@ class Reference(_Lib):
         L.or_ref_index_free.argtypes = [C.c_void_p]
         L.or_ref_anchors.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_int64, C.c_void_p]
         L.or_ref_anchors.restype = C.c_int64
+        L.or_ref_chain.argtypes = [C.c_int32] * 7 + [C.c_float, C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_void_p,
+                                                     C.c_int32, C.c_void_p, C.c_int64, C.c_void_p]
+        L.or_ref_chain.restype = C.c_int32
         assert L.ns_make_tables(1) == 0
         self._opt_init = L.ns_opt_init
         self._opt_set_sp = L.ns_opt_set_sp
@@ -376,6 +379,19 @@ class Reference(_Lib):
         finally:
             self.lib.or_ref_index_free(h)
         return out
+
+    def chain(self, anchors, max_dist_x, max_dist_y, bw, max_skip, max_iter, min_cnt, min_sc, coef, is_spliced, kmer,
+              bbit):
+        """the reference's mp_chain on one query's sorted anchors: (u list, anchors list)"""
+        a = np.ascontiguousarray(anchors, dtype=np.uint64)
+        n = len(a)
+        u = np.zeros(max(1, n), np.uint64)
+        out = np.zeros(max(1, n), np.uint64)
+        n_u = self.lib.or_ref_chain(max_dist_x, max_dist_y, bw, max_skip, max_iter, min_cnt, min_sc, coef, is_spliced,
+                                    kmer, bbit, n, a.ctypes.data if n else None, len(u), u.ctypes.data, len(out),
+                                    out.ctypes.data)
+        m = int(sum(int(x) & 0xffffffff for x in u[:n_u]))
+        return [int(x) for x in u[:n_u]], [int(x) for x in out[:m]]
 
     def orfs(self, ctg_nt4, rev, min_orf):
         """ps_orf_worker on strand rev of a contig: [(start, codons)], residues"""

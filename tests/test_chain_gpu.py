@@ -1,0 +1,88 @@
+"""GPU parity of the batched anchor chaining nsh_chain_batch against the reference's mp_chain
+(chain.c:7-219, as map.c:611-633 calls it): every query's chains (score << 32 | count) and their
+anchors must be identical, in the same order. The checks use the golden records of the compiled
+reference, grouped into batches by parameter set, and random batches (empty queries among them)
+against the pinned restatement tests/chain_ref.py."""
+import gzip
+import json
+import os
+from collections import defaultdict
+
+import numpy as np
+import pytest
+
+import chain_ref
+from cases import anchor_chain_case, unpack_keys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+pytestmark = pytest.mark.gpu
+
+
+def test_chain_golden(product):
+    from protlocus_amd import capi
+    with gzip.open(os.path.join(HERE, "golden", "anchor_chain_cases.json.gz"), "rt") as f:
+        cases = json.load(f)
+    groups = defaultdict(list)
+    for k, c in enumerate(cases):
+        groups[tuple(c["params"])].append(k)
+    for params, ks in groups.items():
+        print("params", params, "queries", len(ks), flush=True)
+        got = capi.chain_batch([unpack_keys(cases[k]["anchors"]) for k in ks], *params)
+        for k, (u, a) in zip(ks, got):
+            assert u.tolist() == unpack_keys(cases[k]["u"]).tolist(), k
+            assert a.tolist() == unpack_keys(cases[k]["a"]).tolist(), k
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4, 5])
+def test_chain_random(product, seed):
+    from protlocus_amd import capi
+    rng = np.random.default_rng(seed)
+    k0 = int(rng.integers(0, 1000))
+    base = anchor_chain_case(rng, k0, n_max=10)
+    params = base["params"]
+    qs = []
+    for j in range(int(rng.integers(20, 70))):
+        c = anchor_chain_case(rng, k0 + 60 * j, n_max=int(rng.choice([5, 100, 900])))   # same kmer, bbit, params
+        qs.append(c["anchors"] if rng.random() > 0.1 else np.zeros(0, np.uint64))
+    print("seed", seed, "params", params, "queries", len(qs), "anchors", sum(len(q) for q in qs), flush=True)
+    got = capi.chain_batch(qs, *params)
+    for j, (q, (u, a)) in enumerate(zip(qs, got)):
+        want = chain_ref.chain([int(x) for x in q], *params)
+        assert (u.tolist(), a.tolist()) == want, (seed, j)
+
+
+def test_chain_long_diagonals(product):
+    """one query of 30,000 anchors in long co-linear runs and a perfect diagonal"""
+    from protlocus_amd import capi
+    n = 20000
+    diag = np.array([(3 * i) << 32 | i for i in range(n)], np.uint64)
+    (u, a), = capi.chain_batch([diag], 5000, 5000, 500, 25, 5000, 2, 0, 0.75, 1, 1, 0)
+    assert u.tolist() == [n << 32 | n] and a.tolist() == diag.tolist()
+    rng = np.random.default_rng(11)
+    c = anchor_chain_case(rng, 1, n_max=1)
+    xs = []
+    for r in range(300):
+        t, q = int(rng.integers(0, 5000)), int(rng.integers(0, 3000))
+        for i in range(100):
+            q += int(rng.integers(1, 4))
+            t += int(rng.integers(0, 3))
+            xs.append(t << 32 | q)
+    big = np.sort(np.array(xs, np.uint64))
+    for params in ((256, 256, 256, 25, 1000000, 2, 0, 0.75, 1, 4, 6), c["params"]):
+        print("params", params, flush=True)
+        (u, a), = capi.chain_batch([big], *params)
+        assert (u.tolist(), a.tolist()) == chain_ref.chain([int(x) for x in big], *params)
+
+
+def test_chain_rejects_bad_arguments(product):
+    from protlocus_amd.capi import lib
+    off = np.array([0, 2, 1], np.int64)
+    a = np.zeros(2, np.uint64)
+    n_u = np.zeros(2, np.int32)
+    out = np.zeros(2, np.uint64)
+    assert lib().nsh_chain_batch(2, off.ctypes.data, a.ctypes.data, 1, 1, 1, 1, 1, 1, 0, 0.75, 1, 3, 8,
+                                 n_u.ctypes.data, out.ctypes.data, out.ctypes.data) < 0
+    off = np.array([0, 1, 2], np.int64)
+    assert lib().nsh_chain_batch(2, off.ctypes.data, a.ctypes.data, 1, 1, 1, 1, 1, 1, 0, 0.75, 1, 0, 8,
+                                 n_u.ctypes.data, out.ctypes.data, out.ctypes.data) < 0
+    assert lib().nsh_chain_batch(0, None, None, 1, 1, 1, 1, 1, 1, 0, 0.75, 1, 3, 8, None, None, None) == 0
