@@ -402,6 +402,72 @@ def bench_index(cpu=True, genome_len=100_000_000, n_ctg=24, log=None):
                                           "cores": 1, "kind": "reference",
                                           "sample": "the first %d queries, %.1f s" % (done, cpu_s)}
         out["anchors"]["identical_to_reference"] = "%d of %d sampled queries" % (same, done)
+    out["chain"] = bench_chain(got, cpu=cpu, log=log)
+    return out
+
+
+def bench_chain(anchors, cpu=True, log=None):
+    """mp_map_coarse's two chaining passes (map.c:611-633, default options, spliced) over every
+    query's anchors: the pre-chain (w = 256, min_cnt 2), each query's kept anchors sorted, then the
+    main chain (max_intron 200000, max_gap 1000, bw 200000, max_skip 25, max_iter 1e6, min_cnt 3,
+    coef 0.75). GPU times include the anchors' upload and the chains' download; the reference's
+    mp_chain (oracle/_ref) runs the same two passes on one host thread over a sample of queries"""
+    import numpy as np
+    import nsffi
+    from protlocus_amd import capi
+    L = capi.lib()
+    pre = (256, 256, 256, 25, 1000000, 2, 0, 0.75, 1, 6, 8)
+    main = (200000, 1000, 200000, 25, 1000000, 3, 0, 0.75, 1, 6, 8)
+
+    def run(lists, params):
+        lens = np.array([len(a) for a in lists], np.int64)
+        off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        a = np.ascontiguousarray(np.concatenate(lists), dtype=np.uint64)
+        n_u = np.zeros(len(lists), np.int32)
+        u, ao = np.empty(max(1, len(a)), np.uint64), np.empty(max(1, len(a)), np.uint64)
+        t0 = time.perf_counter()
+        r = L.nsh_chain_batch(len(lists), off.ctypes.data, a.ctypes.data, *params, n_u.ctypes.data, u.ctypes.data,
+                              ao.ctypes.data)
+        dt = time.perf_counter() - t0
+        assert r >= 0, capi.error()
+        return off, n_u, u, ao, dt
+
+    run(anchors[:200], pre)   # warm-up
+    off, n_u, u, ao, pre_s = run(anchors, pre)
+    kept = []
+    for q in range(len(anchors)):
+        uq = u[off[q]:off[q] + n_u[q]]
+        m = int((uq & np.uint64(0xffffffff)).sum())
+        kept.append(np.sort(ao[off[q]:off[q] + m]))
+    off2, n_u2, u2, ao2, main_s = run(kept, main)
+    out = {"queries": len(anchors), "anchors": int(sum(len(a) for a in anchors)),
+           "prechain_kept_anchors": int(sum(len(a) for a in kept)), "chains": int(n_u2.sum()),
+           "gpu_prechain_ms": round(pre_s * 1e3, 1), "gpu_chain_ms": round(main_s * 1e3, 1)}
+    if log:
+        log("[bench] chain: %d -> %d anchors, %d chains; GPU pre-chain %.1f ms, chain %.1f ms" % (
+            out["anchors"], out["prechain_kept_anchors"], out["chains"], pre_s * 1e3, main_s * 1e3))
+    if cpu and nsffi.have_ref():
+        ref = nsffi.Reference()
+        same, done, t0 = 0, 0, time.perf_counter()
+        for q in range(len(anchors)):
+            pu, pa = ref.chain(anchors[q], *pre)
+            mu, ma = ref.chain(np.sort(np.array(pa, np.uint64)), *main)
+            gu = u2[off2[q]:off2[q] + n_u2[q]]
+            ga = ao2[off2[q]:off2[q] + int((gu & np.uint64(0xffffffff)).sum())]
+            same += gu.tolist() == mu and ga.tolist() == ma
+            done += 1
+            if time.perf_counter() - t0 > 10.0:
+                break
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(cpu_s * 1e3 / done * len(anchors), 1),
+                               "unit": "ms per %d queries, both passes (reference code, one host thread, scaled)" % len(
+                                   anchors), "cores": 1, "kind": "reference",
+                               "sample": "the first %d queries, %.1f s (ctypes list conversion included)" % (done,
+                                                                                                           cpu_s)}
+        out["identical_to_reference"] = "%d of %d sampled queries" % (same, done)
+        if log:
+            log("[bench] chain: reference %s, identical %s" % (out["cpu_baseline"]["value"],
+                                                               out["identical_to_reference"]))
     return out
 
 

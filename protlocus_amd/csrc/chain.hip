@@ -10,12 +10,19 @@
 // sorts that do not keep the order of equal keys, and equal scores and equal blocks are common;
 // the candidates' order, and so the chains, depend on the exact permutation.
 //
-// Every query's chaining is that sequential procedure, so a lane runs one query from start to
-// end (queries ordered by anchor count, so that a wave's lanes finish together), and the batch
-// fills the GPU with queries: the same DP, the same backtrack and the same
-// radix sort (its recursion unrolled into frames in a per-lane scratch area), in the reference's
-// int32 and float32 arithmetic. Per anchor each lane keeps f, p, t and the backtrack list in
-// global scratch laid out like the input.
+// Every query's chaining keeps that procedure's order of decisions; a wavefront runs one query
+// (the batch fills the GPU with queries, largest first) and spreads over its 64 lanes what does
+// not depend on that order:
+//   * the DP's look-back: lane L holds anchor i-1-L (a, f, p shifted one lane per anchor), so the
+//     nearest 64 predecessors are scored in parallel; a prefix maximum gives each lane the best
+//     score before it, and only the few lanes that change the skip counter are walked in order.
+//     The "claimed by a later anchor" marks (t[j] == i) only live within one anchor's scan, so
+//     they are a 64-bit lane mask, or per-anchor words for windows longer than 64;
+//   * the radix sort: key ranges and digit counts are counted by all lanes, a level where every
+//     key has the same digit (the permutation is then the identity) is skipped, and buckets of at
+//     most 64 keys are placed by their stable rank (what the insertion sort yields); the
+//     cycle-leader pass that fixes the order of equal keys runs on one lane;
+//   * the greedy backtrack runs on one lane; building and ordering the chains is lane-parallel.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <climits>
@@ -33,22 +40,70 @@ struct ChainParams {
 	int32_t is_spliced, kmer, bbit;
 };
 
-struct RsFrame {   // one level of the radix sort's recursion
-	int32_t s_child, recurse, k;
-	int32_t b[kRsSize], e[kRsSize];
-};
-
 struct ChArgs {
 	const uint64_t *a; const int64_t *off; int32_t n_q;
-	const int32_t *order;              // queries by decreasing anchor count (a wave's lanes alike)
+	const int32_t *order;              // queries by decreasing anchor count
 	ChainParams cp;
 	int32_t *f, *p, *t, *vv;           // per anchor
 	uint64_t *zx, *zy;                 // per anchor: sort pairs
 	uint64_t *b;                       // per anchor: the chains' anchors before the final order
-	RsFrame *frames;                   // per query: 8 frames
 	uint64_t *u_out, *a_out;           // per anchor slot: chains (score << 32 | count), their anchors
 	int32_t *n_u;                      // per query
 };
+
+struct ChLds {                         // one wavefront's shared state
+	int32_t b[8][kRsSize], e[8][kRsSize];   // radix sort frames: bucket cursors and ends
+	int32_t fk[8], fs[8], frec[8];          // per frame: next bucket, the children's shift, recursion
+	uint64_t cmask, orv;
+	uint64_t sk[64];
+	int32_t nu, nv;
+};
+
+#define DEV __device__ __forceinline__
+
+template <int CTRL, int ROWS = 0xf>
+DEV int dpp(int old, int src) { return __builtin_amdgcn_update_dpp(old, src, CTRL, ROWS, 0xf, false); }
+
+// inclusive scans over lanes 0..63: shifts 1-8 inside each row of 16, then rows 15 and 31 onwards
+DEV int wave_sum_incl(int v)
+{
+	v += dpp<0x111>(0, v);
+	v += dpp<0x112>(0, v);
+	v += dpp<0x114>(0, v);
+	v += dpp<0x118>(0, v);
+	v += dpp<0x142, 0xa>(0, v);
+	v += dpp<0x143, 0xc>(0, v);
+	return v;
+}
+DEV int wave_max_incl(int v)
+{
+	v = max(v, dpp<0x111>(INT_MIN, v));
+	v = max(v, dpp<0x112>(INT_MIN, v));
+	v = max(v, dpp<0x114>(INT_MIN, v));
+	v = max(v, dpp<0x118>(INT_MIN, v));
+	v = max(v, dpp<0x142, 0xa>(INT_MIN, v));
+	v = max(v, dpp<0x143, 0xc>(INT_MIN, v));
+	return v;
+}
+// lane L takes lane L-1's value, lane 0 takes first
+DEV int shr1(int v, int first) { return dpp<0x138>(first, v); }
+DEV uint64_t shr1_64(uint64_t v, uint64_t first)
+{
+	const uint32_t lo = (uint32_t)shr1((int)(uint32_t)v, (int)(uint32_t)first);
+	const uint32_t hi = (uint32_t)shr1((int)(uint32_t)(v >> 32), (int)(uint32_t)(first >> 32));
+	return (uint64_t)hi << 32 | lo;
+}
+DEV uint64_t readlane_64(uint64_t v, int l)
+{
+	return (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l) << 32 |
+		(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+}
+DEV void wave_sync_lds()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // mp_log2 (mppriv.h:143-151): exponent plus a quadratic in the mantissa, for x >= 2
 __device__ float log2_approx(float x)
@@ -102,124 +157,228 @@ __device__ int32_t comput_sc(uint64_t ai, uint64_t aj, const ChainParams &c)
 	return sc;
 }
 
-// rs_insertsort (ksort.h:116-126): stable insertion by key
-__device__ void rs_insert(uint64_t *x, uint64_t *y, int32_t beg, int32_t end)
+// rs_insertsort (ksort.h:116-126) of at most 64 pairs: insertion by a strict comparison is a
+// stable sort, so each pair goes to its stable rank
+DEV void small_sort(uint64_t *x, uint64_t *y, int32_t beg, int32_t end, ChLds &L)
 {
-	for (int32_t i = beg + 1; i < end; ++i)
-		if (x[i] < x[i - 1]) {
-			const uint64_t tx = x[i], ty = y[i];
-			int32_t j = i;
-			for (; j > beg && tx < x[j - 1]; --j) x[j] = x[j - 1], y[j] = y[j - 1];
-			x[j] = tx, y[j] = ty;
+	const int lane = threadIdx.x, n = end - beg;
+	uint64_t kx = 0, ky = 0;
+	if (lane < n) kx = x[beg + lane], ky = y[beg + lane];
+	L.sk[lane] = kx;
+	wave_sync_lds();
+	int r = 0;
+	if (lane < n)
+		for (int m = 0; m < n; ++m) {
+			const uint64_t o = L.sk[m];
+			r += o < kx || (o == kx && m < lane);
 		}
+	if (lane < n) x[beg + r] = kx, y[beg + r] = ky;
+	__syncthreads();
 }
 
-// one distribution pass of rs_sort (ksort.h:127-150) over [beg, end) by the digit at bit s
-__device__ void rs_distribute(uint64_t *x, uint64_t *y, int32_t beg, int32_t end, int32_t s, RsFrame &F)
+// one rs_sort level (ksort.h:127-150) over [beg, end) into frame d: the key range first, so that
+// levels where every key has the same digit (an identity permutation, after which the reference
+// recurses into the same range) are skipped; then digit counts, bucket bounds, and the in-place
+// cycle-leader permutation on lane 0, which fixes the order of equal keys
+DEV void distribute(uint64_t *x, uint64_t *y, int32_t beg, int32_t end, int32_t s, int d, ChLds &L)
 {
-	const int32_t m = kRsSize - 1;
-	for (int32_t k = 0; k < kRsSize; ++k) F.b[k] = F.e[k] = beg;
-	for (int32_t i = beg; i < end; ++i) ++F.e[(x[i] >> s) & m];
-	for (int32_t k = 1; k < kRsSize; ++k) F.e[k] += F.e[k - 1] - beg, F.b[k] = F.e[k - 1];
-	for (int32_t k = 0; k < kRsSize;) {
-		if (F.b[k] != F.e[k]) {
-			int32_t l = (int32_t)((x[F.b[k]] >> s) & m);
-			if (l != k) {
-				uint64_t tx = x[F.b[k]], ty = y[F.b[k]];
-				do {
-					const uint64_t sx = tx, sy = ty;
-					tx = x[F.b[l]], ty = y[F.b[l]];
-					x[F.b[l]] = sx, y[F.b[l]] = sy;
-					++F.b[l];
-					l = (int32_t)((tx >> s) & m);
-				} while (l != k);
-				x[F.b[k]] = tx, y[F.b[k]] = ty;
-				++F.b[k];
-			} else ++F.b[k];
-		} else ++k;
+	const int lane = threadIdx.x;
+	const uint64_t x0 = x[beg];
+	uint64_t df = 0;
+	for (int32_t i = beg + lane; i < end; i += 64) df |= x[i] ^ x0;
+	if (lane == 0) L.orv = 0;
+	wave_sync_lds();
+	if (df) atomicOr((unsigned long long *)&L.orv, (unsigned long long)df);
+	wave_sync_lds();
+	df = L.orv;
+	if (df == 0) {   // every key equal: the identity at every level
+		if (lane == 0) L.frec[d] = 0, L.fk[d] = kRsSize;
+		wave_sync_lds();
+		return;
 	}
-	F.b[0] = beg;
-	for (int32_t k = 1; k < kRsSize; ++k) F.b[k] = F.e[k - 1];
-	F.recurse = s != 0;
-	F.s_child = s > kRsBits? s - kRsBits : 0;
-	F.k = 0;
+	const int s2 = (63 - __builtin_clzll(df)) & ~(kRsBits - 1);
+	if (s2 < s) s = s2;
+	int32_t *B = L.b[d], *E = L.e[d];
+	for (int k = lane; k < kRsSize; k += 64) E[k] = 0;
+	wave_sync_lds();
+	for (int32_t i = beg + lane; i < end; i += 64) atomicAdd(&E[(x[i] >> s) & (kRsSize - 1)], 1);
+	wave_sync_lds();
+	{
+		const int c0 = E[4 * lane], c1 = E[4 * lane + 1], c2 = E[4 * lane + 2], c3 = E[4 * lane + 3];
+		const int tot = c0 + c1 + c2 + c3, ex = beg + wave_sum_incl(tot) - tot;
+		wave_sync_lds();
+		B[4 * lane] = ex, B[4 * lane + 1] = ex + c0, B[4 * lane + 2] = ex + c0 + c1, B[4 * lane + 3] = ex + c0 + c1 + c2;
+		E[4 * lane] = ex + c0, E[4 * lane + 1] = ex + c0 + c1, E[4 * lane + 2] = ex + c0 + c1 + c2, E[4 * lane + 3] = ex + tot;
+	}
+	__syncthreads();
+	if (lane == 0) {
+		const int32_t m = kRsSize - 1;
+		for (int32_t k = 0; k < kRsSize;) {
+			if (B[k] != E[k]) {
+				int32_t l = (int32_t)((x[B[k]] >> s) & m);
+				if (l != k) {
+					uint64_t tx = x[B[k]], ty = y[B[k]];
+					do {
+						const uint64_t sx = tx, sy = ty;
+						tx = x[B[l]], ty = y[B[l]];
+						x[B[l]] = sx, y[B[l]] = sy;
+						++B[l];
+						l = (int32_t)((tx >> s) & m);
+					} while (l != k);
+					x[B[k]] = tx, y[B[k]] = ty;
+					++B[k];
+				} else ++B[k];
+			} else ++k;
+		}
+	}
+	__syncthreads();
+	for (int k = lane; k < kRsSize; k += 64) B[k] = k? E[k - 1] : beg;
+	if (lane == 0) L.fs[d] = s > kRsBits? s - kRsBits : 0, L.frec[d] = s != 0, L.fk[d] = 0;
+	wave_sync_lds();
 }
 
-// radix_sort_mp128x (ksort.h:158-162) of n (x, y) pairs by x; the recursion runs on frames
-__device__ void radix_sort_pairs(uint64_t *x, uint64_t *y, int32_t n, RsFrame *fr)
+// radix_sort_mp128x (ksort.h:158-162) of n (x, y) pairs by x; the recursion runs on LDS frames,
+// visiting each frame's buckets of two or more keys in order
+DEV void radix_sort_pairs(uint64_t *x, uint64_t *y, int32_t n, ChLds &L)
 {
-	if (n <= kRsMin) { rs_insert(x, y, 0, n); return; }
+	const int lane = threadIdx.x;
+	if (n <= kRsMin) { small_sort(x, y, 0, n, L); return; }
+	distribute(x, y, 0, n, 7 * kRsBits, 0, L);
 	int depth = 0;
-	rs_distribute(x, y, 0, n, 7 * kRsBits, fr[0]);
 	while (depth >= 0) {
-		RsFrame &F = fr[depth];
-		if (!F.recurse || F.k >= kRsSize) { --depth; continue; }
-		const int32_t k = F.k++, sz = F.e[k] - F.b[k];
-		if (sz > kRsMin) {
-			rs_distribute(x, y, F.b[k], F.e[k], F.s_child, fr[depth + 1]);
+		const int k0 = L.fk[depth];
+		if (!L.frec[depth] || k0 >= kRsSize) { --depth; continue; }
+		const int base = k0 & ~63, kk = base + lane;
+		const int32_t bb = L.b[depth][kk], sz = L.e[depth][kk] - bb;
+		const uint64_t m = __ballot(sz > 1) & (~0ull << (k0 & 63));
+		const int lb = m? __builtin_ctzll(m) : 64;
+		const int32_t beg = m? __builtin_amdgcn_readlane(bb, lb) : 0, size = m? __builtin_amdgcn_readlane(sz, lb) : 0;
+		wave_sync_lds();
+		if (lane == 0) L.fk[depth] = base + lb + (m? 1 : 0);
+		wave_sync_lds();
+		if (!m) continue;
+		if (size > kRsMin) {
+			distribute(x, y, beg, beg + size, L.fs[depth], depth + 1, L);
 			++depth;
-		} else if (sz > 1) rs_insert(x, y, F.b[k], F.e[k]);
+		} else small_sort(x, y, beg, beg + size, L);
 	}
 }
 
 __global__ void __launch_bounds__(64) chain_kernel(ChArgs A)
 {
-	const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-	if (tid >= A.n_q) return;
-	const int32_t q = A.order[tid];
+	__shared__ ChLds L;
+	const int lane = threadIdx.x;
+	const int32_t q = A.order[blockIdx.x];
 	const int64_t o = A.off[q], n = A.off[q + 1] - o;
 	const uint64_t *a = A.a + o;
 	int32_t *f = A.f + o, *p = A.p + o, *t = A.t + o, *vv = A.vv + o;
 	uint64_t *zx = A.zx + o, *zy = A.zy + o, *b = A.b + o, *uo = A.u_out + o, *ao = A.a_out + o;
-	RsFrame *fr = A.frames + (int64_t)q * 8;
 	ChainParams c = A.cp;
-	A.n_u[q] = 0;
-	if (n == 0) return;
+	if (n == 0) { if (lane == 0) A.n_u[q] = 0; return; }
 	if (c.max_dist_x < c.bw) c.max_dist_x = c.bw;
 	if (c.max_dist_y < c.bw && !c.is_spliced) c.max_dist_y = c.bw;
 	const int32_t max_drop = c.is_spliced? INT32_MAX : c.bw;
-	// the DP (chain.c:178-208)
+	const uint64_t mdx = (uint64_t)(int64_t)c.max_dist_x;
+	for (int64_t i = lane; i < n; i += 64) t[i] = 0;
+	__syncthreads();
+	// the DP (chain.c:178-208); lane L holds anchor i-1-L
 	int64_t st = 0, hi = -1;
 	int32_t hf = 0;
-	for (int64_t i = 0; i < n; ++i) t[i] = 0;
+	uint64_t wa = 0;
+	int32_t wf = 0, wp = -1;
 	for (int64_t i = 0; i < n; ++i) {
-		int64_t max_j = -1;
+		const uint64_t ai = a[i];
+		bool stop = false;
+		while (st < i - 64) {
+			if (((ai >> 32) - (a[st] >> 32)) << c.bbit > mdx) ++st;
+			else { stop = true; break; }
+		}
+		if (!stop && st < i) {   // the window's far end lies among the lanes: far lanes are the highest
+			const int64_t j = i - 1 - lane;
+			const uint64_t fm = __ballot(j >= st && ((ai >> 32) - (wa >> 32)) << c.bbit > mdx);
+			if (fm) st = i - __builtin_ctzll(fm);
+		}
 		int32_t max_f = c.kmer, n_skip = 0;
-		while (st < i && ((a[i] >> 32) - (a[st] >> 32)) << c.bbit > (uint64_t)(int64_t)c.max_dist_x) ++st;
+		int64_t max_j = -1;
 		if (hi >= 0 && hi >= st) {
-			const int32_t sc = hf + comput_sc(a[i], a[hi], c);
+			const uint64_t ah = hi >= i - 64? readlane_64(wa, (int)(i - 1 - hi)) : a[hi];
+			const int32_t sc = hf + comput_sc(ai, ah, c);
 			if (sc > max_f) max_f = sc, max_j = hi;
 		} else hf = 0, hi = -1;
 		if (i - st > c.max_iter) st = i - c.max_iter;
-		for (int64_t j = i - 1; j >= st; --j) {
-			int32_t sc = comput_sc(a[i], a[j], c);
-			if (sc == INT32_MIN) continue;
-			sc += f[j];
-			if (sc > max_f) {
-				max_f = sc, max_j = j;
-				if (n_skip > 0) --n_skip;
-			} else if (t[j] == (int32_t)i) {
-				if (++n_skip > c.max_skip) break;
+		for (int64_t c0 = i - 1; c0 >= st; c0 -= 64) {   // c0: lane 0's predecessor
+			const int64_t j = c0 - lane;
+			const bool in = j >= st;
+			uint64_t aj = wa;
+			int32_t fj = wf, pj = wp;
+			if (c0 != i - 1) {
+				__syncthreads();
+				aj = in? a[j] : 0, fj = in? f[j] : 0, pj = in? p[j] : -1;
 			}
-			if (p[j] >= 0) t[p[j]] = (int32_t)i;
+			const int32_t sc = in? comput_sc(ai, aj, c) : INT32_MIN;
+			const bool valid = sc != INT32_MIN;
+			const uint64_t vm = __ballot(valid);
+			if (!vm) continue;
+			const int32_t scf = sc + fj;
+			// marks t[p[j]] = i of the lanes before: inside this chunk a lane mask, further down words
+			if (lane == 0) L.cmask = 0;
+			wave_sync_lds();
+			if (valid && pj >= st) {
+				if (pj >= c0 - 63) atomicOr((unsigned long long *)&L.cmask, 1ull << (c0 - pj));
+				else __hip_atomic_store(&t[pj], (int32_t)i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+			}
+			__syncthreads();
+			bool tm = (L.cmask >> lane) & 1;
+			if (c0 != i - 1 && in && !tm)
+				tm = __hip_atomic_load(&t[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == (int32_t)i;
+			const int32_t before = shr1(wave_max_incl(valid? scf : INT_MIN), INT_MIN);
+			const bool imp = valid && scf > (before > max_f? before : max_f);
+			const uint64_t im = __ballot(imp);
+			uint64_t ev = __ballot(imp || (valid && tm));
+			int brk = 64;
+			while (ev) {
+				const int l = __builtin_ctzll(ev);
+				ev &= ev - 1;
+				if ((im >> l) & 1) { if (n_skip > 0) --n_skip; }
+				else if (++n_skip > c.max_skip) { brk = l; break; }
+			}
+			const uint64_t imb = brk == 64? im : im & ((1ull << brk) - 1);
+			if (imb) {
+				const int l = 63 - __builtin_clzll(imb);
+				max_f = __builtin_amdgcn_readlane(scf, l), max_j = c0 - l;
+			}
+			if (brk < 64) break;
 		}
-		f[i] = max_f, p[i] = (int32_t)max_j;
+		if (lane == 0) f[i] = max_f, p[i] = (int32_t)max_j;
 		if (hf < max_f) hf = max_f, hi = i;
+		wa = shr1_64(wa, ai), wf = shr1(wf, max_f), wp = shr1(wp, (int32_t)max_j);
 	}
+	__syncthreads();
 	// the backtrack (mp_chain_backtrack, chain.c:27-82): candidates by score, best first
 	int32_t n_z = 0;
-	for (int64_t i = 0; i < n; ++i)
-		if (f[i] >= c.min_sc) zx[n_z] = (uint64_t)(int64_t)f[i], zy[n_z] = (uint64_t)i, ++n_z;
-	if (n_z == 0) return;
-	radix_sort_pairs(zx, zy, n_z, fr);
-	for (int64_t i = 0; i < n; ++i) t[i] = 0;
-	int32_t n_u = 0, n_v = 0;
-	for (int32_t k = n_z - 1; k >= 0; --k) {
-		const int32_t zk = (int32_t)zy[k], zs = (int32_t)zx[k];
-		if (t[zk] != 0) continue;
-		// mp_chain_bk_end (chain.c:8-25)
-		int32_t end_i = -1, max_i = zk, max_s = 0, ii = zk;
-		if (t[ii] == 0) {
+	for (int64_t i0 = 0; i0 < n; i0 += 64) {
+		const int64_t i = i0 + lane;
+		const int32_t fi = i < n? f[i] : 0;
+		const bool ok = i < n && fi >= c.min_sc;
+		const uint64_t m = __ballot(ok);
+		if (ok) {
+			const int r = n_z + __builtin_popcountll(m & ((1ull << lane) - 1));
+			zx[r] = (uint64_t)(int64_t)fi, zy[r] = (uint64_t)i;
+		}
+		n_z += __builtin_popcountll(m);
+	}
+	__syncthreads();
+	if (n_z == 0) { if (lane == 0) A.n_u[q] = 0; return; }
+	radix_sort_pairs(zx, zy, n_z, L);
+	for (int64_t i = lane; i < n; i += 64) t[i] = 0;
+	__syncthreads();
+	if (lane == 0) {
+		int32_t n_u = 0, n_v = 0;
+		for (int32_t k = n_z - 1; k >= 0; --k) {
+			const int32_t zk = (int32_t)zy[k], zs = (int32_t)zx[k];
+			if (t[zk] != 0) continue;
+			// mp_chain_bk_end (chain.c:8-25)
+			int32_t end_i = -1, max_i = zk, max_s = 0, ii = zk;
 			do {
 				t[ii] = 2;
 				end_i = ii = p[ii];
@@ -229,37 +388,48 @@ __global__ void __launch_bounds__(64) chain_kernel(ChArgs A)
 			} while (ii >= 0 && t[ii] == 0);
 			for (ii = zk; ii >= 0 && ii != end_i; ii = p[ii]) t[ii] = 0;
 			end_i = max_i;
-		} else end_i = ii;
-		const int32_t n_v0 = n_v;
-		for (ii = zk; ii != end_i; ii = p[ii]) vv[n_v++] = ii, t[ii] = 1;
-		const int32_t sc = ii < 0? zs : zs - f[ii];
-		if (sc >= c.min_sc && n_v > n_v0 && n_v - n_v0 >= c.min_cnt)
-			uo[n_u++] = (uint64_t)(uint32_t)sc << 32 | (uint32_t)(n_v - n_v0);
-		else n_v = n_v0;
+			const int32_t n_v0 = n_v;
+			for (ii = zk; ii != end_i; ii = p[ii]) vv[n_v++] = ii, t[ii] = 1;
+			const int32_t sc = ii < 0? zs : zs - f[ii];
+			if (sc >= c.min_sc && n_v > n_v0 && n_v - n_v0 >= c.min_cnt) {
+				uo[n_u++] = (uint64_t)(uint32_t)sc << 32 | (uint32_t)(n_v - n_v0);
+				for (int32_t x = n_v0; x < n_v; ++x) b[x] = (uint64_t)(n_v0 + n_v - 1);   // x -> its mirror
+			} else n_v = n_v0;
+		}
+		L.nu = n_u, L.nv = n_v;
 	}
+	__syncthreads();
+	const int32_t n_u = L.nu, n_v = L.nv;
+	if (lane == 0) A.n_u[q] = n_u;
 	if (n_u == 0) return;
 	// compact_a (chain.c:84-110): each chain's anchors in increasing order, then the chains by
 	// their first target block
-	for (int32_t i = 0, k = 0; i < n_u; ++i) {
-		const int32_t k0 = k, ni = (int32_t)(uint32_t)uo[i];
-		for (int32_t j = 0; j < ni; ++j) b[k++] = a[vv[k0 + (ni - j - 1)]];
+	for (int32_t x = lane; x < n_v; x += 64) b[x] = a[vv[(int32_t)b[x] - x]];
+	__syncthreads();
+	for (int32_t i0 = 0, k0 = 0; i0 < n_u; i0 += 64) {
+		const int32_t i = i0 + lane;
+		const int32_t cnt = i < n_u? (int32_t)(uint32_t)uo[i] : 0;
+		const int32_t incl = wave_sum_incl(cnt), k = k0 + incl - cnt;
+		if (i < n_u) zx[i] = b[k] >> 32, zy[i] = (uint64_t)k << 32 | (uint32_t)i;
+		k0 += __builtin_amdgcn_readlane(incl, 63);
 	}
-	for (int32_t i = 0, k = 0; i < n_u; ++i) {
-		zx[i] = b[k] >> 32, zy[i] = (uint64_t)k << 32 | (uint32_t)i;
-		k += (int32_t)(uint32_t)uo[i];
+	__syncthreads();
+	radix_sort_pairs(zx, zy, n_u, L);
+	for (int32_t i0 = 0, k0 = 0; i0 < n_u; i0 += 64) {
+		const int32_t i = i0 + lane;
+		const int32_t j = i < n_u? (int32_t)(uint32_t)zy[i] : 0;
+		const uint64_t uj = i < n_u? uo[j] : 0;
+		const int32_t cnt = (int32_t)(uint32_t)uj;
+		const int32_t incl = wave_sum_incl(cnt), k = k0 + incl - cnt;
+		if (i < n_u) {
+			const int64_t from = (int64_t)(zy[i] >> 32);
+			for (int32_t x = 0; x < cnt; ++x) ao[k + x] = b[from + x];
+			zx[i] = uj;
+		}
+		k0 += __builtin_amdgcn_readlane(incl, 63);
 	}
-	radix_sort_pairs(zx, zy, n_u, fr);
-	// the sorted order: chains into p (as indices), anchors into the output
-	for (int32_t i = 0, k = 0; i < n_u; ++i) {
-		const int32_t j = (int32_t)(uint32_t)zy[i], cnt = (int32_t)(uint32_t)uo[j];
-		const int64_t from = (int64_t)(zy[i] >> 32);
-		for (int32_t x = 0; x < cnt; ++x) ao[k + x] = b[from + x];
-		k += cnt;
-		vv[i] = j;
-	}
-	for (int32_t i = 0; i < n_u; ++i) zx[i] = uo[vv[i]];
-	for (int32_t i = 0; i < n_u; ++i) uo[i] = zx[i];
-	A.n_u[q] = n_u;
+	__syncthreads();
+	for (int32_t i = lane; i < n_u; i += 64) uo[i] = zx[i];
 }
 
 } // namespace
@@ -290,7 +460,6 @@ extern "C" int64_t nsh_chain_batch(int32_t n_q, const int64_t *a_off, const uint
 	int64_t *d_off = nullptr;
 	int32_t *d_f = nullptr, *d_p = nullptr, *d_t = nullptr, *d_vv = nullptr, *d_nu = nullptr, *d_ord = nullptr;
 	std::vector<int32_t> order(n_q);
-	RsFrame *d_fr = nullptr;
 	const size_t na = (size_t)(n_a > 0? n_a : 1);
 	ChArgs A;
 	{
@@ -307,7 +476,6 @@ extern "C" int64_t nsh_chain_batch(int32_t n_q, const int64_t *a_off, const uint
 		CHOK(hipMalloc((void**)&d_u, na * 8));
 		CHOK(hipMalloc((void**)&d_ao, na * 8));
 		CHOK(hipMalloc((void**)&d_nu, (size_t)n_q * 4));
-		CHOK(hipMalloc((void**)&d_fr, (size_t)n_q * 8 * sizeof(RsFrame)));
 		CHOK(hipMalloc((void**)&d_ord, (size_t)n_q * 4));
 		for (int32_t q = 0; q < n_q; ++q) order[q] = q;
 		std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
@@ -317,9 +485,9 @@ extern "C" int64_t nsh_chain_batch(int32_t n_q, const int64_t *a_off, const uint
 		CHOK(hipMemcpyAsync(d_off, a_off, (size_t)(n_q + 1) * 8, hipMemcpyHostToDevice, s));
 		A.a = d_a, A.off = d_off, A.n_q = n_q, A.order = d_ord;
 		A.cp = { max_dist_x, max_dist_y, bw, max_skip, max_iter, min_cnt, min_sc, chn_coef_log, is_spliced, kmer, bbit };
-		A.f = d_f, A.p = d_p, A.t = d_t, A.vv = d_vv, A.zx = d_zx, A.zy = d_zy, A.b = d_b, A.frames = d_fr;
+		A.f = d_f, A.p = d_p, A.t = d_t, A.vv = d_vv, A.zx = d_zx, A.zy = d_zy, A.b = d_b;
 		A.u_out = d_u, A.a_out = d_ao, A.n_u = d_nu;
-		hipLaunchKernelGGL(chain_kernel, dim3((n_q + 63) / 64), dim3(64), 0, s, A);
+		hipLaunchKernelGGL(chain_kernel, dim3(n_q), dim3(64), 0, s, A);
 		CHOK(hipGetLastError());
 		CHOK(hipMemcpyAsync(n_u, d_nu, (size_t)n_q * 4, hipMemcpyDeviceToHost, s));
 		if (n_a > 0) {
@@ -334,7 +502,7 @@ extern "C" int64_t nsh_chain_batch(int32_t n_q, const int64_t *a_off, const uint
 done:
 	if (s) (void)hipStreamSynchronize(s);
 	{
-		void *bufs[] = { d_a, d_off, d_f, d_p, d_t, d_vv, d_zx, d_zy, d_b, d_u, d_ao, d_nu, d_fr, d_ord };
+		void *bufs[] = { d_a, d_off, d_f, d_p, d_t, d_vv, d_zx, d_zy, d_b, d_u, d_ao, d_nu, d_ord };
 		for (void *bb : bufs) if (bb) (void)hipFree(bb);
 	}
 	if (s) (void)hipStreamDestroy(s);
