@@ -374,6 +374,10 @@ def bench_index(cpu=True, genome_len=100_000_000, n_ctg=24, log=None):
         t0 = time.perf_counter()
         got = ix.anchors(qs)
         an_s = time.perf_counter() - t0
+        ix.chain(qs[:100])
+        t0 = time.perf_counter()
+        chains = ix.chain(qs)
+        ch_s = time.perf_counter() - t0
     finally:
         ix.close()
     n_anchor = int(sum(len(a) for a in got))
@@ -403,6 +407,60 @@ def bench_index(cpu=True, genome_len=100_000_000, n_ctg=24, log=None):
                                           "sample": "the first %d queries, %.1f s" % (done, cpu_s)}
         out["anchors"]["identical_to_reference"] = "%d of %d sampled queries" % (same, done)
     out["chain"] = bench_chain(got, cpu=cpu, log=log)
+    out["index_chain"] = bench_index_chain(g, off, ln, qs, chains, ch_s, cpu=cpu, log=log)
+    return out
+
+
+def bench_index_chain(g, off, ln, qs, chains, gpu_s, cpu=True, log=None):
+    """mp_map_coarse's index -> chain slice (map.c:583-633, default options): anchors, pre-chain,
+    each query's kept anchors sorted, chain, all on the device (nsh_index_chain; the queries'
+    upload and the chains' download included). The reference's code (oracle/_ref) runs the same
+    steps per query on one host thread over a sample; its chains must be identical"""
+    import ctypes as C
+    import numpy as np
+    import nsffi
+    n_u = int(sum(len(u) for u, _ in chains))
+    out = {"queries": len(qs), "chains": n_u, "chain_anchors": int(sum(len(a) for _, a in chains)),
+           "gpu_ms": round(gpu_s * 1e3, 1), "gpu_queries_per_s": round(len(qs) / gpu_s, 1)}
+    if log:
+        log("[bench] index -> chain: %d queries, %d chains, GPU %.1f ms" % (len(qs), n_u, gpu_s * 1e3))
+    if not (cpu and nsffi.have_ref()):
+        return out
+    ref = nsffi.Reference()
+    L = ref.lib
+    h = L.or_ref_index_build(len(off), g.ctypes.data, off.ctypes.data, ln.ctypes.data, 30, 6, 1, 8)
+    buf = np.zeros(1 << 23, np.uint64)
+    u1, a1, u2, a2 = (np.zeros(1 << 23, np.uint64) for _ in range(4))
+    same, done, t_ref = 0, 0, 0.0
+    try:
+        t_end = time.perf_counter() + 12.0
+        for k, q in enumerate(qs):
+            t0 = time.perf_counter()
+            n = L.or_ref_anchors(h, q, len(q), 20000, len(buf), buf.ctypes.data)
+            nu1 = L.or_ref_chain(256, 256, 256, 25, 1000000, 2, 0, 0.75, 1, 6, 8, n, buf.ctypes.data, len(u1),
+                                 u1.ctypes.data, len(a1), a1.ctypes.data)
+            m1 = int((u1[:nu1] & np.uint64(0xffffffff)).sum())
+            a1[:m1].sort()
+            nu2 = L.or_ref_chain(200000, 1000, 200000, 25, 1000000, 3, 0, 0.75, 1, 6, 8, m1, a1.ctypes.data,
+                                 len(u2), u2.ctypes.data, len(a2), a2.ctypes.data)
+            t_ref += time.perf_counter() - t0
+            m2 = int((u2[:nu2] & np.uint64(0xffffffff)).sum())
+            gu, ga = chains[k]
+            same += np.array_equal(gu, u2[:nu2]) and np.array_equal(ga, a2[:m2])
+            done += 1
+            if time.perf_counter() > t_end:
+                break
+    finally:
+        L.or_ref_index_free(h)
+    out["cpu_baseline"] = {"value": round(t_ref * 1e3 / done * len(qs), 1),
+                           "unit": "ms per %d queries (reference code, one host thread, scaled)" % len(qs),
+                           "cores": 1, "kind": "reference",
+                           "sample": "the first %d queries, %.1f s of reference work" % (done, t_ref)}
+    out["identical_to_reference"] = "%d of %d sampled queries" % (same, done)
+    out["gpu_vs_cpu_thread"] = round(out["cpu_baseline"]["value"] / out["gpu_ms"], 1)
+    if log:
+        log("[bench] index -> chain: reference %.1f ms (scaled), identical %s" % (out["cpu_baseline"]["value"],
+                                                                                 out["identical_to_reference"]))
     return out
 
 
@@ -447,23 +505,29 @@ def bench_chain(anchors, cpu=True, log=None):
         log("[bench] chain: %d -> %d anchors, %d chains; GPU pre-chain %.1f ms, chain %.1f ms" % (
             out["anchors"], out["prechain_kept_anchors"], out["chains"], pre_s * 1e3, main_s * 1e3))
     if cpu and nsffi.have_ref():
-        ref = nsffi.Reference()
-        same, done, t0 = 0, 0, time.perf_counter()
+        L = nsffi.Reference().lib
+        u1, a1, ur, ar = (np.zeros(1 << 23, np.uint64) for _ in range(4))
+        same, done, cpu_s = 0, 0, 0.0
+        t_end = time.perf_counter() + 10.0
         for q in range(len(anchors)):
-            pu, pa = ref.chain(anchors[q], *pre)
-            mu, ma = ref.chain(np.sort(np.array(pa, np.uint64)), *main)
+            aq = np.ascontiguousarray(anchors[q])
+            t0 = time.perf_counter()
+            nu1 = L.or_ref_chain(*pre, len(aq), aq.ctypes.data, len(u1), u1.ctypes.data, len(a1), a1.ctypes.data)
+            m1 = int((u1[:nu1] & np.uint64(0xffffffff)).sum())
+            a1[:m1].sort()
+            nu2 = L.or_ref_chain(*main, m1, a1.ctypes.data, len(ur), ur.ctypes.data, len(ar), ar.ctypes.data)
+            cpu_s += time.perf_counter() - t0
             gu = u2[off2[q]:off2[q] + n_u2[q]]
             ga = ao2[off2[q]:off2[q] + int((gu & np.uint64(0xffffffff)).sum())]
-            same += gu.tolist() == mu and ga.tolist() == ma
+            m2 = int((ur[:nu2] & np.uint64(0xffffffff)).sum())
+            same += np.array_equal(gu, ur[:nu2]) and np.array_equal(ga, ar[:m2])
             done += 1
-            if time.perf_counter() - t0 > 10.0:
+            if time.perf_counter() > t_end:
                 break
-        cpu_s = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(cpu_s * 1e3 / done * len(anchors), 1),
                                "unit": "ms per %d queries, both passes (reference code, one host thread, scaled)" % len(
                                    anchors), "cores": 1, "kind": "reference",
-                               "sample": "the first %d queries, %.1f s (ctypes list conversion included)" % (done,
-                                                                                                           cpu_s)}
+                               "sample": "the first %d queries, %.1f s of reference work" % (done, cpu_s)}
         out["identical_to_reference"] = "%d of %d sampled queries" % (same, done)
         if log:
             log("[bench] chain: reference %s, identical %s" % (out["cpu_baseline"]["value"],

@@ -358,6 +358,23 @@ int64_t nsh_chain_batch(int32_t n_q, const int64_t *a_off, const uint64_t *ancho
 		int32_t max_dist_y, int32_t bw, int32_t max_skip, int32_t max_iter, int32_t min_cnt, int32_t min_sc,
 		float chn_coef_log, int32_t is_spliced, int32_t kmer, int32_t bbit, int32_t *n_u, uint64_t *u, uint64_t *a_out);
 
+/* mp_map_coarse's index -> chain slice for n_q queries (map.c:583-633 with the default options,
+ * no --accuracy vid chaining): the anchors as nsh_index_anchors collects them, then, when
+ * pre_chain and is_spliced are set, the pre-chain (mp_chain with max_dist_x = max_dist_y = bw =
+ * 1 << bbit, min_cnt 2, min_sc 0) and each query's kept anchors sorted (radix_sort_mp64), then
+ * mp_chain with these options; kmer and bbit are the index's. Everything stays on the device.
+ * Query q's chains (score << 32 | anchors) are u[u_off[q] .. u_off[q+1]) and their anchors, chain
+ * after chain, anchors[c_off[q] .. c_off[q+1]); u_off and c_off (n_q + 1 entries) are always
+ * written, at most max_u chains and max_anchors anchors. Returns the total chain count, or -1. */
+typedef struct {
+	int32_t max_dist_x, max_dist_y, bw, max_skip, max_iter, min_cnt, min_sc;   /* max_intron, max_gap, ... */
+	float chn_coef_log;
+	int32_t is_spliced, pre_chain;
+} nsh_chain_opt_t;
+int64_t nsh_index_chain(const nsh_index_t *ix, int32_t n_q, const char *seq, const int64_t *q_off,
+		const int32_t *q_len, int32_t max_occ, const nsh_chain_opt_t *opt, int64_t *u_off, uint64_t *u,
+		int64_t max_u, int64_t *c_off, uint64_t *anchors, int64_t max_anchors);
+
 /* Learned chain-candidate prescreen: MLP 64 -> 256 -> 256 -> 1 (ReLU, logistic output), BF16
  * MFMA kernels with FP32 accumulation, FP32 Adam masters. Parameter vector (FP32, in order):
  * W1[64][256], b1[256], W2[256][256], b2[256], w3[256], b3. No reference counterpart. */

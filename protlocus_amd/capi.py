@@ -142,6 +142,10 @@ def lib():
         L.nsh_chain_batch.argtypes = ([C.c_int32, C.c_void_p, C.c_void_p] + [C.c_int32] * 7 +
                                       [C.c_float, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p])
         L.nsh_chain_batch.restype = C.c_int64
+        L.nsh_index_chain.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                      C.POINTER(ChainOpt), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                      C.c_int64]
+        L.nsh_index_chain.restype = C.c_int64
         L.nsh_gapless_select_batch.argtypes = [C.c_int32] + [C.c_void_p] * 6 + [C.c_int32] * 3 + [C.c_void_p] * 3
         L.nsh_gapless_prefilter_batch.argtypes = ([C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32,
                                                    C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
@@ -443,6 +447,20 @@ def index_build(genome, ctg_off, ctg_len, min_aa_len=30, kmer=6, mod_bit=1, bbit
     return ki, kb[:n_kb], bo
 
 
+class ChainOpt(C.Structure):
+    """nsh_chain_opt_t; the defaults are the reference's (mp_mapopt_init, options.c) for a
+    spliced search: max_intron 200000, max_gap 1000, bw 200000, max_chn_max_skip 25,
+    max_chn_iter 1000000, min_chn_cnt 3, min_chn_sc 0, chn_coef_log 0.75, pre-chain on"""
+    _fields_ = [("max_dist_x", C.c_int32), ("max_dist_y", C.c_int32), ("bw", C.c_int32), ("max_skip", C.c_int32),
+                ("max_iter", C.c_int32), ("min_cnt", C.c_int32), ("min_sc", C.c_int32), ("chn_coef_log", C.c_float),
+                ("is_spliced", C.c_int32), ("pre_chain", C.c_int32)]
+
+    def __init__(self, max_dist_x=200000, max_dist_y=1000, bw=200000, max_skip=25, max_iter=1000000, min_cnt=3,
+                 min_sc=0, chn_coef_log=0.75, is_spliced=1, pre_chain=1):
+        super().__init__(max_dist_x, max_dist_y, bw, max_skip, max_iter, min_cnt, min_sc, chn_coef_log, is_spliced,
+                         pre_chain)
+
+
 class Index:
     """a k-mer block index resident on the device (nsh_index_*)"""
 
@@ -471,6 +489,25 @@ class Index:
             if n <= cap:
                 return [a[aoff[q]:aoff[q + 1]] for q in range(len(ln))]
             cap = n
+
+    def chain(self, queries, max_occ=20000, opt=None):
+        """mp_map_coarse's anchors -> pre-chain -> chain on the device (nsh_index_chain): per query
+        the chains (score << 32 | count) and their anchors, as numpy uint64 arrays"""
+        opt = opt or ChainOpt()
+        seq = b"".join(bytes(q) for q in queries)
+        ln = np.array([len(q) for q in queries], dtype=np.int32)
+        off = np.concatenate([[0], np.cumsum(ln)[:-1]]).astype(np.int64) if len(ln) else np.zeros(0, np.int64)
+        uoff, coff = np.zeros(len(ln) + 1, np.int64), np.zeros(len(ln) + 1, np.int64)
+        cu, ca = len(seq) + 1024, 8 * len(seq) + 1024   # first guesses; a second call when they do not fit
+        while True:
+            u, a = np.empty(cu, np.uint64), np.empty(ca, np.uint64)
+            n = lib().nsh_index_chain(self.p, len(ln), seq, off.ctypes.data, ln.ctypes.data, max_occ, C.byref(opt),
+                                      uoff.ctypes.data, u.ctypes.data, cu, coff.ctypes.data, a.ctypes.data, ca)
+            if n < 0:
+                raise RuntimeError("nsh_index_chain: " + error())
+            if uoff[-1] <= cu and coff[-1] <= ca:
+                return [(u[uoff[q]:uoff[q + 1]], a[coff[q]:coff[q + 1]]) for q in range(len(ln))]
+            cu, ca = max(cu, int(uoff[-1])), max(ca, int(coff[-1]))
 
     def close(self):
         if self.p:

@@ -34,17 +34,15 @@ namespace {
 constexpr int kRsMin = 64, kRsBits = 8, kRsSize = 1 << kRsBits;
 constexpr int kBlockBonus = 2;   // MP_BLOCK_BONUS (protlocus.h:47)
 
-struct ChainParams {
-	int32_t max_dist_x, max_dist_y, bw, max_skip, max_iter, min_cnt, min_sc;
-	float coef;
-	int32_t is_spliced, kmer, bbit;
-};
+using ChainParams = nsh_chain::Params;
 
 struct ChArgs {
 	const uint64_t *a; const int64_t *off; int32_t n_q;
+	const int64_t *cnt;                // per query anchor count, or null: the offsets' differences
 	const int32_t *order;              // queries by decreasing anchor count
 	ChainParams cp;
 	int32_t *f, *p, *t, *vv;           // per anchor
+	uint8_t *hs;                       // per anchor: another anchor chains to it
 	uint64_t *zx, *zy;                 // per anchor: sort pairs
 	uint64_t *b;                       // per anchor: the chains' anchors before the final order
 	uint64_t *u_out, *a_out;           // per anchor slot: chains (score << 32 | count), their anchors
@@ -211,25 +209,35 @@ DEV void distribute(uint64_t *x, uint64_t *y, int32_t beg, int32_t end, int32_t 
 		E[4 * lane] = ex + c0, E[4 * lane + 1] = ex + c0 + c1, E[4 * lane + 2] = ex + c0 + c1 + c2, E[4 * lane + 3] = ex + tot;
 	}
 	__syncthreads();
-	if (lane == 0) {
-		const int32_t m = kRsSize - 1;
-		for (int32_t k = 0; k < kRsSize;) {
-			if (B[k] != E[k]) {
-				int32_t l = (int32_t)((x[B[k]] >> s) & m);
-				if (l != k) {
-					uint64_t tx = x[B[k]], ty = y[B[k]];
-					do {
-						const uint64_t sx = tx, sy = ty;
-						tx = x[B[l]], ty = y[B[l]];
-						x[B[l]] = sx, y[B[l]] = sy;
-						++B[l];
-						l = (int32_t)((tx >> s) & m);
-					} while (l != k);
-					x[B[k]] = tx, y[B[k]] = ty;
-					++B[k];
-				} else ++B[k];
-			} else ++k;
+	// the cycle-leader pass, bucket by bucket: a run of keys already in their bucket advances the
+	// cursor at once (all lanes look at the next 64); a key out of place starts a cycle on lane 0
+	const int32_t m = kRsSize - 1;
+	for (int32_t k = 0; k < kRsSize;) {
+		const int32_t bk = B[k], ek = E[k];
+		if (bk == ek) { ++k; continue; }
+		const bool in = bk + lane < ek;
+		const uint64_t out = __ballot(in && (int32_t)((x[bk + lane] >> s) & m) != k);
+		const int32_t run = out? __builtin_ctzll(out) : min(64, ek - bk);
+		wave_sync_lds();
+		if (run > 0) {
+			if (lane == 0) B[k] = bk + run;
+			wave_sync_lds();
+			continue;
 		}
+		if (lane == 0) {
+			int32_t l = (int32_t)((x[bk] >> s) & m);
+			uint64_t tx = x[bk], ty = y[bk];
+			do {
+				const uint64_t sx = tx, sy = ty;
+				tx = x[B[l]], ty = y[B[l]];
+				x[B[l]] = sx, y[B[l]] = sy;
+				++B[l];
+				l = (int32_t)((tx >> s) & m);
+			} while (l != k);
+			x[bk] = tx, y[bk] = ty;
+			B[k] = bk + 1;
+		}
+		__syncthreads();
 	}
 	__syncthreads();
 	for (int k = lane; k < kRsSize; k += 64) B[k] = k? E[k - 1] : beg;
@@ -269,10 +277,11 @@ __global__ void __launch_bounds__(64) chain_kernel(ChArgs A)
 	__shared__ ChLds L;
 	const int lane = threadIdx.x;
 	const int32_t q = A.order[blockIdx.x];
-	const int64_t o = A.off[q], n = A.off[q + 1] - o;
+	const int64_t o = A.off[q], n = A.cnt? A.cnt[q] : A.off[q + 1] - o;
 	const uint64_t *a = A.a + o;
 	int32_t *f = A.f + o, *p = A.p + o, *t = A.t + o, *vv = A.vv + o;
 	uint64_t *zx = A.zx + o, *zy = A.zy + o, *b = A.b + o, *uo = A.u_out + o, *ao = A.a_out + o;
+	uint8_t *hs = A.hs + o;
 	ChainParams c = A.cp;
 	if (n == 0) { if (lane == 0) A.n_u[q] = 0; return; }
 	if (c.max_dist_x < c.bw) c.max_dist_x = c.bw;
@@ -370,12 +379,27 @@ __global__ void __launch_bounds__(64) chain_kernel(ChArgs A)
 	__syncthreads();
 	if (n_z == 0) { if (lane == 0) A.n_u[q] = 0; return; }
 	radix_sort_pairs(zx, zy, n_z, L);
-	for (int64_t i = lane; i < n; i += 64) t[i] = 0;
+	// anchors no other anchor chains to (hs = 0): started from one with no predecessor either, the
+	// backtrack only marks it (its chain of one is dropped when min_cnt > 1 or its score is not
+	// positive), and no later walk can reach it, so such candidates are passed over
+	for (int64_t i = lane; i < n; i += 64) t[i] = 0, hs[i] = 0;
 	__syncthreads();
-	if (lane == 0) {
-		int32_t n_u = 0, n_v = 0;
-		for (int32_t k = n_z - 1; k >= 0; --k) {
+	for (int64_t i = lane; i < n; i += 64) if (p[i] >= 0) hs[p[i]] = 1;
+	__syncthreads();
+	int32_t bu = 0, bv = 0;   // lane 0's chain and anchor counts
+	for (int32_t k0 = n_z - 1; k0 >= 0; k0 -= 64) {
+		const int32_t k = k0 - lane;
+		bool live = false;
+		if (k >= 0) {
 			const int32_t zk = (int32_t)zy[k], zs = (int32_t)zx[k];
+			live = p[zk] >= 0 || hs[zk] || (c.min_cnt <= 1 && zs > 0 && zs >= c.min_sc);
+		}
+		uint64_t lm = __ballot(live);
+		while (lm) {
+			const int l = __builtin_ctzll(lm);
+			lm &= lm - 1;
+			if (lane != 0) continue;
+			const int32_t zk = (int32_t)zy[k0 - l], zs = (int32_t)zx[k0 - l];
 			if (t[zk] != 0) continue;
 			// mp_chain_bk_end (chain.c:8-25)
 			int32_t end_i = -1, max_i = zk, max_s = 0, ii = zk;
@@ -388,16 +412,16 @@ __global__ void __launch_bounds__(64) chain_kernel(ChArgs A)
 			} while (ii >= 0 && t[ii] == 0);
 			for (ii = zk; ii >= 0 && ii != end_i; ii = p[ii]) t[ii] = 0;
 			end_i = max_i;
-			const int32_t n_v0 = n_v;
-			for (ii = zk; ii != end_i; ii = p[ii]) vv[n_v++] = ii, t[ii] = 1;
+			const int32_t n_v0 = bv;
+			for (ii = zk; ii != end_i; ii = p[ii]) vv[bv++] = ii, t[ii] = 1;
 			const int32_t sc = ii < 0? zs : zs - f[ii];
-			if (sc >= c.min_sc && n_v > n_v0 && n_v - n_v0 >= c.min_cnt) {
-				uo[n_u++] = (uint64_t)(uint32_t)sc << 32 | (uint32_t)(n_v - n_v0);
-				for (int32_t x = n_v0; x < n_v; ++x) b[x] = (uint64_t)(n_v0 + n_v - 1);   // x -> its mirror
-			} else n_v = n_v0;
+			if (sc >= c.min_sc && bv > n_v0 && bv - n_v0 >= c.min_cnt) {
+				uo[bu++] = (uint64_t)(uint32_t)sc << 32 | (uint32_t)(bv - n_v0);
+				for (int32_t x = n_v0; x < bv; ++x) b[x] = (uint64_t)(n_v0 + bv - 1);   // x -> its mirror
+			} else bv = n_v0;
 		}
-		L.nu = n_u, L.nv = n_v;
 	}
+	if (lane == 0) L.nu = bu, L.nv = bv;
 	__syncthreads();
 	const int32_t n_u = L.nu, n_v = L.nv;
 	if (lane == 0) A.n_u[q] = n_u;
@@ -436,6 +460,48 @@ __global__ void __launch_bounds__(64) chain_kernel(ChArgs A)
 
 #define CHOK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { nsh_set_error("chain: %s: %s", #x, hipGetErrorString(e_)); goto done; } } while (0)
 
+int nsh_chain::run(hipStream_t s, int32_t n_q, const int64_t *d_off, const int64_t *d_cnt, const int64_t *h_cnt,
+		const uint64_t *d_a, int64_t n_a, const Params &prm, int32_t *d_nu, uint64_t *d_u, uint64_t *d_ao)
+{
+	if (n_q == 0) return 1;
+	int ok = 0;
+	const size_t na = (size_t)(n_a > 0? n_a : 1);
+	uint64_t *d_zx = nullptr, *d_zy = nullptr, *d_b = nullptr;
+	int32_t *d_f = nullptr, *d_p = nullptr, *d_t = nullptr, *d_vv = nullptr, *d_ord = nullptr;
+	uint8_t *d_hs = nullptr;
+	std::vector<int32_t> order(n_q);
+	ChArgs A;
+	{
+		CHOK(hipMalloc((void**)&d_f, na * 4));
+		CHOK(hipMalloc((void**)&d_p, na * 4));
+		CHOK(hipMalloc((void**)&d_t, na * 4));
+		CHOK(hipMalloc((void**)&d_vv, na * 4));
+		CHOK(hipMalloc((void**)&d_hs, na));
+		CHOK(hipMalloc((void**)&d_zx, na * 8));
+		CHOK(hipMalloc((void**)&d_zy, na * 8));
+		CHOK(hipMalloc((void**)&d_b, na * 8));
+		CHOK(hipMalloc((void**)&d_ord, (size_t)n_q * 4));
+		for (int32_t q = 0; q < n_q; ++q) order[q] = q;   // largest queries first
+		std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return h_cnt[x] > h_cnt[y]; });
+		CHOK(hipMemcpyAsync(d_ord, order.data(), (size_t)n_q * 4, hipMemcpyHostToDevice, s));
+		A.a = d_a, A.off = d_off, A.cnt = d_cnt, A.n_q = n_q, A.order = d_ord, A.cp = prm;
+		A.f = d_f, A.p = d_p, A.t = d_t, A.vv = d_vv, A.hs = d_hs, A.zx = d_zx, A.zy = d_zy, A.b = d_b;
+		A.u_out = d_u, A.a_out = d_ao, A.n_u = d_nu;
+		hipLaunchKernelGGL(chain_kernel, dim3(n_q), dim3(64), 0, s, A);
+		CHOK(hipGetLastError());
+		CHOK(hipStreamSynchronize(s));
+		nsh_count_launch(1);
+		ok = 1;
+	}
+done:
+	(void)hipStreamSynchronize(s);
+	{
+		void *bufs[] = { d_f, d_p, d_t, d_vv, d_hs, d_zx, d_zy, d_b, d_ord };
+		for (void *bb : bufs) if (bb) (void)hipFree(bb);
+	}
+	return ok;
+}
+
 extern "C" int64_t nsh_chain_batch(int32_t n_q, const int64_t *a_off, const uint64_t *anchors, int32_t max_dist_x,
 		int32_t max_dist_y, int32_t bw, int32_t max_skip, int32_t max_iter, int32_t min_cnt, int32_t min_sc,
 		float chn_coef_log, int32_t is_spliced, int32_t kmer, int32_t bbit, int32_t *n_u, uint64_t *u, uint64_t *a_out)
@@ -447,62 +513,47 @@ extern "C" int64_t nsh_chain_batch(int32_t n_q, const int64_t *a_off, const uint
 	if (n_q == 0) return 0;
 	const int64_t n_a = a_off[n_q];
 	if (a_off[0] != 0 || n_a < 0) { nsh_set_error("chain: a_off must start at 0"); return -1; }
-	for (int32_t q = 0; q < n_q; ++q)
+	std::vector<int64_t> cnt(n_q);
+	for (int32_t q = 0; q < n_q; ++q) {
 		if (a_off[q + 1] < a_off[q] || a_off[q + 1] - a_off[q] > INT32_MAX / 2) {
 			nsh_set_error("chain: query %d has a bad anchor range", q);
 			return -1;
 		}
+		cnt[q] = a_off[q + 1] - a_off[q];
+	}
 	if (n_a > 0 && (!anchors || !u || !a_out)) { nsh_set_error("chain: bad arguments"); return -1; }
 	if (!nsh_ensure_device()) return -1;
 	int64_t total = -1;
 	hipStream_t s = nullptr;
-	uint64_t *d_a = nullptr, *d_zx = nullptr, *d_zy = nullptr, *d_b = nullptr, *d_u = nullptr, *d_ao = nullptr;
+	uint64_t *d_a = nullptr, *d_u = nullptr, *d_ao = nullptr;
 	int64_t *d_off = nullptr;
-	int32_t *d_f = nullptr, *d_p = nullptr, *d_t = nullptr, *d_vv = nullptr, *d_nu = nullptr, *d_ord = nullptr;
-	std::vector<int32_t> order(n_q);
+	int32_t *d_nu = nullptr;
 	const size_t na = (size_t)(n_a > 0? n_a : 1);
-	ChArgs A;
+	const nsh_chain::Params prm = { max_dist_x, max_dist_y, bw, max_skip, max_iter, min_cnt, min_sc, chn_coef_log,
+		is_spliced, kmer, bbit };
 	{
 		CHOK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
 		CHOK(hipMalloc((void**)&d_a, na * 8));
 		CHOK(hipMalloc((void**)&d_off, (size_t)(n_q + 1) * 8));
-		CHOK(hipMalloc((void**)&d_f, na * 4));
-		CHOK(hipMalloc((void**)&d_p, na * 4));
-		CHOK(hipMalloc((void**)&d_t, na * 4));
-		CHOK(hipMalloc((void**)&d_vv, na * 4));
-		CHOK(hipMalloc((void**)&d_zx, na * 8));
-		CHOK(hipMalloc((void**)&d_zy, na * 8));
-		CHOK(hipMalloc((void**)&d_b, na * 8));
 		CHOK(hipMalloc((void**)&d_u, na * 8));
 		CHOK(hipMalloc((void**)&d_ao, na * 8));
 		CHOK(hipMalloc((void**)&d_nu, (size_t)n_q * 4));
-		CHOK(hipMalloc((void**)&d_ord, (size_t)n_q * 4));
-		for (int32_t q = 0; q < n_q; ++q) order[q] = q;
-		std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-			return a_off[x + 1] - a_off[x] > a_off[y + 1] - a_off[y]; });
-		CHOK(hipMemcpyAsync(d_ord, order.data(), (size_t)n_q * 4, hipMemcpyHostToDevice, s));
 		if (n_a > 0) CHOK(hipMemcpyAsync(d_a, anchors, (size_t)n_a * 8, hipMemcpyHostToDevice, s));
 		CHOK(hipMemcpyAsync(d_off, a_off, (size_t)(n_q + 1) * 8, hipMemcpyHostToDevice, s));
-		A.a = d_a, A.off = d_off, A.n_q = n_q, A.order = d_ord;
-		A.cp = { max_dist_x, max_dist_y, bw, max_skip, max_iter, min_cnt, min_sc, chn_coef_log, is_spliced, kmer, bbit };
-		A.f = d_f, A.p = d_p, A.t = d_t, A.vv = d_vv, A.zx = d_zx, A.zy = d_zy, A.b = d_b;
-		A.u_out = d_u, A.a_out = d_ao, A.n_u = d_nu;
-		hipLaunchKernelGGL(chain_kernel, dim3(n_q), dim3(64), 0, s, A);
-		CHOK(hipGetLastError());
+		if (!nsh_chain::run(s, n_q, d_off, nullptr, cnt.data(), d_a, n_a, prm, d_nu, d_u, d_ao)) goto done;
 		CHOK(hipMemcpyAsync(n_u, d_nu, (size_t)n_q * 4, hipMemcpyDeviceToHost, s));
 		if (n_a > 0) {
 			CHOK(hipMemcpyAsync(u, d_u, (size_t)n_a * 8, hipMemcpyDeviceToHost, s));
 			CHOK(hipMemcpyAsync(a_out, d_ao, (size_t)n_a * 8, hipMemcpyDeviceToHost, s));
 		}
 		CHOK(hipStreamSynchronize(s));
-		nsh_count_launch(1);
 		total = 0;
 		for (int32_t q = 0; q < n_q; ++q) total += n_u[q];
 	}
 done:
 	if (s) (void)hipStreamSynchronize(s);
 	{
-		void *bufs[] = { d_a, d_off, d_f, d_p, d_t, d_vv, d_zx, d_zy, d_b, d_u, d_ao, d_nu, d_ord };
+		void *bufs[] = { d_a, d_off, d_u, d_ao, d_nu };
 		for (void *bb : bufs) if (bb) (void)hipFree(bb);
 	}
 	if (s) (void)hipStreamDestroy(s);

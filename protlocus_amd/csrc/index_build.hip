@@ -426,26 +426,34 @@ extern "C" int nsh_index_copy(const nsh_index_t *ix, int64_t *ki, uint32_t *kb, 
 
 #define ANOK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { nsh_set_error("anchors: %s: %s", #x, hipGetErrorString(e_)); goto done; } } while (0)
 
-extern "C" int64_t nsh_index_anchors(const nsh_index_t *ix, int32_t n_q, const char *seq, const int64_t *q_off,
-		const int32_t *q_len, int32_t max_occ, int64_t *a_off, uint64_t *anchors, int64_t max_anchors)
+namespace {
+int check_queries(int32_t n_q, const int64_t *q_off, const int32_t *q_len, int64_t *l_seq, std::vector<int64_t> &sbase)
 {
-	if (!ix || n_q < 0 || (n_q > 0 && (!seq || !q_off || !q_len || !a_off)) || max_anchors < 0 ||
-			(max_anchors > 0 && !anchors)) {
-		nsh_set_error("anchors: bad arguments");
-		return -1;
-	}
-	int64_t l_seq = 0, l_seed = 0;
-	std::vector<int64_t> sbase(n_q + 1, 0);
+	*l_seq = 0;
+	sbase.assign(n_q + 1, 0);
 	for (int32_t q = 0; q < n_q; ++q) {
-		if (q_off[q] < 0 || q_len[q] < 0) { nsh_set_error("anchors: query %d out of range", q); return -1; }
-		l_seq = std::max(l_seq, q_off[q] + q_len[q]);
+		if (q_off[q] < 0 || q_len[q] < 0) { nsh_set_error("anchors: query %d out of range", q); return 0; }
+		*l_seq = std::max(*l_seq, q_off[q] + q_len[q]);
 		sbase[q + 1] = sbase[q] + q_len[q];
 	}
+	if (sbase[n_q] > INT_MAX) {
+		nsh_set_error("anchors: %lld query residues exceed one sort", (long long)sbase[n_q]);
+		return 0;
+	}
+	return 1;
+}
+
+// every query's anchors (map.c:583-597), sorted, left on the device: *pd_anch (aoff[n_q] of them,
+// query q's at aoff[q]) and *pd_aoff (aoff on the device), both for the caller to free. Waits for
+// stream s; 1 on success.
+int collect_anchors(const nsh_index_t *ix, int32_t n_q, const char *seq, const int64_t *q_off, const int32_t *q_len,
+		int32_t max_occ, hipStream_t s, std::vector<int64_t> &aoff, int64_t **pd_aoff, uint64_t **pd_anch)
+{
+	int64_t l_seq = 0, l_seed = 0;
+	std::vector<int64_t> sbase;
+	if (!check_queries(n_q, q_off, q_len, &l_seq, sbase)) return 0;
 	l_seed = sbase[n_q];
-	if (l_seed > INT_MAX) { nsh_set_error("anchors: %lld query residues exceed one sort", (long long)l_seed); return -1; }
-	if (n_q == 0) { if (a_off) a_off[0] = 0; return 0; }
-	int64_t total = -1;
-	hipStream_t s = nullptr;
+	int ok = 0;
 	uint8_t *d_seq = nullptr, *d_aa13 = nullptr;
 	int64_t *d_qoff = nullptr, *d_nanch = nullptr, *d_aoff = nullptr, *d_soff = nullptr, *d_send = nullptr,
 		*d_sbase = nullptr;
@@ -453,11 +461,11 @@ extern "C" int64_t nsh_index_anchors(const nsh_index_t *ix, int32_t n_q, const c
 	uint64_t *d_seed = nullptr, *d_cnt = nullptr, *d_cnt2 = nullptr, *d_anch = nullptr, *d_anch2 = nullptr;
 	void *d_tmp = nullptr;
 	size_t tb = 0, t2 = 0;
-	std::vector<int64_t> aoff(n_q + 1, 0), nan(n_q), soff(n_q), send(n_q);
+	std::vector<int64_t> nan(n_q), soff(n_q), send(n_q);
 	std::vector<int32_t> nseed(n_q);
 	AnArgs a = {};
+	aoff.assign(n_q + 1, 0);
 	{
-		ANOK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
 		ANOK(hipMalloc((void**)&d_seq, (size_t)l_seq + 8));
 		ANOK(hipMalloc((void**)&d_aa13, 256));
 		ANOK(hipMalloc((void**)&d_qoff, (size_t)n_q * 8));
@@ -522,18 +530,161 @@ extern "C" int64_t nsh_index_anchors(const nsh_index_t *ix, int32_t n_q, const c
 		}
 		ANOK(hipcub::DeviceSegmentedRadixSort::SortKeys(d_tmp, tb, d_anch, d_anch2, (int)n_a, n_q, d_aoff, d_aoff + 1,
 				0, 64, s));
+		ANOK(hipStreamSynchronize(s));
+		nsh_count_launch(6);
+		*pd_aoff = d_aoff, *pd_anch = d_anch2;
+		d_aoff = nullptr, d_anch2 = nullptr;
+		ok = 1;
+	}
+done:
+	(void)hipStreamSynchronize(s);
+	{
+		void *bufs[] = { d_seq, d_aa13, d_qoff, d_qlen, d_nseed, d_occ, d_nanch, d_aoff, d_soff, d_send, d_seed, d_cnt,
+			d_cnt2, d_anch, d_anch2, d_tmp, d_sbase };
+		for (void *b : bufs) if (b) (void)hipFree(b);
+	}
+	return ok;
+}
+} // namespace
+
+extern "C" int64_t nsh_index_anchors(const nsh_index_t *ix, int32_t n_q, const char *seq, const int64_t *q_off,
+		const int32_t *q_len, int32_t max_occ, int64_t *a_off, uint64_t *anchors, int64_t max_anchors)
+{
+	if (!ix || n_q < 0 || (n_q > 0 && (!seq || !q_off || !q_len || !a_off)) || max_anchors < 0 ||
+			(max_anchors > 0 && !anchors)) {
+		nsh_set_error("anchors: bad arguments");
+		return -1;
+	}
+	if (n_q == 0) { if (a_off) a_off[0] = 0; return 0; }
+	int64_t total = -1;
+	hipStream_t s = nullptr;
+	int64_t *d_aoff = nullptr;
+	uint64_t *d_anch = nullptr;
+	std::vector<int64_t> aoff;
+	{
+		ANOK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+		if (!collect_anchors(ix, n_q, seq, q_off, q_len, max_occ, s, aoff, &d_aoff, &d_anch)) goto done;
+		const int64_t n_a = aoff[n_q];
 		if (max_anchors > 0 && n_a > 0)
-			ANOK(hipMemcpyAsync(anchors, d_anch2, (size_t)std::min(n_a, max_anchors) * 8, hipMemcpyDeviceToHost, s));
+			ANOK(hipMemcpyAsync(anchors, d_anch, (size_t)std::min(n_a, max_anchors) * 8, hipMemcpyDeviceToHost, s));
 		ANOK(hipStreamSynchronize(s));
 		std::copy(aoff.begin(), aoff.end(), a_off);
-		nsh_count_launch(6);
 		total = n_a;
 	}
 done:
 	if (s) (void)hipStreamSynchronize(s);
+	if (d_aoff) (void)hipFree(d_aoff);
+	if (d_anch) (void)hipFree(d_anch);
+	if (s) (void)hipStreamDestroy(s);
+	return total;
+}
+
+
+namespace {
+// per query: the anchors held by the chains u[off[q], off[q] + nu[q]); end[q] = off[q] + that
+__global__ void kept_kernel(int32_t n_q, const int64_t *off, const int32_t *nu, const uint64_t *u, int64_t *kept,
+		int64_t *end)
+{
+	const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= n_q) return;
+	int64_t m = 0;
+	for (int32_t i = 0; i < nu[q]; ++i) m += (uint32_t)u[off[q] + i];
+	kept[q] = m;
+	end[q] = off[q] + m;
+}
+
+// each query's chains and chain anchors from its region into the packed outputs
+__global__ void pack_chains_kernel(const int64_t *off, const int32_t *nu, const int64_t *kept, const int64_t *uoff,
+		const int64_t *coff, const uint64_t *u, const uint64_t *ao, uint64_t *uc, uint64_t *ac)
+{
+	const int32_t q = blockIdx.x;
+	for (int64_t i = threadIdx.x; i < nu[q]; i += blockDim.x) uc[uoff[q] + i] = u[off[q] + i];
+	for (int64_t i = threadIdx.x; i < kept[q]; i += blockDim.x) ac[coff[q] + i] = ao[off[q] + i];
+}
+} // namespace
+
+extern "C" int64_t nsh_index_chain(const nsh_index_t *ix, int32_t n_q, const char *seq, const int64_t *q_off,
+		const int32_t *q_len, int32_t max_occ, const nsh_chain_opt_t *opt, int64_t *u_off, uint64_t *u,
+		int64_t max_u, int64_t *c_off, uint64_t *anchors, int64_t max_anchors)
+{
+	if (!ix || !opt || n_q < 0 || (n_q > 0 && (!seq || !q_off || !q_len)) || !u_off || !c_off || max_u < 0 ||
+			max_anchors < 0 || (max_u > 0 && !u) || (max_anchors > 0 && !anchors)) {
+		nsh_set_error("index chain: bad arguments");
+		return -1;
+	}
+	u_off[0] = c_off[0] = 0;
+	if (n_q == 0) return 0;
+	int64_t total = -1;
+	hipStream_t s = nullptr;
+	int64_t *d_aoff = nullptr, *d_kept = nullptr, *d_end = nullptr, *d_uoff = nullptr, *d_coff = nullptr;
+	uint64_t *d_anch = nullptr, *d_u = nullptr, *d_ao = nullptr, *d_uc = nullptr, *d_ac = nullptr;
+	int32_t *d_nu = nullptr;
+	void *d_tmp = nullptr;
+	size_t tb = 0;
+	std::vector<int64_t> aoff, cnt(n_q), kept(n_q);
+	std::vector<int32_t> nu(n_q);
+	const int32_t w = 1 << ix->bbit;
+	const nsh_chain::Params pre = { w, w, w, opt->max_skip, opt->max_iter, 2, 0, opt->chn_coef_log, opt->is_spliced,
+		ix->kmer, ix->bbit };
+	const nsh_chain::Params main = { opt->max_dist_x, opt->max_dist_y, opt->bw, opt->max_skip, opt->max_iter,
+		opt->min_cnt, opt->min_sc, opt->chn_coef_log, opt->is_spliced, ix->kmer, ix->bbit };
 	{
-		void *bufs[] = { d_seq, d_aa13, d_qoff, d_qlen, d_nseed, d_occ, d_nanch, d_aoff, d_soff, d_send, d_seed, d_cnt,
-			d_cnt2, d_anch, d_anch2, d_tmp, d_sbase };
+		ANOK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+		if (!collect_anchors(ix, n_q, seq, q_off, q_len, max_occ, s, aoff, &d_aoff, &d_anch)) goto done;
+		const int64_t n_a = aoff[n_q];
+		const size_t na = (size_t)(n_a > 0? n_a : 1);
+		for (int32_t q = 0; q < n_q; ++q) cnt[q] = aoff[q + 1] - aoff[q];
+		ANOK(hipMalloc((void**)&d_nu, (size_t)n_q * 4));
+		ANOK(hipMalloc((void**)&d_kept, (size_t)n_q * 8));
+		ANOK(hipMalloc((void**)&d_end, (size_t)n_q * 8));
+		ANOK(hipMalloc((void**)&d_u, na * 8));
+		ANOK(hipMalloc((void**)&d_ao, na * 8));
+		const int64_t *d_cnt = nullptr, *h_cnt = cnt.data();
+		if (opt->pre_chain && opt->is_spliced) {   // map.c:611-623
+			if (!nsh_chain::run(s, n_q, d_aoff, nullptr, cnt.data(), d_anch, n_a, pre, d_nu, d_u, d_ao)) goto done;
+			hipLaunchKernelGGL(kept_kernel, dim3((n_q + 255) / 256), dim3(256), 0, s, n_q, d_aoff, d_nu, d_u, d_kept,
+					d_end);
+			ANOK(hipGetLastError());
+			ANOK(hipMemcpyAsync(kept.data(), d_kept, (size_t)n_q * 8, hipMemcpyDeviceToHost, s));
+			ANOK(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, d_ao, d_anch, (int)n_a, n_q, d_aoff, d_end,
+					0, 64, s));
+			ANOK(hipMalloc(&d_tmp, tb + 8));
+			ANOK(hipcub::DeviceSegmentedRadixSort::SortKeys(d_tmp, tb, d_ao, d_anch, (int)n_a, n_q, d_aoff, d_end,
+					0, 64, s));
+			ANOK(hipStreamSynchronize(s));
+			d_cnt = d_kept, h_cnt = kept.data();
+			nsh_count_launch(3);
+		}
+		if (!nsh_chain::run(s, n_q, d_aoff, d_cnt, h_cnt, d_anch, n_a, main, d_nu, d_u, d_ao)) goto done;
+		hipLaunchKernelGGL(kept_kernel, dim3((n_q + 255) / 256), dim3(256), 0, s, n_q, d_aoff, d_nu, d_u, d_kept,
+				d_end);
+		ANOK(hipGetLastError());
+		ANOK(hipMemcpyAsync(nu.data(), d_nu, (size_t)n_q * 4, hipMemcpyDeviceToHost, s));
+		ANOK(hipMemcpyAsync(kept.data(), d_kept, (size_t)n_q * 8, hipMemcpyDeviceToHost, s));
+		ANOK(hipStreamSynchronize(s));
+		for (int32_t q = 0; q < n_q; ++q) u_off[q + 1] = u_off[q] + nu[q], c_off[q + 1] = c_off[q] + kept[q];
+		const int64_t n_uc = u_off[n_q], n_ac = c_off[n_q];
+		ANOK(hipMalloc((void**)&d_uoff, (size_t)(n_q + 1) * 8));
+		ANOK(hipMalloc((void**)&d_coff, (size_t)(n_q + 1) * 8));
+		ANOK(hipMalloc((void**)&d_uc, (size_t)n_uc * 8 + 8));
+		ANOK(hipMalloc((void**)&d_ac, (size_t)n_ac * 8 + 8));
+		ANOK(hipMemcpyAsync(d_uoff, u_off, (size_t)(n_q + 1) * 8, hipMemcpyHostToDevice, s));
+		ANOK(hipMemcpyAsync(d_coff, c_off, (size_t)(n_q + 1) * 8, hipMemcpyHostToDevice, s));
+		hipLaunchKernelGGL(pack_chains_kernel, dim3(n_q), dim3(256), 0, s, d_aoff, d_nu, d_kept, d_uoff, d_coff, d_u,
+				d_ao, d_uc, d_ac);
+		ANOK(hipGetLastError());
+		if (max_u > 0 && n_uc > 0)
+			ANOK(hipMemcpyAsync(u, d_uc, (size_t)std::min(n_uc, max_u) * 8, hipMemcpyDeviceToHost, s));
+		if (max_anchors > 0 && n_ac > 0)
+			ANOK(hipMemcpyAsync(anchors, d_ac, (size_t)std::min(n_ac, max_anchors) * 8, hipMemcpyDeviceToHost, s));
+		ANOK(hipStreamSynchronize(s));
+		nsh_count_launch(3);
+		total = n_uc;
+	}
+done:
+	if (s) (void)hipStreamSynchronize(s);
+	{
+		void *bufs[] = { d_aoff, d_kept, d_end, d_uoff, d_coff, d_anch, d_u, d_ao, d_uc, d_ac, d_nu, d_tmp };
 		for (void *b : bufs) if (b) (void)hipFree(b);
 	}
 	if (s) (void)hipStreamDestroy(s);

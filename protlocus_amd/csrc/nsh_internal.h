@@ -73,4 +73,18 @@ int run(hipStream_t s, int32_t n_query, const int32_t *d_off, const int32_t *d_q
 		int32_t top_k, int32_t min_per_vid, int32_t adaptive, uint8_t *d_keep, int32_t *d_k, nsh_gapless_calib_t *cal);
 }
 
+namespace nsh_chain {
+struct Params {                    // mp_chain's arguments (chain.c:160)
+	int32_t max_dist_x, max_dist_y, bw, max_skip, max_iter, min_cnt, min_sc;
+	float coef;
+	int32_t is_spliced, kmer, bbit;
+};
+// mp_chain of every query's sorted anchors d_a[d_off[q], d_off[q] + n_q) on stream s, where the
+// count is h_cnt[q] on the host and d_cnt[q] on the device (both null: the offsets' differences,
+// d_off holding n_q + 1 entries). Per query, at the same offsets, the chains (score << 32 | count)
+// go to d_u and their anchors to d_ao; the chain counts to d_nu. Waits for s; 1 on success.
+int run(hipStream_t s, int32_t n_q, const int64_t *d_off, const int64_t *d_cnt, const int64_t *h_cnt,
+		const uint64_t *d_a, int64_t n_a, const Params &p, int32_t *d_nu, uint64_t *d_u, uint64_t *d_ao);
+}
+
 #endif

@@ -86,3 +86,42 @@ def test_chain_rejects_bad_arguments(product):
     assert lib().nsh_chain_batch(2, off.ctypes.data, a.ctypes.data, 1, 1, 1, 1, 1, 1, 0, 0.75, 1, 0, 8,
                                  n_u.ctypes.data, out.ctypes.data, out.ctypes.data) < 0
     assert lib().nsh_chain_batch(0, None, None, 1, 1, 1, 1, 1, 1, 0, 0.75, 1, 3, 8, None, None, None) == 0
+
+
+def _coarse_ref(anchors, opt, kmer, bbit):
+    """map.c:611-633 on one query's sorted anchors, through the restatement"""
+    a = [int(x) for x in anchors]
+    if opt.pre_chain and opt.is_spliced:
+        w = 1 << bbit
+        _, a = chain_ref.chain(a, w, w, w, opt.max_skip, opt.max_iter, 2, 0, opt.chn_coef_log, opt.is_spliced, kmer,
+                               bbit)
+        a = sorted(a)
+    return chain_ref.chain(a, opt.max_dist_x, opt.max_dist_y, opt.bw, opt.max_skip, opt.max_iter, opt.min_cnt,
+                           opt.min_sc, opt.chn_coef_log, opt.is_spliced, kmer, bbit)
+
+
+def test_index_chain_golden(product):
+    """nsh_index_chain (anchors, pre-chain, sort, chain on the device) against the reference's
+    anchors of the golden records chained by the restatement, for three option sets"""
+    import base64
+    import zlib
+    from protlocus_amd import capi
+    with gzip.open(os.path.join(HERE, "golden", "anchor_cases.json.gz"), "rt") as f:
+        cases = json.load(f)
+    opts = [capi.ChainOpt(), capi.ChainOpt(pre_chain=0, min_cnt=2, max_dist_x=5000, bw=500),
+            capi.ChainOpt(is_spliced=0, max_dist_x=20000, max_dist_y=2000, bw=1000, min_sc=5)]
+    for k, c in enumerate(cases):
+        g = np.frombuffer(zlib.decompress(base64.b64decode(c["genome"])), dtype=np.uint8)
+        ix = capi.Index(g, c["ctg_off"], c["ctg_len"], c["min_aa_len"], c["kmer"], c["mod_bit"], c["bbit"])
+        try:
+            qs = [q.encode() for q in c["queries"]]
+            for oi, opt in enumerate(opts):
+                if k == 3 and oi:   # the repeat-rich genome: its restatement side takes seconds per set
+                    continue
+                print("case", k, "options", oi, flush=True)
+                got = ix.chain(qs, c["max_occ"], opt)
+                for j, (q, (u, a)) in enumerate(zip(c["anchors"], got)):
+                    want = _coarse_ref(unpack_keys(q), opt, c["kmer"], c["bbit"])
+                    assert (u.tolist(), a.tolist()) == want, (k, oi, j)
+        finally:
+            ix.close()
