@@ -50,8 +50,9 @@ struct ChArgs {
 };
 
 struct ChLds {                         // one wavefront's shared state
-	int32_t b[8][kRsSize], e[8][kRsSize];   // radix sort frames: bucket cursors and ends
-	int32_t fk[8], fs[8], frec[8];          // per frame: next bucket, the children's shift, recursion
+	int32_t e[8][kRsSize];                  // radix sort frames: bucket ends (a bucket starts where the last ends)
+	int32_t cur[kRsSize];                   // the distribution pass's bucket cursors
+	int32_t fb[8], fk[8], fs[8], frec[8];   // per frame: range start, next bucket, the children's shift, recursion
 	uint64_t cmask, orv;
 	uint64_t sk[64];
 	int32_t nu, nv;
@@ -196,7 +197,7 @@ DEV void distribute(uint64_t *x, uint64_t *y, int32_t beg, int32_t end, int32_t 
 	}
 	const int s2 = (63 - __builtin_clzll(df)) & ~(kRsBits - 1);
 	if (s2 < s) s = s2;
-	int32_t *B = L.b[d], *E = L.e[d];
+	int32_t *B = L.cur, *E = L.e[d];
 	for (int k = lane; k < kRsSize; k += 64) E[k] = 0;
 	wave_sync_lds();
 	for (int32_t i = beg + lane; i < end; i += 64) atomicAdd(&E[(x[i] >> s) & (kRsSize - 1)], 1);
@@ -240,8 +241,7 @@ DEV void distribute(uint64_t *x, uint64_t *y, int32_t beg, int32_t end, int32_t 
 		__syncthreads();
 	}
 	__syncthreads();
-	for (int k = lane; k < kRsSize; k += 64) B[k] = k? E[k - 1] : beg;
-	if (lane == 0) L.fs[d] = s > kRsBits? s - kRsBits : 0, L.frec[d] = s != 0, L.fk[d] = 0;
+	if (lane == 0) L.fb[d] = beg, L.fs[d] = s > kRsBits? s - kRsBits : 0, L.frec[d] = s != 0, L.fk[d] = 0;
 	wave_sync_lds();
 }
 
@@ -257,7 +257,7 @@ DEV void radix_sort_pairs(uint64_t *x, uint64_t *y, int32_t n, ChLds &L)
 		const int k0 = L.fk[depth];
 		if (!L.frec[depth] || k0 >= kRsSize) { --depth; continue; }
 		const int base = k0 & ~63, kk = base + lane;
-		const int32_t bb = L.b[depth][kk], sz = L.e[depth][kk] - bb;
+		const int32_t bb = kk? L.e[depth][kk - 1] : L.fb[depth], sz = L.e[depth][kk] - bb;
 		const uint64_t m = __ballot(sz > 1) & (~0ull << (k0 & 63));
 		const int lb = m? __builtin_ctzll(m) : 64;
 		const int32_t beg = m? __builtin_amdgcn_readlane(bb, lb) : 0, size = m? __builtin_amdgcn_readlane(sz, lb) : 0;
