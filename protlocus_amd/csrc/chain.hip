@@ -32,6 +32,10 @@
 namespace {
 
 constexpr int kRsMin = 64, kRsBits = 8, kRsSize = 1 << kRsBits;
+// the sorted keys (chain scores, which start at kmer >= 1 and only grow, and target blocks) are
+// below 2^32, so with identity levels skipped the recursion holds at most 4 levels (shifts 24, 16,
+// 8 and 0)
+constexpr int kFrames = 4;
 constexpr int kBlockBonus = 2;   // MP_BLOCK_BONUS (protlocus.h:47)
 
 using ChainParams = nsh_chain::Params;
@@ -50,11 +54,14 @@ struct ChArgs {
 };
 
 struct ChLds {                         // one wavefront's shared state
-	int32_t e[8][kRsSize];                  // radix sort frames: bucket ends (a bucket starts where the last ends)
-	int32_t cur[kRsSize];                   // the distribution pass's bucket cursors
-	int32_t fb[8], fk[8], fs[8], frec[8];   // per frame: range start, next bucket, the children's shift, recursion
+	int32_t e[kFrames][kRsSize];            // radix sort frames: bucket ends (a bucket starts where the last ends)
+	union {
+		int32_t cur[kRsSize];               // the distribution pass's bucket cursors
+		uint64_t sk[64];                    // the keys of a bucket of at most 64
+	};
+	int32_t fb[kFrames], fk[kFrames], fs[kFrames], frec[kFrames];   // per frame: range start, next bucket,
+	                                                                // the children's shift, recursion
 	uint64_t cmask, orv;
-	uint64_t sk[64];
 	int32_t nu, nv;
 };
 
@@ -179,7 +186,7 @@ DEV void small_sort(uint64_t *x, uint64_t *y, int32_t beg, int32_t end, ChLds &L
 // levels where every key has the same digit (an identity permutation, after which the reference
 // recurses into the same range) are skipped; then digit counts, bucket bounds, and the in-place
 // cycle-leader permutation on lane 0, which fixes the order of equal keys
-DEV void distribute(uint64_t *x, uint64_t *y, int32_t beg, int32_t end, int32_t s, int d, ChLds &L)
+DEV bool distribute(uint64_t *x, uint64_t *y, int32_t beg, int32_t end, int32_t s, int d, ChLds &L)
 {
 	const int lane = threadIdx.x;
 	const uint64_t x0 = x[beg];
@@ -193,10 +200,11 @@ DEV void distribute(uint64_t *x, uint64_t *y, int32_t beg, int32_t end, int32_t 
 	if (df == 0) {   // every key equal: the identity at every level
 		if (lane == 0) L.frec[d] = 0, L.fk[d] = kRsSize;
 		wave_sync_lds();
-		return;
+		return true;
 	}
 	const int s2 = (63 - __builtin_clzll(df)) & ~(kRsBits - 1);
-	if (s2 < s) s = s2;
+	if (s2 > s) return false;   // keys beyond the frames' reach
+	s = s2;
 	int32_t *B = L.cur, *E = L.e[d];
 	for (int k = lane; k < kRsSize; k += 64) E[k] = 0;
 	wave_sync_lds();
@@ -243,15 +251,17 @@ DEV void distribute(uint64_t *x, uint64_t *y, int32_t beg, int32_t end, int32_t 
 	__syncthreads();
 	if (lane == 0) L.fb[d] = beg, L.fs[d] = s > kRsBits? s - kRsBits : 0, L.frec[d] = s != 0, L.fk[d] = 0;
 	wave_sync_lds();
+	return true;
 }
 
 // radix_sort_mp128x (ksort.h:158-162) of n (x, y) pairs by x; the recursion runs on LDS frames,
-// visiting each frame's buckets of two or more keys in order
-DEV void radix_sort_pairs(uint64_t *x, uint64_t *y, int32_t n, ChLds &L)
+// visiting each frame's buckets of two or more keys in order. False (nothing sorted) when a key
+// reaches 2^32.
+DEV bool radix_sort_pairs(uint64_t *x, uint64_t *y, int32_t n, ChLds &L)
 {
 	const int lane = threadIdx.x;
-	if (n <= kRsMin) { small_sort(x, y, 0, n, L); return; }
-	distribute(x, y, 0, n, 7 * kRsBits, 0, L);
+	if (n <= kRsMin) { small_sort(x, y, 0, n, L); return true; }
+	if (!distribute(x, y, 0, n, (kFrames - 1) * kRsBits, 0, L)) return false;
 	int depth = 0;
 	while (depth >= 0) {
 		const int k0 = L.fk[depth];
@@ -270,6 +280,7 @@ DEV void radix_sort_pairs(uint64_t *x, uint64_t *y, int32_t n, ChLds &L)
 			++depth;
 		} else small_sort(x, y, beg, beg + size, L);
 	}
+	return true;
 }
 
 __global__ void __launch_bounds__(64) chain_kernel(ChArgs A)
@@ -378,7 +389,7 @@ __global__ void __launch_bounds__(64) chain_kernel(ChArgs A)
 	}
 	__syncthreads();
 	if (n_z == 0) { if (lane == 0) A.n_u[q] = 0; return; }
-	radix_sort_pairs(zx, zy, n_z, L);
+	if (!radix_sort_pairs(zx, zy, n_z, L)) { if (lane == 0) A.n_u[q] = -1; return; }   // a score beyond int32
 	// anchors no other anchor chains to (hs = 0): started from one with no predecessor either, the
 	// backtrack only marks it (its chain of one is dropped when min_cnt > 1 or its score is not
 	// positive), and no later walk can reach it, so such candidates are passed over
@@ -438,7 +449,7 @@ __global__ void __launch_bounds__(64) chain_kernel(ChArgs A)
 		k0 += __builtin_amdgcn_readlane(incl, 63);
 	}
 	__syncthreads();
-	radix_sort_pairs(zx, zy, n_u, L);
+	radix_sort_pairs(zx, zy, n_u, L);   // target blocks: below 2^32
 	for (int32_t i0 = 0, k0 = 0; i0 < n_u; i0 += 64) {
 		const int32_t i = i0 + lane;
 		const int32_t j = i < n_u? (int32_t)(uint32_t)zy[i] : 0;
@@ -492,6 +503,13 @@ int nsh_chain::run(hipStream_t s, int32_t n_q, const int64_t *d_off, const int64
 		CHOK(hipStreamSynchronize(s));
 		nsh_count_launch(1);
 		ok = 1;
+		{
+			int32_t bad = 0;
+			std::vector<int32_t> nu(n_q);
+			CHOK(hipMemcpy(nu.data(), d_nu, (size_t)n_q * 4, hipMemcpyDeviceToHost));
+			for (int32_t q = 0; q < n_q && !bad; ++q) bad = nu[q] < 0;
+			if (bad) { nsh_set_error("chain: a chain score overflows int32"); ok = 0; }
+		}
 	}
 done:
 	(void)hipStreamSynchronize(s);
