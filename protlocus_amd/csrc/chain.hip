@@ -467,6 +467,26 @@ __global__ void __launch_bounds__(64) chain_kernel(ChArgs A)
 	for (int32_t i = lane; i < n_u; i += 64) uo[i] = zx[i];
 }
 
+// per query: the anchors held by its chains u[off[q], off[q] + nu[q])
+__global__ void chain_kept_kernel(int32_t n_q, const int64_t *off, const int32_t *nu, const uint64_t *u,
+		int64_t *kept)
+{
+	const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= n_q) return;
+	int64_t m = 0;
+	for (int32_t i = 0; i < nu[q]; ++i) m += (uint32_t)u[off[q] + i];
+	kept[q] = m;
+}
+
+// each query's chains and chain anchors, packed: the chains first (uoff), then the anchors (coff)
+__global__ void chain_pack_kernel(const int64_t *off, const int32_t *nu, const int64_t *kept, const int64_t *uoff,
+		const int64_t *coff, const uint64_t *u, const uint64_t *ao, uint64_t *pk)
+{
+	const int32_t q = blockIdx.x;
+	for (int64_t i = threadIdx.x; i < nu[q]; i += blockDim.x) pk[uoff[q] + i] = u[off[q] + i];
+	for (int64_t i = threadIdx.x; i < kept[q]; i += blockDim.x) pk[coff[q] + i] = ao[off[q] + i];
+}
+
 } // namespace
 
 #define CHOK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { nsh_set_error("chain: %s: %s", #x, hipGetErrorString(e_)); goto done; } } while (0)
@@ -543,9 +563,11 @@ extern "C" int64_t nsh_chain_batch(int32_t n_q, const int64_t *a_off, const uint
 	if (!nsh_ensure_device()) return -1;
 	int64_t total = -1;
 	hipStream_t s = nullptr;
-	uint64_t *d_a = nullptr, *d_u = nullptr, *d_ao = nullptr;
-	int64_t *d_off = nullptr;
+	uint64_t *d_a = nullptr, *d_u = nullptr, *d_ao = nullptr, *d_pk = nullptr;
+	int64_t *d_off = nullptr, *d_kept = nullptr, *d_poff = nullptr;
 	int32_t *d_nu = nullptr;
+	std::vector<int64_t> kept(n_q), poff(2 * n_q);
+	std::vector<uint64_t> pk;
 	const size_t na = (size_t)(n_a > 0? n_a : 1);
 	const nsh_chain::Params prm = { max_dist_x, max_dist_y, bw, max_skip, max_iter, min_cnt, min_sc, chn_coef_log,
 		is_spliced, kmer, bbit };
@@ -559,19 +581,39 @@ extern "C" int64_t nsh_chain_batch(int32_t n_q, const int64_t *a_off, const uint
 		if (n_a > 0) CHOK(hipMemcpyAsync(d_a, anchors, (size_t)n_a * 8, hipMemcpyHostToDevice, s));
 		CHOK(hipMemcpyAsync(d_off, a_off, (size_t)(n_q + 1) * 8, hipMemcpyHostToDevice, s));
 		if (!nsh_chain::run(s, n_q, d_off, nullptr, cnt.data(), d_a, n_a, prm, d_nu, d_u, d_ao)) goto done;
+		// only the chains and their anchors come back, packed, then go to each query's offset
+		CHOK(hipMalloc((void**)&d_kept, (size_t)n_q * 8));
+		hipLaunchKernelGGL(chain_kept_kernel, dim3((n_q + 255) / 256), dim3(256), 0, s, n_q, d_off, d_nu, d_u, d_kept);
+		CHOK(hipGetLastError());
 		CHOK(hipMemcpyAsync(n_u, d_nu, (size_t)n_q * 4, hipMemcpyDeviceToHost, s));
-		if (n_a > 0) {
-			CHOK(hipMemcpyAsync(u, d_u, (size_t)n_a * 8, hipMemcpyDeviceToHost, s));
-			CHOK(hipMemcpyAsync(a_out, d_ao, (size_t)n_a * 8, hipMemcpyDeviceToHost, s));
-		}
+		CHOK(hipMemcpyAsync(kept.data(), d_kept, (size_t)n_q * 8, hipMemcpyDeviceToHost, s));
 		CHOK(hipStreamSynchronize(s));
+		int64_t np = 0;
+		for (int32_t q = 0; q < n_q; ++q) poff[q] = np, np += n_u[q];
+		for (int32_t q = 0; q < n_q; ++q) poff[n_q + q] = np, np += kept[q];
+		if (np > 0) {
+			CHOK(hipMalloc((void**)&d_poff, (size_t)n_q * 16));
+			CHOK(hipMalloc((void**)&d_pk, (size_t)np * 8));
+			CHOK(hipMemcpyAsync(d_poff, poff.data(), (size_t)n_q * 16, hipMemcpyHostToDevice, s));
+			hipLaunchKernelGGL(chain_pack_kernel, dim3(n_q), dim3(256), 0, s, d_off, d_nu, d_kept, d_poff,
+					d_poff + n_q, d_u, d_ao, d_pk);
+			CHOK(hipGetLastError());
+			pk.resize(np);
+			CHOK(hipMemcpyAsync(pk.data(), d_pk, (size_t)np * 8, hipMemcpyDeviceToHost, s));
+			CHOK(hipStreamSynchronize(s));
+			nsh_count_launch(2);
+			for (int32_t q = 0; q < n_q; ++q) {
+				std::copy(pk.begin() + poff[q], pk.begin() + poff[q] + n_u[q], u + a_off[q]);
+				std::copy(pk.begin() + poff[n_q + q], pk.begin() + poff[n_q + q] + kept[q], a_out + a_off[q]);
+			}
+		}
 		total = 0;
 		for (int32_t q = 0; q < n_q; ++q) total += n_u[q];
 	}
 done:
 	if (s) (void)hipStreamSynchronize(s);
 	{
-		void *bufs[] = { d_a, d_off, d_u, d_ao, d_nu };
+		void *bufs[] = { d_a, d_off, d_u, d_ao, d_nu, d_kept, d_poff, d_pk };
 		for (void *bb : bufs) if (bb) (void)hipFree(bb);
 	}
 	if (s) (void)hipStreamDestroy(s);
