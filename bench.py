@@ -543,6 +543,8 @@ def main():
     ap.add_argument("--pairs", type=int, default=65536)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick each column group's persistent wave count on this batch first (untimed)")
     ap.add_argument("--no-extras", action="store_true", help="headline only (no prescreen / hmm sections)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--only-profile-search", action="store_true",
@@ -600,6 +602,12 @@ def main():
     t_up = time.perf_counter()
     plan = prod.ext_plan(ns, ns_off, nl, as_, as_off, al, prod.opt(**kw))   # host buffers -> HBM (PCIe), once
     plan_build_s = time.perf_counter() - t_up
+    waves, tune_s = None, 0.0
+    if args.autotune:   # persistent waves per column-count group, picked on this batch (untimed)
+        t0 = time.perf_counter()
+        waves = plan.autotune(reps=3)
+        tune_s = time.perf_counter() - t0
+        log("[bench] autotuned waves per CU %s in %.1f s" % (waves, tune_s))
     for _ in range(args.warmup):
         plan.run()
     log("[bench] warmup done")
@@ -630,7 +638,9 @@ def main():
         "dtype": "int16", "data": "synthetic",
         "config": {"workload": "batched extension DP, %d pairs, protein 50-1000 aa log-uniform, "
                                "right extension over the full target" % args.pairs,
-                   "pairs": args.pairs, "seed": args.seed, "cell_definition": "(rows evaluated) x al"},
+                   "pairs": args.pairs, "seed": args.seed, "cell_definition": "(rows evaluated) x al",
+                   "waves_per_cu": waves if waves else "compiled defaults",
+                   "autotune_s": round(tune_s, 2)},
         "proteins_per_s": round(args.pairs / (ms * 1e-3), 1),
         "window_gcups": round(win / (ms * 1e-3) / 1e9, 2),
         "dp_kernel_ms": round(dp_avg, 3),

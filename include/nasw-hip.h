@@ -243,6 +243,12 @@ const float *nsh_ext_plan_features(nsh_ext_plan_t *p, float *host_out);
 int nsh_ext_plan_results(nsh_ext_plan_t *p, nsh_ext_result_t *out); /* copy results (input order) */
 double nsh_ext_plan_cells(const nsh_ext_plan_t *p, int32_t kind);   /* 0: sum nl*al, 1: sum (rows done)*al after a run */
 float nsh_ext_plan_last_ms(nsh_ext_plan_t *p, int32_t which);  /* 0: whole run, 1: DP kernels only (HIP events) */
+/* Autotuner: picks each column-count group's persistent wave count (waves per CU, one of the
+ * compiled default's 1/2, 3/4, 1, 3/2 and 2 multiples) by timing full runs of this plan, one
+ * group at a time (reps runs per candidate, the fastest of them; another setting replaces the
+ * default only when it is 2% faster). Results do not depend on it.
+ * waves_per_cu (may be NULL) receives the chosen values, one per group (3). 1 ok, 0 error. */
+int nsh_ext_plan_autotune(nsh_ext_plan_t *p, int32_t reps, int32_t *waves_per_cu);
 void nsh_ext_plan_destroy(nsh_ext_plan_t *p);
 
 /* Profile-HMM engine behind mp_hmm_dp_score_gpu*: a profile resident in HBM (emissions M x 20 in

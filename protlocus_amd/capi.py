@@ -83,6 +83,8 @@ def lib():
         L.nsh_ext_plan_cells.restype = C.c_double
         L.nsh_ext_plan_last_ms.argtypes = [C.c_void_p, C.c_int32]
         L.nsh_ext_plan_last_ms.restype = C.c_float
+        L.nsh_ext_plan_autotune.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+        L.nsh_ext_plan_autotune.restype = C.c_int
         L.nsh_mlp_param_count.restype = C.c_int32
         L.nsh_mlp_create.argtypes = [C.c_void_p]
         L.nsh_mlp_create.restype = C.c_void_p
@@ -225,6 +227,13 @@ class Plan:
 
     def ms(self, which=1):
         return lib().nsh_ext_plan_last_ms(self.p, which)
+
+    def autotune(self, reps=2):
+        """the persistent wave count per column-count group, picked by timing this plan's runs"""
+        w = (C.c_int32 * 3)()
+        if not lib().nsh_ext_plan_autotune(self.p, reps, w):
+            raise RuntimeError("nsh_ext_plan_autotune: " + error())
+        return list(w)
 
     def features_device(self):
         ptr = lib().nsh_ext_plan_features(self.p, None)

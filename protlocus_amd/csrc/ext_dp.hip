@@ -201,6 +201,7 @@ struct nsh_ext_plan_s {
 	PrepConst *d_pc = nullptr;
 	ExtArgs args[kGroups] = {};
 	int32_t grid[kGroups] = {};
+	int32_t wpc[kGroups] = {};   // persistent waves per CU of each group (defaults, or the autotuner's)
 	double cells_window = 0;     // sum nl*al
 	std::vector<int32_t> al;     // per input index
 	std::vector<int32_t> out_h;  // 4 per problem
@@ -341,7 +342,8 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 			a.go = opt->go, a.ge = opt->ge, a.fs = opt->fs, a.xdrop = opt->xdrop, a.eb = opt->end_bonus;
 			a.counter = p->d_counter + g, a.out = p->d_out;
 			// persistent waves: about as many as the CUs keep resident
-			p->grid[g] = (int32_t)std::min<int64_t>(a.n, (int64_t)ncu * (opt->pssm? kGroupWavesPerCUPssm[g] : kGroupWavesPerCU[g]));
+			p->wpc[g] = opt->pssm? kGroupWavesPerCUPssm[g] : kGroupWavesPerCU[g];
+			p->grid[g] = (int32_t)std::min<int64_t>(a.n, (int64_t)ncu * p->wpc[g]);
 		}
 	}
 	return p;
@@ -432,7 +434,7 @@ extern "C" int nsh_ext_plan_run_subset(nsh_ext_plan_t *p, const uint8_t *keep)
 			if (keep[p->prob_h[k].out]) p->sub_h.push_back(p->prob_h[k]);
 		args[g] = p->args[g];
 		args[g].prob = p->d_prob_sub + first, args[g].n = (int32_t)p->sub_h.size() - first;
-		grid[g] = std::min(args[g].n, p->ncu * (args[g].pssm? kGroupWavesPerCUPssm[g] : kGroupWavesPerCU[g]));
+		grid[g] = std::min(args[g].n, p->ncu * p->wpc[g]);
 	}
 	if (!p->sub_h.empty()) {
 		HIPOK(hipMemcpyAsync(p->d_prob_sub, p->sub_h.data(), p->sub_h.size() * sizeof(ExtProb), hipMemcpyHostToDevice, p->stream));
@@ -503,6 +505,48 @@ extern "C" float nsh_ext_plan_last_ms(nsh_ext_plan_t *p, int32_t which)
 	if (!p || !p->ran) return -1.f;
 	if (hipEventElapsedTime(&ms, which == 1? p->ev1 : p->ev0, p->ev2) != hipSuccess) return -1.f;
 	return ms;
+}
+
+extern "C" int nsh_ext_plan_autotune(nsh_ext_plan_t *p, int32_t reps, int32_t *waves_per_cu)
+{
+	if (!p || reps < 1) { nsh_set_error("nsh_ext_plan_autotune: bad arguments"); return 0; }
+	int32_t *wpc = p->wpc;
+	auto time_run = [&](float *best) -> int {
+		*best = 1e30f;
+		for (int r = 0; r < reps; ++r) {
+			if (!plan_launch(p, p->args, p->grid, false, nullptr, p->n)) return 0;
+			if (hipEventSynchronize(p->ev2) != hipSuccess) { nsh_set_error("nsh_ext_plan_autotune: run failed"); return 0; }
+			float ms = 0;
+			if (hipEventElapsedTime(&ms, p->ev1, p->ev2) != hipSuccess) return 0;
+			*best = std::min(*best, ms);
+		}
+		return 1;
+	};
+	{   // warm-up runs first: the clocks settle before anything is compared
+		float ms;
+		if (!time_run(&ms) || !time_run(&ms)) return 0;
+	}
+	for (int g = 0; g < kGroups; ++g) {
+		const int32_t n = p->args[g].n;
+		if (n <= 0) continue;
+		const int32_t d = wpc[g];
+		const int32_t cand[5] = { d, std::max(1, d / 2), std::max(1, 3 * d / 4), 3 * d / 2, 2 * d };
+		int32_t best_w = d;
+		float best_ms = 1e30f;
+		for (int c = 0; c < 5; ++c) {
+			if (c && (cand[c] == d || cand[c] == cand[c - 1])) continue;
+			p->grid[g] = (int32_t)std::min<int64_t>(n, (int64_t)p->ncu * cand[c]);
+			float ms;
+			if (!time_run(&ms)) return 0;
+			// the default stays unless another setting is clearly (2%) faster: run-to-run noise
+			// is about 1%
+			if (c == 0? true : ms < best_ms * 0.98f) best_ms = ms, best_w = cand[c];
+		}
+		wpc[g] = best_w;
+		p->grid[g] = (int32_t)std::min<int64_t>(n, (int64_t)p->ncu * best_w);
+	}
+	if (waves_per_cu) for (int g = 0; g < kGroups; ++g) waves_per_cu[g] = wpc[g];
+	return 1;
 }
 
 extern "C" void nsh_ext_plan_destroy(nsh_ext_plan_t *p)

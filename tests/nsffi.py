@@ -468,6 +468,8 @@ class Product(_Lib):
         L.nsh_ext_plan_cells.restype = C.c_double
         L.nsh_ext_plan_last_ms.argtypes = [C.c_void_p, C.c_int32]
         L.nsh_ext_plan_last_ms.restype = C.c_float
+        L.nsh_ext_plan_autotune.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+        L.nsh_ext_plan_autotune.restype = C.c_int
         L.nsh_build_info.restype = C.c_char_p
         L.ns_gpu_global.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, C.c_char_p, C.c_int32, C.c_void_p, C.c_char_p,
                                     C.c_void_p]
@@ -550,6 +552,13 @@ class ExtPlan:
 
     def ms(self, which):
         return self.prod.lib.nsh_ext_plan_last_ms(self.p, which)
+
+    def autotune(self, reps=2):
+        """the persistent wave count per column-count group, picked by timing this plan's runs"""
+        w = (C.c_int32 * 3)()
+        if not self.prod.lib.nsh_ext_plan_autotune(self.p, reps, w):
+            raise RuntimeError("nsh_ext_plan_autotune: " + self.prod.error())
+        return list(w)
 
     def close(self):
         if self.p:

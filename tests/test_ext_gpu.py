@@ -137,9 +137,12 @@ def test_ext_plan_config1_batch(oracle, product, seed, n, lo, hi):
         plan.run()
         got = plan.results()
         got2 = (plan.run(), plan.results())[1]   # a second run of the same plan is identical
+        waves = plan.autotune(reps=1)            # the autotuner changes only the persistent grid
+        got3 = (plan.run(), plan.results())[1]
     finally:
         plan.close()
-    assert got == got2
+    assert got == got2 == got3
+    assert len(waves) == 3 and all(1 <= w <= 64 for w in waves), waves
 
     def one(k):
         d, p = pairs[k]
