@@ -342,6 +342,11 @@ void nsh_prescreen_destroy(nsh_prescreen_t *ps);
 typedef struct nsh_index_s nsh_index_t;
 nsh_index_t *nsh_index_build(int32_t n_ctg, const uint8_t *nt, const int64_t *ctg_off, const int64_t *ctg_len,
 		int32_t min_aa_len, int32_t kmer, int32_t mod_bit, int32_t bbit);
+/* the same index from the reference's 4-bit genome as mp_ntseq_read stores it (mp_ntdb_t.seq:
+ * base i in the low nibble of byte i/2 when i is even, the high one when odd; ctg_off in bases):
+ * half the bytes cross PCIe, and the device unpacks them */
+nsh_index_t *nsh_index_build_nt4(int32_t n_ctg, const uint8_t *seq4, const int64_t *ctg_off, const int64_t *ctg_len,
+		int32_t min_aa_len, int32_t kmer, int32_t mod_bit, int32_t bbit);
 int64_t nsh_index_size(const nsh_index_t *ix, int32_t what);
 int nsh_index_copy(const nsh_index_t *ix, int64_t *ki, uint32_t *kb, uint32_t *bo);
 void nsh_index_destroy(nsh_index_t *ix);

@@ -134,6 +134,8 @@ def lib():
         L.nsh_prescreen_stats.argtypes = [C.c_void_p, C.c_void_p]
         L.nsh_index_build.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 4
         L.nsh_index_build.restype = C.c_void_p
+        L.nsh_index_build_nt4.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 4
+        L.nsh_index_build_nt4.restype = C.c_void_p
         L.nsh_index_size.argtypes = [C.c_void_p, C.c_int32]
         L.nsh_index_size.restype = C.c_int64
         L.nsh_index_copy.argtypes = [C.c_void_p] * 4
@@ -436,13 +438,24 @@ def ps_gapped_batch(pssm, orfs, diags, band, go, ge):
     return sw[:len(orfs)].tolist()
 
 
-def index_build(genome, ctg_off, ctg_len, min_aa_len=30, kmer=6, mod_bit=1, bbit=8):
-    """the k-mer block index built on the device (nsh_index_*): (ki, kb, bo) as numpy arrays"""
+def pack_nt4(codes):
+    """one code per base to the reference's 4-bit genome (mp_ntdb_t.seq): base i in the low
+    nibble of byte i/2 when i is even, the high nibble when odd"""
+    c = np.ascontiguousarray(codes, dtype=np.uint8)
+    if len(c) % 2:
+        c = np.append(c, np.uint8(0))
+    return (c[0::2] & 15) | ((c[1::2] & 15) << 4)
+
+
+def index_build(genome, ctg_off, ctg_len, min_aa_len=30, kmer=6, mod_bit=1, bbit=8, packed=False):
+    """the k-mer block index built on the device (nsh_index_*): (ki, kb, bo) as numpy arrays;
+    packed: genome is the 4-bit form of pack_nt4 (nsh_index_build_nt4)"""
     L = lib()
     g = np.ascontiguousarray(genome, dtype=np.uint8)
     off = np.ascontiguousarray(ctg_off, dtype=np.int64)
     ln = np.ascontiguousarray(ctg_len, dtype=np.int64)
-    h = L.nsh_index_build(len(off), g.ctypes.data, off.ctypes.data, ln.ctypes.data, min_aa_len, kmer, mod_bit, bbit)
+    build = L.nsh_index_build_nt4 if packed else L.nsh_index_build
+    h = build(len(off), g.ctypes.data, off.ctypes.data, ln.ctypes.data, min_aa_len, kmer, mod_bit, bbit)
     if not h:
         raise RuntimeError("nsh_index_build: " + error())
     try:

@@ -255,8 +255,37 @@ extern "C" void nsh_index_destroy(nsh_index_t *ix)
 	delete ix;
 }
 
+namespace {
+// the reference's 4-bit genome (mp_ntdb_t.seq: base i in the low nibble of byte i/2 when i is
+// even, the high nibble when odd) to one code per byte
+__global__ void unpack4_kernel(const uint8_t *pk, int64_t n, uint8_t *out)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (2 * i >= n) return;
+	const uint8_t b = pk[i];
+	out[2 * i] = b & 0xf;
+	if (2 * i + 1 < n) out[2 * i + 1] = b >> 4;
+}
+
+nsh_index_t *index_build(int32_t n_ctg, const uint8_t *nt, bool packed, const int64_t *ctg_off,
+		const int64_t *ctg_len, int32_t min_aa_len, int32_t kmer, int32_t mod_bit, int32_t bbit);
+} // namespace
+
 extern "C" nsh_index_t *nsh_index_build(int32_t n_ctg, const uint8_t *nt, const int64_t *ctg_off, const int64_t *ctg_len,
 		int32_t min_aa_len, int32_t kmer, int32_t mod_bit, int32_t bbit)
+{
+	return index_build(n_ctg, nt, false, ctg_off, ctg_len, min_aa_len, kmer, mod_bit, bbit);
+}
+
+extern "C" nsh_index_t *nsh_index_build_nt4(int32_t n_ctg, const uint8_t *seq4, const int64_t *ctg_off,
+		const int64_t *ctg_len, int32_t min_aa_len, int32_t kmer, int32_t mod_bit, int32_t bbit)
+{
+	return index_build(n_ctg, seq4, true, ctg_off, ctg_len, min_aa_len, kmer, mod_bit, bbit);
+}
+
+namespace {
+nsh_index_t *index_build(int32_t n_ctg, const uint8_t *nt, bool packed, const int64_t *ctg_off,
+		const int64_t *ctg_len, int32_t min_aa_len, int32_t kmer, int32_t mod_bit, int32_t bbit)
 {
 	if (n_ctg < 0 || (n_ctg > 0 && (!nt || !ctg_off || !ctg_len)) || min_aa_len < 1 || kmer < 1 || kmer > 7 ||
 			mod_bit < 0 || mod_bit >= 4 * kmer || bbit < 0 || bbit > 30) {
@@ -280,7 +309,7 @@ extern "C" nsh_index_t *nsh_index_build(int32_t n_ctg, const uint8_t *nt, const 
 	if (boff >= (int64_t)UINT32_MAX) { nsh_set_error("index: %lld blocks exceed 32 bits", (long long)boff); delete ix; return nullptr; }
 	ix->bo[(size_t)n_ctg * 2] = ix->n_block = (uint32_t)boff;
 	hipStream_t s = nullptr;
-	uint8_t *d_g = nullptr, *d_rc = nullptr, *d_cod = nullptr, *d_cod13 = nullptr;
+	uint8_t *d_g = nullptr, *d_rc = nullptr, *d_cod = nullptr, *d_cod13 = nullptr, *d_pk = nullptr;
 	uint64_t *d_raw = nullptr, *d_sorted = nullptr;
 	unsigned long long *d_cnt = nullptr;
 	int64_t *d_nuniq = nullptr;
@@ -297,7 +326,13 @@ extern "C" nsh_index_t *nsh_index_build(int32_t n_ctg, const uint8_t *nt, const 
 		IXOK(hipMalloc((void**)&d_cod, 64));
 		IXOK(hipMalloc((void**)&d_cod13, 64));
 		IXOK(hipMalloc((void**)&d_raw, (size_t)cap * 8 + 8));
-		if (tot) IXOK(hipMemcpyAsync(d_g, nt, (size_t)tot, hipMemcpyHostToDevice, s));
+		if (tot && !packed) IXOK(hipMemcpyAsync(d_g, nt, (size_t)tot, hipMemcpyHostToDevice, s));
+		if (tot && packed) {   // half the bytes cross PCIe; the device unpacks them
+			IXOK(hipMalloc((void**)&d_pk, (size_t)(tot + 1) / 2 + 8));
+			IXOK(hipMemcpyAsync(d_pk, nt, (size_t)(tot + 1) / 2, hipMemcpyHostToDevice, s));
+			hipLaunchKernelGGL(unpack4_kernel, dim3((unsigned)(((tot + 1) / 2 + 255) / 256)), dim3(256), 0, s, d_pk, tot, d_g);
+			IXOK(hipGetLastError());
+		}
 		IXOK(hipMemcpyAsync(d_cod, ns_tab_codon, 64, hipMemcpyHostToDevice, s));
 		IXOK(hipMemcpyAsync(d_cod13, ns_tab_codon13, 64, hipMemcpyHostToDevice, s));
 		for (int32_t j = 0; j < 2 * n_ctg; ++j) {   // contig strands (mp_ntseq_get orientation)
@@ -384,7 +419,7 @@ extern "C" nsh_index_t *nsh_index_build(int32_t n_ctg, const uint8_t *nt, const 
 	}
 	{
 		void *bufs[] = { d_g, d_rc, d_cod, d_cod13, d_raw, d_sorted, d_cnt, d_nuniq, d_tmp, d_rcnt, d_roff,
-			d_scan };
+			d_scan, d_pk };
 		for (void *b : bufs) if (b) (void)hipFree(b);
 		ow.release();
 		(void)hipStreamDestroy(s);
@@ -394,7 +429,7 @@ fail:
 	{
 		if (s) (void)hipStreamSynchronize(s);
 		void *bufs[] = { d_g, d_rc, d_cod, d_cod13, d_raw, d_sorted, d_cnt, d_nuniq, d_tmp, d_rcnt, d_roff,
-			d_scan };
+			d_scan, d_pk };
 		for (void *b : bufs) if (b) (void)hipFree(b);
 		ow.release();
 		if (s) (void)hipStreamDestroy(s);
@@ -402,6 +437,7 @@ fail:
 	nsh_index_destroy(ix);
 	return nullptr;
 }
+} // namespace
 
 extern "C" int64_t nsh_index_size(const nsh_index_t *ix, int32_t what)
 {

@@ -139,3 +139,14 @@ def test_anchor_closed_forms():
     assert index_ref.sketch_prot(b"AA*AA", 3, 0, t["aa13"]) == []
     assert index_ref.sketch_prot(b"", 3, 0, t["aa13"]) == []
     assert len(index_ref.sketch_prot(synth.rand_protein(rng, 50), 3, 0, t["aa13"])) == 48
+
+
+def test_pack_nt4_layout():
+    """capi.pack_nt4 stores base i as mp_ntseq_read does (ntseq.c:43: seq[i >> 1] |= b << (i & 1) * 4)"""
+    from protlocus_amd import capi
+    rng = np.random.default_rng(4)
+    for n in (0, 1, 2, 7, 1000):
+        codes = rng.integers(0, 5, n).astype(np.uint8)
+        pk = capi.pack_nt4(codes)
+        assert len(pk) == (n + 1) // 2
+        assert [(int(pk[i >> 1]) >> ((i & 1) * 4)) & 15 for i in range(n)] == codes.tolist()

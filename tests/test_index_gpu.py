@@ -29,6 +29,10 @@ def test_index_golden(product):
                                       c["bbit"])
         assert bo.tolist() == c["bo"] and len(kb) == c["n_kb"], k
         assert np.array_equal(index_keys(ki, kb), unpack_keys(c["keys"])), k
+        # the same genome in the reference's 4-bit form (nsh_index_build_nt4)
+        pki, pkb, pbo = capi.index_build(capi.pack_nt4(g), c["ctg_off"], c["ctg_len"], c["min_aa_len"], c["kmer"],
+                                         c["mod_bit"], c["bbit"], packed=True)
+        assert np.array_equal(pki, ki) and np.array_equal(pkb, kb) and np.array_equal(pbo, bo), k
 
 
 def test_index_random(product):
