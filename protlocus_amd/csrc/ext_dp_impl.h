@@ -250,8 +250,9 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, 
 		mx = std::max(mx, dpp0<ROW_SHR2>(mx));
 		mx = std::max(mx, dpp0<ROW_SHR4>(mx));
 		mx = std::max(mx, dpp0<ROW_SHR8>(mx));
-		const int rowmax = std::max(std::max(__builtin_amdgcn_readlane(mx, 15), __builtin_amdgcn_readlane(mx, 31)),
-				std::max(__builtin_amdgcn_readlane(mx, 47), __builtin_amdgcn_readlane(mx, 63))) - 32768;
+		mx = std::max(mx, dpp0<ROW_BCAST15>(mx));   // rows 1-3 take the row before's last lane
+		mx = std::max(mx, dpp0<ROW_BCAST31>(mx));   // rows 2-3 take lane 31: lane 63 holds the whole wave
+		const int rowmax = __builtin_amdgcn_readlane(mx, 63) - 32768;
 
 		// H(i, al-1)
 		int ev = 0;
@@ -280,17 +281,20 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, 
 		const int Vs = dpp0<WAVE_SHR1>(V);
 		const int V63 = __builtin_amdgcn_readlane(V, 63);
 		nh[R] = up(L == 0? (int)(((unsigned)V63 << 16) | 0x8000u) : Vs);
-		if (i == 2 && L == 0) nh[R2].x = NEG, nh[R1].x = NEG;
 		last_i = i;
 		if (best_log - tlog > xdrop) return false;
 		return ++i < nl;
 	};
 
 	if (nl > 2) {
-		for (;;) {
-			if (!row(std::integral_constant<int, 0>())) break;
+		// row 2 alone: column -1 holds 0 / -fs for it only, so afterwards lane 0's left neighbour
+		// of rows 0 and 1 (slots 1 and 2 when row 2 used slot 0) reads neg_inf in the low half
+		bool more = row(std::integral_constant<int, 0>());
+		if (L == 0) nh[1].x = NEG, nh[2].x = NEG;
+		while (more) {
 			if (!row(std::integral_constant<int, 1>())) break;
 			if (!row(std::integral_constant<int, 2>())) break;
+			if (!row(std::integral_constant<int, 0>())) break;
 		}
 	}
 
