@@ -167,7 +167,7 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, 
 		constexpr int R = decltype(Rc)::value, R2 = (R + 1) % 3, R1 = (R + 2) % 3;
 		s2 *H3 = Hs[R], *H2 = Hs[R2], *H1 = Hs[R1];
 		int *D3 = dring + R * C * 64 + L;
-		if ((i & 63) == 0) { // next block of records
+		if (__builtin_expect((i & 63) == 0, 0)) { // next block of records
 			recv = recn, penv = penn;
 			if (i + 64 < nl) load_blk(i + 64, recn, penn);
 		}
