@@ -341,14 +341,20 @@ __global__ void __launch_bounds__(64) chain_kernel(ChArgs A)
 			if (!vm) continue;
 			const int32_t scf = sc + fj;
 			// marks t[p[j]] = i of the lanes before: inside this chunk a lane mask, further down words
-			if (lane == 0) L.cmask = 0;
-			wave_sync_lds();
-			if (valid && pj >= st) {
-				if (pj >= c0 - 63) atomicOr((unsigned long long *)&L.cmask, 1ull << (c0 - pj));
-				else __hip_atomic_store(&t[pj], (int32_t)i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+			// (read by the next chunk, after its barrier); claims below st are never read
+			const bool inc = valid && pj >= st && pj >= c0 - 63;
+			if (valid && pj >= st && pj < c0 - 63)
+				__hip_atomic_store(&t[pj], (int32_t)i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+			uint64_t cm = 0;
+			if (__ballot(inc)) {
+				if (lane == 0) L.cmask = 0;
+				wave_sync_lds();
+				if (inc) atomicOr((unsigned long long *)&L.cmask, 1ull << (c0 - pj));
+				wave_sync_lds();
+				cm = L.cmask;
+				wave_sync_lds();
 			}
-			__syncthreads();
-			bool tm = (L.cmask >> lane) & 1;
+			bool tm = (cm >> lane) & 1;
 			if (c0 != i - 1 && in && !tm)
 				tm = __hip_atomic_load(&t[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == (int32_t)i;
 			const int32_t before = shr1(wave_max_incl(valid? scf : INT_MIN), INT_MIN);
