@@ -306,8 +306,14 @@ __global__ void __launch_bounds__(64) chain_kernel(ChArgs A)
 	int32_t hf = 0;
 	uint64_t wa = 0;
 	int32_t wf = 0, wp = -1;
+	// the anchors ahead, 64 per register: lane L holds a[i0 + L] of this block and of the next
+	uint64_t ablk = lane < n? a[lane] : 0, anext = 64 + lane < n? a[64 + lane] : 0;
 	for (int64_t i = 0; i < n; ++i) {
-		const uint64_t ai = a[i];
+		if (__builtin_expect((i & 63) == 0 && i > 0, 0)) {
+			ablk = anext;
+			anext = i + 64 + lane < n? a[i + 64 + lane] : 0;
+		}
+		const uint64_t ai = readlane_64(ablk, (int)(i & 63));
 		bool stop = false;
 		while (st < i - 64) {
 			if (((ai >> 32) - (a[st] >> 32)) << c.bbit > mdx) ++st;
@@ -340,6 +346,19 @@ __global__ void __launch_bounds__(64) chain_kernel(ChArgs A)
 			const uint64_t vm = __ballot(valid);
 			if (!vm) continue;
 			const int32_t scf = sc + fj;
+			if (c0 == i - 1 && !(vm & (vm - 1))) {
+				// one scoring predecessor among the nearest 64: nothing before it claims it, so it
+				// only improves or not; its claim matters only to the chunks further down
+				const int l = __builtin_ctzll(vm);
+				if (valid && pj >= st && pj < c0 - 63)
+					__hip_atomic_store(&t[pj], (int32_t)i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+				const int32_t sl = __builtin_amdgcn_readlane(scf, l);
+				if (sl > max_f) {
+					max_f = sl, max_j = c0 - l;
+					if (n_skip > 0) --n_skip;
+				}
+				continue;
+			}
 			// marks t[p[j]] = i of the lanes before: inside this chunk a lane mask, further down words
 			// (read by the next chunk, after its barrier); claims below st are never read
 			const bool inc = valid && pj >= st && pj >= c0 - 63;
