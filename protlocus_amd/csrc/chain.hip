@@ -424,18 +424,22 @@ __global__ void __launch_bounds__(64) chain_kernel(ChArgs A)
 	__syncthreads();
 	int32_t bu = 0, bv = 0;   // lane 0's chain and anchor counts
 	for (int32_t k0 = n_z - 1; k0 >= 0; k0 -= 64) {
+		// the next 64 candidates, read by the lanes; a candidate already marked (t only turns
+		// from 0 to 1 between candidates) is passed over as the reference does
+		__syncthreads();   // lane 0's marks of the chunk before, for the lanes' reads of t
 		const int32_t k = k0 - lane;
 		bool live = false;
+		int32_t zkl = 0, zsl = 0;
 		if (k >= 0) {
-			const int32_t zk = (int32_t)zy[k], zs = (int32_t)zx[k];
-			live = p[zk] >= 0 || hs[zk] || (c.min_cnt <= 1 && zs > 0 && zs >= c.min_sc);
+			zkl = (int32_t)zy[k], zsl = (int32_t)zx[k];
+			live = (p[zkl] >= 0 || hs[zkl] || (c.min_cnt <= 1 && zsl > 0 && zsl >= c.min_sc)) && t[zkl] == 0;
 		}
 		uint64_t lm = __ballot(live);
 		while (lm) {
 			const int l = __builtin_ctzll(lm);
 			lm &= lm - 1;
+			const int32_t zk = __builtin_amdgcn_readlane(zkl, l), zs = __builtin_amdgcn_readlane(zsl, l);
 			if (lane != 0) continue;
-			const int32_t zk = (int32_t)zy[k0 - l], zs = (int32_t)zx[k0 - l];
 			if (t[zk] != 0) continue;
 			// mp_chain_bk_end (chain.c:8-25)
 			int32_t end_i = -1, max_i = zk, max_s = 0, ii = zk;
