@@ -607,7 +607,7 @@ extern "C" int64_t nsh_chain_batch(int32_t n_q, const int64_t *a_off, const uint
 		CHOK(hipMalloc((void**)&d_u, na * 8));
 		CHOK(hipMalloc((void**)&d_ao, na * 8));
 		CHOK(hipMalloc((void**)&d_nu, (size_t)n_q * 4));
-		if (n_a > 0) CHOK(hipMemcpyAsync(d_a, anchors, (size_t)n_a * 8, hipMemcpyHostToDevice, s));
+		if (n_a > 0 && !nsh_copy_to_device(d_a, anchors, (size_t)n_a * 8, s)) goto done;
 		CHOK(hipMemcpyAsync(d_off, a_off, (size_t)(n_q + 1) * 8, hipMemcpyHostToDevice, s));
 		if (!nsh_chain::run(s, n_q, d_off, nullptr, cnt.data(), d_a, n_a, prm, d_nu, d_u, d_ao)) goto done;
 		// only the chains and their anchors come back, packed, then go to each query's offset
@@ -628,8 +628,7 @@ extern "C" int64_t nsh_chain_batch(int32_t n_q, const int64_t *a_off, const uint
 					d_poff + n_q, d_u, d_ao, d_pk);
 			CHOK(hipGetLastError());
 			pk.resize(np);
-			CHOK(hipMemcpyAsync(pk.data(), d_pk, (size_t)np * 8, hipMemcpyDeviceToHost, s));
-			CHOK(hipStreamSynchronize(s));
+			if (!nsh_copy_to_host(pk.data(), d_pk, (size_t)np * 8, s)) goto done;
 			nsh_count_launch(2);
 			for (int32_t q = 0; q < n_q; ++q) {
 				std::copy(pk.begin() + poff[q], pk.begin() + poff[q] + n_u[q], u + a_off[q]);

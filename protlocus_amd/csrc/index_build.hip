@@ -326,10 +326,10 @@ nsh_index_t *index_build(int32_t n_ctg, const uint8_t *nt, bool packed, const in
 		IXOK(hipMalloc((void**)&d_cod, 64));
 		IXOK(hipMalloc((void**)&d_cod13, 64));
 		IXOK(hipMalloc((void**)&d_raw, (size_t)cap * 8 + 8));
-		if (tot && !packed) IXOK(hipMemcpyAsync(d_g, nt, (size_t)tot, hipMemcpyHostToDevice, s));
+		if (tot && !packed && !nsh_copy_to_device(d_g, nt, (size_t)tot, s)) goto fail;
 		if (tot && packed) {   // half the bytes cross PCIe; the device unpacks them
 			IXOK(hipMalloc((void**)&d_pk, (size_t)(tot + 1) / 2 + 8));
-			IXOK(hipMemcpyAsync(d_pk, nt, (size_t)(tot + 1) / 2, hipMemcpyHostToDevice, s));
+			if (!nsh_copy_to_device(d_pk, nt, (size_t)(tot + 1) / 2, s)) goto fail;
 			hipLaunchKernelGGL(unpack4_kernel, dim3((unsigned)(((tot + 1) / 2 + 255) / 256)), dim3(256), 0, s, d_pk, tot, d_g);
 			IXOK(hipGetLastError());
 		}
@@ -448,11 +448,11 @@ extern "C" int64_t nsh_index_size(const nsh_index_t *ix, int32_t what)
 extern "C" int nsh_index_copy(const nsh_index_t *ix, int64_t *ki, uint32_t *kb, uint32_t *bo)
 {
 	if (!ix) { nsh_set_error("index: null index"); return 0; }
-	if (ki && hipMemcpy(ki, ix->d_ki, (size_t)ix->n_bucket * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+	if (ki && !nsh_copy_to_host(ki, ix->d_ki, (size_t)ix->n_bucket * 8, nullptr)) {
 		nsh_set_error("index: copy of the bucket starts failed");
 		return 0;
 	}
-	if (kb && ix->n_kb > 0 && hipMemcpy(kb, ix->d_kb, (size_t)ix->n_kb * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+	if (kb && ix->n_kb > 0 && !nsh_copy_to_host(kb, ix->d_kb, (size_t)ix->n_kb * 4, nullptr)) {
 		nsh_set_error("index: copy of the entries failed");
 		return 0;
 	}
@@ -602,7 +602,7 @@ extern "C" int64_t nsh_index_anchors(const nsh_index_t *ix, int32_t n_q, const c
 		if (!collect_anchors(ix, n_q, seq, q_off, q_len, max_occ, s, aoff, &d_aoff, &d_anch)) goto done;
 		const int64_t n_a = aoff[n_q];
 		if (max_anchors > 0 && n_a > 0)
-			ANOK(hipMemcpyAsync(anchors, d_anch, (size_t)std::min(n_a, max_anchors) * 8, hipMemcpyDeviceToHost, s));
+			if (!nsh_copy_to_host(anchors, d_anch, (size_t)std::min(n_a, max_anchors) * 8, s)) goto done;
 		ANOK(hipStreamSynchronize(s));
 		std::copy(aoff.begin(), aoff.end(), a_off);
 		total = n_a;

@@ -73,6 +73,13 @@ int run(hipStream_t s, int32_t n_query, const int32_t *d_off, const int32_t *d_q
 		int32_t top_k, int32_t min_per_vid, int32_t adaptive, uint8_t *d_keep, int32_t *d_k, nsh_gapless_calib_t *cal);
 }
 
+// copies between the caller's (pageable) host memory and the device; large ones are staged
+// through pinned chunks with the DMA overlapping the host side (host_copy.cpp). Both return
+// 1 on success; to_host waits for the data, to_device waits when it staged (otherwise the copy
+// is queued on s like hipMemcpyAsync).
+int nsh_copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t s);
+int nsh_copy_to_device(void *dst, const void *src, size_t bytes, hipStream_t s);
+
 namespace nsh_chain {
 struct Params {                    // mp_chain's arguments (chain.c:160)
 	int32_t max_dist_x, max_dist_y, bw, max_skip, max_iter, min_cnt, min_sc;
