@@ -125,3 +125,17 @@ def test_index_chain_golden(product):
                     assert (u.tolist(), a.tolist()) == want, (k, oi, j)
         finally:
             ix.close()
+
+
+def test_chain_large_staged(product):
+    """1,000 queries of 9,001 anchors, each on a perfect diagonal: 72 MB each way, more than one
+    64-MB staging chunk of the host copies; every query gives one chain of all its anchors"""
+    from protlocus_amd import capi
+    m, nq = 9001, 1000
+    i = np.arange(m, dtype=np.uint64)
+    qs = [((np.uint64(3) * i + np.uint64(q * 50000)) << np.uint64(32)) | (i + np.uint64(q)) for q in range(nq)]
+    print("anchors", m * nq, flush=True)
+    got = capi.chain_batch(qs, 100, 100, 100, 25, 5000, 2, 0, 0.75, 1, 1, 0)
+    for q, (u, a) in enumerate(got):
+        assert u.tolist() == [m << 32 | m], q
+        assert np.array_equal(a, qs[q]), q
