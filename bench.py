@@ -497,7 +497,7 @@ def bench_chain(anchors, cpu=True, log=None):
         assert r >= 0, capi.error()
         return off, n_u, u, ao, dt
 
-    run(anchors[:200], pre)   # warm-up
+    run(anchors, pre)   # warm-up: the library keeps its chaining workspace between calls
     off, n_u, u, ao, pre_s = run(anchors, pre)
     kept = []
     for q in range(len(anchors)):

@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <climits>
+#include <mutex>
 #include <vector>
 #include "nsh_internal.h"
 
@@ -520,27 +521,49 @@ __global__ void chain_pack_kernel(const int64_t *off, const int32_t *nu, const i
 
 #define CHOK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { nsh_set_error("chain: %s: %s", #x, hipGetErrorString(e_)); goto done; } } while (0)
 
+namespace {
+// device scratch kept between calls and grown on demand (hipMalloc of gigabytes per call costs
+// more than the chaining of a mini-batch); calls on one process are serialised by g_ws_mu
+std::recursive_mutex g_ws_mu;
+struct Grow {
+	void *p = nullptr;
+	size_t cap = 0;
+	hipError_t need(size_t bytes)
+	{
+		if (bytes <= cap) return hipSuccess;
+		if (p) (void)hipFree(p), p = nullptr, cap = 0;
+		const size_t c = bytes + bytes / 4;
+		const hipError_t e = hipMalloc(&p, c);
+		if (e == hipSuccess) cap = c;
+		else p = nullptr;
+		return e;
+	}
+};
+Grow g_scratch, g_order, g_io;
+size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+} // namespace
+
 int nsh_chain::run(hipStream_t s, int32_t n_q, const int64_t *d_off, const int64_t *d_cnt, const int64_t *h_cnt,
 		const uint64_t *d_a, int64_t n_a, const Params &prm, int32_t *d_nu, uint64_t *d_u, uint64_t *d_ao)
 {
 	if (n_q == 0) return 1;
+	std::lock_guard<std::recursive_mutex> lk(g_ws_mu);
 	int ok = 0;
 	const size_t na = (size_t)(n_a > 0? n_a : 1);
+	const size_t o4 = al256(na * 4), o8 = al256(na * 8);   // f, p, t, vv; zx, zy, b; hs
 	uint64_t *d_zx = nullptr, *d_zy = nullptr, *d_b = nullptr;
 	int32_t *d_f = nullptr, *d_p = nullptr, *d_t = nullptr, *d_vv = nullptr, *d_ord = nullptr;
 	uint8_t *d_hs = nullptr;
 	std::vector<int32_t> order(n_q);
 	ChArgs A;
 	{
-		CHOK(hipMalloc((void**)&d_f, na * 4));
-		CHOK(hipMalloc((void**)&d_p, na * 4));
-		CHOK(hipMalloc((void**)&d_t, na * 4));
-		CHOK(hipMalloc((void**)&d_vv, na * 4));
-		CHOK(hipMalloc((void**)&d_hs, na));
-		CHOK(hipMalloc((void**)&d_zx, na * 8));
-		CHOK(hipMalloc((void**)&d_zy, na * 8));
-		CHOK(hipMalloc((void**)&d_b, na * 8));
-		CHOK(hipMalloc((void**)&d_ord, (size_t)n_q * 4));
+		CHOK(g_scratch.need(4 * o4 + 3 * o8 + al256(na)));
+		CHOK(g_order.need((size_t)n_q * 4));
+		char *w = (char*)g_scratch.p;
+		d_f = (int32_t*)w, d_p = (int32_t*)(w + o4), d_t = (int32_t*)(w + 2 * o4), d_vv = (int32_t*)(w + 3 * o4);
+		d_zx = (uint64_t*)(w + 4 * o4), d_zy = (uint64_t*)(w + 4 * o4 + o8), d_b = (uint64_t*)(w + 4 * o4 + 2 * o8);
+		d_hs = (uint8_t*)(w + 4 * o4 + 3 * o8);
+		d_ord = (int32_t*)g_order.p;
 		for (int32_t q = 0; q < n_q; ++q) order[q] = q;   // largest queries first
 		std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return h_cnt[x] > h_cnt[y]; });
 		CHOK(hipMemcpyAsync(d_ord, order.data(), (size_t)n_q * 4, hipMemcpyHostToDevice, s));
@@ -562,10 +585,6 @@ int nsh_chain::run(hipStream_t s, int32_t n_q, const int64_t *d_off, const int64
 	}
 done:
 	(void)hipStreamSynchronize(s);
-	{
-		void *bufs[] = { d_f, d_p, d_t, d_vv, d_hs, d_zx, d_zy, d_b, d_ord };
-		for (void *bb : bufs) if (bb) (void)hipFree(bb);
-	}
 	return ok;
 }
 
@@ -600,12 +619,13 @@ extern "C" int64_t nsh_chain_batch(int32_t n_q, const int64_t *a_off, const uint
 	const size_t na = (size_t)(n_a > 0? n_a : 1);
 	const nsh_chain::Params prm = { max_dist_x, max_dist_y, bw, max_skip, max_iter, min_cnt, min_sc, chn_coef_log,
 		is_spliced, kmer, bbit };
+	std::lock_guard<std::recursive_mutex> lk(g_ws_mu);
 	{
 		CHOK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-		CHOK(hipMalloc((void**)&d_a, na * 8));
+		CHOK(g_io.need(3 * al256(na * 8)));   // the anchors, the chains, their anchors
+		d_a = (uint64_t*)g_io.p, d_u = (uint64_t*)((char*)g_io.p + al256(na * 8));
+		d_ao = (uint64_t*)((char*)g_io.p + 2 * al256(na * 8));
 		CHOK(hipMalloc((void**)&d_off, (size_t)(n_q + 1) * 8));
-		CHOK(hipMalloc((void**)&d_u, na * 8));
-		CHOK(hipMalloc((void**)&d_ao, na * 8));
 		CHOK(hipMalloc((void**)&d_nu, (size_t)n_q * 4));
 		if (n_a > 0 && !nsh_copy_to_device(d_a, anchors, (size_t)n_a * 8, s)) goto done;
 		CHOK(hipMemcpyAsync(d_off, a_off, (size_t)(n_q + 1) * 8, hipMemcpyHostToDevice, s));
@@ -641,7 +661,7 @@ extern "C" int64_t nsh_chain_batch(int32_t n_q, const int64_t *a_off, const uint
 done:
 	if (s) (void)hipStreamSynchronize(s);
 	{
-		void *bufs[] = { d_a, d_off, d_u, d_ao, d_nu, d_kept, d_poff, d_pk };
+		void *bufs[] = { d_off, d_nu, d_kept, d_poff, d_pk };   // d_a, d_u, d_ao: the kept workspace
 		for (void *bb : bufs) if (bb) (void)hipFree(bb);
 	}
 	if (s) (void)hipStreamDestroy(s);
