@@ -1,7 +1,7 @@
 // Learned chain-candidate prescreen: a 64 -> 256 -> 256 -> 1 MLP (ReLU, logistic output) with
 // hand-written BF16 MFMA kernels for inference and training (FP32 accumulation, FP32 Adam
 // master weights). No reference counterpart: it ranks candidates ahead of the spliced DP,
-// which stays the fallback and the judge of recall.
+// which stays the fallback and the arbiter of recall.
 //
 // Matrix products use v_mfma_f32_32x32x16_bf16 (gfx950): a wave computes a 32 x 32 FP32 tile;
 // lane l supplies row (l % 32) of A and column (l % 32) of B, eight K values each (the K

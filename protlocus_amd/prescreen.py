@@ -10,7 +10,7 @@ semantics, every row evaluated) accepts a candidate when score >= 2*al - 150. Th
 backward kernels; a logit threshold is calibrated on held-out candidates for >= 99.8% recall
 of DP-accepted candidates. At inference the network runs ahead of the batched DP on the
 plan's resident inputs; rejected candidates are skipped except for a 2% random audit that
-still goes through the full DP, which stays the judge of every accepted alignment.
+still goes through the full DP, which stays the arbiter of every accepted alignment.
 """
 import time
 
