@@ -139,3 +139,28 @@ def test_chain_large_staged(product):
     for q, (u, a) in enumerate(got):
         assert u.tolist() == [m << 32 | m], q
         assert np.array_equal(a, qs[q]), q
+
+
+def test_index_chain_edges(product):
+    """nsh_index_chain: a null index or options are refused; no queries give no chains; empty and
+    3-residue queries give none either"""
+    import ctypes as C
+    from protlocus_amd import capi
+    L = capi.lib()
+    uoff, coff = np.zeros(3, np.int64), np.zeros(3, np.int64)
+    opt = capi.ChainOpt()
+    assert L.nsh_index_chain(None, 0, b"", None, None, 20000, C.byref(opt), uoff.ctypes.data, None, 0,
+                             coff.ctypes.data, None, 0) < 0
+    rng = np.random.default_rng(8)
+    g = rng.integers(0, 4, 30000).astype(np.uint8)
+    ix = capi.Index(g, [0], [len(g)])
+    try:
+        assert L.nsh_index_chain(ix.p, 0, b"", None, None, 20000, None, uoff.ctypes.data, None, 0,
+                                 coff.ctypes.data, None, 0) < 0
+        assert L.nsh_index_chain(ix.p, 0, b"", None, None, 20000, C.byref(opt), uoff.ctypes.data, None, 0,
+                                 coff.ctypes.data, None, 0) == 0
+        assert uoff[0] == 0 and coff[0] == 0
+        got = ix.chain([b"", b"MKV"])
+        assert [(len(u), len(a)) for u, a in got] == [(0, 0), (0, 0)]
+    finally:
+        ix.close()
