@@ -47,23 +47,72 @@ EXT_DP_TRAFFIC_BYTES = 932.2e6
 
 
 def threads():
+    """host threads of the CPU baselines: the box's CPU share for one GPU (OMP_NUM_THREADS, set to 16
+    there), else every core this process may run on"""
     try:
-        return min(len(os.sched_getaffinity(0)), 16)
+        n_aff = len(os.sched_getaffinity(0))
     except AttributeError:
-        return min(os.cpu_count() or 1, 16)
+        n_aff = os.cpu_count() or 1
+    try:
+        return max(1, min(n_aff, int(os.environ["OMP_NUM_THREADS"])))
+    except (KeyError, ValueError):
+        return n_aff
 
 
-def cpu_baseline(pairs, kw, seconds=15.0):
-    """the reference CPU path (nasw-sse.c via oracle/_ref when built, else the oracle port) on a
-    bounded sample of the same workload, threaded like kt_for"""
+def host_cpu():
+    """the host CPU's model name and the cores this process may run on"""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        n_aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n_aff = os.cpu_count() or 1
+    return {"model": model, "affinity_cores": n_aff, "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline(pairs, kw, seconds=15.0, packed=None):
+    """the reference CPU path on the same workload: nasw-sse.c's ns_global_gs16b compiled from the
+    reference sources (oracle/_ref), driven from C threads that take requests from a shared counter
+    as kt_for does, over every pair unless `seconds` runs out first; without oracle/_ref the oracle
+    port on a bounded sample from Python threads. Returns (baseline object, reference results by
+    pair index or None)."""
+    import numpy as np
     import nsffi
-    if nsffi.have_ref():
-        lib, kind = nsffi.Reference(), "reference"
-        fn = lambda d, p: lib.run("ns_global_gs16b", d, p, lib.opt(**kw))
-    else:
-        lib, kind = nsffi.Oracle(), "port"
-        fn = lambda d, p: lib.ext(d, p, lib.opt(**kw))[0]
     nt = threads()
+    if nsffi.have_ref():
+        ref = nsffi.Reference()
+        ns, ns_off, nl, as_, as_off, al = packed
+        out = np.full(3 * len(pairs), -1, dtype=np.int32)
+        wall = C.c_double()
+        opt = ref.opt(**kw)
+        ref.lib.or_ref_ext_batch.argtypes = [C.c_int32, C.c_char_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_void_p,
+                                             C.c_void_p]
+        ref.lib.or_ref_ext_batch.restype = C.c_int32
+        ns_off = np.ascontiguousarray(ns_off, dtype=np.int64)
+        as_off = np.ascontiguousarray(as_off, dtype=np.int64)
+        nl = np.ascontiguousarray(nl, dtype=np.int32)
+        al = np.ascontiguousarray(al, dtype=np.int32)
+        done = ref.lib.or_ref_ext_batch(len(pairs), ns, ns_off.ctypes.data, nl.ctypes.data, as_, as_off.ctypes.data,
+                                        al.ctypes.data, None, C.byref(opt), nt, seconds, out.ctypes.data,
+                                        C.byref(wall))
+        res = out.reshape(-1, 3)
+        got = res[:, 0] >= 0
+        win = float((nl[got].astype(np.float64) * al[got]).sum())
+        base = {"value": win / wall.value / 1e9, "unit": "GCUPS (nl*al window cells)", "cores": nt,
+                "kind": "reference", "cpu": host_cpu(),
+                "sample": "%d of the %d pairs (%s), %.1f s wall, nasw-sse.c ns_global_gs16b from %d C threads"
+                          % (done, len(pairs), "all" if done == len(pairs) else "time-bounded", wall.value, nt)}
+        return base, (res if done == len(pairs) else None)
+    lib, kind = nsffi.Oracle(), "port"
+    fn = lambda d, p: lib.ext(d, p, lib.opt(**kw))[0]
     n, t0 = 0, time.perf_counter()
     with cf.ThreadPoolExecutor(nt) as ex:
         while n < len(pairs) and time.perf_counter() - t0 < seconds:
@@ -73,7 +122,7 @@ def cpu_baseline(pairs, kw, seconds=15.0):
     dt = time.perf_counter() - t0
     win = sum(len(d) * len(p) for d, p in pairs[:n])
     return {"value": win / dt / 1e9, "unit": "GCUPS (nl*al window cells)", "cores": nt, "kind": kind,
-            "sample": "%d of the pairs, %.1f s wall" % (n, dt)}
+            "cpu": host_cpu(), "sample": "%d of the pairs, %.1f s wall (oracle port, Python threads)" % (n, dt)}, None
 
 
 def bench_hmm(seconds_cpu=4.0, log=None):
@@ -627,7 +676,7 @@ def main():
         dp_ms.append(plan.ms(1))
     prod.lib.nsh_device_synchronize()
     wall = time.perf_counter() - t0
-    plan.results()
+    gpu_res = plan.results()
     cells = plan.cells(1)          # evaluated cells of the last run
     win = plan.cells(0)
     plan.close()
@@ -662,8 +711,16 @@ def main():
     }
     log("[bench] gpu: %.2f GCUPS, %.1f ms/step, dp %.1f ms" % (gcups, ms, dp_avg))
     if not args.no_cpu:
-        cb = cpu_baseline(pairs, kw, seconds=args.cpu_seconds)
+        cb, ref_res = cpu_baseline(pairs, kw, seconds=args.cpu_seconds, packed=(ns, ns_off, nl, as_, as_off, al))
         out["cpu_baseline"] = cb
+        if ref_res is not None:   # every pair's (nt_len, aa_len, score) against the reference's
+            same = sum(1 for k, g in enumerate(gpu_res) if tuple(int(v) for v in ref_res[k]) == g)
+            out["identical_to_reference"] = "%d of %d" % (same, len(gpu_res))
+            log("[bench] identical to the reference: %d of %d pairs" % (same, len(gpu_res)))
+            if same != len(gpu_res):
+                bad = [k for k, g in enumerate(gpu_res) if tuple(int(v) for v in ref_res[k]) != g][:5]
+                raise SystemExit("headline results differ from the reference's on %d pairs, e.g. %s" %
+                                 (len(gpu_res) - same, [(k, gpu_res[k], tuple(ref_res[k])) for k in bad]))
         out["gpu_window_vs_cpu"] = round(out["window_gcups"] / cb["value"], 1) if cb["value"] else None
         log("[bench] cpu baseline: %.2f GCUPS (%s, %d threads)" % (cb["value"], cb["kind"], cb["cores"]))
     if not args.no_extras:

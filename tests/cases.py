@@ -490,3 +490,25 @@ def anchor_chain_case(rng, k, n_max=3000):
             xs.append(xs[int(rng.integers(0, len(xs)))])
     a = np.sort(np.array(xs[:n], dtype=np.uint64))
     return {"anchors": a, "params": CHAIN_VARIANTS[k % len(CHAIN_VARIANTS)](kmer, bbit)}
+
+
+def cli_mat(blosum62, pen=23):
+    """the CLI's score matrix (options.c:137-138): BLOSUM62 with the stop row and column set to -pen
+    by ns_set_stop_sc (nasw-tab.c:149-156), which keeps the stop/stop entry"""
+    m = list(blosum62)
+    stop = 20
+    keep = m[stop * 22 + stop]
+    for i in range(22):
+        m[stop * 22 + i] = m[i * 22 + stop] = -pen
+    m[stop * 22 + stop] = keep
+    return m
+
+
+def cli_opt_kw(blosum62, flag=0):
+    """ns_opt_t keyword arguments of the CLI's defaults (options.c:125-138) as mp_map2ns_opt maps
+    them (align.c:59-71): fs 23, GENERIC splice model scaled by sp_scale 1.0 (+.499 rounding),
+    stop scores -23 except stop/stop, gpu_min_cells 1e6"""
+    sp = [int(v * 1.0 + .499) for v in (8, 15, 21, 30, 0, 0)]
+    return dict(flag=flag, go=11, ge=1, io=29, fs=23, xdrop=100, end_bonus=5, sp_null_bonus=-7, ie_coef=.5,
+                gpu_min_cells=1000000, sp=sp, mat=cli_mat(blosum62))
+

@@ -18,6 +18,8 @@ Outputs (inputs and expected outputs only):
   index_cases.json.gz  mp_idx_boff + mp_sketch_nt4 + build_bidx (index.c, sketch.c): k-mer block indexes
   anchor_cases.json.gz  mp_sketch_query + mp_cal_max_occ + mp_collect_anchors (map.c): query anchors
   anchor_chain_cases.json.gz  mp_chain (chain.c) on sorted anchors: chains and their anchors
+  config0_cases.json.gz  single-pair workload (300 aa vs ~2.4-3 kb, 500-nt flanks) under the CLI option set:
+                     ns_global_gs_auto(NS_F_CIGAR) and the PAF / GFF3 records of its alignment
   tables.json        ns_tab_nt4 / ns_tab_aa20 / ns_tab_codon / ns_mat_blosum62
 """
 import base64
@@ -35,7 +37,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [os.path.join(ROOT, "tests"), ROOT]
 
 import nsffi                                       # noqa: E402
-from cases import (CHAIN_VARIANTS, OUTPUT_VARIANTS, anchor_case, anchor_chain_case, chain_case, gapless_select_case, gapped_case, index_case,  # noqa: E402
+from cases import (CHAIN_VARIANTS, cli_opt_kw, OUTPUT_VARIANTS, anchor_case, anchor_chain_case, chain_case, gapless_select_case, gapped_case, index_case,  # noqa: E402
                    index_keys,
                    nt4_codes, orf_case, output_case, pack_keys,
                    parse_cigar, prescreen_batch_case, prescreen_case,
@@ -153,6 +155,27 @@ def output_cases(ref):
                     "window": B64(c["window"]), "qs": c["qs"], "qe": c["qe"], "rev": c["rev"], "vs": c["vs"],
                     "ve": c["ve"], "ve_aln": ve_aln, "as_": c["as_"], "ae": c["ae"], "cigar": cigar, "score": score,
                     "pssm": None if pssm is None else pssm.tolist(), "extra": extra, "expected": exp})
+    return out
+
+
+def config0_cases(ref):
+    """the single-pair workload, seeds 0-7: the reference's ns_global_gs_auto(NS_F_CIGAR) under the
+    CLI option set (mp_map2ns_opt of the options.c defaults), then mp_extra_* + mp_write_output
+    of that alignment on a one-contig genome holding the pair's DNA (PAF and GFF3)"""
+    blosum = list((C.c_int8 * 484).in_dll(ref.lib, "ns_mat_blosum62"))
+    out = []
+    for seed in range(8):
+        dna, prot = synth.single_pair(seed)
+        kw = cli_opt_kw(blosum, flag=NS_F_CIGAR)
+        o = ref.opt(**kw)
+        nl, al, score, cg = ref.run("ns_global_gs_auto", dna, prot, o)
+        assert (nl, al) == (len(dna), len(prot))
+        cigar = parse_cigar(cg)
+        g = nt4_codes(dna)
+        recs = [B64(ref.output(b"chr1", g, b"q1", prot, 0, 0, len(prot), 0, len(dna), 0, len(dna), cigar, score, o,
+                               flag=f)) for f in (0, 0x8)]
+        out.append({"seed": seed, "dna": B64(dna), "prot": B64(prot), "score": score, "cigar": cigar,
+                    "paf": recs[0], "gff": recs[1]})
     return out
 
 
@@ -335,7 +358,7 @@ def main():
             ("prescreen_batch_cases", prescreen_batch_cases), ("gapless_select_cases", gapless_select_cases),
             ("index_cases", index_cases), ("anchor_cases", anchor_cases),
             ("anchor_chain_cases", anchor_chain_cases),
-            ("output_cases", output_cases),
+            ("output_cases", output_cases), ("config0_cases", config0_cases),
             ("hmm_cases", hmm_cases)]
     want = set(sys.argv[1:])
     for name, fn in gens:
