@@ -265,7 +265,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		PrepProb &r = prep[k];
 		r.ns_off = l_ns, r.as_off = l_as, r.rec_off = l_ns, r.aa_off = l_as;
 		r.nl = q.nl, r.al = q.al, r.left = q.left, r.io = q.io, r.has_ss = q.ss != nullptr;
-		if (opt->pssm) e.pssm_off = l_pssm, l_pssm += (int64_t)q.al * asize;
+		if (q.pssm) e.pssm_off = l_pssm, l_pssm += (int64_t)q.al * asize;
 		l_ns += q.nl + 1, l_as += q.al + 1;
 		if (q.nl > max_nl) max_nl = q.nl;
 		p->max_nl = std::max(p->max_nl, q.nl), p->max_al = std::max(p->max_al, q.al);
@@ -282,7 +282,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		if (q.nl) std::memcpy(&h_ns[prep[k].ns_off], q.ns, q.nl);
 		std::memcpy(&h_as[prep[k].as_off], q.as, q.al);
 		if (q.ss && q.nl) std::memcpy(&h_ss[prep[k].ns_off], q.ss, q.nl);
-		if (opt->pssm) std::memcpy(&h_pssm[prob[k].pssm_off], opt->pssm, (size_t)q.al * asize);
+		if (q.pssm) std::memcpy(&h_pssm[prob[k].pssm_off], q.pssm, (size_t)q.al * asize);
 	}
 	pen.resize(max_nl + 2);
 	for (int32_t d = 0; d < (int32_t)pen.size(); ++d) pen[d] = nsh_ie_penalty(opt->ie_coef, d);
@@ -342,7 +342,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 			a.go = opt->go, a.ge = opt->ge, a.fs = opt->fs, a.xdrop = opt->xdrop, a.eb = opt->end_bonus;
 			a.counter = p->d_counter + g, a.out = p->d_out;
 			// persistent waves: about as many as the CUs keep resident
-			p->wpc[g] = opt->pssm? kGroupWavesPerCUPssm[g] : kGroupWavesPerCU[g];
+			p->wpc[g] = l_pssm? kGroupWavesPerCUPssm[g] : kGroupWavesPerCU[g];
 			p->grid[g] = (int32_t)std::min<int64_t>(a.n, (int64_t)ncu * p->wpc[g]);
 		}
 	}
@@ -370,6 +370,7 @@ extern "C" nsh_ext_plan_t *nsh_ext_plan_create(int32_t n, const char *ns, const 
 		q.left = (flag & NS_F_EXT_LEFT) != 0;
 		q.io = opt->io;
 		q.mode = global;
+		q.pssm = opt->pssm;
 		const int32_t w = nsh_host_stripe_width(global? 0 : flag, q.al);
 		q.npad = (q.al + w - 1) / w * w;
 	}

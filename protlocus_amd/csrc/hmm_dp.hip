@@ -12,6 +12,7 @@
 // maximum of cm and ci; one reduction per region gives the score. Work-groups are persistent
 // and take regions, longest first, from a queue.
 #include <hip/hip_runtime.h>
+#include <dlfcn.h>
 #include <algorithm>
 #include <chrono>
 #include <climits>
@@ -267,9 +268,11 @@ fail:
 
 // ---------------------------------------------------------------------------------------
 // the reference's GPU entry points (mppriv.h:128-134). The profile object and its accessors
-// belong to the caller's hmm.c; they are weak references here so the library also loads in
-// processes without it (the entry points then fail and the caller keeps its CPU path).
-// The device copy of a profile is cached in the caller's slot 0 and freed by
+// belong to the caller's hmm.c. They are weak references here, bound when the caller's program is
+// linked against this library (the linker then exports the caller's definitions); a caller that
+// loads its accessors after this library (a plugin, a test harness) is found by name in the
+// process's global scope instead. Without them the entry points fail and the caller keeps its CPU
+// path. The device copy of a profile is cached in the caller's slot 0 and freed by
 // mp_hmm_gpu_release() (called from mp_hmm_destroy).
 // ---------------------------------------------------------------------------------------
 typedef struct mp_hmm_s mp_hmm_t;   // the caller's profile type (hmm.c)
@@ -282,16 +285,45 @@ __attribute__((weak)) void **mp_hmm_gpu_slot(const mp_hmm_t *h, int which);
 
 static std::mutex g_slot_mu;
 
+namespace {
+struct HmmAccess {
+	int32_t (*length)(const mp_hmm_t*);
+	const int16_t *(*emissions)(const mp_hmm_t*);
+	const int16_t *(*transitions)(const mp_hmm_t*);
+	void **(*slot)(const mp_hmm_t*, int);
+};
+// the caller's accessors: the link-time references, else a lookup by name (once found, kept)
+bool hmm_access(HmmAccess &a)
+{
+	static HmmAccess found = {};
+	static std::mutex mu;
+	std::lock_guard<std::mutex> lk(mu);
+	if (!found.length) {
+		HmmAccess t = { mp_hmm_length, mp_hmm_emissions, mp_hmm_transitions, mp_hmm_gpu_slot };
+		if (!t.length || !t.emissions || !t.transitions || !t.slot) {
+			t.length = (int32_t (*)(const mp_hmm_t*))dlsym(RTLD_DEFAULT, "mp_hmm_length");
+			t.emissions = (const int16_t *(*)(const mp_hmm_t*))dlsym(RTLD_DEFAULT, "mp_hmm_emissions");
+			t.transitions = (const int16_t *(*)(const mp_hmm_t*))dlsym(RTLD_DEFAULT, "mp_hmm_transitions");
+			t.slot = (void **(*)(const mp_hmm_t*, int))dlsym(RTLD_DEFAULT, "mp_hmm_gpu_slot");
+		}
+		if (t.length && t.emissions && t.transitions && t.slot) found = t;
+	}
+	a = found;
+	return found.length != nullptr;
+}
+} // namespace
+
 static nsh_hmm_t *hmm_of(const mp_hmm_t *h)
 {
-	if (!mp_hmm_length || !mp_hmm_emissions || !mp_hmm_transitions || !mp_hmm_gpu_slot) {
+	HmmAccess acc;
+	if (!hmm_access(acc)) {
 		nsh_set_error("hmm: the caller's profile accessors (hmm.c) are not linked");
 		return nullptr;
 	}
-	void **slot = mp_hmm_gpu_slot(h, 0);
+	void **slot = acc.slot(h, 0);
 	if (!slot) return nullptr;
 	std::lock_guard<std::mutex> lk(g_slot_mu);
-	if (!*slot) *slot = nsh_hmm_create(mp_hmm_length(h), mp_hmm_emissions(h), mp_hmm_transitions(h));
+	if (!*slot) *slot = nsh_hmm_create(acc.length(h), acc.emissions(h), acc.transitions(h));
 	return (nsh_hmm_t*)*slot;
 }
 
@@ -312,8 +344,9 @@ extern "C" int32_t mp_hmm_dp_score_gpu(const mp_hmm_t *h, const uint8_t *nt, int
 
 extern "C" void mp_hmm_gpu_release(mp_hmm_t *h)
 {
-	if (!h || !mp_hmm_gpu_slot) return;
-	void **slot = mp_hmm_gpu_slot(h, 0);
+	HmmAccess acc;
+	if (!h || !hmm_access(acc)) return;
+	void **slot = acc.slot(h, 0);
 	if (!slot) return;
 	std::lock_guard<std::mutex> lk(g_slot_mu);
 	if (*slot) nsh_hmm_destroy((nsh_hmm_t*)*slot), *slot = nullptr;

@@ -5,9 +5,12 @@
 // 64-MB staging buffers instead: while one chunk crosses PCIe by DMA, the host copies the chunk
 // before it between the caller's buffer and the other staging buffer with several threads. The
 // staging buffers are allocated on the first large copy and kept for the process's lifetime
-// (128 MB of pinned memory); a mutex serialises their users.
+// (128 MB of pinned memory); a mutex serialises their users. When they cannot be allocated (or
+// NSH_NO_PINNED_STAGING=1 is set), a copy is one hipMemcpyAsync on the caller's stream followed by
+// a wait on that stream, so it stays ordered after the kernels queued there.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <thread>
@@ -25,6 +28,8 @@ void *g_pin[2] = {};
 
 bool ensure_pinned()
 {
+	const char *off = std::getenv("NSH_NO_PINNED_STAGING");   // read per copy, so a test can switch it
+	if (off && std::atoi(off)) return false;
 	for (int b = 0; b < 2; ++b)
 		if (!g_pin[b] && hipHostMalloc(&g_pin[b], kChunk, hipHostMallocDefault) != hipSuccess) {
 			g_pin[b] = nullptr;
@@ -47,21 +52,23 @@ void par_memcpy(void *dst, const void *src, size_t n)
 	for (auto &x : th) x.join();
 }
 
+// one copy on stream s, then a wait on s (the copy follows the kernels already queued on s)
+int plain_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s, const char *what)
+{
+	hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+	if (e == hipSuccess) e = hipStreamSynchronize(s);
+	if (e != hipSuccess) { nsh_set_error("%s of %zu bytes failed: %s", what, bytes, hipGetErrorString(e)); return 0; }
+	return 1;
+}
+
 } // namespace
 
 int nsh_copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t s)
 {
 	if (bytes == 0) return 1;
-	if (bytes < kDirect) {
-		if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s) != hipSuccess ||
-				hipStreamSynchronize(s) != hipSuccess) {
-			nsh_set_error("copy to host failed");
-			return 0;
-		}
-		return 1;
-	}
+	if (bytes < kDirect) return plain_copy(dst, src, bytes, hipMemcpyDeviceToHost, s, "copy to host");
 	std::lock_guard<std::mutex> lk(g_mu);
-	if (!ensure_pinned()) return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+	if (!ensure_pinned()) return plain_copy(dst, src, bytes, hipMemcpyDeviceToHost, s, "copy to host");
 	hipEvent_t ev[2] = {};
 	int ok = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) == hipSuccess &&
 			hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) == hipSuccess;
@@ -90,9 +97,13 @@ int nsh_copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t s)
 int nsh_copy_to_device(void *dst, const void *src, size_t bytes, hipStream_t s)
 {
 	if (bytes == 0) return 1;
-	if (bytes < kDirect) return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) == hipSuccess;
+	if (bytes < kDirect) {
+		const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+		if (e != hipSuccess) { nsh_set_error("copy to device of %zu bytes failed: %s", bytes, hipGetErrorString(e)); return 0; }
+		return 1;
+	}
 	std::lock_guard<std::mutex> lk(g_mu);
-	if (!ensure_pinned()) return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+	if (!ensure_pinned()) return plain_copy(dst, src, bytes, hipMemcpyHostToDevice, s, "copy to device");
 	hipEvent_t ev[2] = {};
 	int ok = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) == hipSuccess &&
 			hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) == hipSuccess;

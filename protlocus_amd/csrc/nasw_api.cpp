@@ -168,6 +168,7 @@ static int nsh_ext_one(const char *ns, int32_t nl, const char *as, int32_t al, c
 	q.ns = ns, q.nl = nl, q.as = as, q.al = al, q.ss = ss;
 	q.left = (opt->flag & NS_F_EXT_LEFT) != 0, q.io = opt->io;
 	q.mode = (opt->flag & (NS_F_EXT_LEFT | NS_F_EXT_RIGHT)) == 0;
+	q.pssm = opt->pssm;
 	const int32_t w = nsh_host_stripe_width(opt->flag, al);
 	q.npad = (al + w - 1) / w * w;
 	nsh_ext_plan_t *p = nsh_ext_plan_build(1, &q, opt);
@@ -232,7 +233,8 @@ extern "C" void ns_global_gs_auto(void *km, const char *ns, int32_t nl, const ch
 // CIGAR. Like the reference, a request is refused (NULL / 0) when nl < 3, al < 1, asize < 1 or
 // the best possible score al * max(sc) + end_bonus exceeds 30000 (the caller then takes the int32
 // path). The extension flags are ignored and the CIGAR is always produced; the splice-score track
-// and a PSSM are honoured, so the result equals ns_global_gs16b with NS_F_CIGAR. Each submission
+// is honoured and opt->pssm is ignored (the reference's kernel scores with opt->sc only,
+// nasw-cuda.cu:1068-1110), so the result equals ns_global_gs16b with NS_F_CIGAR and no PSSM. Each submission
 // is computed before submit returns and is consumed exactly once by ns_gpu_wait; the CIGAR is
 // allocated from km.
 struct ns_gpu_request_s { ns_rst_t rst; void *km; int ok; };
@@ -251,6 +253,7 @@ static int nsh_gpu_global_run(void *km, const char *ns, int32_t nl, const char *
 	if (!nsh_ensure_device() || !nsh_gpu_global_accepts(nl, al, opt)) return 0;
 	ns_opt_t o = *opt;
 	o.flag = (o.flag | NS_F_CIGAR) & ~(NS_F_GPU | NS_F_EXT_LEFT | NS_F_EXT_RIGHT);
+	o.pssm = nullptr;
 	if (!nsh_global_run(km, ns, nl, as, al, &o, ss, 0, r)) return 0;
 	++g_dispatch;
 	return 1;
@@ -289,8 +292,8 @@ extern "C" int ns_gpu_global(void *km, const char *ns, int32_t nl, const char *a
 
 // ---------------------------------------------------------------------------------------
 // batched extension API (nasw.h:196-212). A flush turns the requests added since the last
-// flush into one generation: requests with identical options (io and direction may differ)
-// share one device plan; the plans run asynchronously on their own streams.
+// flush into one generation: requests with identical options (io, direction and the PSSM rows
+// may differ) share one device plan; the plans run asynchronously on their own streams.
 // ---------------------------------------------------------------------------------------
 namespace {
 struct ExtGen {
@@ -306,7 +309,7 @@ bool opt_same(const ns_opt_t &a, const ns_opt_t &b)
 {
 	return a.go == b.go && a.ge == b.ge && a.fs == b.fs && a.xdrop == b.xdrop && a.end_bonus == b.end_bonus &&
 		a.asize == b.asize && a.ie_coef == b.ie_coef && a.sp_null_bonus == b.sp_null_bonus &&
-		std::memcmp(a.sp, b.sp, sizeof(a.sp)) == 0 && a.pssm == b.pssm &&
+		std::memcmp(a.sp, b.sp, sizeof(a.sp)) == 0 &&
 		(a.sc == b.sc || std::memcmp(a.sc, b.sc, (size_t)a.asize * a.asize) == 0) &&
 		(a.nt4 == b.nt4 || std::memcmp(a.nt4, b.nt4, 256) == 0) &&
 		(a.aa20 == b.aa20 || std::memcmp(a.aa20, b.aa20, 256) == 0) &&
@@ -398,6 +401,7 @@ extern "C" int32_t ns_ext_batch_flush(ns_ext_batch_t *b)
 			nsh_ext_req &q = req[k];
 			q.ns = x.ns, q.nl = x.nl, q.as = x.as, q.al = x.al, q.ss = x.ss;
 			q.left = (x.opt.flag & NS_F_EXT_LEFT) != 0, q.io = x.opt.io;
+			q.pssm = x.opt.pssm;   // per request: requests that differ only in their PSSM share a plan
 			const int32_t w = nsh_host_stripe_width(x.opt.flag, x.al);
 			q.npad = (x.al + w - 1) / w * w;
 		}

@@ -27,6 +27,7 @@ struct nsh_ext_req {
 	int32_t left, io;
 	int32_t npad;     // al rounded up to the reference CPU's stripe width
 	int32_t mode = 0; // 1: score-only global DP without extension flags (nasw-sse.c:459, 559)
+	const int8_t *pssm = nullptr; // al x asize PSSM rows in query order (ns_pssm_sc, nasw-sse.c:217-221); null: the matrix
 };
 nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_opt_t *opt);
 int nsh_ext_plan_query(nsh_ext_plan_t *p);   // 1 when the last run has completed

@@ -566,7 +566,7 @@ int search_chunk(nsh_prescreen_t *ps, const SearchIn &in, int32_t p0, int32_t p1
 	const int D = std::max(1, bits_for((uint64_t)maxL + (uint64_t)ps->max_orf_len));
 	const int O = std::max(1, bits_for((uint64_t)(ps->n_orf - 1)));
 	const int P = bits_for((uint64_t)(np - 1));
-	if (P + O + D > 63) return 2;
+	if (P + O + D > 63) { commit.on = false; return 2; }   // retried at half size: no counts from this attempt
 	int8_t *d_pssm, *d_sc;
 	uint8_t *d_ai, *d_in;
 	int64_t *d_prow, *d_wrow;

@@ -512,3 +512,80 @@ def cli_opt_kw(blosum62, flag=0):
     return dict(flag=flag, go=11, ge=1, io=29, fs=23, xdrop=100, end_bonus=5, sp_null_bonus=-7, ie_coef=.5,
                 gpu_min_cells=1000000, sp=sp, mat=cli_mat(blosum62))
 
+
+def gapless_closed_forms(codon):
+    """Candidates of the gapless X-drop scorer (nasw-cuda.cu:1182-1295) whose scores are fixed by
+    construction. The matrix scores +5 on identity and -20 elsewhere; exons are the query's residues
+    written with one codon each, spacers are W codons (the queries hold no W). k-mer 3, block width 1,
+    so each anchor's seed ends exactly at its target_block. Returns (name, genome codes, query codes,
+    [(query_length, [(target_end, query_end), ...])], kmer, block width, xdrop, matrix, [expected])."""
+    W = 17                                       # aa20 code of W
+    first = {}
+    for c in range(64):
+        first.setdefault(codon[c], c)
+    mat = np.full(22 * 22, -20, dtype=np.int8)
+    for a in range(22):
+        mat[a * 22 + a] = 5
+    rng = np.random.default_rng(31337)
+    alphabet = [a for a in range(20) if a != W]
+
+    def query(n):
+        return [int(x) for x in rng.choice(alphabet, size=n)]
+
+    def genome(blocks):
+        """blocks: lists of target residues, concatenated; -> base codes, codon index of each block"""
+        out, starts, at = [], [], 0
+        for b in blocks:
+            starts.append(at)
+            for r in b:
+                c = first[r]
+                out += [c >> 4, (c >> 2) & 3, c & 3]
+            at += len(b)
+        return np.array(out, dtype=np.uint8), starts
+
+    end = lambda codon_idx: 3 * codon_idx + 2    # last base of a codon
+    cases = []
+    # 1. two exons overlapping by 10 query residues on different diagonals: A = Q[0..59] (300),
+    #    B = Q[50..99] (250); the weaker is prorated: 300 + 250 * (50 - 10) / 50
+    q = query(110)
+    g, st = genome([q[0:60], [W] * 30, q[50:100], [W] * 30, q[100:106], [W] * 30])
+    a_A, a_B, a_C = (end(st[0] + 30), 30), (end(st[2] + 20), 70), (end(st[4] + 3), 103)
+    cases.append(("prorated overlap", g, q, [(100, [a_A, a_B])], 3, 1, 30, mat, [300 + 250 * 40 // 50]))
+    # 2. the same with a third exon Q[100..105] (score 30) at x-drop 30: a later segment counts only
+    #    when its score is not below the x-drop (strict <); with 5 residues (25) it is left out
+    g5, st5 = genome([q[0:60], [W] * 30, q[50:100], [W] * 30, q[100:105], [W] * 31])
+    a_C5 = (end(st5[4] + 3), 103)
+    cases.append(("third exon at the x-drop", g, q, [(110, [a_A, a_B, a_C])], 3, 1, 30, mat, [500 + 30]))
+    cases.append(("third exon below the x-drop", g5, q, [(110, [a_A, a_B, a_C5])], 3, 1, 30, mat, [500]))
+    # ... and once a segment falls below the x-drop nothing after it counts: the list is sorted by
+    #    score, so every later segment is weaker (the C segment appears twice here, both out)
+    cases.append(("segments after one below the x-drop", g5, q, [(110, [a_C5, a_A, a_C5, a_B])], 3, 1, 30, mat,
+                  [500]))
+    # 3. a segment covered by the sum of the overlaps with two better segments is dropped whole:
+    #    Q[45..64] overlaps A by 15 and B by 15 (30 >= 20)
+    gD, stD = genome([q[0:60], [W] * 30, q[50:100], [W] * 30, q[45:65], [W] * 30])
+    a_D = (end(stD[4] + 10), 55)
+    cases.append(("segment covered by two better ones", gD, q, [(100, [a_A, a_D, a_B])], 3, 1, 30, mat, [500]))
+    # 4. equal scores keep the anchor order: P = Q[0..11] (60, 12 residues), R = Q[5..21] with one
+    #    mismatch at Q[13] (16 * 5 - 20 = 60, 17 residues); they share Q[5..11] (7 residues), so the
+    #    one taken second is prorated: P first 60 + 60 * 10 / 17, R first 60 + 60 * 5 / 12
+    q4 = query(60)
+    r_mid = list(q4[5:22])
+    r_mid[13 - 5] = alphabet[(alphabet.index(q4[13]) + 1) % len(alphabet)]
+    g4, st4 = genome([q4[0:12], [W] * 10, r_mid, [W] * 10])
+    a_P, a_R = (end(st4[0] + 8), 8), (end(st4[2] + 15), 20)
+    cases.append(("equal scores, P first", g4, q4, [(60, [a_P, a_R])], 3, 1, 30, mat, [60 + 600 // 17]))
+    cases.append(("equal scores, R first", g4, q4, [(60, [a_R, a_P])], 3, 1, 30, mat, [60 + 300 // 12]))
+    # 5. the walk stops only when best - running exceeds the x-drop (strict >): two mismatches
+    #    (-40) inside an exact copy of Q[0..39]; at x-drop 40 the right walk crosses them
+    #    (15 + 8 * 5 + (9 * 5 - 40 + 18 * 5)), at 39 it stops before them (15 + 40 + 45)
+    q5 = query(40)
+    t5 = list(q5)
+    for i in (20, 21):
+        t5[i] = alphabet[(alphabet.index(q5[i]) + 3) % len(alphabet)]
+    g5b, st5b = genome([t5])
+    a5 = (end(10), 10)
+    cases.append(("x-drop equal continues", g5b, q5, [(40, [a5])], 3, 1, 40, mat, [15 + 40 + 95]))
+    cases.append(("x-drop exceeded stops", g5b, q5, [(40, [a5])], 3, 1, 39, mat, [15 + 40 + 45]))
+    return cases
+

@@ -20,6 +20,9 @@ Outputs (inputs and expected outputs only):
   anchor_chain_cases.json.gz  mp_chain (chain.c) on sorted anchors: chains and their anchors
   config0_cases.json.gz  single-pair workload (300 aa vs ~2.4-3 kb, 500-nt flanks) under the CLI option set:
                      ns_global_gs_auto(NS_F_CIGAR) and the PAF / GFF3 records of its alignment
+  dropin_cases.json.gz  the reference's own program (oracle/_ref/protlocus, its CPU build) on the inputs
+                     of its integration tests (dropin/: DPP3 and rescue) under several option sets:
+                     the expected PAF / GFF3 of the drop-in check
   tables.json        ns_tab_nt4 / ns_tab_aa20 / ns_tab_codon / ns_mat_blosum62
 """
 import base64
@@ -176,6 +179,30 @@ def config0_cases(ref):
                                flag=f)) for f in (0, 0x8)]
         out.append({"seed": seed, "dna": B64(dna), "prot": B64(prot), "score": score, "cigar": cigar,
                     "paf": recs[0], "gff": recs[1]})
+    return out
+
+
+# option sets of the drop-in check (test/test_integration.py:33-87 uses these inputs and options)
+DROPIN_RUNS = [("DPP3", []), ("DPP3", ["--gff"]), ("DPP3", ["-t", "4"]), ("DPP3", ["-c0", "-u"]),
+               ("DPP3", ["--accuracy"]), ("DPP3", ["--accuracy", "--acc-no-vid-chain", "--acc-no-rescue",
+                                                   "--acc-no-rerank", "--acc-no-junc-refine", "-c0", "-u"]),
+               ("rescue", ["-M0", "-n2", "-u"]),
+               ("rescue", ["--accuracy", "--acc-no-repeat", "--acc-no-vid-chain", "--acc-no-rerank",
+                           "--acc-no-junc-refine", "-M0", "-n2", "-u"])]
+DROPIN_INPUTS = {"DPP3": ("DPP3-hs.gen.fa.gz", "DPP3-mm.pep.fa.gz"), "rescue": ("rescue.gen.fa", "rescue.pep.fa")}
+
+
+def dropin_cases(ref):
+    """stdout of the reference's CPU program for each option set of DROPIN_RUNS"""
+    import subprocess
+    prog = os.path.join(ROOT, "oracle", "_ref", "protlocus")
+    if not os.path.exists(prog):
+        raise SystemExit("build the reference's program first: make -C oracle cpu-prog")
+    out = []
+    for name, args in DROPIN_RUNS:
+        g, p = (os.path.join(HERE, "dropin", f) for f in DROPIN_INPUTS[name])
+        r = subprocess.run([prog] + args + [g, p], stdout=subprocess.PIPE, stderr=subprocess.PIPE, check=True)
+        out.append({"input": name, "args": args, "stdout": B64(r.stdout)})
     return out
 
 
@@ -358,7 +385,7 @@ def main():
             ("prescreen_batch_cases", prescreen_batch_cases), ("gapless_select_cases", gapless_select_cases),
             ("index_cases", index_cases), ("anchor_cases", anchor_cases),
             ("anchor_chain_cases", anchor_chain_cases),
-            ("output_cases", output_cases), ("config0_cases", config0_cases),
+            ("output_cases", output_cases), ("config0_cases", config0_cases), ("dropin_cases", dropin_cases),
             ("hmm_cases", hmm_cases)]
     want = set(sys.argv[1:])
     for name, fn in gens:

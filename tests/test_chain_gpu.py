@@ -141,6 +141,29 @@ def test_chain_large_staged(product):
         assert np.array_equal(a, qs[q]), q
 
 
+def test_copies_without_pinned_staging(product, monkeypatch):
+    """NSH_NO_PINNED_STAGING=1 makes the large host copies plain copies on the entry point's stream
+    followed by a wait on it: the chain batch of 72 MB each way and the index of a 40-Mb genome
+    must come out as they do through the pinned staging buffers"""
+    from protlocus_amd import capi
+    m, nq = 9001, 1000
+    i = np.arange(m, dtype=np.uint64)
+    qs = [((np.uint64(3) * i + np.uint64(q * 50000)) << np.uint64(32)) | (i + np.uint64(q)) for q in range(nq)]
+    rng = np.random.default_rng(8)
+    g = rng.integers(0, 4, size=40_000_000, dtype=np.uint8)
+    staged = capi.index_build(g, [0, 25_000_000], [25_000_000, 15_000_000], 30, 6, 1, 8)
+    monkeypatch.setenv("NSH_NO_PINNED_STAGING", "1")
+    print("chain batch, unstaged", flush=True)
+    got = capi.chain_batch(qs, 100, 100, 100, 25, 5000, 2, 0, 0.75, 1, 1, 0)
+    for q, (u, a) in enumerate(got):
+        assert u.tolist() == [m << 32 | m], q
+        assert np.array_equal(a, qs[q]), q
+    print("index build, unstaged", flush=True)
+    plain = capi.index_build(g, [0, 25_000_000], [25_000_000, 15_000_000], 30, 6, 1, 8)
+    for x, y in zip(staged, plain):
+        assert np.array_equal(x, y)
+
+
 def test_index_chain_edges(product):
     """nsh_index_chain: a null index or options are refused; no queries give no chains; empty and
     3-residue queries give none either"""

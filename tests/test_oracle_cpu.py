@@ -10,7 +10,8 @@ What each oracle function is pinned by:
                                   gs16b == gs32b when unsaturated (test/test_dp32.c:7-31);
                                   CIGAR spans (nl, al) and its re-scored value equals the score
   or_hmm_parse / or_hmm_viterbi   golden vectors; one-state closed form; brute force on tiny inputs
-  or_gapless_score                closed forms and brute force (no reference CPU form exists)
+  or_gapless_score                constructed candidates with exact scores for every segment rule
+                                  (no reference CPU form exists), closed forms, brute-force seeds
   or_pssm_ungapped                brute force over all segments of the diagonal
 """
 import base64
@@ -260,7 +261,7 @@ def test_gapless_closed_forms(oracle):
     """a query equal to the translation of the target: the seed sits on the true diagonal and both
     walks run to the ends, so the score is the sum of the BLOSUM62 diagonal over the query; on the
     reverse strand the same holds for the reverse complement; a second anchor on the same segment
-    is fully overlapped and adds nothing; two disjoint exons add up"""
+    is fully overlapped and adds nothing"""
     from cases import oriented
     rng = np.random.default_rng(1)
     aa = "ARNDCQEGHILKMFPSTWYV"
@@ -288,16 +289,34 @@ def test_gapless_closed_forms(oracle):
                              anchor_offset=0, anchor_count=1, reverse=0)
     assert _gapless_one(oracle, g, q, [cand], [nsffi.GaplessAnchor(target_block=0, query_end=2)], bw, kmer, 50) == \
         [-2 ** 31]
-    # two exons: query halves planted 600 nt apart, one anchor each; after the first segment only
-    # segments scoring >= xdrop count, so xdrop is kept below the half scores
-    g2 = np.concatenate([g[:150], rng.integers(0, 4, size=600).astype(np.uint8), g[150:]])
-    cand = nsffi.GaplessCand(target_offset=0, target_length=len(g2), query_offset=0, query_length=100,
-                             anchor_offset=0, anchor_count=2, reverse=0)
-    an = [nsffi.GaplessAnchor(target_block=3 * 20 + 2, query_end=20),
-          nsffi.GaplessAnchor(target_block=600 + 3 * 70 + 2, query_end=70)]
-    s = _gapless_one(oracle, g2, q, [cand], an, 1, kmer, 30)[0]
-    assert abs(s - diag) <= 60   # each walk ends a little past the exon edge, inside the random spacer
-    assert _gapless_one(oracle, g2, q, [cand], an, 1, kmer, 10 ** 6)[0] < diag * 3 // 4   # one segment only
+
+
+def _closed_form_scores(score_fn, codon):
+    """score every constructed candidate of cases.gapless_closed_forms with score_fn(genome codes,
+    query codes, [GaplessCand], [GaplessAnchor], bw, kmer, xdrop, matrix)"""
+    from cases import gapless_closed_forms
+    out = []
+    for name, g, q, cands, kmer, bw, xdrop, mat, exp in gapless_closed_forms(codon):
+        cs, ans = [], []
+        for qlen, anchors in cands:
+            cs.append(nsffi.GaplessCand(target_offset=0, target_length=len(g), query_offset=0, query_length=qlen,
+                                        anchor_offset=len(ans), anchor_count=len(anchors), reverse=0))
+            ans += [nsffi.GaplessAnchor(target_block=te, query_end=qe) for te, qe in anchors]
+        out.append((name, score_fn(g, q, cs, ans, bw, kmer, xdrop, mat), exp))
+    return out
+
+
+def test_gapless_segment_rules_closed_forms(oracle):
+    """exact scores of constructed candidates (cases.gapless_closed_forms): two overlapping exons,
+    the weaker prorated as score * (len - overlap) / len (nasw-cuda.cu:1284-1290); later segments
+    only when not below the x-drop (1283); a segment covered by the overlaps with two better ones
+    dropped whole; equal-score segments in anchor order; the walk's strict > x-drop stop"""
+    res = _closed_form_scores(lambda g, q, cs, ans, bw, kmer, xdrop, mat:
+                              _gapless_one(oracle, g, q, cs, ans, bw, kmer, xdrop, mat), oracle.codon)
+    for name, got, exp in res:
+        assert got == exp, name
+    names = [r[0] for r in res]
+    assert len(names) == 9 and res[names.index("equal scores, P first")][2] != res[names.index("equal scores, R first")][2]
 
 
 def test_gapless_brute_seed_choice(oracle):

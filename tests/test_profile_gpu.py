@@ -117,6 +117,27 @@ def test_gapless_batch(oracle, product, seed, kmer, bw, xdrop):
     assert prof.batches >= 1 and prof.candidates >= len(cands)
 
 
+def test_gapless_closed_form_rules(oracle, product):
+    """the constructed candidates whose scores are fixed by construction (cases.gapless_closed_forms:
+    proration of an overlapped segment, the x-drop rule for later segments, a covered segment
+    dropped, equal scores in anchor order, the strict x-drop stop of the walks), on the GPU"""
+    from cases import pack4
+    from test_oracle_cpu import _closed_form_scores
+    L = product.lib
+
+    def gpu(g, q, cs, ans, bw, kmer, xdrop, mat):
+        CA = (nsffi.GaplessCand * len(cs))(*cs)
+        AN = (nsffi.GaplessAnchor * len(ans))(*ans)
+        out = np.zeros(len(cs), dtype=np.int32)
+        L.ns_gpu_gapless_reset_target()
+        ok = L.ns_gpu_gapless_score_batch(pack4(g), len(g), bytes(q), len(q), CA, len(cs), AN, len(ans), bw, kmer,
+                                          xdrop, mat.ctypes.data, 22, oracle.codon, out.ctypes.data)
+        assert ok, product.error()
+        return out.tolist()
+    for name, got, exp in _closed_form_scores(gpu, oracle.codon):
+        assert got == exp, name
+
+
 def test_gapless_rejects_out_of_range(oracle, product):
     from cases import pack4
     g = np.zeros(100, dtype=np.uint8)
