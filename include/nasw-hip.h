@@ -251,6 +251,20 @@ float nsh_ext_plan_last_ms(nsh_ext_plan_t *p, int32_t which);  /* 0: whole run, 
 int nsh_ext_plan_autotune(nsh_ext_plan_t *p, int32_t reps, int32_t *waves_per_cu);
 void nsh_ext_plan_destroy(nsh_ext_plan_t *p);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): nsh_ext_plan_create + run +
+ * results in one call that calls no hipMalloc/hipFree. Arguments as nsh_ext_plan_create (ss and a
+ * PSSM are not supported), results as nsh_ext_plan_results (input order); the work is queued on
+ * `stream` (a hipStream_t; NULL = the null stream) and waited for before returning. Device memory
+ * comes from one fixed 128-MB array per GPU held until the process ends; limits: 16,384 problems,
+ * DNA <= 262,143 nt and padded protein <= 2,048 per problem, and the inputs' device footprint
+ * (about 5 B per DNA base + 2 B per residue + 144 B per problem) within 128 MB; beyond them it
+ * returns 0 with the reason in ns_gpu_error(). Calls on one GPU share the array: they are
+ * serialised by the library, and two processes' calls do not share it. 1 ok, 0 error. */
+int nsh_ext_run_noalloc(int32_t n, const char *ns, const int64_t *ns_off, const int32_t *nl,
+		const char *as, const int64_t *as_off, const int32_t *al, const uint8_t *dirs, const ns_opt_t *opt,
+		nsh_ext_result_t *out, void *stream);
+
 /* Profile-HMM engine behind mp_hmm_dp_score_gpu*: a profile resident in HBM (emissions M x 20 in
  * HMMER column order, best residue at +4, transitions M x 7 m->m m->i m->d i->m i->i d->m d->d, as
  * mp_hmm_read1 stores them, hmm.c:112-195). Scores equal mp_hmm_dp_score (hmm.c:252-291). M <= 1024. */

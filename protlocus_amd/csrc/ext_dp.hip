@@ -29,6 +29,7 @@
 #include <cstdio>
 #include <cstring>
 #include <type_traits>
+#include <mutex>
 #include <vector>
 #include "nsh_internal.h"
 #include "ext_dp_types.h"
@@ -547,6 +548,159 @@ extern "C" int nsh_ext_plan_autotune(nsh_ext_plan_t *p, int32_t reps, int32_t *w
 		p->grid[g] = (int32_t)std::min<int64_t>(n, (int64_t)p->ncu * best_w);
 	}
 	if (waves_per_cu) for (int g = 0; g < kGroups; ++g) waves_per_cu[g] = wpc[g];
+	return 1;
+}
+
+// ======================================================================================
+// nsh_ext_run_noalloc: nsh_ext_plan_create + run + results in one call that allocates no device
+// memory. Every device buffer of the run (the raw inputs, the row records, descriptors, the
+// length-penalty table, queue counters, results) is carved out of one file-scope device array
+// (`scratch`, kNoAllocScratch bytes per GPU, held until the process ends); host-side descriptors
+// live in file-scope host arrays. Calls serialise on a host mutex; the device array keeps the last
+// call's contents, and everything a run reads is written by that run (the queue counters are
+// cleared by a fill kernel, not a memset, so the launches can also be captured in a graph).
+// ======================================================================================
+namespace {
+constexpr size_t kNoAllocScratch = (size_t)128 << 20;   // device scratch per GPU
+constexpr int32_t kNoAllocMaxProbs = 16384;               // problems per call
+constexpr int32_t kNoAllocMaxNl = 262143;                 // DNA length per problem
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratch];
+
+std::mutex g_na_mu;
+constexpr int kMaxDev = 64;
+uint8_t *g_na_ptr[kMaxDev] = {};       // device address of `scratch` on each GPU
+ExtProb g_na_prob[kNoAllocMaxProbs];
+PrepProb g_na_prep[kNoAllocMaxProbs];
+int32_t g_na_order[kNoAllocMaxProbs];
+int32_t g_na_out[4 * kNoAllocMaxProbs];
+int32_t g_na_pen[kNoAllocMaxNl + 2];
+PrepConst g_na_pc;
+size_t na_align(size_t x) { return (x + 255) & ~(size_t)255; }
+} // namespace
+
+extern "C" int nsh_ext_run_noalloc(int32_t n, const char *ns, const int64_t *ns_off, const int32_t *nl,
+		const char *as, const int64_t *as_off, const int32_t *al, const uint8_t *dirs, const ns_opt_t *opt,
+		nsh_ext_result_t *out, void *stream_)
+{
+	hipStream_t stream = (hipStream_t)stream_;
+	if (n < 1 || !ns || !ns_off || !nl || !as || !as_off || !al || !opt || !out) {
+		nsh_set_error("nsh_ext_run_noalloc: bad arguments"); return 0;
+	}
+	if (n > kNoAllocMaxProbs) { nsh_set_error("nsh_ext_run_noalloc: %d problems; the limit is %d", n, kNoAllocMaxProbs); return 0; }
+	if (!nsh_ensure_device()) return 0;
+	if (opt->asize != 22) { nsh_set_error("asize %d not supported (22 expected)", opt->asize); return 0; }
+	if (opt->pssm) { nsh_set_error("nsh_ext_run_noalloc: PSSM rows are not supported (matrix scores only)"); return 0; }
+	if (opt->ge < 0 || opt->go < 0 || (int64_t)opt->ge * CMAX * 64 > 32767) {
+		nsh_set_error("gap penalties go=%d ge=%d out of the supported range", opt->go, opt->ge); return 0;
+	}
+	for (int t = 0; t < 64; ++t)
+		if (opt->codon[t] >= 22) { nsh_set_error("codon table entry %d out of range", t); return 0; }
+	for (int t = 0; t < 256; ++t)
+		if (opt->aa20[t] >= 22 || opt->nt4[t] > 4) { nsh_set_error("alphabet table entry %d out of range", t); return 0; }
+	std::lock_guard<std::mutex> lk(g_na_mu);
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) { nsh_set_error("nsh_ext_run_noalloc: no current device"); return 0; }
+	if (!g_na_ptr[dev]) {   // once per device, with that device current
+		void *ptr = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !ptr) { nsh_set_error("hipGetSymbolAddress(scratch): %s", hipGetErrorString(e)); return 0; }
+		g_na_ptr[dev] = (uint8_t*)ptr;
+	}
+	uint8_t *base = g_na_ptr[dev];
+	const bool global = !dirs && !(opt->flag & (NS_F_EXT_LEFT | NS_F_EXT_RIGHT));
+	int64_t span_ns = 0, span_as = 0, l_rec = 0, l_aa = 0;
+	int32_t max_nl = 2, n_grp[kGroups] = {};
+	for (int32_t k = 0; k < n; ++k) {
+		if (al[k] < 1 || nl[k] < 0 || ns_off[k] < 0 || as_off[k] < 0) {
+			nsh_set_error("nsh_ext_run_noalloc: problem %d: nl=%d al=%d", k, nl[k], al[k]); return 0;
+		}
+		if (nl[k] > kNoAllocMaxNl) { nsh_set_error("nsh_ext_run_noalloc: problem %d: nl=%d exceeds %d", k, nl[k], kNoAllocMaxNl); return 0; }
+		const int32_t flag = dirs? (dirs[k]? NS_F_EXT_LEFT : NS_F_EXT_RIGHT) : (opt->flag & (NS_F_EXT_LEFT | NS_F_EXT_RIGHT));
+		const int32_t w = nsh_host_stripe_width(global? 0 : flag, al[k]);
+		const int32_t npad = (al[k] + w - 1) / w * w;
+		const int32_t C = std::max(1, (npad + 127) / 128);
+		if (C > CMAX) { nsh_set_error("nsh_ext_run_noalloc: problem %d: padded protein length %d exceeds %d", k, npad, 128 * CMAX); return 0; }
+		ExtProb &e = g_na_prob[k];
+		e.nl = nl[k], e.al = al[k], e.npad = npad, e.io = opt->io, e.out = k, e.C = C;
+		e.left = (flag & NS_F_EXT_LEFT) != 0, e.mode = global;
+		e.rec_off = l_rec, e.aa_off = l_aa, e.pssm_off = -1;
+		PrepProb &r = g_na_prep[k];
+		r.ns_off = ns_off[k], r.as_off = as_off[k], r.rec_off = l_rec, r.aa_off = l_aa;
+		r.nl = nl[k], r.al = al[k], r.left = e.left, r.io = opt->io, r.has_ss = 0;
+		l_rec += nl[k] + 1, l_aa += al[k] + 1;
+		span_ns = std::max(span_ns, ns_off[k] + nl[k]), span_as = std::max(span_as, as_off[k] + al[k]);
+		max_nl = std::max(max_nl, nl[k]);
+		++n_grp[group_of(C)];
+	}
+	// scratch layout
+	size_t o = 0;
+	const size_t o_ns = o; o = na_align(o + span_ns + 1);
+	const size_t o_as = o; o = na_align(o + span_as + 1);
+	const size_t o_aas = o; o = na_align(o + l_aa);
+	const size_t o_rec = o; o = na_align(o + (size_t)l_rec * 4);
+	const size_t o_mat = o; o = na_align(o + 22 * 22);
+	const size_t o_pen = o; o = na_align(o + (size_t)(max_nl + 2) * 4);
+	const size_t o_cnt = o; o = na_align(o + kGroups * 4);
+	const size_t o_out = o; o = na_align(o + (size_t)n * 16);
+	const size_t o_prob = o; o = na_align(o + (size_t)n * sizeof(ExtProb));
+	const size_t o_prep = o; o = na_align(o + (size_t)n * sizeof(PrepProb));
+	const size_t o_pc = o; o = na_align(o + sizeof(PrepConst));
+	if (o > kNoAllocScratch) {
+		nsh_set_error("nsh_ext_run_noalloc: the inputs need %zu bytes of device scratch; the limit is %zu", o, kNoAllocScratch);
+		return 0;
+	}
+	for (int32_t d = 0; d < max_nl + 2; ++d) g_na_pen[d] = nsh_ie_penalty(opt->ie_coef, d);
+	std::memcpy(g_na_pc.nt4, opt->nt4, 256), std::memcpy(g_na_pc.aa20, opt->aa20, 256), std::memcpy(g_na_pc.codon, opt->codon, 64);
+	for (int t = 0; t < 6; ++t) g_na_pc.sp[t] = opt->sp[t];
+	g_na_pc.sp_null = opt->sp_null_bonus;
+	// queue order as in a plan: by column-count group, longest work first (index breaks ties)
+	for (int32_t k = 0; k < n; ++k) g_na_order[k] = k;
+	std::sort(g_na_order, g_na_order + n, [](int32_t x, int32_t y) {
+		const ExtProb &a = g_na_prob[x], &b = g_na_prob[y];
+		const int ga = group_of(a.C), gb = group_of(b.C);
+		if (ga != gb) return ga < gb;
+		const int64_t ca = (int64_t)a.nl * a.C, cb = (int64_t)b.nl * b.C;
+		return ca != cb? ca > cb : x < y; });
+	ExtProb *d_prob = (ExtProb*)(base + o_prob);
+	int ok = 1;
+	auto cp = [&](void *dst, const void *src, size_t bytes) {
+		if (ok && bytes && hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream) != hipSuccess) ok = 0;
+	};
+	cp(base + o_ns, ns, (size_t)span_ns);
+	cp(base + o_as, as, (size_t)span_as);
+	cp(base + o_mat, opt->sc, 22 * 22);
+	cp(base + o_pen, g_na_pen, (size_t)(max_nl + 2) * 4);
+	cp(base + o_prep, g_na_prep, (size_t)n * sizeof(PrepProb));
+	cp(base + o_pc, &g_na_pc, sizeof(PrepConst));
+	for (int32_t k = 0; k < n; ++k) cp(d_prob + k, &g_na_prob[g_na_order[k]], sizeof(ExtProb));
+	if (!ok) { nsh_set_error("nsh_ext_run_noalloc: copy to the device failed"); return 0; }
+	nsh_prep::launch((const PrepProb*)(base + o_prep), n, base + o_ns, nullptr, base + o_as, (uint32_t*)(base + o_rec),
+			base + o_aas, (const PrepConst*)(base + o_pc), stream);
+	int32_t *d_cnt = (int32_t*)(base + o_cnt);
+	hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, stream, d_cnt, kGroups, 0);
+	int ncu = 0;
+	(void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+	if (ncu <= 0) ncu = 256;
+	int64_t first = 0;
+	for (int g = 0; g < kGroups; ++g) {
+		ExtArgs a = {};
+		a.prob = d_prob + first, a.n = n_grp[g];
+		first += n_grp[g];
+		if (a.n == 0) continue;
+		a.rec = (const uint32_t*)(base + o_rec), a.aas = base + o_aas, a.pssm = nullptr, a.mat = (const int8_t*)(base + o_mat);
+		a.asize = 22, a.pen = (const int32_t*)(base + o_pen), a.pen_n = max_nl + 2;
+		a.go = opt->go, a.ge = opt->ge, a.fs = opt->fs, a.xdrop = opt->xdrop, a.eb = opt->end_bonus;
+		a.counter = d_cnt + g, a.out = (int32_t*)(base + o_out);
+		const int grid = (int)std::min<int64_t>(a.n, (int64_t)ncu * kGroupWavesPerCU[g]);
+		launch_group(g, a, grid, stream);   // one stream: the groups run one after the other
+	}
+	hipError_t e = hipGetLastError();
+	if (e == hipSuccess) e = hipMemcpyAsync(g_na_out, base + o_out, (size_t)n * 16, hipMemcpyDeviceToHost, stream);
+	if (e == hipSuccess) e = hipStreamSynchronize(stream);
+	if (e != hipSuccess) { nsh_set_error("nsh_ext_run_noalloc: %s", hipGetErrorString(e)); return 0; }
+	nsh_count_launch(2 + kGroups);
+	for (int32_t k = 0; k < n; ++k)
+		out[k].nt_len = g_na_out[4 * k], out[k].aa_len = g_na_out[4 * k + 1], out[k].score = g_na_out[4 * k + 2];
 	return 1;
 }
 
