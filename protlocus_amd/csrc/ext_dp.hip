@@ -33,6 +33,7 @@
 #include <vector>
 #include "nsh_internal.h"
 #include "ext_dp_types.h"
+#include "global_dp_types.h"
 #include "prep.h"
 
 namespace nsh_ext {
@@ -211,6 +212,14 @@ struct nsh_ext_plan_s {
 	int32_t max_al = 0, max_nl = 0;
 	std::vector<ExtProb> prob_h;             // descriptors in queue order (host copy)
 	int32_t grp_first[kGroups] = {}, grp_n[kGroups] = {};
+	// problems wider than 128 * kCmax padded columns: the work-group-per-problem kernel in extension
+	// mode (global_dp_impl.h), in buckets of equal shape, on their own stream
+	std::vector<nsh_glob::GlobProb> wide_h;
+	std::vector<nsh_glob::GlobProb> wide_sub_h;
+	nsh_glob::GlobProb *d_wide = nullptr, *d_wide_sub = nullptr;
+	hipStream_t ws = nullptr;
+	hipEvent_t wev = nullptr;
+	nsh_glob::GlobArgs wargs = {};
 	// subset runs and features
 	ExtProb *d_prob_sub = nullptr;
 	std::vector<ExtProb> sub_h;
@@ -222,7 +231,7 @@ struct nsh_ext_plan_s {
 static void plan_free_dev(nsh_ext_plan_t *p)
 {
 	void *bufs[] = { p->d_ns, p->d_ss, p->d_as, p->d_aas, p->d_rec, p->d_mat, p->d_pssm, p->d_pen, p->d_counter,
-		p->d_out, p->d_prob, p->d_prep, p->d_pc, p->d_prob_sub, p->d_idx_sub, p->d_feat };
+		p->d_out, p->d_prob, p->d_prep, p->d_pc, p->d_prob_sub, p->d_idx_sub, p->d_feat, p->d_wide, p->d_wide_sub };
 	for (void *b : bufs) if (b) (void)hipFree(b);
 }
 
@@ -254,12 +263,17 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 	if (opt->ge < 0 || opt->go < 0 || (int64_t)opt->ge * CMAX * 64 > 32767) {
 		nsh_set_error("gap penalties go=%d ge=%d out of the supported range", opt->go, opt->ge); delete p; return nullptr;
 	}
+	std::vector<char> is_wide(n, 0);
 	for (int32_t k = 0; k < n; ++k) {
 		const nsh_ext_req &q = req[k];
 		int32_t C = (q.npad + 127) / 128;
 		if (q.al < 1 || q.nl < 0 || q.npad < q.al) { nsh_set_error("request %d: nl=%d al=%d npad=%d", k, q.nl, q.al, q.npad); delete p; return nullptr; }
-		if (C > CMAX) { nsh_set_error("request %d: padded protein length %d exceeds %d", k, q.npad, 128 * CMAX); delete p; return nullptr; }
+		if (q.npad > 64 * nsh_glob::kCMaxG * nsh_glob::kNWMax) {
+			nsh_set_error("request %d: padded protein length %d exceeds %d", k, q.npad, 64 * nsh_glob::kCMaxG * nsh_glob::kNWMax);
+			delete p; return nullptr;
+		}
 		if (C < 1) C = 1;
+		is_wide[k] = C > CMAX;
 		ExtProb &e = prob[k];
 		e.nl = q.nl, e.al = q.al, e.npad = q.npad, e.io = q.io, e.out = k, e.C = C, e.left = q.left, e.mode = q.mode;
 		e.rec_off = l_ns, e.aa_off = l_as, e.pssm_off = -1;
@@ -271,7 +285,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		if (q.nl > max_nl) max_nl = q.nl;
 		p->max_nl = std::max(p->max_nl, q.nl), p->max_al = std::max(p->max_al, q.al);
 		any_ss |= q.ss != nullptr;
-		++n_grp[group_of(C)];
+		if (!is_wide[k]) ++n_grp[group_of(C)];
 		p->al[k] = q.al;
 		p->cells_window += (double)q.nl * q.al;
 	}
@@ -290,17 +304,32 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 	std::memcpy(pc.nt4, opt->nt4, 256), std::memcpy(pc.aa20, opt->aa20, 256), std::memcpy(pc.codon, opt->codon, 64);
 	for (int t = 0; t < 6; ++t) pc.sp[t] = opt->sp[t];
 	pc.sp_null = opt->sp_null_bonus;
+	// wide problems: descriptors of the work-group kernel, in buckets of equal shape
+	for (int32_t k = 0; k < n; ++k) {
+		if (!is_wide[k]) continue;
+		const ExtProb &e = prob[k];
+		nsh_glob::GlobProb g = {};
+		g.rec_off = e.rec_off, g.aa_off = e.aa_off, g.tr_off = 0, g.pssm_off = e.pssm_off;
+		g.nl = e.nl, g.al = e.al, g.io = e.io, g.out = e.out, g.left = e.left, g.npad = e.npad, g.mode = e.mode;
+		g.slen = 1;
+		nsh_glob::shape(e.npad, g.C, g.NW);
+		p->wide_h.push_back(g);
+	}
+	std::stable_sort(p->wide_h.begin(), p->wide_h.end(), [](const nsh_glob::GlobProb &x, const nsh_glob::GlobProb &y) {
+		return x.NW != y.NW? x.NW > y.NW : x.C > y.C; });
 	// queue order: small-C problems first (their launch), then large; longest work first in each
-	for (int32_t k = 0; k < n; ++k) order[k] = k;
+	order.clear();
+	for (int32_t k = 0; k < n; ++k) if (!is_wide[k]) order.push_back(k);
 	std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
 		const int bx = group_of(prob[x].C), by = group_of(prob[y].C);
 		if (bx != by) return bx < by;
 		return (int64_t)prob[x].nl * prob[x].C > (int64_t)prob[y].nl * prob[y].C; });
 	{
-		std::vector<ExtProb> sorted(n);
-		for (int32_t k = 0; k < n; ++k) sorted[k] = prob[order[k]];
+		std::vector<ExtProb> sorted(order.size());
+		for (size_t k = 0; k < order.size(); ++k) sorted[k] = prob[order[k]];
 		prob.swap(sorted);
 	}
+	const int32_t n_narrow = (int32_t)prob.size();
 	p->prob_h = prob;
 	p->out_h.assign((size_t)4 * n, 0);
 	HIPOK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
@@ -315,14 +344,21 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 	HIPOK(dalloc(&p->d_mat, (size_t)asize * asize));
 	if (l_pssm) HIPOK(dalloc(&p->d_pssm, l_pssm));
 	HIPOK(dalloc(&p->d_pen, pen.size())); HIPOK(dalloc(&p->d_counter, kGroups)); HIPOK(dalloc(&p->d_out, (size_t)4 * n));
-	HIPOK(dalloc(&p->d_prob, n)); HIPOK(dalloc(&p->d_prep, n)); HIPOK(dalloc(&p->d_pc, 1));
+	HIPOK(dalloc(&p->d_prob, std::max(1, n_narrow))); HIPOK(dalloc(&p->d_prep, n)); HIPOK(dalloc(&p->d_pc, 1));
+	if (!p->wide_h.empty()) {
+		HIPOK(dalloc(&p->d_wide, p->wide_h.size()));
+		HIPOK(hipStreamCreateWithFlags(&p->ws, hipStreamNonBlocking));
+		HIPOK(hipEventCreateWithFlags(&p->wev, hipEventDisableTiming));
+	}
 	HIPOK(hipMemcpyAsync(p->d_ns, h_ns.data(), l_ns, hipMemcpyHostToDevice, p->stream));
 	HIPOK(hipMemcpyAsync(p->d_as, h_as.data(), l_as, hipMemcpyHostToDevice, p->stream));
 	if (any_ss) HIPOK(hipMemcpyAsync(p->d_ss, h_ss.data(), l_ns, hipMemcpyHostToDevice, p->stream));
 	HIPOK(hipMemcpyAsync(p->d_mat, opt->sc, (size_t)asize * asize, hipMemcpyHostToDevice, p->stream));
 	if (l_pssm) HIPOK(hipMemcpyAsync(p->d_pssm, h_pssm.data(), l_pssm, hipMemcpyHostToDevice, p->stream));
 	HIPOK(hipMemcpyAsync(p->d_pen, pen.data(), pen.size() * 4, hipMemcpyHostToDevice, p->stream));
-	HIPOK(hipMemcpyAsync(p->d_prob, prob.data(), (size_t)n * sizeof(ExtProb), hipMemcpyHostToDevice, p->stream));
+	if (n_narrow) HIPOK(hipMemcpyAsync(p->d_prob, prob.data(), (size_t)n_narrow * sizeof(ExtProb), hipMemcpyHostToDevice, p->stream));
+	if (!p->wide_h.empty())
+		HIPOK(hipMemcpyAsync(p->d_wide, p->wide_h.data(), p->wide_h.size() * sizeof(nsh_glob::GlobProb), hipMemcpyHostToDevice, p->stream));
 	HIPOK(hipMemcpyAsync(p->d_prep, prep.data(), (size_t)n * sizeof(PrepProb), hipMemcpyHostToDevice, p->stream));
 	HIPOK(hipMemcpyAsync(p->d_pc, &pc, sizeof(pc), hipMemcpyHostToDevice, p->stream));
 	HIPOK(hipStreamSynchronize(p->stream));
@@ -346,6 +382,10 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 			p->wpc[g] = l_pssm? kGroupWavesPerCUPssm[g] : kGroupWavesPerCU[g];
 			p->grid[g] = (int32_t)std::min<int64_t>(a.n, (int64_t)ncu * p->wpc[g]);
 		}
+		nsh_glob::GlobArgs &w = p->wargs;
+		w.rec = p->d_rec, w.aas = p->d_aas, w.pssm = p->d_pssm, w.mat = p->d_mat, w.asize = asize;
+		w.go = opt->go, w.ge = opt->ge, w.fs = opt->fs, w.trace = nullptr, w.score = nullptr;
+		w.pen = p->d_pen, w.pen_n = (int32_t)pen.size(), w.xdrop = opt->xdrop, w.eb = opt->end_bonus, w.ext_out = p->d_out;
 	}
 	return p;
 fail:
@@ -385,9 +425,31 @@ __global__ void clear_out_kernel(int32_t *out, int32_t n)
 	if (i < n) out[4 * i] = 0, out[4 * i + 1] = 0, out[4 * i + 2] = INT_MIN, out[4 * i + 3] = 0;
 }
 
+// the wide problems' buckets (runs of equal NW and C in list) on the plan's wide stream
+static int launch_wide(nsh_ext_plan_t *p, const std::vector<nsh_glob::GlobProb> &list, const nsh_glob::GlobProb *d_list)
+{
+	if (list.empty()) return 1;
+	HIPOK(hipStreamWaitEvent(p->ws, p->ev1, 0));
+	for (size_t k = 0; k < list.size();) {
+		size_t e = k;
+		while (e < list.size() && list[e].NW == list[k].NW && list[e].C == list[k].C) ++e;
+		nsh_glob::GlobArgs a = p->wargs;
+		a.prob = d_list + k;
+		nsh_glob::launch_ext16(a, (int)(e - k), list[k].NW, list[k].C, p->ws);
+		HIPOK(hipGetLastError());
+		nsh_count_launch(1);
+		k = e;
+	}
+	HIPOK(hipEventRecord(p->wev, p->ws));
+	HIPOK(hipStreamWaitEvent(p->stream, p->wev, 0));
+	return 1;
+fail:
+	return 0;
+}
+
 // idx/n_idx: prepare only these inputs (subset runs); nullptr prepares all p->n
 static int plan_launch(nsh_ext_plan_t *p, const ExtArgs *args, const int32_t *grid, bool clear_out, const int32_t *idx,
-		int32_t n_idx)
+		int32_t n_idx, const std::vector<nsh_glob::GlobProb> *wide, const nsh_glob::GlobProb *d_wide)
 {
 	HIPOK(hipEventRecord(p->ev0, p->stream));
 	if (n_idx > 0)
@@ -406,6 +468,7 @@ static int plan_launch(nsh_ext_plan_t *p, const ExtArgs *args, const int32_t *gr
 		HIPOK(hipStreamWaitEvent(p->stream, p->gev[g], 0));
 		nsh_count_launch(1);
 	}
+	if (wide && !launch_wide(p, *wide, d_wide)) goto fail;
 	HIPOK(hipEventRecord(p->ev2, p->stream));
 	nsh_count_launch(2);
 	p->ran = 1;
@@ -416,7 +479,7 @@ fail:
 
 extern "C" int nsh_ext_plan_run(nsh_ext_plan_t *p)
 {
-	return p? plan_launch(p, p->args, p->grid, false, nullptr, p->n) : 0;
+	return p? plan_launch(p, p->args, p->grid, false, nullptr, p->n, &p->wide_h, p->d_wide) : 0;
 }
 
 extern "C" int nsh_ext_plan_run_subset(nsh_ext_plan_t *p, const uint8_t *keep)
@@ -438,11 +501,17 @@ extern "C" int nsh_ext_plan_run_subset(nsh_ext_plan_t *p, const uint8_t *keep)
 		args[g].prob = p->d_prob_sub + first, args[g].n = (int32_t)p->sub_h.size() - first;
 		grid[g] = std::min(args[g].n, p->ncu * p->wpc[g]);
 	}
-	if (!p->sub_h.empty()) {
+	p->wide_sub_h.clear();
+	for (const nsh_glob::GlobProb &g : p->wide_h) if (keep[g.out]) p->wide_sub_h.push_back(g);
+	if (!p->wide_sub_h.empty() && !p->d_wide_sub) HIPOK(dalloc(&p->d_wide_sub, p->wide_h.size()));
+	if (!p->sub_h.empty())
 		HIPOK(hipMemcpyAsync(p->d_prob_sub, p->sub_h.data(), p->sub_h.size() * sizeof(ExtProb), hipMemcpyHostToDevice, p->stream));
+	if (!p->idx_h.empty())
 		HIPOK(hipMemcpyAsync(p->d_idx_sub, p->idx_h.data(), p->idx_h.size() * sizeof(int32_t), hipMemcpyHostToDevice, p->stream));
-	}
-	return plan_launch(p, args, grid, true, p->d_idx_sub, (int32_t)p->idx_h.size());
+	if (!p->wide_sub_h.empty())
+		HIPOK(hipMemcpyAsync(p->d_wide_sub, p->wide_sub_h.data(), p->wide_sub_h.size() * sizeof(nsh_glob::GlobProb),
+				hipMemcpyHostToDevice, p->stream));
+	return plan_launch(p, args, grid, true, p->d_idx_sub, (int32_t)p->idx_h.size(), &p->wide_sub_h, p->d_wide_sub);
 fail:
 	return 0;
 }
@@ -516,7 +585,7 @@ extern "C" int nsh_ext_plan_autotune(nsh_ext_plan_t *p, int32_t reps, int32_t *w
 	auto time_run = [&](float *best) -> int {
 		*best = 1e30f;
 		for (int r = 0; r < reps; ++r) {
-			if (!plan_launch(p, p->args, p->grid, false, nullptr, p->n)) return 0;
+			if (!plan_launch(p, p->args, p->grid, false, nullptr, p->n, &p->wide_h, p->d_wide)) return 0;
 			if (hipEventSynchronize(p->ev2) != hipSuccess) { nsh_set_error("nsh_ext_plan_autotune: run failed"); return 0; }
 			float ms = 0;
 			if (hipEventElapsedTime(&ms, p->ev1, p->ev2) != hipSuccess) return 0;
@@ -716,6 +785,8 @@ extern "C" void nsh_ext_plan_destroy(nsh_ext_plan_t *p)
 		if (p->gev[g]) (void)hipEventDestroy(p->gev[g]);
 		if (p->gs[g]) (void)hipStreamDestroy(p->gs[g]);
 	}
+	if (p->wev) (void)hipEventDestroy(p->wev);
+	if (p->ws) (void)hipStreamDestroy(p->ws);
 	if (p->stream) (void)hipStreamDestroy(p->stream);
 	delete p;
 }

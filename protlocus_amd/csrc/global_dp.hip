@@ -1,522 +1,515 @@
-// Global spliced DP with CIGAR on gfx950 (ns_global_gs16b / ns_global_gs32b:
-// nasw-sse.c:351-567 and 779-973; backtrack nasw-sse.c:44-93).
+// Global spliced DP engine (ns_global_gs16b / ns_global_gs32b with NS_F_CIGAR, the int32
+// score-only DP, ns_gpu_submit / ns_gpu_wait; reference semantics nasw-sse.c:351-567, 779-973,
+// backtrack 44-93; the reference's batching front end nasw-cuda.cu:419-455, 827-1066).
 //
-// The reference's trace words depend on its striped SIMD evaluation order: with vsize lanes
-// (8 for int16, 4 for int32) and slen = ceil(al/vsize), column c belongs to segment c/slen,
-// the first pass restricts the insertion to the segment (reset at its start), and a lazy-F pass
-// carries insertions across segments. This kernel evaluates the same quantities row by row:
-//   * phase A, per cell, independent across columns: the nine non-insertion candidates, their
-//     first strict maximum (order D, A, B, C, F1, F2, G1, G2 after M) and the extension bits 5-8;
-//   * the segment-local first-pass insertion I_fp (reset at every segment start) and the exact
-//     insertion I_ex, each a chain inside the lane plus a max-plus scan across lanes and waves
-//     (the I_fp scan is segmented: a lane holding a segment start stops the carry);
-//   * H_fp = max(non-insertion, I_fp) gives bits 0-4 exactly as the first pass does; the lazy-F
-//     pass raises a cell iff I_ex > H_fp (bit 9 for int16; for int32 the word is overwritten
-//     with 1 | ext, ext taken from the segment that delivered the carry), and H = max(., I_ex).
-// A work-group of NW <= 16 waves solves one problem (NW*64 lanes, C <= 4 columns per lane); waves
-// exchange their scan totals and boundary columns through LDS with two barriers per row.
-// Values live in 32-bit registers with the reference's arithmetic (int16 saturating or int32
-// wrapping). Trace words go to HBM, one uint16 per real cell; the backtrack walks them on the host.
+// Requests are queued (inputs copied at submission) and run in batches by one executor thread:
+//   * batching: the executor takes everything queued once a caller waits on a queued request, or
+//     after a 2-ms window, or at 4096 requests; concurrent callers (kt_for workers) therefore share
+//     batches, and a submit-all-then-wait caller (mp_align_gap) gets one batch for all its gaps;
+//   * a batch is grouped by options (gap, frameshift, splice and matrix terms; io, the PSSM rows
+//     and the strand are per request), then cut so that its trace fits the trace budget;
+//   * inputs travel as one block through a pinned staging buffer; the device preparation kernel
+//     writes the row records; problems are sorted into buckets of equal shape (waves per
+//     work-group, columns per lane; global_dp_impl.h) launched on four streams;
+//   * the trace (one uint16 word per cell, the reference's bit layout) stays in HBM: a backtrack
+//     kernel walks it, one lane per problem, writes each CIGAR (reversed, tiny U/V turned into G,
+//     nasw-sse.c:34-42) at an offset reserved with one atomic, and only the CIGARs cross PCIe;
+//   * device and pinned buffers are grow-only pools owned by the executor, so a batch allocates
+//     nothing once the pools have reached the workload's size.
+// The submitting thread copies the CIGAR into the caller's km (kalloc arenas are per thread).
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <chrono>
 #include <climits>
+#include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <mutex>
-#include <type_traits>
+#include <thread>
 #include <vector>
 #include "nsh_internal.h"
 #include "prep.h"
+#include "global_dp_types.h"
+
+using namespace nsh_glob;
 
 namespace {
 
-#define DEV static __device__ __forceinline__
-
-// arithmetic of the kernel being matched
-struct Arith16 {
-	static constexpr int NEG = -32768;
-	static constexpr bool IS32 = false;
-	DEV int add(int a, int b) { return min(max(a + b, -32768), 32767); }
-	DEV int sub(int a, int b) { return min(max(a - b, -32768), 32767); }
-};
-struct Arith32 {
-	static constexpr int NEG = -0x40000000;
-	static constexpr bool IS32 = true;
-	DEV int add(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
-	DEV int sub(int a, int b) { return (int)((unsigned)a - (unsigned)b); }
-};
-
-enum : int { ROW_SHR1 = 0x111, ROW_SHR2 = 0x112, ROW_SHR4 = 0x114, ROW_SHR8 = 0x118,
-	WAVE_SHR1 = 0x138, ROW_BCAST15 = 0x142, ROW_BCAST31 = 0x143 };
-template <int CTRL> DEV int dppk(int x) { return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xf, 0xf, false); }
-
-constexpr int CMAXG = 4, NWMAX = 16;      // al <= 64 * 4 * 16 = 4096
-
-struct GlobProb {
-	int64_t rec_off, aa_off, tr_off, pssm_off;
-	int32_t nl, al, io, out;
-	int32_t C, NW, slen, left;
-};
-struct GlobArgs {
-	const GlobProb *prob;
-	const uint32_t *rec; const uint8_t *aas; const int8_t *pssm; const int8_t *mat; int32_t asize;
-	int32_t go, ge, fs;
-	uint16_t *trace;        // per problem: nl rows x al columns (rows 0, 1 unused)
-	int32_t *score;         // per problem: H(nl-1, al-1) (nasw-sse.c:559/962)
-};
-
-// LDS exchange area of a work-group (besides the profile)
-struct Xch {
-	int nbr[2][NWMAX];            // H of the wave's last column, by row parity
-	int lfp[2][NWMAX][2];         // I_fp, H_fp of the wave's last column, by row parity
-	int tex[NWMAX], tfp[NWMAX], ffp[NWMAX];
-	int seg[8];                   // int32: O and flag of segments 0..2
-};
-
-// in-wave max-plus scan, decay d per lane: S(L) = max_{L'<=L} X(L') - d (L - L')
-template <class AR>
-DEV int scan_maxplus(int S, int d, int L)
-{
-	S = max(S, AR::sub(dppk<ROW_SHR1>(S), d));
-	S = max(S, AR::sub(dppk<ROW_SHR2>(S), 2 * d));
-	S = max(S, AR::sub(dppk<ROW_SHR4>(S), 4 * d));
-	S = max(S, AR::sub(dppk<ROW_SHR8>(S), 8 * d));
-	{ const int t = dppk<ROW_BCAST15>(S); if (L & 16) S = max(S, AR::sub(t, d * ((L & 15) + 1))); }
-	{ const int t = dppk<ROW_BCAST31>(S); if (L >= 32) S = max(S, AR::sub(t, d * (L - 31))); }
-	return S;
-}
-// segmented: F(L) = 1 when lane L holds a segment start; the carry does not cross it. On return
-// F(L) = 1 when some lane in [0, L] holds one.
-template <class AR>
-DEV int scan_maxplus_seg(int S, int &F, int d, int L)
-{
-#define NSH_SEG_STEP(CTRL, DIST, COND) { \
-		const int t = dppk<CTRL>(S), tf = dppk<CTRL>(F); \
-		if ((COND) && !F) S = max(S, AR::sub(t, (DIST))); \
-		if (COND) F |= tf; }
-	NSH_SEG_STEP(ROW_SHR1, d, (L & 15) >= 1)
-	NSH_SEG_STEP(ROW_SHR2, 2 * d, (L & 15) >= 2)
-	NSH_SEG_STEP(ROW_SHR4, 4 * d, (L & 15) >= 4)
-	NSH_SEG_STEP(ROW_SHR8, 8 * d, (L & 15) >= 8)
-	NSH_SEG_STEP(ROW_BCAST15, d * ((L & 15) + 1), (L & 16) != 0)
-	NSH_SEG_STEP(ROW_BCAST31, d * (L - 31), L >= 32)
-#undef NSH_SEG_STEP
-	return S;
-}
-
-template <class AR, int C, bool TRACE>
-DEV void glob_solve(const GlobArgs &a, const GlobProb &P, int8_t *prof, Xch &X)
-{
-	const int NEG = AR::NEG;
-	const int L = threadIdx.x & 63, w = threadIdx.x >> 6, NW = P.NW, G = threadIdx.x;
-	const int nl = P.nl, al = P.al, slen = P.slen, asz = a.asize;
-	const int go = a.go, ge = a.ge, fs = a.fs, io = P.io, goe = go + ge;
-	const uint8_t *aas = a.aas + P.aa_off;
-	const int8_t *pssm = P.pssm_off >= 0? a.pssm + P.pssm_off : nullptr;
-	const uint32_t *rec = a.rec + P.rec_off;
-	const int W = 64 * C * NW;                   // columns covered by the work-group
-	// column constants
-	int jpos[C];
-	bool valid[C];
-#pragma unroll
-	for (int k = 0; k < C; ++k) { const int c = G * C + k; valid[k] = c < al; jpos[k] = c % slen; }
-	int lane_reset = 0;
-#pragma unroll
-	for (int k = 0; k < C; ++k) lane_reset |= valid[k] && jpos[k] == 0;
-	// int8 profile in LDS: prof[r * W + c]
-#pragma unroll
-	for (int k = 0; k < C; ++k) {
-		const int c = G * C + k;
-		const int aa = valid[k]? aas[c] : 0;
-		for (int r = 0; r < 22; ++r)
-			prof[r * W + c] = valid[k]? (pssm? pssm[(P.left? al - 1 - c : c) * asz + r] : a.mat[r * asz + aa]) : 0;
-	}
-	__syncthreads();
-
-	int Hs[3][C], Ds[3][C], A[C], B[C], Cc[C], nh[3];
-#pragma unroll
-	for (int k = 0; k < C; ++k) {
-		Hs[0][k] = Hs[1][k] = Hs[2][k] = Ds[0][k] = Ds[1][k] = Ds[2][k] = NEG;
-		A[k] = B[k] = Cc[k] = NEG;
-	}
-	// left neighbour of the lane's first column, rows i-3 (nh[0]), i-2 (nh[1]), i-1 (nh[2]) at row 2:
-	// column -1 holds H(-1,-1)=0 and H(0,-1)=H(1,-1)=-fs when row 2 is computed
-	nh[0] = nh[1] = nh[2] = NEG;
-	if (G == 0) nh[0] = 0, nh[1] = nh[2] = -fs;
-
-	int recv = 0, recn = 0;
-	auto load_blk = [&](int i0) { const int i = i0 + L; return i < nl? (int)rec[i] : 0; };
-	recv = load_blk(0);
-	if (nl > 64) recn = load_blk(64);
-	int d0 = 0, dp1 = 0, am1 = 0, a0 = 0, dm1 = 0, am2 = 0;
-	if (nl >= 3) {
-		const int r0 = __builtin_amdgcn_readlane(recv, 0), r1 = __builtin_amdgcn_readlane(recv, 1);
-		d0 = (int)(int8_t)(r0 >> 8), dp1 = (int)(int8_t)(r1 >> 8);
-		am1 = (int)(int8_t)(r0 >> 16), a0 = (int)(int8_t)(r1 >> 16);
-	}
-	// the I_fp a segment starts with: I = last_h = neg_inf -> sub(max(sub(NEG, go), NEG), ge)
-	const int fp_start = AR::sub(max(AR::sub(NEG, go), NEG), ge);
-	const int geC = ge * C, geW = ge * C * 64;
-	uint16_t *tr = TRACE? a.trace + P.tr_off : nullptr;
-	// the first cell of lane 0 of waves w > 0 is finished one row later (its bit 4 needs the
-	// previous wave's last column): pending word, its fire flag and column
-	int z0_pend = 0, f0_pend = 0, r0_pend = 0;
-	int i = 2;
-
-	auto row = [&](auto Rc) {
-		constexpr int R = decltype(Rc)::value, R2 = (R + 1) % 3, R1 = (R + 2) % 3;
-		int *H3 = Hs[R], *H2 = Hs[R2], *H1 = Hs[R1], *D3 = Ds[R];
-		__syncthreads();   // B1: last row's boundary values are in LDS
-		if (L == 0 && w > 0 && i > 2) {
-			nh[R1] = X.nbr[(i - 1) & 1][w - 1];
-			// finish the previous row's first word of this wave
-			if (TRACE) {
-				const int ipl = X.lfp[(i - 1) & 1][w - 1][0], hpl = X.lfp[(i - 1) & 1][w - 1][1];
-				const bool reset0 = valid[0] && jpos[0] == 0;
-				if (!(AR::IS32 && f0_pend) && !reset0 && ipl > AR::sub(hpl, go)) z0_pend |= 1 << 4;
-				if (valid[0]) tr[(int64_t)r0_pend * al + G * C] = (uint16_t)z0_pend;
-			}
-		}
-		if ((i & 63) == 0) { recv = recn; if (i + 64 < nl) recn = load_blk(i + 64); }
-		const int rc = __builtin_amdgcn_readlane(recv, i & 63);
-		const int res = rc & 0xff;
-		dm1 = d0, d0 = dp1, dp1 = (int)(int8_t)(rc >> 8);
-		am2 = am1, am1 = a0, a0 = (int)(int8_t)(rc >> 16);
-		const int gei = res == 20? fs : ge;
-		const int8_t *pr = prof + res * W + G * C;
-		const int n1 = nh[R1], n2 = nh[R2], n3 = nh[R];
-
-		// phase A
-		int Mv[C], rest[C], yrest[C], bits[C], tgo[C];
-#pragma unroll
-		for (int k = 0; k < C; ++k) {
-			const int h3p = k? H3[k - 1] : n3, h1p = k? H1[k - 1] : n1, h2p = k? H2[k - 1] : n2;
-			int zz = 0, t, u;
-			Mv[k] = AR::add(h3p, (int)pr[k]);
-			u = AR::sub(H3[k], go);
-			if (D3[k] > u) zz |= 1 << 5;
-			const int dn = AR::sub(max(u, D3[k]), gei);
-			t = AR::sub(AR::sub(H1[k], io), dm1);
-			if (A[k] > t) zz |= 1 << 6;
-			A[k] = max(t, A[k]);
-			const int av = AR::sub(A[k], a0);
-			u = AR::sub(h1p, io);
-			t = AR::sub(u, d0);
-			if (B[k] > t) zz |= 1 << 7;
-			B[k] = max(t, B[k]);
-			const int bv = AR::sub(B[k], am2);
-			t = AR::sub(u, dp1);
-			if (Cc[k] > t) zz |= 1 << 8;
-			Cc[k] = max(t, Cc[k]);
-			const int cv = AR::sub(Cc[k], am1);
-			const int f6 = AR::sub(H1[k], fs), f7 = AR::sub(H2[k], fs), f8 = AR::sub(h1p, fs), f9 = AR::sub(h2p, fs);
-			int best = dn, y = 2;
-			if (av > best) best = av, y = 3;
-			if (bv > best) best = bv, y = 4;
-			if (cv > best) best = cv, y = 5;
-			if (f6 > best) best = f6, y = 6;
-			if (f7 > best) best = f7, y = 7;
-			if (f8 > best) best = f8, y = 8;
-			if (f9 > best) best = f9, y = 9;
-			rest[k] = best, yrest[k] = y, bits[k] = zz;
-			tgo[k] = AR::sub(max(Mv[k], best), go);
-			D3[k] = dn;
-		}
-		// local chains with no carry, the lane's outgoing insertions
-		int Iex = NEG, Ifp = NEG;
-#pragma unroll
-		for (int k = 1; k < C; ++k) {
-			Iex = AR::sub(max(tgo[k - 1], Iex), ge);
-			Ifp = (valid[k] && jpos[k] == 0)? fp_start : AR::sub(max(tgo[k - 1], Ifp), ge);
-		}
-		int Sex = scan_maxplus<AR>(AR::sub(max(tgo[C - 1], Iex), ge), geC, L);
-		int Fl = lane_reset;
-		int Sfp = scan_maxplus_seg<AR>(AR::sub(max(tgo[C - 1], Ifp), ge), Fl, geC, L);
-		if (L == 63) X.tex[w] = Sex, X.tfp[w] = Sfp, X.ffp[w] = Fl;
-		__syncthreads();   // B2: wave totals
-		// carries entering this wave (the column before it; column -1 for wave 0)
-		int cw_ex = fp_start, cw_fp = fp_start;
-		for (int v = 0; v < w; ++v) {
-			cw_ex = max(X.tex[v], AR::sub(cw_ex, geW));
-			cw_fp = X.ffp[v]? X.tfp[v] : max(X.tfp[v], AR::sub(cw_fp, geW));
-		}
-		int cin_ex = dppk<WAVE_SHR1>(Sex), cin_fp = dppk<WAVE_SHR1>(Sfp), fx = dppk<WAVE_SHR1>(Fl);
-		if (L == 0) cin_ex = cw_ex, cin_fp = cw_fp;
-		else {
-			cin_ex = max(cin_ex, AR::sub(cw_ex, geC * L));
-			if (!fx) cin_fp = max(cin_fp, AR::sub(cw_fp, geC * L));
-		}
-		if (AR::IS32) {
-			// first-pass outgoing of segments 0..2 for the segment-start words of lazy-F
-			int ifp = cin_fp;
-#pragma unroll
-			for (int k = 0; k < C; ++k) {
-				if (k) ifp = AR::sub(max(tgo[k - 1], ifp), ge);
-				if (valid[k] && jpos[k] == 0) ifp = fp_start;
-				const int c = G * C + k;
-				if (valid[k] && jpos[k] == slen - 1 && c / slen < 3) {
-					const int hfp = max(Mv[k], max(ifp, rest[k]));
-					const int ho = AR::sub(hfp, goe), ie = AR::sub(ifp, ge);
-					X.seg[2 * (c / slen)] = max(ho, ie), X.seg[2 * (c / slen) + 1] = ie > ho;
-				}
-			}
-			__syncthreads();
-		}
-		// final pass
-		int z[C], Hf[C], fires0 = 0;
-		int iex = cin_ex, ifp = cin_fp, hfp_prev = NEG, ifp_prev = NEG;
-#pragma unroll
-		for (int k = 0; k < C; ++k) {
-			if (k) {
-				iex = AR::sub(max(tgo[k - 1], iex), ge);
-				ifp = AR::sub(max(tgo[k - 1], ifp), ge);
-			}
-			const bool reset = valid[k] && jpos[k] == 0;
-			if (reset) ifp = fp_start;
-			int zz = bits[k];
-			if (reset) { if (AR::IS32) zz |= 1 << 4; }
-			else if (k && ifp_prev > AR::sub(hfp_prev, go)) zz |= 1 << 4;
-			const int m = Mv[k], r = rest[k];
-			const int hfp = max(m, max(ifp, r));
-			zz |= (m >= ifp && m >= r)? 0 : (ifp >= r? 1 : yrest[k]);
-			const bool fire = iex > hfp;
-			if (!AR::IS32) { if (fire) zz |= 1 << 9; }
-			else if (fire) {
-				int ext = 16;
-				if (reset) {   // segment start: which lazy-F iteration delivered the carry
-					const int sm = (G * C + k) / slen;
-					int F = NEG;
-					for (int q = 2; q <= sm; ++q) F = AR::sub(max(q == 2? NEG : F, X.seg[2 * (q - 2)]), ge * slen);
-					ext = (sm == 1 || X.seg[2 * (sm - 1)] >= F)? (X.seg[2 * (sm - 1) + 1]? 16 : 0) : 16;
-				}
-				zz = 1 | ext;
-			}
-			if (k == 0) fires0 = fire;
-			z[k] = zz;
-			Hf[k] = max(hfp, iex);
-			hfp_prev = hfp, ifp_prev = ifp;
-		}
-		// bit 4 of slot 0: previous lane's last column (inside the wave)
-		{
-			const int ipl = dppk<WAVE_SHR1>(ifp_prev), hpl = dppk<WAVE_SHR1>(hfp_prev);
-			const bool reset0 = valid[0] && jpos[0] == 0;
-			if (L > 0 && !(AR::IS32 && fires0) && !reset0 && ipl > AR::sub(hpl, go)) z[0] |= 1 << 4;
-		}
-		if (TRACE) {
-			uint16_t *dst = tr + (int64_t)i * al + G * C;
-#pragma unroll
-			for (int k = 0; k < C; ++k)
-				if (valid[k] && (k || L || w == 0)) dst[k] = (uint16_t)z[k];
-			if (L == 0 && w > 0) z0_pend = z[0], f0_pend = fires0, r0_pend = i;
-		}
-		if (L == 63) {
-			X.nbr[i & 1][w] = Hf[C - 1];
-			X.lfp[i & 1][w][0] = ifp_prev, X.lfp[i & 1][w][1] = hfp_prev;
-		}
-#pragma unroll
-		for (int k = 0; k < C; ++k) H3[k] = Hf[k];
-		const int Vs = dppk<WAVE_SHR1>(Hf[C - 1]);
-		nh[R] = L == 0? NEG : Vs;                 // lane 0 of waves > 0 is filled from LDS next row
-		if (i == 2 && G == 0) nh[R2] = NEG, nh[R1] = NEG;
-		++i;
-	};
-	if (nl > 2) {
-		for (;;) {
-			row(std::integral_constant<int, 0>()); if (i >= nl) break;
-			row(std::integral_constant<int, 1>()); if (i >= nl) break;
-			row(std::integral_constant<int, 2>()); if (i >= nl) break;
-		}
-		__syncthreads();
-		if (TRACE && L == 0 && w > 0) {
-			const int ipl = X.lfp[(i - 1) & 1][w - 1][0], hpl = X.lfp[(i - 1) & 1][w - 1][1];
-			const bool reset0 = valid[0] && jpos[0] == 0;
-			if (!(AR::IS32 && f0_pend) && !reset0 && ipl > AR::sub(hpl, go)) z0_pend |= 1 << 4;
-			if (valid[0]) tr[(int64_t)r0_pend * al + G * C] = (uint16_t)z0_pend;
-		}
-	}
-	// score = H(nl-1, al-1); the last row computed sits in Hs[(nl-3) % 3]
-	const int last = nl > 2? (nl - 3) % 3 : -1;
-	const int ge_ = (al - 1) / C, ke = (al - 1) % C;
-	if (G == ge_) {
-		int sc = NEG;
-#pragma unroll
-		for (int k = 0; k < C; ++k)
-			if (k == ke) sc = last == 0? Hs[0][k] : last == 1? Hs[1][k] : last == 2? Hs[2][k] : NEG;
-		a.score[P.out] = sc;
-	}
-}
-
-// MAXT: the work-group size bound (512 keeps the largest column count free of spills; 1024 serves
-// proteins longer than 2048)
-template <class AR, bool TRACE, int MAXT>
-__global__ void __launch_bounds__(MAXT) glob_kernel(GlobArgs a)
-{
-	extern __shared__ int8_t prof[];
-	__shared__ Xch X;
-	const GlobProb P = a.prob[blockIdx.x];
-	switch (P.C) {
-	case 1: glob_solve<AR, 1, TRACE>(a, P, prof, X); break;
-	case 2: glob_solve<AR, 2, TRACE>(a, P, prof, X); break;
-	case 3: glob_solve<AR, 3, TRACE>(a, P, prof, X); break;
-	default: glob_solve<AR, 4, TRACE>(a, P, prof, X); break;
-	}
-}
-
 // ---------------------------------------------------------------------------------------
-// host side
+// device backtrack: one lane per problem walks its trace words from (nl-1, al-1)
 // ---------------------------------------------------------------------------------------
-#define HIPOK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { nsh_set_error("%s: %s", #x, hipGetErrorString(e_)); goto fail; } } while (0)
+struct BtProb {
+	int64_t tr_off, tmp_off;   // trace words; scratch for the ops in push order
+	int32_t nl, al;
+};
 
-// nasw-sse.c:34-42
-void fix_tiny_uv(int32_t n, uint32_t *cigar)
+__device__ __forceinline__ void bt_push(uint32_t *t, int32_t &n, uint32_t op, int32_t len)
 {
-	for (int32_t k = 0; k < n; ++k) {
-		const int32_t op = cigar[k] & 0xf;
-		if ((op == NS_CIGAR_U || op == NS_CIGAR_V) && cigar[k] >> 4 < 3) cigar[k] = cigar[k] >> 4 << 4 | NS_CIGAR_G;
-	}
+	// ns_push_cigar (nasw.h:219-229): F and G are never merged
+	if (n == 0 || op != (t[n - 1] & 0xf) || op == NS_CIGAR_F || op == NS_CIGAR_G) t[n++] = (uint32_t)len << 4 | op;
+	else t[n - 1] += (uint32_t)len << 4;
 }
 
-// walk the trace words from (nl-1, al-1) (nasw-sse.c:44-93); tr[i*al + j]
-void backtrack(void *km, const uint16_t *tr, int32_t nl, int32_t al, ns_rst_t *r)
+__global__ void __launch_bounds__(64) backtrack_kernel(const BtProb *bp, int32_t n, const uint16_t *trace,
+		uint32_t *tmp, uint32_t *packed, unsigned long long *total, int64_t *meta)
 {
-	int32_t i = nl - 1, j = al - 1, last = 0;
-	uint32_t *cigar = r->cigar;
-	while (i >= 2 && j >= 0) {
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n) return;
+	const BtProb P = bp[k];
+	const uint16_t *tr = trace + P.tr_off;
+	uint32_t *t = tmp + P.tmp_off;
+	int32_t i = P.nl - 1, j = P.al - 1, last = 0, nc = 0;
+	const int64_t al = P.al;
+	while (i >= 2 && j >= 0) {   // nasw-sse.c:44-93
 		int32_t x = tr[(int64_t)i * al + j], state, ext;
 		if (x >> 9 & 1) x = 1 | x >> 4 << 4;
 		state = last == 0? x & 0xf : last;
 		ext = state >= 1 && state <= 5? x >> (state + 3) & 1 : 0;
 		switch (state) {
-		case 0: cigar = ns_push_cigar(km, &r->n_cigar, &r->m_cigar, cigar, NS_CIGAR_M, 1), i -= 3, --j; break;
-		case 1: cigar = ns_push_cigar(km, &r->n_cigar, &r->m_cigar, cigar, NS_CIGAR_I, 1), --j; break;
-		case 2: cigar = ns_push_cigar(km, &r->n_cigar, &r->m_cigar, cigar, NS_CIGAR_D, 1), i -= 3; break;
-		case 3: cigar = ns_push_cigar(km, &r->n_cigar, &r->m_cigar, cigar, NS_CIGAR_N, 1), --i; break;
-		case 4: cigar = ns_push_cigar(km, &r->n_cigar, &r->m_cigar, cigar, NS_CIGAR_U, 1), --i; if (!ext) --j; break;
-		case 5: cigar = ns_push_cigar(km, &r->n_cigar, &r->m_cigar, cigar, NS_CIGAR_V, 1), --i; if (!ext) --j; break;
-		case 6: cigar = ns_push_cigar(km, &r->n_cigar, &r->m_cigar, cigar, NS_CIGAR_F, 1), --i; break;
-		case 7: cigar = ns_push_cigar(km, &r->n_cigar, &r->m_cigar, cigar, NS_CIGAR_F, 2), i -= 2; break;
-		case 8: cigar = ns_push_cigar(km, &r->n_cigar, &r->m_cigar, cigar, NS_CIGAR_G, 1), --i, --j; break;
-		case 9: cigar = ns_push_cigar(km, &r->n_cigar, &r->m_cigar, cigar, NS_CIGAR_G, 2), i -= 2, --j; break;
-		default: i = -1; break;   // not produced by the kernel
+		case 0: bt_push(t, nc, NS_CIGAR_M, 1), i -= 3, --j; break;
+		case 1: bt_push(t, nc, NS_CIGAR_I, 1), --j; break;
+		case 2: bt_push(t, nc, NS_CIGAR_D, 1), i -= 3; break;
+		case 3: bt_push(t, nc, NS_CIGAR_N, 1), --i; break;
+		case 4: bt_push(t, nc, NS_CIGAR_U, 1), --i; if (!ext) --j; break;
+		case 5: bt_push(t, nc, NS_CIGAR_V, 1), --i; if (!ext) --j; break;
+		case 6: bt_push(t, nc, NS_CIGAR_F, 1), --i; break;
+		case 7: bt_push(t, nc, NS_CIGAR_F, 2), i -= 2; break;
+		case 8: bt_push(t, nc, NS_CIGAR_G, 1), --i, --j; break;
+		case 9: bt_push(t, nc, NS_CIGAR_G, 2), i -= 2, --j; break;
+		default: i = -1; break;   // not produced by the kernels
 		}
 		last = state >= 1 && state <= 5 && ext? state : 0;
 	}
-	if (j > 0) cigar = ns_push_cigar(km, &r->n_cigar, &r->m_cigar, cigar, NS_CIGAR_I, j);
+	if (j > 0) bt_push(t, nc, NS_CIGAR_I, j);
 	if (i >= 0) {
-		const int32_t l = (i + 1) / 3, t = (i + 1) % 3;
-		if (l > 0) cigar = ns_push_cigar(km, &r->n_cigar, &r->m_cigar, cigar, NS_CIGAR_D, l);
-		if (t != 0) cigar = ns_push_cigar(km, &r->n_cigar, &r->m_cigar, cigar, NS_CIGAR_F, t);
+		const int32_t l = (i + 1) / 3, r = (i + 1) % 3;
+		if (l > 0) bt_push(t, nc, NS_CIGAR_D, l);
+		if (r != 0) bt_push(t, nc, NS_CIGAR_F, r);
 	}
-	for (int32_t k = 0; k < r->n_cigar >> 1; ++k) std::swap(cigar[k], cigar[r->n_cigar - 1 - k]);
-	fix_tiny_uv(r->n_cigar, cigar);
-	r->cigar = cigar;
+	const unsigned long long off = atomicAdd(total, (unsigned long long)nc);
+	for (int32_t q = 0; q < nc; ++q) {   // reversed, U/V shorter than 3 become G (nasw-sse.c:34-42)
+		uint32_t c = t[nc - 1 - q];
+		const uint32_t op = c & 0xf;
+		if ((op == NS_CIGAR_U || op == NS_CIGAR_V) && c >> 4 < 3) c = c >> 4 << 4 | NS_CIGAR_G;
+		packed[off + q] = c;
+	}
+	meta[2 * k] = (int64_t)off, meta[2 * k + 1] = nc;
 }
 
-template <typename T>
-hipError_t dalloc(T **p, size_t n) { return hipMalloc((void**)p, n * sizeof(T) + 16); }
+__global__ void fill_u64_zero(unsigned long long *p) { if (threadIdx.x == 0) *p = 0; }
 
-std::mutex g_glob_mu;   // one problem at a time through the shared stream
+#define HIPOK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { nsh_set_error("%s: %s", #x, hipGetErrorString(e_)); goto fail; } } while (0)
+
+struct DevPool {   // grow-only device buffer
+	void *p = nullptr;
+	size_t cap = 0;
+	hipError_t ensure(size_t n)
+	{
+		if (n <= cap) return hipSuccess;
+		if (p) (void)hipFree(p);
+		p = nullptr, cap = 0;
+		n = std::max(n + (n >> 3), (size_t)1 << 20);
+		const hipError_t e = hipMalloc(&p, n);
+		if (e == hipSuccess) cap = n;
+		return e;
+	}
+	template <typename T> T *as(size_t byte_off = 0) const { return (T*)((char*)p + byte_off); }
+};
+struct HostPool {  // grow-only pinned host buffer
+	void *p = nullptr;
+	size_t cap = 0;
+	hipError_t ensure(size_t n)
+	{
+		if (n <= cap) return hipSuccess;
+		if (p) (void)hipHostFree(p);
+		p = nullptr, cap = 0;
+		n = std::max(n + (n >> 3), (size_t)1 << 20);
+		const hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+		if (e == hipSuccess) cap = n;
+		return e;
+	}
+};
+
+size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 } // namespace
+
+// ---------------------------------------------------------------------------------------
+// requests
+// ---------------------------------------------------------------------------------------
+struct nsh_glob_req {
+	std::vector<uint8_t> ns, as, ss;
+	std::vector<int8_t> pssm;
+	int32_t nl = 0, al = 0, is32 = 0, trace = 0;
+	ns_opt_t opt;                // sc/nt4/aa20/codon point into the copies below
+	int8_t mat[22 * 22];
+	uint8_t nt4[256], aa20[256], codon[64];
+	// results
+	int done = 0, ok = 0;
+	int32_t score = 0;
+	std::vector<uint32_t> cigar;
+};
+
+namespace {
+
+bool same_terms(const nsh_glob_req *a, const nsh_glob_req *b)
+{
+	const ns_opt_t &x = a->opt, &y = b->opt;
+	return x.go == y.go && x.ge == y.ge && x.fs == y.fs && x.sp_null_bonus == y.sp_null_bonus &&
+		std::memcmp(x.sp, y.sp, sizeof(x.sp)) == 0 && std::memcmp(a->mat, b->mat, sizeof(a->mat)) == 0 &&
+		std::memcmp(a->nt4, b->nt4, 256) == 0 && std::memcmp(a->aa20, b->aa20, 256) == 0 &&
+		std::memcmp(a->codon, b->codon, 64) == 0;
+}
+
+int kernel_variant(const nsh_glob_req *q) { return q->is32? (q->trace? 1 : 2) : 0; }   // trace16, trace32, score32
+
+class Engine {
+public:
+	std::mutex mu;
+	std::condition_variable cv_work, cv_done;
+	std::deque<nsh_glob_req*> queue;
+	int n_waiting = 0;
+	bool started = false;
+	// executor-owned
+	hipStream_t st[4] = {};
+	hipEvent_t ev_prep = nullptr, ev_b[4] = {};
+	DevPool d_in, d_rec, d_aas, d_tr, d_tmp, d_packed, d_meta;
+	HostPool h_in, h_meta, h_packed;
+	// trace bytes per batch part: NSH_TRACE_BUDGET_MB (read per batch), 16 GiB by default
+	static size_t trace_budget()
+	{
+		const char *s = std::getenv("NSH_TRACE_BUDGET_MB");
+		return (size_t)(s? std::max(1L, std::atol(s)) : 16384L) << 20;
+	}
+
+	void start()
+	{
+		if (started) return;
+		started = true;
+		std::thread([this] { loop(); }).detach();
+	}
+
+	void submit(nsh_glob_req *q)
+	{
+		std::lock_guard<std::mutex> lk(mu);
+		start();
+		queue.push_back(q);
+		cv_work.notify_one();
+	}
+
+	void wait(nsh_glob_req *q)
+	{
+		std::unique_lock<std::mutex> lk(mu);
+		if (q->done) return;
+		++n_waiting;
+		cv_work.notify_one();   // a waiter ends the batching window
+		cv_done.wait(lk, [&] { return q->done != 0; });
+		--n_waiting;
+	}
+
+	void loop()
+	{
+		for (;;) {
+			std::vector<nsh_glob_req*> batch;
+			{
+				std::unique_lock<std::mutex> lk(mu);
+				cv_work.wait(lk, [&] { return !queue.empty(); });
+				// batching window: until a caller waits, 4096 requests are queued, or 2 ms pass
+				const auto until = std::chrono::steady_clock::now() + std::chrono::milliseconds(2);
+				while (n_waiting == 0 && queue.size() < 4096 &&
+						cv_work.wait_until(lk, until) != std::cv_status::timeout) {}
+				batch.assign(queue.begin(), queue.end());
+				queue.clear();
+			}
+			run(batch);
+			{
+				std::lock_guard<std::mutex> lk(mu);
+				for (nsh_glob_req *q : batch) q->done = 1;
+			}
+			cv_done.notify_all();
+		}
+	}
+
+	void run(std::vector<nsh_glob_req*> &batch)
+	{
+		if (!nsh_ensure_device()) { for (auto *q : batch) q->ok = 0; return; }
+		if (!st[0]) {
+			for (int k = 0; k < 4; ++k) {
+				if (hipStreamCreateWithFlags(&st[k], hipStreamNonBlocking) != hipSuccess ||
+						hipEventCreateWithFlags(&ev_b[k], hipEventDisableTiming) != hipSuccess) {
+					nsh_set_error("global DP engine: stream set-up failed");
+					for (auto *q : batch) q->ok = 0;
+					return;
+				}
+			}
+			(void)hipEventCreateWithFlags(&ev_prep, hipEventDisableTiming);
+		}
+		// option groups, then trace-budget cuts
+		const size_t budget = trace_budget();
+		std::vector<char> taken(batch.size(), 0);
+		for (size_t a = 0; a < batch.size(); ++a) {
+			if (taken[a]) continue;
+			std::vector<nsh_glob_req*> grp;
+			for (size_t b = a; b < batch.size(); ++b)
+				if (!taken[b] && same_terms(batch[a], batch[b])) taken[b] = 1, grp.push_back(batch[b]);
+			size_t lo = 0;
+			while (lo < grp.size()) {
+				size_t hi = lo, tr = 0;
+				while (hi < grp.size()) {
+					const size_t t = grp[hi]->trace? (size_t)grp[hi]->nl * grp[hi]->al * 2 : 0;
+					if (hi > lo && tr + t > budget) break;
+					tr += t, ++hi;
+				}
+				std::vector<nsh_glob_req*> part(grp.begin() + lo, grp.begin() + hi);
+				if (!run_part(part)) for (auto *q : part) q->ok = 0;
+				lo = hi;
+			}
+		}
+	}
+
+	int run_part(std::vector<nsh_glob_req*> &reqs);
+};
+
+Engine &engine()
+{
+	static Engine *e = new Engine();   // never destroyed: its thread lives until the process ends
+	return *e;
+}
+
+int Engine::run_part(std::vector<nsh_glob_req*> &reqs)
+{
+	const auto t0 = std::chrono::steady_clock::now();
+	const int32_t n = (int32_t)reqs.size();
+	const nsh_glob_req *r0 = reqs[0];
+	// layout of the input block: ns | as | ss | pssm | PrepProb | GlobProb | BtProb | PrepConst | mat
+	int64_t l_ns = 0, l_as = 0, l_pssm = 0, l_tr = 0, l_tmp = 0;
+	int any_ss = 0, n_bt = 0;
+	std::vector<nsh_prep::PrepProb> pp(n);
+	std::vector<GlobProb> gp(n);
+	std::vector<int32_t> order(n);
+	for (int32_t k = 0; k < n; ++k) {
+		const nsh_glob_req *q = reqs[k];
+		nsh_prep::PrepProb &p = pp[k];
+		p = {};
+		p.ns_off = l_ns, p.as_off = l_as, p.rec_off = l_ns, p.aa_off = l_as;
+		// the strand only matters to int32 score-only requests with an extension flag (nasw-sse.c:246-247)
+		const int left = (q->opt.flag & NS_F_EXT_LEFT) != 0;
+		p.nl = q->nl, p.al = q->al, p.left = left, p.io = q->opt.io, p.has_ss = !q->ss.empty();
+		GlobProb &g = gp[k];
+		g = {};
+		g.rec_off = l_ns, g.aa_off = l_as, g.pssm_off = q->pssm.empty()? -1 : l_pssm;
+		g.tr_off = q->trace? l_tr : 0;
+		g.nl = q->nl, g.al = q->al, g.io = q->opt.io, g.out = k, g.left = left;
+		shape(q->al, g.C, g.NW);
+		const int vs = q->is32? 4 : 8;
+		g.slen = (q->al + vs - 1) / vs, g.npad = q->al, g.mode = 0;
+		l_ns += q->nl + 1, l_as += q->al + 1;
+		if (!q->pssm.empty()) l_pssm += (int64_t)q->al * 22;
+		if (q->trace) l_tr += (int64_t)q->nl * q->al, l_tmp += (int64_t)q->nl + q->al + 8, ++n_bt;
+		any_ss |= !q->ss.empty();
+		order[k] = k;
+	}
+	// buckets of equal kernel variant and shape
+	std::sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+		const int vx = kernel_variant(reqs[x]), vy = kernel_variant(reqs[y]);
+		if (vx != vy) return vx < vy;
+		if (gp[x].NW != gp[y].NW) return gp[x].NW > gp[y].NW;
+		if (gp[x].C != gp[y].C) return gp[x].C > gp[y].C;
+		return x < y; });
+	std::vector<GlobProb> gsorted(n);
+	for (int32_t k = 0; k < n; ++k) gsorted[k] = gp[order[k]];
+	std::vector<BtProb> bt;
+	std::vector<int32_t> bt_req;   // request index of each backtrack slot
+	{
+		int64_t tmp = 0;
+		for (int32_t k = 0; k < n; ++k) {
+			const nsh_glob_req *q = reqs[k];
+			if (!q->trace) continue;
+			bt.push_back({ gp[k].tr_off, tmp, q->nl, q->al });
+			bt_req.push_back(k);
+			tmp += (int64_t)q->nl + q->al + 8;
+		}
+	}
+	const size_t o_ns = 0, o_as = al256(o_ns + l_ns), o_ss = al256(o_as + l_as), o_pssm = al256(o_ss + (any_ss? l_ns : 0));
+	const size_t o_pp = al256(o_pssm + l_pssm), o_gp = al256(o_pp + (size_t)n * sizeof(nsh_prep::PrepProb));
+	const size_t o_bt = al256(o_gp + (size_t)n * sizeof(GlobProb)), o_pc = al256(o_bt + bt.size() * sizeof(BtProb));
+	const size_t o_mat = al256(o_pc + sizeof(nsh_prep::PrepConst)), o_pen = al256(o_mat + 484);
+	const size_t in_bytes = al256(o_pen + 16);
+	int ok = 0;
+	uint8_t *h = nullptr, *d = nullptr;
+	int32_t *d_score = nullptr;
+	int64_t *bmeta = nullptr, *d_btmeta = nullptr;
+	unsigned long long total = 0, *d_total = nullptr;
+	// device result block: scores (n ints) | CIGAR total | (offset, count) per backtracked problem
+	const size_t meta_bytes = al256((size_t)n * 4) + 8 + bt.size() * 16;
+	HIPOK(h_in.ensure(in_bytes));
+	HIPOK(d_in.ensure(in_bytes));
+	HIPOK(d_rec.ensure((size_t)l_ns * 4 + 256));
+	HIPOK(d_aas.ensure((size_t)l_as + 256));
+	if (l_tr) HIPOK(d_tr.ensure((size_t)l_tr * 2 + 256));
+	if (l_tmp) { HIPOK(d_tmp.ensure((size_t)l_tmp * 4 + 256)); HIPOK(d_packed.ensure((size_t)l_tmp * 4 + 256)); }
+	HIPOK(d_meta.ensure(meta_bytes + 256));
+	HIPOK(h_meta.ensure(meta_bytes + 256));
+	h = (uint8_t*)h_in.p, d = (uint8_t*)d_in.p;
+	// pack the inputs
+	{
+		nsh_prep::PrepConst pc;
+		std::memcpy(pc.nt4, r0->nt4, 256), std::memcpy(pc.aa20, r0->aa20, 256), std::memcpy(pc.codon, r0->codon, 64);
+		for (int t = 0; t < 6; ++t) pc.sp[t] = r0->opt.sp[t];
+		pc.sp_null = r0->opt.sp_null_bonus;
+		for (int32_t k = 0; k < n; ++k) {
+			const nsh_glob_req *q = reqs[k];
+			if (q->nl) std::memcpy(h + o_ns + pp[k].ns_off, q->ns.data(), q->nl);
+			std::memcpy(h + o_as + pp[k].as_off, q->as.data(), q->al);
+			if (any_ss) {
+				if (!q->ss.empty()) std::memcpy(h + o_ss + pp[k].ns_off, q->ss.data(), q->nl);
+				else std::memset(h + o_ss + pp[k].ns_off, 0xff, q->nl);
+			}
+			if (!q->pssm.empty()) std::memcpy(h + o_pssm + gp[k].pssm_off, q->pssm.data(), q->pssm.size());
+		}
+		std::memcpy(h + o_pp, pp.data(), (size_t)n * sizeof(nsh_prep::PrepProb));
+		std::memcpy(h + o_gp, gsorted.data(), (size_t)n * sizeof(GlobProb));
+		if (!bt.empty()) std::memcpy(h + o_bt, bt.data(), bt.size() * sizeof(BtProb));
+		std::memcpy(h + o_pc, &pc, sizeof(pc));
+		std::memcpy(h + o_mat, r0->mat, 484);
+	}
+	{
+		const auto t1 = std::chrono::steady_clock::now();
+		HIPOK(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, st[0]));
+		nsh_add_times(std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count(), 0, 0);
+	}
+	d_score = d_meta.as<int32_t>();
+	nsh_prep::launch((const nsh_prep::PrepProb*)(d + o_pp), n, d + o_ns, any_ss? d + o_ss : nullptr, d + o_as,
+			d_rec.as<uint32_t>(), d_aas.as<uint8_t>(), (const nsh_prep::PrepConst*)(d + o_pc), st[0]);
+	HIPOK(hipGetLastError());
+	HIPOK(hipEventRecord(ev_prep, st[0]));
+	{
+		GlobArgs ga = {};
+		ga.rec = d_rec.as<uint32_t>(), ga.aas = d_aas.as<uint8_t>(), ga.pssm = (const int8_t*)(d + o_pssm);
+		ga.mat = (const int8_t*)(d + o_mat), ga.asize = 22;
+		ga.go = r0->opt.go, ga.ge = r0->opt.ge, ga.fs = r0->opt.fs;
+		ga.trace = l_tr? d_tr.as<uint16_t>() : nullptr, ga.score = d_score;
+		ga.pen = (const int32_t*)(d + o_pen), ga.pen_n = 1;
+		const GlobProb *d_gp = (const GlobProb*)(d + o_gp);
+		int nb = 0;
+		for (int32_t k = 0; k < n;) {
+			int32_t e = k;
+			const int v = kernel_variant(reqs[order[k]]);
+			while (e < n && kernel_variant(reqs[order[e]]) == v && gsorted[e].NW == gsorted[k].NW && gsorted[e].C == gsorted[k].C) ++e;
+			hipStream_t s = st[nb & 3];
+			HIPOK(hipStreamWaitEvent(s, ev_prep, 0));
+			ga.prob = d_gp + k;
+			if (v == 0) launch_trace16(ga, e - k, gsorted[k].NW, gsorted[k].C, s);
+			else if (v == 1) launch_trace32(ga, e - k, gsorted[k].NW, gsorted[k].C, s);
+			else launch_score32(ga, e - k, gsorted[k].NW, gsorted[k].C, s);
+			HIPOK(hipGetLastError());
+			++nb, k = e;
+		}
+		for (int b = 1; b < std::min(nb, 4); ++b) {
+			HIPOK(hipEventRecord(ev_b[b], st[b]));
+			HIPOK(hipStreamWaitEvent(st[0], ev_b[b], 0));
+		}
+		nsh_count_launch(1 + nb);
+	}
+	d_btmeta = (int64_t*)((char*)d_meta.p + al256((size_t)n * 4) + 8);
+	d_total = (unsigned long long*)((char*)d_meta.p + al256((size_t)n * 4));
+	if (!bt.empty()) {
+		hipLaunchKernelGGL(fill_u64_zero, dim3(1), dim3(64), 0, st[0], d_total);
+		hipLaunchKernelGGL(backtrack_kernel, dim3(((int)bt.size() + 63) / 64), dim3(64), 0, st[0], (const BtProb*)(d + o_bt),
+				(int32_t)bt.size(), d_tr.as<uint16_t>(), d_tmp.as<uint32_t>(), d_packed.as<uint32_t>(), d_total, d_btmeta);
+		HIPOK(hipGetLastError());
+		nsh_count_launch(2);
+	}
+	// scores, backtrack counts and offsets, then the CIGARs
+	HIPOK(hipMemcpyAsync(h_meta.p, d_meta.p, meta_bytes, hipMemcpyDeviceToHost, st[0]));
+	HIPOK(hipStreamSynchronize(st[0]));
+	{
+		const auto t2 = std::chrono::steady_clock::now();
+		nsh_add_times(0, std::chrono::duration<double>(t2 - t0).count(), 0);
+	}
+	bmeta = (int64_t*)((char*)h_meta.p + al256((size_t)n * 4) + 8);
+	total = bt.empty()? 0 : *(unsigned long long*)((char*)h_meta.p + al256((size_t)n * 4));
+	if (total) {
+		const auto t3 = std::chrono::steady_clock::now();
+		HIPOK(h_packed.ensure((size_t)total * 4));
+		HIPOK(hipMemcpyAsync(h_packed.p, d_packed.p, (size_t)total * 4, hipMemcpyDeviceToHost, st[0]));
+		HIPOK(hipStreamSynchronize(st[0]));
+		nsh_add_times(0, 0, std::chrono::duration<double>(std::chrono::steady_clock::now() - t3).count());
+	}
+	{
+		const int32_t *hs = (const int32_t*)h_meta.p;
+		for (int32_t k = 0; k < n; ++k) {
+			nsh_glob_req *q = reqs[k];
+			q->score = q->nl > 2? hs[k] : (q->is32? -0x40000000 : -32768);
+			q->ok = 1;
+		}
+		const uint32_t *pk = (const uint32_t*)h_packed.p;
+		for (size_t b = 0; b < bt.size(); ++b) {
+			nsh_glob_req *q = reqs[bt_req[b]];
+			const int64_t off = bmeta[2 * b], cnt = bmeta[2 * b + 1];
+			q->cigar.assign(pk + off, pk + off + cnt);
+		}
+	}
+	ok = 1;
+fail:
+	return ok;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------------------
+// internal API (nsh_internal.h)
+// ---------------------------------------------------------------------------------------
+nsh_glob_req *nsh_glob_submit(const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt,
+		const uint8_t *ss, int32_t is32)
+{
+	if (!nsh_ensure_device()) return nullptr;
+	const bool cigar = (opt->flag & NS_F_CIGAR) && !(opt->flag & (NS_F_EXT_LEFT | NS_F_EXT_RIGHT));
+	if (!is32 && !cigar) { nsh_set_error("score-only int16 DP goes through the extension engine"); return nullptr; }
+	if (al < 1 || nl < 0) { nsh_set_error("global DP: nl=%d al=%d", nl, al); return nullptr; }
+	if (al > 64 * kCMaxG * kNWMax) { nsh_set_error("global DP: protein length %d exceeds %d", al, 64 * kCMaxG * kNWMax); return nullptr; }
+	if (opt->asize != 22) { nsh_set_error("asize %d not supported", opt->asize); return nullptr; }
+	if (opt->ge < 0 || opt->go < 0 || opt->ge > 32767 || opt->go > 32767 || opt->fs < 0 || opt->fs > 32767) {
+		nsh_set_error("gap penalties go=%d ge=%d fs=%d out of the supported range", opt->go, opt->ge, opt->fs); return nullptr;
+	}
+	for (int t = 0; t < 64; ++t)
+		if (opt->codon[t] >= 22) { nsh_set_error("codon table entry %d out of range", t); return nullptr; }
+	for (int t = 0; t < 256; ++t)
+		if (opt->aa20[t] >= 22 || opt->nt4[t] > 4) { nsh_set_error("alphabet table entry %d out of range", t); return nullptr; }
+	nsh_glob_req *q = new nsh_glob_req();
+	q->nl = nl, q->al = al, q->is32 = is32 != 0, q->trace = cigar;
+	q->ns.assign((const uint8_t*)ns, (const uint8_t*)ns + nl);
+	q->as.assign((const uint8_t*)as, (const uint8_t*)as + al);
+	if (ss) q->ss.assign(ss, ss + nl);
+	if (opt->pssm) q->pssm.assign(opt->pssm, opt->pssm + (size_t)al * 22);
+	q->opt = *opt;
+	std::memcpy(q->mat, opt->sc, 484), std::memcpy(q->nt4, opt->nt4, 256), std::memcpy(q->aa20, opt->aa20, 256);
+	std::memcpy(q->codon, opt->codon, 64);
+	q->opt.sc = q->mat, q->opt.nt4 = q->nt4, q->opt.aa20 = q->aa20, q->opt.codon = q->codon, q->opt.pssm = nullptr;
+	engine().submit(q);
+	return q;
+}
+
+int nsh_glob_wait(nsh_glob_req *q, void *km, ns_rst_t *r)
+{
+	if (!q) return 0;
+	engine().wait(q);
+	const int ok = q->ok;
+	if (ok) {   // the caller's r may already own a CIGAR buffer (ns_rst_init): append into it
+		for (uint32_t c : q->cigar) r->cigar = ns_push_cigar(km, &r->n_cigar, &r->m_cigar, r->cigar, c & 0xf, (int32_t)(c >> 4));
+		r->nt_len = q->nl, r->aa_len = q->al, r->score = q->score;
+	}
+	delete q;
+	return ok;
+}
 
 int nsh_global_run(void *km, const char *ns, int32_t nl, const char *as, int32_t al,
 		const ns_opt_t *opt, const uint8_t *ss, int32_t is32, ns_rst_t *r)
 {
-	if (!nsh_ensure_device()) return 0;
-	const bool cigar = (opt->flag & NS_F_CIGAR) && !(opt->flag & (NS_F_EXT_LEFT | NS_F_EXT_RIGHT));
-	if (!is32 && !cigar) { nsh_set_error("score-only int16 DP goes through the extension engine"); return 0; }
-	if (al < 1 || nl < 0) { nsh_set_error("global DP: nl=%d al=%d", nl, al); return 0; }
-	if (opt->asize != 22) { nsh_set_error("asize %d not supported", opt->asize); return 0; }
-	const int32_t NW = (al + 64 * CMAXG - 1) / (64 * CMAXG);
-	if (NW > NWMAX) { nsh_set_error("global DP: protein length %d exceeds %d", al, 64 * CMAXG * NWMAX); return 0; }
-	const int32_t C = (al + 64 * NW - 1) / (64 * NW);
-	if (opt->ge < 0 || opt->go < 0 || opt->ge > 32767 || opt->go > 32767 || opt->fs < 0 || opt->fs > 32767) {
-		nsh_set_error("gap penalties go=%d ge=%d fs=%d out of the supported range", opt->go, opt->ge, opt->fs); return 0;
-	}
-	for (int t = 0; t < 64; ++t)
-		if (opt->codon[t] >= 22) { nsh_set_error("codon table entry %d out of range", t); return 0; }
-	for (int t = 0; t < 256; ++t)
-		if (opt->aa20[t] >= 22 || opt->nt4[t] > 4) { nsh_set_error("alphabet table entry %d out of range", t); return 0; }
-	const int vs = is32? 4 : 8;
-	std::lock_guard<std::mutex> lk(g_glob_mu);
-	static hipStream_t stream = nullptr;
-	uint8_t *d_ns = nullptr, *d_ss = nullptr, *d_as = nullptr, *d_aas = nullptr;
-	uint32_t *d_rec = nullptr;
-	int8_t *d_mat = nullptr, *d_pssm = nullptr;
-	uint16_t *d_tr = nullptr;
-	int32_t *d_score = nullptr;
-	GlobProb *d_prob = nullptr;
-	nsh_prep::PrepProb *d_pp = nullptr;
-	nsh_prep::PrepConst *d_pc = nullptr;
-	int ok = 0;
-	std::vector<uint16_t> tr;
-	const auto t0 = std::chrono::steady_clock::now();
-	{
-		if (!stream) {
-			HIPOK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-			const int big = 22 * 64 * CMAXG * NWMAX;   // profile bytes of the largest work-group
-			HIPOK(hipFuncSetAttribute(reinterpret_cast<const void*>(&glob_kernel<Arith16, true, 1024>), hipFuncAttributeMaxDynamicSharedMemorySize, big));
-			HIPOK(hipFuncSetAttribute(reinterpret_cast<const void*>(&glob_kernel<Arith32, true, 1024>), hipFuncAttributeMaxDynamicSharedMemorySize, big));
-			HIPOK(hipFuncSetAttribute(reinterpret_cast<const void*>(&glob_kernel<Arith32, false, 1024>), hipFuncAttributeMaxDynamicSharedMemorySize, big));
-		}
-		nsh_prep::PrepProb pp = {};
-		pp.ns_off = 0, pp.as_off = 0, pp.rec_off = 0, pp.aa_off = 0;
-		pp.nl = nl, pp.al = al, pp.left = (opt->flag & NS_F_EXT_LEFT) != 0, pp.io = opt->io, pp.has_ss = ss != nullptr;
-		nsh_prep::PrepConst pc;
-		std::memcpy(pc.nt4, opt->nt4, 256), std::memcpy(pc.aa20, opt->aa20, 256), std::memcpy(pc.codon, opt->codon, 64);
-		for (int t = 0; t < 6; ++t) pc.sp[t] = opt->sp[t];
-		pc.sp_null = opt->sp_null_bonus;
-		GlobProb gp = {};
-		gp.rec_off = 0, gp.aa_off = 0, gp.tr_off = 0, gp.pssm_off = opt->pssm? 0 : -1;
-		gp.nl = nl, gp.al = al, gp.io = opt->io, gp.out = 0, gp.C = C, gp.NW = NW, gp.slen = (al + vs - 1) / vs, gp.left = pp.left;
-		HIPOK(dalloc(&d_ns, nl + 1)); HIPOK(dalloc(&d_as, al + 1)); HIPOK(dalloc(&d_aas, al + 1)); HIPOK(dalloc(&d_rec, nl + 1));
-		if (ss) HIPOK(dalloc(&d_ss, nl + 1));
-		HIPOK(dalloc(&d_mat, 22 * 22));
-		if (opt->pssm) HIPOK(dalloc(&d_pssm, (size_t)al * 22));
-		if (cigar) HIPOK(dalloc(&d_tr, (size_t)nl * al));
-		HIPOK(dalloc(&d_score, 1)); HIPOK(dalloc(&d_prob, 1)); HIPOK(dalloc(&d_pp, 1)); HIPOK(dalloc(&d_pc, 1));
-		if (nl) HIPOK(hipMemcpyAsync(d_ns, ns, nl, hipMemcpyHostToDevice, stream));
-		HIPOK(hipMemcpyAsync(d_as, as, al, hipMemcpyHostToDevice, stream));
-		if (ss && nl) HIPOK(hipMemcpyAsync(d_ss, ss, nl, hipMemcpyHostToDevice, stream));
-		HIPOK(hipMemcpyAsync(d_mat, opt->sc, 22 * 22, hipMemcpyHostToDevice, stream));
-		if (opt->pssm) HIPOK(hipMemcpyAsync(d_pssm, opt->pssm, (size_t)al * 22, hipMemcpyHostToDevice, stream));
-		HIPOK(hipMemcpyAsync(d_prob, &gp, sizeof(gp), hipMemcpyHostToDevice, stream));
-		HIPOK(hipMemcpyAsync(d_pp, &pp, sizeof(pp), hipMemcpyHostToDevice, stream));
-		HIPOK(hipMemcpyAsync(d_pc, &pc, sizeof(pc), hipMemcpyHostToDevice, stream));
-		nsh_prep::launch(d_pp, 1, d_ns, d_ss, d_as, d_rec, d_aas, d_pc, stream);
-		GlobArgs ga;
-		ga.prob = d_prob, ga.rec = d_rec, ga.aas = d_aas, ga.pssm = d_pssm, ga.mat = d_mat, ga.asize = 22;
-		ga.go = opt->go, ga.ge = opt->ge, ga.fs = opt->fs, ga.trace = d_tr, ga.score = d_score;
-		const size_t lds = (size_t)22 * 64 * C * NW;
-		const dim3 grid(1), block(64 * NW);
-		if (NW <= 8) {
-			if (!is32) hipLaunchKernelGGL((glob_kernel<Arith16, true, 512>), grid, block, lds, stream, ga);
-			else if (cigar) hipLaunchKernelGGL((glob_kernel<Arith32, true, 512>), grid, block, lds, stream, ga);
-			else hipLaunchKernelGGL((glob_kernel<Arith32, false, 512>), grid, block, lds, stream, ga);
-		} else {
-			if (!is32) hipLaunchKernelGGL((glob_kernel<Arith16, true, 1024>), grid, block, lds, stream, ga);
-			else if (cigar) hipLaunchKernelGGL((glob_kernel<Arith32, true, 1024>), grid, block, lds, stream, ga);
-			else hipLaunchKernelGGL((glob_kernel<Arith32, false, 1024>), grid, block, lds, stream, ga);
-		}
-		HIPOK(hipGetLastError());
-		nsh_count_launch(2);
-		int32_t score = 0;
-		HIPOK(hipMemcpyAsync(&score, d_score, 4, hipMemcpyDeviceToHost, stream));
-		if (cigar) {
-			tr.resize((size_t)nl * al);
-			HIPOK(hipMemcpyAsync(tr.data(), d_tr, tr.size() * 2, hipMemcpyDeviceToHost, stream));
-		}
-		HIPOK(hipStreamSynchronize(stream));
-		r->n_cigar = 0, r->nt_len = nl, r->aa_len = al, r->score = nl > 2? score : (is32? -0x40000000 : -32768);
-		if (cigar) backtrack(km, tr.data(), nl, al, r);
-		ok = 1;
-	}
-fail:
-	{
-		void *bufs[] = { d_ns, d_ss, d_as, d_aas, d_rec, d_mat, d_pssm, d_tr, d_score, d_prob, d_pp, d_pc };
-		for (void *b : bufs) if (b) (void)hipFree(b);
-	}
-	nsh_add_times(0, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(), 0);
-	return ok;
+	nsh_glob_req *q = nsh_glob_submit(ns, nl, as, al, opt, ss, is32);
+	if (!q) return 0;
+	r->n_cigar = 0;
+	return nsh_glob_wait(q, km, r);
 }

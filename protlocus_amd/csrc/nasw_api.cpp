@@ -234,10 +234,11 @@ extern "C" void ns_global_gs_auto(void *km, const char *ns, int32_t nl, const ch
 // the best possible score al * max(sc) + end_bonus exceeds 30000 (the caller then takes the int32
 // path). The extension flags are ignored and the CIGAR is always produced; the splice-score track
 // is honoured and opt->pssm is ignored (the reference's kernel scores with opt->sc only,
-// nasw-cuda.cu:1068-1110), so the result equals ns_global_gs16b with NS_F_CIGAR and no PSSM. Each submission
-// is computed before submit returns and is consumed exactly once by ns_gpu_wait; the CIGAR is
-// allocated from km.
-struct ns_gpu_request_s { ns_rst_t rst; void *km; int ok; };
+// nasw-cuda.cu:1068-1110), so the result equals ns_global_gs16b with NS_F_CIGAR and no PSSM.
+// Submission copies the inputs and queues the problem in the global DP engine (global_dp.hip),
+// which batches the submissions of all threads; each submission is consumed exactly once by
+// ns_gpu_wait, which copies the CIGAR into km on the calling thread.
+struct ns_gpu_request_s { nsh_glob_req *g; void *km; };
 
 static int nsh_gpu_global_accepts(int32_t nl, int32_t al, const ns_opt_t *opt)
 {
@@ -247,47 +248,44 @@ static int nsh_gpu_global_accepts(int32_t nl, int32_t al, const ns_opt_t *opt)
 	return (int64_t)al * max_sub + opt->end_bonus <= 30000;
 }
 
-static int nsh_gpu_global_run(void *km, const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt,
-		const uint8_t *ss, ns_rst_t *r)
+// the request ns_gpu_submit queues: int16, CIGAR, no extension, matrix scores
+static nsh_glob_req *nsh_gpu_global_submit(const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt,
+		const uint8_t *ss)
 {
-	if (!nsh_ensure_device() || !nsh_gpu_global_accepts(nl, al, opt)) return 0;
+	if (!nsh_ensure_device() || !nsh_gpu_global_accepts(nl, al, opt)) return nullptr;
 	ns_opt_t o = *opt;
 	o.flag = (o.flag | NS_F_CIGAR) & ~(NS_F_GPU | NS_F_EXT_LEFT | NS_F_EXT_RIGHT);
 	o.pssm = nullptr;
-	if (!nsh_global_run(km, ns, nl, as, al, &o, ss, 0, r)) return 0;
-	++g_dispatch;
-	return 1;
+	return nsh_glob_submit(ns, nl, as, al, &o, ss, 0);
 }
 
 extern "C" ns_gpu_request_t *ns_gpu_submit(void *km, const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, const uint8_t *ss)
 {
-	ns_gpu_request_t *q = (ns_gpu_request_t*)std::calloc(1, sizeof(ns_gpu_request_t));
-	if (!q) return nullptr;
-	q->rst.n_cigar = q->rst.m_cigar = 0, q->rst.cigar = nullptr, q->km = km;
-	q->ok = nsh_gpu_global_run(km, ns, nl, as, al, opt, ss, &q->rst);
-	if (!q->ok) { std::free(q); return nullptr; }
+	nsh_glob_req *g = nsh_gpu_global_submit(ns, nl, as, al, opt, ss);
+	if (!g) return nullptr;
+	ns_gpu_request_t *q = (ns_gpu_request_t*)std::malloc(sizeof(ns_gpu_request_t));
+	if (!q) { ns_rst_t tmp; ns_rst_init(&tmp); (void)nsh_glob_wait(g, nullptr, &tmp); std::free(tmp.cigar); return nullptr; }
+	q->g = g, q->km = km;
 	return q;
 }
 
 extern "C" int ns_gpu_wait(ns_gpu_request_t *q, ns_rst_t *r)
 {
 	if (!q) return 0;
-	const int ok = q->ok;
-	if (ok) {   // the caller's r may already own a CIGAR buffer (ns_rst_init): append into it
-		for (int32_t k = 0; k < q->rst.n_cigar; ++k) {
-			const uint32_t c = q->rst.cigar[k];
-			r->cigar = ns_push_cigar(q->km, &r->n_cigar, &r->m_cigar, r->cigar, c & 0xf, c >> 4);
-		}
-		r->nt_len = q->rst.nt_len, r->aa_len = q->rst.aa_len, r->score = q->rst.score;
-	}
-	if (q->rst.cigar) kfree(q->km, q->rst.cigar);
+	const int ok = nsh_glob_wait(q->g, q->km, r);   // appends to the CIGAR r may already hold
 	std::free(q);
+	if (ok) ++g_dispatch;
 	return ok;
 }
 
 extern "C" int ns_gpu_global(void *km, const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, const uint8_t *ss, ns_rst_t *r)
 {
-	return nsh_gpu_global_run(km, ns, nl, as, al, opt, ss, r);
+	nsh_glob_req *g = nsh_gpu_global_submit(ns, nl, as, al, opt, ss);
+	if (!g) return 0;
+	r->n_cigar = 0;
+	const int ok = nsh_glob_wait(g, km, r);
+	if (ok) ++g_dispatch;
+	return ok;
 }
 
 // ---------------------------------------------------------------------------------------

@@ -36,8 +36,17 @@ int nsh_ext_plan_query(nsh_ext_plan_t *p);   // 1 when the last run has complete
 void nsh_prep_host(const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt,
 		const uint8_t *ss, int left, uint8_t *nas, uint8_t *aas, int8_t *donor, int8_t *acc);
 
-// ---- global DP (global_dp.hip) ----
-// runs ns_global_gs16b/gs32b semantics for one problem on the GPU; 1 on success
+// ---- global DP engine (global_dp.hip) ----
+// one queued problem of ns_global_gs16b (NS_F_CIGAR) / ns_global_gs32b (with or without CIGAR)
+// semantics; the inputs are copied, so they need not outlive the call. nullptr (and the reason in
+// ns_gpu_error) when the problem is outside the engine's limits.
+struct nsh_glob_req;
+nsh_glob_req *nsh_glob_submit(const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt,
+		const uint8_t *ss, int32_t is32);
+// waits for q, appends its CIGAR to r (ns_push_cigar from km) and sets nt_len/aa_len/score; frees q.
+// 1 on success
+int nsh_glob_wait(nsh_glob_req *q, void *km, ns_rst_t *r);
+// submit + wait; 1 on success
 int nsh_global_run(void *km, const char *ns, int32_t nl, const char *as, int32_t al,
 		const ns_opt_t *opt, const uint8_t *ss, int32_t is32, ns_rst_t *r);
 

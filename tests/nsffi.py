@@ -463,6 +463,7 @@ class Product(_Lib):
         L.nsh_ext_plan_create.restype = C.c_void_p
         for f in (L.nsh_ext_plan_run, L.nsh_ext_plan_sync, L.nsh_ext_plan_destroy):
             f.argtypes = [C.c_void_p]
+        L.nsh_ext_plan_run_subset.argtypes = [C.c_void_p, C.c_char_p]
         L.nsh_ext_plan_results.argtypes = [C.c_void_p, C.c_void_p]
         L.nsh_ext_plan_cells.argtypes = [C.c_void_p, C.c_int32]
         L.nsh_ext_plan_cells.restype = C.c_double
