@@ -187,13 +187,48 @@ void nsh_prep::launch(const PrepProb *pp, int32_t n, const uint8_t *raw_ns, cons
 // ======================================================================================
 #define HIPOK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { nsh_set_error("%s: %s", #x, hipGetErrorString(e_)); goto fail; } } while (0)
 
+// A plan's device buffers, pinned staging, streams and events live in a shell. Destroying a plan
+// returns its shell to a small pool (at most kPoolShells, each at most kPoolBytes of device
+// buffers), and a new plan takes a pooled shell and only grows the buffers it needs larger, so
+// repeated extension batches of similar size allocate nothing.
+template <typename T>
+struct DBuf {
+	T *p = nullptr;
+	size_t cap = 0;                       // elements
+	hipError_t ensure(size_t n)
+	{
+		if (n <= cap && p) return hipSuccess;
+		if (p) (void)hipFree(p);
+		p = nullptr, cap = 0;
+		const size_t want = std::max(n + (n >> 3), (size_t)64);
+		const hipError_t e = hipMalloc((void**)&p, want * sizeof(T) + 16);
+		if (e == hipSuccess) cap = want;
+		return e;
+	}
+	size_t bytes() const { return p? cap * sizeof(T) : 0; }
+	void release() { if (p) (void)hipFree(p); p = nullptr, cap = 0; }
+};
+
 struct nsh_ext_plan_s {
 	int32_t n = 0;
 	hipStream_t stream = nullptr;
 	hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
 	hipStream_t gs[kGroups] = {};          // one stream per column-count group
 	hipEvent_t gev[kGroups] = {};          // completion of each group's launch
-	// device buffers
+	// device buffers (grow-only in a pooled shell)
+	DBuf<uint8_t> b_ns, b_ss, b_as, b_aas;
+	DBuf<uint32_t> b_rec;
+	DBuf<int8_t> b_mat, b_pssm;
+	DBuf<int32_t> b_pen, b_counter, b_out, b_idx_sub;
+	DBuf<ExtProb> b_prob, b_prob_sub;
+	DBuf<PrepProb> b_prep;
+	DBuf<PrepConst> b_pc;
+	DBuf<float> b_feat;
+	DBuf<nsh_glob::GlobProb> b_wide, b_wide_sub;
+	// pinned host staging of the inputs (grow-only)
+	void *h_stage = nullptr;
+	size_t h_stage_cap = 0;
+	// the current plan's view of the buffers
 	uint8_t *d_ns = nullptr, *d_ss = nullptr, *d_as = nullptr, *d_aas = nullptr;
 	uint32_t *d_rec = nullptr;
 	int8_t *d_mat = nullptr, *d_pssm = nullptr;
@@ -226,51 +261,109 @@ struct nsh_ext_plan_s {
 	int32_t *d_idx_sub = nullptr;            // kept input indices (their inputs are prepared)
 	std::vector<int32_t> idx_h;
 	float *d_feat = nullptr;
+
+	size_t device_bytes() const
+	{
+		return b_ns.bytes() + b_ss.bytes() + b_as.bytes() + b_aas.bytes() + b_rec.bytes() + b_mat.bytes() + b_pssm.bytes() +
+			b_pen.bytes() + b_counter.bytes() + b_out.bytes() + b_idx_sub.bytes() + b_prob.bytes() + b_prob_sub.bytes() +
+			b_prep.bytes() + b_pc.bytes() + b_feat.bytes() + b_wide.bytes() + b_wide_sub.bytes();
+	}
+	void reset()   // forget the last plan, keep the resources
+	{
+		n = 0, ran = 0, cells_window = 0, max_al = max_nl = 0;
+		al.clear(), out_h.clear(), prob_h.clear(), wide_h.clear(), wide_sub_h.clear(), sub_h.clear(), idx_h.clear();
+		d_ns = d_ss = d_as = d_aas = nullptr, d_rec = nullptr, d_mat = d_pssm = nullptr;
+		d_pen = d_counter = d_out = nullptr, d_prob = nullptr, d_prep = nullptr, d_pc = nullptr;
+		d_wide = d_wide_sub = nullptr, d_prob_sub = nullptr, d_idx_sub = nullptr, d_feat = nullptr;
+		for (int g = 0; g < kGroups; ++g) args[g] = ExtArgs{}, grid[g] = wpc[g] = grp_first[g] = grp_n[g] = 0;
+		wargs = nsh_glob::GlobArgs{};
+	}
+	void release_all()
+	{
+		b_ns.release(), b_ss.release(), b_as.release(), b_aas.release(), b_rec.release(), b_mat.release(), b_pssm.release();
+		b_pen.release(), b_counter.release(), b_out.release(), b_idx_sub.release(), b_prob.release(), b_prob_sub.release();
+		b_prep.release(), b_pc.release(), b_feat.release(), b_wide.release(), b_wide_sub.release();
+		if (h_stage) (void)hipHostFree(h_stage);
+		h_stage = nullptr, h_stage_cap = 0;
+		if (ev0) (void)hipEventDestroy(ev0);
+		if (ev1) (void)hipEventDestroy(ev1);
+		if (ev2) (void)hipEventDestroy(ev2);
+		for (int g = 0; g < kGroups; ++g) {
+			if (gev[g]) (void)hipEventDestroy(gev[g]);
+			if (gs[g]) (void)hipStreamDestroy(gs[g]);
+		}
+		if (wev) (void)hipEventDestroy(wev);
+		if (ws) (void)hipStreamDestroy(ws);
+		if (stream) (void)hipStreamDestroy(stream);
+		ev0 = ev1 = ev2 = wev = nullptr, ws = stream = nullptr;
+		for (int g = 0; g < kGroups; ++g) gev[g] = nullptr, gs[g] = nullptr;
+	}
 };
 
-static void plan_free_dev(nsh_ext_plan_t *p)
+namespace {
+constexpr int kPoolShells = 4;
+constexpr size_t kPoolBytes = (size_t)2 << 30;
+std::mutex g_pool_mu;
+std::vector<nsh_ext_plan_t*> g_pool;
+
+// a shell with its streams and events (pooled or new); nullptr on failure
+nsh_ext_plan_t *shell_acquire()
 {
-	void *bufs[] = { p->d_ns, p->d_ss, p->d_as, p->d_aas, p->d_rec, p->d_mat, p->d_pssm, p->d_pen, p->d_counter,
-		p->d_out, p->d_prob, p->d_prep, p->d_pc, p->d_prob_sub, p->d_idx_sub, p->d_feat, p->d_wide, p->d_wide_sub };
-	for (void *b : bufs) if (b) (void)hipFree(b);
+	{
+		std::lock_guard<std::mutex> lk(g_pool_mu);
+		if (!g_pool.empty()) { nsh_ext_plan_t *p = g_pool.back(); g_pool.pop_back(); return p; }
+	}
+	nsh_ext_plan_t *p = new nsh_ext_plan_s();
+	bool ok = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess &&
+		hipEventCreate(&p->ev0) == hipSuccess && hipEventCreate(&p->ev1) == hipSuccess && hipEventCreate(&p->ev2) == hipSuccess &&
+		hipStreamCreateWithFlags(&p->ws, hipStreamNonBlocking) == hipSuccess &&
+		hipEventCreateWithFlags(&p->wev, hipEventDisableTiming) == hipSuccess;
+	for (int g = 0; ok && g < kGroups; ++g)
+		ok = hipStreamCreateWithFlags(&p->gs[g], hipStreamNonBlocking) == hipSuccess &&
+			hipEventCreateWithFlags(&p->gev[g], hipEventDisableTiming) == hipSuccess;
+	if (!ok) { nsh_set_error("extension plan: stream set-up failed"); p->release_all(); delete p; return nullptr; }
+	return p;
 }
 
-template <typename T>
-static hipError_t dalloc(T **p, size_t n) { return hipMalloc((void**)p, n * sizeof(T) + 16); }
+void shell_release(nsh_ext_plan_t *p)
+{
+	if (p->stream) (void)hipStreamSynchronize(p->stream);
+	p->reset();
+	{
+		std::lock_guard<std::mutex> lk(g_pool_mu);
+		if ((int)g_pool.size() < kPoolShells && p->device_bytes() <= kPoolBytes) { g_pool.push_back(p); return; }
+	}
+	p->release_all();
+	delete p;
+}
+} // namespace
 
 nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_opt_t *opt)
 {
 	if (!nsh_ensure_device()) return nullptr;
 	if (n < 1) { nsh_set_error("empty extension plan"); return nullptr; }
-	nsh_ext_plan_t *p = new nsh_ext_plan_s();
-	p->n = n;
+	const int32_t asize = opt->asize;
+	if (asize != 22) { nsh_set_error("asize %d not supported (22 expected)", asize); return nullptr; }
+	for (int t = 0; t < 64; ++t)
+		if (opt->codon[t] >= 22) { nsh_set_error("codon table entry %d out of range", t); return nullptr; }
+	for (int t = 0; t < 256; ++t)
+		if (opt->aa20[t] >= 22 || opt->nt4[t] > 4) { nsh_set_error("alphabet table entry %d out of range", t); return nullptr; }
+	if (opt->ge < 0 || opt->go < 0 || (int64_t)opt->ge * CMAX * 64 > 32767) {
+		nsh_set_error("gap penalties go=%d ge=%d out of the supported range", opt->go, opt->ge); return nullptr;
+	}
 	int64_t l_ns = 0, l_as = 0, l_pssm = 0;
 	int32_t max_nl = 2, any_ss = 0, n_grp[kGroups] = {};
 	std::vector<ExtProb> prob(n);
 	std::vector<PrepProb> prep(n);
-	std::vector<uint8_t> h_ns, h_ss, h_as;
-	std::vector<int8_t> h_pssm;
-	std::vector<int32_t> pen;
-	std::vector<int32_t> order(n);
-	PrepConst pc;
-	p->al.resize(n);
-	const int32_t asize = opt->asize;
-	if (asize != 22) { nsh_set_error("asize %d not supported (22 expected)", asize); delete p; return nullptr; }
-	for (int t = 0; t < 64; ++t)
-		if (opt->codon[t] >= 22) { nsh_set_error("codon table entry %d out of range", t); delete p; return nullptr; }
-	for (int t = 0; t < 256; ++t)
-		if (opt->aa20[t] >= 22 || opt->nt4[t] > 4) { nsh_set_error("alphabet table entry %d out of range", t); delete p; return nullptr; }
-	if (opt->ge < 0 || opt->go < 0 || (int64_t)opt->ge * CMAX * 64 > 32767) {
-		nsh_set_error("gap penalties go=%d ge=%d out of the supported range", opt->go, opt->ge); delete p; return nullptr;
-	}
+	std::vector<int32_t> order;
 	std::vector<char> is_wide(n, 0);
 	for (int32_t k = 0; k < n; ++k) {
 		const nsh_ext_req &q = req[k];
 		int32_t C = (q.npad + 127) / 128;
-		if (q.al < 1 || q.nl < 0 || q.npad < q.al) { nsh_set_error("request %d: nl=%d al=%d npad=%d", k, q.nl, q.al, q.npad); delete p; return nullptr; }
+		if (q.al < 1 || q.nl < 0 || q.npad < q.al) { nsh_set_error("request %d: nl=%d al=%d npad=%d", k, q.nl, q.al, q.npad); return nullptr; }
 		if (q.npad > 64 * nsh_glob::kCMaxG * nsh_glob::kNWMax) {
 			nsh_set_error("request %d: padded protein length %d exceeds %d", k, q.npad, 64 * nsh_glob::kCMaxG * nsh_glob::kNWMax);
-			delete p; return nullptr;
+			return nullptr;
 		}
 		if (C < 1) C = 1;
 		is_wide[k] = C > CMAX;
@@ -278,32 +371,24 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		e.nl = q.nl, e.al = q.al, e.npad = q.npad, e.io = q.io, e.out = k, e.C = C, e.left = q.left, e.mode = q.mode;
 		e.rec_off = l_ns, e.aa_off = l_as, e.pssm_off = -1;
 		PrepProb &r = prep[k];
+		r = PrepProb{};
 		r.ns_off = l_ns, r.as_off = l_as, r.rec_off = l_ns, r.aa_off = l_as;
 		r.nl = q.nl, r.al = q.al, r.left = q.left, r.io = q.io, r.has_ss = q.ss != nullptr;
 		if (q.pssm) e.pssm_off = l_pssm, l_pssm += (int64_t)q.al * asize;
 		l_ns += q.nl + 1, l_as += q.al + 1;
 		if (q.nl > max_nl) max_nl = q.nl;
-		p->max_nl = std::max(p->max_nl, q.nl), p->max_al = std::max(p->max_al, q.al);
 		any_ss |= q.ss != nullptr;
 		if (!is_wide[k]) ++n_grp[group_of(C)];
-		p->al[k] = q.al;
-		p->cells_window += (double)q.nl * q.al;
 	}
-	h_ns.resize(l_ns), h_as.resize(l_as);
-	if (any_ss) h_ss.assign(l_ns, 0xff);
-	if (l_pssm) h_pssm.resize(l_pssm);
+	nsh_ext_plan_t *p = shell_acquire();
+	if (!p) return nullptr;
+	p->n = n;
+	p->al.resize(n);
 	for (int32_t k = 0; k < n; ++k) {
-		const nsh_ext_req &q = req[k];
-		if (q.nl) std::memcpy(&h_ns[prep[k].ns_off], q.ns, q.nl);
-		std::memcpy(&h_as[prep[k].as_off], q.as, q.al);
-		if (q.ss && q.nl) std::memcpy(&h_ss[prep[k].ns_off], q.ss, q.nl);
-		if (q.pssm) std::memcpy(&h_pssm[prob[k].pssm_off], q.pssm, (size_t)q.al * asize);
+		p->max_nl = std::max(p->max_nl, req[k].nl), p->max_al = std::max(p->max_al, req[k].al);
+		p->al[k] = req[k].al;
+		p->cells_window += (double)req[k].nl * req[k].al;
 	}
-	pen.resize(max_nl + 2);
-	for (int32_t d = 0; d < (int32_t)pen.size(); ++d) pen[d] = nsh_ie_penalty(opt->ie_coef, d);
-	std::memcpy(pc.nt4, opt->nt4, 256), std::memcpy(pc.aa20, opt->aa20, 256), std::memcpy(pc.codon, opt->codon, 64);
-	for (int t = 0; t < 6; ++t) pc.sp[t] = opt->sp[t];
-	pc.sp_null = opt->sp_null_bonus;
 	// wide problems: descriptors of the work-group kernel, in buckets of equal shape
 	for (int32_t k = 0; k < n; ++k) {
 		if (!is_wide[k]) continue;
@@ -318,50 +403,79 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 	std::stable_sort(p->wide_h.begin(), p->wide_h.end(), [](const nsh_glob::GlobProb &x, const nsh_glob::GlobProb &y) {
 		return x.NW != y.NW? x.NW > y.NW : x.C > y.C; });
 	// queue order: small-C problems first (their launch), then large; longest work first in each
-	order.clear();
 	for (int32_t k = 0; k < n; ++k) if (!is_wide[k]) order.push_back(k);
 	std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
 		const int bx = group_of(prob[x].C), by = group_of(prob[y].C);
 		if (bx != by) return bx < by;
 		return (int64_t)prob[x].nl * prob[x].C > (int64_t)prob[y].nl * prob[y].C; });
-	{
-		std::vector<ExtProb> sorted(order.size());
-		for (size_t k = 0; k < order.size(); ++k) sorted[k] = prob[order[k]];
-		prob.swap(sorted);
-	}
-	const int32_t n_narrow = (int32_t)prob.size();
-	p->prob_h = prob;
+	const int32_t n_narrow = (int32_t)order.size();
+	p->prob_h.resize(n_narrow);
+	for (int32_t k = 0; k < n_narrow; ++k) p->prob_h[k] = prob[order[k]];
 	p->out_h.assign((size_t)4 * n, 0);
-	HIPOK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
-	HIPOK(hipEventCreate(&p->ev0)); HIPOK(hipEventCreate(&p->ev1)); HIPOK(hipEventCreate(&p->ev2));
-	for (int g = 0; g < kGroups; ++g) {
-		HIPOK(hipStreamCreateWithFlags(&p->gs[g], hipStreamNonBlocking));
-		HIPOK(hipEventCreateWithFlags(&p->gev[g], hipEventDisableTiming));
+	// inputs staged in pinned memory: ns | as | ss | pssm | pen | prob | prep | pc | mat | wide
+	const int32_t n_pen = max_nl + 2;
+	const size_t s_ns = 0, s_as = (s_ns + l_ns + 255) & ~(size_t)255, s_ss = (s_as + l_as + 255) & ~(size_t)255;
+	const size_t s_pssm = (s_ss + (any_ss? l_ns : 0) + 255) & ~(size_t)255;
+	const size_t s_pen = (s_pssm + l_pssm + 255) & ~(size_t)255;
+	const size_t s_prob = (s_pen + (size_t)n_pen * 4 + 255) & ~(size_t)255;
+	const size_t s_prep = (s_prob + (size_t)n_narrow * sizeof(ExtProb) + 255) & ~(size_t)255;
+	const size_t s_pc = (s_prep + (size_t)n * sizeof(PrepProb) + 255) & ~(size_t)255;
+	const size_t s_mat = (s_pc + sizeof(PrepConst) + 255) & ~(size_t)255;
+	const size_t s_wide = (s_mat + 484 + 255) & ~(size_t)255;
+	const size_t s_end = s_wide + p->wide_h.size() * sizeof(nsh_glob::GlobProb);
+	uint8_t *hs = nullptr;
+	if (p->h_stage_cap < s_end) {
+		if (p->h_stage) (void)hipHostFree(p->h_stage);
+		p->h_stage = nullptr, p->h_stage_cap = 0;
+		const size_t want = std::max(s_end + (s_end >> 3), (size_t)1 << 20);
+		HIPOK(hipHostMalloc(&p->h_stage, want, hipHostMallocDefault));
+		p->h_stage_cap = want;
 	}
-	HIPOK(dalloc(&p->d_ns, l_ns)); HIPOK(dalloc(&p->d_as, l_as)); HIPOK(dalloc(&p->d_aas, l_as));
-	HIPOK(dalloc(&p->d_rec, l_ns));
-	if (any_ss) HIPOK(dalloc(&p->d_ss, l_ns));
-	HIPOK(dalloc(&p->d_mat, (size_t)asize * asize));
-	if (l_pssm) HIPOK(dalloc(&p->d_pssm, l_pssm));
-	HIPOK(dalloc(&p->d_pen, pen.size())); HIPOK(dalloc(&p->d_counter, kGroups)); HIPOK(dalloc(&p->d_out, (size_t)4 * n));
-	HIPOK(dalloc(&p->d_prob, std::max(1, n_narrow))); HIPOK(dalloc(&p->d_prep, n)); HIPOK(dalloc(&p->d_pc, 1));
-	if (!p->wide_h.empty()) {
-		HIPOK(dalloc(&p->d_wide, p->wide_h.size()));
-		HIPOK(hipStreamCreateWithFlags(&p->ws, hipStreamNonBlocking));
-		HIPOK(hipEventCreateWithFlags(&p->wev, hipEventDisableTiming));
+	hs = (uint8_t*)p->h_stage;
+	for (int32_t k = 0; k < n; ++k) {
+		const nsh_ext_req &q = req[k];
+		if (q.nl) std::memcpy(hs + s_ns + prep[k].ns_off, q.ns, q.nl);
+		std::memcpy(hs + s_as + prep[k].as_off, q.as, q.al);
+		if (any_ss) {
+			if (q.ss) std::memcpy(hs + s_ss + prep[k].ns_off, q.ss, q.nl);
+			else std::memset(hs + s_ss + prep[k].ns_off, 0xff, q.nl);
+		}
+		if (q.pssm) std::memcpy(hs + s_pssm + prob[k].pssm_off, q.pssm, (size_t)q.al * asize);
 	}
-	HIPOK(hipMemcpyAsync(p->d_ns, h_ns.data(), l_ns, hipMemcpyHostToDevice, p->stream));
-	HIPOK(hipMemcpyAsync(p->d_as, h_as.data(), l_as, hipMemcpyHostToDevice, p->stream));
-	if (any_ss) HIPOK(hipMemcpyAsync(p->d_ss, h_ss.data(), l_ns, hipMemcpyHostToDevice, p->stream));
-	HIPOK(hipMemcpyAsync(p->d_mat, opt->sc, (size_t)asize * asize, hipMemcpyHostToDevice, p->stream));
-	if (l_pssm) HIPOK(hipMemcpyAsync(p->d_pssm, h_pssm.data(), l_pssm, hipMemcpyHostToDevice, p->stream));
-	HIPOK(hipMemcpyAsync(p->d_pen, pen.data(), pen.size() * 4, hipMemcpyHostToDevice, p->stream));
-	if (n_narrow) HIPOK(hipMemcpyAsync(p->d_prob, prob.data(), (size_t)n_narrow * sizeof(ExtProb), hipMemcpyHostToDevice, p->stream));
-	if (!p->wide_h.empty())
-		HIPOK(hipMemcpyAsync(p->d_wide, p->wide_h.data(), p->wide_h.size() * sizeof(nsh_glob::GlobProb), hipMemcpyHostToDevice, p->stream));
-	HIPOK(hipMemcpyAsync(p->d_prep, prep.data(), (size_t)n * sizeof(PrepProb), hipMemcpyHostToDevice, p->stream));
-	HIPOK(hipMemcpyAsync(p->d_pc, &pc, sizeof(pc), hipMemcpyHostToDevice, p->stream));
-	HIPOK(hipStreamSynchronize(p->stream));
+	{
+		int32_t *pen = (int32_t*)(hs + s_pen);
+		for (int32_t d = 0; d < n_pen; ++d) pen[d] = nsh_ie_penalty(opt->ie_coef, d);
+		PrepConst &pc = *(PrepConst*)(hs + s_pc);
+		std::memcpy(pc.nt4, opt->nt4, 256), std::memcpy(pc.aa20, opt->aa20, 256), std::memcpy(pc.codon, opt->codon, 64);
+		for (int t = 0; t < 6; ++t) pc.sp[t] = opt->sp[t];
+		pc.sp_null = opt->sp_null_bonus;
+		if (n_narrow) std::memcpy(hs + s_prob, p->prob_h.data(), (size_t)n_narrow * sizeof(ExtProb));
+		std::memcpy(hs + s_prep, prep.data(), (size_t)n * sizeof(PrepProb));
+		std::memcpy(hs + s_mat, opt->sc, (size_t)asize * asize);
+		if (!p->wide_h.empty()) std::memcpy(hs + s_wide, p->wide_h.data(), p->wide_h.size() * sizeof(nsh_glob::GlobProb));
+	}
+	HIPOK(p->b_ns.ensure(l_ns)); HIPOK(p->b_as.ensure(l_as)); HIPOK(p->b_aas.ensure(l_as)); HIPOK(p->b_rec.ensure(l_ns));
+	if (any_ss) HIPOK(p->b_ss.ensure(l_ns));
+	HIPOK(p->b_mat.ensure((size_t)asize * asize));
+	if (l_pssm) HIPOK(p->b_pssm.ensure(l_pssm));
+	HIPOK(p->b_pen.ensure(n_pen)); HIPOK(p->b_counter.ensure(kGroups)); HIPOK(p->b_out.ensure((size_t)4 * n));
+	HIPOK(p->b_prob.ensure(std::max(1, n_narrow))); HIPOK(p->b_prep.ensure(n)); HIPOK(p->b_pc.ensure(1));
+	if (!p->wide_h.empty()) HIPOK(p->b_wide.ensure(p->wide_h.size()));
+	p->d_ns = p->b_ns.p, p->d_as = p->b_as.p, p->d_aas = p->b_aas.p, p->d_rec = p->b_rec.p;
+	p->d_ss = any_ss? p->b_ss.p : nullptr, p->d_mat = p->b_mat.p, p->d_pssm = l_pssm? p->b_pssm.p : nullptr;
+	p->d_pen = p->b_pen.p, p->d_counter = p->b_counter.p, p->d_out = p->b_out.p, p->d_prob = p->b_prob.p;
+	p->d_prep = p->b_prep.p, p->d_pc = p->b_pc.p, p->d_wide = p->wide_h.empty()? nullptr : p->b_wide.p;
+	{   // from pinned memory: asynchronous; the stream orders them before the runs
+		auto cp = [&](void *dst, size_t off, size_t bytes) {
+			return bytes == 0 || hipMemcpyAsync(dst, hs + off, bytes, hipMemcpyHostToDevice, p->stream) == hipSuccess;
+		};
+		const bool ok = cp(p->d_ns, s_ns, l_ns) && cp(p->d_as, s_as, l_as) && (!any_ss || cp(p->d_ss, s_ss, l_ns)) &&
+			(!l_pssm || cp(p->d_pssm, s_pssm, l_pssm)) && cp(p->d_pen, s_pen, (size_t)n_pen * 4) &&
+			cp(p->d_prob, s_prob, (size_t)n_narrow * sizeof(ExtProb)) && cp(p->d_prep, s_prep, (size_t)n * sizeof(PrepProb)) &&
+			cp(p->d_pc, s_pc, sizeof(PrepConst)) && cp(p->d_mat, s_mat, (size_t)asize * asize) &&
+			(p->wide_h.empty() || cp(p->d_wide, s_wide, p->wide_h.size() * sizeof(nsh_glob::GlobProb)));
+		if (!ok) { nsh_set_error("extension plan: copy to the device failed"); goto fail; }
+	}
 	{
 		int dev = 0, ncu = 0;
 		(void)hipGetDevice(&dev);
@@ -375,7 +489,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 			a.prob = p->d_prob + first, a.n = n_grp[g];
 			first += n_grp[g];
 			a.rec = p->d_rec, a.aas = p->d_aas, a.pssm = p->d_pssm, a.mat = p->d_mat, a.asize = asize;
-			a.pen = p->d_pen, a.pen_n = (int32_t)pen.size();
+			a.pen = p->d_pen, a.pen_n = n_pen;
 			a.go = opt->go, a.ge = opt->ge, a.fs = opt->fs, a.xdrop = opt->xdrop, a.eb = opt->end_bonus;
 			a.counter = p->d_counter + g, a.out = p->d_out;
 			// persistent waves: about as many as the CUs keep resident
@@ -385,12 +499,11 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		nsh_glob::GlobArgs &w = p->wargs;
 		w.rec = p->d_rec, w.aas = p->d_aas, w.pssm = p->d_pssm, w.mat = p->d_mat, w.asize = asize;
 		w.go = opt->go, w.ge = opt->ge, w.fs = opt->fs, w.trace = nullptr, w.score = nullptr;
-		w.pen = p->d_pen, w.pen_n = (int32_t)pen.size(), w.xdrop = opt->xdrop, w.eb = opt->end_bonus, w.ext_out = p->d_out;
+		w.pen = p->d_pen, w.pen_n = n_pen, w.xdrop = opt->xdrop, w.eb = opt->end_bonus, w.ext_out = p->d_out;
 	}
 	return p;
 fail:
-	plan_free_dev(p);
-	delete p;
+	shell_release(p);
 	return nullptr;
 }
 
@@ -487,8 +600,8 @@ extern "C" int nsh_ext_plan_run_subset(nsh_ext_plan_t *p, const uint8_t *keep)
 	if (!p || !keep) { nsh_set_error("nsh_ext_plan_run_subset: bad arguments"); return 0; }
 	ExtArgs args[kGroups];
 	int32_t grid[kGroups] = {};
-	if (!p->d_prob_sub) HIPOK(dalloc(&p->d_prob_sub, p->n));
-	if (!p->d_idx_sub) HIPOK(dalloc(&p->d_idx_sub, p->n));
+	HIPOK(p->b_prob_sub.ensure(p->n)); HIPOK(p->b_idx_sub.ensure(p->n));
+	p->d_prob_sub = p->b_prob_sub.p, p->d_idx_sub = p->b_idx_sub.p;
 	HIPOK(hipStreamSynchronize(p->stream));   // the previous subset lists may still be in flight
 	p->sub_h.clear(), p->idx_h.clear();
 	for (int32_t k = 0; k < p->n; ++k)
@@ -503,7 +616,7 @@ extern "C" int nsh_ext_plan_run_subset(nsh_ext_plan_t *p, const uint8_t *keep)
 	}
 	p->wide_sub_h.clear();
 	for (const nsh_glob::GlobProb &g : p->wide_h) if (keep[g.out]) p->wide_sub_h.push_back(g);
-	if (!p->wide_sub_h.empty() && !p->d_wide_sub) HIPOK(dalloc(&p->d_wide_sub, p->wide_h.size()));
+	if (!p->wide_sub_h.empty()) { HIPOK(p->b_wide_sub.ensure(p->wide_h.size())); p->d_wide_sub = p->b_wide_sub.p; }
 	if (!p->sub_h.empty())
 		HIPOK(hipMemcpyAsync(p->d_prob_sub, p->sub_h.data(), p->sub_h.size() * sizeof(ExtProb), hipMemcpyHostToDevice, p->stream));
 	if (!p->idx_h.empty())
@@ -523,7 +636,7 @@ extern "C" const float *nsh_ext_plan_features(nsh_ext_plan_t *p, float *host_out
 		nsh_set_error("nsh_ext_plan_features: candidates limited to al <= 4096, nl <= 24000 (plan has %d, %d)", p->max_al, p->max_nl);
 		return nullptr;
 	}
-	if (!p->d_feat) HIPOK(dalloc(&p->d_feat, (size_t)p->n * 64));
+	if (!p->d_feat) { HIPOK(p->b_feat.ensure((size_t)p->n * 64)); p->d_feat = p->b_feat.p; }
 	nsh_feat::launch(p->d_prep, p->n, p->d_ns, p->d_as, p->d_pc, p->d_mat, p->d_feat, p->stream);
 	HIPOK(hipGetLastError());
 	if (host_out) HIPOK(hipMemcpyAsync(host_out, p->d_feat, (size_t)p->n * 64 * 4, hipMemcpyDeviceToHost, p->stream));
@@ -776,17 +889,5 @@ extern "C" int nsh_ext_run_noalloc(int32_t n, const char *ns, const int64_t *ns_
 extern "C" void nsh_ext_plan_destroy(nsh_ext_plan_t *p)
 {
 	if (!p) return;
-	if (p->stream) (void)hipStreamSynchronize(p->stream);
-	plan_free_dev(p);
-	if (p->ev0) (void)hipEventDestroy(p->ev0);
-	if (p->ev1) (void)hipEventDestroy(p->ev1);
-	if (p->ev2) (void)hipEventDestroy(p->ev2);
-	for (int g = 0; g < kGroups; ++g) {
-		if (p->gev[g]) (void)hipEventDestroy(p->gev[g]);
-		if (p->gs[g]) (void)hipStreamDestroy(p->gs[g]);
-	}
-	if (p->wev) (void)hipEventDestroy(p->wev);
-	if (p->ws) (void)hipStreamDestroy(p->ws);
-	if (p->stream) (void)hipStreamDestroy(p->stream);
-	delete p;
+	shell_release(p);   // waits for the plan's work; its resources go back to the pool (or are freed)
 }

@@ -7,6 +7,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <chrono>
+#include <thread>
+#include <deque>
+#include <condition_variable>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -161,26 +165,8 @@ extern "C" uint64_t ns_ext_gpu_request_count(void) { return g_ext_req.load(); }
 extern "C" int32_t ns_ext_stripe_width(const ns_opt_t *opt, int32_t al) { return nsh_host_stripe_width(opt->flag, al); }
 extern "C" int32_t ns_ext_ie_penalty(const ns_opt_t *opt, int32_t d) { return nsh_ie_penalty(opt->ie_coef, d); }
 
-// one extension request through a single-request plan
-static int nsh_ext_one(const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, const uint8_t *ss, ns_rst_t *r)
-{
-	nsh_ext_req q;
-	q.ns = ns, q.nl = nl, q.as = as, q.al = al, q.ss = ss;
-	q.left = (opt->flag & NS_F_EXT_LEFT) != 0, q.io = opt->io;
-	q.mode = (opt->flag & (NS_F_EXT_LEFT | NS_F_EXT_RIGHT)) == 0;
-	q.pssm = opt->pssm;
-	const int32_t w = nsh_host_stripe_width(opt->flag, al);
-	q.npad = (al + w - 1) / w * w;
-	nsh_ext_plan_t *p = nsh_ext_plan_build(1, &q, opt);
-	if (!p) return 0;
-	nsh_ext_result_t res;
-	int ok = nsh_ext_plan_run(p) && nsh_ext_plan_results(p, &res);
-	nsh_ext_plan_destroy(p);
-	if (!ok) return 0;
-	r->n_cigar = 0, r->nt_len = res.nt_len, r->aa_len = res.aa_len, r->score = res.score;
-	++g_dispatch, ++g_ext_launch;
-	return 1;
-}
+// one extension request, queued in the shared extension executor (below) and waited for
+static int nsh_ext_one(const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, const uint8_t *ss, ns_rst_t *r);
 
 static void nsh_fail_result(ns_rst_t *r, int32_t nl, int32_t al)
 {
@@ -289,19 +275,24 @@ extern "C" int ns_gpu_global(void *km, const char *ns, int32_t nl, const char *a
 }
 
 // ---------------------------------------------------------------------------------------
-// batched extension API (nasw.h:196-212). A flush turns the requests added since the last
-// flush into one generation: requests with identical options (io, direction and the PSSM rows
-// may differ) share one device plan; the plans run asynchronously on their own streams.
+// batched extension API (nasw.h:196-212) and single extension calls, on one shared executor.
+//
+// A flush turns the requests added since the last flush into a generation and queues it; a
+// synchronous ns_global_gs16b(EXT) call queues a generation of one. One executor thread takes every
+// queued generation of every batch and thread at once (after a 1-ms window, at once when a caller
+// is waiting, or at 65,536 queued requests), groups their requests by options (io, direction and
+// the PSSM rows may differ within a group) and runs one device plan per group; plans come from a
+// pool, so a dispatch allocates nothing once the pool has grown to the workload. Inputs stay the
+// caller's until their results are fetched (nasw.h:196-209), as the reference requires.
 // ---------------------------------------------------------------------------------------
 namespace {
+struct ExtReq { const char *ns, *as; const uint8_t *ss; int32_t nl, al; ns_opt_t opt; nsh_ext_result_t res; int ok; };
 struct ExtGen {
-	int32_t base = 0, n = 0, n_got = 0;
-	std::vector<ns_rst_t> rst;
-	std::vector<nsh_ext_plan_t*> plans;
-	std::vector<std::vector<int32_t>> members; // request index (within the generation) per plan slot
-	bool finalized = false, failed = false;
+	std::vector<ExtReq> req;
+	std::vector<ns_rst_t> rst;    // filled when done
+	int32_t base = 0, n_got = 0;
+	int done = 0;
 };
-struct ExtReq { const char *ns, *as; const uint8_t *ss; int32_t nl, al; ns_opt_t opt; };
 
 bool opt_same(const ns_opt_t &a, const ns_opt_t &b)
 {
@@ -313,7 +304,130 @@ bool opt_same(const ns_opt_t &a, const ns_opt_t &b)
 		(a.aa20 == b.aa20 || std::memcmp(a.aa20, b.aa20, 256) == 0) &&
 		(a.codon == b.codon || std::memcmp(a.codon, b.codon, 64) == 0);
 }
+
+class ExtExec {
+public:
+	std::mutex mu;
+	std::condition_variable cv_work, cv_done;
+	std::deque<ExtGen*> queue;
+	size_t queued_req = 0;
+	int n_waiting = 0;
+	bool started = false;
+
+	void submit(ExtGen *g)
+	{
+		std::lock_guard<std::mutex> lk(mu);
+		if (!started) { started = true; std::thread([this] { loop(); }).detach(); }
+		queue.push_back(g);
+		queued_req += g->req.size();
+		cv_work.notify_one();
+	}
+	bool is_done(ExtGen *g)
+	{
+		std::lock_guard<std::mutex> lk(mu);
+		return g->done != 0;
+	}
+	// blocks until pred() holds (checked under the lock whenever a generation completes)
+	template <class P> void wait_until(P pred)
+	{
+		std::unique_lock<std::mutex> lk(mu);
+		if (pred()) return;
+		++n_waiting;
+		cv_work.notify_one();   // a waiter ends the batching window
+		cv_done.wait(lk, pred);
+		--n_waiting;
+	}
+	void loop()
+	{
+		for (;;) {
+			std::vector<ExtGen*> gens;
+			{
+				std::unique_lock<std::mutex> lk(mu);
+				cv_work.wait(lk, [&] { return !queue.empty(); });
+				const auto until = std::chrono::steady_clock::now() + std::chrono::milliseconds(1);
+				while (n_waiting == 0 && queued_req < 65536 && cv_work.wait_until(lk, until) != std::cv_status::timeout) {}
+				gens.assign(queue.begin(), queue.end());
+				queue.clear();
+				queued_req = 0;
+			}
+			run(gens);
+			{
+				std::lock_guard<std::mutex> lk(mu);
+				for (ExtGen *g : gens) g->done = 1;
+			}
+			cv_done.notify_all();
+		}
+	}
+	void run(std::vector<ExtGen*> &gens)
+	{
+		std::vector<ExtReq*> all;
+		for (ExtGen *g : gens) for (ExtReq &r : g->req) all.push_back(&r);
+		std::vector<char> taken(all.size(), 0);
+		bool failed = false;
+		for (size_t a = 0; a < all.size(); ++a) {
+			if (taken[a]) continue;
+			std::vector<ExtReq*> grp;
+			for (size_t b = a; b < all.size(); ++b)
+				if (!taken[b] && opt_same(all[a]->opt, all[b]->opt)) taken[b] = 1, grp.push_back(all[b]);
+			std::vector<nsh_ext_req> req(grp.size());
+			for (size_t k = 0; k < grp.size(); ++k) {
+				const ExtReq &x = *grp[k];
+				nsh_ext_req &q = req[k];
+				q.ns = x.ns, q.nl = x.nl, q.as = x.as, q.al = x.al, q.ss = x.ss;
+				q.left = (x.opt.flag & NS_F_EXT_LEFT) != 0, q.io = x.opt.io;
+				q.mode = (x.opt.flag & (NS_F_EXT_LEFT | NS_F_EXT_RIGHT)) == 0;
+				q.pssm = x.opt.pssm;
+				const int32_t w = nsh_host_stripe_width(x.opt.flag, x.al);
+				q.npad = (x.al + w - 1) / w * w;
+			}
+			std::vector<nsh_ext_result_t> res(grp.size());
+			nsh_ext_plan_t *p = nsh_ext_plan_build((int32_t)req.size(), req.data(), &grp[0]->opt);
+			const int ok = p && nsh_ext_plan_run(p) && nsh_ext_plan_results(p, res.data());
+			if (p) nsh_ext_plan_destroy(p), g_ext_launch += 1;
+			for (size_t k = 0; k < grp.size(); ++k) grp[k]->ok = ok, grp[k]->res = res[k];
+			failed |= !ok;
+		}
+		if (failed) std::fprintf(stderr, "[nasw_hip] extension batch failed on the GPU: %s\n", ns_gpu_error());
+		for (ExtGen *g : gens) {
+			g->rst.resize(g->req.size());
+			for (size_t k = 0; k < g->req.size(); ++k) {
+				ns_rst_t &r = g->rst[k];
+				ns_rst_init(&r);
+				const ExtReq &x = g->req[k];
+				if (x.ok) r.nt_len = x.res.nt_len, r.aa_len = x.res.aa_len, r.score = x.res.score;
+				else r.nt_len = x.nl, r.aa_len = x.al, r.score = INT32_MIN;
+			}
+		}
+	}
+};
+
+ExtExec &ext_exec()
+{
+	static ExtExec *e = new ExtExec();   // never destroyed: its thread lives until the process ends
+	return *e;
+}
+
+ExtReq make_req(const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, const uint8_t *ss)
+{
+	ExtReq q;
+	q.ns = ns, q.as = as, q.ss = ss, q.nl = nl, q.al = al, q.opt = *opt, q.ok = 0, q.res = {};
+	q.opt.flag &= ~(NS_F_CIGAR | NS_F_GPU);
+	return q;
+}
 } // namespace
+
+static int nsh_ext_one(const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, const uint8_t *ss, ns_rst_t *r)
+{
+	if (!nsh_ensure_device()) return 0;
+	ExtGen g;
+	g.req.push_back(make_req(ns, nl, as, al, opt, ss));
+	ext_exec().submit(&g);
+	ext_exec().wait_until([&] { return g.done != 0; });
+	if (!g.req[0].ok) return 0;
+	r->n_cigar = 0, r->nt_len = g.req[0].res.nt_len, r->aa_len = g.req[0].res.aa_len, r->score = g.req[0].res.score;
+	++g_dispatch;
+	return 1;
+}
 
 struct ns_ext_batch_s {
 	void *km = nullptr;
@@ -332,40 +446,15 @@ extern "C" ns_ext_batch_t *ns_ext_batch_init(void *km)
 
 extern "C" int32_t ns_ext_batch_add(ns_ext_batch_t *b, const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt, const uint8_t *ss)
 {
-	ExtReq q;
-	q.ns = ns, q.as = as, q.ss = ss, q.nl = nl, q.al = al, q.opt = *opt;
-	q.opt.flag &= ~(NS_F_CIGAR | NS_F_GPU);
-	b->pending.push_back(q);
+	b->pending.push_back(make_req(ns, nl, as, al, opt, ss));
 	return b->next_id++;
-}
-
-static void gen_finalize(ExtGen *g)
-{
-	if (g->finalized) return;
-	std::vector<nsh_ext_result_t> res;
-	for (size_t s = 0; s < g->plans.size(); ++s) {
-		const std::vector<int32_t> &mem = g->members[s];
-		res.resize(mem.size());
-		nsh_ext_plan_t *p = g->plans[s];
-		const int ok = p && nsh_ext_plan_results(p, res.data());
-		for (size_t k = 0; k < mem.size(); ++k) {
-			ns_rst_t &r = g->rst[mem[k]];
-			ns_rst_init(&r);
-			if (ok) r.nt_len = res[k].nt_len, r.aa_len = res[k].aa_len, r.score = res[k].score;
-			else r.score = INT32_MIN, g->failed = true;
-		}
-		nsh_ext_plan_destroy(p);
-		g->plans[s] = nullptr;
-	}
-	if (g->failed) std::fprintf(stderr, "[nasw_hip] extension batch failed on the GPU: %s\n", ns_gpu_error());
-	g->finalized = true;
 }
 
 static void batch_reap(ns_ext_batch_t *b)
 {
 	std::vector<ExtGen*> keep;
 	for (ExtGen *g : b->gens) {
-		if (g->finalized && g->n_got >= g->n) delete g;
+		if (ext_exec().is_done(g) && g->n_got >= (int32_t)g->req.size()) delete g;
 		else keep.push_back(g);
 	}
 	b->gens.swap(keep);
@@ -378,86 +467,43 @@ extern "C" int32_t ns_ext_batch_flush(ns_ext_batch_t *b)
 	const int32_t n = (int32_t)b->pending.size();
 	if (n == 0) return 0;
 	ExtGen *g = new ExtGen();
-	g->base = b->next_id - n, g->n = n;
-	g->rst.resize(n);
-	// group by options (io and direction are per request)
-	std::vector<int32_t> group(n, -1);
-	std::vector<const ns_opt_t*> reps;
-	for (int32_t k = 0; k < n; ++k) {
-		int32_t s = 0;
-		for (; s < (int32_t)reps.size(); ++s) if (opt_same(*reps[s], b->pending[k].opt)) break;
-		if (s == (int32_t)reps.size()) reps.push_back(&b->pending[k].opt);
-		group[k] = s;
-	}
-	g->members.resize(reps.size());
-	for (int32_t k = 0; k < n; ++k) g->members[group[k]].push_back(k);
-	for (size_t s = 0; s < reps.size(); ++s) {
-		const std::vector<int32_t> &mem = g->members[s];
-		std::vector<nsh_ext_req> req(mem.size());
-		for (size_t k = 0; k < mem.size(); ++k) {
-			const ExtReq &x = b->pending[mem[k]];
-			nsh_ext_req &q = req[k];
-			q.ns = x.ns, q.nl = x.nl, q.as = x.as, q.al = x.al, q.ss = x.ss;
-			q.left = (x.opt.flag & NS_F_EXT_LEFT) != 0, q.io = x.opt.io;
-			q.pssm = x.opt.pssm;   // per request: requests that differ only in their PSSM share a plan
-			const int32_t w = nsh_host_stripe_width(x.opt.flag, x.al);
-			q.npad = (x.al + w - 1) / w * w;
-		}
-		nsh_ext_plan_t *p = nsh_ext_plan_build((int32_t)req.size(), req.data(), reps[s]);
-		if (p && !nsh_ext_plan_run(p)) nsh_ext_plan_destroy(p), p = nullptr;
-		g->plans.push_back(p);
-		if (p) g_ext_launch += 1;
-	}
+	g->base = b->next_id - n;
+	g->req.swap(b->pending);
+	b->gens.push_back(g);
 	g_ext_req += n;
 	g_dispatch += n;
-	b->pending.clear();
-	b->gens.push_back(g);
+	ext_exec().submit(g);
 	return n;
 }
 
 static ExtGen *batch_find(ns_ext_batch_t *b, int32_t id)
 {
 	for (ExtGen *g : b->gens)
-		if (id >= g->base && id < g->base + g->n) return g;
+		if (id >= g->base && id < g->base + (int32_t)g->req.size()) return g;
 	return nullptr;
-}
-
-static bool gen_done(ExtGen *g)
-{
-	if (g->finalized) return true;
-	for (nsh_ext_plan_t *p : g->plans)
-		if (p && nsh_ext_plan_query(p) == 0) return false;
-	return true;
 }
 
 extern "C" int ns_ext_batch_ready(ns_ext_batch_t *b, int32_t id)
 {
 	ExtGen *g = batch_find(b, id);
-	if (!g) return 0;
-	if (!gen_done(g)) return 0;
-	gen_finalize(g);
-	return 1;
+	return g && ext_exec().is_done(g);
 }
 
 extern "C" void ns_ext_batch_wait(ns_ext_batch_t *b)
 {
 	if (!b) return;
-	// block until some unfinalized generation completes (or none is outstanding)
-	for (ExtGen *g : b->gens) {
-		if (g->finalized) continue;
-		for (nsh_ext_plan_t *p : g->plans) if (p) nsh_ext_plan_sync(p);
-		return;
-	}
+	// block until some generation of this batch that is still running completes (or none is)
+	std::vector<ExtGen*> open;
+	for (ExtGen *g : b->gens) if (!ext_exec().is_done(g)) open.push_back(g);
+	if (open.empty()) return;
+	ext_exec().wait_until([&] { for (ExtGen *g : open) if (g->done) return true; return false; });
 }
 
 extern "C" const ns_rst_t *ns_ext_batch_get(ns_ext_batch_t *b, int32_t id)
 {
 	ExtGen *g = batch_find(b, id);
 	if (!g) return nullptr;
-	if (!g->finalized) {
-		for (nsh_ext_plan_t *p : g->plans) if (p) nsh_ext_plan_sync(p);
-		gen_finalize(g);
-	}
+	ext_exec().wait_until([&] { return g->done != 0; });
 	++g->n_got;
 	return &g->rst[id - g->base];
 }
@@ -465,8 +511,8 @@ extern "C" const ns_rst_t *ns_ext_batch_get(ns_ext_batch_t *b, int32_t id)
 extern "C" void ns_ext_batch_destroy(ns_ext_batch_t *b)
 {
 	if (!b) return;
-	for (ExtGen *g : b->gens) {
-		for (nsh_ext_plan_t *p : g->plans) if (p) nsh_ext_plan_destroy(p);
+	for (ExtGen *g : b->gens) {   // the executor may still be working on a generation
+		ext_exec().wait_until([&] { return g->done != 0; });
 		delete g;
 	}
 	delete b;

@@ -81,10 +81,10 @@ def test_pssm_batch_one_plan(oracle, product):
         reqs.append((L.ns_ext_batch_add(b, ns, len(ns), ab, len(a), C.byref(o), None), ns, a, kw))
     before = L.ns_ext_gpu_launch_count()
     assert L.ns_ext_batch_flush(b) == len(reqs)
-    assert L.ns_ext_gpu_launch_count() - before == 1     # the PSSM pointer does not split the plan
     for rid, ns, a, kw in reqs:
         r = L.ns_ext_batch_get(b, rid).contents
         assert (r.nt_len, r.aa_len, r.score) == _ora(oracle, product, ns, a, kw), (len(a), kw["flag"], "pssm" in kw)
+    assert L.ns_ext_gpu_launch_count() - before == 1     # the PSSM pointer does not split the plan
     L.ns_ext_batch_destroy(b)
 
 
