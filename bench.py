@@ -125,6 +125,130 @@ def cpu_baseline(pairs, kw, seconds=15.0, packed=None):
             "cpu": host_cpu(), "sample": "%d of the pairs, %.1f s wall (oracle port, Python threads)" % (n, dt)}, None
 
 
+def _drivers():
+    lib = C.CDLL(os.path.join(ROOT, "protlocus_amd", "libnsh_drivers.so"))
+    for f in (lib.nshd_ext_batch,):
+        f.argtypes = [C.c_int32, C.c_char_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_char_p,
+                      C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
+    lib.nshd_ext_single.argtypes = [C.c_int32, C.c_char_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p,
+                                    C.c_char_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+    lib.nshd_gpu_submit.argtypes = [C.c_int32, C.c_char_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
+    for f in (lib.nshd_ext_batch, lib.nshd_ext_single, lib.nshd_gpu_submit):
+        f.restype = C.c_int32
+    return lib
+
+
+def bench_dropin(prod, pairs, packed, kw, ref_res, log, n_single=16384, n_global=16384, cpu=True):
+    """the drop-in path: the reference's entry points driven from C threads the way align.c calls
+    them (extension batches per query, single extension calls, ns_gpu_submit/wait of gap DPs),
+    against the resident-plan headline; results against the reference's"""
+    import numpy as np
+    import nsffi
+    from protlocus_amd import synth
+    drv = _drivers()
+    nt = threads()
+    ns, ns_off, nl, as_, as_off, al = packed
+    ns_off = np.ascontiguousarray(ns_off, dtype=np.int64)
+    as_off = np.ascontiguousarray(as_off, dtype=np.int64)
+    nl = np.ascontiguousarray(nl, dtype=np.int32)
+    al = np.ascontiguousarray(al, dtype=np.int32)
+    opt = prod.opt(**kw)
+    win = float((nl.astype(np.float64) * al).sum())
+    out = {"threads": nt, "workload": "the %d headline pairs" % len(pairs)}
+    res = np.zeros(3 * len(pairs), dtype=np.int32)
+    wall = C.c_double()
+    for rep in range(2):   # the first pass grows the plan pool
+        ok = drv.nshd_ext_batch(len(pairs), ns, ns_off.ctypes.data, nl.ctypes.data, as_, as_off.ctypes.data,
+                                al.ctypes.data, None, C.byref(opt), nt, 8, res.ctypes.data, C.byref(wall))
+    r = res.reshape(-1, 3)
+    same = None if ref_res is None else int((r == ref_res).all(axis=1).sum())
+    out["ext_batch_api"] = {"pattern": "per group of 8 requests (one query's candidates): ns_ext_batch_init, add, flush, "
+                                       "ready/wait, get, destroy; %d C threads" % nt,
+                            "ok": bool(ok), "wall_s": round(wall.value, 3), "requests_per_s": round(len(pairs) / wall.value, 1),
+                            "window_gcups": round(win / wall.value / 1e9, 2),
+                            "identical_to_reference": None if same is None else "%d of %d" % (same, len(pairs))}
+    log("[bench] drop-in ext batch API: %.1f window GCUPS, %.0f req/s" % (win / wall.value / 1e9, len(pairs) / wall.value))
+    m = min(n_single, len(pairs))
+    res1 = np.zeros(3 * m, dtype=np.int32)
+    ok = drv.nshd_ext_single(m, ns, ns_off.ctypes.data, nl.ctypes.data, as_, as_off.ctypes.data, al.ctypes.data, None,
+                             C.byref(opt), nt, res1.ctypes.data, C.byref(wall))
+    r1 = res1.reshape(-1, 3)
+    same1 = None if ref_res is None else int((r1 == ref_res[:m]).all(axis=1).sum())
+    win1 = float((nl[:m].astype(np.float64) * al[:m]).sum())
+    out["ext_single_calls"] = {"pattern": "one ns_global_gs16b per request from %d C threads" % nt, "requests": m,
+                               "ok": bool(ok), "wall_s": round(wall.value, 3), "requests_per_s": round(m / wall.value, 1),
+                               "window_gcups": round(win1 / wall.value / 1e9, 2),
+                               "identical_to_reference": None if same1 is None else "%d of %d" % (same1, m)}
+    log("[bench] drop-in single ext calls: %.1f window GCUPS" % (win1 / wall.value / 1e9))
+    # global DP with CIGAR through ns_gpu_submit / ns_gpu_wait
+    gp = synth.extension_batch(n_global, seed=5, min_len=20, max_len=400)
+    gns, gns_off, gnl, gas, gas_off, gal = synth.pack_batch(gp)
+    gns_off = np.ascontiguousarray(gns_off, dtype=np.int64)
+    gas_off = np.ascontiguousarray(gas_off, dtype=np.int64)
+    gnl = np.ascontiguousarray(gnl, dtype=np.int32)
+    gal = np.ascontiguousarray(gal, dtype=np.int32)
+    gopt = prod.opt(flag=nsffi.NS_F_CIGAR)
+    cells = float(((gnl.astype(np.float64) - 2).clip(0) * gal).sum())
+    gres = np.zeros(3 * n_global, dtype=np.int32)
+    L = prod.lib
+    L.ns_gpu_kernel_seconds.restype = C.c_double
+    L.ns_gpu_d2h_seconds.restype = C.c_double
+    for rep in range(2):
+        k0, d0 = L.ns_gpu_kernel_seconds(), L.ns_gpu_d2h_seconds()
+        ok = drv.nshd_gpu_submit(n_global, gns, gns_off.ctypes.data, gnl.ctypes.data, gas, gas_off.ctypes.data,
+                                 gal.ctypes.data, C.byref(gopt), nt, 64, gres.ctypes.data, C.byref(wall))
+        k1, d1 = L.ns_gpu_kernel_seconds(), L.ns_gpu_d2h_seconds()
+    gr = gres.reshape(-1, 3)
+    gcups = cells / wall.value / 1e9
+    g = {"workload": "%d global CIGAR problems (extension-batch generator, protein 20-400 aa, 200-nt flanks), "
+                     "cells = (nl - 2) x al" % n_global,
+         "pattern": "per group of 64: ns_gpu_submit of each, then ns_gpu_wait of each; %d C threads" % nt,
+         "ok": bool(ok), "wall_s": round(wall.value, 3), "cells": cells, "gcups": round(gcups, 2),
+         "engine_batch_s": round(k1 - k0, 3), "cigar_d2h_s": round(d1 - d0, 4),
+         "roofline": {"bound": "valu", "achieved": round(gcups * 31 / 1e3, 3), "peak": round(VALU_I32_TOPS, 2),
+                      "unit": "TOP/s", "frac": round(gcups * 31 / 1e3 / VALU_I32_TOPS, 4), "ops_per_cell": 31}}
+    if cpu and nsffi.have_ref():
+        ref = nsffi.Reference()
+        L2 = ref.lib
+        L2.or_ref_global_batch.argtypes = [C.c_int32, C.c_char_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_void_p,
+                                           C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+        ms = min(n_global, 4096)
+        rr = np.zeros(2 * ms, dtype=np.int32)
+        ropt = ref.opt(flag=nsffi.NS_F_CIGAR)
+        L2.or_ref_global_batch(ms, gns, gns_off.ctypes.data, gnl.ctypes.data, gas, gas_off.ctypes.data, gal.ctypes.data,
+                               C.byref(ropt), nt, rr.ctypes.data, C.byref(wall))
+        rr = rr.reshape(-1, 2)
+        same_g = int((rr == gr[:ms, :2]).all(axis=1).sum())
+        ccells = float(((gnl[:ms].astype(np.float64) - 2).clip(0) * gal[:ms]).sum())
+        g["identical_to_reference"] = "%d of %d (score and CIGAR length)" % (same_g, ms)
+        g["cpu_baseline"] = {"value": round(ccells / wall.value / 1e9, 3), "unit": "GCUPS", "cores": nt, "kind": "reference",
+                             "cpu": host_cpu(), "sample": "first %d problems, %.2f s, ns_global_gs16b with CIGAR" % (ms, wall.value)}
+    # the same problems as one engine batch (nsh_global_batch), as a pipeline hands them over
+    L.nsh_global_batch.argtypes = [C.c_int32, C.c_char_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p,
+                                   C.c_char_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                   C.POINTER(C.POINTER(C.c_uint32))]
+    sc = np.zeros(n_global, dtype=np.int32)
+    nc = np.zeros(n_global, dtype=np.int32)
+    best = 1e30
+    for rep in range(3):
+        cig = C.POINTER(C.c_uint32)()
+        t0 = time.perf_counter()
+        okb = L.nsh_global_batch(n_global, gns, gns_off.ctypes.data, gnl.ctypes.data, gas, gas_off.ctypes.data,
+                                 gal.ctypes.data, None, C.byref(gopt), 0, sc.ctypes.data, nc.ctypes.data, C.byref(cig))
+        best = min(best, time.perf_counter() - t0)
+        prod.free(C.cast(cig, C.c_void_p))
+    bg = cells / best / 1e9
+    g["batch"] = {"entry": "nsh_global_batch, all problems in one engine batch (host inputs, CIGARs back on the host)",
+                  "ok": bool(okb), "wall_s": round(best, 4), "gcups": round(bg, 2),
+                  "same_as_submit_path": bool((sc == gr[:, 0]).all() and (nc == gr[:, 1]).all()),
+                  "roofline": {"bound": "valu", "achieved": round(bg * 31 / 1e3, 3), "peak": round(VALU_I32_TOPS, 2),
+                               "unit": "TOP/s", "frac": round(bg * 31 / 1e3 / VALU_I32_TOPS, 4), "ops_per_cell": 31}}
+    out["global_cigar"] = g
+    log("[bench] drop-in global CIGAR: %.1f GCUPS; one batch %.1f GCUPS" % (gcups, bg))
+    return out
+
+
 def bench_hmm(seconds_cpu=4.0, log=None):
     """64 profiles x a 10-Mb genome in 6 frames: Viterbi cells/s on the GPU (kernel time) and the
     reference's mp_hmm_dp_score (or the oracle) on a bounded sample of the same regions"""
@@ -608,6 +732,8 @@ def main():
     ap.add_argument("--only-gapless-prefilter", action="store_true",
                     help="run only the gapless prefilter section and print its JSON object")
     ap.add_argument("--only-index", action="store_true", help="run only the k-mer index section and print its JSON")
+    ap.add_argument("--only-dropin", action="store_true",
+                    help="run only the drop-in section (reference calling patterns from C threads) and print its JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device three times (a copy kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
@@ -649,6 +775,15 @@ def main():
     from protlocus_amd import synth
     t0 = time.time()
     pairs = synth.extension_batch(args.pairs, seed=args.seed)
+    if args.only_dropin:
+        prod = nsffi.Product()
+        if not prod.lib.ns_gpu_init(0):
+            raise SystemExit("GPU not usable: " + prod.error())
+        kw = dict(flag=nsffi.NS_F_EXT_RIGHT)
+        packed = synth.pack_batch(pairs)
+        ref_res = None if args.no_cpu else cpu_baseline(pairs, kw, seconds=args.cpu_seconds, packed=packed)[1]
+        print(json.dumps(bench_dropin(prod, pairs, packed, kw, ref_res, log, cpu=not args.no_cpu)), flush=True)
+        return
     log("[bench] generated %d pairs in %.1f s" % (len(pairs), time.time() - t0))
     prod = nsffi.Product()
     if not prod.lib.ns_gpu_init(0):
@@ -710,6 +845,7 @@ def main():
                      "ops_per_cell": OPS_PER_CELL},
     }
     log("[bench] gpu: %.2f GCUPS, %.1f ms/step, dp %.1f ms" % (gcups, ms, dp_avg))
+    ref_res = None
     if not args.no_cpu:
         cb, ref_res = cpu_baseline(pairs, kw, seconds=args.cpu_seconds, packed=(ns, ns_off, nl, as_, as_off, al))
         out["cpu_baseline"] = cb
@@ -724,6 +860,8 @@ def main():
         out["gpu_window_vs_cpu"] = round(out["window_gcups"] / cb["value"], 1) if cb["value"] else None
         log("[bench] cpu baseline: %.2f GCUPS (%s, %d threads)" % (cb["value"], cb["kind"], cb["cores"]))
     if not args.no_extras:
+        out["drop_in"] = bench_dropin(prod, pairs, (ns, ns_off, nl, as_, as_off, al), kw,
+                                      ref_res if not args.no_cpu else None, log, cpu=not args.no_cpu)
         from protlocus_amd import prescreen
         del pairs
         t0 = time.time()

@@ -251,6 +251,17 @@ float nsh_ext_plan_last_ms(nsh_ext_plan_t *p, int32_t which);  /* 0: whole run, 
 int nsh_ext_plan_autotune(nsh_ext_plan_t *p, int32_t reps, int32_t *waves_per_cu);
 void nsh_ext_plan_destroy(nsh_ext_plan_t *p);
 
+/* Batched global spliced DP with CIGAR (no reference counterpart; the batch form of
+ * ns_global_gs16b / ns_global_gs32b / ns_global_gs_auto with NS_F_CIGAR): n problems with one option
+ * set (opt->flag must hold NS_F_CIGAR and no extension flag; opt->pssm applies to every problem, so it
+ * is normally NULL), ns/as concatenated with offsets, ss optional (ns offsets). mode 0: int16
+ * (gs16b), 1: int32 (gs32b), 2: per problem as ns_global_gs_auto. All problems go to the engine as one
+ * batch. Results in input order: score[k], n_cigar[k]; *cigar receives one malloc'd array holding every
+ * CIGAR (len << 4 | op) one after another (free() it; NULL when no problem has one). 1 ok, 0 error. */
+int nsh_global_batch(int32_t n, const char *ns, const int64_t *ns_off, const int32_t *nl, const char *as,
+		const int64_t *as_off, const int32_t *al, const uint8_t *ss, const ns_opt_t *opt, int32_t mode,
+		int32_t *score, int32_t *n_cigar, uint32_t **cigar);
+
 /* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc): nsh_ext_plan_create + run +
  * results in one call that calls no hipMalloc/hipFree. Arguments as nsh_ext_plan_create (ss and a

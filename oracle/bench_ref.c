@@ -72,3 +72,47 @@ int32_t or_ref_ext_batch(int32_t n, const char *ns, const int64_t *ns_off, const
 	free(tid);
 	return j.done;
 }
+
+typedef struct {
+	int32_t n;
+	const char *ns; const int64_t *ns_off; const int32_t *nl;
+	const char *as; const int64_t *as_off; const int32_t *al;
+	const ns_opt_t *opt;
+	int32_t *out;                /* 2 ints per request: score, CIGAR length */
+	volatile int32_t next;
+} glob_job_t;
+
+static void *glob_worker(void *arg)
+{
+	glob_job_t *j = (glob_job_t*)arg;
+	for (;;) {
+		int32_t k = __sync_fetch_and_add(&j->next, 1);
+		ns_rst_t r;
+		if (k >= j->n) break;
+		ns_rst_init(&r);
+		ns_global_gs16b(0, j->ns + j->ns_off[k], j->nl[k], j->as + j->as_off[k], j->al[k], j->opt, 0, &r);
+		j->out[2 * k] = r.score, j->out[2 * k + 1] = r.n_cigar;
+		free(r.cigar);
+	}
+	return 0;
+}
+
+/* The reference's global spliced DP with CIGAR (ns_global_gs16b with NS_F_CIGAR in opt->flag) over
+ * a batch from n_threads threads: out[2k] = score, out[2k+1] = CIGAR length; *wall = seconds. */
+void or_ref_global_batch(int32_t n, const char *ns, const int64_t *ns_off, const int32_t *nl, const char *as,
+		const int64_t *as_off, const int32_t *al, const ns_opt_t *opt, int32_t n_threads, int32_t *out, double *wall)
+{
+	glob_job_t j;
+	pthread_t *tid;
+	int32_t t;
+	double t0;
+	if (n_threads < 1) n_threads = 1;
+	j.n = n, j.ns = ns, j.ns_off = ns_off, j.nl = nl, j.as = as, j.as_off = as_off, j.al = al, j.opt = opt;
+	j.out = out, j.next = 0;
+	tid = (pthread_t*)malloc(sizeof(pthread_t) * n_threads);
+	t0 = now_s();
+	for (t = 0; t < n_threads; ++t) pthread_create(&tid[t], 0, glob_worker, &j);
+	for (t = 0; t < n_threads; ++t) pthread_join(tid[t], 0);
+	*wall = now_s() - t0;
+	free(tid);
+}

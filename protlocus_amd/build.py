@@ -54,6 +54,14 @@ def build(verbose=False, jobs=8):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, OUT)
+    # C drivers of the reference's calling patterns over this library (the benchmark's drop-in path)
+    drv = os.path.join(HERE, "libnsh_drivers.so")
+    cmd = ["gcc", "-std=c99", "-O2", "-fPIC", "-shared", "-o", drv + ".tmp", os.path.join(HERE, "drivers", "caller_patterns.c"),
+           "-L" + HERE, "-l:libnasw_hip.so", "-Wl,-rpath,$ORIGIN", "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed:\n" + r.stdout + r.stderr)
+    os.replace(drv + ".tmp", drv)
     return OUT
 
 

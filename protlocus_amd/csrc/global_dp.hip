@@ -45,12 +45,23 @@ struct BtProb {
 	int32_t nl, al;
 };
 
-__device__ __forceinline__ void bt_push(uint32_t *t, int32_t &n, uint32_t op, int32_t len)
-{
-	// ns_push_cigar (nasw.h:219-229): F and G are never merged
-	if (n == 0 || op != (t[n - 1] & 0xf) || op == NS_CIGAR_F || op == NS_CIGAR_G) t[n++] = (uint32_t)len << 4 | op;
-	else t[n - 1] += (uint32_t)len << 4;
-}
+// the ops in push order go to scratch; the op being extended stays in registers (cur, one
+// uint32 len << 4 | op) so that a step needs no read of the scratch
+struct BtRun {
+	uint32_t *t;
+	int32_t n;       // ops written to t
+	uint32_t cur;    // the open op (0xffffffff: none)
+	__device__ void push(uint32_t op, int32_t len)
+	{
+		// ns_push_cigar (nasw.h:219-229): F and G are never merged
+		if (cur != 0xffffffffu && op == (cur & 0xf) && op != NS_CIGAR_F && op != NS_CIGAR_G) cur += (uint32_t)len << 4;
+		else {
+			if (cur != 0xffffffffu) t[n++] = cur;
+			cur = (uint32_t)len << 4 | op;
+		}
+	}
+	__device__ void close() { if (cur != 0xffffffffu) t[n++] = cur, cur = 0xffffffffu; }
+};
 
 __global__ void __launch_bounds__(64) backtrack_kernel(const BtProb *bp, int32_t n, const uint16_t *trace,
 		uint32_t *tmp, uint32_t *packed, unsigned long long *total, int64_t *meta)
@@ -59,8 +70,8 @@ __global__ void __launch_bounds__(64) backtrack_kernel(const BtProb *bp, int32_t
 	if (k >= n) return;
 	const BtProb P = bp[k];
 	const uint16_t *tr = trace + P.tr_off;
-	uint32_t *t = tmp + P.tmp_off;
-	int32_t i = P.nl - 1, j = P.al - 1, last = 0, nc = 0;
+	BtRun b = { tmp + P.tmp_off, 0, 0xffffffffu };
+	int32_t i = P.nl - 1, j = P.al - 1, last = 0;
 	const int64_t al = P.al;
 	while (i >= 2 && j >= 0) {   // nasw-sse.c:44-93
 		int32_t x = tr[(int64_t)i * al + j], state, ext;
@@ -68,26 +79,29 @@ __global__ void __launch_bounds__(64) backtrack_kernel(const BtProb *bp, int32_t
 		state = last == 0? x & 0xf : last;
 		ext = state >= 1 && state <= 5? x >> (state + 3) & 1 : 0;
 		switch (state) {
-		case 0: bt_push(t, nc, NS_CIGAR_M, 1), i -= 3, --j; break;
-		case 1: bt_push(t, nc, NS_CIGAR_I, 1), --j; break;
-		case 2: bt_push(t, nc, NS_CIGAR_D, 1), i -= 3; break;
-		case 3: bt_push(t, nc, NS_CIGAR_N, 1), --i; break;
-		case 4: bt_push(t, nc, NS_CIGAR_U, 1), --i; if (!ext) --j; break;
-		case 5: bt_push(t, nc, NS_CIGAR_V, 1), --i; if (!ext) --j; break;
-		case 6: bt_push(t, nc, NS_CIGAR_F, 1), --i; break;
-		case 7: bt_push(t, nc, NS_CIGAR_F, 2), i -= 2; break;
-		case 8: bt_push(t, nc, NS_CIGAR_G, 1), --i, --j; break;
-		case 9: bt_push(t, nc, NS_CIGAR_G, 2), i -= 2, --j; break;
+		case 0: b.push(NS_CIGAR_M, 1), i -= 3, --j; break;
+		case 1: b.push(NS_CIGAR_I, 1), --j; break;
+		case 2: b.push(NS_CIGAR_D, 1), i -= 3; break;
+		case 3: b.push(NS_CIGAR_N, 1), --i; break;
+		case 4: b.push(NS_CIGAR_U, 1), --i; if (!ext) --j; break;
+		case 5: b.push(NS_CIGAR_V, 1), --i; if (!ext) --j; break;
+		case 6: b.push(NS_CIGAR_F, 1), --i; break;
+		case 7: b.push(NS_CIGAR_F, 2), i -= 2; break;
+		case 8: b.push(NS_CIGAR_G, 1), --i, --j; break;
+		case 9: b.push(NS_CIGAR_G, 2), i -= 2, --j; break;
 		default: i = -1; break;   // not produced by the kernels
 		}
 		last = state >= 1 && state <= 5 && ext? state : 0;
 	}
-	if (j > 0) bt_push(t, nc, NS_CIGAR_I, j);
+	if (j > 0) b.push(NS_CIGAR_I, j);
 	if (i >= 0) {
 		const int32_t l = (i + 1) / 3, r = (i + 1) % 3;
-		if (l > 0) bt_push(t, nc, NS_CIGAR_D, l);
-		if (r != 0) bt_push(t, nc, NS_CIGAR_F, r);
+		if (l > 0) b.push(NS_CIGAR_D, l);
+		if (r != 0) b.push(NS_CIGAR_F, r);
 	}
+	b.close();
+	const int32_t nc = b.n;
+	uint32_t *t = b.t;
 	const unsigned long long off = atomicAdd(total, (unsigned long long)nc);
 	for (int32_t q = 0; q < nc; ++q) {   // reversed, U/V shorter than 3 become G (nasw-sse.c:34-42)
 		uint32_t c = t[nc - 1 - q];
@@ -196,6 +210,15 @@ public:
 		std::lock_guard<std::mutex> lk(mu);
 		start();
 		queue.push_back(q);
+		cv_work.notify_one();
+	}
+
+	void submit_many(const std::vector<nsh_glob_req*> &qs)   // queued together: one batch
+	{
+		std::lock_guard<std::mutex> lk(mu);
+		start();
+		for (nsh_glob_req *q : qs) queue.push_back(q);
+		++n_waiting;              // the caller waits next: no batching window
 		cv_work.notify_one();
 	}
 
@@ -462,7 +485,8 @@ fail:
 // ---------------------------------------------------------------------------------------
 // internal API (nsh_internal.h)
 // ---------------------------------------------------------------------------------------
-nsh_glob_req *nsh_glob_submit(const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt,
+// a request with its inputs copied (nullptr and the reason in ns_gpu_error() when it is refused)
+static nsh_glob_req *glob_make(const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt,
 		const uint8_t *ss, int32_t is32)
 {
 	if (!nsh_ensure_device()) return nullptr;
@@ -488,7 +512,14 @@ nsh_glob_req *nsh_glob_submit(const char *ns, int32_t nl, const char *as, int32_
 	std::memcpy(q->mat, opt->sc, 484), std::memcpy(q->nt4, opt->nt4, 256), std::memcpy(q->aa20, opt->aa20, 256);
 	std::memcpy(q->codon, opt->codon, 64);
 	q->opt.sc = q->mat, q->opt.nt4 = q->nt4, q->opt.aa20 = q->aa20, q->opt.codon = q->codon, q->opt.pssm = nullptr;
-	engine().submit(q);
+	return q;
+}
+
+nsh_glob_req *nsh_glob_submit(const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt,
+		const uint8_t *ss, int32_t is32)
+{
+	nsh_glob_req *q = glob_make(ns, nl, as, al, opt, ss, is32);
+	if (q) engine().submit(q);
 	return q;
 }
 
@@ -513,3 +544,53 @@ int nsh_global_run(void *km, const char *ns, int32_t nl, const char *as, int32_t
 	r->n_cigar = 0;
 	return nsh_glob_wait(q, km, r);
 }
+
+// n problems of one option set in one engine batch (nasw-hip.h)
+extern "C" int nsh_global_batch(int32_t n, const char *ns, const int64_t *ns_off, const int32_t *nl, const char *as,
+		const int64_t *as_off, const int32_t *al, const uint8_t *ss, const ns_opt_t *opt, int32_t mode,
+		int32_t *score, int32_t *n_cigar, uint32_t **cigar)
+{
+	if (n < 0 || (n > 0 && (!ns || !ns_off || !nl || !as || !as_off || !al)) || !opt || !score || !n_cigar || !cigar) {
+		nsh_set_error("nsh_global_batch: bad arguments");
+		return 0;
+	}
+	*cigar = nullptr;
+	if (n == 0) return 1;
+	int32_t max_sub = 0;
+	for (int32_t t = 0; t < opt->asize * opt->asize; ++t) if (max_sub < opt->sc[t]) max_sub = opt->sc[t];
+	std::vector<nsh_glob_req*> qs(n, nullptr);
+	int ok = 1;
+	for (int32_t k = 0; k < n && ok; ++k) {
+		// mode 2: ns_global_gs_auto's choice (nasw-sse.c:980-999), per problem
+		const int32_t is32 = mode == 1 || (mode == 2 && (int64_t)al[k] * max_sub + opt->end_bonus > 30000);
+		qs[k] = glob_make(ns + ns_off[k], nl[k], as + as_off[k], al[k], opt, ss? ss + ns_off[k] : nullptr, is32);
+		if (!qs[k]) ok = 0;
+	}
+	if (!ok) { for (nsh_glob_req *q : qs) delete q; return 0; }
+	engine().submit_many(qs);
+	int64_t tot = 0;
+	for (int32_t k = 0; k < n; ++k) {
+		engine().wait(qs[k]);
+		ok &= qs[k]->ok;
+		tot += (int64_t)qs[k]->cigar.size();
+	}
+	{
+		std::lock_guard<std::mutex> lk(engine().mu);   // submit_many counted the caller as waiting
+		--engine().n_waiting;
+	}
+	uint32_t *out = ok? (uint32_t*)std::malloc(std::max<int64_t>(tot, 1) * sizeof(uint32_t)) : nullptr;
+	if (ok && !out) { nsh_set_error("nsh_global_batch: out of host memory"); ok = 0; }
+	int64_t o = 0;
+	for (int32_t k = 0; k < n; ++k) {
+		nsh_glob_req *q = qs[k];
+		if (ok) {
+			score[k] = q->score, n_cigar[k] = (int32_t)q->cigar.size();
+			if (!q->cigar.empty()) std::memcpy(out + o, q->cigar.data(), q->cigar.size() * sizeof(uint32_t));
+			o += (int64_t)q->cigar.size();
+		}
+		delete q;
+	}
+	*cigar = out;
+	return ok;
+}
+

@@ -174,3 +174,39 @@ def test_trace_budget_cuts_a_batch(oracle, product, monkeypatch):
     assert L.ns_gpu_launch_count() - launches0 >= 8    # at least two parts of prep, DP and backtrack
     for (ns, a), g in zip(cases, got):
         assert g == oracle.global_dp(ns, a, oracle.opt(flag=NS_F_CIGAR), mode=16)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_global_batch_entry(oracle, product, mode):
+    """nsh_global_batch: one engine batch of 300 problems (int16, int32, or per problem as
+    ns_global_gs_auto); scores and CIGARs in input order, bit-exact"""
+    from protlocus_amd import synth
+    L = product.lib
+    L.nsh_global_batch.argtypes = [C.c_int32, C.c_char_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p,
+                                   C.c_char_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                   C.POINTER(C.POINTER(C.c_uint32))]
+    rng = np.random.default_rng(300 + mode)
+    pairs = [planted_case(rng, int(rng.integers(1, 900)) if k % 50 else 2800) for k in range(300)]
+    ns, ns_off, nl, as_, as_off, al = synth.pack_batch([(d, p.encode()) for d, p in pairs])
+    ns_off, as_off = (np.ascontiguousarray(x, dtype=np.int64) for x in (ns_off, as_off))
+    nl, al = (np.ascontiguousarray(x, dtype=np.int32) for x in (nl, al))
+    score = np.zeros(len(pairs), dtype=np.int32)
+    ncig = np.zeros(len(pairs), dtype=np.int32)
+    cig = C.POINTER(C.c_uint32)()
+    o = product.opt(flag=NS_F_CIGAR)
+    assert L.nsh_global_batch(len(pairs), ns, ns_off.ctypes.data, nl.ctypes.data, as_, as_off.ctypes.data,
+                              al.ctypes.data, None, C.byref(o), mode, score.ctypes.data, ncig.ctypes.data,
+                              C.byref(cig)), product.error()
+    off = 0
+    ops = "MIDNSHP=XBFGUVE"
+    got = []
+    for k in range(len(pairs)):
+        s = "".join("%d%s" % (cig[off + t] >> 4, ops[cig[off + t] & 0xf]) for t in range(ncig[k]))
+        got.append((int(score[k]), s))
+        off += int(ncig[k])
+    product.free(C.cast(cig, C.c_void_p))
+    m = {0: 16, 1: 32, 2: "auto"}[mode]
+    with cf.ThreadPoolExecutor(8) as ex:
+        exp = list(ex.map(lambda pr: oracle.global_dp(pr[0], pr[1], oracle.opt(flag=NS_F_CIGAR), mode=m), pairs))
+    bad = [k for k in range(len(pairs)) if got[k] != (exp[k][2], exp[k][3])]
+    assert not bad, [(k, len(pairs[k][1]), got[k][0], exp[k][2]) for k in bad[:4]]
