@@ -251,6 +251,49 @@ float nsh_ext_plan_last_ms(nsh_ext_plan_t *p, int32_t which);  /* 0: whole run, 
 int nsh_ext_plan_autotune(nsh_ext_plan_t *p, int32_t reps, int32_t *waves_per_cu);
 void nsh_ext_plan_destroy(nsh_ext_plan_t *p);
 
+/* The align stage of the map path for many candidate regions at once (no single reference
+ * counterpart: the reference runs it per query through mp_align_batch, align.c:745-777, with
+ * mp_mapopt_t; this is its default-option path). Each candidate is a refined region as mp_map_finish
+ * hands it over: its anchors (refine layout: target end of the k-mer relative to vs << 32 | query end)
+ * and mp_cal_max_ext's limits. Every stage of mp_align_step (trusted-anchor filter, left extension and
+ * io_end retry, gap DPs, right extension and retry, DP over the right extension) runs for all
+ * candidates together on the device engines. */
+typedef struct {
+	int32_t go, ge, io, io_end, fs, xdrop, end_bonus;
+	int32_t max_ext, max_intron, kmer2, end_retry_win;   /* mp_mapopt_t max_ext, max_intron, kmer2, acc_end_retry_win */
+	float ie_coef, sp_scale;
+	int32_t sp_model, sp_null_bonus;
+	int8_t mat[484];                                    /* mp_mapopt_t.mat: BLOSUM62 with the stop scores set */
+} nsh_align_opt_t;
+typedef struct {
+	int32_t query;          /* index into the query arrays */
+	uint32_t vid;           /* contig << 1 | strand */
+	int32_t qs, qe;         /* in: the refined chain's span; out: the alignment's */
+	int64_t vs, ve;         /* same, in the strand's coordinates */
+	int32_t cnt;            /* anchors anchors[a_off .. a_off + cnt) ; set to 0 when the candidate is dropped */
+	int32_t chn_sc, chn_sc_ungap;
+	int64_t a_off;
+	uint64_t ext;           /* mp_cal_max_ext: extl << 32 | extr, 0 for none */
+	/* out */
+	int32_t status;         /* 1 aligned, 0 dropped (no trusted anchor) */
+	int32_t dp_score, n_cigar;
+	int64_t cigar_off;      /* into the CIGAR array */
+	int64_t win_as, win_ae; /* the search window the statistics read */
+} nsh_align_reg_t;
+void nsh_align_opt_default(nsh_align_opt_t *o);    /* mp_mapopt_init's values (options.c:42-138) */
+/* nt4: the genome as base codes 0..4 (contig c at ctg_off[c], ctg_len[c] bases); aa/qlen: the
+ * queries (ASCII). *cigar receives one malloc'd array with every aligned candidate's CIGAR. 1 ok. */
+int nsh_align_stage(int32_t n_ctg, const uint8_t *nt4, const int64_t *ctg_off, const int64_t *ctg_len,
+		const char *const *aa, const int32_t *qlen, int32_t n_reg, nsh_align_reg_t *reg, const uint64_t *anchors,
+		const nsh_align_opt_t *opt, uint32_t **cigar);
+/* The records of one aligned candidate (mp_extra_* and mp_write_output as mp_align_finish and the map
+ * output write them; flag as nsh_write_output, id and hit_idx as the caller numbers its hits).
+ * Returns the length (writes at most size - 1 bytes and a NUL), 0 for a dropped candidate, -1 on error. */
+int64_t nsh_align_records(int32_t n_ctg, const uint8_t *nt4, const int64_t *ctg_off, const int64_t *ctg_len,
+		const char *const *ctg_name, const char *const *qname, const char *const *aa, const int32_t *qlen,
+		const nsh_align_reg_t *r, const uint32_t *cigar, const nsh_align_opt_t *opt, uint32_t flag, int64_t id,
+		int32_t hit_idx, char *out, int64_t size);
+
 /* Batched global spliced DP with CIGAR (no reference counterpart; the batch form of
  * ns_global_gs16b / ns_global_gs32b / ns_global_gs_auto with NS_F_CIGAR): n problems with one option
  * set (opt->flag must hold NS_F_CIGAR and no extension flag; opt->pssm applies to every problem, so it

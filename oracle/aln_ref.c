@@ -59,3 +59,65 @@ int64_t or_ref_output(const char *ctg_name, const uint8_t *ctg_nt4, int64_t ctg_
 	free(s.s), free(r.p), free(r.feat), free(reg), free(nt.seq);
 	return l;
 }
+
+/* The reference's align stage (mp_align_batch, align.c:745-777, default options) on the regions of
+ * one query as or_ref_align_inputs returns them, against an index from or_ref_index_build; then
+ * each aligned region's record through mp_write_output (format.c:457-477; flag selects PAF or
+ * GFF3). Per region: st[k] = 1 when aligned, 0 when dropped; fields[8k..] = vid, qs, qe,
+ * vs, ve, dp_score, dp_max, n_cigar; the CIGARs go to cig (up to ccap ops in all), the records,
+ * one after another, to text (up to size bytes). Returns the bytes of text written, -1 on overflow. */
+typedef struct {
+	int32_t vid, qs, qe, cnt, chn_sc, chn_sc_ungap, id, parent;
+	int64_t vs, ve;
+	uint64_t ext;
+} or_areg2_t;
+const mp_idx_t *or_ref_index_mi(void *handle);
+
+int64_t or_ref_align_stage(void *idx_handle, const char *qname, const char *aa, int32_t qlen, int32_t n,
+		const or_areg2_t *in, const uint64_t *anch, uint32_t flag, int32_t *st, int64_t *fields, int64_t ccap,
+		uint32_t *cig, char *text, int64_t size)
+{
+	const mp_idx_t *mi = or_ref_index_mi(idx_handle);
+	mp_mapopt_t opt;
+	mp_reg1_t *reg, **rp;
+	uint64_t *ext;
+	mp_bseq1_t seq;
+	kstring_t s = {0, 0, 0};
+	int64_t k = 0, nc = 0, tl = 0, ret;
+	int32_t i;
+	mp_mapopt_init(&opt);
+	opt.flag |= flag;
+	reg = (mp_reg1_t*)calloc(n + 1, sizeof(mp_reg1_t));
+	rp = (mp_reg1_t**)calloc(n + 1, sizeof(void*));
+	ext = (uint64_t*)calloc(n + 1, 8);
+	for (i = 0; i < n; ++i) {
+		mp_reg1_t *r = &reg[i];
+		r->vid = in[i].vid, r->qs = in[i].qs, r->qe = in[i].qe, r->cnt = in[i].cnt, r->chn_sc = in[i].chn_sc;
+		r->chn_sc_ungap = in[i].chn_sc_ungap, r->id = in[i].id, r->parent = in[i].parent;
+		r->vs = in[i].vs, r->ve = in[i].ve, r->off = 0;
+		r->a = (uint64_t*)malloc((size_t)(r->cnt + 1) * 8);
+		memcpy(r->a, &anch[k], (size_t)r->cnt * 8);
+		k += r->cnt;
+		ext[i] = in[i].ext, rp[i] = r;
+	}
+	mp_align_batch(0, &opt, mi, qlen, aa, n, rp, ext, 0, 0, st, 0);
+	memset(&seq, 0, sizeof(seq));
+	seq.l_seq = qlen, seq.name = (char*)qname, seq.seq = (char*)aa;
+	for (i = 0; i < n; ++i) {
+		const mp_reg1_t *r = &reg[i];
+		int64_t *f = &fields[8 * i];
+		st[i] = r->p != 0;   /* aligned, or dropped (mp_align_start found no trusted anchor) */
+		f[0] = r->vid, f[1] = r->qs, f[2] = r->qe, f[3] = r->vs, f[4] = r->ve;
+		f[5] = r->p? r->p->dp_score : 0, f[6] = r->p? r->p->dp_max : 0, f[7] = r->p? r->p->n_cigar : 0;
+		if (!r->p) continue;
+		if (nc + r->p->n_cigar <= ccap) memcpy(&cig[nc], r->p->cigar, (size_t)r->p->n_cigar * 4);
+		nc += r->p->n_cigar;
+		mp_write_output(&s, 0, mi, &seq, r, &opt, i, 0);   /* resets s: copy each record out */
+		if (tl + (int64_t)s.l <= size) memcpy(text + tl, s.s, s.l);
+		tl += (int64_t)s.l;
+	}
+	ret = nc > ccap || tl > size? -1 : tl;
+	for (i = 0; i < n; ++i) { free(reg[i].p); free(reg[i].feat); free(reg[i].a); }
+	free(reg), free(rp), free(ext), free(s.s);
+	return ret;
+}

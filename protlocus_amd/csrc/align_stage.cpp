@@ -1,0 +1,366 @@
+// The align stage of the map path for many candidate regions at once, on the device engines
+// (nsh_align_stage; the reference runs it per query through mp_align_batch, align.c:745-777).
+//
+// Each candidate goes through the stages of mp_align_step (align.c:648-743) with the reference's
+// default options: trusted-anchor filter, left extension (and its io_end retry for a 5'-end exon),
+// the gap DPs between trusted anchors, right extension (and its retry), the DP over the right
+// extension, then the alignment's statistics and records. Here every stage runs for all candidates
+// together: one extension plan per extension stage, one global-DP engine batch per DP stage, so a
+// stage costs one round trip to the GPU however many proteins are in flight. The host only does
+// what the reference does between the DPs: interval bookkeeping, the ungapped scores of anchor-to-
+// anchor steps and CIGAR concatenation.
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include "nsh_internal.h"
+
+namespace {
+
+constexpr uint64_t kTrusted = 1ULL << 31;   // mark of a trusted anchor in the query field
+
+int32_t anchor_q(uint64_t a) { return (int32_t)((uint32_t)a << 1) >> 1; }   // query field without the mark
+int64_t anchor_x(uint64_t a) { return (int64_t)(a >> 32); }
+
+// mp_filter_seed (align.c:8-34): anchors in runs of at least min_cnt whose steps are whole codons,
+// at most max_aa_dist residues long, are trusted, except those within trim_back residues of the
+// run's ends
+void trust_runs(int32_t cnt, uint64_t *a, int32_t max_aa_dist, int32_t min_cnt, int32_t kmer2, int32_t trim_back)
+{
+	for (int32_t i = 0; i < cnt; ++i) {
+		int32_t j = i + 1;
+		for (; j < cnt; ++j) {
+			const int32_t dx = (int32_t)(a[j] >> 32) - (int32_t)(a[j - 1] >> 32), dy = (int32_t)a[j] - (int32_t)a[j - 1];
+			if (dx % 3 != 0 || dx > max_aa_dist * 3 || dy > max_aa_dist) break;
+		}
+		if (j - i < min_cnt) continue;
+		int32_t k = j - 2;
+		const int32_t last = (int32_t)a[j - 1];
+		while (k >= i && last - (int32_t)a[k] < trim_back) --k;
+		const int32_t first = (int32_t)a[i] + 1 - kmer2;
+		while (i < k && (int32_t)a[i] + 1 - first < trim_back) ++i;
+		for (; i <= k; ++i) a[i] |= kTrusted;
+		i = j - 1;
+	}
+}
+
+// the op appended as ns_push_cigar does (nasw.h:219-229): F and G are never merged
+void push_op(std::vector<uint32_t> &c, uint32_t op, int32_t len)
+{
+	if (c.empty() || op != (c.back() & 0xf) || op == NS_CIGAR_F || op == NS_CIGAR_G) c.push_back((uint32_t)len << 4 | op);
+	else c.back() += (uint32_t)len << 4;
+}
+
+// oriented bases of [st, en) of strand vid (mp_ntseq_get_by_v, ntseq.c:92-118); -1 when st is out of range
+int64_t strand_bases(const uint8_t *nt4, const int64_t *ctg_off, const int64_t *ctg_len, uint32_t vid, int64_t st,
+		int64_t en, uint8_t *out)
+{
+	const int64_t L = ctg_len[vid >> 1], base = ctg_off[vid >> 1];
+	if (st < 0 || en < 0 || st >= L) return -1;
+	if (en > L) en = L;
+	int64_t k = 0;
+	if (!(vid & 1)) {
+		for (int64_t i = st; i < en; ++i) out[k++] = nt4[base + i];
+	} else {   // reverse strand: complement of the forward bases, right to left
+		for (int64_t i = L - st - 1; i >= L - en; --i) {
+			const uint8_t c = nt4[base + i];
+			out[k++] = c >= 4? c : (uint8_t)(3 - c);
+		}
+	}
+	return k;
+}
+
+struct Cand {
+	nsh_align_reg_t *r = nullptr;
+	const char *aa = nullptr;
+	int32_t len = 0;
+	std::vector<uint64_t> a;
+	std::vector<uint8_t> nt;     // oriented bases of [as, ae)
+	int64_t as = 0, ae = 0, vs0 = 0, vs1 = 0;
+	int32_t i0 = 0, ne0 = 0, ae0 = 0, score = 0;
+	int32_t as1 = 0, as1_win = 0, aa_win = 0, nt_len = 0, aa_len = 0;
+	bool alive = false, right = false;
+	std::vector<uint32_t> cigar;
+};
+
+// one DP of mp_align_seq (align.c:103-145): an anchor-to-anchor step of exactly alen codons and at
+// most kmer2 residues is scored without a DP; anything else goes to the global DP with CIGAR
+struct Task { Cand *c; int64_t nt_off; int32_t nlen, aa_off, alen; int32_t score; std::vector<uint32_t> cig; bool dp; };
+
+}   // namespace
+
+extern "C" void nsh_align_opt_default(nsh_align_opt_t *o)
+{
+	// mp_mapopt_init (options.c:42-138) as main() leaves it without -I / -G
+	std::memset(o, 0, sizeof(*o));
+	o->go = 11, o->ge = 1, o->io = 29, o->io_end = 19, o->fs = 23, o->xdrop = 100, o->end_bonus = 5;
+	o->max_ext = 10000, o->max_intron = 200000, o->kmer2 = 5, o->end_retry_win = 10000;
+	o->ie_coef = .5f, o->sp_scale = 1.0f, o->sp_model = NS_S_GENERIC, o->sp_null_bonus = -7;
+	std::memcpy(o->mat, ns_mat_blosum62, 484);
+	ns_set_stop_sc(22, o->mat, (int8_t)o->fs);
+}
+
+static int ext_round(std::vector<Cand*> &cs, std::vector<nsh_ext_req> &req, const ns_opt_t &o, std::vector<nsh_ext_result_t> &res)
+{
+	res.assign(req.size(), nsh_ext_result_t{});
+	if (req.empty()) return 1;
+	nsh_ext_plan_t *p = nsh_ext_plan_build((int32_t)req.size(), req.data(), &o);
+	const int ok = p && nsh_ext_plan_run(p) && nsh_ext_plan_results(p, res.data());
+	if (p) nsh_ext_plan_destroy(p);
+	(void)cs;
+	return ok;
+}
+
+static nsh_ext_req ext_req(const Cand *c, int64_t nt_off, int32_t nl, int32_t aa_off, int32_t al, int32_t left, int32_t io)
+{
+	nsh_ext_req q;
+	q.ns = (const char*)c->nt.data() + nt_off, q.nl = nl, q.as = c->aa + aa_off, q.al = al, q.ss = nullptr;
+	q.left = left, q.io = io, q.mode = 0, q.pssm = nullptr;
+	const int32_t w = nsh_host_stripe_width(left? NS_F_EXT_LEFT : NS_F_EXT_RIGHT, al);
+	q.npad = (al + w - 1) / w * w;
+	return q;
+}
+
+// the DPs of a stage: ungapped steps on the host, the others as one engine batch
+static int dp_round(std::vector<Task> &tasks, const ns_opt_t &o, const nsh_align_opt_t *ao)
+{
+	std::vector<int32_t> idx;
+	for (size_t t = 0; t < tasks.size(); ++t) {
+		Task &k = tasks[t];
+		const Cand *c = k.c;
+		k.dp = !(k.nlen == k.alen * 3 && k.alen <= ao->kmer2);
+		if (k.dp) { idx.push_back((int32_t)t); continue; }
+		// mp_score_ungapped (align.c:36-46): codons with an N score as X. Its loop bound is the
+		// residue count while its index steps over bases (i < alen, i += 3), so only the first
+		// ceil(alen / 3) residues of the step are scored; the same bound is kept here.
+		int32_t s = 0;
+		const uint8_t *nseq = c->nt.data() + k.nt_off;
+		for (int32_t i = 0, j = 0; i < k.alen; i += 3, ++j) {
+			const uint8_t b0 = nseq[i], b1 = nseq[i + 1], b2 = nseq[i + 2];
+			const int32_t ta = b0 > 3 || b1 > 3 || b2 > 3? ns_tab_aa20['X'] : ns_tab_codon[b0 << 4 | b1 << 2 | b2];
+			s += ao->mat[ta * 22 + ns_tab_aa20[(uint8_t)c->aa[k.aa_off + j]]];
+		}
+		k.score = s;
+		k.cig.assign(1, (uint32_t)k.alen << 4 | NS_CIGAR_M);
+	}
+	if (idx.empty()) return 1;
+	const int32_t n = (int32_t)idx.size();
+	std::vector<int64_t> ns_off(n), as_off(n);
+	std::vector<int32_t> nl(n), al(n), score(n), ncig(n);
+	std::vector<char> ns, as;
+	for (int32_t k = 0; k < n; ++k) {
+		const Task &t = tasks[idx[k]];
+		ns_off[k] = (int64_t)ns.size(), as_off[k] = (int64_t)as.size();
+		ns.insert(ns.end(), (const char*)t.c->nt.data() + t.nt_off, (const char*)t.c->nt.data() + t.nt_off + t.nlen);
+		as.insert(as.end(), t.c->aa + t.aa_off, t.c->aa + t.aa_off + t.alen);
+		nl[k] = t.nlen, al[k] = t.alen;
+	}
+	ns.push_back(0), as.push_back(0);
+	uint32_t *cig = nullptr;
+	ns_opt_t oc = o;
+	oc.flag = NS_F_CIGAR;
+	// mode 2: ns_global_gs_auto's int16 / int32 choice, as mp_align_seq calls it
+	if (!nsh_global_batch(n, ns.data(), ns_off.data(), nl.data(), as.data(), as_off.data(), al.data(), nullptr, &oc, 2,
+			score.data(), ncig.data(), &cig))
+		return 0;
+	int64_t off = 0;
+	for (int32_t k = 0; k < n; ++k) {
+		Task &t = tasks[idx[k]];
+		t.score = score[k];
+		t.cig.assign(cig + off, cig + off + ncig[k]);
+		off += ncig[k];
+	}
+	std::free(cig);
+	return 1;
+}
+
+extern "C" int nsh_align_stage(int32_t n_ctg, const uint8_t *nt4, const int64_t *ctg_off, const int64_t *ctg_len,
+		const char *const *aa, const int32_t *qlen, int32_t n_reg, nsh_align_reg_t *reg, const uint64_t *anchors,
+		const nsh_align_opt_t *ao, uint32_t **cigar_out)
+{
+	if (!nt4 || !ctg_off || !ctg_len || !aa || !qlen || (n_reg > 0 && (!reg || !anchors)) || !ao || !cigar_out || n_reg < 0) {
+		nsh_set_error("nsh_align_stage: bad arguments");
+		return 0;
+	}
+	*cigar_out = nullptr;
+	if (!nsh_ensure_device()) return 0;
+	// ns_opt_t of the DPs (mp_map2ns_opt, align.c:59-71)
+	ns_opt_t o;
+	ns_opt_init(&o);
+	o.go = ao->go, o.ge = ao->ge, o.io = ao->io, o.fs = ao->fs, o.xdrop = ao->xdrop, o.sc = ao->mat;
+	o.ie_coef = ao->ie_coef, o.end_bonus = ao->end_bonus, o.sp_null_bonus = ao->sp_null_bonus;
+	ns_opt_set_sp(&o, ao->sp_model);
+	for (int t = 0; t < 6; ++t) o.sp[t] = (int32_t)(o.sp[t] * ao->sp_scale + .499f);
+	std::vector<Cand> cand(n_reg);
+	std::vector<Cand*> live;
+	// MP_AS_START: trusted anchors, the search window, its bases (mp_align_start, align.c:431-480)
+	for (int32_t k = 0; k < n_reg; ++k) {
+		nsh_align_reg_t *r = &reg[k];
+		Cand &c = cand[k];
+		c.r = r, r->status = 0, r->dp_score = 0, r->n_cigar = 0, r->cigar_off = 0;
+		if ((r->vid >> 1) >= (uint32_t)n_ctg || r->query < 0 || r->cnt <= 0) continue;
+		c.aa = aa[r->query], c.len = qlen[r->query];
+		c.a.assign(anchors + r->a_off, anchors + r->a_off + r->cnt);
+		trust_runs(r->cnt, c.a.data(), 6, 3, ao->kmer2, ao->kmer2 + 1);
+		int32_t i = 0;
+		while (i < r->cnt && !(c.a[i] & kTrusted)) ++i;
+		if (i == r->cnt) { r->cnt = 0; continue; }   // no trusted anchor: the candidate is dropped
+		c.i0 = i;
+		int32_t extl = ao->max_ext, extr = ao->max_ext;
+		if (r->qs >= 10) extl = ao->max_intron / 2;
+		if (c.len - r->qe >= 10) extr = ao->max_intron / 2;
+		const int32_t extl0 = (int32_t)(r->ext >> 32), extr0 = (int32_t)r->ext;
+		if (extl0 > 0) extl = std::min(extl, extl0);
+		if (extr0 > 0) extr = std::min(extr, extr0);
+		const int64_t clen = ctg_len[r->vid >> 1];
+		c.as = r->vs > extl? r->vs - extl : 0;
+		c.ae = r->ve + extr < clen? r->ve + extr : clen;
+		c.nt.resize((size_t)std::max<int64_t>(1, c.ae - c.as));
+		if (strand_bases(nt4, ctg_off, ctg_len, r->vid, c.as, c.ae, c.nt.data()) != c.ae - c.as) {
+			nsh_set_error("nsh_align_stage: region %d outside its contig", k);
+			return 0;
+		}
+		c.vs0 = r->vs;
+		c.vs1 = c.vs0 + anchor_x(c.a[i]) + 1;
+		c.as1 = anchor_q(c.a[i]) + 1;
+		c.as1_win = c.as1;
+		c.alive = true;
+		live.push_back(&c);
+	}
+	std::vector<nsh_ext_req> req;
+	std::vector<nsh_ext_result_t> res;
+	std::vector<Cand*> who;
+	// MP_AS_LEFT: all left extensions
+	for (Cand *c : live) req.push_back(ext_req(c, 0, (int32_t)(c->vs1 - c->as), 0, c->as1_win, 1, ao->io));
+	if (!ext_round(live, req, o, res)) return 0;
+	req.clear();
+	for (size_t k = 0; k < live.size(); ++k) {
+		Cand *c = live[k];
+		c->nt_len = res[k].nt_len, c->aa_len = res[k].aa_len;
+		// MP_AS_LEFT_RETRY: a 5'-end exon missed by the extension gets another try at io_end
+		if (res[k].aa_len != c->as1_win && res[k].nt_len < ao->max_ext && ao->io > ao->io_end) {
+			const int64_t as_alt = c->vs1 - c->as > ao->end_retry_win? c->vs1 - ao->end_retry_win : c->as;
+			req.push_back(ext_req(c, as_alt - c->as, (int32_t)(c->vs1 - as_alt), 0, c->as1_win, 1, ao->io_end));
+			who.push_back(c);
+		}
+	}
+	if (!ext_round(who, req, o, res)) return 0;
+	for (size_t k = 0; k < who.size(); ++k)
+		if (res[k].aa_len == who[k]->as1_win) who[k]->nt_len = res[k].nt_len, who[k]->aa_len = res[k].aa_len;
+	// MP_AS_GAP: the DPs between consecutive trusted anchors (mp_align_gap, align.c:482-540)
+	std::vector<Task> tasks;
+	for (Cand *c : live) {
+		nsh_align_reg_t *r = c->r;
+		r->vs = c->vs1 - c->nt_len;
+		r->qs = c->as1 - c->aa_len;
+		c->ne0 = (int32_t)(r->vs - c->vs0);
+		c->ae0 = r->qs;
+		for (int32_t i = c->i0; i < r->cnt; ++i) {
+			if (!(c->a[i] & kTrusted)) continue;
+			const int32_t ne1 = (int32_t)anchor_x(c->a[i]) + 1, ae1 = anchor_q(c->a[i]) + 1;
+			Task t;
+			t.c = c, t.nt_off = c->ne0 + c->vs0 - c->as, t.nlen = ne1 - c->ne0, t.aa_off = c->ae0, t.alen = ae1 - c->ae0;
+			t.score = 0, t.dp = false;
+			tasks.push_back(std::move(t));
+			c->i0 = i, c->ne0 = ne1, c->ae0 = ae1;
+		}
+		r->ve = c->ne0 + c->vs0, r->qe = c->ae0;
+	}
+	if (!dp_round(tasks, o, ao)) return 0;
+	for (Task &t : tasks) {
+		t.c->score += t.score;
+		for (uint32_t op : t.cig) push_op(t.c->cigar, op & 0xf, (int32_t)(op >> 4));
+	}
+	// MP_AS_RIGHT: right extensions where protein and window remain
+	req.clear(), who.clear();
+	for (Cand *c : live) {
+		nsh_align_reg_t *r = c->r;
+		if (r->qe < c->len && r->ve < c->ae) {
+			c->aa_win = c->len - r->qe;
+			c->right = true;
+			req.push_back(ext_req(c, r->ve - c->as, (int32_t)(c->ae - r->ve), r->qe, c->aa_win, 0, ao->io));
+			who.push_back(c);
+		}
+	}
+	if (!ext_round(who, req, o, res)) return 0;
+	std::vector<Cand*> rwho;
+	std::vector<nsh_ext_req> rreq;
+	for (size_t k = 0; k < who.size(); ++k) {
+		Cand *c = who[k];
+		c->nt_len = res[k].nt_len, c->aa_len = res[k].aa_len;
+		// MP_AS_RIGHT_RETRY: the same for a 3'-end exon
+		if (c->aa_len < c->aa_win && c->nt_len < ao->max_ext && ao->io > ao->io_end) {
+			const nsh_align_reg_t *r = c->r;
+			const int32_t l_ext = (int32_t)std::min<int64_t>(c->ae - r->ve, ao->end_retry_win);
+			rreq.push_back(ext_req(c, r->ve - c->as, l_ext, r->qe, c->aa_win, 0, ao->io_end));
+			rwho.push_back(c);
+		}
+	}
+	if (!ext_round(rwho, rreq, o, res)) return 0;
+	for (size_t k = 0; k < rwho.size(); ++k)
+		if (res[k].aa_len == rwho[k]->aa_win) rwho[k]->nt_len = res[k].nt_len, rwho[k]->aa_len = res[k].aa_len;
+	// MP_AS_RIGHT_APPLY: the DP over the right extension
+	tasks.clear();
+	for (Cand *c : who) {
+		const nsh_align_reg_t *r = c->r;
+		Task t;
+		t.c = c, t.nt_off = r->ve - c->as, t.nlen = c->nt_len, t.aa_off = r->qe, t.alen = c->aa_len, t.score = 0, t.dp = false;
+		tasks.push_back(std::move(t));
+	}
+	if (!dp_round(tasks, o, ao)) return 0;
+	for (Task &t : tasks) {
+		t.c->score += t.score;
+		for (uint32_t op : t.cig) push_op(t.c->cigar, op & 0xf, (int32_t)(op >> 4));
+		t.c->r->ve += t.c->nt_len, t.c->r->qe += t.c->aa_len;
+	}
+	// MP_AS_FINISH: the CIGARs, one after another
+	int64_t tot = 0;
+	for (Cand *c : live) tot += (int64_t)c->cigar.size();
+	uint32_t *out = (uint32_t*)std::malloc((size_t)std::max<int64_t>(tot, 1) * sizeof(uint32_t));
+	if (!out) { nsh_set_error("nsh_align_stage: out of host memory"); return 0; }
+	int64_t off = 0;
+	for (Cand *c : live) {
+		nsh_align_reg_t *r = c->r;
+		r->status = 1, r->dp_score = c->score, r->win_as = c->as, r->win_ae = c->ae;
+		r->cigar_off = off, r->n_cigar = (int32_t)c->cigar.size();
+		if (!c->cigar.empty()) std::memcpy(out + off, c->cigar.data(), c->cigar.size() * sizeof(uint32_t));
+		off += (int64_t)c->cigar.size();
+	}
+	*cigar_out = out;
+	return 1;
+}
+
+extern "C" int64_t nsh_align_records(int32_t n_ctg, const uint8_t *nt4, const int64_t *ctg_off, const int64_t *ctg_len,
+		const char *const *ctg_name, const char *const *qname, const char *const *aa, const int32_t *qlen,
+		const nsh_align_reg_t *r, const uint32_t *cigar, const nsh_align_opt_t *ao, uint32_t flag, int64_t id,
+		int32_t hit_idx, char *out, int64_t size)
+{
+	if (!r || r->status == 0 || (r->vid >> 1) >= (uint32_t)n_ctg) return 0;
+	ns_opt_t o;
+	ns_opt_init(&o);
+	o.go = ao->go, o.ge = ao->ge, o.fs = ao->fs, o.sc = ao->mat;
+	std::vector<uint8_t> win((size_t)std::max<int64_t>(1, r->win_ae - r->win_as));
+	if (strand_bases(nt4, ctg_off, ctg_len, r->vid, r->win_as, r->win_ae, win.data()) != r->win_ae - r->win_as) {
+		nsh_set_error("nsh_align_records: window outside its contig");
+		return -1;
+	}
+	nsh_aln_t a = {};
+	a.qname = qname[r->query], a.aa = aa[r->query], a.qlen = qlen[r->query], a.qs = r->qs, a.qe = r->qe;
+	a.ctg_name = ctg_name[r->vid >> 1], a.ctg_len = ctg_len[r->vid >> 1], a.rev = r->vid & 1, a.vs = r->vs, a.ve = r->ve;
+	a.dp_score = r->dp_score, a.profile_score = INT32_MIN, a.chn_sc = r->chn_sc, a.chn_sc_ungap = r->chn_sc_ungap;
+	a.cnt = r->cnt, a.cigar = cigar + r->cigar_off, a.n_cigar = r->n_cigar;
+	nsh_aln_stats_t st;
+	std::vector<nsh_feat_t> feat(256);
+	int32_t nf = nsh_aln_stats(&a, win.data(), r->win_as, r->win_ae, &o, nullptr, &st, feat.data(), (int32_t)feat.size());
+	if (nf < 0) {   // more exons than room: once more with enough
+		feat.resize(4096);
+		nf = nsh_aln_stats(&a, win.data(), r->win_as, r->win_ae, &o, nullptr, &st, feat.data(), (int32_t)feat.size());
+		if (nf < 0) return -1;
+	}
+	// mp_set_rank_score (hit.c:383-399): the aligned region ranks by its peak DP score
+	a.rank_sc = nf > 0? st.dp_max : 0;
+	return nsh_write_output(out, size, &a, nf > 0? &st : nullptr, feat.data(), win.data(), r->win_as, flag, 0, nullptr,
+			id, -1, hit_idx);
+}

@@ -360,3 +360,81 @@ def gapless_minibatch(n_query=5000, genome_len=20_000_000, n_contigs=24, seed=9,
     for k2, v in cand.items():
         out[k2] = np.array(v, dtype=np.uint32 if k2 == "vid" else np.int64)
     return out
+
+
+def genome_with_genes(n_prot=20000, genome_len=100_000_000, n_ctg=24, seed=21, gc=0.42, repeat_frac=0.05,
+                      med_len=400, max_len=3000):
+    """Genome-scale workload: proteins with log-normal lengths (median med_len, cap max_len) and a
+    genome of genome_len bases (GC gc) in n_ctg contigs, with repeat_frac of it in low-complexity
+    tandem repeats (units of 1-6 bases) and one planted gene model per protein on a random strand:
+    Poisson(6) exons (at least 1), GT..AG introns log-normal (median 1 kb, cap 50 kb; scaled down
+    together when the genes would not fit in 90% of the genome), 5-35% amino-acid substitutions.
+    Returns (contig names, contig sequences as bytes, protein names, protein sequences as bytes,
+    intron scale)."""
+    rng = np.random.default_rng(seed)
+    p = np.array([(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2])
+    g = NT[rng.choice(4, size=genome_len, p=p)]
+    # low-complexity tandem repeats
+    n_rep, covered = 0, 0
+    while covered < repeat_frac * genome_len:
+        unit = NT[rng.integers(0, 4, size=int(rng.integers(1, 7)))]
+        L = int(rng.integers(50, 2000))
+        s = int(rng.integers(0, genome_len - L))
+        g[s:s + L] = np.resize(unit, L)
+        covered += L
+        n_rep += 1
+    lens = np.minimum(max_len, np.maximum(30, np.round(np.exp(rng.normal(np.log(med_len), 0.6, size=n_prot))))).astype(int)
+    prots = [rand_protein(rng, int(L)) for L in lens]
+    n_exon = np.maximum(1, rng.poisson(6, size=n_prot))
+    introns = [np.minimum(50_000, np.maximum(40, np.exp(rng.normal(np.log(1000), 0.8, size=k - 1)))) for k in n_exon]
+    cds_total = int(3 * lens.sum())
+    intr_total = float(sum(x.sum() for x in introns))
+    budget = 0.9 * genome_len - cds_total - 200 * n_prot
+    scale = min(1.0, budget / intr_total) if intr_total > 0 else 1.0
+    genes = []
+    for k in range(n_prot):
+        rate = rng.uniform(0.05, 0.35)
+        cds = reverse_translate(rng, substitute(rng, prots[k], rate))
+        ni = len(introns[k])
+        cuts = sorted(rng.choice(np.arange(1, len(cds)), size=min(ni, len(cds) - 1), replace=False).tolist()) if ni else []
+        seq, last = bytearray(), 0
+        for c, il in zip(cuts, introns[k]):
+            seq += cds[last:c]
+            L = max(8, int(il * scale))
+            seq += b"GT" + rand_dna(rng, L - 4) + b"AG"
+            last = c
+        seq += cds[last:]
+        if rng.random() < 0.5:   # reverse strand
+            seq = bytes(seq[::-1]).translate(bytes.maketrans(b"ACGT", b"TGCA"))
+        genes.append(bytes(seq))
+    # place the genes one after another at random spacing, in random order
+    order = rng.permutation(n_prot)
+    free = genome_len - sum(len(x) for x in genes)
+    gaps = rng.dirichlet(np.ones(n_prot + 1)) * free
+    pos = 0.0
+    for j, k in enumerate(order):
+        pos += gaps[j]
+        s = int(pos)
+        g[s:s + len(genes[k])] = np.frombuffer(genes[k], dtype=np.uint8)
+        pos += len(genes[k])
+    cuts = np.sort(rng.choice(np.arange(10_000, genome_len - 10_000), n_ctg - 1, replace=False))
+    bounds = np.concatenate([[0], cuts, [genome_len]])
+    gb = g.tobytes()
+    ctg_names = ["chr%d" % (i + 1) for i in range(n_ctg)]
+    ctgs = [gb[bounds[i]:bounds[i + 1]] for i in range(n_ctg)]
+    names = ["p%05d" % i for i in range(n_prot)]
+    return ctg_names, ctgs, names, prots, scale
+
+
+def write_fasta(path, names, seqs, width=0):
+    """FASTA (one line per sequence unless width > 0); gzip-compressed when the path ends in .gz"""
+    import gzip
+    op = gzip.open if path.endswith(".gz") else open
+    with op(path, "wb") as f:
+        for n, s in zip(names, seqs):
+            f.write(b">" + n.encode() + b"\n")
+            if width:
+                for i in range(0, len(s), width):
+                    f.write(s[i:i + width] + b"\n")
+            else:
+                f.write(s + b"\n")

@@ -23,6 +23,9 @@ Outputs (inputs and expected outputs only):
   dropin_cases.json.gz  the reference's own program (oracle/_ref/protlocus, its CPU build) on the inputs
                      of its integration tests (dropin/: DPP3 and rescue) under several option sets:
                      the expected PAF / GFF3 of the drop-in check
+  config4_align_cases.json.gz  reduced genome-scale workload (300 proteins x 5 Mb, synth.genome_with_genes seed 3):
+                     each query's refined candidates as the reference's mp_map_finish hands them to
+                     mp_align_batch, and what its align stage makes of them (fields, CIGARs, PAF, GFF3)
   tables.json        ns_tab_nt4 / ns_tab_aa20 / ns_tab_codon / ns_mat_blosum62
 """
 import base64
@@ -206,6 +209,67 @@ def dropin_cases(ref):
     return out
 
 
+class AReg(C.Structure):
+    """or_areg_t of oracle/map_ref.c"""
+    _fields_ = [("vid", C.c_int32), ("qs", C.c_int32), ("qe", C.c_int32), ("cnt", C.c_int32), ("chn_sc", C.c_int32),
+                ("chn_sc_ungap", C.c_int32), ("id", C.c_int32), ("parent", C.c_int32), ("vs", C.c_int64),
+                ("ve", C.c_int64), ("ext", C.c_uint64)]
+
+
+def config4_reduced():
+    """the reduced genome-scale workload: (contig codes, offsets, lengths, protein names, sequences)"""
+    cn, cs, pn, ps, _ = synth.genome_with_genes(n_prot=300, genome_len=5_000_000, n_ctg=4, seed=3)
+    lut = np.full(256, 4, np.uint8)
+    for i, c in enumerate(b"ACGT"):
+        lut[c] = i
+    g = lut[np.frombuffer(b"".join(cs), np.uint8)]
+    ln = np.array([len(x) for x in cs], np.int64)
+    off = np.concatenate([[0], np.cumsum(ln)[:-1]]).astype(np.int64)
+    return g, off, ln, pn, ps
+
+
+def config4_align_cases(ref):
+    """per query: the reference's refined candidates (or_ref_align_inputs) and its align stage on them
+    (or_ref_align_stage): per-candidate status, spans, DP score, CIGAR, PAF and GFF3 records"""
+    L = ref.lib
+    L.or_ref_align_inputs.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64,
+                                      C.c_void_p, C.c_void_p]
+    L.or_ref_align_inputs.restype = C.c_int32
+    L.or_ref_align_stage.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                     C.c_uint32, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_char_p, C.c_int64]
+    L.or_ref_align_stage.restype = C.c_int64
+    g, off, ln, pn, ps = config4_reduced()
+    h = L.or_ref_index_build(len(off), g.ctypes.data, off.ctypes.data, ln.ctypes.data, 30, 6, 1, 8)
+    out = []
+    for qi in range(len(ps)):
+        regs = (AReg * 512)()
+        anch = np.zeros(1 << 21, np.uint64)
+        na = C.c_int64()
+        n = L.or_ref_align_inputs(h, pn[qi].encode(), ps[qi], len(ps[qi]), 512, regs, len(anch), anch.ctypes.data,
+                                  C.byref(na))
+        assert n >= 0, qi
+        rec = {"q": qi, "regions": [[regs[k].vid, regs[k].qs, regs[k].qe, regs[k].cnt, regs[k].chn_sc,
+                                     regs[k].chn_sc_ungap, regs[k].vs, regs[k].ve, str(regs[k].ext)] for k in range(n)],
+               "anchors": B64(anch[:na.value].tobytes())}
+        for flag, key in ((0, "paf"), (0x8, "gff")):
+            st = np.zeros(max(n, 1), np.int32)
+            f = np.zeros(8 * max(n, 1), np.int64)
+            cig = np.zeros(1 << 20, np.uint32)
+            txt = C.create_string_buffer(1 << 22)
+            r = L.or_ref_align_stage(h, pn[qi].encode(), ps[qi], len(ps[qi]), n, regs, anch.ctypes.data, flag,
+                                     st.ctypes.data, f.ctypes.data, len(cig), cig.ctypes.data, txt, 1 << 22)
+            assert r >= 0, qi
+            rec[key] = B64(txt.raw[:r])
+            if flag == 0:
+                fs = f[:8 * n].reshape(-1, 8) if n else np.zeros((0, 8), np.int64)
+                rec["status"] = st[:n].tolist()
+                rec["fields"] = fs.tolist()
+                rec["cigar"] = B64(cig[:int(fs[:, 7].sum())].tobytes())
+        out.append(rec)
+    L.or_ref_index_free(h)
+    return out
+
+
 def pssm_chain_cases(ref):
     """mp_chain_pssm_ev (map.c:123-139) and mp_diag_seg (97-120) of the compiled reference on
     refine chains from cases.chain_case"""
@@ -386,6 +450,7 @@ def main():
             ("index_cases", index_cases), ("anchor_cases", anchor_cases),
             ("anchor_chain_cases", anchor_chain_cases),
             ("output_cases", output_cases), ("config0_cases", config0_cases), ("dropin_cases", dropin_cases),
+            ("config4_align_cases", config4_align_cases),
             ("hmm_cases", hmm_cases)]
     want = set(sys.argv[1:])
     for name, fn in gens:
