@@ -294,6 +294,38 @@ int64_t nsh_align_records(int32_t n_ctg, const uint8_t *nt4, const int64_t *ctg_
 		const nsh_align_reg_t *r, const uint32_t *cigar, const nsh_align_opt_t *opt, uint32_t flag, int64_t id,
 		int32_t hit_idx, char *out, int64_t size);
 
+/* The map path for a batch of proteins (what the reference's worker_pipeline does for one mini-batch
+ * with its default options: mp_map_coarse, mp_map_finish and the output step, map.c:547-870,
+ * 1143-1166; no --accuracy, HMM queries, --prescreen, --cand-table or --gpu-prefilter). The mapper
+ * holds the genome (a copy of the base codes 0..4, contigs one after another, and their names) and its
+ * k-mer block index on the device (mp_idx_build's options: min_aa_len, kmer, mod_bit, bbit; the tables
+ * must be set up first, ns_make_tables). nsh_map_batch maps n_q proteins (ASCII) and returns the
+ * records mp_write_output writes for them, in query order, in one malloc'd NUL-terminated buffer
+ * (*out, *out_len bytes; free() it): PAF, or GFF3/GTF by opt->flag (MP_F_GFF 0x8, MP_F_NO_PAF 0x10,
+ * MP_F_GTF 0x20, MP_F_NO_CS 0x200); the "##gff-version 3" line that mp_map_file prints first is the
+ * caller's, and unmapped proteins write nothing (MP_F_SHOW_UNMAP is not supported). *id is the
+ * output step's running record id (pipeline_t.id), advanced by the records written. stats (8
+ * entries, may be NULL): coarse chains, refine candidates, refine anchor pairs, DP candidates,
+ * proteins with a record, records, 0, 0. 1 ok, 0 error (reason in ns_gpu_error). */
+typedef struct {
+	nsh_align_opt_t dp;                   /* the DP options, and max_ext, max_intron, kmer2 */
+	int32_t max_occ, max_gap, bw, max_chn_max_skip, max_chn_iter, min_chn_cnt, min_chn_sc;
+	float chn_coef_log;
+	int32_t max_ava, best_n, out_n, mask_len;
+	float pri_ratio, mask_level, out_sim, out_cov;
+	uint32_t flag;                        /* output flags, as above */
+	int32_t gff_delim;
+	const char *gff_prefix;               /* NULL: "PL" */
+	int32_t n_threads;                    /* host threads of the host steps */
+} nsh_map_opt_t;
+typedef struct nsh_mapper_s nsh_mapper_t;
+void nsh_map_opt_default(nsh_map_opt_t *o);   /* mp_mapopt_init (options.c:42-138), 16 host threads */
+nsh_mapper_t *nsh_mapper_init(int32_t n_ctg, const char *const *ctg_name, const uint8_t *nt4, const int64_t *ctg_len,
+		int32_t min_aa_len, int32_t kmer, int32_t mod_bit, int32_t bbit);
+int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qname, const char *const *seq, const int32_t *qlen,
+		const nsh_map_opt_t *opt, int64_t *id, char **out, int64_t *out_len, int64_t *stats);
+void nsh_mapper_destroy(nsh_mapper_t *m);
+
 /* Batched global spliced DP with CIGAR (no reference counterpart; the batch form of
  * ns_global_gs16b / ns_global_gs32b / ns_global_gs_auto with NS_F_CIGAR): n problems with one option
  * set (opt->flag must hold NS_F_CIGAR and no extension flag; opt->pssm applies to every problem, so it

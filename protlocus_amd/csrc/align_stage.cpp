@@ -53,25 +53,6 @@ void push_op(std::vector<uint32_t> &c, uint32_t op, int32_t len)
 	else c.back() += (uint32_t)len << 4;
 }
 
-// oriented bases of [st, en) of strand vid (mp_ntseq_get_by_v, ntseq.c:92-118); -1 when st is out of range
-int64_t strand_bases(const uint8_t *nt4, const int64_t *ctg_off, const int64_t *ctg_len, uint32_t vid, int64_t st,
-		int64_t en, uint8_t *out)
-{
-	const int64_t L = ctg_len[vid >> 1], base = ctg_off[vid >> 1];
-	if (st < 0 || en < 0 || st >= L) return -1;
-	if (en > L) en = L;
-	int64_t k = 0;
-	if (!(vid & 1)) {
-		for (int64_t i = st; i < en; ++i) out[k++] = nt4[base + i];
-	} else {   // reverse strand: complement of the forward bases, right to left
-		for (int64_t i = L - st - 1; i >= L - en; --i) {
-			const uint8_t c = nt4[base + i];
-			out[k++] = c >= 4? c : (uint8_t)(3 - c);
-		}
-	}
-	return k;
-}
-
 struct Cand {
 	nsh_align_reg_t *r = nullptr;
 	const char *aa = nullptr;
@@ -90,6 +71,25 @@ struct Cand {
 struct Task { Cand *c; int64_t nt_off; int32_t nlen, aa_off, alen; int32_t score; std::vector<uint32_t> cig; bool dp; };
 
 }   // namespace
+
+// oriented bases of [st, en) of strand vid (mp_ntseq_get_by_v, ntseq.c:92-118); -1 when st is out of range
+int64_t nsh_strand_bases(const uint8_t *nt4, const int64_t *ctg_off, const int64_t *ctg_len, uint32_t vid, int64_t st,
+		int64_t en, uint8_t *out)
+{
+	const int64_t L = ctg_len[vid >> 1], base = ctg_off[vid >> 1];
+	if (st < 0 || en < 0 || st >= L) return -1;
+	if (en > L) en = L;
+	int64_t k = 0;
+	if (!(vid & 1)) {
+		for (int64_t i = st; i < en; ++i) out[k++] = nt4[base + i];
+	} else {   // reverse strand: complement of the forward bases, right to left
+		for (int64_t i = L - st - 1; i >= L - en; --i) {
+			const uint8_t c = nt4[base + i];
+			out[k++] = c >= 4? c : (uint8_t)(3 - c);
+		}
+	}
+	return k;
+}
 
 extern "C" void nsh_align_opt_default(nsh_align_opt_t *o)
 {
@@ -218,7 +218,7 @@ extern "C" int nsh_align_stage(int32_t n_ctg, const uint8_t *nt4, const int64_t 
 		c.as = r->vs > extl? r->vs - extl : 0;
 		c.ae = r->ve + extr < clen? r->ve + extr : clen;
 		c.nt.resize((size_t)std::max<int64_t>(1, c.ae - c.as));
-		if (strand_bases(nt4, ctg_off, ctg_len, r->vid, c.as, c.ae, c.nt.data()) != c.ae - c.as) {
+		if (nsh_strand_bases(nt4, ctg_off, ctg_len, r->vid, c.as, c.ae, c.nt.data()) != c.ae - c.as) {
 			nsh_set_error("nsh_align_stage: region %d outside its contig", k);
 			return 0;
 		}
@@ -342,7 +342,7 @@ extern "C" int64_t nsh_align_records(int32_t n_ctg, const uint8_t *nt4, const in
 	ns_opt_init(&o);
 	o.go = ao->go, o.ge = ao->ge, o.fs = ao->fs, o.sc = ao->mat;
 	std::vector<uint8_t> win((size_t)std::max<int64_t>(1, r->win_ae - r->win_as));
-	if (strand_bases(nt4, ctg_off, ctg_len, r->vid, r->win_as, r->win_ae, win.data()) != r->win_ae - r->win_as) {
+	if (nsh_strand_bases(nt4, ctg_off, ctg_len, r->vid, r->win_as, r->win_ae, win.data()) != r->win_ae - r->win_as) {
 		nsh_set_error("nsh_align_records: window outside its contig");
 		return -1;
 	}

@@ -50,6 +50,12 @@ int nsh_glob_wait(nsh_glob_req *q, void *km, ns_rst_t *r);
 int nsh_global_run(void *km, const char *ns, int32_t nl, const char *as, int32_t al,
 		const ns_opt_t *opt, const uint8_t *ss, int32_t is32, ns_rst_t *r);
 
+// oriented bases (codes 0..4) of [st, en) of strand vid = contig << 1 | reverse, the contig's bases
+// at nt4 + ctg_off[c] (mp_ntseq_get_by_v, ntseq.c:92-118); en is clipped to the contig. Returns the
+// number written, -1 when st is out of range
+int64_t nsh_strand_bases(const uint8_t *nt4, const int64_t *ctg_off, const int64_t *ctg_len, uint32_t vid, int64_t st,
+		int64_t en, uint8_t *out);
+
 // ---- banded gapped stage of the prescreen search (ps_gapped.hip), device-resident items ----
 #include <hip/hip_runtime.h>
 namespace nsh_psg {

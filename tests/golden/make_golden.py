@@ -26,6 +26,8 @@ Outputs (inputs and expected outputs only):
   config4_align_cases.json.gz  reduced genome-scale workload (300 proteins x 5 Mb, synth.genome_with_genes seed 3):
                      each query's refined candidates as the reference's mp_map_finish hands them to
                      mp_align_batch, and what its align stage makes of them (fields, CIGARs, PAF, GFF3)
+  config4_map_cases.json.gz  the reference's program (PAF, then --gff) on the same reduced workload: the
+                     whole map path, index to records
   tables.json        ns_tab_nt4 / ns_tab_aa20 / ns_tab_codon / ns_mat_blosum62
 """
 import base64
@@ -270,6 +272,27 @@ def config4_align_cases(ref):
     return out
 
 
+def config4_map_cases(ref):
+    """stdout of the reference's CPU program (PAF, then --gff) on the reduced genome-scale workload:
+    the whole map path, index to records, for 300 proteins against 5 Mb"""
+    import subprocess
+    import tempfile
+    prog = os.path.join(ROOT, "oracle", "_ref", "protlocus")
+    if not os.path.exists(prog):
+        raise SystemExit("build the reference's program first: make -C oracle cpu-prog")
+    cn, cs, pn, ps, _ = synth.genome_with_genes(n_prot=300, genome_len=5_000_000, n_ctg=4, seed=3)
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        g, p = os.path.join(d, "g.fa"), os.path.join(d, "p.fa")
+        synth.write_fasta(g, cn, cs)
+        synth.write_fasta(p, pn, ps)
+        for key, args in (("paf", []), ("gff", ["--gff"])):
+            r = subprocess.run([prog, "-t", "4"] + args + [g, p], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                               check=True)
+            out[key] = B64(r.stdout)
+    return out
+
+
 def pssm_chain_cases(ref):
     """mp_chain_pssm_ev (map.c:123-139) and mp_diag_seg (97-120) of the compiled reference on
     refine chains from cases.chain_case"""
@@ -450,7 +473,7 @@ def main():
             ("index_cases", index_cases), ("anchor_cases", anchor_cases),
             ("anchor_chain_cases", anchor_chain_cases),
             ("output_cases", output_cases), ("config0_cases", config0_cases), ("dropin_cases", dropin_cases),
-            ("config4_align_cases", config4_align_cases),
+            ("config4_align_cases", config4_align_cases), ("config4_map_cases", config4_map_cases),
             ("hmm_cases", hmm_cases)]
     want = set(sys.argv[1:])
     for name, fn in gens:
