@@ -591,6 +591,95 @@ def bench_index(cpu=True, genome_len=100_000_000, n_ctg=24, log=None):
     return out
 
 
+def bench_map(cpu=True, log=None, n_prot=20000, batch=20000, sample=2000):
+    """the whole map path on the genome-scale workload: 20,000 proteins (log-normal lengths, median
+    400 aa, max 3000) with one planted gene model each in a 100-Mb genome (24 contigs, GC 0.42, 5% in
+    tandem repeats; synth.genome_with_genes seed 21). The index is built once (timed apart); then
+    nsh_map_batch maps the proteins (coarse chains, refinement, align stage, records) and the proteins
+    per second are over that call. The reference's own program (oracle/_ref/protlocus, its CPU build)
+    maps a sample of the same proteins against the same genome at threads() threads; its mapping time
+    runs from its index build's last log line to its last "mapped" line, and the PAF records of the
+    sampled proteins must be identical to the GPU path's"""
+    import re
+    import subprocess
+    import numpy as np
+    from protlocus_amd import capi, synth
+    t0 = time.perf_counter()
+    cn, cs, pn, ps, scale = synth.genome_with_genes(n_prot=n_prot)
+    gen_s = time.perf_counter() - t0
+    capi.lib()
+    t0 = time.perf_counter()
+    mp = capi.Mapper(cn, cs)
+    index_s = time.perf_counter() - t0
+    try:
+        mp.map(pn[:32], ps[:32])   # warm-up: device pools, streams, code objects
+        recs, stats = [], np.zeros(16, np.int64)
+        t0 = time.perf_counter()
+        for b0 in range(0, n_prot, batch):
+            r, st = mp.map(pn[b0:b0 + batch], ps[b0:b0 + batch])
+            recs.append(r)
+            stats += st
+        map_s = time.perf_counter() - t0
+    finally:
+        mp.close()
+    paf = b"".join(recs)
+    out = {"workload": "%d proteins (log-normal lengths, median 400 aa, max 3000; %d residues) against a 100-Mb "
+                       "genome in 24 contigs with one planted gene model per protein (Poisson(6) exons, GT..AG "
+                       "introns log-normal, median 1 kb scaled by %.3f to fit, 5-35%% divergence), default options, "
+                       "PAF" % (n_prot, sum(len(p) for p in ps), scale),
+           "batch": batch, "index_ms": round(index_s * 1e3, 1), "gpu_s": round(map_s, 3),
+           "proteins_per_s": round(n_prot / map_s, 1), "records": int(stats[5]), "mapped": int(stats[4]),
+           "counts": {"coarse_chains": int(stats[0]), "refine_candidates": int(stats[1]),
+                      "refine_pairs": int(stats[2]), "dp_candidates": int(stats[3])}}
+    if batch >= n_prot:   # one call: the stage clock is that call's
+        prev, stages = 0, {}
+        for k, name in enumerate(capi.MAP_STAGES):
+            stages[name] = round((stats[8 + k] - prev) / 1e3, 1)
+            prev = stats[8 + k]
+        out["stage_ms"] = stages
+    if log:
+        log("[bench] map: %d proteins in %.2f s (%.0f proteins/s), %d records; index %.1f s; synth %.1f s" % (
+            n_prot, map_s, n_prot / map_s, stats[5], index_s, gen_s))
+    prog = os.path.join(ROOT, "oracle", "_ref", "protlocus")
+    if cpu and os.path.exists(prog):
+        nth = threads()
+        with tempfile.TemporaryDirectory() as d:
+            g, p = os.path.join(d, "g.fa"), os.path.join(d, "p.fa")
+            synth.write_fasta(g, cn, cs)
+            synth.write_fasta(p, pn[:sample], ps[:sample])
+            r = subprocess.run([prog, "-t", str(nth), g, p], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                               check=True)
+        err = r.stderr.decode()
+        t_idx = max(float(x) for x in re.findall(r"\[M::mp_idx_build@([0-9.]+)\*", err))
+        t_map = max(float(x) for x in re.findall(r"\[M::worker_pipeline::([0-9.]+)\*[0-9.]+\] mapped", err))
+        cpu_s = t_map - t_idx
+        names = set(x.encode() for x in pn[:sample])
+        mine = b"".join(line + b"\n" for line in paf.split(b"\n") if line and line.split(b"\t", 1)[0] in names)
+        same = mine == r.stdout
+        out["cpu_baseline"] = {"value": round(sample / cpu_s, 1), "unit": "proteins/s (map stage, index excluded)",
+                               "cores": nth, "kind": "reference", "cpu": host_cpu(),
+                               "sample": "the first %d proteins against the whole genome: %.2f s of mapping at %d "
+                                         "threads" % (sample, cpu_s, nth)}
+        out["identical_to_reference"] = bool(same)
+        out["gpu_vs_cpu"] = round(out["proteins_per_s"] / out["cpu_baseline"]["value"], 2)
+        if not same:
+            got = {}
+            for line in mine.split(b"\n"):
+                if line:
+                    got.setdefault(line.split(b"\t", 1)[0], []).append(line)
+            exp = {}
+            for line in r.stdout.split(b"\n"):
+                if line:
+                    exp.setdefault(line.split(b"\t", 1)[0], []).append(line)
+            bad = [q for q in sorted(names) if got.get(q) != exp.get(q)]
+            out["identical_to_reference"] = "%d of %d proteins" % (sample - len(bad), sample)
+            out["first_differences"] = [q.decode() for q in bad[:5]]
+        if log:
+            log("[bench] map: reference %.1f proteins/s at %d threads (%.2f s for %d), identical %s" % (
+                sample / cpu_s, nth, cpu_s, sample, out["identical_to_reference"]))
+    return out
+
+
 def bench_index_chain(g, off, ln, qs, chains, gpu_s, cpu=True, log=None):
     """mp_map_coarse's index -> chain slice (map.c:583-633, default options): anchors, pre-chain,
     each query's kept anchors sorted, chain, all on the device (nsh_index_chain; the queries'
@@ -732,6 +821,9 @@ def main():
     ap.add_argument("--only-gapless-prefilter", action="store_true",
                     help="run only the gapless prefilter section and print its JSON object")
     ap.add_argument("--only-index", action="store_true", help="run only the k-mer index section and print its JSON")
+    ap.add_argument("--only-map", action="store_true",
+                    help="run only the genome-scale map path section and print its JSON")
+    ap.add_argument("--map-batch", type=int, default=20000, help="proteins per nsh_map_batch call")
     ap.add_argument("--only-dropin", action="store_true",
                     help="run only the drop-in section (reference calling patterns from C threads) and print its JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -769,6 +861,9 @@ def main():
         return
     if args.only_index:
         print(json.dumps(bench_index(cpu=not args.no_cpu, log=log)), flush=True)
+        return
+    if args.only_map:
+        print(json.dumps(bench_map(cpu=not args.no_cpu, log=log, batch=args.map_batch)), flush=True)
         return
 
     import nsffi
@@ -883,6 +978,7 @@ def main():
         out["profile_search"] = bench_profile_search(cpu=not args.no_cpu, log=log)
         out["gapless_prefilter"] = bench_gapless_prefilter(cpu=not args.no_cpu, log=log)
         out["index"] = bench_index(cpu=not args.no_cpu, log=log)
+        out["map"] = bench_map(cpu=not args.no_cpu, log=log, batch=args.map_batch)
     print(json.dumps(out), flush=True)
 
 

@@ -304,9 +304,12 @@ int64_t nsh_align_records(int32_t n_ctg, const uint8_t *nt4, const int64_t *ctg_
  * (*out, *out_len bytes; free() it): PAF, or GFF3/GTF by opt->flag (MP_F_GFF 0x8, MP_F_NO_PAF 0x10,
  * MP_F_GTF 0x20, MP_F_NO_CS 0x200); the "##gff-version 3" line that mp_map_file prints first is the
  * caller's, and unmapped proteins write nothing (MP_F_SHOW_UNMAP is not supported). *id is the
- * output step's running record id (pipeline_t.id), advanced by the records written. stats (8
- * entries, may be NULL): coarse chains, refine candidates, refine anchor pairs, DP candidates,
- * proteins with a record, records, 0, 0. 1 ok, 0 error (reason in ns_gpu_error). */
+ * output step's running record id (pipeline_t.id), advanced by the records written. stats (16
+ * entries, may be NULL): [0] coarse chains, [1] refine candidates, [2] refine anchor pairs, [3] DP
+ * candidates, [4] proteins with a record, [5] records, [8..13] microseconds since the call began at
+ * the end of: the coarse chains, the region selection, the refine sketches and pairs, the refine
+ * chains and selection, the align stage, the statistics and records. 1 ok, 0 error (reason in
+ * ns_gpu_error). */
 typedef struct {
 	nsh_align_opt_t dp;                   /* the DP options, and max_ext, max_intron, kmer2 */
 	int32_t max_occ, max_gap, bw, max_chn_max_skip, max_chn_iter, min_chn_cnt, min_chn_sc;

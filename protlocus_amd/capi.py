@@ -156,6 +156,13 @@ def lib():
                                                    C.c_int32, C.c_void_p, C.c_int32] + [C.c_void_p] * 5 +
                                                   [C.c_int32] * 3 + [C.c_void_p] * 4)
         L.nsh_prescreen_destroy.argtypes = [C.c_void_p]
+        L.nsh_map_opt_default.argtypes = [C.c_void_p]
+        L.nsh_mapper_init.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int32] * 4
+        L.nsh_mapper_init.restype = C.c_void_p
+        L.nsh_map_batch.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int64), C.c_void_p]
+        L.nsh_mapper_destroy.argtypes = [C.c_void_p]
+        L.free.argtypes = [C.c_void_p]
         L.nsh_device_synchronize.restype = C.c_int
         L.nsh_build_info.restype = C.c_char_p
         if L.ns_make_tables(1) != 0:
@@ -686,4 +693,72 @@ class ProfileSearch:
     def close(self):
         if self.p:
             lib().nsh_prescreen_destroy(self.p)
+            self.p = None
+
+
+class AlignOpt(C.Structure):
+    """nsh_align_opt_t: the DP options of the map path (mp_mapopt_t's go .. mat)"""
+    _fields_ = [("go", C.c_int32), ("ge", C.c_int32), ("io", C.c_int32), ("io_end", C.c_int32), ("fs", C.c_int32),
+                ("xdrop", C.c_int32), ("end_bonus", C.c_int32), ("max_ext", C.c_int32), ("max_intron", C.c_int32),
+                ("kmer2", C.c_int32), ("end_retry_win", C.c_int32), ("ie_coef", C.c_float), ("sp_scale", C.c_float),
+                ("sp_model", C.c_int32), ("sp_null_bonus", C.c_int32), ("mat", C.c_int8 * 484)]
+
+
+class MapOpt(C.Structure):
+    """nsh_map_opt_t: mp_mapopt_t's fields the map path reads (defaults from nsh_map_opt_default)"""
+    _fields_ = [("dp", AlignOpt), ("max_occ", C.c_int32), ("max_gap", C.c_int32), ("bw", C.c_int32),
+                ("max_chn_max_skip", C.c_int32), ("max_chn_iter", C.c_int32), ("min_chn_cnt", C.c_int32),
+                ("min_chn_sc", C.c_int32), ("chn_coef_log", C.c_float), ("max_ava", C.c_int32), ("best_n", C.c_int32),
+                ("out_n", C.c_int32), ("mask_len", C.c_int32), ("pri_ratio", C.c_float), ("mask_level", C.c_float),
+                ("out_sim", C.c_float), ("out_cov", C.c_float), ("flag", C.c_uint32), ("gff_delim", C.c_int32),
+                ("gff_prefix", C.c_char_p), ("n_threads", C.c_int32)]
+
+    @classmethod
+    def default(cls, **kw):
+        o = cls()
+        lib().nsh_map_opt_default(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+
+MP_F_GFF, MP_F_NO_PAF, MP_F_GTF, MP_F_NO_CS = 0x8, 0x10, 0x20, 0x200
+MAP_STAGES = ("coarse chains", "region selection", "refine sketches", "refine chains", "align stage", "records")
+
+
+class Mapper:
+    """the map path on one GPU (nsh_mapper_*): a genome and its device index, proteins mapped in
+    batches to the records mp_write_output writes (PAF, or GFF3 with MP_F_GFF)"""
+
+    def __init__(self, ctg_names, ctg_seqs, min_aa_len=30, kmer=6, mod_bit=1, bbit=8):
+        lut = np.full(256, 4, np.uint8)
+        for i, c in enumerate(b"ACGT"):
+            lut[c] = lut[c + 32] = i
+        g = np.ascontiguousarray(lut[np.frombuffer(b"".join(ctg_seqs), np.uint8)])
+        ln = np.array([len(x) for x in ctg_seqs], np.int64)
+        names = (C.c_char_p * len(ctg_names))(*[n.encode() if isinstance(n, str) else n for n in ctg_names])
+        self.p = lib().nsh_mapper_init(len(ctg_names), names, g.ctypes.data, ln.ctypes.data, min_aa_len, kmer,
+                                       mod_bit, bbit)
+        if not self.p:
+            raise RuntimeError("nsh_mapper_init: " + error())
+        self.id = C.c_int64(0)
+
+    def map(self, names, seqs, opt=None):
+        """records of one batch of proteins (bytes) and the call's 16 counters (nsh_map_batch)"""
+        opt = opt or MapOpt.default()
+        qn = (C.c_char_p * len(seqs))(*[n.encode() if isinstance(n, str) else n for n in names])
+        qs = (C.c_char_p * len(seqs))(*[bytes(x) for x in seqs])
+        ql = np.array([len(x) for x in seqs], np.int32)
+        buf, n, st = C.c_void_p(), C.c_int64(), np.zeros(16, np.int64)
+        if not lib().nsh_map_batch(self.p, len(seqs), qn, qs, ql.ctypes.data, C.byref(opt), C.byref(self.id),
+                                   C.byref(buf), C.byref(n), st.ctypes.data):
+            raise RuntimeError("nsh_map_batch: " + error())
+        try:
+            return C.string_at(buf, n.value), st
+        finally:
+            lib().free(buf)
+
+    def close(self):
+        if self.p:
+            lib().nsh_mapper_destroy(self.p)
             self.p = None

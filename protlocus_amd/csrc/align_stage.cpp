@@ -10,6 +10,7 @@
 // what the reference does between the DPs: interval bookkeeping, the ungapped scores of anchor-to-
 // anchor steps and CIGAR concatenation.
 #include <algorithm>
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -102,12 +103,29 @@ extern "C" void nsh_align_opt_default(nsh_align_opt_t *o)
 	ns_set_stop_sc(22, o->mat, (int8_t)o->fs);
 }
 
+// with NSH_PROFILE set in the environment, the host time of each step goes to stderr
+struct StepClock {
+	bool on = std::getenv("NSH_PROFILE") != nullptr;
+	std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+	void lap(const char *what, int64_t n)
+	{
+		if (!on) return;
+		const auto now = std::chrono::steady_clock::now();
+		std::fprintf(stderr, "[nsh_align_stage] %-22s %9.1f ms  n=%lld\n", what,
+				std::chrono::duration<double, std::milli>(now - t).count(), (long long)n);
+		t = now;
+	}
+};
+
 static int ext_round(std::vector<Cand*> &cs, std::vector<nsh_ext_req> &req, const ns_opt_t &o, std::vector<nsh_ext_result_t> &res)
 {
 	res.assign(req.size(), nsh_ext_result_t{});
 	if (req.empty()) return 1;
+	StepClock clk;
 	nsh_ext_plan_t *p = nsh_ext_plan_build((int32_t)req.size(), req.data(), &o);
+	clk.lap("  plan build", (int64_t)req.size());
 	const int ok = p && nsh_ext_plan_run(p) && nsh_ext_plan_results(p, res.data());
+	clk.lap("  plan run", (int64_t)req.size());
 	if (p) nsh_ext_plan_destroy(p);
 	(void)cs;
 	return ok;
@@ -193,6 +211,7 @@ extern "C" int nsh_align_stage(int32_t n_ctg, const uint8_t *nt4, const int64_t 
 	o.ie_coef = ao->ie_coef, o.end_bonus = ao->end_bonus, o.sp_null_bonus = ao->sp_null_bonus;
 	ns_opt_set_sp(&o, ao->sp_model);
 	for (int t = 0; t < 6; ++t) o.sp[t] = (int32_t)(o.sp[t] * ao->sp_scale + .499f);
+	StepClock clk;
 	std::vector<Cand> cand(n_reg);
 	std::vector<Cand*> live;
 	// MP_AS_START: trusted anchors, the search window, its bases (mp_align_start, align.c:431-480)
@@ -229,6 +248,7 @@ extern "C" int nsh_align_stage(int32_t n_ctg, const uint8_t *nt4, const int64_t 
 		c.alive = true;
 		live.push_back(&c);
 	}
+	clk.lap("start", (int64_t)live.size());
 	std::vector<nsh_ext_req> req;
 	std::vector<nsh_ext_result_t> res;
 	std::vector<Cand*> who;
@@ -249,6 +269,7 @@ extern "C" int nsh_align_stage(int32_t n_ctg, const uint8_t *nt4, const int64_t 
 	if (!ext_round(who, req, o, res)) return 0;
 	for (size_t k = 0; k < who.size(); ++k)
 		if (res[k].aa_len == who[k]->as1_win) who[k]->nt_len = res[k].nt_len, who[k]->aa_len = res[k].aa_len;
+	clk.lap("left extensions", (int64_t)who.size());
 	// MP_AS_GAP: the DPs between consecutive trusted anchors (mp_align_gap, align.c:482-540)
 	std::vector<Task> tasks;
 	for (Cand *c : live) {
@@ -268,7 +289,9 @@ extern "C" int nsh_align_stage(int32_t n_ctg, const uint8_t *nt4, const int64_t 
 		}
 		r->ve = c->ne0 + c->vs0, r->qe = c->ae0;
 	}
+	clk.lap("gap tasks", (int64_t)tasks.size());
 	if (!dp_round(tasks, o, ao)) return 0;
+	clk.lap("gap DPs", (int64_t)tasks.size());
 	for (Task &t : tasks) {
 		t.c->score += t.score;
 		for (uint32_t op : t.cig) push_op(t.c->cigar, op & 0xf, (int32_t)(op >> 4));
@@ -301,6 +324,7 @@ extern "C" int nsh_align_stage(int32_t n_ctg, const uint8_t *nt4, const int64_t 
 	if (!ext_round(rwho, rreq, o, res)) return 0;
 	for (size_t k = 0; k < rwho.size(); ++k)
 		if (res[k].aa_len == rwho[k]->aa_win) rwho[k]->nt_len = res[k].nt_len, rwho[k]->aa_len = res[k].aa_len;
+	clk.lap("right extensions", (int64_t)rwho.size());
 	// MP_AS_RIGHT_APPLY: the DP over the right extension
 	tasks.clear();
 	for (Cand *c : who) {
@@ -315,6 +339,7 @@ extern "C" int nsh_align_stage(int32_t n_ctg, const uint8_t *nt4, const int64_t 
 		for (uint32_t op : t.cig) push_op(t.c->cigar, op & 0xf, (int32_t)(op >> 4));
 		t.c->r->ve += t.c->nt_len, t.c->r->qe += t.c->aa_len;
 	}
+	clk.lap("right DPs", (int64_t)tasks.size());
 	// MP_AS_FINISH: the CIGARs, one after another
 	int64_t tot = 0;
 	for (Cand *c : live) tot += (int64_t)c->cigar.size();

@@ -6,8 +6,8 @@
 // the whole batch at once, on the device where the work is data-parallel:
 //   coarse   index -> anchors -> pre-chain -> chain of every query   (nsh_index_chain, device)
 //            regions from the chain blocks, sort, parents, selection   (host, per query)
-//   refine   query and window sketches, all-vs-all bucket pairs       (host threads)
-//            the window chains of every region of every query          (nsh_chain_batch, device)
+//   refine   the query sketches                                        (host)
+//            window k-mers shared with the query, bucket pairs, chains  (refine.hip, device)
 //   align    extensions and gap DPs of every candidate                 (nsh_align_stage, device)
 //            statistics, ranking, selection and the PAF/GFF3 records   (host threads)
 // Every step keeps the reference's arithmetic and order, so the records are the reference's byte
@@ -15,6 +15,7 @@
 // radix sort of ksort.h), that sort is restated step for step.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -106,52 +107,6 @@ void sketch_prot(const char *seq, int32_t len, int32_t kmer, int32_t mod_bit, st
 			}
 		} else x = 0, l = 0;
 	}
-}
-
-// mp_sketch_clean_orf (sketch.c:111-131) with bbit 0 and no block offset: the k-mers of a clean ORF
-// [st, en) in the 13-letter alphabet, positioned at the last base of their last codon
-void sketch_orf(const uint8_t *seq, int64_t st, int64_t en, int32_t kmer, uint32_t mask_k, std::vector<uint64_t> &out)
-{
-	uint32_t x = 0;
-	int32_t l = 0;
-	for (int64_t i = st; i < en; i += 3) {
-		const uint8_t codon = (uint8_t)(seq[i] << 4 | seq[i + 1] << 2 | seq[i + 2]);
-		x = (x << 4 | ns_tab_codon13[codon]) & mask_k;
-		if (++l >= kmer) out.push_back((uint64_t)hash32_mask(x, mask_k) << 32 | (uint64_t)(i + 2));
-	}
-}
-
-// mp_sketch_nt4 (sketch.c:133-169) with mod_bit 0 and bbit 0, as mp_refine_reg sketches a window:
-// clean ORFs of at least min_aa_len codons in the three frames, then sorted and deduplicated
-void sketch_window(const uint8_t *seq, int64_t len, int32_t min_aa_len, int32_t kmer, std::vector<uint64_t> &out)
-{
-	const uint32_t mask_k = (1U << kmer * 4) - 1;
-	int64_t e[3] = { -1, -1, -1 }, k[3] = { 0, 0, 0 }, l = 0;
-	uint8_t codon = 0, p = 1;
-	out.clear();
-	for (int64_t i = 0; i < len; ++i, ++p) {
-		if (p == 3) p = 0;
-		if (seq[i] < 4) {
-			codon = (uint8_t)((codon << 2 | seq[i]) & 0x3f);
-			if (++l >= 3) {
-				if (ns_tab_codon[codon] >= 20) {   // a stop ends the ORF of this frame
-					if (k[p] >= min_aa_len) sketch_orf(seq, e[p] + 1 - k[p] * 3, e[p] + 1, kmer, mask_k, out);
-					k[p] = 0, e[p] = -1;
-				} else e[p] = i, ++k[p];
-			}
-		} else {   // an ambiguous base ends the ORFs of all frames
-			for (int q = 0; q < 3; ++q) {
-				if (k[q] >= min_aa_len) sketch_orf(seq, e[q] + 1 - k[q] * 3, e[q] + 1, kmer, mask_k, out);
-				k[q] = 0, e[q] = -1;
-			}
-			l = 0, codon = 0;
-		}
-	}
-	for (int q = 0; q < 3; ++q)
-		if (k[q] >= min_aa_len) sketch_orf(seq, e[q] + 1 - k[q] * 3, e[q] + 1, kmer, mask_k, out);
-	if (out.size() <= 1) return;
-	std::sort(out.begin(), out.end());
-	out.erase(std::unique(out.begin(), out.end()), out.end());
 }
 
 // one region of a query (the fields of mp_reg1_t the map path reads)
@@ -431,7 +386,9 @@ struct nsh_mapper_s {
 	std::vector<uint8_t> nt;                 // base codes 0..4, contigs one after another
 	std::vector<int64_t> off, len;
 	std::vector<uint32_t> bo;                // first block of each strand, 2 n_ctg + 1 entries
+	int64_t total = 0;                       // bases
 	nsh_index_t *ix = nullptr;
+	uint8_t *d_g = nullptr;                  // HBM: the contigs, then their reverse complements
 };
 
 extern "C" void nsh_map_opt_default(nsh_map_opt_t *o)
@@ -465,10 +422,12 @@ extern "C" nsh_mapper_t *nsh_mapper_init(int32_t n_ctg, const char *const *ctg_n
 	}
 	for (auto &s : m->name) m->name_p.push_back(s.c_str());
 	m->nt.assign(nt4, nt4 + tot);
+	m->total = tot;
+	if (!nsh_refine::oriented_genome(m->nt.data(), m->off.data(), m->len.data(), n_ctg, tot, &m->d_g)) { delete m; return nullptr; }
 	m->ix = nsh_index_build(n_ctg, m->nt.data(), m->off.data(), m->len.data(), min_aa_len, kmer, mod_bit, bbit);
-	if (!m->ix) { delete m; return nullptr; }
+	if (!m->ix) { nsh_mapper_destroy(m); return nullptr; }
 	m->bo.resize(2 * (size_t)n_ctg + 1);
-	if (!nsh_index_copy(m->ix, nullptr, nullptr, m->bo.data())) { nsh_index_destroy(m->ix); delete m; return nullptr; }
+	if (!nsh_index_copy(m->ix, nullptr, nullptr, m->bo.data())) { nsh_mapper_destroy(m); return nullptr; }
 	return m;
 }
 
@@ -476,6 +435,7 @@ extern "C" void nsh_mapper_destroy(nsh_mapper_t *m)
 {
 	if (!m) return;
 	if (m->ix) nsh_index_destroy(m->ix);
+	if (m->d_g) (void)hipFree(m->d_g);
 	delete m;
 }
 
@@ -489,7 +449,11 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 	*out = nullptr, *out_len = 0;
 	const int32_t nth = o->n_threads > 0? o->n_threads : 1;
 	const int32_t kmer2 = o->dp.kmer2;
-	int64_t st[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+	int64_t st[16] = { 0 };
+	const auto t_start = std::chrono::steady_clock::now();
+	auto lap = [&](int k) {   // microseconds since the call began, at the end of stage k
+		st[8 + k] = (int64_t)std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t_start).count();
+	};
 	if (n_q == 0) {
 		*out = (char*)std::malloc(1);
 		if (stats) std::memcpy(stats, st, sizeof(st));
@@ -518,6 +482,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 		cap_u = std::max(cap_u, nu), cap_a = std::max(cap_a, c_off[n_q]);
 	}
 	st[0] = u_off[n_q];
+	lap(0);
 
 	std::vector<std::vector<Reg>> pool(n_q);
 	std::vector<Regs> regs(n_q);
@@ -532,8 +497,9 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 		select_sub(pool[q], regs[q], sel_ratio * sel_ratio, m->kmer * 2, sel_n);
 	});
 	if (bad) { nsh_set_error("nsh_map_batch: a chain block is outside the index"); return 0; }
+	lap(1);
 
-	// ---- refine: window sketches and bucket pairs per region on the host (map.c:141-203, 666-684)
+	// ---- refine: every region's window against its query (map.c:141-258, 666-684)
 	struct RefJob { int32_t q, r; int64_t as; };
 	std::vector<RefJob> jobs;
 	std::vector<std::vector<uint64_t>> qsk(n_q);
@@ -548,49 +514,33 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 		sketch_prot(seq[q], qlen[q], kmer2, 0, qsk[q]);
 		std::sort(qsk[q].begin(), qsk[q].end());
 	});
-	std::vector<std::vector<uint64_t>> pairs(jobs.size());
-	parallel_for((int64_t)jobs.size(), nth, [&](int64_t t) {
+	// the windows (mp_refine_reg, map.c:152-156): slices of the oriented genome in HBM
+	std::vector<nsh_refine::Job> rjobs(jobs.size());
+	for (size_t t = 0; t < jobs.size(); ++t) {
 		RefJob &jb = jobs[t];
-		Reg &r = pool[jb.q][regs[jb.q][jb.r]];
+		const Reg &r = pool[jb.q][regs[jb.q][jb.r]];
 		const int32_t extl = (int32_t)(ext0[jb.q][jb.r] >> 32), extr = (int32_t)ext0[jb.q][jb.r];
 		const int64_t clen = m->len[r.vid >> 1];
 		const int64_t as = r.vs > extl? r.vs - extl : 0, ae = r.ve + extr < clen? r.ve + extr : clen;
 		jb.as = as;
-		std::vector<uint8_t> nt((size_t)std::max<int64_t>(1, ae - as));
-		const int64_t l_nt = nsh_strand_bases(m->nt.data(), m->off.data(), m->len.data(), r.vid, as, ae, nt.data());
-		std::vector<uint64_t> sd;
-		if (l_nt > 0) sketch_window(nt.data(), l_nt, m->min_aa_len, kmer2, sd);
-		for (uint64_t x : qsk[jb.q]) sd.push_back(x | 1ULL << 31);
-		std::sort(sd.begin(), sd.end());
-		std::vector<uint64_t> &a = pairs[t];
-		const int64_t n = (int64_t)sd.size();
-		for (int64_t k = 0, i = 1; i <= n; ++i) {
-			if (i == n || sd[k] >> 32 != sd[i] >> 32) {
-				int64_t j = k;
-				while (j < i && !(sd[j] >> 31 & 1)) ++j;
-				const int32_t n1 = (int32_t)(j - k), n2 = (int32_t)(i - k - n1);
-				if (n1 > 0 && n2 > 0 && n1 * n2 <= o->max_ava)
-					for (int64_t i1 = k; i1 < k + n1; ++i1)
-						for (int64_t i2 = k + n1; i2 < i; ++i2)
-							a.push_back((uint64_t)(uint32_t)sd[i1] << 32 | ((uint32_t)sd[i2] << 1 >> 1));
-				k = i;
-			}
-		}
-		std::sort(a.begin(), a.end());
-	});
-	// the window chains of every region, one device batch (map.c:203)
-	std::vector<int64_t> p_off(jobs.size() + 1, 0);
-	for (size_t t = 0; t < jobs.size(); ++t) p_off[t + 1] = p_off[t] + (int64_t)pairs[t].size();
-	std::vector<uint64_t> pa((size_t)p_off[jobs.size()] + 1), pu(pa.size()), pao(pa.size());
-	for (size_t t = 0; t < jobs.size(); ++t) std::copy(pairs[t].begin(), pairs[t].end(), pa.begin() + p_off[t]);
-	pairs.clear();
-	pairs.shrink_to_fit();
-	std::vector<int32_t> pn_u(jobs.size() + 1, 0);
-	st[2] = p_off[jobs.size()];
-	if (!jobs.empty() && nsh_chain_batch((int32_t)jobs.size(), p_off.data(), pa.data(), o->dp.max_intron, o->max_gap,
-			o->bw, o->max_chn_max_skip, o->max_chn_iter, o->min_chn_cnt, o->min_chn_sc, o->chn_coef_log, 1, kmer2, 0,
-			pn_u.data(), pu.data(), pao.data()) < 0)
+		rjobs[t].g_off = (r.vid & 1? m->total : 0) + m->off[r.vid >> 1] + as;
+		rjobs[t].len = (int32_t)std::max<int64_t>(0, ae - as);
+		rjobs[t].q = jb.q;
+	}
+	std::vector<int64_t> qoff(n_q + 1, 0);
+	for (int32_t q = 0; q < n_q; ++q) qoff[q + 1] = qoff[q] + (int64_t)qsk[q].size();
+	std::vector<uint64_t> qall((size_t)qoff[n_q]);
+	for (int32_t q = 0; q < n_q; ++q) std::copy(qsk[q].begin(), qsk[q].end(), qall.begin() + qoff[q]);
+	lap(2);
+	// window k-mers shared with the query, bucket pairs and the window chains of every region (map.c:153-203)
+	std::vector<int64_t> p_off;
+	std::vector<int32_t> pn_u;
+	std::vector<uint64_t> pu, pao;
+	const nsh_chain::Params rcp = { o->dp.max_intron, o->max_gap, o->bw, o->max_chn_max_skip, o->max_chn_iter,
+		o->min_chn_cnt, o->min_chn_sc, o->chn_coef_log, 1, kmer2, 0 };
+	if (!nsh_refine::run(m->d_g, rjobs, qoff, qall, kmer2, m->min_aa_len, o->max_ava, rcp, p_off, pn_u, pu, pao))
 		return 0;
+	st[2] = p_off.empty()? 0 : p_off.back();
 	parallel_for((int64_t)jobs.size(), nth, [&](int64_t t) {   // the best chain becomes the region (map.c:209-258)
 		const RefJob &jb = jobs[t];
 		Reg &r = pool[jb.q][regs[jb.q][jb.r]];
@@ -616,7 +566,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 			r.a[i] = (uint64_t)((int64_t)(a[k + i] >> 32) + jb.as - r.vs) << 32 | (a[k + i] & 0xffffffffULL);
 		r.chn_sc_ungap = chn_sc_ungap(r.a, kmer2);
 	});
-	pa.clear(), pu.clear(), pao.clear();
+	pu.clear(), pao.clear();
 	parallel_for(n_q, nth, [&](int64_t q) {   // map.c:685-698
 		Regs kept;
 		for (int32_t i : regs[q])
@@ -627,6 +577,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 		select_sub(pool[q], regs[q], sel_ratio * sel_ratio, m->kmer * 2, sel_n);
 	});
 
+	lap(3);
 	// ---- align: every candidate of every query through the device stages (map.c:701-739)
 	std::vector<nsh_align_reg_t> cand;
 	std::vector<uint64_t> canch;
@@ -651,6 +602,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 	if (!cand.empty() && !nsh_align_stage(m->n_ctg, m->nt.data(), m->off.data(), m->len.data(), aa_p.data(), qlen,
 			(int32_t)cand.size(), cand.data(), canch.data(), &o->dp, &cig))
 		return 0;
+	lap(4);
 	ns_opt_t so;
 	ns_opt_init(&so);
 	so.go = o->dp.go, so.ge = o->dp.ge, so.fs = o->dp.fs, so.sc = o->dp.mat;
@@ -759,6 +711,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 	*out = o_buf, *out_len = w;
 	*id += id0[n_q];
 	for (int32_t q = 0; q < n_q; ++q) st[4] += n_out[q] > 0, st[5] += n_out[q];
+	lap(5);
 	if (stats) std::memcpy(stats, st, sizeof(st));
 	return 1;
 }

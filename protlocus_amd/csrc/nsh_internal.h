@@ -58,6 +58,7 @@ int64_t nsh_strand_bases(const uint8_t *nt4, const int64_t *ctg_off, const int64
 
 // ---- banded gapped stage of the prescreen search (ps_gapped.hip), device-resident items ----
 #include <hip/hip_runtime.h>
+#include <vector>
 namespace nsh_psg {
 // d_prow/d_plen (both null, or both set): per item, the first row of its PSSM in d_pssm and its length
 void launch(int C, bool full, const int8_t *d_pssm, int32_t L, const int64_t *d_prow, const int32_t *d_plen,
@@ -108,6 +109,22 @@ struct Params {                    // mp_chain's arguments (chain.c:160)
 // go to d_u and their anchors to d_ao; the chain counts to d_nu. Waits for s; 1 on success.
 int run(hipStream_t s, int32_t n_q, const int64_t *d_off, const int64_t *d_cnt, const int64_t *h_cnt,
 		const uint64_t *d_a, int64_t n_a, const Params &p, int32_t *d_nu, uint64_t *d_u, uint64_t *d_ao);
+}
+
+// ---- the refinement step of the map path on the device (refine.hip) ----
+namespace nsh_refine {
+struct Job { int64_t g_off; int32_t len, q; };   // a window: its first base in the oriented genome, its query
+// the oriented genome in HBM: the n_ctg contigs (codes 0..4, contig c at ctg_off[c], total bases)
+// followed by their reverse complements at total + ctg_off[c]; *d_g is hipFree'd by the caller
+int oriented_genome(const uint8_t *nt4, const int64_t *ctg_off, const int64_t *ctg_len, int32_t n_ctg, int64_t total,
+		uint8_t **d_g);
+// mp_refine_reg's sketch, bucket pairs and mp_chain (map.c:153-203) for every job: qsk holds each
+// query's sorted k-mers (hash << 32 | position, mp_sketch_prot with mod_bit 0) at qoff[q]. Job j's
+// pairs start at p_off[j] (p_off has n + 1 entries); its n_u[j] chains (score << 32 | count) are
+// u[p_off[j] ..) and their anchors, chain after chain, ao[p_off[j] ..). 1 on success.
+int run(const uint8_t *d_g, const std::vector<Job> &jobs, const std::vector<int64_t> &qoff,
+		const std::vector<uint64_t> &qsk, int32_t kmer, int32_t min_aa_len, int32_t max_ava, const nsh_chain::Params &cp,
+		std::vector<int64_t> &p_off, std::vector<int32_t> &n_u, std::vector<uint64_t> &u, std::vector<uint64_t> &ao);
 }
 
 #endif

@@ -51,7 +51,7 @@ def map_workload(product, cn, cs, pn, ps, flag=0, batch=None):
     o = MapOpt()
     L.nsh_map_opt_default(C.byref(o))
     o.flag = flag
-    out, rid, stats = [], C.c_int64(0), np.zeros(8, np.int64)
+    out, rid, stats = [], C.c_int64(0), np.zeros(16, np.int64)
     batch = batch or len(ps)
     try:
         for b0 in range(0, len(ps), batch):
@@ -60,7 +60,7 @@ def map_workload(product, cn, cs, pn, ps, flag=0, batch=None):
             qs = (C.c_char_p * len(sub))(*sub)
             ql = np.array([len(x) for x in sub], np.int32)
             buf, n = C.c_void_p(), C.c_int64()
-            st = np.zeros(8, np.int64)
+            st = np.zeros(16, np.int64)
             assert L.nsh_map_batch(m, len(sub), qn, qs, ql.ctypes.data, C.byref(o), C.byref(rid), C.byref(buf),
                                    C.byref(n), st.ctypes.data), product.error()
             out.append(C.string_at(buf, n.value))
