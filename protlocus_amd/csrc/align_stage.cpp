@@ -59,7 +59,9 @@ struct Cand {
 	const char *aa = nullptr;
 	int32_t len = 0;
 	std::vector<uint64_t> a;
-	std::vector<uint8_t> nt;     // oriented bases of [as, ae)
+	const uint8_t *nt = nullptr; // oriented bases of [as, ae) in the host genome view
+	int64_t g_off = 0;           // where they start in the oriented genome (host and HBM alike)
+	const uint8_t *d_g = nullptr; // the oriented genome in HBM
 	int64_t as = 0, ae = 0, vs0 = 0, vs1 = 0;
 	int32_t i0 = 0, ne0 = 0, ae0 = 0, score = 0;
 	int32_t as1 = 0, as1_win = 0, aa_win = 0, nt_len = 0, aa_len = 0;
@@ -134,7 +136,8 @@ static int ext_round(std::vector<Cand*> &cs, std::vector<nsh_ext_req> &req, cons
 static nsh_ext_req ext_req(const Cand *c, int64_t nt_off, int32_t nl, int32_t aa_off, int32_t al, int32_t left, int32_t io)
 {
 	nsh_ext_req q;
-	q.ns = (const char*)c->nt.data() + nt_off, q.nl = nl, q.as = c->aa + aa_off, q.al = al, q.ss = nullptr;
+	q.ns = (const char*)c->nt + nt_off, q.nl = nl, q.as = c->aa + aa_off, q.al = al, q.ss = nullptr;
+	q.d_g = c->d_g, q.g_off = c->g_off + nt_off;   // the extension reads its bases from the genome in HBM
 	q.left = left, q.io = io, q.mode = 0, q.pssm = nullptr;
 	const int32_t w = nsh_host_stripe_width(left? NS_F_EXT_LEFT : NS_F_EXT_RIGHT, al);
 	q.npad = (al + w - 1) / w * w;
@@ -154,7 +157,7 @@ static int dp_round(std::vector<Task> &tasks, const ns_opt_t &o, const nsh_align
 		// residue count while its index steps over bases (i < alen, i += 3), so only the first
 		// ceil(alen / 3) residues of the step are scored; the same bound is kept here.
 		int32_t s = 0;
-		const uint8_t *nseq = c->nt.data() + k.nt_off;
+		const uint8_t *nseq = c->nt + k.nt_off;
 		for (int32_t i = 0, j = 0; i < k.alen; i += 3, ++j) {
 			const uint8_t b0 = nseq[i], b1 = nseq[i + 1], b2 = nseq[i + 2];
 			const int32_t ta = b0 > 3 || b1 > 3 || b2 > 3? ns_tab_aa20['X'] : ns_tab_codon[b0 << 4 | b1 << 2 | b2];
@@ -171,7 +174,7 @@ static int dp_round(std::vector<Task> &tasks, const ns_opt_t &o, const nsh_align
 	for (int32_t k = 0; k < n; ++k) {
 		const Task &t = tasks[idx[k]];
 		ns_off[k] = (int64_t)ns.size(), as_off[k] = (int64_t)as.size();
-		ns.insert(ns.end(), (const char*)t.c->nt.data() + t.nt_off, (const char*)t.c->nt.data() + t.nt_off + t.nlen);
+		ns.insert(ns.end(), (const char*)t.c->nt + t.nt_off, (const char*)t.c->nt + t.nt_off + t.nlen);
 		as.insert(as.end(), t.c->aa + t.aa_off, t.c->aa + t.aa_off + t.alen);
 		nl[k] = t.nlen, al[k] = t.alen;
 	}
@@ -198,12 +201,40 @@ extern "C" int nsh_align_stage(int32_t n_ctg, const uint8_t *nt4, const int64_t 
 		const char *const *aa, const int32_t *qlen, int32_t n_reg, nsh_align_reg_t *reg, const uint64_t *anchors,
 		const nsh_align_opt_t *ao, uint32_t **cigar_out)
 {
-	if (!nt4 || !ctg_off || !ctg_len || !aa || !qlen || (n_reg > 0 && (!reg || !anchors)) || !ao || !cigar_out || n_reg < 0) {
+	if (!nt4 || !ctg_off || !ctg_len || n_ctg <= 0 || !cigar_out) {
+		nsh_set_error("nsh_align_stage: bad arguments");
+		return 0;
+	}
+	// the oriented genome of this call: the caller's contigs packed one after another, then their
+	// reverse complements, on the host and in HBM
+	std::vector<int64_t> off(n_ctg);
+	int64_t total = 0;
+	for (int32_t c = 0; c < n_ctg; ++c) off[c] = total, total += ctg_len[c];
+	std::vector<uint8_t> hg((size_t)(2 * total + 1));
+	for (int32_t c = 0; c < n_ctg; ++c) {
+		std::memcpy(hg.data() + off[c], nt4 + ctg_off[c], (size_t)ctg_len[c]);
+		nsh_strand_bases(nt4, ctg_off, ctg_len, (uint32_t)c << 1 | 1, 0, ctg_len[c], hg.data() + total + off[c]);
+	}
+	uint8_t *d_g = nullptr;
+	if (!nsh_refine::oriented_genome(hg.data(), off.data(), ctg_len, n_ctg, total, &d_g)) return 0;
+	nsh_genome_view g;
+	g.n_ctg = n_ctg, g.off = off.data(), g.len = ctg_len, g.total = total, g.h_g = hg.data(), g.d_g = d_g;
+	const int ok = nsh_align_stage_view(g, aa, qlen, n_reg, reg, anchors, ao, cigar_out);
+	(void)hipFree(d_g);
+	return ok;
+}
+
+int nsh_align_stage_view(const nsh_genome_view &g, const char *const *aa, const int32_t *qlen, int32_t n_reg,
+		nsh_align_reg_t *reg, const uint64_t *anchors, const nsh_align_opt_t *ao, uint32_t **cigar_out)
+{
+	if (!g.h_g || !g.d_g || !aa || !qlen || (n_reg > 0 && (!reg || !anchors)) || !ao || !cigar_out || n_reg < 0) {
 		nsh_set_error("nsh_align_stage: bad arguments");
 		return 0;
 	}
 	*cigar_out = nullptr;
 	if (!nsh_ensure_device()) return 0;
+	const int32_t n_ctg = g.n_ctg;
+	const int64_t *ctg_len = g.len;
 	// ns_opt_t of the DPs (mp_map2ns_opt, align.c:59-71)
 	ns_opt_t o;
 	ns_opt_init(&o);
@@ -236,11 +267,11 @@ extern "C" int nsh_align_stage(int32_t n_ctg, const uint8_t *nt4, const int64_t 
 		const int64_t clen = ctg_len[r->vid >> 1];
 		c.as = r->vs > extl? r->vs - extl : 0;
 		c.ae = r->ve + extr < clen? r->ve + extr : clen;
-		c.nt.resize((size_t)std::max<int64_t>(1, c.ae - c.as));
-		if (nsh_strand_bases(nt4, ctg_off, ctg_len, r->vid, c.as, c.ae, c.nt.data()) != c.ae - c.as) {
+		if (c.as >= clen || c.ae <= c.as) {
 			nsh_set_error("nsh_align_stage: region %d outside its contig", k);
 			return 0;
 		}
+		c.g_off = g.at(r->vid, c.as), c.nt = g.h_g + c.g_off, c.d_g = g.d_g;
 		c.vs0 = r->vs;
 		c.vs1 = c.vs0 + anchor_x(c.a[i]) + 1;
 		c.as1 = anchor_q(c.a[i]) + 1;

@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(256) ext_prep_kernel(const PrepProb *pp, const
 	for (int k = threadIdx.x; k < (int)sizeof(PrepConst); k += blockDim.x) ((uint8_t*)&pc)[k] = ((const uint8_t*)pcg)[k];
 	__syncthreads();
 	PrepView v;
-	v.ns = raw_ns + P.ns_off, v.ss = P.has_ss? raw_ss + P.ns_off : nullptr;
+	v.ns = (P.from_genome? pc.genome : raw_ns) + P.ns_off, v.ss = P.has_ss? raw_ss + P.ns_off : nullptr;
 	v.nl = P.nl, v.left = P.left, v.has_ss = P.has_ss, v.max_spsc = (P.io + 1) / 2 - 1;
 	v.nt4 = pc.nt4, v.sp = pc.sp, v.sp_null = pc.sp_null;
 	const int X = pc.aa20['X'];
@@ -211,6 +211,7 @@ struct DBuf {
 
 struct nsh_ext_plan_s {
 	int32_t n = 0;
+	bool from_genome = false;              // some problems read their bases from a genome in HBM
 	hipStream_t stream = nullptr;
 	hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
 	hipStream_t gs[kGroups] = {};          // one stream per column-count group
@@ -351,7 +352,8 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 	if (opt->ge < 0 || opt->go < 0 || (int64_t)opt->ge * CMAX * 64 > 32767) {
 		nsh_set_error("gap penalties go=%d ge=%d out of the supported range", opt->go, opt->ge); return nullptr;
 	}
-	int64_t l_ns = 0, l_as = 0, l_pssm = 0;
+	int64_t l_ns = 0, l_raw = 0, l_as = 0, l_pssm = 0;
+	const uint8_t *genome = nullptr;
 	int32_t max_nl = 2, any_ss = 0, n_grp[kGroups] = {};
 	std::vector<ExtProb> prob(n);
 	std::vector<PrepProb> prep(n);
@@ -372,7 +374,11 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		e.rec_off = l_ns, e.aa_off = l_as, e.pssm_off = -1;
 		PrepProb &r = prep[k];
 		r = PrepProb{};
-		r.ns_off = l_ns, r.as_off = l_as, r.rec_off = l_ns, r.aa_off = l_as;
+		if (q.g_off >= 0) {   // bases read from the oriented genome in HBM
+			if (!q.d_g || q.ss || (genome && genome != q.d_g)) { nsh_set_error("request %d: bad genome window", k); return nullptr; }
+			genome = q.d_g, r.from_genome = 1, r.ns_off = q.g_off;
+		} else r.ns_off = l_raw, l_raw += q.nl + 1;
+		r.as_off = l_as, r.rec_off = l_ns, r.aa_off = l_as;
 		r.nl = q.nl, r.al = q.al, r.left = q.left, r.io = q.io, r.has_ss = q.ss != nullptr;
 		if (q.pssm) e.pssm_off = l_pssm, l_pssm += (int64_t)q.al * asize;
 		l_ns += q.nl + 1, l_as += q.al + 1;
@@ -383,6 +389,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 	nsh_ext_plan_t *p = shell_acquire();
 	if (!p) return nullptr;
 	p->n = n;
+	p->from_genome = genome != nullptr;
 	p->al.resize(n);
 	for (int32_t k = 0; k < n; ++k) {
 		p->max_nl = std::max(p->max_nl, req[k].nl), p->max_al = std::max(p->max_al, req[k].al);
@@ -414,8 +421,8 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 	p->out_h.assign((size_t)4 * n, 0);
 	// inputs staged in pinned memory: ns | as | ss | pssm | pen | prob | prep | pc | mat | wide
 	const int32_t n_pen = max_nl + 2;
-	const size_t s_ns = 0, s_as = (s_ns + l_ns + 255) & ~(size_t)255, s_ss = (s_as + l_as + 255) & ~(size_t)255;
-	const size_t s_pssm = (s_ss + (any_ss? l_ns : 0) + 255) & ~(size_t)255;
+	const size_t s_ns = 0, s_as = (s_ns + l_raw + 255) & ~(size_t)255, s_ss = (s_as + l_as + 255) & ~(size_t)255;
+	const size_t s_pssm = (s_ss + (any_ss? l_raw : 0) + 255) & ~(size_t)255;
 	const size_t s_pen = (s_pssm + l_pssm + 255) & ~(size_t)255;
 	const size_t s_prob = (s_pen + (size_t)n_pen * 4 + 255) & ~(size_t)255;
 	const size_t s_prep = (s_prob + (size_t)n_narrow * sizeof(ExtProb) + 255) & ~(size_t)255;
@@ -434,9 +441,9 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 	hs = (uint8_t*)p->h_stage;
 	for (int32_t k = 0; k < n; ++k) {
 		const nsh_ext_req &q = req[k];
-		if (q.nl) std::memcpy(hs + s_ns + prep[k].ns_off, q.ns, q.nl);
+		if (q.nl && !prep[k].from_genome) std::memcpy(hs + s_ns + prep[k].ns_off, q.ns, q.nl);
 		std::memcpy(hs + s_as + prep[k].as_off, q.as, q.al);
-		if (any_ss) {
+		if (any_ss && !prep[k].from_genome) {
 			if (q.ss) std::memcpy(hs + s_ss + prep[k].ns_off, q.ss, q.nl);
 			else std::memset(hs + s_ss + prep[k].ns_off, 0xff, q.nl);
 		}
@@ -449,13 +456,15 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		std::memcpy(pc.nt4, opt->nt4, 256), std::memcpy(pc.aa20, opt->aa20, 256), std::memcpy(pc.codon, opt->codon, 64);
 		for (int t = 0; t < 6; ++t) pc.sp[t] = opt->sp[t];
 		pc.sp_null = opt->sp_null_bonus;
+		pc.genome = genome;
 		if (n_narrow) std::memcpy(hs + s_prob, p->prob_h.data(), (size_t)n_narrow * sizeof(ExtProb));
 		std::memcpy(hs + s_prep, prep.data(), (size_t)n * sizeof(PrepProb));
 		std::memcpy(hs + s_mat, opt->sc, (size_t)asize * asize);
 		if (!p->wide_h.empty()) std::memcpy(hs + s_wide, p->wide_h.data(), p->wide_h.size() * sizeof(nsh_glob::GlobProb));
 	}
-	HIPOK(p->b_ns.ensure(l_ns)); HIPOK(p->b_as.ensure(l_as)); HIPOK(p->b_aas.ensure(l_as)); HIPOK(p->b_rec.ensure(l_ns));
-	if (any_ss) HIPOK(p->b_ss.ensure(l_ns));
+	HIPOK(p->b_ns.ensure(std::max<int64_t>(l_raw, 1))); HIPOK(p->b_as.ensure(l_as)); HIPOK(p->b_aas.ensure(l_as));
+	HIPOK(p->b_rec.ensure(l_ns));
+	if (any_ss) HIPOK(p->b_ss.ensure(std::max<int64_t>(l_raw, 1)));
 	HIPOK(p->b_mat.ensure((size_t)asize * asize));
 	if (l_pssm) HIPOK(p->b_pssm.ensure(l_pssm));
 	HIPOK(p->b_pen.ensure(n_pen)); HIPOK(p->b_counter.ensure(kGroups)); HIPOK(p->b_out.ensure((size_t)4 * n));
@@ -469,7 +478,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		auto cp = [&](void *dst, size_t off, size_t bytes) {
 			return bytes == 0 || hipMemcpyAsync(dst, hs + off, bytes, hipMemcpyHostToDevice, p->stream) == hipSuccess;
 		};
-		const bool ok = cp(p->d_ns, s_ns, l_ns) && cp(p->d_as, s_as, l_as) && (!any_ss || cp(p->d_ss, s_ss, l_ns)) &&
+		const bool ok = cp(p->d_ns, s_ns, l_raw) && cp(p->d_as, s_as, l_as) && (!any_ss || cp(p->d_ss, s_ss, l_raw)) &&
 			(!l_pssm || cp(p->d_pssm, s_pssm, l_pssm)) && cp(p->d_pen, s_pen, (size_t)n_pen * 4) &&
 			cp(p->d_prob, s_prob, (size_t)n_narrow * sizeof(ExtProb)) && cp(p->d_prep, s_prep, (size_t)n * sizeof(PrepProb)) &&
 			cp(p->d_pc, s_pc, sizeof(PrepConst)) && cp(p->d_mat, s_mat, (size_t)asize * asize) &&
@@ -632,6 +641,7 @@ fail:
 extern "C" const float *nsh_ext_plan_features(nsh_ext_plan_t *p, float *host_out)
 {
 	if (!p) return nullptr;
+	if (p->from_genome) { nsh_set_error("nsh_ext_plan_features: the plan reads its bases from a genome"); return nullptr; }
 	if (p->max_al > 4096 || p->max_nl > 24000) {
 		nsh_set_error("nsh_ext_plan_features: candidates limited to al <= 4096, nl <= 24000 (plan has %d, %d)", p->max_al, p->max_nl);
 		return nullptr;

@@ -383,12 +383,13 @@ struct nsh_mapper_s {
 	int32_t n_ctg = 0, kmer = 0, bbit = 0, min_aa_len = 0;
 	std::vector<std::string> name;
 	std::vector<const char*> name_p;
-	std::vector<uint8_t> nt;                 // base codes 0..4, contigs one after another
+	std::vector<uint8_t> nt;                 // base codes 0..4: the contigs one after another, then their reverse complements
 	std::vector<int64_t> off, len;
 	std::vector<uint32_t> bo;                // first block of each strand, 2 n_ctg + 1 entries
 	int64_t total = 0;                       // bases
 	nsh_index_t *ix = nullptr;
 	uint8_t *d_g = nullptr;                  // HBM: the contigs, then their reverse complements
+	nsh_genome_view view;                    // nt (both orientations) and d_g
 };
 
 extern "C" void nsh_map_opt_default(nsh_map_opt_t *o)
@@ -421,11 +422,17 @@ extern "C" nsh_mapper_t *nsh_mapper_init(int32_t n_ctg, const char *const *ctg_n
 		tot += ctg_len[c];
 	}
 	for (auto &s : m->name) m->name_p.push_back(s.c_str());
-	m->nt.assign(nt4, nt4 + tot);
+	m->nt.resize((size_t)(2 * tot));
+	std::memcpy(m->nt.data(), nt4, (size_t)tot);
+	parallel_for(n_ctg, 16, [&](int64_t c) {
+		nsh_strand_bases(nt4, m->off.data(), m->len.data(), (uint32_t)c << 1 | 1, 0, m->len[c], m->nt.data() + tot + m->off[c]);
+	});
 	m->total = tot;
 	if (!nsh_refine::oriented_genome(m->nt.data(), m->off.data(), m->len.data(), n_ctg, tot, &m->d_g)) { delete m; return nullptr; }
 	m->ix = nsh_index_build(n_ctg, m->nt.data(), m->off.data(), m->len.data(), min_aa_len, kmer, mod_bit, bbit);
 	if (!m->ix) { nsh_mapper_destroy(m); return nullptr; }
+	m->view.n_ctg = n_ctg, m->view.off = m->off.data(), m->view.len = m->len.data(), m->view.total = tot;
+	m->view.h_g = m->nt.data(), m->view.d_g = m->d_g;
 	m->bo.resize(2 * (size_t)n_ctg + 1);
 	if (!nsh_index_copy(m->ix, nullptr, nullptr, m->bo.data())) { nsh_mapper_destroy(m); return nullptr; }
 	return m;
@@ -599,8 +606,8 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 	st[3] = (int64_t)cand.size();
 	uint32_t *cig = nullptr;
 	std::vector<const char*> aa_p(seq, seq + n_q);
-	if (!cand.empty() && !nsh_align_stage(m->n_ctg, m->nt.data(), m->off.data(), m->len.data(), aa_p.data(), qlen,
-			(int32_t)cand.size(), cand.data(), canch.data(), &o->dp, &cig))
+	if (!cand.empty() && !nsh_align_stage_view(m->view, aa_p.data(), qlen, (int32_t)cand.size(), cand.data(), canch.data(),
+			&o->dp, &cig))
 		return 0;
 	lap(4);
 	ns_opt_t so;
@@ -611,8 +618,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 	parallel_for((int64_t)cand.size(), nth, [&](int64_t k) {   // mp_align_finish: statistics, rank score
 		const nsh_align_reg_t &c = cand[k];
 		if (!c.status) return;
-		std::vector<uint8_t> win((size_t)std::max<int64_t>(1, c.win_ae - c.win_as));
-		nsh_strand_bases(m->nt.data(), m->off.data(), m->len.data(), c.vid, c.win_as, c.win_ae, win.data());
+		const uint8_t *win = m->nt.data() + m->view.at(c.vid, c.win_as);
 		nsh_aln_t al = {};
 		al.qname = qname[c.query], al.aa = seq[c.query], al.qlen = qlen[c.query], al.qs = c.qs, al.qe = c.qe;
 		al.ctg_name = m->name_p[c.vid >> 1], al.ctg_len = m->len[c.vid >> 1], al.rev = c.vid & 1;
@@ -620,10 +626,10 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 		al.cigar = cig + c.cigar_off, al.n_cigar = c.n_cigar;
 		Out &r = res[k];
 		r.feat.resize(64);
-		int32_t nf = nsh_aln_stats(&al, win.data(), c.win_as, c.win_ae, &so, nullptr, &r.st, r.feat.data(), 64);
+		int32_t nf = nsh_aln_stats(&al, win, c.win_as, c.win_ae, &so, nullptr, &r.st, r.feat.data(), 64);
 		if (nf < 0) {
 			r.feat.resize(8192);
-			nf = nsh_aln_stats(&al, win.data(), c.win_as, c.win_ae, &so, nullptr, &r.st, r.feat.data(), 8192);
+			nf = nsh_aln_stats(&al, win, c.win_as, c.win_ae, &so, nullptr, &r.st, r.feat.data(), 8192);
 			if (nf < 0) { fail = true; return; }
 		}
 		r.feat.resize(std::max(nf, 0));
@@ -680,8 +686,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 			const int32_t j = emit[q][e];
 			const Reg &r = pool[q][regs[q][j]];
 			const nsh_align_reg_t &c = cand[r.slot];
-			std::vector<uint8_t> win((size_t)std::max<int64_t>(1, c.win_ae - c.win_as));
-			nsh_strand_bases(m->nt.data(), m->off.data(), m->len.data(), c.vid, c.win_as, c.win_ae, win.data());
+			const uint8_t *win = m->nt.data() + m->view.at(c.vid, c.win_as);
 			nsh_aln_t al = {};
 			al.qname = qname[q], al.aa = seq[q], al.qlen = qlen[q], al.qs = r.qs, al.qe = r.qe;
 			al.ctg_name = m->name_p[r.vid >> 1], al.ctg_len = m->len[r.vid >> 1], al.rev = r.vid & 1;
@@ -690,7 +695,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 			al.cigar = cig + c.cigar_off, al.n_cigar = c.n_cigar;
 			const Out &x = res[r.slot];
 			for (;;) {
-				const int64_t n = nsh_write_output(buf.data(), (int64_t)buf.size(), &al, &x.st, x.feat.data(), win.data(),
+				const int64_t n = nsh_write_output(buf.data(), (int64_t)buf.size(), &al, &x.st, x.feat.data(), win,
 						c.win_as, (int32_t)o->flag, 0, o->gff_prefix, *id + id0[q] + (int64_t)e + 1, o->gff_delim, j + 1);
 				if (n < 0) { fail = true; return; }
 				if (n < (int64_t)buf.size()) { s.append(buf.data(), (size_t)n); break; }

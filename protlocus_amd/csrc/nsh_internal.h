@@ -28,6 +28,8 @@ struct nsh_ext_req {
 	int32_t npad;     // al rounded up to the reference CPU's stripe width
 	int32_t mode = 0; // 1: score-only global DP without extension flags (nasw-sse.c:459, 559)
 	const int8_t *pssm = nullptr; // al x asize PSSM rows in query order (ns_pssm_sc, nasw-sse.c:217-221); null: the matrix
+	const uint8_t *d_g = nullptr;  // with g_off >= 0: the bases are d_g[g_off ..) in HBM (ns is not read; no ss)
+	int64_t g_off = -1;
 };
 nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_opt_t *opt);
 int nsh_ext_plan_query(nsh_ext_plan_t *p);   // 1 when the last run has completed
@@ -110,6 +112,20 @@ struct Params {                    // mp_chain's arguments (chain.c:160)
 int run(hipStream_t s, int32_t n_q, const int64_t *d_off, const int64_t *d_cnt, const int64_t *h_cnt,
 		const uint64_t *d_a, int64_t n_a, const Params &p, int32_t *d_nu, uint64_t *d_u, uint64_t *d_ao);
 }
+
+// ---- a genome in both orientations, on the host and in HBM (the map path's windows) ----
+struct nsh_genome_view {
+	int32_t n_ctg = 0;
+	const int64_t *off = nullptr, *len = nullptr;   // contig c: its bases at off[c] (packed), len[c] of them
+	int64_t total = 0;                              // bases of all contigs
+	const uint8_t *h_g = nullptr;   // host: the contigs (codes 0..4), then their reverse complements at total + off[c]
+	const uint8_t *d_g = nullptr;   // the same in HBM
+	// where base pos of strand vid (contig << 1 | reverse, oriented coordinates) is
+	int64_t at(uint32_t vid, int64_t pos) const { return (vid & 1? total : 0) + off[vid >> 1] + pos; }
+};
+// nsh_align_stage over a genome view (align_stage.cpp)
+int nsh_align_stage_view(const nsh_genome_view &g, const char *const *aa, const int32_t *qlen, int32_t n_reg,
+		nsh_align_reg_t *reg, const uint64_t *anchors, const nsh_align_opt_t *opt, uint32_t **cigar);
 
 // ---- the refinement step of the map path on the device (refine.hip) ----
 namespace nsh_refine {

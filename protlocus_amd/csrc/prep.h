@@ -6,13 +6,14 @@
 
 namespace nsh_prep {
 struct PrepProb {
-	int64_t ns_off, as_off, rec_off, aa_off;
+	int64_t ns_off, as_off, rec_off, aa_off;   // ns_off: into the raw bases, or into PrepConst::genome when from_genome
 	int32_t nl, al, left, io;
-	int32_t has_ss, pad_[3];
+	int32_t has_ss, from_genome, pad_[2];
 };
 struct PrepConst {
 	uint8_t nt4[256], aa20[256], codon[64];
 	int32_t sp[6], sp_null;
+	const uint8_t *genome;                     // an oriented genome in HBM that problems may read their bases from
 };
 
 // one work-group per problem; rec[i] = nas[i] | donor[i+1] << 8 | acceptor[i] << 16 (bytes)
