@@ -166,26 +166,30 @@ static int dp_round(std::vector<Task> &tasks, const ns_opt_t &o, const nsh_align
 		k.score = s;
 		k.cig.assign(1, (uint32_t)k.alen << 4 | NS_CIGAR_M);
 	}
+	StepClock clk;
+	clk.lap("  ungapped steps", (int64_t)(tasks.size() - idx.size()));
 	if (idx.empty()) return 1;
 	const int32_t n = (int32_t)idx.size();
-	std::vector<int64_t> ns_off(n), as_off(n);
+	// the bases stay in HBM (offsets into the oriented genome); only the protein pieces are packed
+	std::vector<int64_t> ns_off(n, 0), g_off(n), as_off(n);
 	std::vector<int32_t> nl(n), al(n), score(n), ncig(n);
-	std::vector<char> ns, as;
+	std::vector<char> as;
 	for (int32_t k = 0; k < n; ++k) {
 		const Task &t = tasks[idx[k]];
-		ns_off[k] = (int64_t)ns.size(), as_off[k] = (int64_t)as.size();
-		ns.insert(ns.end(), (const char*)t.c->nt + t.nt_off, (const char*)t.c->nt + t.nt_off + t.nlen);
+		g_off[k] = t.c->g_off + t.nt_off, as_off[k] = (int64_t)as.size();
 		as.insert(as.end(), t.c->aa + t.aa_off, t.c->aa + t.aa_off + t.alen);
 		nl[k] = t.nlen, al[k] = t.alen;
 	}
-	ns.push_back(0), as.push_back(0);
+	as.push_back(0);
+	clk.lap("  DP inputs", n);
 	uint32_t *cig = nullptr;
 	ns_opt_t oc = o;
 	oc.flag = NS_F_CIGAR;
 	// mode 2: ns_global_gs_auto's int16 / int32 choice, as mp_align_seq calls it
-	if (!nsh_global_batch(n, ns.data(), ns_off.data(), nl.data(), as.data(), as_off.data(), al.data(), nullptr, &oc, 2,
-			score.data(), ncig.data(), &cig))
+	if (!nsh_global_batch_g(n, nullptr, ns_off.data(), nl.data(), as.data(), as_off.data(), al.data(), nullptr, &oc, 2,
+			tasks[idx[0]].c->d_g, g_off.data(), score.data(), ncig.data(), &cig))
 		return 0;
+	clk.lap("  DP batch", n);
 	int64_t off = 0;
 	for (int32_t k = 0; k < n; ++k) {
 		Task &t = tasks[idx[k]];

@@ -168,6 +168,236 @@ struct nsh_glob_req {
 
 namespace {
 
+// one problem as a batch run sees it (inputs borrowed for the run)
+struct GlobIn {
+	const uint8_t *ns, *as, *ss;
+	const int8_t *pssm;
+	int64_t g_off;             // >= 0: the bases are in HBM at genome + g_off (ns unused)
+	int32_t nl, al, is32, trace, left, io;
+};
+// what every problem of a run shares (same_terms)
+struct GlobTerms {
+	int32_t go, ge, fs, sp_null, sp[6];
+	int8_t mat[484];
+	uint8_t nt4[256], aa20[256], codon[64];
+	const uint8_t *genome;     // the oriented genome in HBM that problems with g_off >= 0 read
+};
+
+GlobTerms terms_of(const ns_opt_t &o, const uint8_t *genome)
+{
+	GlobTerms t;
+	t.go = o.go, t.ge = o.ge, t.fs = o.fs, t.sp_null = o.sp_null_bonus;
+	for (int k = 0; k < 6; ++k) t.sp[k] = o.sp[k];
+	std::memcpy(t.mat, o.sc, 484), std::memcpy(t.nt4, o.nt4, 256), std::memcpy(t.aa20, o.aa20, 256);
+	std::memcpy(t.codon, o.codon, 64);
+	t.genome = genome;
+	return t;
+}
+
+int kernel_variant(const GlobIn &q) { return q.is32? (q.trace? 1 : 2) : 0; }   // trace16, trace32, score32
+
+// the streams, events and grow-only pools of one thread that runs batches
+struct RunCtx {
+	hipStream_t st[4] = {};
+	hipEvent_t ev_prep = nullptr, ev_b[4] = {};
+	DevPool d_in, d_rec, d_aas, d_tr, d_tmp, d_packed, d_meta;
+	HostPool h_in, h_meta, h_packed;
+	int init()
+	{
+		if (st[0]) return 1;
+		for (int k = 0; k < 4; ++k)
+			if (hipStreamCreateWithFlags(&st[k], hipStreamNonBlocking) != hipSuccess ||
+					hipEventCreateWithFlags(&ev_b[k], hipEventDisableTiming) != hipSuccess) {
+				nsh_set_error("global DP engine: stream set-up failed");
+				return 0;
+			}
+		(void)hipEventCreateWithFlags(&ev_prep, hipEventDisableTiming);
+		return 1;
+	}
+};
+
+// trace bytes per batch part: NSH_TRACE_BUDGET_MB (read per batch), 16 GiB by default
+size_t trace_budget()
+{
+	const char *s = std::getenv("NSH_TRACE_BUDGET_MB");
+	return (size_t)(s? std::max(1L, std::atol(s)) : 16384L) << 20;
+}
+
+// One batch part on the device: prep, the DP in buckets of equal shape on four streams, the
+// backtrack; score[k] and, for problems with a trace, their CIGAR: cig_n[k] ops at cig_off[k] of
+// the packed array pk (valid until the context's next run). 1 on success.
+int run_list(RunCtx &c, const GlobIn *P, int32_t n, const GlobTerms &T, int32_t *score, int64_t *cig_off,
+		int32_t *cig_n, const uint32_t **pk_out)
+{
+	const auto t0 = std::chrono::steady_clock::now();
+	// layout of the input block: ns | as | ss | pssm | PrepProb | GlobProb | BtProb | PrepConst | mat
+	int64_t l_raw = 0, l_rec = 0, l_as = 0, l_pssm = 0, l_tr = 0, l_tmp = 0;
+	int any_ss = 0;
+	std::vector<nsh_prep::PrepProb> pp(n);
+	std::vector<GlobProb> gp(n);
+	std::vector<int32_t> order(n);
+	for (int32_t k = 0; k < n; ++k) {
+		const GlobIn &q = P[k];
+		nsh_prep::PrepProb &p = pp[k];
+		p = {};
+		if (q.g_off >= 0) p.from_genome = 1, p.ns_off = q.g_off;
+		else p.ns_off = l_raw, l_raw += q.nl + 1;
+		p.as_off = l_as, p.rec_off = l_rec, p.aa_off = l_as;
+		// the strand only matters to int32 score-only requests with an extension flag (nasw-sse.c:246-247)
+		p.nl = q.nl, p.al = q.al, p.left = q.left, p.io = q.io, p.has_ss = q.ss != nullptr;
+		GlobProb &g = gp[k];
+		g = {};
+		g.rec_off = l_rec, g.aa_off = l_as, g.pssm_off = q.pssm? l_pssm : -1;
+		g.tr_off = q.trace? l_tr : 0;
+		g.nl = q.nl, g.al = q.al, g.io = q.io, g.out = k, g.left = q.left;
+		shape(q.al, g.C, g.NW);
+		const int vs = q.is32? 4 : 8;
+		g.slen = (q.al + vs - 1) / vs, g.npad = q.al, g.mode = 0;
+		l_rec += q.nl + 1, l_as += q.al + 1;
+		if (q.pssm) l_pssm += (int64_t)q.al * 22;
+		if (q.trace) l_tr += (int64_t)q.nl * q.al, l_tmp += (int64_t)q.nl + q.al + 8;
+		any_ss |= q.ss != nullptr;
+		order[k] = k;
+	}
+	// buckets of equal kernel variant and shape
+	std::sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+		const int vx = kernel_variant(P[x]), vy = kernel_variant(P[y]);
+		if (vx != vy) return vx < vy;
+		if (gp[x].NW != gp[y].NW) return gp[x].NW > gp[y].NW;
+		if (gp[x].C != gp[y].C) return gp[x].C > gp[y].C;
+		return x < y; });
+	std::vector<GlobProb> gsorted(n);
+	for (int32_t k = 0; k < n; ++k) gsorted[k] = gp[order[k]];
+	std::vector<BtProb> bt;
+	std::vector<int32_t> bt_req;   // problem index of each backtrack slot
+	{
+		int64_t tmp = 0;
+		for (int32_t k = 0; k < n; ++k) {
+			if (!P[k].trace) continue;
+			bt.push_back({ gp[k].tr_off, tmp, P[k].nl, P[k].al });
+			bt_req.push_back(k);
+			tmp += (int64_t)P[k].nl + P[k].al + 8;
+		}
+	}
+	const size_t o_ns = 0, o_as = al256(o_ns + l_raw), o_ss = al256(o_as + l_as), o_pssm = al256(o_ss + (any_ss? l_raw : 0));
+	const size_t o_pp = al256(o_pssm + l_pssm), o_gp = al256(o_pp + (size_t)n * sizeof(nsh_prep::PrepProb));
+	const size_t o_bt = al256(o_gp + (size_t)n * sizeof(GlobProb)), o_pc = al256(o_bt + bt.size() * sizeof(BtProb));
+	const size_t o_mat = al256(o_pc + sizeof(nsh_prep::PrepConst)), o_pen = al256(o_mat + 484);
+	const size_t in_bytes = al256(o_pen + 16);
+	int ok = 0;
+	uint8_t *h = nullptr, *d = nullptr;
+	int32_t *d_score = nullptr;
+	int64_t *bmeta = nullptr, *d_btmeta = nullptr;
+	unsigned long long total = 0, *d_total = nullptr;
+	// device result block: scores (n ints) | CIGAR total | (offset, count) per backtracked problem
+	const size_t meta_bytes = al256((size_t)n * 4) + 8 + bt.size() * 16;
+	*pk_out = nullptr;
+	HIPOK(c.h_in.ensure(in_bytes));
+	HIPOK(c.d_in.ensure(in_bytes));
+	HIPOK(c.d_rec.ensure((size_t)l_rec * 4 + 256));
+	HIPOK(c.d_aas.ensure((size_t)l_as + 256));
+	if (l_tr) HIPOK(c.d_tr.ensure((size_t)l_tr * 2 + 256));
+	if (l_tmp) { HIPOK(c.d_tmp.ensure((size_t)l_tmp * 4 + 256)); HIPOK(c.d_packed.ensure((size_t)l_tmp * 4 + 256)); }
+	HIPOK(c.d_meta.ensure(meta_bytes + 256));
+	HIPOK(c.h_meta.ensure(meta_bytes + 256));
+	h = (uint8_t*)c.h_in.p, d = (uint8_t*)c.d_in.p;
+	// pack the inputs
+	{
+		nsh_prep::PrepConst pc;
+		std::memcpy(pc.nt4, T.nt4, 256), std::memcpy(pc.aa20, T.aa20, 256), std::memcpy(pc.codon, T.codon, 64);
+		for (int t = 0; t < 6; ++t) pc.sp[t] = T.sp[t];
+		pc.sp_null = T.sp_null;
+		pc.genome = T.genome;
+		for (int32_t k = 0; k < n; ++k) {
+			const GlobIn &q = P[k];
+			if (q.nl && !pp[k].from_genome) std::memcpy(h + o_ns + pp[k].ns_off, q.ns, q.nl);
+			std::memcpy(h + o_as + pp[k].as_off, q.as, q.al);
+			if (any_ss && !pp[k].from_genome) {
+				if (q.ss) std::memcpy(h + o_ss + pp[k].ns_off, q.ss, q.nl);
+				else std::memset(h + o_ss + pp[k].ns_off, 0xff, q.nl);
+			}
+			if (q.pssm) std::memcpy(h + o_pssm + gp[k].pssm_off, q.pssm, (size_t)q.al * 22);
+		}
+		std::memcpy(h + o_pp, pp.data(), (size_t)n * sizeof(nsh_prep::PrepProb));
+		std::memcpy(h + o_gp, gsorted.data(), (size_t)n * sizeof(GlobProb));
+		if (!bt.empty()) std::memcpy(h + o_bt, bt.data(), bt.size() * sizeof(BtProb));
+		std::memcpy(h + o_pc, &pc, sizeof(pc));
+		std::memcpy(h + o_mat, T.mat, 484);
+	}
+	{
+		const auto t1 = std::chrono::steady_clock::now();
+		HIPOK(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, c.st[0]));
+		nsh_add_times(std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count(), 0, 0);
+	}
+	d_score = c.d_meta.as<int32_t>();
+	nsh_prep::launch((const nsh_prep::PrepProb*)(d + o_pp), n, d + o_ns, any_ss? d + o_ss : nullptr, d + o_as,
+			c.d_rec.as<uint32_t>(), c.d_aas.as<uint8_t>(), (const nsh_prep::PrepConst*)(d + o_pc), c.st[0]);
+	HIPOK(hipGetLastError());
+	HIPOK(hipEventRecord(c.ev_prep, c.st[0]));
+	{
+		GlobArgs ga = {};
+		ga.rec = c.d_rec.as<uint32_t>(), ga.aas = c.d_aas.as<uint8_t>(), ga.pssm = (const int8_t*)(d + o_pssm);
+		ga.mat = (const int8_t*)(d + o_mat), ga.asize = 22;
+		ga.go = T.go, ga.ge = T.ge, ga.fs = T.fs;
+		ga.trace = l_tr? c.d_tr.as<uint16_t>() : nullptr, ga.score = d_score;
+		ga.pen = (const int32_t*)(d + o_pen), ga.pen_n = 1;
+		const GlobProb *d_gp = (const GlobProb*)(d + o_gp);
+		int nb = 0;
+		for (int32_t k = 0; k < n;) {
+			int32_t e = k;
+			const int v = kernel_variant(P[order[k]]);
+			while (e < n && kernel_variant(P[order[e]]) == v && gsorted[e].NW == gsorted[k].NW && gsorted[e].C == gsorted[k].C) ++e;
+			hipStream_t s = c.st[nb & 3];
+			HIPOK(hipStreamWaitEvent(s, c.ev_prep, 0));
+			ga.prob = d_gp + k;
+			if (v == 0) launch_trace16(ga, e - k, gsorted[k].NW, gsorted[k].C, s);
+			else if (v == 1) launch_trace32(ga, e - k, gsorted[k].NW, gsorted[k].C, s);
+			else launch_score32(ga, e - k, gsorted[k].NW, gsorted[k].C, s);
+			HIPOK(hipGetLastError());
+			++nb, k = e;
+		}
+		for (int b = 1; b < std::min(nb, 4); ++b) {
+			HIPOK(hipEventRecord(c.ev_b[b], c.st[b]));
+			HIPOK(hipStreamWaitEvent(c.st[0], c.ev_b[b], 0));
+		}
+		nsh_count_launch(1 + nb);
+	}
+	d_btmeta = (int64_t*)((char*)c.d_meta.p + al256((size_t)n * 4) + 8);
+	d_total = (unsigned long long*)((char*)c.d_meta.p + al256((size_t)n * 4));
+	if (!bt.empty()) {
+		hipLaunchKernelGGL(fill_u64_zero, dim3(1), dim3(64), 0, c.st[0], d_total);
+		hipLaunchKernelGGL(backtrack_kernel, dim3(((int)bt.size() + 63) / 64), dim3(64), 0, c.st[0], (const BtProb*)(d + o_bt),
+				(int32_t)bt.size(), c.d_tr.as<uint16_t>(), c.d_tmp.as<uint32_t>(), c.d_packed.as<uint32_t>(), d_total, d_btmeta);
+		HIPOK(hipGetLastError());
+		nsh_count_launch(2);
+	}
+	// scores, backtrack counts and offsets, then the CIGARs
+	HIPOK(hipMemcpyAsync(c.h_meta.p, c.d_meta.p, meta_bytes, hipMemcpyDeviceToHost, c.st[0]));
+	HIPOK(hipStreamSynchronize(c.st[0]));
+	nsh_add_times(0, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(), 0);
+	bmeta = (int64_t*)((char*)c.h_meta.p + al256((size_t)n * 4) + 8);
+	total = bt.empty()? 0 : *(unsigned long long*)((char*)c.h_meta.p + al256((size_t)n * 4));
+	if (total) {
+		const auto t3 = std::chrono::steady_clock::now();
+		HIPOK(c.h_packed.ensure((size_t)total * 4));
+		HIPOK(hipMemcpyAsync(c.h_packed.p, c.d_packed.p, (size_t)total * 4, hipMemcpyDeviceToHost, c.st[0]));
+		HIPOK(hipStreamSynchronize(c.st[0]));
+		nsh_add_times(0, 0, std::chrono::duration<double>(std::chrono::steady_clock::now() - t3).count());
+	}
+	{
+		const int32_t *hs = (const int32_t*)c.h_meta.p;
+		for (int32_t k = 0; k < n; ++k) {
+			score[k] = P[k].nl > 2? hs[k] : (P[k].is32? -0x40000000 : -32768);
+			cig_off[k] = 0, cig_n[k] = 0;
+		}
+		for (size_t b = 0; b < bt.size(); ++b) cig_off[bt_req[b]] = bmeta[2 * b], cig_n[bt_req[b]] = (int32_t)bmeta[2 * b + 1];
+		*pk_out = (const uint32_t*)c.h_packed.p;
+	}
+	ok = 1;
+fail:
+	return ok;
+}
+
 bool same_terms(const nsh_glob_req *a, const nsh_glob_req *b)
 {
 	const ns_opt_t &x = a->opt, &y = b->opt;
@@ -177,8 +407,6 @@ bool same_terms(const nsh_glob_req *a, const nsh_glob_req *b)
 		std::memcmp(a->codon, b->codon, 64) == 0;
 }
 
-int kernel_variant(const nsh_glob_req *q) { return q->is32? (q->trace? 1 : 2) : 0; }   // trace16, trace32, score32
-
 class Engine {
 public:
 	std::mutex mu;
@@ -186,17 +414,7 @@ public:
 	std::deque<nsh_glob_req*> queue;
 	int n_waiting = 0;
 	bool started = false;
-	// executor-owned
-	hipStream_t st[4] = {};
-	hipEvent_t ev_prep = nullptr, ev_b[4] = {};
-	DevPool d_in, d_rec, d_aas, d_tr, d_tmp, d_packed, d_meta;
-	HostPool h_in, h_meta, h_packed;
-	// trace bytes per batch part: NSH_TRACE_BUDGET_MB (read per batch), 16 GiB by default
-	static size_t trace_budget()
-	{
-		const char *s = std::getenv("NSH_TRACE_BUDGET_MB");
-		return (size_t)(s? std::max(1L, std::atol(s)) : 16384L) << 20;
-	}
+	RunCtx ctx;   // executor-owned
 
 	void start()
 	{
@@ -257,18 +475,7 @@ public:
 
 	void run(std::vector<nsh_glob_req*> &batch)
 	{
-		if (!nsh_ensure_device()) { for (auto *q : batch) q->ok = 0; return; }
-		if (!st[0]) {
-			for (int k = 0; k < 4; ++k) {
-				if (hipStreamCreateWithFlags(&st[k], hipStreamNonBlocking) != hipSuccess ||
-						hipEventCreateWithFlags(&ev_b[k], hipEventDisableTiming) != hipSuccess) {
-					nsh_set_error("global DP engine: stream set-up failed");
-					for (auto *q : batch) q->ok = 0;
-					return;
-				}
-			}
-			(void)hipEventCreateWithFlags(&ev_prep, hipEventDisableTiming);
-		}
+		if (!nsh_ensure_device() || !ctx.init()) { for (auto *q : batch) q->ok = 0; return; }
 		// option groups, then trace-budget cuts
 		const size_t budget = trace_budget();
 		std::vector<char> taken(batch.size(), 0);
@@ -292,7 +499,27 @@ public:
 		}
 	}
 
-	int run_part(std::vector<nsh_glob_req*> &reqs);
+	int run_part(std::vector<nsh_glob_req*> &reqs)
+	{
+		const int32_t n = (int32_t)reqs.size();
+		std::vector<GlobIn> in(n);
+		for (int32_t k = 0; k < n; ++k) {
+			const nsh_glob_req *q = reqs[k];
+			in[k] = { q->ns.data(), q->as.data(), q->ss.empty()? nullptr : q->ss.data(),
+				q->pssm.empty()? nullptr : q->pssm.data(), -1, q->nl, q->al, q->is32, q->trace,
+				(q->opt.flag & NS_F_EXT_LEFT) != 0, q->opt.io };
+		}
+		std::vector<int32_t> sc(n), cn(n);
+		std::vector<int64_t> co(n);
+		const uint32_t *pk = nullptr;
+		if (!run_list(ctx, in.data(), n, terms_of(reqs[0]->opt, nullptr), sc.data(), co.data(), cn.data(), &pk)) return 0;
+		for (int32_t k = 0; k < n; ++k) {
+			nsh_glob_req *q = reqs[k];
+			q->score = sc[k], q->ok = 1;
+			if (cn[k]) q->cigar.assign(pk + co[k], pk + co[k] + cn[k]);
+		}
+		return 1;
+	}
 };
 
 Engine &engine()
@@ -301,183 +528,12 @@ Engine &engine()
 	return *e;
 }
 
-int Engine::run_part(std::vector<nsh_glob_req*> &reqs)
+// the direct batch path's own context (nsh_global_batch runs on the caller's thread)
+std::mutex g_direct_mu;
+RunCtx &direct_ctx()
 {
-	const auto t0 = std::chrono::steady_clock::now();
-	const int32_t n = (int32_t)reqs.size();
-	const nsh_glob_req *r0 = reqs[0];
-	// layout of the input block: ns | as | ss | pssm | PrepProb | GlobProb | BtProb | PrepConst | mat
-	int64_t l_ns = 0, l_as = 0, l_pssm = 0, l_tr = 0, l_tmp = 0;
-	int any_ss = 0, n_bt = 0;
-	std::vector<nsh_prep::PrepProb> pp(n);
-	std::vector<GlobProb> gp(n);
-	std::vector<int32_t> order(n);
-	for (int32_t k = 0; k < n; ++k) {
-		const nsh_glob_req *q = reqs[k];
-		nsh_prep::PrepProb &p = pp[k];
-		p = {};
-		p.ns_off = l_ns, p.as_off = l_as, p.rec_off = l_ns, p.aa_off = l_as;
-		// the strand only matters to int32 score-only requests with an extension flag (nasw-sse.c:246-247)
-		const int left = (q->opt.flag & NS_F_EXT_LEFT) != 0;
-		p.nl = q->nl, p.al = q->al, p.left = left, p.io = q->opt.io, p.has_ss = !q->ss.empty();
-		GlobProb &g = gp[k];
-		g = {};
-		g.rec_off = l_ns, g.aa_off = l_as, g.pssm_off = q->pssm.empty()? -1 : l_pssm;
-		g.tr_off = q->trace? l_tr : 0;
-		g.nl = q->nl, g.al = q->al, g.io = q->opt.io, g.out = k, g.left = left;
-		shape(q->al, g.C, g.NW);
-		const int vs = q->is32? 4 : 8;
-		g.slen = (q->al + vs - 1) / vs, g.npad = q->al, g.mode = 0;
-		l_ns += q->nl + 1, l_as += q->al + 1;
-		if (!q->pssm.empty()) l_pssm += (int64_t)q->al * 22;
-		if (q->trace) l_tr += (int64_t)q->nl * q->al, l_tmp += (int64_t)q->nl + q->al + 8, ++n_bt;
-		any_ss |= !q->ss.empty();
-		order[k] = k;
-	}
-	// buckets of equal kernel variant and shape
-	std::sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-		const int vx = kernel_variant(reqs[x]), vy = kernel_variant(reqs[y]);
-		if (vx != vy) return vx < vy;
-		if (gp[x].NW != gp[y].NW) return gp[x].NW > gp[y].NW;
-		if (gp[x].C != gp[y].C) return gp[x].C > gp[y].C;
-		return x < y; });
-	std::vector<GlobProb> gsorted(n);
-	for (int32_t k = 0; k < n; ++k) gsorted[k] = gp[order[k]];
-	std::vector<BtProb> bt;
-	std::vector<int32_t> bt_req;   // request index of each backtrack slot
-	{
-		int64_t tmp = 0;
-		for (int32_t k = 0; k < n; ++k) {
-			const nsh_glob_req *q = reqs[k];
-			if (!q->trace) continue;
-			bt.push_back({ gp[k].tr_off, tmp, q->nl, q->al });
-			bt_req.push_back(k);
-			tmp += (int64_t)q->nl + q->al + 8;
-		}
-	}
-	const size_t o_ns = 0, o_as = al256(o_ns + l_ns), o_ss = al256(o_as + l_as), o_pssm = al256(o_ss + (any_ss? l_ns : 0));
-	const size_t o_pp = al256(o_pssm + l_pssm), o_gp = al256(o_pp + (size_t)n * sizeof(nsh_prep::PrepProb));
-	const size_t o_bt = al256(o_gp + (size_t)n * sizeof(GlobProb)), o_pc = al256(o_bt + bt.size() * sizeof(BtProb));
-	const size_t o_mat = al256(o_pc + sizeof(nsh_prep::PrepConst)), o_pen = al256(o_mat + 484);
-	const size_t in_bytes = al256(o_pen + 16);
-	int ok = 0;
-	uint8_t *h = nullptr, *d = nullptr;
-	int32_t *d_score = nullptr;
-	int64_t *bmeta = nullptr, *d_btmeta = nullptr;
-	unsigned long long total = 0, *d_total = nullptr;
-	// device result block: scores (n ints) | CIGAR total | (offset, count) per backtracked problem
-	const size_t meta_bytes = al256((size_t)n * 4) + 8 + bt.size() * 16;
-	HIPOK(h_in.ensure(in_bytes));
-	HIPOK(d_in.ensure(in_bytes));
-	HIPOK(d_rec.ensure((size_t)l_ns * 4 + 256));
-	HIPOK(d_aas.ensure((size_t)l_as + 256));
-	if (l_tr) HIPOK(d_tr.ensure((size_t)l_tr * 2 + 256));
-	if (l_tmp) { HIPOK(d_tmp.ensure((size_t)l_tmp * 4 + 256)); HIPOK(d_packed.ensure((size_t)l_tmp * 4 + 256)); }
-	HIPOK(d_meta.ensure(meta_bytes + 256));
-	HIPOK(h_meta.ensure(meta_bytes + 256));
-	h = (uint8_t*)h_in.p, d = (uint8_t*)d_in.p;
-	// pack the inputs
-	{
-		nsh_prep::PrepConst pc;
-		std::memcpy(pc.nt4, r0->nt4, 256), std::memcpy(pc.aa20, r0->aa20, 256), std::memcpy(pc.codon, r0->codon, 64);
-		for (int t = 0; t < 6; ++t) pc.sp[t] = r0->opt.sp[t];
-		pc.sp_null = r0->opt.sp_null_bonus;
-		for (int32_t k = 0; k < n; ++k) {
-			const nsh_glob_req *q = reqs[k];
-			if (q->nl) std::memcpy(h + o_ns + pp[k].ns_off, q->ns.data(), q->nl);
-			std::memcpy(h + o_as + pp[k].as_off, q->as.data(), q->al);
-			if (any_ss) {
-				if (!q->ss.empty()) std::memcpy(h + o_ss + pp[k].ns_off, q->ss.data(), q->nl);
-				else std::memset(h + o_ss + pp[k].ns_off, 0xff, q->nl);
-			}
-			if (!q->pssm.empty()) std::memcpy(h + o_pssm + gp[k].pssm_off, q->pssm.data(), q->pssm.size());
-		}
-		std::memcpy(h + o_pp, pp.data(), (size_t)n * sizeof(nsh_prep::PrepProb));
-		std::memcpy(h + o_gp, gsorted.data(), (size_t)n * sizeof(GlobProb));
-		if (!bt.empty()) std::memcpy(h + o_bt, bt.data(), bt.size() * sizeof(BtProb));
-		std::memcpy(h + o_pc, &pc, sizeof(pc));
-		std::memcpy(h + o_mat, r0->mat, 484);
-	}
-	{
-		const auto t1 = std::chrono::steady_clock::now();
-		HIPOK(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, st[0]));
-		nsh_add_times(std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count(), 0, 0);
-	}
-	d_score = d_meta.as<int32_t>();
-	nsh_prep::launch((const nsh_prep::PrepProb*)(d + o_pp), n, d + o_ns, any_ss? d + o_ss : nullptr, d + o_as,
-			d_rec.as<uint32_t>(), d_aas.as<uint8_t>(), (const nsh_prep::PrepConst*)(d + o_pc), st[0]);
-	HIPOK(hipGetLastError());
-	HIPOK(hipEventRecord(ev_prep, st[0]));
-	{
-		GlobArgs ga = {};
-		ga.rec = d_rec.as<uint32_t>(), ga.aas = d_aas.as<uint8_t>(), ga.pssm = (const int8_t*)(d + o_pssm);
-		ga.mat = (const int8_t*)(d + o_mat), ga.asize = 22;
-		ga.go = r0->opt.go, ga.ge = r0->opt.ge, ga.fs = r0->opt.fs;
-		ga.trace = l_tr? d_tr.as<uint16_t>() : nullptr, ga.score = d_score;
-		ga.pen = (const int32_t*)(d + o_pen), ga.pen_n = 1;
-		const GlobProb *d_gp = (const GlobProb*)(d + o_gp);
-		int nb = 0;
-		for (int32_t k = 0; k < n;) {
-			int32_t e = k;
-			const int v = kernel_variant(reqs[order[k]]);
-			while (e < n && kernel_variant(reqs[order[e]]) == v && gsorted[e].NW == gsorted[k].NW && gsorted[e].C == gsorted[k].C) ++e;
-			hipStream_t s = st[nb & 3];
-			HIPOK(hipStreamWaitEvent(s, ev_prep, 0));
-			ga.prob = d_gp + k;
-			if (v == 0) launch_trace16(ga, e - k, gsorted[k].NW, gsorted[k].C, s);
-			else if (v == 1) launch_trace32(ga, e - k, gsorted[k].NW, gsorted[k].C, s);
-			else launch_score32(ga, e - k, gsorted[k].NW, gsorted[k].C, s);
-			HIPOK(hipGetLastError());
-			++nb, k = e;
-		}
-		for (int b = 1; b < std::min(nb, 4); ++b) {
-			HIPOK(hipEventRecord(ev_b[b], st[b]));
-			HIPOK(hipStreamWaitEvent(st[0], ev_b[b], 0));
-		}
-		nsh_count_launch(1 + nb);
-	}
-	d_btmeta = (int64_t*)((char*)d_meta.p + al256((size_t)n * 4) + 8);
-	d_total = (unsigned long long*)((char*)d_meta.p + al256((size_t)n * 4));
-	if (!bt.empty()) {
-		hipLaunchKernelGGL(fill_u64_zero, dim3(1), dim3(64), 0, st[0], d_total);
-		hipLaunchKernelGGL(backtrack_kernel, dim3(((int)bt.size() + 63) / 64), dim3(64), 0, st[0], (const BtProb*)(d + o_bt),
-				(int32_t)bt.size(), d_tr.as<uint16_t>(), d_tmp.as<uint32_t>(), d_packed.as<uint32_t>(), d_total, d_btmeta);
-		HIPOK(hipGetLastError());
-		nsh_count_launch(2);
-	}
-	// scores, backtrack counts and offsets, then the CIGARs
-	HIPOK(hipMemcpyAsync(h_meta.p, d_meta.p, meta_bytes, hipMemcpyDeviceToHost, st[0]));
-	HIPOK(hipStreamSynchronize(st[0]));
-	{
-		const auto t2 = std::chrono::steady_clock::now();
-		nsh_add_times(0, std::chrono::duration<double>(t2 - t0).count(), 0);
-	}
-	bmeta = (int64_t*)((char*)h_meta.p + al256((size_t)n * 4) + 8);
-	total = bt.empty()? 0 : *(unsigned long long*)((char*)h_meta.p + al256((size_t)n * 4));
-	if (total) {
-		const auto t3 = std::chrono::steady_clock::now();
-		HIPOK(h_packed.ensure((size_t)total * 4));
-		HIPOK(hipMemcpyAsync(h_packed.p, d_packed.p, (size_t)total * 4, hipMemcpyDeviceToHost, st[0]));
-		HIPOK(hipStreamSynchronize(st[0]));
-		nsh_add_times(0, 0, std::chrono::duration<double>(std::chrono::steady_clock::now() - t3).count());
-	}
-	{
-		const int32_t *hs = (const int32_t*)h_meta.p;
-		for (int32_t k = 0; k < n; ++k) {
-			nsh_glob_req *q = reqs[k];
-			q->score = q->nl > 2? hs[k] : (q->is32? -0x40000000 : -32768);
-			q->ok = 1;
-		}
-		const uint32_t *pk = (const uint32_t*)h_packed.p;
-		for (size_t b = 0; b < bt.size(); ++b) {
-			nsh_glob_req *q = reqs[bt_req[b]];
-			const int64_t off = bmeta[2 * b], cnt = bmeta[2 * b + 1];
-			q->cigar.assign(pk + off, pk + off + cnt);
-		}
-	}
-	ok = 1;
-fail:
-	return ok;
+	static RunCtx *c = new RunCtx();
+	return *c;
 }
 
 } // namespace
@@ -545,52 +601,85 @@ int nsh_global_run(void *km, const char *ns, int32_t nl, const char *as, int32_t
 	return nsh_glob_wait(q, km, r);
 }
 
-// n problems of one option set in one engine batch (nasw-hip.h)
-extern "C" int nsh_global_batch(int32_t n, const char *ns, const int64_t *ns_off, const int32_t *nl, const char *as,
+// n problems of one option set in one batch, run on the caller's thread (nasw-hip.h); with d_g,
+// problem k with g_off[k] >= 0 reads its bases from d_g + g_off[k] in HBM instead of ns
+int nsh_global_batch_g(int32_t n, const char *ns, const int64_t *ns_off, const int32_t *nl, const char *as,
 		const int64_t *as_off, const int32_t *al, const uint8_t *ss, const ns_opt_t *opt, int32_t mode,
-		int32_t *score, int32_t *n_cigar, uint32_t **cigar)
+		const uint8_t *d_g, const int64_t *g_off, int32_t *score, int32_t *n_cigar, uint32_t **cigar)
 {
-	if (n < 0 || (n > 0 && (!ns || !ns_off || !nl || !as || !as_off || !al)) || !opt || !score || !n_cigar || !cigar) {
+	if (n < 0 || (n > 0 && (!ns_off || !nl || !as || !as_off || !al || (!ns && !g_off) || (g_off && !d_g))) || !opt ||
+			!score || !n_cigar || !cigar) {
 		nsh_set_error("nsh_global_batch: bad arguments");
 		return 0;
 	}
 	*cigar = nullptr;
 	if (n == 0) return 1;
+	if (!nsh_ensure_device()) return 0;
+	// the checks glob_make makes, once for the shared options and per problem for the lengths
+	const bool trace = (opt->flag & NS_F_CIGAR) && !(opt->flag & (NS_F_EXT_LEFT | NS_F_EXT_RIGHT));
+	if (opt->asize != 22) { nsh_set_error("asize %d not supported", opt->asize); return 0; }
+	if (opt->ge < 0 || opt->go < 0 || opt->ge > 32767 || opt->go > 32767 || opt->fs < 0 || opt->fs > 32767) {
+		nsh_set_error("gap penalties go=%d ge=%d fs=%d out of the supported range", opt->go, opt->ge, opt->fs); return 0;
+	}
+	for (int t = 0; t < 64; ++t)
+		if (opt->codon[t] >= 22) { nsh_set_error("codon table entry %d out of range", t); return 0; }
+	for (int t = 0; t < 256; ++t)
+		if (opt->aa20[t] >= 22 || opt->nt4[t] > 4) { nsh_set_error("alphabet table entry %d out of range", t); return 0; }
 	int32_t max_sub = 0;
 	for (int32_t t = 0; t < opt->asize * opt->asize; ++t) if (max_sub < opt->sc[t]) max_sub = opt->sc[t];
-	std::vector<nsh_glob_req*> qs(n, nullptr);
-	int ok = 1;
-	for (int32_t k = 0; k < n && ok; ++k) {
+	std::vector<GlobIn> in(n);
+	for (int32_t k = 0; k < n; ++k) {
+		if (al[k] < 1 || nl[k] < 0 || al[k] > 64 * kCMaxG * kNWMax) {
+			nsh_set_error("nsh_global_batch: problem %d: nl=%d al=%d (protein limit %d)", k, nl[k], al[k], 64 * kCMaxG * kNWMax);
+			return 0;
+		}
 		// mode 2: ns_global_gs_auto's choice (nasw-sse.c:980-999), per problem
 		const int32_t is32 = mode == 1 || (mode == 2 && (int64_t)al[k] * max_sub + opt->end_bonus > 30000);
-		qs[k] = glob_make(ns + ns_off[k], nl[k], as + as_off[k], al[k], opt, ss? ss + ns_off[k] : nullptr, is32);
-		if (!qs[k]) ok = 0;
+		if (!is32 && !trace) { nsh_set_error("score-only int16 DP goes through the extension engine"); return 0; }
+		const bool from_g = g_off && g_off[k] >= 0;
+		if (from_g && ss) { nsh_set_error("nsh_global_batch: splice scores need bases on the host"); return 0; }
+		in[k] = { from_g? nullptr : (const uint8_t*)ns + ns_off[k], (const uint8_t*)as + as_off[k],
+			ss? ss + ns_off[k] : nullptr, opt->pssm, from_g? g_off[k] : -1, nl[k], al[k], is32, trace,
+			(opt->flag & NS_F_EXT_LEFT) != 0, opt->io };
 	}
-	if (!ok) { for (nsh_glob_req *q : qs) delete q; return 0; }
-	engine().submit_many(qs);
-	int64_t tot = 0;
-	for (int32_t k = 0; k < n; ++k) {
-		engine().wait(qs[k]);
-		ok &= qs[k]->ok;
-		tot += (int64_t)qs[k]->cigar.size();
-	}
-	{
-		std::lock_guard<std::mutex> lk(engine().mu);   // submit_many counted the caller as waiting
-		--engine().n_waiting;
-	}
-	uint32_t *out = ok? (uint32_t*)std::malloc(std::max<int64_t>(tot, 1) * sizeof(uint32_t)) : nullptr;
-	if (ok && !out) { nsh_set_error("nsh_global_batch: out of host memory"); ok = 0; }
-	int64_t o = 0;
-	for (int32_t k = 0; k < n; ++k) {
-		nsh_glob_req *q = qs[k];
-		if (ok) {
-			score[k] = q->score, n_cigar[k] = (int32_t)q->cigar.size();
-			if (!q->cigar.empty()) std::memcpy(out + o, q->cigar.data(), q->cigar.size() * sizeof(uint32_t));
-			o += (int64_t)q->cigar.size();
+	const GlobTerms T = terms_of(*opt, d_g);
+	std::vector<int64_t> co(n);
+	std::vector<int32_t> cn(n);
+	std::vector<uint32_t> all;
+	std::lock_guard<std::mutex> lk(g_direct_mu);
+	RunCtx &ctx = direct_ctx();
+	if (!ctx.init()) return 0;
+	// trace-budget cuts over the input order
+	const size_t budget = trace_budget();
+	for (int32_t lo = 0; lo < n;) {
+		int32_t hi = lo;
+		size_t tr = 0;
+		while (hi < n) {
+			const size_t t = trace? (size_t)nl[hi] * al[hi] * 2 : 0;
+			if (hi > lo && tr + t > budget) break;
+			tr += t, ++hi;
 		}
-		delete q;
+		const uint32_t *pk = nullptr;
+		if (!run_list(ctx, in.data() + lo, hi - lo, T, score + lo, co.data() + lo, cn.data() + lo, &pk)) return 0;
+		for (int32_t k = lo; k < hi; ++k) {
+			const int64_t o = (int64_t)all.size();
+			if (cn[k]) all.insert(all.end(), pk + co[k], pk + co[k] + cn[k]);
+			co[k] = o;
+		}
+		lo = hi;
 	}
+	uint32_t *out = (uint32_t*)std::malloc(std::max<size_t>(all.size(), 1) * sizeof(uint32_t));
+	if (!out) { nsh_set_error("nsh_global_batch: out of host memory"); return 0; }
+	if (!all.empty()) std::memcpy(out, all.data(), all.size() * sizeof(uint32_t));
+	for (int32_t k = 0; k < n; ++k) n_cigar[k] = cn[k];
 	*cigar = out;
-	return ok;
+	return 1;
 }
 
+extern "C" int nsh_global_batch(int32_t n, const char *ns, const int64_t *ns_off, const int32_t *nl, const char *as,
+		const int64_t *as_off, const int32_t *al, const uint8_t *ss, const ns_opt_t *opt, int32_t mode,
+		int32_t *score, int32_t *n_cigar, uint32_t **cigar)
+{
+	if (n > 0 && !ns) { nsh_set_error("nsh_global_batch: bad arguments"); return 0; }
+	return nsh_global_batch_g(n, ns, ns_off, nl, as, as_off, al, ss, opt, mode, nullptr, nullptr, score, n_cigar, cigar);
+}

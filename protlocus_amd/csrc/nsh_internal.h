@@ -52,6 +52,12 @@ int nsh_glob_wait(nsh_glob_req *q, void *km, ns_rst_t *r);
 int nsh_global_run(void *km, const char *ns, int32_t nl, const char *as, int32_t al,
 		const ns_opt_t *opt, const uint8_t *ss, int32_t is32, ns_rst_t *r);
 
+// nsh_global_batch where problem k with g_off[k] >= 0 reads its bases from d_g + g_off[k] in HBM
+// (ns may then be null; no ss); run on the caller's thread with its own streams and pools
+int nsh_global_batch_g(int32_t n, const char *ns, const int64_t *ns_off, const int32_t *nl, const char *as,
+		const int64_t *as_off, const int32_t *al, const uint8_t *ss, const ns_opt_t *opt, int32_t mode,
+		const uint8_t *d_g, const int64_t *g_off, int32_t *score, int32_t *n_cigar, uint32_t **cigar);
+
 // oriented bases (codes 0..4) of [st, en) of strand vid = contig << 1 | reverse, the contig's bases
 // at nt4 + ctg_off[c] (mp_ntseq_get_by_v, ntseq.c:92-118); en is clipped to the contig. Returns the
 // number written, -1 when st is out of range
