@@ -122,3 +122,23 @@ int nsh_copy_to_device(void *dst, const void *src, size_t bytes, hipStream_t s)
 	if (!ok) nsh_set_error("copy to device failed");
 	return ok;
 }
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+
+static double nsh_now_ms()
+{
+	return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+nsh_step_clock::nsh_step_clock(const char *w) : who(w), on(std::getenv("NSH_PROFILE") != nullptr), t(nsh_now_ms()) {}
+
+void nsh_step_clock::lap(const char *what, int64_t n)
+{
+	if (!on) return;
+	const double now = nsh_now_ms();
+	if (n >= 0) std::fprintf(stderr, "[%s] %-24s %9.1f ms  n=%lld\n", who, what, now - t, (long long)n);
+	else std::fprintf(stderr, "[%s] %-24s %9.1f ms\n", who, what, now - t);
+	t = now;
+}

@@ -540,40 +540,31 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 	for (int32_t q = 0; q < n_q; ++q) std::copy(qsk[q].begin(), qsk[q].end(), qall.begin() + qoff[q]);
 	lap(2);
 	// window k-mers shared with the query, bucket pairs and the window chains of every region (map.c:153-203)
-	std::vector<int64_t> p_off;
-	std::vector<int32_t> pn_u;
-	std::vector<uint64_t> pu, pao;
+	std::vector<uint64_t> best_u, best_a;
+	std::vector<int64_t> best_off;
 	const nsh_chain::Params rcp = { o->dp.max_intron, o->max_gap, o->bw, o->max_chn_max_skip, o->max_chn_iter,
 		o->min_chn_cnt, o->min_chn_sc, o->chn_coef_log, 1, kmer2, 0 };
-	if (!nsh_refine::run(m->d_g, rjobs, qoff, qall, kmer2, m->min_aa_len, o->max_ava, rcp, p_off, pn_u, pu, pao))
+	if (!nsh_refine::run(m->d_g, rjobs, qoff, qall, kmer2, m->min_aa_len, o->max_ava, rcp, best_u, best_off, best_a,
+			&st[2]))
 		return 0;
-	st[2] = p_off.empty()? 0 : p_off.back();
-	parallel_for((int64_t)jobs.size(), nth, [&](int64_t t) {   // the best chain becomes the region (map.c:209-258)
+	parallel_for((int64_t)jobs.size(), nth, [&](int64_t t) {   // the best chain becomes the region (map.c:249-258)
 		const RefJob &jb = jobs[t];
 		Reg &r = pool[jb.q][regs[jb.q][jb.r]];
-		const int32_t n_u = pn_u[t];
-		const uint64_t *u = pu.data() + p_off[t], *a = pao.data() + p_off[t];
-		if (n_u == 0) { r.cnt = 0, r.a.clear(); return; }
-		int32_t max_i = 0;
-		uint64_t max_sc = u[0] >> 32;
-		for (int32_t i = 1; i < n_u; ++i)
-			if (max_sc < u[i] >> 32) max_sc = u[i] >> 32, max_i = i;
-		int64_t k = 0;
-		for (int32_t i = 0; i < max_i; ++i) k += (uint32_t)u[i];
-		const int32_t n_a = (int32_t)(uint32_t)u[max_i];
-		r.chn_sc = (int32_t)(u[max_i] >> 32);
+		const int32_t n_a = (int32_t)(uint32_t)best_u[t];
+		if (n_a == 0) { r.cnt = 0, r.a.clear(); return; }
+		const uint64_t *a = best_a.data() + best_off[t];
+		r.chn_sc = (int32_t)(best_u[t] >> 32);
 		r.rank_sc = 0;
 		r.cnt = n_a;
-		r.qs = (int32_t)((uint32_t)a[k] - (uint32_t)(kmer2 - 1));
-		r.qe = (int32_t)((uint32_t)a[k + n_a - 1] + 1);
-		r.vs = jb.as + (int64_t)(a[k] >> 32) + 1 - 3 * kmer2;
-		r.ve = jb.as + (int64_t)(a[k + n_a - 1] >> 32) + 1;
+		r.qs = (int32_t)((uint32_t)a[0] - (uint32_t)(kmer2 - 1));
+		r.qe = (int32_t)((uint32_t)a[n_a - 1] + 1);
+		r.vs = jb.as + (int64_t)(a[0] >> 32) + 1 - 3 * kmer2;
+		r.ve = jb.as + (int64_t)(a[n_a - 1] >> 32) + 1;
 		r.a.resize(n_a);
 		for (int32_t i = 0; i < n_a; ++i)
-			r.a[i] = (uint64_t)((int64_t)(a[k + i] >> 32) + jb.as - r.vs) << 32 | (a[k + i] & 0xffffffffULL);
+			r.a[i] = (uint64_t)((int64_t)(a[i] >> 32) + jb.as - r.vs) << 32 | (a[i] & 0xffffffffULL);
 		r.chn_sc_ungap = chn_sc_ungap(r.a, kmer2);
 	});
-	pu.clear(), pao.clear();
 	parallel_for(n_q, nth, [&](int64_t q) {   // map.c:685-698
 		Regs kept;
 		for (int32_t i : regs[q])

@@ -15,6 +15,15 @@ void nsh_count_launch(uint64_t n);
 void nsh_add_times(double h2d, double kernel, double d2h);
 int32_t nsh_host_stripe_width(int32_t flag, int32_t al); // the reference CPU's stripe width on this host
 
+// with NSH_PROFILE set in the environment, each lap() prints the host time since the last one to stderr
+struct nsh_step_clock {
+	const char *who;
+	bool on;
+	double t;
+	explicit nsh_step_clock(const char *w);
+	void lap(const char *what, int64_t n = -1);
+};
+
 // length penalty of the extension DP: (int)(ie_coef * log2approx(d) + .5f) for d >= 2 (nasw-sse.c:341-349, 758-761)
 int32_t nsh_ie_penalty(float ie_coef, int32_t d);
 
@@ -140,13 +149,14 @@ struct Job { int64_t g_off; int32_t len, q; };   // a window: its first base in 
 // followed by their reverse complements at total + ctg_off[c]; *d_g is hipFree'd by the caller
 int oriented_genome(const uint8_t *nt4, const int64_t *ctg_off, const int64_t *ctg_len, int32_t n_ctg, int64_t total,
 		uint8_t **d_g);
-// mp_refine_reg's sketch, bucket pairs and mp_chain (map.c:153-203) for every job: qsk holds each
-// query's sorted k-mers (hash << 32 | position, mp_sketch_prot with mod_bit 0) at qoff[q]. Job j's
-// pairs start at p_off[j] (p_off has n + 1 entries); its n_u[j] chains (score << 32 | count) are
-// u[p_off[j] ..) and their anchors, chain after chain, ao[p_off[j] ..). 1 on success.
+// mp_refine_reg's sketch, bucket pairs, mp_chain and chain choice (map.c:153-247) for every job: qsk
+// holds each query's sorted k-mers (hash << 32 | position, mp_sketch_prot with mod_bit 0) at qoff[q].
+// Job j's chosen chain is best_u[j] (score << 32 | count; 0 when the job has no chain) and its anchors
+// (window position << 32 | query position) are anchors[best_off[j] ..). *n_pairs: the anchor pairs
+// chained. 1 on success.
 int run(const uint8_t *d_g, const std::vector<Job> &jobs, const std::vector<int64_t> &qoff,
 		const std::vector<uint64_t> &qsk, int32_t kmer, int32_t min_aa_len, int32_t max_ava, const nsh_chain::Params &cp,
-		std::vector<int64_t> &p_off, std::vector<int32_t> &n_u, std::vector<uint64_t> &u, std::vector<uint64_t> &ao);
+		std::vector<uint64_t> &best_u, std::vector<int64_t> &best_off, std::vector<uint64_t> &anchors, int64_t *n_pairs);
 }
 
 #endif

@@ -7,21 +7,24 @@
 // bucket's n1 x n2 exceeds max_ava. The pairs go to mp_chain.
 //
 // Only window k-mers whose hash the query also has can take part in a pair, so the window is never
-// sketched as a whole: each work-group holds its query's hashes in an LDS hash set, sweeps its
-// window in tiles staged through LDS, and keeps a position only when its k-mer's hash is in the set
-// and its ORF (the run of sense codons around it, cut at the window's ends) is long enough. That is
+// sketched as a whole: each work-group holds a bit filter of its query's hashes in LDS, sweeps its
+// window in tiles staged through LDS (bases, then the codon code ending at each base), and keeps a
+// position only when its k-mer's hash passes the filter and is found among the query's sorted
+// k-mers, and its ORF (the run of sense codons around it, cut at the window's ends) is long enough. That is
 // exactly the subset of the reference's sketch that its bucket loop can pair. The windows are
 // slices of the oriented genome resident in HBM (forward strand, then the reverse complement).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
+#include <algorithm>
 #include <vector>
 #include "nsh_internal.h"
 
 namespace {
 
-constexpr int kSetBits = 14, kSetSize = 1 << kSetBits;   // LDS hash set: 16,384 slots (64 KB)
-constexpr uint32_t kEmpty = 0xffffffffu;
-constexpr int kTile = 4096, kHalo = 24;                   // window bases per sweep step; k-mer history
+constexpr int kFilterBits = 16;                          // LDS filter of the query's hashes: 2^16 bits (8 KB)
+// window bases per sweep step; bases staged before and after it: the k-mer's history (at most 20
+// bases) and an ORF walk of 30 codons either way
+constexpr int kTile = 4096, kHalo = 120, kAhead = 96;
 
 struct RConst { uint8_t codon[64], codon13[64]; };
 
@@ -36,8 +39,6 @@ __device__ inline uint32_t hash32_mask(uint32_t key, uint32_t mask)   // sketch.
 	return key;
 }
 
-__device__ inline uint32_t slot_of(uint32_t h) { return (h * 2654435761u) >> (32 - kSetBits); }
-
 // the sense codon ending at window position e (its three bases valid and not a stop)
 __device__ inline bool sense_at(const uint8_t *w, int64_t e, const uint8_t *codon)
 {
@@ -45,102 +46,99 @@ __device__ inline bool sense_at(const uint8_t *w, int64_t e, const uint8_t *codo
 	return b0 < 4 && b1 < 4 && b2 < 4 && codon[b0 << 4 | b1 << 2 | b2] < 20;
 }
 
-// Window k-mers of each job that its query shares: counted per job (out == nullptr) or written to
-// out[woff[j] ..) as hash << 32 | position of the k-mer's last base (the entries mp_sketch_nt4 makes
-// with bbit 0). One work-group of 256 per job.
-__global__ void __launch_bounds__(256) window_hits_kernel(const nsh_refine::Job *jobs, const uint8_t *g,
-		const int64_t *qoff, const uint64_t *qsk, int32_t kmer, int32_t min_aa_len, const RConst *rcg, int64_t *cnt,
-		const int64_t *woff, uint64_t *out)
+// Window k-mers of each job that its query shares, written unsorted to out[cap_off[j] ..) (at most
+// cap[j] of them; cnt[j] receives how many there are, so a job with cnt[j] > cap[j] is run again
+// with more room) as hash << 32 | position of the k-mer's last base (the entries mp_sketch_nt4
+// makes with bbit 0). One work-group of 256 per job (idx: the jobs to run, or null for all).
+__global__ void __launch_bounds__(256) window_hits_kernel(const nsh_refine::Job *jobs, const int32_t *idx,
+		const uint8_t *g, const int64_t *qoff, const uint64_t *qsk, int32_t kmer, int32_t min_aa_len, const RConst *rcg,
+		int64_t *cnt, const int64_t *cap_off, const int64_t *cap, uint64_t *out)
 {
-	__shared__ uint32_t set[kSetSize];
-	__shared__ uint8_t tile[kTile + kHalo];
+	__shared__ uint32_t filter[(1 << kFilterBits) / 32];   // bit (h mod 2^16) set for each query hash h
+	__shared__ uint8_t base[kTile + kHalo + kAhead + 2];   // window bases [t0 - kHalo - 2, t0 + kTile + kAhead)
+	__shared__ uint8_t code[kTile + kHalo + kAhead];       // codon13 of the sense codon ending at t0 - kHalo + k, 0xff if none
 	__shared__ uint8_t codon[64], codon13[64];
 	__shared__ int32_t n_hit;
-	const nsh_refine::Job J = jobs[blockIdx.x];
+	const int32_t jb = idx? idx[blockIdx.x] : (int32_t)blockIdx.x;
+	const nsh_refine::Job J = jobs[jb];
 	const uint64_t *qs = qsk + qoff[J.q];
 	const int64_t nq = qoff[J.q + 1] - qoff[J.q];
-	const bool in_lds = nq <= kSetSize / 2;
 	const uint32_t mask_k = (uint32_t)(((uint64_t)1 << (4 * kmer)) - 1);
 	const uint8_t *w = g + J.g_off;
-	const int64_t len = J.len;
+	const int64_t len = J.len, base_out = cap_off[jb], room = cap[jb];
 	if (threadIdx.x < 64) codon[threadIdx.x] = rcg->codon[threadIdx.x], codon13[threadIdx.x] = rcg->codon13[threadIdx.x];
 	if (threadIdx.x == 0) n_hit = 0;
-	if (in_lds) {
-		for (int k = threadIdx.x; k < kSetSize; k += blockDim.x) set[k] = kEmpty;
-		__syncthreads();
-		for (int64_t k = threadIdx.x; k < nq; k += blockDim.x) {
-			const uint32_t h = (uint32_t)(qs[k] >> 32);
-			for (uint32_t s = slot_of(h);; s = (s + 1) & (kSetSize - 1)) {
-				const uint32_t old = atomicCAS(&set[s], kEmpty, h);
-				if (old == kEmpty || old == h) break;
-			}
-		}
+	for (int k = threadIdx.x; k < (1 << kFilterBits) / 32; k += blockDim.x) filter[k] = 0;
+	__syncthreads();
+	for (int64_t k = threadIdx.x; k < nq; k += blockDim.x) {
+		const uint32_t b = (uint32_t)(qs[k] >> 32) & ((1u << kFilterBits) - 1);
+		atomicOr(&filter[b >> 5], 1u << (b & 31));
 	}
 	__syncthreads();
-	const int64_t wbase = woff? woff[blockIdx.x] : 0;
 	for (int64_t t0 = 0; t0 < len; t0 += kTile) {
-		for (int k = threadIdx.x; k < kTile + kHalo; k += blockDim.x) {
-			const int64_t p = t0 - kHalo + k;
-			tile[k] = p >= 0 && p < len? w[p] : 4;
+		for (int k = threadIdx.x; k < kTile + kHalo + kAhead + 2; k += blockDim.x) {
+			const int64_t p = t0 - kHalo - 2 + k;
+			base[k] = p >= 0 && p < len? w[p] : 4;
+		}
+		__syncthreads();
+		for (int k = threadIdx.x; k < kTile + kHalo + kAhead; k += blockDim.x) {   // position t0 - kHalo + k ends a codon?
+			const uint8_t b0 = base[k], b1 = base[k + 1], b2 = base[k + 2];
+			const int cd = b0 << 4 | b1 << 2 | b2;
+			code[k] = b0 < 4 && b1 < 4 && b2 < 4 && codon[cd] < 20? codon13[cd] : 0xff;
 		}
 		__syncthreads();
 		for (int k = threadIdx.x; k < kTile; k += blockDim.x) {
 			const int64_t i = t0 + k;   // the k-mer's last base
 			if (i >= len || i - 3 * (kmer - 1) - 2 < 0) continue;
-			const uint8_t *tw = tile + kHalo + k - i;   // tw[p] is window base p for p in [i - kHalo, i]
 			uint32_t x = 0;
 			bool ok = true;
 			for (int c = kmer - 1; c >= 0 && ok; --c) {
-				const int64_t e = i - 3 * c;
-				const uint8_t b0 = tw[e - 2], b1 = tw[e - 1], b2 = tw[e];
-				const int cd = b0 << 4 | b1 << 2 | b2;
-				ok = b0 < 4 && b1 < 4 && b2 < 4 && codon[cd] < 20;
-				x = (x << 4 | codon13[cd]) & mask_k;
+				const uint8_t v = code[kHalo + k - 3 * c];
+				ok = v != 0xff;
+				x = (x << 4 | v) & mask_k;
 			}
 			if (!ok) continue;
-			const uint32_t y = hash32_mask(x, mask_k);
-			bool hit = false;
-			if (in_lds) {
-				for (uint32_t s = slot_of(y);; s = (s + 1) & (kSetSize - 1)) {
-					const uint32_t v = set[s];
-					if (v == y) { hit = true; break; }
-					if (v == kEmpty) break;
-				}
-			} else {   // a query longer than the LDS set holds: binary search of its sorted k-mers
-				int64_t lo = 0, hi = nq;
-				while (lo < hi) {
-					const int64_t m = (lo + hi) >> 1;
-					if ((uint32_t)(qs[m] >> 32) < y) lo = m + 1;
-					else hi = m;
-				}
-				hit = lo < nq && (uint32_t)(qs[lo] >> 32) == y;
+			const uint32_t y = hash32_mask(x, mask_k), fb = y & ((1u << kFilterBits) - 1);
+			if (!(filter[fb >> 5] >> (fb & 31) & 1)) continue;
+			// past the filter: a binary search of the query's sorted k-mers decides
+			int64_t lo = 0, hi = nq;
+			while (lo < hi) {
+				const int64_t m = (lo + hi) >> 1;
+				if ((uint32_t)(qs[m] >> 32) < y) lo = m + 1;
+				else hi = m;
 			}
-			if (!hit) continue;
-			// its ORF: the run of sense codons around it in its frame, cut at the window's ends
+			if (lo >= nq || (uint32_t)(qs[lo] >> 32) != y) continue;
+			// its ORF: the run of sense codons around it in its frame, cut at the window's ends, need
+			// only be known to be min_aa_len codons long; staged codes serve the walk, global memory
+			// beyond them
+			auto sense = [&](int64_t e) {
+				const int64_t c = e - (t0 - kHalo);
+				return c >= 0 && c < kTile + kHalo + kAhead? code[c] != 0xff : sense_at(w, e, codon);
+			};
 			int64_t s = i - 3 * (kmer - 1), e = i;
-			while (s - 3 >= 2 && sense_at(w, s - 3, codon)) s -= 3;
-			while (e + 3 < len && sense_at(w, e + 3, codon)) e += 3;
+			while ((e - s) / 3 + 1 < min_aa_len && s - 3 >= 2 && sense(s - 3)) s -= 3;
+			while ((e - s) / 3 + 1 < min_aa_len && e + 3 < len && sense(e + 3)) e += 3;
 			if ((e - s) / 3 + 1 < min_aa_len) continue;
 			const int32_t slot = atomicAdd(&n_hit, 1);
-			if (out) out[wbase + slot] = (uint64_t)y << 32 | (uint64_t)i;
+			if (slot < room) out[base_out + slot] = (uint64_t)y << 32 | (uint64_t)i;
 		}
 		__syncthreads();
 	}
-	if (!out && threadIdx.x == 0) cnt[blockIdx.x] = n_hit;
+	if (threadIdx.x == 0) cnt[jb] = n_hit;
 }
 
 // The bucket pairs of each job (map.c:165-198): a window entry with hash h pairs with every query
 // k-mer of hash h when n1 x n2 <= max_ava (n1, n2: the bucket's window and query counts, their
 // product in int32 as the reference computes it). Counted (out == nullptr) or written as
-// window position << 32 | query position. went: each job's entries sorted.
-__global__ void __launch_bounds__(256) pairs_kernel(const nsh_refine::Job *jobs, const int64_t *woff,
-		const uint64_t *went, const int64_t *qoff, const uint64_t *qsk, int32_t max_ava, int64_t *cnt,
+// window position << 32 | query position. went: each job's entries, sorted, at [wbeg[j], wend[j]).
+__global__ void __launch_bounds__(256) pairs_kernel(const nsh_refine::Job *jobs, const int64_t *wbeg,
+		const int64_t *wend, const uint64_t *went, const int64_t *qoff, const uint64_t *qsk, int32_t max_ava, int64_t *cnt,
 		const int64_t *poff, uint64_t *out)
 {
 	__shared__ int64_t n_pair;
 	const nsh_refine::Job J = jobs[blockIdx.x];
-	const uint64_t *E = went + woff[blockIdx.x], *Q = qsk + qoff[J.q];
-	const int64_t ne = woff[blockIdx.x + 1] - woff[blockIdx.x], nq = qoff[J.q + 1] - qoff[J.q];
+	const uint64_t *E = went + wbeg[blockIdx.x], *Q = qsk + qoff[J.q];
+	const int64_t ne = wend[blockIdx.x] - wbeg[blockIdx.x], nq = qoff[J.q + 1] - qoff[J.q];
 	if (threadIdx.x == 0) n_pair = 0;
 	__syncthreads();
 	auto lower = [](const uint64_t *a, int64_t n, uint32_t h) {
@@ -166,6 +164,36 @@ __global__ void __launch_bounds__(256) pairs_kernel(const nsh_refine::Job *jobs,
 	}
 	__syncthreads();
 	if (!out && threadIdx.x == 0) cnt[blockIdx.x] = n_pair;
+}
+
+// mp_refine_reg's choice among a job's chains (map.c:240-247 without the --accuracy rules): the
+// first of the highest score; its anchors are copied to a compact array at an offset reserved with
+// one atomic. best[j] = { offset, the chain's score << 32 | count } (count 0: no chain)
+__global__ void __launch_bounds__(64) best_chain_kernel(const int64_t *poff, const int32_t *nu, const uint64_t *u,
+		const uint64_t *ao, unsigned long long *total, int64_t *best, uint64_t *compact)
+{
+	__shared__ int64_t k_first, off;
+	__shared__ int32_t n_a;
+	const int32_t j = blockIdx.x;
+	const int32_t n_u = nu[j];
+	const uint64_t *uj = u + poff[j];
+	if (threadIdx.x == 0) {
+		int32_t max_i = 0;
+		int64_t k = 0;
+		if (n_u > 0) {
+			uint64_t max_sc = uj[0] >> 32;
+			for (int32_t i = 1; i < n_u; ++i)
+				if (max_sc < uj[i] >> 32) max_sc = uj[i] >> 32, max_i = i;
+			for (int32_t i = 0; i < max_i; ++i) k += (uint32_t)uj[i];
+		}
+		n_a = n_u > 0? (int32_t)(uint32_t)uj[max_i] : 0;
+		k_first = k;
+		off = n_a > 0? (int64_t)atomicAdd(total, (unsigned long long)n_a) : 0;
+		best[2 * j] = off, best[2 * j + 1] = n_u > 0? (int64_t)uj[max_i] : 0;
+	}
+	__syncthreads();
+	const uint64_t *src = ao + poff[j] + k_first;
+	for (int32_t t = threadIdx.x; t < n_a; t += blockDim.x) compact[off + t] = src[t];
 }
 
 __global__ void revcomp_contigs_kernel(const uint8_t *fwd, const int64_t *off, const int64_t *len, int32_t n_ctg,
@@ -218,39 +246,49 @@ done:
 
 int nsh_refine::run(const uint8_t *d_g, const std::vector<Job> &jobs, const std::vector<int64_t> &qoff,
 		const std::vector<uint64_t> &qsk, int32_t kmer, int32_t min_aa_len, int32_t max_ava, const nsh_chain::Params &cp,
-		std::vector<int64_t> &p_off, std::vector<int32_t> &n_u, std::vector<uint64_t> &u, std::vector<uint64_t> &ao)
+		std::vector<uint64_t> &best_u, std::vector<int64_t> &best_off, std::vector<uint64_t> &anchors, int64_t *n_pairs)
 {
 	const int32_t n = (int32_t)jobs.size();
-	p_off.assign((size_t)n + 1, 0), n_u.assign(n, 0), u.clear(), ao.clear();
+	std::vector<int64_t> p_off((size_t)n + 1, 0);
+	best_u.assign(n, 0), best_off.assign(n, 0), anchors.clear();
+	if (n_pairs) *n_pairs = 0;
 	if (n == 0) return 1;
 	if (kmer < 1 || kmer > 7) { nsh_set_error("refine: k-mer length %d outside 1..7", kmer); return 0; }
 	if (!nsh_ensure_device()) return 0;
+	nsh_step_clock clk("refine");
 	int ok = 0;
 	hipStream_t s = nullptr;
 	Job *d_jobs = nullptr;
-	int64_t *d_qoff = nullptr, *d_cnt = nullptr, *d_woff = nullptr, *d_poff = nullptr;
+	int64_t *d_qoff = nullptr, *d_cnt = nullptr, *d_poff = nullptr;
 	uint64_t *d_qsk = nullptr, *d_w = nullptr, *d_ws = nullptr, *d_p = nullptr, *d_ps = nullptr, *d_u = nullptr;
 	uint64_t *d_ao = nullptr;
 	int32_t *d_nu = nullptr;
+	int64_t *d_best = nullptr;
+	uint64_t *d_compact = nullptr;
+	unsigned long long *d_total = nullptr;
+	int32_t *d_over = nullptr;
 	void *d_tmp = nullptr;
 	size_t tmp_cap = 0;
 	RConst rc;
 	RConst *d_rc = nullptr;
-	std::vector<int64_t> cnt(n), woff((size_t)n + 1, 0), pc(n);
+	std::vector<int64_t> cnt(n), cap(n), cap_off(n), wend(n), pc(n);
+	int64_t *d_cap = nullptr, *d_capoff = nullptr, *d_wend = nullptr;
 	for (int t = 0; t < 64; ++t) rc.codon[t] = ns_tab_codon[t], rc.codon13[t] = ns_tab_codon13[t];
-	auto sort_segments = [&](const uint64_t *in, uint64_t *outk, int64_t total, const int64_t *d_seg) -> bool {
+	// a segmented sort of the keys in [beg[j], end[j]) of each segment (others untouched)
+	auto sort_segments = [&](const uint64_t *in, uint64_t *outk, int64_t total, const int64_t *d_beg,
+			const int64_t *d_end) -> bool {
 		if (total <= 0) return true;
 		size_t need = 0;
-		if (hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, need, in, outk, (int)total, n, d_seg, d_seg + 1, 0, 64,
-				s) != hipSuccess) return false;
+		if (hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, need, in, outk, (int)total, n, d_beg, d_end, 0, 64, s) !=
+				hipSuccess) return false;
 		if (need > tmp_cap) {
 			if (d_tmp) (void)hipFree(d_tmp);
 			d_tmp = nullptr, tmp_cap = 0;
 			if (hipMalloc(&d_tmp, need + 256) != hipSuccess) return false;
 			tmp_cap = need + 256;
 		}
-		return hipcub::DeviceSegmentedRadixSort::SortKeys(d_tmp, need, in, outk, (int)total, n, d_seg, d_seg + 1, 0, 64,
-				s) == hipSuccess;
+		return hipcub::DeviceSegmentedRadixSort::SortKeys(d_tmp, need, in, outk, (int)total, n, d_beg, d_end, 0, 64, s) ==
+				hipSuccess;
 	};
 	{
 		RFOK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -258,44 +296,89 @@ int nsh_refine::run(const uint8_t *d_g, const std::vector<Job> &jobs, const std:
 		RFOK(hipMalloc((void**)&d_qoff, qoff.size() * 8));
 		RFOK(hipMalloc((void**)&d_qsk, (qsk.size() + 1) * 8));
 		RFOK(hipMalloc((void**)&d_cnt, (size_t)n * 8));
-		RFOK(hipMalloc((void**)&d_woff, ((size_t)n + 1) * 8));
+		RFOK(hipMalloc((void**)&d_cap, (size_t)n * 8));
+		RFOK(hipMalloc((void**)&d_capoff, (size_t)n * 8));
+		RFOK(hipMalloc((void**)&d_wend, (size_t)n * 8));
 		RFOK(hipMalloc((void**)&d_poff, ((size_t)n + 1) * 8));
 		RFOK(hipMalloc((void**)&d_rc, sizeof(RConst)));
 		RFOK(hipMemcpyAsync(d_jobs, jobs.data(), (size_t)n * sizeof(Job), hipMemcpyHostToDevice, s));
 		RFOK(hipMemcpyAsync(d_qoff, qoff.data(), qoff.size() * 8, hipMemcpyHostToDevice, s));
 		if (!qsk.empty()) RFOK(hipMemcpyAsync(d_qsk, qsk.data(), qsk.size() * 8, hipMemcpyHostToDevice, s));
 		RFOK(hipMemcpyAsync(d_rc, &rc, sizeof(RConst), hipMemcpyHostToDevice, s));
-		// the shared window k-mers: count, offsets, write, sort per job
-		hipLaunchKernelGGL(window_hits_kernel, dim3(n), dim3(256), 0, s, d_jobs, d_g, d_qoff, d_qsk, kmer, min_aa_len,
-				d_rc, d_cnt, (const int64_t*)nullptr, (uint64_t*)nullptr);
+		// the shared window k-mers of every job, in one sweep: each job writes at most its room
+		// (its window length, at most 2,048, a few hundred being usual); a job with more is swept
+		// again with room for all of them
+		for (int32_t j = 0; j < n; ++j) cap[j] = std::min<int64_t>(std::max<int32_t>(jobs[j].len, 1), 2048);
+		int64_t n_cap = 0;
+		for (int32_t j = 0; j < n; ++j) cap_off[j] = n_cap, n_cap += cap[j];
+		if (n_cap > INT32_MAX / 2) { nsh_set_error("refine: %lld window k-mer slots exceed the sort's range", (long long)n_cap); goto done; }
+		RFOK(hipMalloc((void**)&d_w, (size_t)(n_cap + 1) * 8));
+		RFOK(hipMemcpyAsync(d_cap, cap.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
+		RFOK(hipMemcpyAsync(d_capoff, cap_off.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
+		hipLaunchKernelGGL(window_hits_kernel, dim3(n), dim3(256), 0, s, d_jobs, (const int32_t*)nullptr, d_g, d_qoff,
+				d_qsk, kmer, min_aa_len, d_rc, d_cnt, (const int64_t*)d_capoff, (const int64_t*)d_cap, d_w);
 		RFOK(hipGetLastError());
 		RFOK(hipMemcpyAsync(cnt.data(), d_cnt, (size_t)n * 8, hipMemcpyDeviceToHost, s));
 		RFOK(hipStreamSynchronize(s));
-		for (int32_t j = 0; j < n; ++j) woff[j + 1] = woff[j] + cnt[j];
-		const int64_t n_w = woff[n];
-		RFOK(hipMemcpyAsync(d_woff, woff.data(), ((size_t)n + 1) * 8, hipMemcpyHostToDevice, s));
-		RFOK(hipMalloc((void**)&d_w, (size_t)(n_w + 1) * 8));
-		RFOK(hipMalloc((void**)&d_ws, (size_t)(n_w + 1) * 8));
-		hipLaunchKernelGGL(window_hits_kernel, dim3(n), dim3(256), 0, s, d_jobs, d_g, d_qoff, d_qsk, kmer, min_aa_len,
-				d_rc, d_cnt, (const int64_t*)d_woff, d_w);
-		RFOK(hipGetLastError());
-		if (!sort_segments(d_w, d_ws, n_w, d_woff)) { nsh_set_error("refine: sorting the window k-mers failed"); goto done; }
+		nsh_count_launch(1);
+		{   // jobs with more than their room: swept again into room appended after everything else
+			std::vector<int32_t> over;
+			int64_t extra = 0;
+			for (int32_t j = 0; j < n; ++j)
+				if (cnt[j] > cap[j]) over.push_back(j), cap_off[j] = n_cap + extra, cap[j] = cnt[j], extra += cnt[j];
+			if (!over.empty()) {
+				if (n_cap + extra > INT32_MAX / 2) { nsh_set_error("refine: too many window k-mers"); goto done; }
+				uint64_t *w2 = nullptr;
+				RFOK(hipMalloc((void**)&w2, (size_t)(n_cap + extra + 1) * 8));
+				if (hipMemcpyAsync(w2, d_w, (size_t)n_cap * 8, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+					(void)hipFree(w2);
+					nsh_set_error("refine: copying the window k-mers failed");
+					goto done;
+				}
+				(void)hipStreamSynchronize(s);
+				(void)hipFree(d_w);
+				d_w = w2, n_cap += extra;
+				RFOK(hipMalloc((void**)&d_over, over.size() * 4));
+				RFOK(hipMemcpyAsync(d_over, over.data(), over.size() * 4, hipMemcpyHostToDevice, s));
+				RFOK(hipMemcpyAsync(d_cap, cap.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
+				RFOK(hipMemcpyAsync(d_capoff, cap_off.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
+				hipLaunchKernelGGL(window_hits_kernel, dim3((unsigned)over.size()), dim3(256), 0, s, d_jobs,
+						(const int32_t*)d_over, d_g, d_qoff, d_qsk, kmer, min_aa_len, d_rc, d_cnt, (const int64_t*)d_capoff,
+						(const int64_t*)d_cap, d_w);
+				RFOK(hipGetLastError());
+				RFOK(hipMemcpyAsync(cnt.data(), d_cnt, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+				RFOK(hipStreamSynchronize(s));
+				nsh_count_launch(1);
+				for (int32_t j : over)
+					if (cnt[j] != cap[j]) { nsh_set_error("refine: window k-mer counts changed between sweeps"); goto done; }
+			}
+		}
+		clk.lap("window k-mers", n);
+		for (int32_t j = 0; j < n; ++j) wend[j] = cap_off[j] + cnt[j];
+		RFOK(hipMemcpyAsync(d_wend, wend.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
+		RFOK(hipMalloc((void**)&d_ws, (size_t)(n_cap + 1) * 8));
+		if (!sort_segments(d_w, d_ws, n_cap, d_capoff, d_wend)) { nsh_set_error("refine: sorting the window k-mers failed"); goto done; }
+		RFOK(hipStreamSynchronize(s));
+		clk.lap("sort window k-mers");
 		// the bucket pairs: count, offsets, write, sort per job
-		hipLaunchKernelGGL(pairs_kernel, dim3(n), dim3(256), 0, s, d_jobs, d_woff, d_ws, d_qoff, d_qsk, max_ava, d_cnt,
-				(const int64_t*)nullptr, (uint64_t*)nullptr);
+		hipLaunchKernelGGL(pairs_kernel, dim3(n), dim3(256), 0, s, d_jobs, (const int64_t*)d_capoff,
+				(const int64_t*)d_wend, d_ws, d_qoff, d_qsk, max_ava, d_cnt, (const int64_t*)nullptr, (uint64_t*)nullptr);
 		RFOK(hipGetLastError());
 		RFOK(hipMemcpyAsync(pc.data(), d_cnt, (size_t)n * 8, hipMemcpyDeviceToHost, s));
 		RFOK(hipStreamSynchronize(s));
 		for (int32_t j = 0; j < n; ++j) p_off[j + 1] = p_off[j] + pc[j];
 		const int64_t n_p = p_off[n];
+		if (n_pairs) *n_pairs = n_p;
 		RFOK(hipMemcpyAsync(d_poff, p_off.data(), ((size_t)n + 1) * 8, hipMemcpyHostToDevice, s));
 		RFOK(hipMalloc((void**)&d_p, (size_t)(n_p + 1) * 8));
 		RFOK(hipMalloc((void**)&d_ps, (size_t)(n_p + 1) * 8));
-		hipLaunchKernelGGL(pairs_kernel, dim3(n), dim3(256), 0, s, d_jobs, d_woff, d_ws, d_qoff, d_qsk, max_ava, d_cnt,
-				(const int64_t*)d_poff, d_p);
+		hipLaunchKernelGGL(pairs_kernel, dim3(n), dim3(256), 0, s, d_jobs, (const int64_t*)d_capoff,
+				(const int64_t*)d_wend, d_ws, d_qoff, d_qsk, max_ava, d_cnt, (const int64_t*)d_poff, d_p);
 		RFOK(hipGetLastError());
-		if (!sort_segments(d_p, d_ps, n_p, d_poff)) { nsh_set_error("refine: sorting the anchor pairs failed"); goto done; }
-		nsh_count_launch(4);
+		if (!sort_segments(d_p, d_ps, n_p, d_poff, d_poff + 1)) { nsh_set_error("refine: sorting the anchor pairs failed"); goto done; }
+		RFOK(hipStreamSynchronize(s));
+		clk.lap("pairs and their sort", p_off[n]);
+		nsh_count_launch(3);
 		// mp_chain of every job's pairs (map.c:203), then the chains and their anchors to the host
 		RFOK(hipMalloc((void**)&d_u, (size_t)(n_p + 1) * 8));
 		RFOK(hipMalloc((void**)&d_ao, (size_t)(n_p + 1) * 8));
@@ -303,19 +386,35 @@ int nsh_refine::run(const uint8_t *d_g, const std::vector<Job> &jobs, const std:
 		if (n_p > 0) {
 			if (!nsh_chain::run(s, n, d_poff, nullptr, pc.data(), d_ps, n_p, cp, d_nu, d_u, d_ao)) goto done;
 		} else RFOK(hipMemsetAsync(d_nu, 0, (size_t)n * 4, s));
-		u.resize((size_t)n_p), ao.resize((size_t)n_p);
-		RFOK(hipMemcpyAsync(n_u.data(), d_nu, (size_t)n * 4, hipMemcpyDeviceToHost, s));
-		if (n_p > 0) {
-			if (!nsh_copy_to_host(u.data(), d_u, (size_t)n_p * 8, s)) goto done;
-			if (!nsh_copy_to_host(ao.data(), d_ao, (size_t)n_p * 8, s)) goto done;
+		clk.lap("chains");
+		// the chosen chain of every job and its anchors, compacted on the device
+		RFOK(hipMalloc((void**)&d_best, (size_t)n * 16));
+		RFOK(hipMalloc((void**)&d_compact, (size_t)(n_p + 1) * 8));
+		RFOK(hipMalloc((void**)&d_total, 8));
+		RFOK(hipMemsetAsync(d_total, 0, 8, s));
+		hipLaunchKernelGGL(best_chain_kernel, dim3(n), dim3(64), 0, s, (const int64_t*)d_poff, (const int32_t*)d_nu,
+				(const uint64_t*)d_u, (const uint64_t*)d_ao, d_total, d_best, d_compact);
+		RFOK(hipGetLastError());
+		nsh_count_launch(1);
+		{
+			unsigned long long n_best = 0;
+			std::vector<int64_t> b2((size_t)2 * n);
+			RFOK(hipMemcpyAsync(&n_best, d_total, 8, hipMemcpyDeviceToHost, s));
+			RFOK(hipMemcpyAsync(b2.data(), d_best, (size_t)n * 16, hipMemcpyDeviceToHost, s));
+			RFOK(hipStreamSynchronize(s));
+			for (int32_t j = 0; j < n; ++j) best_off[j] = b2[2 * j], best_u[j] = (uint64_t)b2[2 * j + 1];
+			anchors.resize((size_t)n_best);
+			if (n_best && !nsh_copy_to_host(anchors.data(), d_compact, (size_t)n_best * 8, s)) goto done;
 		}
 		RFOK(hipStreamSynchronize(s));
+		clk.lap("results to the host");
 		ok = 1;
 	}
 done:
 	if (s) (void)hipStreamSynchronize(s);
 	{
-		void *bufs[] = { d_jobs, d_qoff, d_cnt, d_woff, d_poff, d_qsk, d_w, d_ws, d_p, d_ps, d_u, d_ao, d_nu, d_tmp, d_rc };
+		void *bufs[] = { d_jobs, d_qoff, d_cnt, d_cap, d_capoff, d_wend, d_poff, d_qsk, d_w, d_ws, d_p, d_ps, d_u, d_ao, d_nu,
+			d_tmp, d_rc, d_best, d_compact, d_total, d_over };
 		for (void *b : bufs) if (b) (void)hipFree(b);
 	}
 	if (s) (void)hipStreamDestroy(s);
