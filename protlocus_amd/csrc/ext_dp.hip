@@ -581,7 +581,10 @@ static int plan_launch(nsh_ext_plan_t *p, const ExtArgs *args, const int32_t *gr
 	if (clear_out) hipLaunchKernelGGL(clear_out_kernel, dim3((p->n + 255) / 256), dim3(256), 0, p->stream, p->d_out, p->n);
 	HIPOK(hipGetLastError());
 	HIPOK(hipEventRecord(p->ev1, p->stream));
-	for (int g = 0; g < kGroups; ++g) {
+	// the widest problems first: each group's persistent waves take the CUs in launch order, and a
+	// group started after group 0 (the most waves) would wait for its queue to drain
+	if (wide && !launch_wide(p, *wide, d_wide)) goto fail;
+	for (int g = kGroups - 1; g >= 0; --g) {
 		if (grid[g] <= 0) continue;
 		HIPOK(hipStreamWaitEvent(p->gs[g], p->ev1, 0));
 		launch_group(g, args[g], grid[g], p->gs[g]);
@@ -590,7 +593,6 @@ static int plan_launch(nsh_ext_plan_t *p, const ExtArgs *args, const int32_t *gr
 		HIPOK(hipStreamWaitEvent(p->stream, p->gev[g], 0));
 		nsh_count_launch(1);
 	}
-	if (wide && !launch_wide(p, *wide, d_wide)) goto fail;
 	HIPOK(hipEventRecord(p->ev2, p->stream));
 	nsh_count_launch(2);
 	p->ran = 1;

@@ -324,6 +324,8 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, 
 	}
 }
 
+constexpr int64_t kLongWork = (int64_t)1 << 18;   // rows x 128-column chunks of a long problem
+
 // persistent waves over one group's queue; CLO..CHI columns per lane per half. The waves-per-SIMD
 // target trades registers for latency hiding (measured: 6 / 3 / 1 for C <= 4 / <= 8 / <= 16).
 template <int CLO, int CHI, bool PROF>
@@ -348,6 +350,11 @@ ext_dp_kernel(ExtArgs a)
 		if (p >= a.n) break;
 		const ExtProb P = a.prob[p];
 		const int C = P.C < CLO? CLO : P.C > CHI? CHI : P.C;
+		// a problem of many rows (an extension over a window of tens of kilobases) sets the length of
+		// the whole launch: its wave gets the issue priority over the waves of short problems sharing
+		// its SIMD, which fill the cycles it leaves
+		if ((int64_t)P.nl * C >= kLongWork) __builtin_amdgcn_s_setprio(2);
+		else __builtin_amdgcn_s_setprio(0);
 #define NSH_C(K) if (K >= CLO && K <= CHI && C == K) { ext_solve<(K >= CLO && K <= CHI)? K : CLO, PROF>(a, P, prof, snap, dring, L); continue; }
 		NSH_C(1) NSH_C(2) NSH_C(3) NSH_C(4) NSH_C(5) NSH_C(6) NSH_C(7) NSH_C(8)
 		NSH_C(9) NSH_C(10) NSH_C(11) NSH_C(12) NSH_C(13) NSH_C(14) NSH_C(15) NSH_C(16)
