@@ -71,7 +71,21 @@ struct Cand {
 
 // one DP of mp_align_seq (align.c:103-145): an anchor-to-anchor step of exactly alen codons and at
 // most kmer2 residues is scored without a DP; anything else goes to the global DP with CIGAR
-struct Task { Cand *c; int64_t nt_off; int32_t nlen, aa_off, alen; int32_t score; std::vector<uint32_t> cig; bool dp; };
+struct Task {
+	Cand *c;
+	int64_t nt_off;
+	int32_t nlen, aa_off, alen;
+	int32_t score;
+	int64_t cig_off;      // a DP's CIGAR: n_cig ops at cig_off of its round's CIGAR array
+	int32_t n_cig;
+	bool dp;
+};
+// a stage's DP tasks, candidate after candidate, and the CIGARs of its DPs
+struct TaskSet {
+	std::vector<Task> t;
+	std::vector<int64_t> first;   // tasks of candidate k: t[first[k] .. first[k + 1])
+	std::vector<uint32_t> cig;
+};
 
 }   // namespace
 
@@ -144,43 +158,55 @@ static nsh_ext_req ext_req(const Cand *c, int64_t nt_off, int32_t nl, int32_t aa
 	return q;
 }
 
-// the DPs of a stage: ungapped steps on the host, the others as one engine batch
-static int dp_round(std::vector<Task> &tasks, const ns_opt_t &o, const nsh_align_opt_t *ao)
+// the DPs of a stage: ungapped steps on the host threads, the others as one global-DP batch whose
+// bases stay in HBM
+static int dp_round(TaskSet &ts, const ns_opt_t &o, const nsh_align_opt_t *ao, int32_t nth)
 {
-	std::vector<int32_t> idx;
-	for (size_t t = 0; t < tasks.size(); ++t) {
-		Task &k = tasks[t];
-		const Cand *c = k.c;
-		k.dp = !(k.nlen == k.alen * 3 && k.alen <= ao->kmer2);
-		if (k.dp) { idx.push_back((int32_t)t); continue; }
-		// mp_score_ungapped (align.c:36-46): codons with an N score as X. Its loop bound is the
-		// residue count while its index steps over bases (i < alen, i += 3), so only the first
-		// ceil(alen / 3) residues of the step are scored; the same bound is kept here.
-		int32_t s = 0;
-		const uint8_t *nseq = c->nt + k.nt_off;
-		for (int32_t i = 0, j = 0; i < k.alen; i += 3, ++j) {
-			const uint8_t b0 = nseq[i], b1 = nseq[i + 1], b2 = nseq[i + 2];
-			const int32_t ta = b0 > 3 || b1 > 3 || b2 > 3? ns_tab_aa20['X'] : ns_tab_codon[b0 << 4 | b1 << 2 | b2];
-			s += ao->mat[ta * 22 + ns_tab_aa20[(uint8_t)c->aa[k.aa_off + j]]];
-		}
-		k.score = s;
-		k.cig.assign(1, (uint32_t)k.alen << 4 | NS_CIGAR_M);
-	}
 	StepClock clk;
-	clk.lap("  ungapped steps", (int64_t)(tasks.size() - idx.size()));
+	std::vector<Task> &tasks = ts.t;
+	const int64_t nt = (int64_t)tasks.size();
+	nsh_parallel_for((nt + 4095) / 4096, nth, [&](int64_t b) {
+		for (int64_t t = b * 4096; t < std::min(nt, (b + 1) * 4096); ++t) {
+			Task &k = tasks[t];
+			const Cand *c = k.c;
+			k.dp = !(k.nlen == k.alen * 3 && k.alen <= ao->kmer2);
+			k.n_cig = 0, k.cig_off = 0;
+			if (k.dp) continue;
+			// mp_score_ungapped (align.c:36-46): codons with an N score as X. Its loop bound is the
+			// residue count while its index steps over bases (i < alen, i += 3), so only the first
+			// ceil(alen / 3) residues of the step are scored; the same bound is kept here.
+			int32_t sc = 0;
+			const uint8_t *nseq = c->nt + k.nt_off;
+			for (int32_t i = 0, j = 0; i < k.alen; i += 3, ++j) {
+				const uint8_t b0 = nseq[i], b1 = nseq[i + 1], b2 = nseq[i + 2];
+				const int32_t ta = b0 > 3 || b1 > 3 || b2 > 3? ns_tab_aa20['X'] : ns_tab_codon[b0 << 4 | b1 << 2 | b2];
+				sc += ao->mat[ta * 22 + ns_tab_aa20[(uint8_t)c->aa[k.aa_off + j]]];
+			}
+			k.score = sc;
+		}
+	});
+	std::vector<int64_t> idx;
+	for (int64_t t = 0; t < nt; ++t) if (tasks[t].dp) idx.push_back(t);
+	clk.lap("  ungapped steps", nt - (int64_t)idx.size());
+	ts.cig.clear();
 	if (idx.empty()) return 1;
 	const int32_t n = (int32_t)idx.size();
 	// the bases stay in HBM (offsets into the oriented genome); only the protein pieces are packed
 	std::vector<int64_t> ns_off(n, 0), g_off(n), as_off(n);
 	std::vector<int32_t> nl(n), al(n), score(n), ncig(n);
-	std::vector<char> as;
+	int64_t l_as = 0;
 	for (int32_t k = 0; k < n; ++k) {
 		const Task &t = tasks[idx[k]];
-		g_off[k] = t.c->g_off + t.nt_off, as_off[k] = (int64_t)as.size();
-		as.insert(as.end(), t.c->aa + t.aa_off, t.c->aa + t.aa_off + t.alen);
+		g_off[k] = t.c->g_off + t.nt_off, as_off[k] = l_as, l_as += t.alen;
 		nl[k] = t.nlen, al[k] = t.alen;
 	}
-	as.push_back(0);
+	std::vector<char> as((size_t)l_as + 1);
+	nsh_parallel_for((n + 4095) / 4096, nth, [&](int64_t b) {
+		for (int32_t k = (int32_t)(b * 4096); k < std::min<int64_t>(n, (b + 1) * 4096); ++k) {
+			const Task &t = tasks[idx[k]];
+			std::memcpy(as.data() + as_off[k], t.c->aa + t.aa_off, (size_t)t.alen);
+		}
+	});
 	clk.lap("  DP inputs", n);
 	uint32_t *cig = nullptr;
 	ns_opt_t oc = o;
@@ -193,12 +219,26 @@ static int dp_round(std::vector<Task> &tasks, const ns_opt_t &o, const nsh_align
 	int64_t off = 0;
 	for (int32_t k = 0; k < n; ++k) {
 		Task &t = tasks[idx[k]];
-		t.score = score[k];
-		t.cig.assign(cig + off, cig + off + ncig[k]);
+		t.score = score[k], t.cig_off = off, t.n_cig = ncig[k];
 		off += ncig[k];
 	}
+	ts.cig.assign(cig, cig + off);
 	std::free(cig);
 	return 1;
+}
+
+// each candidate's task results, in its order: scores added, CIGARs appended (ns_push_cigar's merge)
+static void merge_tasks(const TaskSet &ts, const std::vector<Cand*> &cs, int32_t nth)
+{
+	nsh_parallel_for((int64_t)cs.size(), nth, [&](int64_t k) {
+		Cand *c = cs[k];
+		for (int64_t t = ts.first[k]; t < ts.first[k + 1]; ++t) {
+			const Task &x = ts.t[t];
+			c->score += x.score;
+			if (!x.dp) push_op(c->cigar, NS_CIGAR_M, x.alen);
+			else for (int32_t j = 0; j < x.n_cig; ++j) push_op(c->cigar, ts.cig[x.cig_off + j] & 0xf, (int32_t)(ts.cig[x.cig_off + j] >> 4));
+		}
+	});
 }
 
 extern "C" int nsh_align_stage(int32_t n_ctg, const uint8_t *nt4, const int64_t *ctg_off, const int64_t *ctg_len,
@@ -306,31 +346,39 @@ int nsh_align_stage_view(const nsh_genome_view &g, const char *const *aa, const 
 		if (res[k].aa_len == who[k]->as1_win) who[k]->nt_len = res[k].nt_len, who[k]->aa_len = res[k].aa_len;
 	clk.lap("left extensions", (int64_t)who.size());
 	// MP_AS_GAP: the DPs between consecutive trusted anchors (mp_align_gap, align.c:482-540)
-	std::vector<Task> tasks;
-	for (Cand *c : live) {
+	const int32_t nth = 16;
+	TaskSet ts;
+	ts.first.assign(live.size() + 1, 0);
+	nsh_parallel_for((int64_t)live.size(), nth, [&](int64_t k) {   // tasks per candidate
+		const Cand *c = live[k];
+		int64_t m = 0;
+		for (int32_t i = c->i0; i < c->r->cnt; ++i) m += (c->a[i] & kTrusted) != 0;
+		ts.first[k + 1] = m;
+	});
+	for (size_t k = 0; k < live.size(); ++k) ts.first[k + 1] += ts.first[k];
+	ts.t.resize((size_t)ts.first[live.size()]);
+	nsh_parallel_for((int64_t)live.size(), nth, [&](int64_t k) {
+		Cand *c = live[k];
 		nsh_align_reg_t *r = c->r;
 		r->vs = c->vs1 - c->nt_len;
 		r->qs = c->as1 - c->aa_len;
 		c->ne0 = (int32_t)(r->vs - c->vs0);
 		c->ae0 = r->qs;
+		int64_t w = ts.first[k];
 		for (int32_t i = c->i0; i < r->cnt; ++i) {
 			if (!(c->a[i] & kTrusted)) continue;
 			const int32_t ne1 = (int32_t)anchor_x(c->a[i]) + 1, ae1 = anchor_q(c->a[i]) + 1;
-			Task t;
+			Task &t = ts.t[w++];
 			t.c = c, t.nt_off = c->ne0 + c->vs0 - c->as, t.nlen = ne1 - c->ne0, t.aa_off = c->ae0, t.alen = ae1 - c->ae0;
 			t.score = 0, t.dp = false;
-			tasks.push_back(std::move(t));
 			c->i0 = i, c->ne0 = ne1, c->ae0 = ae1;
 		}
 		r->ve = c->ne0 + c->vs0, r->qe = c->ae0;
-	}
-	clk.lap("gap tasks", (int64_t)tasks.size());
-	if (!dp_round(tasks, o, ao)) return 0;
-	clk.lap("gap DPs", (int64_t)tasks.size());
-	for (Task &t : tasks) {
-		t.c->score += t.score;
-		for (uint32_t op : t.cig) push_op(t.c->cigar, op & 0xf, (int32_t)(op >> 4));
-	}
+	});
+	clk.lap("gap tasks", (int64_t)ts.t.size());
+	if (!dp_round(ts, o, ao, nth)) return 0;
+	merge_tasks(ts, live, nth);
+	clk.lap("gap DPs", (int64_t)ts.t.size());
 	// MP_AS_RIGHT: right extensions where protein and window remain
 	req.clear(), who.clear();
 	for (Cand *c : live) {
@@ -361,20 +409,19 @@ int nsh_align_stage_view(const nsh_genome_view &g, const char *const *aa, const 
 		if (res[k].aa_len == rwho[k]->aa_win) rwho[k]->nt_len = res[k].nt_len, rwho[k]->aa_len = res[k].aa_len;
 	clk.lap("right extensions", (int64_t)rwho.size());
 	// MP_AS_RIGHT_APPLY: the DP over the right extension
-	tasks.clear();
-	for (Cand *c : who) {
+	ts.t.assign(who.size(), Task{});
+	ts.first.assign(who.size() + 1, 0);
+	for (size_t k = 0; k < who.size(); ++k) {
+		Cand *c = who[k];
 		const nsh_align_reg_t *r = c->r;
-		Task t;
+		Task &t = ts.t[k];
 		t.c = c, t.nt_off = r->ve - c->as, t.nlen = c->nt_len, t.aa_off = r->qe, t.alen = c->aa_len, t.score = 0, t.dp = false;
-		tasks.push_back(std::move(t));
+		ts.first[k + 1] = (int64_t)k + 1;
 	}
-	if (!dp_round(tasks, o, ao)) return 0;
-	for (Task &t : tasks) {
-		t.c->score += t.score;
-		for (uint32_t op : t.cig) push_op(t.c->cigar, op & 0xf, (int32_t)(op >> 4));
-		t.c->r->ve += t.c->nt_len, t.c->r->qe += t.c->aa_len;
-	}
-	clk.lap("right DPs", (int64_t)tasks.size());
+	if (!dp_round(ts, o, ao, nth)) return 0;
+	merge_tasks(ts, who, nth);
+	for (Cand *c : who) c->r->ve += c->nt_len, c->r->qe += c->aa_len;
+	clk.lap("right DPs", (int64_t)ts.t.size());
 	// MP_AS_FINISH: the CIGARs, one after another
 	int64_t tot = 0;
 	for (Cand *c : live) tot += (int64_t)c->cigar.size();

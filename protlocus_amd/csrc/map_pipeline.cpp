@@ -20,7 +20,6 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
-#include <thread>
 #include <vector>
 #include "nsh_internal.h"
 
@@ -356,21 +355,6 @@ int32_t chn_sc_ungap(const std::vector<uint64_t> &a, int32_t kmer)
 	return x;
 }
 
-template <class F> void parallel_for(int64_t n, int32_t n_threads, F f)
-{
-	if (n <= 0) return;
-	const int32_t nt = (int32_t)std::max<int64_t>(1, std::min<int64_t>(n_threads, n));
-	if (nt == 1) {
-		for (int64_t i = 0; i < n; ++i) f(i);
-		return;
-	}
-	std::atomic<int64_t> next(0);
-	std::vector<std::thread> th;
-	for (int32_t t = 0; t < nt; ++t)
-		th.emplace_back([&] { for (int64_t i; (i = next.fetch_add(1)) < n;) f(i); });
-	for (auto &x : th) x.join();
-}
-
 // what the align stage leaves for the output of one aligned candidate
 struct Out {
 	nsh_aln_stats_t st;
@@ -424,7 +408,7 @@ extern "C" nsh_mapper_t *nsh_mapper_init(int32_t n_ctg, const char *const *ctg_n
 	for (auto &s : m->name) m->name_p.push_back(s.c_str());
 	m->nt.resize((size_t)(2 * tot));
 	std::memcpy(m->nt.data(), nt4, (size_t)tot);
-	parallel_for(n_ctg, 16, [&](int64_t c) {
+	nsh_parallel_for(n_ctg, 16, [&](int64_t c) {
 		nsh_strand_bases(nt4, m->off.data(), m->len.data(), (uint32_t)c << 1 | 1, 0, m->len[c], m->nt.data() + tot + m->off[c]);
 	});
 	m->total = tot;
@@ -496,7 +480,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 	const int32_t sel_n = o->best_n;
 	const float sel_ratio = o->pri_ratio;
 	std::atomic<bool> bad(false);
-	parallel_for(n_q, nth, [&](int64_t q) {   // mp_reg_gen_from_block, sort, parents, selection (map.c:634-639)
+	nsh_parallel_for(n_q, nth, [&](int64_t q) {   // mp_reg_gen_from_block, sort, parents, selection (map.c:634-639)
 		if (!regs_from_chains(m->bo, m->bbit, m->kmer, (int32_t)(u_off[q + 1] - u_off[q]), cu.data() + u_off[q],
 				ca.data() + c_off[q], pool[q], regs[q])) { bad = true; return; }
 		sort_reg(pool[q], regs[q]);
@@ -517,7 +501,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 		for (int32_t i = 0; i < (int32_t)regs[q].size(); ++i) jobs.push_back({ q, i, 0 });
 	}
 	st[1] = (int64_t)jobs.size();
-	parallel_for(n_q, nth, [&](int64_t q) {
+	nsh_parallel_for(n_q, nth, [&](int64_t q) {
 		sketch_prot(seq[q], qlen[q], kmer2, 0, qsk[q]);
 		std::sort(qsk[q].begin(), qsk[q].end());
 	});
@@ -547,7 +531,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 	if (!nsh_refine::run(m->d_g, rjobs, qoff, qall, kmer2, m->min_aa_len, o->max_ava, rcp, best_u, best_off, best_a,
 			&st[2]))
 		return 0;
-	parallel_for((int64_t)jobs.size(), nth, [&](int64_t t) {   // the best chain becomes the region (map.c:249-258)
+	nsh_parallel_for((int64_t)jobs.size(), nth, [&](int64_t t) {   // the best chain becomes the region (map.c:249-258)
 		const RefJob &jb = jobs[t];
 		Reg &r = pool[jb.q][regs[jb.q][jb.r]];
 		const int32_t n_a = (int32_t)(uint32_t)best_u[t];
@@ -565,7 +549,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 			r.a[i] = (uint64_t)((int64_t)(a[i] >> 32) + jb.as - r.vs) << 32 | (a[i] & 0xffffffffULL);
 		r.chn_sc_ungap = chn_sc_ungap(r.a, kmer2);
 	});
-	parallel_for(n_q, nth, [&](int64_t q) {   // map.c:685-698
+	nsh_parallel_for(n_q, nth, [&](int64_t q) {   // map.c:685-698
 		Regs kept;
 		for (int32_t i : regs[q])
 			if (pool[q][i].cnt > 0) kept.push_back(i);
@@ -606,7 +590,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 	so.go = o->dp.go, so.ge = o->dp.ge, so.fs = o->dp.fs, so.sc = o->dp.mat;
 	std::vector<Out> res(cand.size());
 	std::atomic<bool> fail(false);
-	parallel_for((int64_t)cand.size(), nth, [&](int64_t k) {   // mp_align_finish: statistics, rank score
+	nsh_parallel_for((int64_t)cand.size(), nth, [&](int64_t k) {   // mp_align_finish: statistics, rank score
 		const nsh_align_reg_t &c = cand[k];
 		if (!c.status) return;
 		const uint8_t *win = m->nt.data() + m->view.at(c.vid, c.win_as);
@@ -631,7 +615,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 	// ---- after the DP: ranking and selection per query (map.c:840-866), then the records (map.c:1143-1166)
 	std::vector<std::string> text(n_q);
 	std::vector<int32_t> n_out(n_q, 0);
-	parallel_for(n_q, nth, [&](int64_t q) {
+	nsh_parallel_for(n_q, nth, [&](int64_t q) {
 		Regs kept;
 		for (int32_t i : regs[q]) {
 			Reg &r = pool[q][i];
@@ -670,7 +654,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 		}
 		id0[q + 1] = id0[q] + (int64_t)emit[q].size();
 	}
-	parallel_for(n_q, nth, [&](int64_t q) {
+	nsh_parallel_for(n_q, nth, [&](int64_t q) {
 		std::string &s = text[q];
 		std::vector<char> buf(1 << 16);
 		for (size_t e = 0; e < emit[q].size(); ++e) {

@@ -2,8 +2,12 @@
 #ifndef NSH_INTERNAL_H
 #define NSH_INTERNAL_H
 
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstddef>
+#include <thread>
+#include <vector>
 #include "../../include/nasw-hip.h"
 
 #define NSH_NEG16 (-32768)
@@ -14,6 +18,23 @@ void nsh_set_error(const char *fmt, ...);
 void nsh_count_launch(uint64_t n);
 void nsh_add_times(double h2d, double kernel, double d2h);
 int32_t nsh_host_stripe_width(int32_t flag, int32_t al); // the reference CPU's stripe width on this host
+
+// f(i) for i in [0, n) on up to n_threads host threads (the calling thread included)
+template <class F> void nsh_parallel_for(int64_t n, int32_t n_threads, F f)
+{
+	if (n <= 0) return;
+	const int32_t nt = (int32_t)std::max<int64_t>(1, std::min<int64_t>(n_threads, n));
+	if (nt == 1) {
+		for (int64_t i = 0; i < n; ++i) f(i);
+		return;
+	}
+	std::atomic<int64_t> next(0);
+	auto work = [&] { for (int64_t i; (i = next.fetch_add(1)) < n;) f(i); };
+	std::vector<std::thread> th;
+	for (int32_t t = 1; t < nt; ++t) th.emplace_back(work);
+	work();
+	for (auto &x : th) x.join();
+}
 
 // with NSH_PROFILE set in the environment, each lap() prints the host time since the last one to stderr
 struct nsh_step_clock {
