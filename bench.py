@@ -17,7 +17,10 @@ Beside it, in the same JSON line:
   * hmm: profile-HMM Viterbi scan, 64 profiles (M 100-800) x a 10-Mb genome in 6 frames;
   * profile_search: the profile prescreen search of the same profiles over the same genome;
   * gapless_prefilter: gapless scores and top-K selection of one --gpu-prefilter mini-batch;
-  * index: the k-mer block index of a 100-Mb genome, and the anchors of 20,000 proteins against it.
+  * index: the k-mer block index of a 100-Mb genome, and the anchors of 20,000 proteins against it;
+  * map: the whole map path (nsh_map_batch) on the genome-scale workload, 20,000 proteins against
+    100 Mb with planted gene models, in proteins per second, beside the reference's own program
+    on the host cores, with its records compared.
 Output: one JSON line.
 """
 import argparse
@@ -30,6 +33,10 @@ import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# An extension plan runs its column-count groups on streams of their own, which overlap only on
+# distinct hardware queues; HIP's default of 4 queues per process is shared by every stream the
+# engines create, so give the process more before HIP starts (the runtime reads it once).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, ROOT)
 
