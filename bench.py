@@ -20,7 +20,10 @@ Beside it, in the same JSON line:
   * index: the k-mer block index of a 100-Mb genome, and the anchors of 20,000 proteins against it;
   * map: the whole map path (nsh_map_batch) on the genome-scale workload, 20,000 proteins against
     100 Mb with planted gene models, in proteins per second, beside the reference's own program
-    on the host cores, with its records compared.
+    on the host cores, with its records compared;
+  * map_prescreen: the learned prescreen on the map path's own candidates (trained on a second
+    such workload, labels = which candidates became records), the map path timed plain and
+    prescreened, with the record recall.
 Output: one JSON line.
 """
 import argparse
@@ -620,7 +623,7 @@ def bench_map(cpu=True, log=None, n_prot=20000, batch=20000, sample=2000):
     index_s = time.perf_counter() - t0
     try:
         mp.map(pn[:32], ps[:32])   # warm-up: device pools, streams, code objects
-        recs, stats = [], np.zeros(16, np.int64)
+        recs, stats = [], np.zeros(24, np.int64)
         t0 = time.perf_counter()
         for b0 in range(0, n_prot, batch):
             r, st = mp.map(pn[b0:b0 + batch], ps[b0:b0 + batch])
@@ -831,6 +834,8 @@ def main():
     ap.add_argument("--only-map", action="store_true",
                     help="run only the genome-scale map path section and print its JSON")
     ap.add_argument("--map-batch", type=int, default=20000, help="proteins per nsh_map_batch call")
+    ap.add_argument("--only-map-prescreen", action="store_true",
+                    help="run only the learned prescreen on the map path's candidates and print its JSON")
     ap.add_argument("--only-dropin", action="store_true",
                     help="run only the drop-in section (reference calling patterns from C threads) and print its JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -871,6 +876,10 @@ def main():
         return
     if args.only_map:
         print(json.dumps(bench_map(cpu=not args.no_cpu, log=log, batch=args.map_batch)), flush=True)
+        return
+    if args.only_map_prescreen:
+        from protlocus_amd import prescreen
+        print(json.dumps(prescreen.map_evaluate(log=log)), flush=True)
         return
 
     import nsffi
@@ -986,6 +995,7 @@ def main():
         out["gapless_prefilter"] = bench_gapless_prefilter(cpu=not args.no_cpu, log=log)
         out["index"] = bench_index(cpu=not args.no_cpu, log=log)
         out["map"] = bench_map(cpu=not args.no_cpu, log=log, batch=args.map_batch)
+        out["map_prescreen"] = prescreen.map_evaluate(log=log)
     print(json.dumps(out), flush=True)
 
 

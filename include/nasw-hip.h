@@ -304,12 +304,26 @@ int64_t nsh_align_records(int32_t n_ctg, const uint8_t *nt4, const int64_t *ctg_
  * (*out, *out_len bytes; free() it): PAF, or GFF3/GTF by opt->flag (MP_F_GFF 0x8, MP_F_NO_PAF 0x10,
  * MP_F_GTF 0x20, MP_F_NO_CS 0x200); the "##gff-version 3" line that mp_map_file prints first is the
  * caller's, and unmapped proteins write nothing (MP_F_SHOW_UNMAP is not supported). *id is the
- * output step's running record id (pipeline_t.id), advanced by the records written. stats (16
+ * output step's running record id (pipeline_t.id), advanced by the records written. stats (24
  * entries, may be NULL): [0] coarse chains, [1] refine candidates, [2] refine anchor pairs, [3] DP
- * candidates, [4] proteins with a record, [5] records, [8..13] microseconds since the call began at
- * the end of: the coarse chains, the region selection, the refine sketches and pairs, the refine
- * chains and selection, the align stage, the statistics and records. 1 ok, 0 error (reason in
- * ns_gpu_error). */
+ * candidates, [4] proteins with a record, [5] records, [6] candidates offered to the prescreen,
+ * [7] candidates it dropped, [8..13] microseconds since the call began at the end of: the coarse
+ * chains, the region selection, the refine set-up, the refinement, the align stage, the
+ * statistics and records; [14] rejected candidates kept by the audit, [15] of those, the ones that
+ * became records; [16] microseconds at the end of the prescreen. 1 ok, 0 error (reason in
+ * ns_gpu_error).
+ *
+ * The learned prescreen (no reference counterpart; the reference's --gpu-prefilter sits at the same
+ * point, map.c:1114-1117) runs between the coarse region selection and the refinement when
+ * opt->prescreen is set: each candidate's 64 features (the device feature kernel over the region's
+ * window, at most 24,000 bases, and the protein's first 4,096 residues; slots 46-59 replaced by the
+ * chain's own: score over length and over the query's best, ungapped score, anchors, query
+ * coverage and ends, span, primary flag, rank, candidate count, score over its primary's, gap to
+ * the best), the network's logit, and the cut at ps_tau; a ps_audit fraction of the rejected
+ * (drawn by a hash of ps_seed and the candidate) is kept anyway, and stats[15] counts the audited
+ * ones that became records. cand_feat / cand_label (both optional, room for cand_cap candidates)
+ * receive every offered candidate's features and whether it became a record, in query order: the
+ * training data. */
 typedef struct {
 	nsh_align_opt_t dp;                   /* the DP options, and max_ext, max_intron, kmer2 */
 	int32_t max_occ, max_gap, bw, max_chn_max_skip, max_chn_iter, min_chn_cnt, min_chn_sc;
@@ -320,6 +334,12 @@ typedef struct {
 	int32_t gff_delim;
 	const char *gff_prefix;               /* NULL: "PL" */
 	int32_t n_threads;                    /* host threads of the host steps */
+	struct nsh_mlp_s *prescreen;          /* the learned candidate prescreen (nsh_mlp_*); NULL: none */
+	float ps_tau, ps_audit;               /* keep logit >= ps_tau, and a ps_audit fraction of the rest */
+	uint64_t ps_seed;
+	float *cand_feat;                     /* optional out: 64 features per candidate */
+	uint8_t *cand_label;                  /* optional out: 1 when the candidate became a record */
+	int64_t cand_cap;                     /* room in cand_feat / cand_label, in candidates */
 } nsh_map_opt_t;
 typedef struct nsh_mapper_s nsh_mapper_t;
 void nsh_map_opt_default(nsh_map_opt_t *o);   /* mp_mapopt_init (options.c:42-138), 16 host threads */

@@ -711,7 +711,9 @@ class MapOpt(C.Structure):
                 ("min_chn_sc", C.c_int32), ("chn_coef_log", C.c_float), ("max_ava", C.c_int32), ("best_n", C.c_int32),
                 ("out_n", C.c_int32), ("mask_len", C.c_int32), ("pri_ratio", C.c_float), ("mask_level", C.c_float),
                 ("out_sim", C.c_float), ("out_cov", C.c_float), ("flag", C.c_uint32), ("gff_delim", C.c_int32),
-                ("gff_prefix", C.c_char_p), ("n_threads", C.c_int32)]
+                ("gff_prefix", C.c_char_p), ("n_threads", C.c_int32), ("prescreen", C.c_void_p), ("ps_tau", C.c_float),
+                ("ps_audit", C.c_float), ("ps_seed", C.c_uint64), ("cand_feat", C.c_void_p), ("cand_label", C.c_void_p),
+                ("cand_cap", C.c_int64)]
 
     @classmethod
     def default(cls, **kw):
@@ -744,12 +746,12 @@ class Mapper:
         self.id = C.c_int64(0)
 
     def map(self, names, seqs, opt=None):
-        """records of one batch of proteins (bytes) and the call's 16 counters (nsh_map_batch)"""
+        """records of one batch of proteins (bytes) and the call's 24 counters (nsh_map_batch)"""
         opt = opt or MapOpt.default()
         qn = (C.c_char_p * len(seqs))(*[n.encode() if isinstance(n, str) else n for n in names])
         qs = (C.c_char_p * len(seqs))(*[bytes(x) for x in seqs])
         ql = np.array([len(x) for x in seqs], np.int32)
-        buf, n, st = C.c_void_p(), C.c_int64(), np.zeros(16, np.int64)
+        buf, n, st = C.c_void_p(), C.c_int64(), np.zeros(24, np.int64)
         if not lib().nsh_map_batch(self.p, len(seqs), qn, qs, ql.ctypes.data, C.byref(opt), C.byref(self.id),
                                    C.byref(buf), C.byref(n), st.ctypes.data):
             raise RuntimeError("nsh_map_batch: " + error())

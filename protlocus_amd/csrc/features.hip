@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(NT) feature_kernel(FeatArgs a)
 	__shared__ int8_t smat[22 * 22];
 	const int t = threadIdx.x, c = blockIdx.x, lane = t & 63;
 	const nsh_prep::PrepProb P = a.pp[c];
-	const uint8_t *praw = a.as + P.as_off, *wraw = a.ns + P.ns_off;
+	const uint8_t *praw = a.as + P.as_off, *wraw = (P.from_genome? a.pc->genome : a.ns) + P.ns_off;
 	const int al = P.al, nl = P.nl;
 	if (t < 256) nt4[t] = a.pc->nt4[t], aa20[t] = a.pc->aa20[t];
 	if (t < 64) codon[t] = a.pc->codon[t];
@@ -393,5 +393,59 @@ fail:
 	}
 	if (s) (void)hipStreamDestroy(s);
 	nsh_add_times(0, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(), 0);
+	return ok;
+}
+
+// features of n candidates whose windows are slices of an oriented genome in HBM (d_g + g_off[k],
+// nl[k] bases) and whose proteins are host strings; out: n x 64 on the host
+int nsh_feat::from_genome(const uint8_t *d_g, int32_t n, const int64_t *g_off, const int32_t *nl, const char *as,
+		const int64_t *as_off, const int32_t *al, int64_t l_as, float *out)
+{
+	if (!nsh_ensure_device()) return 0;
+	if (n < 0 || (n > 0 && (!d_g || !g_off || !nl || !as || !as_off || !al || !out))) { nsh_set_error("features: bad arguments"); return 0; }
+	if (n == 0) return 1;
+	std::vector<nsh_prep::PrepProb> pp(n);
+	for (int32_t k = 0; k < n; ++k) {
+		if (al[k] < 1 || al[k] > AL_MAX || nl[k] < 0 || nl[k] > NL_MAX || as_off[k] < 0 || as_off[k] + al[k] > l_as || g_off[k] < 0) {
+			nsh_set_error("features: candidate %d out of range (al 1..%d, nl 0..%d)", k, AL_MAX, NL_MAX);
+			return 0;
+		}
+		pp[k] = {};
+		pp[k].ns_off = g_off[k], pp[k].from_genome = 1, pp[k].as_off = as_off[k], pp[k].nl = nl[k], pp[k].al = al[k];
+	}
+	nsh_prep::PrepConst pc;
+	std::memcpy(pc.nt4, ns_tab_nt4, 256), std::memcpy(pc.aa20, ns_tab_aa20, 256), std::memcpy(pc.codon, ns_tab_codon, 64);
+	pc.genome = d_g;
+	uint8_t *d_as = nullptr;
+	nsh_prep::PrepProb *d_pp = nullptr;
+	nsh_prep::PrepConst *d_pc = nullptr;
+	int8_t *d_mat = nullptr;
+	float *d_out = nullptr;
+	hipStream_t s = nullptr;
+	int ok = 0;
+	{
+		FOK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+		FOK(hipMalloc((void**)&d_as, (size_t)l_as + 4));
+		FOK(hipMalloc((void**)&d_pp, (size_t)n * sizeof(nsh_prep::PrepProb)));
+		FOK(hipMalloc((void**)&d_pc, sizeof(pc)));
+		FOK(hipMalloc((void**)&d_out, (size_t)n * NF * 4));
+		FOK(hipMalloc((void**)&d_mat, 22 * 22));
+		FOK(hipMemcpyAsync(d_mat, ns_mat_blosum62, 22 * 22, hipMemcpyHostToDevice, s));
+		FOK(hipMemcpyAsync(d_as, as, (size_t)l_as, hipMemcpyHostToDevice, s));
+		FOK(hipMemcpyAsync(d_pp, pp.data(), (size_t)n * sizeof(nsh_prep::PrepProb), hipMemcpyHostToDevice, s));
+		FOK(hipMemcpyAsync(d_pc, &pc, sizeof(pc), hipMemcpyHostToDevice, s));
+		nsh_feat::launch(d_pp, n, nullptr, d_as, d_pc, d_mat, d_out, s);
+		FOK(hipGetLastError());
+		FOK(hipMemcpyAsync(out, d_out, (size_t)n * NF * 4, hipMemcpyDeviceToHost, s));
+		FOK(hipStreamSynchronize(s));
+		nsh_count_launch(1);
+		ok = 1;
+	}
+fail:
+	{
+		void *bufs[] = { d_as, d_pp, d_pc, d_mat, d_out };
+		for (void *b : bufs) if (b) (void)hipFree(b);
+	}
+	if (s) (void)hipStreamDestroy(s);
 	return ok;
 }

@@ -16,12 +16,14 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 #include "nsh_internal.h"
+#include "prep.h"
 
 namespace {
 
@@ -116,6 +118,7 @@ struct Reg {
 	int64_t vs = 0, ve = 0, off = 0;   // off: the region's first anchor among the query's coarse anchors
 	std::vector<uint64_t> a;           // refined anchors: x relative to vs << 32 | query position
 	bool p = false;                    // aligned (mp_reg1_t.p != NULL)
+	bool audited = false;              // rejected by the prescreen but kept by its audit
 	int32_t n_exon = 0, slot = -1;     // slot: the candidate's index in the align stage
 };
 
@@ -387,6 +390,7 @@ extern "C" void nsh_map_opt_default(nsh_map_opt_t *o)
 	o->pri_ratio = 0.7f, o->mask_level = 0.5f, o->out_sim = 0.99f, o->out_cov = 0.1f;
 	o->flag = 0, o->gff_delim = -1, o->gff_prefix = nullptr;
 	o->n_threads = 16;
+	o->ps_audit = 0.02f;
 }
 
 extern "C" nsh_mapper_t *nsh_mapper_init(int32_t n_ctg, const char *const *ctg_name, const uint8_t *nt4,
@@ -440,7 +444,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 	*out = nullptr, *out_len = 0;
 	const int32_t nth = o->n_threads > 0? o->n_threads : 1;
 	const int32_t kmer2 = o->dp.kmer2;
-	int64_t st[16] = { 0 };
+	int64_t st[24] = { 0 };
 	const auto t_start = std::chrono::steady_clock::now();
 	auto lap = [&](int k) {   // microseconds since the call began, at the end of stage k
 		st[8 + k] = (int64_t)std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t_start).count();
@@ -489,6 +493,79 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 	});
 	if (bad) { nsh_set_error("nsh_map_batch: a chain block is outside the index"); return 0; }
 	lap(1);
+
+	// ---- the learned prescreen between coarse chaining and refinement (where the reference's
+	// --gpu-prefilter sits, map.c:1114-1117): every candidate's 64 features (the window-sequence
+	// features of the device kernel, slots 46-59 replaced by the chain's own), the network, the cut
+	// and a random audit of what it rejects
+	std::vector<std::pair<int32_t, int32_t>> cands;   // (query, pool index) of every candidate offered
+	if (o->prescreen || (o->cand_feat && o->cand_cap > 0)) {
+		for (int32_t q = 0; q < n_q; ++q)
+			for (int32_t i : regs[q]) cands.push_back({ q, i });
+		const int32_t nc = (int32_t)cands.size();
+		std::vector<int64_t> g_off(nc), a_off(nc);
+		std::vector<int32_t> wl(nc), al(nc);
+		for (int32_t c = 0; c < nc; ++c) {
+			const int32_t q = cands[c].first;
+			const Reg &r = pool[q][cands[c].second];
+			const int64_t clen = m->len[r.vid >> 1], vs = std::min<int64_t>(std::max<int64_t>(r.vs, 0), clen);
+			const int64_t ve = std::min<int64_t>(std::max<int64_t>(r.ve, vs), clen);
+			g_off[c] = m->view.at(r.vid, vs);
+			wl[c] = (int32_t)std::min<int64_t>(ve - vs, nsh_feat::kMaxWindow);
+			a_off[c] = q_off[q], al[c] = std::max(1, std::min(qlen[q], nsh_feat::kMaxProtein));
+		}
+		std::vector<float> feat((size_t)std::max(nc, 1) * 64);
+		if (nc > 0 && !nsh_feat::from_genome(m->d_g, nc, g_off.data(), wl.data(), qs_all.data(), a_off.data(), al.data(),
+				(int64_t)qs_all.size(), feat.data()))
+			return 0;
+		std::vector<int64_t> cfirst(n_q + 1, 0);   // first candidate of each query (candidates are in query order)
+		for (int32_t q = 0; q < n_q; ++q) cfirst[q + 1] = cfirst[q] + (int64_t)regs[q].size();
+		nsh_parallel_for(n_q, nth, [&](int64_t q) {   // the chain's own features
+			const Regs &R = regs[q];
+			int32_t best = 1;
+			for (int32_t i : R) best = std::max(best, pool[q][i].chn_sc);
+			const double L = std::max(qlen[q], 1), n = (double)R.size();
+			const int64_t c0 = cfirst[q];
+			for (size_t k = 0; k < R.size(); ++k) {
+				const Reg &r = pool[q][R[k]];
+				const Reg &par = pool[q][R[std::max(0, std::min<int32_t>(r.parent, (int32_t)R.size() - 1))]];
+				const double span = (double)(r.ve - r.vs);
+				float *f = feat.data() + (c0 + (int64_t)k) * 64;
+				f[46] = (float)(r.chn_sc / L), f[47] = (float)((double)r.chn_sc / best);
+				f[48] = (float)(r.chn_sc_ungap / L), f[49] = (float)(r.cnt / L);
+				f[50] = (float)((r.qe - r.qs) / L), f[51] = (float)(r.qs / L), f[52] = (float)((L - r.qe) / L);
+				f[53] = (float)std::log2(1.0 + span), f[54] = (float)(span / (3.0 * (r.qe - r.qs) + 1.0));
+				f[55] = r.parent == (int32_t)k? 1.0f : 0.0f, f[56] = (float)(k / n), f[57] = (float)std::log2(1.0 + n);
+				f[58] = (float)((double)r.chn_sc / std::max(par.chn_sc, 1)), f[59] = (float)((best - r.chn_sc) / L);
+			}
+		});
+		st[6] = nc;
+		if (o->cand_feat) std::memcpy(o->cand_feat, feat.data(), (size_t)std::min<int64_t>(nc, o->cand_cap) * 64 * 4);
+		if (o->prescreen && nc > 0) {
+			std::vector<float> logit(nc);
+			if (!nsh_mlp_infer(o->prescreen, feat.data(), nc, 0, logit.data())) return 0;
+			std::vector<char> keep(nc);
+			for (int32_t c = 0; c < nc; ++c) {
+				keep[c] = logit[c] >= o->ps_tau;
+				if (!keep[c]) {   // the audit: a fixed fraction of the rejected, drawn by a hash of the seed and the candidate
+					uint64_t h = o->ps_seed ^ ((uint64_t)(uint32_t)cands[c].first << 32 | (uint32_t)cands[c].second);
+					h ^= h >> 33, h *= 0xff51afd7ed558ccdULL, h ^= h >> 33, h *= 0xc4ceb9fe1a85ec53ULL, h ^= h >> 33;
+					if ((double)(h >> 11) * (1.0 / 9007199254740992.0) < o->ps_audit) keep[c] = 2, ++st[14];
+					else ++st[7];
+				}
+			}
+			for (int32_t q = 0, c = 0; q < n_q; ++q) {
+				Regs kept;
+				for (int32_t i : regs[q]) {
+					if (keep[c]) kept.push_back(i);
+					if (keep[c] == 2) pool[q][i].audited = true;
+					++c;
+				}
+				regs[q].swap(kept);
+			}
+		}
+	}
+	if (!cands.empty()) st[16] = (int64_t)std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t_start).count();
 
 	// ---- refine: every region's window against its query (map.c:141-258, 666-684)
 	struct RefJob { int32_t q, r; int64_t as; };
@@ -653,6 +730,20 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 			emit[q].push_back((int32_t)j);
 		}
 		id0[q + 1] = id0[q] + (int64_t)emit[q].size();
+	}
+	if (!cands.empty()) {   // which candidates became records: the labels, and what the audit caught
+		std::vector<std::vector<char>> rec(n_q);
+		for (int32_t q = 0; q < n_q; ++q) {
+			rec[q].assign(pool[q].size(), 0);
+			for (int32_t j : emit[q]) rec[q][regs[q][j]] = 1;
+		}
+		int64_t caught = 0;
+		for (size_t c = 0; c < cands.size(); ++c) {
+			const char r = rec[cands[c].first][cands[c].second];
+			if (o->cand_label && (int64_t)c < o->cand_cap) o->cand_label[c] = (uint8_t)r;
+			if (r && pool[cands[c].first][cands[c].second].audited) ++caught;
+		}
+		st[15] = caught;
 	}
 	nsh_parallel_for(n_q, nth, [&](int64_t q) {
 		std::string &s = text[q];

@@ -25,5 +25,11 @@ namespace nsh_feat {
 // the 64 prescreen features of n candidates (features.hip); out: n x 64 floats, device
 void launch(const nsh_prep::PrepProb *pp, int32_t n, const uint8_t *ns, const uint8_t *as,
 		const nsh_prep::PrepConst *pc, const int8_t *mat, float *out, hipStream_t s);
+// the same for candidates whose windows are slices of an oriented genome in HBM (d_g + g_off[k],
+// nl[k] bases, at most 24,000) and whose proteins are host strings (al[k] at most 4,096);
+// out: n x 64 floats on the host. 1 on success
+int from_genome(const uint8_t *d_g, int32_t n, const int64_t *g_off, const int32_t *nl, const char *as,
+		const int64_t *as_off, const int32_t *al, int64_t l_as, float *out);
+constexpr int kMaxWindow = 24000, kMaxProtein = 4096;
 } // namespace nsh_feat
 #endif

@@ -216,3 +216,132 @@ def evaluate(n_train=16384, n_val=4096, n_eval=16384, seed=11, epochs=12, pos_we
     net.close()
     ev.close()
     return out
+
+
+# ---------------------------------------------------------------------------------------------
+# the prescreen on the map path's own candidates
+# ---------------------------------------------------------------------------------------------
+
+def map_candidates(mapper, names, seqs):
+    """every candidate the map path offers between coarse chaining and refinement, with its 64
+    features, whether it became a record (the reference's own keep rule: the output step's filters,
+    map.c:1143-1166), and its query index; one plain nsh_map_batch call"""
+    import ctypes as C
+    cap = 64 * len(seqs) + 1024
+    while True:
+        feat = np.zeros((cap, capi.N_FEATURES), np.float32)
+        lab = np.zeros(cap, np.uint8)
+        o = capi.MapOpt.default(cand_feat=feat.ctypes.data, cand_label=lab.ctypes.data, cand_cap=cap)
+        recs, st = mapper.map(names, seqs, o)
+        n = int(st[6])
+        if n <= cap:
+            break
+        cap = n
+    return feat[:n], lab[:n].astype(np.float32), recs, st
+
+
+def map_evaluate(n_prot=20000, seed_train=22, seed_eval=21, epochs=30, lr=2e-3, log=print, target=TARGET_RECALL,
+                 audit=AUDIT_FRAC):
+    """train the network on a genome-scale workload's candidates, calibrate its cut on held-out
+    queries, then map a second workload plain and prescreened: records, recall, time"""
+    t0 = time.time()
+    cn, cs, pn, ps, _ = synth.genome_with_genes(n_prot=n_prot, seed=seed_train)
+    mp = capi.Mapper(cn, cs)
+    try:
+        X, y, _, st = map_candidates(mp, pn, ps)
+    finally:
+        mp.close()
+    del cs
+    n = len(y)
+    if log:
+        log("[map prescreen] training workload: %d candidates (%.1f%% became records) in %.1f s" %
+            (n, 100 * y.mean(), time.time() - t0))
+    cut = int(0.8 * n)   # candidates are in query order: the last fifth belongs to other queries
+    tr, va = np.arange(cut), np.arange(cut, n)
+    # a few training settings, the one that keeps the fewest held-out candidates at the target wins
+    t1 = time.time()
+    best = None
+    for ep, pw in ((epochs, 1.0), (2 * epochs, 1.0), (2 * epochs, 2.0), (4 * epochs, 2.0)):
+        params, mu, sd = train(X[tr], y[tr], epochs=ep, seed=seed_train, lr=lr, pos_weight=pw, log=None)
+        cand = capi.MLP(params, mu, sd)
+        lv = cand.infer(X[va])
+        tau = calibrate(lv, y[va], target)
+        kf = float((lv >= tau).mean())
+        if log:
+            log("[map prescreen] epochs %d, positive weight %.0f: keeps %.4f of held-out candidates" % (ep, pw, kf))
+        if best is None or kf < best[0]:
+            if best is not None:
+                best[1].close()
+            best = (kf, cand, tau, lv, ep, pw)
+        else:
+            cand.close()
+    train_s = time.time() - t1
+    _, net, tau, lv, best_ep, best_pw = best
+    best_f, best_keep, best_th = None, 1.0, None
+    for f in range(capi.N_FEATURES):
+        for sign in (1, -1):
+            th = calibrate(sign * X[va, f], y[va], target)
+            kf = float((sign * X[va, f] >= th).mean())
+            if kf < best_keep:
+                best_f, best_keep, best_th = (f, sign), kf, th
+    if log:
+        log("[map prescreen] trained on %d candidates in %.1f s; tau %.3f keeps %.3f of held-out candidates "
+            "(best single feature %s: %.3f)" % (len(tr), train_s, tau, float((lv >= tau).mean()), best_f, best_keep))
+    # the evaluation workload: plain, then prescreened
+    cn, cs, pn, ps, _ = synth.genome_with_genes(n_prot=n_prot, seed=seed_eval)
+    mp = capi.Mapper(cn, cs)
+    try:
+        Xe, ye, _, _ = map_candidates(mp, pn, ps)      # untimed: features and labels of every candidate
+        mp.map(pn[:32], ps[:32])
+        t = time.perf_counter()
+        plain, st_p = mp.map(pn, ps)
+        plain_s = time.perf_counter() - t
+        o = capi.MapOpt.default(prescreen=net.h, ps_tau=tau, ps_audit=audit, ps_seed=seed_eval)
+        mp.map(pn[:32], ps[:32], o)
+        t = time.perf_counter()
+        pre, st_s = mp.map(pn, ps, o)
+        pre_s = time.perf_counter() - t
+        infer_ms = float(net.last_ms())
+    finally:
+        mp.close()
+        net.close()
+    a = set(plain.split(b"\n")) - {b""}
+    b = set(pre.split(b"\n")) - {b""}
+    recall = len(a & b) / max(1, len(a))
+    sign = best_f[1]
+    base_keep = float((sign * Xe[:, best_f[0]] >= best_th).mean())
+    base_recall = float((sign * Xe[ye > 0, best_f[0]] >= best_th).mean()) if (ye > 0).any() else 1.0
+    offered, dropped, audited, caught = int(st_s[6]), int(st_s[7]), int(st_s[14]), int(st_s[15])
+    flop = 2 * (64 * 256 + 256 * 256 + 256)
+    out = {
+        "workload": "map path, %d proteins against 100 Mb (the map section's workload); trained on the "
+                    "candidates of a second such workload (seed %d), cut calibrated on its last fifth" % (n_prot, seed_train),
+        "candidates": offered, "became_records_frac": round(float(ye.mean()), 4),
+        "kept_frac": round((offered - dropped) / max(1, offered), 4), "audited": audited,
+        "audited_that_became_records": caught, "tau": round(tau, 4),
+        "record_recall": round(recall, 5), "records_plain": len(a), "records_prescreened": len(b),
+        "plain_s": round(plain_s, 3), "prescreened_s": round(pre_s, 3), "speedup": round(plain_s / pre_s, 3),
+        "plain_proteins_per_s": round(n_prot / plain_s, 1), "prescreened_proteins_per_s": round(n_prot / pre_s, 1),
+        "dp_candidates_plain": int(st_p[3]), "dp_candidates_prescreened": int(st_s[3]),
+        "prescreen_ms": round((st_s[16] - st_s[9]) / 1e3, 1),
+        "baseline_feature": "%d%s" % (best_f[0], "" if sign > 0 else " (low)"),
+        "baseline_kept_frac": round(base_keep, 4), "baseline_candidate_recall": round(base_recall, 5),
+        "train_candidates": int(len(tr)), "train_s": round(train_s, 2), "epochs": best_ep, "positive_weight": best_pw,
+        "mlp_infer_ms": round(infer_ms, 4),
+        "mfma": {"achieved": round(offered * flop / max(infer_ms, 1e-6) / 1e9, 3), "unit": "TFLOP/s",
+                 "peak": 2500.0, "flop_per_candidate": flop},
+    }
+    out["mfma"]["frac"] = round(out["mfma"]["achieved"] / 2500.0, 5)
+
+    def stages(st):
+        prev, d = 0, {}
+        for k, name in enumerate(capi.MAP_STAGES):
+            d[name] = round((int(st[8 + k]) - prev) / 1e3, 1)
+            prev = int(st[8 + k])
+        return d
+    out["stage_ms_plain"], out["stage_ms_prescreened"] = stages(st_p), stages(st_s)
+    if log:
+        log("[map prescreen] kept %.3f of %d candidates, record recall %.5f (%d -> %d records), %.2f s -> %.2f s "
+            "(%.2fx); best single feature keeps %.3f" % (out["kept_frac"], offered, recall, len(a), len(b), plain_s,
+                                                         pre_s, plain_s / pre_s, base_keep))
+    return out
