@@ -339,6 +339,16 @@ void shell_release(nsh_ext_plan_t *p)
 }
 } // namespace
 
+int nsh_ext_req_fits(const nsh_ext_req &q, int32_t k)
+{
+	if (q.al < 1 || q.nl < 0 || q.npad < q.al) { nsh_set_error("request %d: nl=%d al=%d npad=%d", k, q.nl, q.al, q.npad); return 0; }
+	if (q.npad > 64 * nsh_glob::kCMaxG * nsh_glob::kNWMax) {
+		nsh_set_error("request %d: padded protein length %d exceeds %d", k, q.npad, 64 * nsh_glob::kCMaxG * nsh_glob::kNWMax);
+		return 0;
+	}
+	return 1;
+}
+
 nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_opt_t *opt)
 {
 	if (!nsh_ensure_device()) return nullptr;
@@ -362,11 +372,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 	for (int32_t k = 0; k < n; ++k) {
 		const nsh_ext_req &q = req[k];
 		int32_t C = (q.npad + 127) / 128;
-		if (q.al < 1 || q.nl < 0 || q.npad < q.al) { nsh_set_error("request %d: nl=%d al=%d npad=%d", k, q.nl, q.al, q.npad); return nullptr; }
-		if (q.npad > 64 * nsh_glob::kCMaxG * nsh_glob::kNWMax) {
-			nsh_set_error("request %d: padded protein length %d exceeds %d", k, q.npad, 64 * nsh_glob::kCMaxG * nsh_glob::kNWMax);
-			return nullptr;
-		}
+		if (!nsh_ext_req_fits(q, k)) return nullptr;
 		if (C < 1) C = 1;
 		is_wide[k] = C > CMAX;
 		ExtProb &e = prob[k];

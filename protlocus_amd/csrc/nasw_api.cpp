@@ -367,19 +367,27 @@ public:
 		for (size_t a = 0; a < all.size(); ++a) {
 			if (taken[a]) continue;
 			std::vector<ExtReq*> grp;
-			for (size_t b = a; b < all.size(); ++b)
-				if (!taken[b] && opt_same(all[a]->opt, all[b]->opt)) taken[b] = 1, grp.push_back(all[b]);
-			std::vector<nsh_ext_req> req(grp.size());
-			for (size_t k = 0; k < grp.size(); ++k) {
-				const ExtReq &x = *grp[k];
-				nsh_ext_req &q = req[k];
+			std::vector<nsh_ext_req> req;
+			for (size_t b = a; b < all.size(); ++b) {
+				if (taken[b] || !opt_same(all[a]->opt, all[b]->opt)) continue;
+				taken[b] = 1;
+				const ExtReq &x = *all[b];
+				nsh_ext_req q;
 				q.ns = x.ns, q.nl = x.nl, q.as = x.as, q.al = x.al, q.ss = x.ss;
 				q.left = (x.opt.flag & NS_F_EXT_LEFT) != 0, q.io = x.opt.io;
 				q.mode = (x.opt.flag & (NS_F_EXT_LEFT | NS_F_EXT_RIGHT)) == 0;
 				q.pssm = x.opt.pssm;
 				const int32_t w = nsh_host_stripe_width(x.opt.flag, x.al);
 				q.npad = (x.al + w - 1) / w * w;
+				// a request beyond the device limits fails alone; the rest of its group still runs
+				if (!nsh_ext_req_fits(q, (int32_t)b)) {
+					all[b]->ok = 0;
+					std::fprintf(stderr, "[nasw_hip] extension request not run (nl=%d al=%d): %s\n", x.nl, x.al, ns_gpu_error());
+					continue;
+				}
+				grp.push_back(all[b]), req.push_back(q);
 			}
+			if (grp.empty()) continue;
 			std::vector<nsh_ext_result_t> res(grp.size());
 			nsh_ext_plan_t *p = nsh_ext_plan_build((int32_t)req.size(), req.data(), &grp[0]->opt);
 			const int ok = p && nsh_ext_plan_run(p) && nsh_ext_plan_results(p, res.data());

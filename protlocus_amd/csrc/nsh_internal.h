@@ -63,6 +63,8 @@ struct nsh_ext_req {
 };
 nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_opt_t *opt);
 int nsh_ext_plan_query(nsh_ext_plan_t *p);   // 1 when the last run has completed
+// 1 when nsh_ext_plan_build accepts the request; otherwise 0 and the reason in ns_gpu_error
+int nsh_ext_req_fits(const nsh_ext_req &q, int32_t k);
 
 // host-side input preparation (prep_host.cpp), used by the ns_ext_prep() building block
 void nsh_prep_host(const char *ns, int32_t nl, const char *as, int32_t al, const ns_opt_t *opt,

@@ -125,6 +125,36 @@ def test_ext_batch_api_matches_oracle(oracle, product):
     L.ns_ext_batch_destroy(b)
 
 
+def test_ext_batch_over_limit_request_fails_alone(oracle, product):
+    # one request past the device's 8192 padded columns, in the same option group as in-limit
+    # ones: it alone gets INT32_MIN; the others are still computed, bit-exact
+    L = product.lib
+    reqs = [r for r in ref_style_requests() if r[0] == 0][:10]
+    kw0 = reqs[0][3]
+    rng = np.random.default_rng(5)
+    big_a = bytes(rng.choice(list(b"ACDEFGHIKLMNPQRSTVWY"), 8200).astype(np.uint8))
+    big_ns = bytes(rng.choice(list(b"ACGT"), 600).astype(np.uint8))
+    b = L.ns_ext_batch_init(None)
+    assert b, product.error()
+    keep, ids = [], []
+    for k, (g, ns, a, kw, ss) in enumerate(reqs):
+        if k == 4:
+            o = product.opt(**kw0)
+            keep.append(o)
+            ids.append((L.ns_ext_batch_add(b, big_ns, len(big_ns), big_a, len(big_a), nsffi.C.byref(o), None), None))
+        o = product.opt(**kw)
+        keep.append(o)
+        ids.append((L.ns_ext_batch_add(b, ns, len(ns), a, len(a), nsffi.C.byref(o), ss), (ns, a, kw, ss)))
+    assert L.ns_ext_batch_flush(b) == len(ids)
+    for rid, x in ids:
+        r = L.ns_ext_batch_get(b, rid).contents
+        if x is None:
+            assert (r.nt_len, r.aa_len, r.score) == (len(big_ns), len(big_a), -2 ** 31)
+        else:
+            assert (r.nt_len, r.aa_len, r.score) == _oracle_ext(oracle, product, *x)
+    L.ns_ext_batch_destroy(b)
+
+
 @pytest.mark.parametrize("seed,n,lo,hi", [(1, 384, 50, 1000), (2, 256, 20, 140), (3, 64, 900, 2000)])
 def test_ext_plan_config1_batch(oracle, product, seed, n, lo, hi):
     from protlocus_amd import synth
