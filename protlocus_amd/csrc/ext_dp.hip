@@ -18,11 +18,14 @@
 //   * Per row the wave reduces the row maximum (padded columns included, columns beyond the
 //     reference's padding excluded), applies the ie_coef*log2 length penalty from a host
 //     table, updates the best row (snapshot of its H values in registers) and checks x-drop.
-//   * Scores: the 22 x 22 matrix is staged once per work-group in LDS as a table indexed by
-//     column residue x 22 + row residue (a PSSM is staged per problem as an int8 profile of
-//     22 residues x 128C columns), so occupancy is bound by registers, not LDS; the per-row
-//     residue, donor[i+1] and acceptor[i] come from a 4-byte record per row prepared on the
-//     device; records and penalties are streamed 64 rows at a time into one VGPR each.
+//   * Scores: the 22 x 22 matrix is staged once per work-group in LDS as a table of row
+//     residue x column residue (a PSSM is staged per problem as an int8 profile of 22 residues
+//     x 128C columns), so occupancy is bound by registers, not LDS. Per row one LDS read puts
+//     the row residue's table row into a register (lane j = column residue j) and each column
+//     takes its score from the lane of its residue with ds_bpermute, whose source addresses are
+//     fixed per problem. The per-row residue, donor[i+1] and acceptor[i] come from a 4-byte
+//     record per row prepared on the device; records and length penalties are read through the
+//     scalar data cache a row before they are needed.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <climits>
@@ -341,7 +344,8 @@ void shell_release(nsh_ext_plan_t *p)
 
 int nsh_ext_req_fits(const nsh_ext_req &q, int32_t k)
 {
-	if (q.al < 1 || q.nl < 0 || q.npad < q.al) { nsh_set_error("request %d: nl=%d al=%d npad=%d", k, q.nl, q.al, q.npad); return 0; }
+	// npad is the protein rounded up to a stripe width of 8 or 16; the kernels rely on a multiple of 8
+	if (q.al < 1 || q.nl < 0 || q.npad < q.al || q.npad % 8) { nsh_set_error("request %d: nl=%d al=%d npad=%d", k, q.nl, q.al, q.npad); return 0; }
 	if (q.npad > 64 * nsh_glob::kCMaxG * nsh_glob::kNWMax) {
 		nsh_set_error("request %d: padded protein length %d exceeds %d", k, q.npad, 64 * nsh_glob::kCMaxG * nsh_glob::kNWMax);
 		return 0;

@@ -38,7 +38,7 @@ constexpr int NRES = 22;                 // residue codes of the profile rows
 
 // DPP controls (GFX9 encoding)
 enum : int { ROW_SHR1 = 0x111, ROW_SHR2 = 0x112, ROW_SHR4 = 0x114, ROW_SHR8 = 0x118,
-	WAVE_SHR1 = 0x138, ROW_BCAST15 = 0x142, ROW_BCAST31 = 0x143 };
+	WAVE_SHR1 = 0x138, WAVE_ROR1 = 0x13C, ROW_BCAST15 = 0x142, ROW_BCAST31 = 0x143 };
 // lanes without a source read 0 (bound_ctrl); every lane is written
 template <int CTRL>
 DEV int dpp0(int x) { return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xf, 0xf, true); }
@@ -48,15 +48,29 @@ DEV int dppk(int x) { return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xf, 0xf, f
 
 // PROF: scores are staged per problem as an int8 profile (22 residues x 128C columns) in LDS,
 // from the PSSM rows or the matrix; otherwise (matrix only) the 22 x 22 matrix is staged once per
-// work-group as an int16 23 x 22 table (row 22 = padding, -32768) indexed by column residue * 22
-// + row residue,
-// which leaves the LDS small enough for register-bound occupancy.
-constexpr int TAB_PAD = 22;    // table row of the padded columns (all -128)
+// work-group as an int16 table of 22 row residues x 23 column residues (column 22 = padding,
+// -32768), which leaves the LDS small enough for register-bound occupancy. Per row, lane j < 23
+// reads the entry (row residue, j) into one register, and every column takes its score from the
+// lane of its residue with ds_bpermute: the lanes' source addresses are fixed for the problem, so
+// a column's score costs no address arithmetic, only the perm that packs the two halves.
+constexpr int TAB_PAD = 22;    // column residue of the padded columns (score -32768)
+constexpr int TAB_W = 23;      // entries per table row
+
+// uniform loads through the scalar data cache (the row records and the penalty table are not
+// written while the kernels run)
+typedef const uint32_t __attribute__((address_space(4))) *cu32_t;
+typedef const int32_t __attribute__((address_space(4))) *ci32_t;
+DEV int sld(const uint32_t *p, int64_t i) { return (int)((cu32_t)p)[i]; }
+DEV int sld(const int32_t *p, int64_t i) { return ((ci32_t)p)[i]; }
+typedef const uint16_t __attribute__((address_space(3))) *lds_u16_t;   // 32-bit LDS addressing
 
 template <int C, bool PROF>
 DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, int *dring, const int L)
 {
 	constexpr int HALF = 64 * C, ROWB = 128 * C;
+	// npad is a multiple of 8 (the host checks it), so when C divides 8 a lane's columns of one half
+	// are either all below npad or all at or beyond it: the row maximum is masked once per lane
+	constexpr bool LANE_MASK = (8 % C) == 0;
 	const int nl = P.nl, al = P.al, npad = P.npad, asz = a.asize;
 	const int go = a.go, ge = a.ge, fs = a.fs, io = P.io, eb = a.eb, xdrop = a.xdrop;
 	const uint8_t *aas = a.aas + P.aa_off;
@@ -64,22 +78,23 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, 
 	const uint32_t *rec = a.rec + P.rec_off;
 	const int pen_len = 3 * al;
 
-	// ---- per-lane column constants and the int8 query profile in LDS ----
+	// ---- per-lane column constants and the scores ----
 	// sel[k] (profile path): v_perm selector packing the two sign-extended profile bytes of column k into an
 	// int16 pair; padded columns (al <= c < npad) become -32768 like the reference profile.
 	// lim[k]: columns at or beyond npad (not part of the reference's row maximum) are masked.
 	uint32_t sel[PROF? C : 1];
-	s2 lim[C];
+	s2 lim[LANE_MASK? 1 : C];
 #pragma unroll
 	for (int k = 0; k < C; ++k) {
 		const int clo = L * C + k, chi = HALF + L * C + k;
 		const bool plo = clo >= al && clo < npad, phi = chi >= al && chi < npad;
 		if (PROF) sel[k] = (plo? 0x000Cu : 0x0100u) | (phi? 0x040C0000u : 0x05040000u);
-		lim[k] = s2{(short)(clo < npad? 32767 : NEG), (short)(chi < npad? 32767 : NEG)};
+		if (!LANE_MASK) lim[k] = s2{(short)(clo < npad? 32767 : NEG), (short)(chi < npad? 32767 : NEG)};
 	}
-	// table offsets of the lane's columns (matrix scores): column residue * 22 (padding row past
-	// al), low half in bits 0-15, high half in bits 16-31
-	int ta[C];
+	if (LANE_MASK) lim[0] = s2{(short)(L * C < npad? 32767 : NEG), (short)(HALF + L * C < npad? 32767 : NEG)};
+	// table path: ds_bpermute source addresses (column residue * 4, padding past al) of the lane's
+	// columns, low and high half
+	int bl[PROF? 1 : C], bh[PROF? 1 : C];
 	if (PROF) {
 		__syncthreads();
 #pragma unroll
@@ -99,9 +114,12 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, 
 #pragma unroll
 		for (int k = 0; k < C; ++k) {
 			const int clo = L * C + k, chi = HALF + L * C + k;
-			ta[k] = (clo < al? aas[clo] : TAB_PAD) * NRES | (chi < al? aas[chi] : TAB_PAD) * NRES << 16;
+			bl[k] = (clo < al? aas[clo] : TAB_PAD) * 4;
+			bh[k] = (chi < al? aas[chi] : TAB_PAD) * 4;
 		}
 	}
+	const int trow_col = L < TAB_W? L : TAB_PAD;   // the lane's entry in a table row
+	const lds_u16_t tab = (lds_u16_t)prof;
 
 	// ---- scan constants (biased unsigned): the insertion carry decays by ge per column ----
 	const int geC = ge * C;
@@ -110,6 +128,9 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, 
 	const u2 DEC31 = splu(L >= 32? geC * (L - 31) : 0xffff);       // row_bcast:31 (rows 0, 1: no source)
 	const u2 DECL = u2{0, (unsigned short)(geC * L)};              // low half's total into the high half
 	const s2 GO = spl(go), GE = spl(ge), IO = spl(io), FS = spl(fs);
+	// v_perm selector of the left-neighbour exchange: lane 0 takes -32768 in the low half and the
+	// rotated-in value's low half (lane 63's last low column) in the high half
+	const uint32_t nsel = L == 0? 0x05040100u : 0x07060504u;
 
 	// ---- DP state: three rows of H and D, intron states, best-row snapshot ----
 	// (the best row's H values are kept in LDS: snap[k * 64 + L], written on each new best row)
@@ -130,35 +151,40 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, 
 	// end column (al-1): half, lane and slot
 	const int he = (al - 1) >= HALF, qe = (al - 1) - he * HALF, Le = qe / C, ke = qe % C;
 
-	// ---- row records and penalties, 64 rows per VGPR, double buffered ----
-	auto load_blk = [&](int i0, int &rv, int &pv) {
-		const int i = i0 + L;
-		rv = i < nl? (int)rec[i] : 0;
-		const int d = i - pen_len;
-		pv = d < 2? 0 : a.pen[d < a.pen_n? d : a.pen_n - 1];
+	// ---- row records and penalties: scalar loads issued a row before they are needed ----
+	auto pen_of = [&](int r) -> int {
+		const int d = r - pen_len;
+		return d < 2? 0 : sld(a.pen, d < a.pen_n? d : a.pen_n - 1);
 	};
-	int recv, penv, recn = 0, penn = 0;
-	load_blk(0, recv, penv);
-	if (nl > 64) load_blk(64, recn, penn);
 	int d0 = 0, dp1 = 0, am1 = 0, a0 = 0, dm1 = 0, am2 = 0;
-	// profile bytes of the next row, loaded one row ahead
+	int rc1 = 0, rc2 = 0, rc3 = 0; // records of rows i, i+1 and i+2 on entry to row i
+	int pc1 = 0;                   // the length penalty of row i
+	int trow = 0;                  // table path: the table row of row i+1's residue
+	// profile bytes / table values of the next row, loaded one row ahead
 	int pf_lo[C], pf_hi[C];
-	auto prefetch = [&](int res) {
-		if (PROF) {
-			const int8_t *pr = prof + res * ROWB + L;
+	auto fetch_row = [&](int res) -> int { return (int)tab[res * TAB_W + trow_col]; };
+	auto prefetch_tab = [&](int tr) {
 #pragma unroll
-			for (int k = 0; k < C; ++k) pf_lo[k] = pr[k * 64], pf_hi[k] = pr[HALF + k * 64];
-		} else {
-			const int16_t *tab = (const int16_t*)prof;
+		for (int k = 0; k < C; ++k)
+			pf_lo[k] = __builtin_amdgcn_ds_bpermute(bl[k], tr), pf_hi[k] = __builtin_amdgcn_ds_bpermute(bh[k], tr);
+	};
+	auto prefetch_prof = [&](int res) {
+		const int8_t *pr = prof + res * ROWB + L;
 #pragma unroll
-			for (int k = 0; k < C; ++k) pf_lo[k] = tab[(ta[k] & 0xffff) + res], pf_hi[k] = tab[(ta[k] >> 16) + res];
-		}
+		for (int k = 0; k < C; ++k) pf_lo[k] = pr[k * 64], pf_hi[k] = pr[HALF + k * 64];
 	};
 	if (nl >= 3) {
-		const int r0 = __builtin_amdgcn_readlane(recv, 0), r1 = __builtin_amdgcn_readlane(recv, 1);
+		const int r0 = sld(rec, 0), r1 = sld(rec, 1);
 		d0 = (int)(int8_t)(r0 >> 8), dp1 = (int)(int8_t)(r1 >> 8);
 		am1 = (int)(int8_t)(r0 >> 16), a0 = (int)(int8_t)(r1 >> 16);
-		if (C <= 8) prefetch(__builtin_amdgcn_readlane(recv, 2) & 0xff);
+		rc1 = sld(rec, 2);
+		if (nl > 3) rc2 = sld(rec, 3);
+		if (nl > 4) rc3 = sld(rec, 4);
+		pc1 = pen_of(2);
+		if (!PROF) {
+			prefetch_tab(fetch_row(rc1 & 0xff));
+			if (nl > 3) trow = fetch_row(rc2 & 0xff);
+		} else if (C <= 8) prefetch_prof(rc1 & 0xff);
 	}
 
 	int best_log = INT_MIN, best_i = -1, last_i = 1, i = 2, last_ev = NEG;
@@ -167,27 +193,27 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, 
 		constexpr int R = decltype(Rc)::value, R2 = (R + 1) % 3, R1 = (R + 2) % 3;
 		s2 *H3 = Hs[R], *H2 = Hs[R2], *H1 = Hs[R1];
 		int *D3 = dring + R * C * 64 + L;
-		if (__builtin_expect((i & 63) == 0, 0)) { // next block of records
-			recv = recn, penv = penn;
-			if (i + 64 < nl) load_blk(i + 64, recn, penn);
-		}
-		const int rc = __builtin_amdgcn_readlane(recv, i & 63);
-		const int pen_i = __builtin_amdgcn_readlane(penv, i & 63);
+		const int rc = rc1, rn = rc2, rnn = rc3, pen_i = pc1;
+		rc1 = rn, rc2 = rnn;
+		rc3 = i + 3 < nl? sld(rec, i + 3) : 0;
+		pc1 = pen_of(i + 1);
 		const int res = rc & 0xff;
 		dm1 = d0, d0 = dp1, dp1 = (int)(int8_t)(rc >> 8);
 		am2 = am1, am1 = a0, a0 = (int)(int8_t)(rc >> 16);
 		const s2 GEI = spl(res == 20? fs : ge);
 		const s2 DM1 = spl(dm1), D0 = spl(d0), DP1 = spl(dp1), A0 = spl(a0), AM1 = spl(am1), AM2 = spl(am2);
 
-		// this row's profile values (bytes prefetched during the previous row)
+		// this row's scores (fetched during the previous row), then the next row's
 		s2 s[C];
-		if (C > 8) prefetch(res);
+		if (PROF && C > 8) prefetch_prof(res);
 #pragma unroll
-		for (int k = 0; k < C; ++k)   // table path: int16 values, the low halves of the two words
+		for (int k = 0; k < C; ++k)   // table path: the low halves of the two words
 			s[k] = up((int)__builtin_amdgcn_perm((uint32_t)pf_hi[k], (uint32_t)pf_lo[k], PROF? sel[k] : 0x05040100u));
-		if (C <= 8 && i + 1 < nl) {
-			const int j = (i + 1) & 63;
-			prefetch((j? __builtin_amdgcn_readlane(recv, j) : __builtin_amdgcn_readlane(recn, 0)) & 0xff);
+		if (i + 1 < nl) {
+			if (!PROF) {
+				prefetch_tab(trow);
+				if (i + 2 < nl) trow = fetch_row(rnn & 0xff);
+			} else if (C <= 8) prefetch_prof(rn & 0xff);
 		}
 
 		// phase A: every term but the insertion; independent across columns
@@ -231,12 +257,16 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, 
 		const s2 cin = up(pku(cinu) ^ BIAS);
 
 		// phase C: fold the carry into the cells, masked row maximum
-		s2 mk[C];
+		// (the carry decays by ge per column: two interleaved chains of saturating subtractions, which
+		// equal one subtraction of ge * k, so only ge and 2 ge are kept in scalar registers)
+		s2 mk[C], ck[2] = {cin, subs(cin, GE)};
+		const s2 GE2 = spl(2 * ge);
 #pragma unroll
 		for (int k = 0; k < C; ++k) {
-			const s2 hk = k? vmax(hh[k], subs(cin, spl(ge * k))) : vmax(hh[0], cin);
+			if (k >= 2) ck[k & 1] = subs(ck[k & 1], GE2);
+			const s2 hk = vmax(hh[k], ck[k & 1]);
 			H3[k] = hk;
-			mk[k] = vmin(hk, lim[k]);
+			mk[k] = LANE_MASK? hk : vmin(hk, lim[k]);
 		}
 #pragma unroll
 		for (int w = 1; w < C; w *= 2)
@@ -244,7 +274,7 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, 
 			for (int k = 0; k + w < C; k += 2 * w) mk[k] = vmax(mk[k], mk[k + w]);
 		// row maximum over both halves and all lanes
 		// (biased by +32768 so that the zero a source-less lane reads is neutral)
-		const s2 rm = mk[0];
+		const s2 rm = LANE_MASK? vmin(mk[0], lim[0]) : mk[0];
 		int mx = std::max((int)rm.x, (int)rm.y) + 32768;
 		mx = std::max(mx, dpp0<ROW_SHR1>(mx));
 		mx = std::max(mx, dpp0<ROW_SHR2>(mx));
@@ -275,12 +305,10 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, 
 #pragma unroll
 			for (int k = 0; k < C; ++k) snap[k * 64 + L] = pk(H3[k]);
 		}
-		// left neighbour of the next row: lane L-1's last column; lane 0 gets column -1
+		// left neighbour of the next row: lane L-1's last column (rotated in); lane 0 gets column -1
 		// (neg_inf for rows > 2) in the low half and the low half's last column in the high half
-		const int V = pk(H3[C - 1]);
-		const int Vs = dpp0<WAVE_SHR1>(V);
-		const int V63 = __builtin_amdgcn_readlane(V, 63);
-		nh[R] = up(L == 0? (int)(((unsigned)V63 << 16) | 0x8000u) : Vs);
+		const int Vr = __builtin_amdgcn_update_dpp(0, pk(H3[C - 1]), WAVE_ROR1, 0xf, 0xf, false);
+		nh[R] = up((int)__builtin_amdgcn_perm((uint32_t)Vr, 0x8000u, nsel));
 		last_i = i;
 		if (best_log - tlog > xdrop) return false;
 		return ++i < nl;
@@ -332,13 +360,13 @@ template <int CLO, int CHI, bool PROF>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CHI <= 4? 6 : CHI <= 8? 3 : 1)))
 ext_dp_kernel(ExtArgs a)
 {
-	__shared__ __attribute__((aligned(16))) int8_t prof[PROF? NRES * 128 * CHI : (TAB_PAD + 1) * NRES * 2];
+	__shared__ __attribute__((aligned(16))) int8_t prof[PROF? NRES * 128 * CHI : NRES * TAB_W * 2];
 	__shared__ int snap[64 * CHI], dring[3 * 64 * CHI];
 	const int L = threadIdx.x;
-	if (!PROF) {   // the matrix as an int16 [column residue][row residue] table, padding row -32768
+	if (!PROF) {   // the matrix as an int16 [row residue][column residue] table, padding column -32768
 		int16_t *tab = (int16_t*)prof;
-		for (int t = L; t < (TAB_PAD + 1) * NRES; t += 64) {
-			const int aa = t / NRES, r = t % NRES;
+		for (int t = L; t < NRES * TAB_W; t += 64) {
+			const int r = t / TAB_W, aa = t % TAB_W;
 			tab[t] = (int16_t)(aa < TAB_PAD? a.mat[r * a.asize + aa] : NEG);
 		}
 		__syncthreads();
