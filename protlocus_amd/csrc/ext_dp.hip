@@ -2,8 +2,10 @@
 // reference semantics: nasw-sse.c:360-459 and 596-741, recurrence nasw-sse.c:19-26).
 //
 // Execution model
-//   * One wave64 solves one problem; waves are persistent and pull problems from a queue
-//     ordered by decreasing cost, so long problems start first and the tail is short.
+//   * One wave64 solves one problem, or two problems of at most 256 padded columns (matrix
+//     scores), one in each 16-bit half, which share the wave's fixed per-row work; waves are
+//     persistent and pull work items from a queue ordered by decreasing cost, so long problems
+//     start first and the tail is short.
 //   * Row-synchronous: each lane owns C consecutive columns in each of two column halves
 //     ([0, 64C) and [64C, 128C)); the two halves are packed into the 16-bit lanes of a
 //     32-bit register and advanced together with VOP3P saturating instructions
@@ -56,6 +58,7 @@ using namespace nsh_ext;
 using nsh_prep::PrepProb;
 using nsh_prep::PrepConst;
 constexpr int CMAX = kCmax;
+constexpr int kPairMaxCols = 256;   // widest problem that shares a wave (4 columns per lane per half)
 #define DEV static __device__ __forceinline__
 
 __global__ void fill_i32_kernel(int32_t *p, int32_t n, int32_t v)
@@ -419,15 +422,46 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 	}
 	std::stable_sort(p->wide_h.begin(), p->wide_h.end(), [](const nsh_glob::GlobProb &x, const nsh_glob::GlobProb &y) {
 		return x.NW != y.NW? x.NW > y.NW : x.C > y.C; });
-	// queue order: small-C problems first (their launch), then large; longest work first in each
-	for (int32_t k = 0; k < n; ++k) if (!is_wide[k]) order.push_back(k);
+	// two problems per wave (matrix scores only): problems of at most 256 padded columns and at least
+	// 3 rows, paired with a problem of the same column count per lane (ceil(npad / 64)) and the
+	// nearest length; the first of a pair is the queue item, its partner is stored after all items
+	std::vector<int32_t> partner(n, -1);
+	std::vector<char> second(n, 0);
+	if (!l_pssm) {
+		std::vector<int32_t> cand;
+		for (int32_t k = 0; k < n; ++k)
+			if (!is_wide[k] && prob[k].npad <= kPairMaxCols && prob[k].nl >= 3) cand.push_back(k);
+		auto cp = [&](int32_t k) { return (prob[k].npad + 63) / 64; };
+		std::stable_sort(cand.begin(), cand.end(), [&](int32_t x, int32_t y) {
+			return cp(x) != cp(y)? cp(x) < cp(y) : prob[x].nl > prob[y].nl; });
+		for (size_t t = 0; t + 1 < cand.size();) {
+			if (cp(cand[t]) != cp(cand[t + 1])) { ++t; continue; }
+			partner[cand[t]] = cand[t + 1], second[cand[t + 1]] = 1;
+			--n_grp[group_of(prob[cand[t + 1]].C)];
+			t += 2;
+		}
+	}
+	// queue order: small-C problems first (their launch), then large; the most wave time first in
+	// each (rows x per-row instructions, about 41 + 30 C for one problem and 49 + 30 C for a pair)
+	auto cost = [&](int32_t k) -> int64_t {
+		if (partner[k] < 0) return (int64_t)prob[k].nl * (41 + 30 * prob[k].C);
+		return (int64_t)std::max(prob[k].nl, prob[partner[k]].nl) * (49 + 30 * ((prob[k].npad + 63) / 64));
+	};
+	for (int32_t k = 0; k < n; ++k) if (!is_wide[k] && !second[k]) order.push_back(k);
 	std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
 		const int bx = group_of(prob[x].C), by = group_of(prob[y].C);
 		if (bx != by) return bx < by;
-		return (int64_t)prob[x].nl * prob[x].C > (int64_t)prob[y].nl * prob[y].C; });
+		return cost(x) > cost(y); });
+	const int32_t n_items = (int32_t)order.size();
+	for (int32_t k = 0; k < n_items; ++k) if (partner[order[k]] >= 0) order.push_back(partner[order[k]]);
 	const int32_t n_narrow = (int32_t)order.size();
 	p->prob_h.resize(n_narrow);
 	for (int32_t k = 0; k < n_narrow; ++k) p->prob_h[k] = prob[order[k]];
+	for (int32_t k = 0, t = n_items; k < n_items; ++k) {
+		if (partner[order[k]] < 0) continue;
+		ExtProb &e = p->prob_h[k];
+		e.pair = t++, e.C = (e.npad + 63) / 64;   // the partner's index; C per lane per problem
+	}
 	p->out_h.assign((size_t)4 * n, 0);
 	// inputs staged in pinned memory: ns | as | ss | pssm | pen | prob | prep | pc | mat | wide
 	const int32_t n_pen = max_nl + 2;
@@ -505,7 +539,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		for (int g = 0; g < kGroups; ++g) {
 			ExtArgs &a = p->args[g];
 			p->grp_first[g] = (int32_t)first, p->grp_n[g] = n_grp[g];
-			a.prob = p->d_prob + first, a.n = n_grp[g];
+			a.prob = p->d_prob + first, a.n = n_grp[g], a.pbase = p->d_prob;
 			first += n_grp[g];
 			a.rec = p->d_rec, a.aas = p->d_aas, a.pssm = p->d_pssm, a.mat = p->d_mat, a.asize = asize;
 			a.pen = p->d_pen, a.pen_n = n_pen;
@@ -629,8 +663,15 @@ extern "C" int nsh_ext_plan_run_subset(nsh_ext_plan_t *p, const uint8_t *keep)
 		if (keep[k]) p->idx_h.push_back(k);
 	for (int g = 0; g < kGroups; ++g) {
 		const int32_t first = (int32_t)p->sub_h.size();
-		for (int32_t k = p->grp_first[g]; k < p->grp_first[g] + p->grp_n[g]; ++k)
-			if (keep[p->prob_h[k].out]) p->sub_h.push_back(p->prob_h[k]);
+		for (int32_t k = p->grp_first[g]; k < p->grp_first[g] + p->grp_n[g]; ++k) {
+			ExtProb e = p->prob_h[k];
+			if (e.pair >= 0) {   // a pair runs as two single problems here
+				ExtProb f = p->prob_h[e.pair];
+				e.pair = -1, e.C = (e.npad + 127) / 128;
+				if (keep[f.out]) p->sub_h.push_back(f);
+			}
+			if (keep[e.out]) p->sub_h.push_back(e);
+		}
 		args[g] = p->args[g];
 		args[g].prob = p->d_prob_sub + first, args[g].n = (int32_t)p->sub_h.size() - first;
 		grid[g] = std::min(args[g].n, p->ncu * p->wpc[g]);
@@ -888,7 +929,7 @@ extern "C" int nsh_ext_run_noalloc(int32_t n, const char *ns, const int64_t *ns_
 	int64_t first = 0;
 	for (int g = 0; g < kGroups; ++g) {
 		ExtArgs a = {};
-		a.prob = d_prob + first, a.n = n_grp[g];
+		a.prob = d_prob + first, a.n = n_grp[g], a.pbase = d_prob;
 		first += n_grp[g];
 		if (a.n == 0) continue;
 		a.rec = (const uint32_t*)(base + o_rec), a.aas = base + o_aas, a.pssm = nullptr, a.mat = (const int8_t*)(base + o_mat);

@@ -157,10 +157,10 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, 
 		return d < 2? 0 : sld(a.pen, d < a.pen_n? d : a.pen_n - 1);
 	};
 	int d0 = 0, dp1 = 0, am1 = 0, a0 = 0, dm1 = 0, am2 = 0;
-	int rc1 = 0, rc2 = 0, rc3 = 0; // records of rows i, i+1 and i+2 on entry to row i
+	int rc1 = 0, rc2 = 0;          // records of rows i and i+1 on entry to row i
 	int pc1 = 0;                   // the length penalty of row i
-	int trow = 0;                  // table path: the table row of row i+1's residue
-	// profile bytes / table values of the next row, loaded one row ahead
+	int trow = 0;                  // table path: the table row of row i's residue (read a row ahead)
+	// profile bytes (profile path, loaded one row ahead) or table values (gathered at the row's start)
 	int pf_lo[C], pf_hi[C];
 	auto fetch_row = [&](int res) -> int { return (int)tab[res * TAB_W + trow_col]; };
 	auto prefetch_tab = [&](int tr) {
@@ -179,12 +179,9 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, 
 		am1 = (int)(int8_t)(r0 >> 16), a0 = (int)(int8_t)(r1 >> 16);
 		rc1 = sld(rec, 2);
 		if (nl > 3) rc2 = sld(rec, 3);
-		if (nl > 4) rc3 = sld(rec, 4);
 		pc1 = pen_of(2);
-		if (!PROF) {
-			prefetch_tab(fetch_row(rc1 & 0xff));
-			if (nl > 3) trow = fetch_row(rc2 & 0xff);
-		} else if (C <= 8) prefetch_prof(rc1 & 0xff);
+		if (!PROF) trow = fetch_row(rc1 & 0xff);
+		else if (C <= 8) prefetch_prof(rc1 & 0xff);
 	}
 
 	int best_log = INT_MIN, best_i = -1, last_i = 1, i = 2, last_ev = NEG;
@@ -193,9 +190,9 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, 
 		constexpr int R = decltype(Rc)::value, R2 = (R + 1) % 3, R1 = (R + 2) % 3;
 		s2 *H3 = Hs[R], *H2 = Hs[R2], *H1 = Hs[R1];
 		int *D3 = dring + R * C * 64 + L;
-		const int rc = rc1, rn = rc2, rnn = rc3, pen_i = pc1;
-		rc1 = rn, rc2 = rnn;
-		rc3 = i + 3 < nl? sld(rec, i + 3) : 0;
+		const int rc = rc1, rn = rc2, pen_i = pc1;
+		rc1 = rn;
+		rc2 = i + 2 < nl? sld(rec, i + 2) : 0;
 		pc1 = pen_of(i + 1);
 		const int res = rc & 0xff;
 		dm1 = d0, d0 = dp1, dp1 = (int)(int8_t)(rc >> 8);
@@ -203,18 +200,17 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, 
 		const s2 GEI = spl(res == 20? fs : ge);
 		const s2 DM1 = spl(dm1), D0 = spl(d0), DP1 = spl(dp1), A0 = spl(a0), AM1 = spl(am1), AM2 = spl(am2);
 
-		// this row's scores (fetched during the previous row), then the next row's
+		// this row's scores: the table path gathers them from the table row read during the previous
+		// row (and reads the next one); the profile path loaded them during the previous row
 		s2 s[C];
-		if (PROF && C > 8) prefetch_prof(res);
+		if (!PROF) {
+			prefetch_tab(trow);
+			if (i + 1 < nl) trow = fetch_row(rn & 0xff);
+		} else if (C > 8) prefetch_prof(res);
 #pragma unroll
 		for (int k = 0; k < C; ++k)   // table path: the low halves of the two words
 			s[k] = up((int)__builtin_amdgcn_perm((uint32_t)pf_hi[k], (uint32_t)pf_lo[k], PROF? sel[k] : 0x05040100u));
-		if (i + 1 < nl) {
-			if (!PROF) {
-				prefetch_tab(trow);
-				if (i + 2 < nl) trow = fetch_row(rnn & 0xff);
-			} else if (C <= 8) prefetch_prof(rn & 0xff);
-		}
+		if (PROF && C <= 8 && i + 1 < nl) prefetch_prof(rn & 0xff);
 
 		// phase A: every term but the insertion; independent across columns
 		const s2 n1 = nh[R1], n2 = nh[R2], n3 = nh[R];
@@ -352,12 +348,264 @@ DEV void ext_solve(const ExtArgs &a, const ExtProb &P, int8_t *prof, int *snap, 
 	}
 }
 
+// ---- two problems per wave (matrix scores) ----
+// Problem A lives in the low 16-bit halves and problem B in the high halves, each with C columns per
+// lane (column c of either problem: lane c / C, slot c % C). The wave's fixed per-row work (the lane
+// scan, the row maximum, the neighbour exchange, the loop control) is shared by two problems, and a
+// problem's padding granularity is 64 columns instead of 128. Everything that belongs to one problem
+// (rows, records, splice signals, length penalty, best row, x-drop) is tracked per half; a problem
+// that has stopped leaves its half running on neutral records until the other one stops. The host
+// pairs problems of equal C with at least 3 rows each.
+DEV int pk2(int lo, int hi) { return (lo & 0xffff) | (hi << 16); }
+DEV int sx8(int v) { return (int)(int8_t)(v & 0xff); }
+
+template <int C>
+DEV void ext_solve2(const ExtArgs &a, const ExtProb &PA, const ExtProb &PB, int8_t *prof, int *snap, int *dring, const int L)
+{
+	constexpr bool LANE_MASK = (8 % C) == 0;
+	const int go = a.go, ge = a.ge, fs = a.fs, eb = a.eb, xdrop = a.xdrop;
+	const int nlA = PA.nl, nlB = PB.nl, alA = PA.al, alB = PB.al;
+	const int nlmax = nlA > nlB? nlA : nlB;
+	const uint32_t *recA = a.rec + PA.rec_off, *recB = a.rec + PB.rec_off;
+	const int plenA = 3 * alA, plenB = 3 * alB;
+
+	// per-lane column constants: row-maximum masks, ds_bpermute sources (B's scores sit in lanes 32..54)
+	s2 lim[LANE_MASK? 1 : C];
+	int bl[C], bh[C];
+	{
+		const uint8_t *aasA = a.aas + PA.aa_off, *aasB = a.aas + PB.aa_off;
+#pragma unroll
+		for (int k = 0; k < C; ++k) {
+			const int c = L * C + k;
+			if (!LANE_MASK) lim[k] = s2{(short)(c < PA.npad? 32767 : NEG), (short)(c < PB.npad? 32767 : NEG)};
+			bl[k] = (c < alA? aasA[c] : TAB_PAD) * 4;
+			bh[k] = (32 + (c < alB? aasB[c] : TAB_PAD)) * 4;
+		}
+	}
+	if (LANE_MASK) lim[0] = s2{(short)(L * C < PA.npad? 32767 : NEG), (short)(L * C < PB.npad? 32767 : NEG)};
+	const int trow_col = (L & 31) < TAB_W? (L & 31) : TAB_PAD;
+	const bool hiL = L >= 32;
+	const lds_u16_t tab = (lds_u16_t)prof;
+
+	const int geC = ge * C;
+	const u2 DEC1 = splu(geC), DEC2 = splu(2 * geC), DEC4 = splu(4 * geC), DEC8 = splu(8 * geC);
+	const u2 DEC15 = splu(geC * ((L & 15) + 1));
+	const u2 DEC31 = splu(L >= 32? geC * (L - 31) : 0xffff);
+	const s2 GO = spl(go), GE = spl(ge), FS = spl(fs), GE2 = spl(2 * ge);
+	const s2 IO = s2{(short)PA.io, (short)PB.io};
+
+	s2 Hs[3][C], A[C], B[C], Cc[C], nh[3];
+#pragma unroll
+	for (int k = 0; k < C; ++k) {
+		Hs[0][k] = Hs[1][k] = Hs[2][k] = up(PACKNEG);
+		dring[(0 * C + k) * 64 + L] = dring[(1 * C + k) * 64 + L] = dring[(2 * C + k) * 64 + L] = PACKNEG;
+		A[k] = B[k] = Cc[k] = up(PACKNEG);
+		snap[k * 64 + L] = PACKNEG;
+	}
+	// column -1 of both problems: H(-1,-1) = 0 and H(0,-1) = H(1,-1) = -fs for row 2 only
+	nh[0] = nh[1] = nh[2] = up(PACKNEG);
+	if (L == 0) nh[0] = s2{0, 0}, nh[1] = nh[2] = s2{(short)-fs, (short)-fs};
+
+	const int LeA = (alA - 1) / C, keA = (alA - 1) % C, LeB = (alB - 1) / C, keB = (alB - 1) % C;
+
+	auto pen_of = [&](int r, int plen) -> int {
+		const int d = r - plen;
+		return d < 2? 0 : sld(a.pen, d < a.pen_n? d : a.pen_n - 1);
+	};
+	auto rec_of = [&](const uint32_t *rec, int nl, int r) -> int { return r < nl? sld(rec, r) : 0; };
+	auto fetch_rows = [&](int resA, int resB) -> int { return (int)tab[(hiL? resB : resA) * TAB_W + trow_col]; };
+	int pf_lo[C], pf_hi[C];
+	auto prefetch_tab = [&](int tr) {
+#pragma unroll
+		for (int k = 0; k < C; ++k)
+			pf_lo[k] = __builtin_amdgcn_ds_bpermute(bl[k], tr), pf_hi[k] = __builtin_amdgcn_ds_bpermute(bh[k], tr);
+	};
+
+	// splice signals of rows i-1..i+1 as (A, B) pairs; records of rows i and i+1 and the penalty of row i
+	int d0p, dp1p, am1p, a0p, dm1p = 0, am2p = 0;
+	int rc1A, rc2A, rc1B, rc2B, pc1A, pc1B, trow;
+	{
+		const int r0A = sld(recA, 0), r1A = sld(recA, 1), r0B = sld(recB, 0), r1B = sld(recB, 1);
+		d0p = pk2(sx8(r0A >> 8), sx8(r0B >> 8)), dp1p = pk2(sx8(r1A >> 8), sx8(r1B >> 8));
+		am1p = pk2(sx8(r0A >> 16), sx8(r0B >> 16)), a0p = pk2(sx8(r1A >> 16), sx8(r1B >> 16));
+		rc1A = sld(recA, 2), rc2A = rec_of(recA, nlA, 3);
+		rc1B = sld(recB, 2), rc2B = rec_of(recB, nlB, 3);
+		pc1A = pen_of(2, plenA), pc1B = pen_of(2, plenB);
+		trow = fetch_rows(rc1A & 0xff, rc1B & 0xff);
+	}
+
+	int bestA = INT_MIN, bestiA = -1, lastA = 1, levA = NEG;
+	int bestB = INT_MIN, bestiB = -1, lastB = 1, levB = NEG;
+	bool liveA = true, liveB = true;
+	int i = 2;
+
+	auto row = [&](auto Rc) -> bool {
+		constexpr int R = decltype(Rc)::value, R2 = (R + 1) % 3, R1 = (R + 2) % 3;
+		s2 *H3 = Hs[R], *H2 = Hs[R2], *H1 = Hs[R1];
+		int *D3 = dring + R * C * 64 + L;
+		const int rcA = rc1A, rcB = rc1B, rnA = rc2A, rnB = rc2B, penA = pc1A, penB = pc1B;
+		rc1A = rc2A, rc2A = rec_of(recA, nlA, i + 2);
+		rc1B = rc2B, rc2B = rec_of(recB, nlB, i + 2);
+		pc1A = pen_of(i + 1, plenA), pc1B = pen_of(i + 1, plenB);
+		const int resA = rcA & 0xff, resB = rcB & 0xff;
+		dm1p = d0p, d0p = dp1p, dp1p = pk2(sx8(rcA >> 8), sx8(rcB >> 8));
+		am2p = am1p, am1p = a0p, a0p = pk2(sx8(rcA >> 16), sx8(rcB >> 16));
+		const s2 GEI = up(pk2(resA == 20? fs : ge, resB == 20? fs : ge));
+		const s2 DM1 = up(dm1p), D0 = up(d0p), DP1 = up(dp1p), A0 = up(a0p), AM1 = up(am1p), AM2 = up(am2p);
+
+		// this row's scores from the table rows read during the previous row; then the next rows'
+		s2 s[C];
+		prefetch_tab(trow);
+		if (i + 1 < nlmax) trow = fetch_rows(rnA & 0xff, rnB & 0xff);
+#pragma unroll
+		for (int k = 0; k < C; ++k) s[k] = up((int)__builtin_amdgcn_perm((uint32_t)pf_hi[k], (uint32_t)pf_lo[k], 0x05040100u));
+
+		// phases A and B: as for one problem (every operation acts on the two halves separately)
+		const s2 n1 = nh[R1], n2 = nh[R2], n3 = nh[R];
+		s2 u[C], m[C], nonI[C];
+#pragma unroll
+		for (int k = 0; k < C; ++k) u[k] = subs(H1[k], IO), m[k] = vmax(H1[k], H2[k]);
+		const s2 u_n = subs(n1, IO), m_n = vmax(n1, n2);
+#pragma unroll
+		for (int k = 0; k < C; ++k) {
+			const s2 h3p = k? H3[k - 1] : n3, up_ = k? u[k - 1] : u_n, mp = k? m[k - 1] : m_n;
+			const s2 hm = adds(h3p, s[k]);
+			const s2 dn = subs(vmax(subs(H3[k], GO), up(D3[k * 64])), GEI);
+			A[k] = vmax(subs(u[k], DM1), A[k]);
+			B[k] = vmax(subs(up_, D0), B[k]);
+			Cc[k] = vmax(subs(up_, DP1), Cc[k]);
+			const s2 f = subs(vmax(m[k], mp), FS);
+			nonI[k] = vmax(vmax(vmax(hm, dn), vmax(subs(A[k], A0), subs(B[k], AM2))), vmax(subs(Cc[k], AM1), f));
+			D3[k * 64] = pk(dn);
+		}
+		s2 I = up(PACKNEG), hh[C];
+#pragma unroll
+		for (int k = 0; k < C; ++k) {
+			if (k) I = subs(vmax(subs(nonI[k - 1], GO), I), GE);
+			hh[k] = vmax(nonI[k], I);
+		}
+		const s2 X = subs(vmax(subs(nonI[C - 1], GO), I), GE);
+
+		// lane scan of the outgoing insertions, both halves at once; lane 0 takes no carry
+		int S = pk(X) ^ BIAS;
+		S = pku(maxu(upu(S), subu(upu(dpp0<ROW_SHR1>(S)), DEC1)));
+		S = pku(maxu(upu(S), subu(upu(dpp0<ROW_SHR2>(S)), DEC2)));
+		S = pku(maxu(upu(S), subu(upu(dpp0<ROW_SHR4>(S)), DEC4)));
+		S = pku(maxu(upu(S), subu(upu(dpp0<ROW_SHR8>(S)), DEC8)));
+		S = pku(maxu(upu(S), subu(upu(dpp0<ROW_BCAST15>(S)), DEC15)));
+		S = pku(maxu(upu(S), subu(upu(dpp0<ROW_BCAST31>(S)), DEC31)));
+		const s2 cin = up(dpp0<WAVE_SHR1>(S) ^ BIAS);
+
+		s2 mk[C], ck[2] = {cin, subs(cin, GE)};
+#pragma unroll
+		for (int k = 0; k < C; ++k) {
+			if (k >= 2) ck[k & 1] = subs(ck[k & 1], GE2);
+			const s2 hk = vmax(hh[k], ck[k & 1]);
+			H3[k] = hk;
+			mk[k] = LANE_MASK? hk : vmin(hk, lim[k]);
+		}
+#pragma unroll
+		for (int w = 1; w < C; w *= 2)
+#pragma unroll
+			for (int k = 0; k + w < C; k += 2 * w) mk[k] = vmax(mk[k], mk[k + w]);
+		// both problems' row maxima at once, as biased unsigned pairs (a source-less lane reads 0)
+		int mx = pk(LANE_MASK? vmin(mk[0], lim[0]) : mk[0]) ^ BIAS;
+		mx = pku(maxu(upu(mx), upu(dpp0<ROW_SHR1>(mx))));
+		mx = pku(maxu(upu(mx), upu(dpp0<ROW_SHR2>(mx))));
+		mx = pku(maxu(upu(mx), upu(dpp0<ROW_SHR4>(mx))));
+		mx = pku(maxu(upu(mx), upu(dpp0<ROW_SHR8>(mx))));
+		mx = pku(maxu(upu(mx), upu(dpp0<ROW_BCAST15>(mx))));
+		mx = pku(maxu(upu(mx), upu(dpp0<ROW_BCAST31>(mx))));
+		const int rmx = __builtin_amdgcn_readlane(mx, 63);
+
+		// H(i, al-1) of each problem
+		int evA = 0, evB = 0;
+		switch (keA) {
+#define NSH_CASE(K) case K: if (K < C) evA = __builtin_amdgcn_readlane(pk(H3[K < C? K : 0]), LeA); break;
+		NSH_CASE(0) NSH_CASE(1) NSH_CASE(2) NSH_CASE(3)
+#undef NSH_CASE
+		default: break;
+		}
+		switch (keB) {
+#define NSH_CASE(K) case K: if (K < C) evB = __builtin_amdgcn_readlane(pk(H3[K < C? K : 0]), LeB); break;
+		NSH_CASE(0) NSH_CASE(1) NSH_CASE(2) NSH_CASE(3)
+#undef NSH_CASE
+		default: break;
+		}
+		evA = lo16(evA), evB = hi16(evB);
+
+		// row score, length penalty, best row and x-drop of each live problem (nasw-sse.c:438-447)
+		if (liveA) {
+			const int rowmax = (rmx & 0xffff) - 32768, es = evA + eb;
+			const int tlog = (rowmax > es? rowmax : es) - penA;
+			levA = evA;
+			if (tlog > bestA) {
+				bestA = tlog, bestiA = i;
+#pragma unroll
+				for (int k = 0; k < C; ++k) ((short*)snap)[2 * (k * 64 + L)] = H3[k].x;
+			}
+			lastA = i;
+			if (bestA - tlog > xdrop || i + 1 >= nlA) liveA = false;
+		}
+		if (liveB) {
+			const int rowmax = (int)((unsigned)rmx >> 16) - 32768, es = evB + eb;
+			const int tlog = (rowmax > es? rowmax : es) - penB;
+			levB = evB;
+			if (tlog > bestB) {
+				bestB = tlog, bestiB = i;
+#pragma unroll
+				for (int k = 0; k < C; ++k) ((short*)snap)[2 * (k * 64 + L) + 1] = H3[k].y;
+			}
+			lastB = i;
+			if (bestB - tlog > xdrop || i + 1 >= nlB) liveB = false;
+		}
+		// left neighbour of the next row: lane L-1's last column; lane 0 keeps column -1 (neg_inf)
+		nh[R] = up(__builtin_amdgcn_update_dpp(PACKNEG, pk(H3[C - 1]), WAVE_SHR1, 0xf, 0xf, false));
+		++i;
+		return liveA || liveB;
+	};
+
+	bool more = row(std::integral_constant<int, 0>());
+	if (L == 0) nh[1] = nh[2] = up(PACKNEG);
+	while (more) {
+		if (!row(std::integral_constant<int, 1>())) break;
+		if (!row(std::integral_constant<int, 2>())) break;
+		if (!row(std::integral_constant<int, 0>())) break;
+	}
+
+	// each problem's first real column reaching its best row's maximum (nasw-sse.c:448-458)
+#pragma unroll
+	for (int h = 0; h < 2; ++h) {
+		const ExtProb &P = h? PB : PA;
+		const int al = P.al;
+		int bv = INT_MIN, bj = INT_MAX;
+#pragma unroll
+		for (int k = 0; k < C; ++k) {
+			const int j = L * C + k;
+			if (j < al) {
+				const s2 hm = up(snap[k * 64 + L]);
+				const int v = (h? (int)hm.y : (int)hm.x) + (j == al - 1? eb : 0);
+				if (v > bv) bv = v, bj = j;
+			}
+		}
+#pragma unroll
+		for (int off = 32; off >= 1; off >>= 1) {
+			const int ov = __shfl_xor(bv, off), oj = __shfl_xor(bj, off);
+			if (ov > bv || (ov == bv && oj < bj)) bv = ov, bj = oj;
+		}
+		int32_t *o = a.out + 4 * (int64_t)P.out;
+		if (P.mode) o[0] = P.nl, o[1] = al, o[2] = h? levB : levA;
+		else o[0] = (h? bestiB : bestiA) + 1, o[1] = bj + 1, o[2] = bv;
+		o[3] = (h? lastB : lastA) - 1;
+	}
+}
+
 constexpr int64_t kLongWork = (int64_t)1 << 18;   // rows x 128-column chunks of a long problem
 
 // persistent waves over one group's queue; CLO..CHI columns per lane per half. The waves-per-SIMD
 // target trades registers for latency hiding (measured: 6 / 3 / 1 for C <= 4 / <= 8 / <= 16).
 template <int CLO, int CHI, bool PROF>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CHI <= 4? 6 : CHI <= 8? 3 : 1)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CHI <= 4? NSH_EXT_WPE0 : CHI <= 8? NSH_EXT_WPE1 : 1)))
 ext_dp_kernel(ExtArgs a)
 {
 	__shared__ __attribute__((aligned(16))) int8_t prof[PROF? NRES * 128 * CHI : NRES * TAB_W * 2];
@@ -377,6 +625,15 @@ ext_dp_kernel(ExtArgs a)
 		const int p = __builtin_amdgcn_readfirstlane(atomicAdd(a.counter, L == 0? 1 : 0));
 		if (p >= a.n) break;
 		const ExtProb P = a.prob[p];
+		if (!PROF && CHI <= 4 && P.pair >= 0) {   // two problems of P.C columns per lane each
+			const ExtProb Q = a.pbase[P.pair];
+			if ((int64_t)(P.nl > Q.nl? P.nl : Q.nl) * P.C >= kLongWork) __builtin_amdgcn_s_setprio(2);
+			else __builtin_amdgcn_s_setprio(0);
+#define NSH_C2(K) if (!PROF && K <= CHI && P.C == K) { ext_solve2<(K <= CHI)? K : 1>(a, P, Q, prof, snap, dring, L); continue; }
+			NSH_C2(1) NSH_C2(2) NSH_C2(3) NSH_C2(4)
+#undef NSH_C2
+			continue;
+		}
 		const int C = P.C < CLO? CLO : P.C > CHI? CHI : P.C;
 		// a problem of many rows (an extension over a window of tens of kilobases) sets the length of
 		// the whole launch: its wave gets the issue priority over the waves of short problems sharing

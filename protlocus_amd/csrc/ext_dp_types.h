@@ -12,10 +12,13 @@ struct ExtProb {
 	int32_t nl, al, npad, io;
 	int32_t out, C, left;
 	int32_t mode;      // 0: extension result; 1: plain score-only global DP, score H(last row, al-1)
+	int32_t pair = -1; // >= 0: this problem and prob[pair] (from ExtArgs::pbase) share a wave, C per lane each
+	int32_t pad_ = 0;
 };
 
 struct ExtArgs {
 	const ExtProb *prob; int32_t n;
+	const ExtProb *pbase;   // the whole descriptor array (pair partners)
 	const uint32_t *rec; const uint8_t *aas; const int8_t *pssm; const int8_t *mat; int32_t asize;
 	const int32_t *pen; int32_t pen_n;
 	int32_t go, ge, fs, xdrop, eb;
@@ -32,7 +35,13 @@ constexpr int kGroupLo[kGroups] = { 1, 5, 9 }, kGroupHi[kGroups] = { 4, 8, 16 };
 // (the register budget that measured fastest: more waves hide the per-row dependency chain better
 // than the few spilled registers cost); with per-problem profiles LDS (22 x 128C bytes per wave)
 // also bounds residency.
-constexpr int kGroupWavesPerCU[kGroups] = { 24, 12, 3 };
+#ifndef NSH_EXT_WPE0
+#define NSH_EXT_WPE0 6
+#endif
+#ifndef NSH_EXT_WPE1
+#define NSH_EXT_WPE1 3
+#endif
+constexpr int kGroupWavesPerCU[kGroups] = { 4 * NSH_EXT_WPE0, 4 * NSH_EXT_WPE1, 3 };
 constexpr int kGroupWavesPerCUPssm[kGroups] = { 12, 7, 3 };
 constexpr int kCmax = 16;
 inline int group_of(int C) { return C <= 4? 0 : C <= 8? 1 : 2; }

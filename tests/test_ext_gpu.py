@@ -182,3 +182,51 @@ def test_ext_plan_config1_batch(oracle, product, seed, n, lo, hi):
         exp = list(ex.map(one, range(n)))
     bad = [k for k in range(n) if got[k] != exp[k]]
     assert not bad, [(k, len(pairs[k][0]), len(pairs[k][1]), got[k], exp[k]) for k in bad[:5]]
+
+
+@pytest.mark.parametrize("xdrop,global_mode", [(20, False), (60, False), (100, True)])
+def test_ext_plan_two_problems_per_wave(oracle, product, xdrop, global_mode):
+    # problems of at most 256 padded columns share a wave two at a time (one per 16-bit half):
+    # lengths 1-250 aa, pairs whose partners stop at very different rows (small x-drop, targets of
+    # 3 to 3,000 nt), both directions in one plan, or the score-only global mode; every result must
+    # equal the oracle's, like the one-problem-per-wave path
+    from protlocus_amd import synth
+    rng = np.random.default_rng(xdrop)
+    pairs = []
+    for k in range(300):
+        al = int(rng.integers(1, 251))
+        p = synth.rand_protein(rng, al)
+        if k % 3 == 0:   # random DNA of 3 to 3,000 nt: an extension stops early or runs to the end
+            d = synth.rand_dna(rng, int(rng.integers(3, 3000)))
+        else:            # the protein's own gene (introns, frameshifts, 200-nt flanks)
+            d = synth.gene_of(rng, p, 0.2, int(rng.integers(0, 3)), lambda: int(rng.integers(40, 400)), 1, 200)
+        pairs.append((d, p))
+    ns, ns_off, nl, as_, as_off, al = synth.pack_batch(pairs)
+    if global_mode:
+        flag, dirs = 0, None
+    else:
+        flag, dirs = NS_F_EXT_RIGHT, rng.integers(0, 2, size=len(pairs)).astype(np.uint8)
+    plan = product.ext_plan(ns, ns_off, nl, as_, as_off, al, product.opt(flag=flag, xdrop=xdrop), dirs=dirs)
+    try:
+        plan.run()
+        got = plan.results()
+        keep = np.zeros(len(pairs), np.uint8)
+        keep[::2] = 1           # a subset run unpacks the pairs into single problems
+        assert product.lib.nsh_ext_plan_run_subset(plan.p, bytes(keep)), product.error()
+        assert product.lib.nsh_ext_plan_sync(plan.p)
+        sub = plan.results()
+    finally:
+        plan.close()
+
+    def one(k):
+        d, p = pairs[k]
+        if global_mode:   # ns_global_gs16b without CIGAR or EXT flags: (nl, al, H(last row, al-1))
+            return (len(d), len(p), oracle.global_dp(d, p, oracle.opt(flag=0, xdrop=xdrop), mode=16)[2])
+        kw = dict(flag=NS_F_EXT_LEFT if dirs[k] else NS_F_EXT_RIGHT, xdrop=xdrop)
+        return _oracle_ext(oracle, product, d, p, kw, None)
+    with cf.ThreadPoolExecutor(8) as ex:
+        exp = list(ex.map(one, range(len(pairs))))
+    bad = [k for k in range(len(pairs)) if got[k] != exp[k]]
+    assert not bad, [(k, len(pairs[k][0]), len(pairs[k][1]), got[k], exp[k]) for k in bad[:5]]
+    bad = [k for k in range(0, len(pairs), 2) if sub[k] != exp[k]]
+    assert not bad, [(k, sub[k], exp[k]) for k in bad[:5]]
