@@ -53,7 +53,7 @@ MLP_FLOP_PER_SAMPLE = 2 * (64 * 256 + 256 * 256 + 256)
 # pairs, seed 1): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of one step, each scaled by what the
 # counter reads for a 1-GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); the summary and the
 # commands are in profiles/ext_dp_kernel_65536pairs.md
-EXT_DP_TRAFFIC_BYTES = 952.3e6
+EXT_DP_TRAFFIC_BYTES = 1277.4e6
 
 
 def threads():
