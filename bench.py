@@ -39,7 +39,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 # An extension plan runs its column-count groups on streams of their own, which overlap only on
 # distinct hardware queues; HIP's default of 4 queues per process is shared by every stream the
 # engines create, so give the process more before HIP starts (the runtime reads it once).
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "32")
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, ROOT)
 

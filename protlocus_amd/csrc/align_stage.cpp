@@ -138,6 +138,25 @@ static int ext_round(std::vector<Cand*> &cs, std::vector<nsh_ext_req> &req, cons
 	res.assign(req.size(), nsh_ext_result_t{});
 	if (req.empty()) return 1;
 	StepClock clk;
+	if (clk.on) {   // the batch's shape: rows, rows x padded columns, and its longest problems
+		int64_t rows = 0, cells = 0, nl_max = 0, n_long = 0, rows_long = 0;
+		for (const nsh_ext_req &q : req) {
+			rows += q.nl, cells += (int64_t)q.nl * q.npad, nl_max = std::max<int64_t>(nl_max, q.nl);
+			if (q.nl >= 20000) ++n_long, rows_long += q.nl;
+		}
+		std::fprintf(stderr, "[nsh_align_stage]   batch: %zu problems, %lld rows, %.3g padded cells, longest %lld rows, "
+				"%lld of >= 20,000 rows (%lld rows)\n", req.size(), (long long)rows, (double)cells, (long long)nl_max,
+				(long long)n_long, (long long)rows_long);
+		std::vector<std::pair<int64_t, int32_t>> big;
+		for (int32_t k = 0; k < (int32_t)req.size(); ++k) big.push_back({(int64_t)req[k].nl * ((req[k].npad + 127) / 128), k});
+		std::sort(big.begin(), big.end(), std::greater<std::pair<int64_t, int32_t>>());
+		int64_t hist[5] = {};
+		for (const nsh_ext_req &q : req) hist[std::min(4, (q.npad - 1) / 128)] += q.nl;
+		std::fprintf(stderr, "[nsh_align_stage]   rows by padded width <=128/256/384/512/more: %lld %lld %lld %lld %lld; costliest (nl, al):",
+				(long long)hist[0], (long long)hist[1], (long long)hist[2], (long long)hist[3], (long long)hist[4]);
+		for (size_t t = 0; t < big.size() && t < 8; ++t) std::fprintf(stderr, " (%d, %d)", req[big[t].second].nl, req[big[t].second].al);
+		std::fprintf(stderr, "\n");
+	}
 	nsh_ext_plan_t *p = nsh_ext_plan_build((int32_t)req.size(), req.data(), &o);
 	clk.lap("  plan build", (int64_t)req.size());
 	const int ok = p && nsh_ext_plan_run(p) && nsh_ext_plan_results(p, res.data());

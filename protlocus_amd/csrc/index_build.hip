@@ -501,6 +501,7 @@ int collect_anchors(const nsh_index_t *ix, int32_t n_q, const char *seq, const i
 	std::vector<int32_t> nseed(n_q);
 	AnArgs a = {};
 	aoff.assign(n_q + 1, 0);
+	nsh_step_clock clk("collect_anchors");
 	{
 		ANOK(hipMalloc((void**)&d_seq, (size_t)l_seq + 8));
 		ANOK(hipMalloc((void**)&d_aa13, 256));
@@ -531,6 +532,7 @@ int collect_anchors(const nsh_index_t *ix, int32_t n_q, const char *seq, const i
 		ANOK(hipGetLastError());
 		ANOK(hipMemcpyAsync(nseed.data(), d_nseed, (size_t)n_q * 4, hipMemcpyDeviceToHost, s));
 		ANOK(hipStreamSynchronize(s));
+		clk.lap("upload and seeds", l_seed);
 		// the bucket sizes of each query's seeds, sorted (segments of the per-query regions)
 		for (int32_t q = 0; q < n_q; ++q) soff[q] = sbase[q], send[q] = sbase[q] + nseed[q];
 		ANOK(hipMemcpyAsync(d_soff, soff.data(), (size_t)n_q * 8, hipMemcpyHostToDevice, s));
@@ -546,6 +548,7 @@ int collect_anchors(const nsh_index_t *ix, int32_t n_q, const char *seq, const i
 		ANOK(hipGetLastError());
 		ANOK(hipMemcpyAsync(nan.data(), d_nanch, (size_t)n_q * 8, hipMemcpyDeviceToHost, s));
 		ANOK(hipStreamSynchronize(s));
+		clk.lap("seed sort and counts");
 		for (int32_t q = 0; q < n_q; ++q) aoff[q + 1] = aoff[q] + nan[q];
 		const int64_t n_a = aoff[n_q];
 		if (n_a > INT_MAX) { nsh_set_error("anchors: %lld anchors exceed one sort", (long long)n_a); goto done; }
@@ -568,6 +571,7 @@ int collect_anchors(const nsh_index_t *ix, int32_t n_q, const char *seq, const i
 				0, 64, s));
 		ANOK(hipStreamSynchronize(s));
 		nsh_count_launch(6);
+		clk.lap("anchors and sort");
 		*pd_aoff = d_aoff, *pd_anch = d_anch2;
 		d_aoff = nullptr, d_anch2 = nullptr;
 		ok = 1;
@@ -579,6 +583,7 @@ done:
 			d_cnt2, d_anch, d_anch2, d_tmp, d_sbase };
 		for (void *b : bufs) if (b) (void)hipFree(b);
 	}
+	clk.lap("free");
 	return ok;
 }
 } // namespace
@@ -597,9 +602,11 @@ extern "C" int64_t nsh_index_anchors(const nsh_index_t *ix, int32_t n_q, const c
 	int64_t *d_aoff = nullptr;
 	uint64_t *d_anch = nullptr;
 	std::vector<int64_t> aoff;
+	nsh_step_clock clk("nsh_index_chain");
 	{
 		ANOK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
 		if (!collect_anchors(ix, n_q, seq, q_off, q_len, max_occ, s, aoff, &d_aoff, &d_anch)) goto done;
+		clk.lap("anchors", aoff[n_q]);
 		const int64_t n_a = aoff[n_q];
 		if (max_anchors > 0 && n_a > 0)
 			if (!nsh_copy_to_host(anchors, d_anch, (size_t)std::min(n_a, max_anchors) * 8, s)) goto done;
@@ -664,9 +671,11 @@ extern "C" int64_t nsh_index_chain(const nsh_index_t *ix, int32_t n_q, const cha
 		ix->kmer, ix->bbit };
 	const nsh_chain::Params main = { opt->max_dist_x, opt->max_dist_y, opt->bw, opt->max_skip, opt->max_iter,
 		opt->min_cnt, opt->min_sc, opt->chn_coef_log, opt->is_spliced, ix->kmer, ix->bbit };
+	nsh_step_clock clk("nsh_index_chain");
 	{
 		ANOK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
 		if (!collect_anchors(ix, n_q, seq, q_off, q_len, max_occ, s, aoff, &d_aoff, &d_anch)) goto done;
+		clk.lap("anchors", aoff[n_q]);
 		const int64_t n_a = aoff[n_q];
 		const size_t na = (size_t)(n_a > 0? n_a : 1);
 		for (int32_t q = 0; q < n_q; ++q) cnt[q] = aoff[q + 1] - aoff[q];
@@ -690,6 +699,7 @@ extern "C" int64_t nsh_index_chain(const nsh_index_t *ix, int32_t n_q, const cha
 			ANOK(hipStreamSynchronize(s));
 			d_cnt = d_kept, h_cnt = kept.data();
 			nsh_count_launch(3);
+			clk.lap("pre-chain and sort");
 		}
 		if (!nsh_chain::run(s, n_q, d_aoff, d_cnt, h_cnt, d_anch, n_a, main, d_nu, d_u, d_ao)) goto done;
 		hipLaunchKernelGGL(kept_kernel, dim3((n_q + 255) / 256), dim3(256), 0, s, n_q, d_aoff, d_nu, d_u, d_kept,
@@ -698,6 +708,7 @@ extern "C" int64_t nsh_index_chain(const nsh_index_t *ix, int32_t n_q, const cha
 		ANOK(hipMemcpyAsync(nu.data(), d_nu, (size_t)n_q * 4, hipMemcpyDeviceToHost, s));
 		ANOK(hipMemcpyAsync(kept.data(), d_kept, (size_t)n_q * 8, hipMemcpyDeviceToHost, s));
 		ANOK(hipStreamSynchronize(s));
+		clk.lap("chain");
 		for (int32_t q = 0; q < n_q; ++q) u_off[q + 1] = u_off[q] + nu[q], c_off[q + 1] = c_off[q] + kept[q];
 		const int64_t n_uc = u_off[n_q], n_ac = c_off[n_q];
 		ANOK(hipMalloc((void**)&d_uoff, (size_t)(n_q + 1) * 8));
@@ -716,6 +727,7 @@ extern "C" int64_t nsh_index_chain(const nsh_index_t *ix, int32_t n_q, const cha
 		ANOK(hipStreamSynchronize(s));
 		nsh_count_launch(3);
 		total = n_uc;
+		clk.lap("pack and download", n_ac);
 	}
 done:
 	if (s) (void)hipStreamSynchronize(s);
@@ -724,5 +736,6 @@ done:
 		for (void *b : bufs) if (b) (void)hipFree(b);
 	}
 	if (s) (void)hipStreamDestroy(s);
+	clk.lap("free");
 	return total;
 }
