@@ -925,6 +925,17 @@ def main():
     gpu_res = plan.results()
     cells = plan.cells(1)          # evaluated cells of the last run
     win = plan.cells(0)
+    # the same step replayed as one captured hipGraph (preparation, both group kernels): its wall
+    # clock per step and whether every replay's results equal the stream-launched run's
+    plan.run(graph=True)           # capture (untimed)
+    prod.lib.nsh_device_synchronize()
+    g_same, t0 = True, time.perf_counter()
+    for _ in range(args.steps):
+        plan.run(sync=False, graph=True)
+        plan.prod.lib.nsh_ext_plan_sync(plan.p)
+    prod.lib.nsh_device_synchronize()
+    g_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    g_same = plan.results() == gpu_res
     plan.close()
     ms = wall / args.steps * 1e3
     gcups = cells / (ms * 1e-3) / 1e9
@@ -948,6 +959,8 @@ def main():
         "dp_kernel_ms": round(dp_avg, 3),
         # one step's work when the inputs start in host memory: the plan build (packing, PCIe upload)
         # added to a step; never the headline
+        "graph_replay": {"ms_per_step": round(g_ms, 3), "gcups": round(cells / (g_ms * 1e-3) / 1e9, 2),
+                         "identical_to_stream_run": g_same},
         "pcie_inclusive": {"plan_build_ms": round(plan_build_s * 1e3, 2),
                            "gcups": round(cells / ((ms + plan_build_s * 1e3) * 1e-3) / 1e9, 2)},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),

@@ -232,6 +232,10 @@ nsh_ext_plan_t *nsh_ext_plan_create(int32_t n, const char *ns, const int64_t *ns
 		const char *as, const int64_t *as_off, const int32_t *al, const uint8_t *ss,
 		const uint8_t *dirs, const ns_opt_t *opt);
 int nsh_ext_plan_run(nsh_ext_plan_t *p);                      /* enqueue on the plan's stream; 1 ok, 0 error */
+/* the same run as one hipGraph: captured on the first call (and again after the autotuner changed a
+ * persistent grid), replayed by later calls; same results. No reference counterpart: for callers
+ * that replay one resident batch many times. nsh_ext_plan_last_ms then covers the whole graph. */
+int nsh_ext_plan_run_graph(nsh_ext_plan_t *p);
 int nsh_ext_plan_sync(nsh_ext_plan_t *p);                     /* wait for the last run */
 /* run only the problems with keep[i] != 0 (input order); the others report nt_len = aa_len = 0,
  * score INT32_MIN and no evaluated rows */

@@ -220,7 +220,12 @@ struct nsh_ext_plan_s {
 	bool from_genome = false;              // some problems read their bases from a genome in HBM
 	hipStream_t stream = nullptr;
 	hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+	hipEvent_t evf = nullptr;              // fork point of the group streams (no timing: usable in a capture)
 	hipStream_t gs[kGroups] = {};          // one stream per column-count group
+	// graph mode: the whole run (preparation, group kernels, wide problems) captured once and replayed
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t gexec = nullptr;
+	int32_t ggrid[kGroups] = {};           // the persistent grids the graph was captured with
 	hipEvent_t gev[kGroups] = {};          // completion of each group's launch
 	// device buffers (grow-only in a pooled shell)
 	DBuf<uint8_t> b_ns, b_ss, b_as, b_aas;
@@ -275,8 +280,15 @@ struct nsh_ext_plan_s {
 			b_pen.bytes() + b_counter.bytes() + b_out.bytes() + b_idx_sub.bytes() + b_prob.bytes() + b_prob_sub.bytes() +
 			b_prep.bytes() + b_pc.bytes() + b_feat.bytes() + b_wide.bytes() + b_wide_sub.bytes();
 	}
+	void drop_graph()
+	{
+		if (gexec) (void)hipGraphExecDestroy(gexec);
+		if (graph) (void)hipGraphDestroy(graph);
+		gexec = nullptr, graph = nullptr;
+	}
 	void reset()   // forget the last plan, keep the resources
 	{
+		drop_graph();
 		n = 0, ran = 0, cells_window = 0, max_al = max_nl = 0;
 		al.clear(), out_h.clear(), prob_h.clear(), wide_h.clear(), wide_sub_h.clear(), sub_h.clear(), idx_h.clear();
 		d_ns = d_ss = d_as = d_aas = nullptr, d_rec = nullptr, d_mat = d_pssm = nullptr;
@@ -295,6 +307,7 @@ struct nsh_ext_plan_s {
 		if (ev0) (void)hipEventDestroy(ev0);
 		if (ev1) (void)hipEventDestroy(ev1);
 		if (ev2) (void)hipEventDestroy(ev2);
+		if (evf) (void)hipEventDestroy(evf);
 		for (int g = 0; g < kGroups; ++g) {
 			if (gev[g]) (void)hipEventDestroy(gev[g]);
 			if (gs[g]) (void)hipStreamDestroy(gs[g]);
@@ -302,7 +315,7 @@ struct nsh_ext_plan_s {
 		if (wev) (void)hipEventDestroy(wev);
 		if (ws) (void)hipStreamDestroy(ws);
 		if (stream) (void)hipStreamDestroy(stream);
-		ev0 = ev1 = ev2 = wev = nullptr, ws = stream = nullptr;
+		ev0 = ev1 = ev2 = evf = wev = nullptr, ws = stream = nullptr;
 		for (int g = 0; g < kGroups; ++g) gev[g] = nullptr, gs[g] = nullptr;
 	}
 };
@@ -323,6 +336,7 @@ nsh_ext_plan_t *shell_acquire()
 	nsh_ext_plan_t *p = new nsh_ext_plan_s();
 	bool ok = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess &&
 		hipEventCreate(&p->ev0) == hipSuccess && hipEventCreate(&p->ev1) == hipSuccess && hipEventCreate(&p->ev2) == hipSuccess &&
+		hipEventCreateWithFlags(&p->evf, hipEventDisableTiming) == hipSuccess &&
 		hipStreamCreateWithFlags(&p->ws, hipStreamNonBlocking) == hipSuccess &&
 		hipEventCreateWithFlags(&p->wev, hipEventDisableTiming) == hipSuccess;
 	for (int g = 0; ok && g < kGroups; ++g)
@@ -595,7 +609,7 @@ __global__ void clear_out_kernel(int32_t *out, int32_t n)
 static int launch_wide(nsh_ext_plan_t *p, const std::vector<nsh_glob::GlobProb> &list, const nsh_glob::GlobProb *d_list)
 {
 	if (list.empty()) return 1;
-	HIPOK(hipStreamWaitEvent(p->ws, p->ev1, 0));
+	HIPOK(hipStreamWaitEvent(p->ws, p->evf, 0));
 	for (size_t k = 0; k < list.size();) {
 		size_t e = k;
 		while (e < list.size() && list[e].NW == list[k].NW && list[e].C == list[k].C) ++e;
@@ -614,30 +628,33 @@ fail:
 }
 
 // idx/n_idx: prepare only these inputs (subset runs); nullptr prepares all p->n
+// capture: the launches are being recorded into a graph; the timing events are left out (graph mode
+// records them around the graph launch)
 static int plan_launch(nsh_ext_plan_t *p, const ExtArgs *args, const int32_t *grid, bool clear_out, const int32_t *idx,
-		int32_t n_idx, const std::vector<nsh_glob::GlobProb> *wide, const nsh_glob::GlobProb *d_wide)
+		int32_t n_idx, const std::vector<nsh_glob::GlobProb> *wide, const nsh_glob::GlobProb *d_wide, bool capture = false)
 {
-	HIPOK(hipEventRecord(p->ev0, p->stream));
+	if (!capture) HIPOK(hipEventRecord(p->ev0, p->stream));
 	if (n_idx > 0)
 		hipLaunchKernelGGL(ext_prep_kernel, dim3(n_idx), dim3(256), 0, p->stream, p->d_prep, idx, p->d_ns, p->d_ss, p->d_as,
 				p->d_rec, p->d_aas, p->d_pc);
 	hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, p->stream, p->d_counter, kGroups, 0);
 	if (clear_out) hipLaunchKernelGGL(clear_out_kernel, dim3((p->n + 255) / 256), dim3(256), 0, p->stream, p->d_out, p->n);
 	HIPOK(hipGetLastError());
-	HIPOK(hipEventRecord(p->ev1, p->stream));
+	if (!capture) HIPOK(hipEventRecord(p->ev1, p->stream));
+	HIPOK(hipEventRecord(p->evf, p->stream));
 	// the widest problems first: each group's persistent waves take the CUs in launch order, and a
 	// group started after group 0 (the most waves) would wait for its queue to drain
 	if (wide && !launch_wide(p, *wide, d_wide)) goto fail;
 	for (int g = kGroups - 1; g >= 0; --g) {
 		if (grid[g] <= 0) continue;
-		HIPOK(hipStreamWaitEvent(p->gs[g], p->ev1, 0));
+		HIPOK(hipStreamWaitEvent(p->gs[g], p->evf, 0));
 		launch_group(g, args[g], grid[g], p->gs[g]);
 		HIPOK(hipGetLastError());
 		HIPOK(hipEventRecord(p->gev[g], p->gs[g]));
 		HIPOK(hipStreamWaitEvent(p->stream, p->gev[g], 0));
 		nsh_count_launch(1);
 	}
-	HIPOK(hipEventRecord(p->ev2, p->stream));
+	if (!capture) HIPOK(hipEventRecord(p->ev2, p->stream));
 	nsh_count_launch(2);
 	p->ran = 1;
 	return 1;
@@ -648,6 +665,40 @@ fail:
 extern "C" int nsh_ext_plan_run(nsh_ext_plan_t *p)
 {
 	return p? plan_launch(p, p->args, p->grid, false, nullptr, p->n, &p->wide_h, p->d_wide) : 0;
+}
+
+// the same run as one hipGraph: captured from the plan's streams on the first call (and again when
+// the autotuner has changed a persistent grid), replayed afterwards. The queue counters are cleared
+// by a kernel inside the graph, so every replay starts from zero. The timed span of a graph run
+// (nsh_ext_plan_last_ms) covers the whole graph, preparation included.
+extern "C" int nsh_ext_plan_run_graph(nsh_ext_plan_t *p)
+{
+	if (!p) { nsh_set_error("nsh_ext_plan_run_graph: no plan"); return 0; }
+	if (p->gexec && std::memcmp(p->ggrid, p->grid, sizeof(p->grid))) p->drop_graph();
+	if (!p->gexec) {
+		HIPOK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+		const int ok = plan_launch(p, p->args, p->grid, false, nullptr, p->n, &p->wide_h, p->d_wide, true);
+		hipGraph_t g = nullptr;
+		const hipError_t e = hipStreamEndCapture(p->stream, &g);
+		if (!ok || e != hipSuccess || !g) {
+			if (g) (void)hipGraphDestroy(g);
+			if (ok) nsh_set_error("nsh_ext_plan_run_graph: capture failed: %s", hipGetErrorString(e));
+			return 0;
+		}
+		p->graph = g;
+		HIPOK(hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0));
+		std::memcpy(p->ggrid, p->grid, sizeof(p->grid));
+	}
+	HIPOK(hipEventRecord(p->ev0, p->stream));
+	HIPOK(hipEventRecord(p->ev1, p->stream));
+	HIPOK(hipGraphLaunch(p->gexec, p->stream));
+	HIPOK(hipEventRecord(p->ev2, p->stream));
+	nsh_count_launch(1);
+	p->ran = 1;
+	return 1;
+fail:
+	p->drop_graph();
+	return 0;
 }
 
 extern "C" int nsh_ext_plan_run_subset(nsh_ext_plan_t *p, const uint8_t *keep)

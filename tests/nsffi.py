@@ -464,6 +464,7 @@ class Product(_Lib):
         for f in (L.nsh_ext_plan_run, L.nsh_ext_plan_sync, L.nsh_ext_plan_destroy):
             f.argtypes = [C.c_void_p]
         L.nsh_ext_plan_run_subset.argtypes = [C.c_void_p, C.c_char_p]
+        L.nsh_ext_plan_run_graph.argtypes = [C.c_void_p]
         L.nsh_ext_plan_results.argtypes = [C.c_void_p, C.c_void_p]
         L.nsh_ext_plan_cells.argtypes = [C.c_void_p, C.c_int32]
         L.nsh_ext_plan_cells.restype = C.c_double
@@ -535,10 +536,10 @@ class ExtPlan:
     def __init__(self, prod, p, n):
         self.prod, self.p, self.n = prod, p, n
 
-    def run(self, sync=True):
+    def run(self, sync=True, graph=False):
         L = self.prod.lib
-        if not L.nsh_ext_plan_run(self.p):
-            raise RuntimeError("nsh_ext_plan_run: " + self.prod.error())
+        if not (L.nsh_ext_plan_run_graph(self.p) if graph else L.nsh_ext_plan_run(self.p)):
+            raise RuntimeError("nsh_ext_plan_run%s: %s" % ("_graph" if graph else "", self.prod.error()))
         if sync and not L.nsh_ext_plan_sync(self.p):
             raise RuntimeError("nsh_ext_plan_sync: " + self.prod.error())
 

@@ -230,3 +230,46 @@ def test_ext_plan_two_problems_per_wave(oracle, product, xdrop, global_mode):
     assert not bad, [(k, len(pairs[k][0]), len(pairs[k][1]), got[k], exp[k]) for k in bad[:5]]
     bad = [k for k in range(0, len(pairs), 2) if sub[k] != exp[k]]
     assert not bad, [(k, sub[k], exp[k]) for k in bad[:5]]
+
+
+def test_ext_plan_graph_replays_match_eager_runs(oracle, product):
+    # the plan's run captured as one hipGraph and replayed: every replay equals the eager run's
+    # results, with other eager work and synchronisations between the replays; a second plan's
+    # graph is independent of the first; after the autotuner changes the grids the graph is
+    # captured again
+    from protlocus_amd import synth
+    pairs = synth.extension_batch(192, seed=8, min_len=20, max_len=1200)
+    rng = np.random.default_rng(8)
+    dirs = rng.integers(0, 2, size=len(pairs)).astype(np.uint8)
+    ns, ns_off, nl, as_, as_off, al = synth.pack_batch(pairs)
+    other = synth.extension_batch(64, seed=9, min_len=50, max_len=400)
+    ons, ons_off, onl, oas, oas_off, oal = synth.pack_batch(other)
+    plan = product.ext_plan(ns, ns_off, nl, as_, as_off, al, product.opt(flag=NS_F_EXT_RIGHT), dirs=dirs)
+    plan2 = product.ext_plan(ons, ons_off, onl, oas, oas_off, oal, product.opt(flag=NS_F_EXT_LEFT))
+    try:
+        plan.run()
+        eager = plan.results()
+        plan2.run()
+        eager2 = plan2.results()
+        for rep in range(4):
+            plan.run(graph=True)
+            assert plan.results() == eager, rep
+            assert plan.ms(1) > 0
+            plan2.run()                     # eager work of another plan between the replays
+            assert plan2.results() == eager2, rep
+            plan2.run(graph=True)
+            assert plan2.results() == eager2, rep
+        plan.autotune(reps=1)
+        for rep in range(2):
+            plan.run(graph=True)
+            assert plan.results() == eager, ("after autotune", rep)
+    finally:
+        plan.close()
+        plan2.close()
+
+    def one(k):
+        d, p = pairs[k]
+        return _oracle_ext(oracle, product, d, p, dict(flag=NS_F_EXT_LEFT if dirs[k] else NS_F_EXT_RIGHT), None)
+    with cf.ThreadPoolExecutor(8) as ex:
+        exp = list(ex.map(one, range(len(pairs))))
+    assert eager == exp
