@@ -909,7 +909,7 @@ def main():
         t0 = time.perf_counter()
         waves = plan.autotune(reps=3)
         tune_s = time.perf_counter() - t0
-        log("[bench] autotuned waves per CU %s in %.1f s" % (waves, tune_s))
+        log("[bench] autotuned waves per CU %s, pairs up to %d rows, in %.1f s" % (waves, plan.pair_rows(), tune_s))
     for _ in range(args.warmup):
         plan.run()
     log("[bench] warmup done")
@@ -936,6 +936,7 @@ def main():
     prod.lib.nsh_device_synchronize()
     g_ms = (time.perf_counter() - t0) / args.steps * 1e3
     g_same = plan.results() == gpu_res
+    pair_rows = plan.pair_rows()
     plan.close()
     ms = wall / args.steps * 1e3
     gcups = cells / (ms * 1e-3) / 1e9
@@ -953,6 +954,7 @@ def main():
                                "right extension over the full target" % args.pairs,
                    "pairs": args.pairs, "seed": args.seed, "cell_definition": "(rows evaluated) x al",
                    "waves_per_cu": waves if waves else "compiled defaults",
+                   "pair_rows": pair_rows,
                    "autotune_s": round(tune_s, 2)},
         "proteins_per_s": round(args.pairs / (ms * 1e-3), 1),
         "window_gcups": round(win / (ms * 1e-3) / 1e9, 2),

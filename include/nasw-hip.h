@@ -248,11 +248,15 @@ int nsh_ext_plan_results(nsh_ext_plan_t *p, nsh_ext_result_t *out); /* copy resu
 double nsh_ext_plan_cells(const nsh_ext_plan_t *p, int32_t kind);   /* 0: sum nl*al, 1: sum (rows done)*al after a run */
 float nsh_ext_plan_last_ms(nsh_ext_plan_t *p, int32_t which);  /* 0: whole run, 1: DP kernels only (HIP events) */
 /* Autotuner: picks each column-count group's persistent wave count (waves per CU, one of the
- * compiled default's 1/2, 3/4, 1, 3/2 and 2 multiples) by timing full runs of this plan, one
- * group at a time (reps runs per candidate, the fastest of them; another setting replaces the
- * default only when it is 2% faster). Results do not depend on it.
+ * compiled default's 1/2, 3/4, 1, 3/2 and 2 multiples), then the wavefronts per problem: whether
+ * problems of at most 256 padded columns share a wave two at a time for every length, only up to
+ * 16,384 or 65,536 rows, or never. Each choice is made by timing full runs of this plan, one at a
+ * time (reps runs per candidate, the fastest of them; another setting replaces the default only
+ * when it is 2% faster). Results do not depend on it.
  * waves_per_cu (may be NULL) receives the chosen values, one per group (3). 1 ok, 0 error. */
 int nsh_ext_plan_autotune(nsh_ext_plan_t *p, int32_t reps, int32_t *waves_per_cu);
+/* the row limit up to which small problems share a wave (INT32_MAX: all of them, 0: none) */
+int32_t nsh_ext_plan_pair_rows(const nsh_ext_plan_t *p);
 void nsh_ext_plan_destroy(nsh_ext_plan_t *p);
 
 /* The align stage of the map path for many candidate regions at once (no single reference

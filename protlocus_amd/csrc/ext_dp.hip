@@ -258,6 +258,9 @@ struct nsh_ext_plan_s {
 	int ncu = 256;
 	int32_t max_al = 0, max_nl = 0;
 	std::vector<ExtProb> prob_h;             // descriptors in queue order (host copy)
+	std::vector<ExtProb> prob_in;            // the narrow problems' descriptors in input order
+	bool pssm_plan = false;                  // some problem scores with a PSSM (no pairs then)
+	int32_t pair_rows = INT32_MAX;           // problems of more rows run one per wave (the autotuner's choice)
 	int32_t grp_first[kGroups] = {}, grp_n[kGroups] = {};
 	// problems wider than 128 * kCmax padded columns: the work-group-per-problem kernel in extension
 	// mode (global_dp_impl.h), in buckets of equal shape, on their own stream
@@ -291,6 +294,7 @@ struct nsh_ext_plan_s {
 		drop_graph();
 		n = 0, ran = 0, cells_window = 0, max_al = max_nl = 0;
 		al.clear(), out_h.clear(), prob_h.clear(), wide_h.clear(), wide_sub_h.clear(), sub_h.clear(), idx_h.clear();
+		prob_in.clear(), pssm_plan = false, pair_rows = INT32_MAX;
 		d_ns = d_ss = d_as = d_aas = nullptr, d_rec = nullptr, d_mat = d_pssm = nullptr;
 		d_pen = d_counter = d_out = nullptr, d_prob = nullptr, d_prep = nullptr, d_pc = nullptr;
 		d_wide = d_wide_sub = nullptr, d_prob_sub = nullptr, d_idx_sub = nullptr, d_feat = nullptr;
@@ -358,6 +362,83 @@ void shell_release(nsh_ext_plan_t *p)
 	delete p;
 }
 } // namespace
+
+// The persistent queue of a plan's narrow problems (p->prob_in) into p->prob_h: work items in
+// queue order, then the pair partners. Two problems per wave (matrix scores only): problems of at
+// most 256 padded columns, at least 3 rows and at most pair_rows rows, paired with a problem of the
+// same column count per lane (ceil(npad / 64)) and the nearest length; the first of a pair is the
+// queue item. Items are ordered small-C group first (its launch), then by the most wave time first
+// in each group (rows x per-row instructions, about 41 + 30 C for one problem and 49 + 30 C for a
+// pair). n_grp receives the items per column-count group.
+static void plan_order(nsh_ext_plan_t *p, int32_t pair_rows, int32_t n_grp[kGroups])
+{
+	const std::vector<ExtProb> &prob = p->prob_in;
+	const int32_t n = (int32_t)prob.size();
+	std::vector<int32_t> partner(n, -1), order;
+	std::vector<char> second(n, 0);
+	for (int g = 0; g < kGroups; ++g) n_grp[g] = 0;
+	if (!p->pssm_plan) {
+		std::vector<int32_t> cand;
+		for (int32_t k = 0; k < n; ++k)
+			if (prob[k].npad <= kPairMaxCols && prob[k].nl >= 3 && prob[k].nl <= pair_rows) cand.push_back(k);
+		auto cp = [&](int32_t k) { return (prob[k].npad + 63) / 64; };
+		std::stable_sort(cand.begin(), cand.end(), [&](int32_t x, int32_t y) {
+			return cp(x) != cp(y)? cp(x) < cp(y) : prob[x].nl > prob[y].nl; });
+		for (size_t t = 0; t + 1 < cand.size();) {
+			if (cp(cand[t]) != cp(cand[t + 1])) { ++t; continue; }
+			partner[cand[t]] = cand[t + 1], second[cand[t + 1]] = 1;
+			t += 2;
+		}
+	}
+	auto cost = [&](int32_t k) -> int64_t {
+		if (partner[k] < 0) return (int64_t)prob[k].nl * (41 + 30 * prob[k].C);
+		return (int64_t)std::max(prob[k].nl, prob[partner[k]].nl) * (49 + 30 * ((prob[k].npad + 63) / 64));
+	};
+	for (int32_t k = 0; k < n; ++k) if (!second[k]) order.push_back(k), ++n_grp[group_of(prob[k].C)];
+	std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+		const int bx = group_of(prob[x].C), by = group_of(prob[y].C);
+		if (bx != by) return bx < by;
+		return cost(x) > cost(y); });
+	const int32_t n_items = (int32_t)order.size();
+	for (int32_t k = 0; k < n_items; ++k) if (partner[order[k]] >= 0) order.push_back(partner[order[k]]);
+	p->prob_h.resize(order.size());
+	for (size_t k = 0; k < order.size(); ++k) p->prob_h[k] = prob[order[k]];
+	for (int32_t k = 0, t = n_items; k < n_items; ++k) {
+		if (partner[order[k]] < 0) continue;
+		ExtProb &e = p->prob_h[k];
+		e.pair = t++, e.C = (e.npad + 63) / 64;   // the partner's index; C per lane per problem
+	}
+}
+
+// the groups' queue views of p->prob_h after plan_order (descriptors already on the device)
+static void plan_set_groups(nsh_ext_plan_t *p, const int32_t n_grp[kGroups])
+{
+	int64_t first = 0;
+	for (int g = 0; g < kGroups; ++g) {
+		ExtArgs &a = p->args[g];
+		p->grp_first[g] = (int32_t)first, p->grp_n[g] = n_grp[g];
+		a.prob = p->d_prob + first, a.n = n_grp[g], a.pbase = p->d_prob;
+		first += n_grp[g];
+		p->grid[g] = (int32_t)std::min<int64_t>(a.n, (int64_t)p->ncu * p->wpc[g]);
+	}
+}
+
+// the queue rebuilt with another pairing limit and uploaded (the plan's stream is idle after this)
+static int plan_requeue(nsh_ext_plan_t *p, int32_t pair_rows)
+{
+	int32_t n_grp[kGroups];
+	HIPOK(hipStreamSynchronize(p->stream));
+	p->drop_graph();   // a captured graph holds the old queue sizes
+	p->pair_rows = pair_rows;
+	plan_order(p, pair_rows, n_grp);
+	if (!p->prob_h.empty())
+		HIPOK(hipMemcpyAsync(p->d_prob, p->prob_h.data(), p->prob_h.size() * sizeof(ExtProb), hipMemcpyHostToDevice, p->stream));
+	HIPOK(hipStreamSynchronize(p->stream));
+	plan_set_groups(p, n_grp);
+	return 1;
+fail:
+	return 0;
+}
 
 int nsh_ext_req_fits(const nsh_ext_req &q, int32_t k)
 {
@@ -436,46 +517,10 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 	}
 	std::stable_sort(p->wide_h.begin(), p->wide_h.end(), [](const nsh_glob::GlobProb &x, const nsh_glob::GlobProb &y) {
 		return x.NW != y.NW? x.NW > y.NW : x.C > y.C; });
-	// two problems per wave (matrix scores only): problems of at most 256 padded columns and at least
-	// 3 rows, paired with a problem of the same column count per lane (ceil(npad / 64)) and the
-	// nearest length; the first of a pair is the queue item, its partner is stored after all items
-	std::vector<int32_t> partner(n, -1);
-	std::vector<char> second(n, 0);
-	if (!l_pssm) {
-		std::vector<int32_t> cand;
-		for (int32_t k = 0; k < n; ++k)
-			if (!is_wide[k] && prob[k].npad <= kPairMaxCols && prob[k].nl >= 3) cand.push_back(k);
-		auto cp = [&](int32_t k) { return (prob[k].npad + 63) / 64; };
-		std::stable_sort(cand.begin(), cand.end(), [&](int32_t x, int32_t y) {
-			return cp(x) != cp(y)? cp(x) < cp(y) : prob[x].nl > prob[y].nl; });
-		for (size_t t = 0; t + 1 < cand.size();) {
-			if (cp(cand[t]) != cp(cand[t + 1])) { ++t; continue; }
-			partner[cand[t]] = cand[t + 1], second[cand[t + 1]] = 1;
-			--n_grp[group_of(prob[cand[t + 1]].C)];
-			t += 2;
-		}
-	}
-	// queue order: small-C problems first (their launch), then large; the most wave time first in
-	// each (rows x per-row instructions, about 41 + 30 C for one problem and 49 + 30 C for a pair)
-	auto cost = [&](int32_t k) -> int64_t {
-		if (partner[k] < 0) return (int64_t)prob[k].nl * (41 + 30 * prob[k].C);
-		return (int64_t)std::max(prob[k].nl, prob[partner[k]].nl) * (49 + 30 * ((prob[k].npad + 63) / 64));
-	};
-	for (int32_t k = 0; k < n; ++k) if (!is_wide[k] && !second[k]) order.push_back(k);
-	std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-		const int bx = group_of(prob[x].C), by = group_of(prob[y].C);
-		if (bx != by) return bx < by;
-		return cost(x) > cost(y); });
-	const int32_t n_items = (int32_t)order.size();
-	for (int32_t k = 0; k < n_items; ++k) if (partner[order[k]] >= 0) order.push_back(partner[order[k]]);
-	const int32_t n_narrow = (int32_t)order.size();
-	p->prob_h.resize(n_narrow);
-	for (int32_t k = 0; k < n_narrow; ++k) p->prob_h[k] = prob[order[k]];
-	for (int32_t k = 0, t = n_items; k < n_items; ++k) {
-		if (partner[order[k]] < 0) continue;
-		ExtProb &e = p->prob_h[k];
-		e.pair = t++, e.C = (e.npad + 63) / 64;   // the partner's index; C per lane per problem
-	}
+	for (int32_t k = 0; k < n; ++k) if (!is_wide[k]) p->prob_in.push_back(prob[k]);
+	p->pssm_plan = l_pssm > 0;
+	plan_order(p, p->pair_rows, n_grp);
+	const int32_t n_narrow = (int32_t)p->prob_h.size();
 	p->out_h.assign((size_t)4 * n, 0);
 	// inputs staged in pinned memory: ns | as | ss | pssm | pen | prob | prep | pc | mat | wide
 	const int32_t n_pen = max_nl + 2;
@@ -843,8 +888,27 @@ extern "C" int nsh_ext_plan_autotune(nsh_ext_plan_t *p, int32_t reps, int32_t *w
 		wpc[g] = best_w;
 		p->grid[g] = (int32_t)std::min<int64_t>(n, (int64_t)p->ncu * best_w);
 	}
+	// wavefronts per problem: two small problems per wave for every length, only up to 16,384 or
+	// 65,536 rows, or never (a long problem alone on a wave runs its rows with less latency)
+	if (!p->pssm_plan && p->grp_n[0] > 0) {
+		const int32_t cand[4] = { INT32_MAX, 65536, 16384, 0 };
+		int32_t best_r = p->pair_rows;
+		float best_ms = 1e30f;
+		for (int c = 0; c < 4; ++c) {
+			if (!plan_requeue(p, cand[c])) return 0;
+			float ms;
+			if (!time_run(&ms)) return 0;
+			if (c == 0? true : ms < best_ms * 0.98f) best_ms = ms, best_r = cand[c];
+		}
+		if (!plan_requeue(p, best_r)) return 0;
+	}
 	if (waves_per_cu) for (int g = 0; g < kGroups; ++g) waves_per_cu[g] = wpc[g];
 	return 1;
+}
+
+extern "C" int32_t nsh_ext_plan_pair_rows(const nsh_ext_plan_t *p)
+{
+	return p? p->pair_rows : -1;
 }
 
 // ======================================================================================

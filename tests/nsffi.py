@@ -465,6 +465,7 @@ class Product(_Lib):
             f.argtypes = [C.c_void_p]
         L.nsh_ext_plan_run_subset.argtypes = [C.c_void_p, C.c_char_p]
         L.nsh_ext_plan_run_graph.argtypes = [C.c_void_p]
+        L.nsh_ext_plan_pair_rows.argtypes = [C.c_void_p]
         L.nsh_ext_plan_results.argtypes = [C.c_void_p, C.c_void_p]
         L.nsh_ext_plan_cells.argtypes = [C.c_void_p, C.c_int32]
         L.nsh_ext_plan_cells.restype = C.c_double
@@ -555,8 +556,13 @@ class ExtPlan:
     def ms(self, which):
         return self.prod.lib.nsh_ext_plan_last_ms(self.p, which)
 
+    def pair_rows(self):
+        """the row limit up to which small problems share a wave (2**31 - 1: all, 0: none)"""
+        return self.prod.lib.nsh_ext_plan_pair_rows(self.p)
+
     def autotune(self, reps=2):
-        """the persistent wave count per column-count group, picked by timing this plan's runs"""
+        """the persistent wave count per column-count group and the wavefronts per problem, picked
+        by timing this plan's runs"""
         w = (C.c_int32 * 3)()
         if not self.prod.lib.nsh_ext_plan_autotune(self.p, reps, w):
             raise RuntimeError("nsh_ext_plan_autotune: " + self.prod.error())
