@@ -73,11 +73,16 @@ __global__ void fill_i32_kernel(int32_t *p, int32_t n, int32_t v)
 // ------------------------------------------------------------------------------------
 DEV int wrap8(int v) { return (int)(int8_t)(uint8_t)(v & 0xff); }
 
+// Every base a row's record needs lies within 8 positions of the row (codon i-2..i, donor up to
+// i+8 on the left strand, acceptor down to i-6), so the work-group stages the nt4 codes of a chunk
+// of rows plus that margin in LDS and reads them from there.
+constexpr int kPrepChunk = 4096, kPrepHalo = 8;
+
 struct PrepView {
 	const uint8_t *ns, *ss; int nl, left, has_ss, max_spsc;
-	const uint8_t *nt4;
+	const uint8_t *stage; int s0;          // nt4 codes of DP positions s0.. in LDS
 	const int32_t *sp; int sp_null;
-	__device__ int b(int x) const { return nt4[ns[left? nl - 1 - x : x]]; } // nt4 code in DP order
+	__device__ int b(int x) const { return stage[x - s0]; } // nt4 code in DP order
 	__device__ int spsc(int s) const { int v = (int)(int8_t)(s >> 1) - 64; return v > max_spsc? max_spsc : v; }
 	__device__ int donor(int p) const {
 		int d;
@@ -156,27 +161,39 @@ __global__ void __launch_bounds__(256) ext_prep_kernel(const PrepProb *pp, const
 		const uint8_t *raw_ss, const uint8_t *raw_as, uint32_t *rec, uint8_t *aas, const PrepConst *pcg)
 {
 	__shared__ PrepConst pc;
+	__shared__ uint8_t stage[kPrepChunk + 2 * kPrepHalo];
 	const PrepProb P = pp[idx? idx[blockIdx.x] : (int32_t)blockIdx.x];
 	for (int k = threadIdx.x; k < (int)sizeof(PrepConst); k += blockDim.x) ((uint8_t*)&pc)[k] = ((const uint8_t*)pcg)[k];
 	__syncthreads();
 	PrepView v;
 	v.ns = (P.from_genome? pc.genome : raw_ns) + P.ns_off, v.ss = P.has_ss? raw_ss + P.ns_off : nullptr;
 	v.nl = P.nl, v.left = P.left, v.has_ss = P.has_ss, v.max_spsc = (P.io + 1) / 2 - 1;
-	v.nt4 = pc.nt4, v.sp = pc.sp, v.sp_null = pc.sp_null;
+	v.stage = stage, v.sp = pc.sp, v.sp_null = pc.sp_null;
 	const int X = pc.aa20['X'];
 	const uint8_t *as = raw_as + P.as_off;
 	for (int j = threadIdx.x; j < P.al; j += blockDim.x)
 		aas[P.aa_off + j] = pc.aa20[as[P.left? P.al - 1 - j : j]];
-	for (int i = threadIdx.x; i < P.nl; i += blockDim.x) {
-		// residue of the codon ending at DP position i (left: reversed, shifted by two)
-		int res = X;
-		const int k = P.left? P.nl + 1 - i : i;
-		if (i >= 2 && k >= 2) {
-			const int c0 = pc.nt4[v.ns[k - 2]], c1 = pc.nt4[v.ns[k - 1]], c2 = pc.nt4[v.ns[k]];
-			if (c0 < 4 && c1 < 4 && c2 < 4) res = pc.codon[c0 << 4 | c1 << 2 | c2];
+	for (int i0 = 0; i0 < P.nl; i0 += kPrepChunk) {
+		// the chunk's bases and the margin on both sides, as nt4 codes in DP order
+		v.s0 = i0 - kPrepHalo;
+		__syncthreads();
+		for (int t = threadIdx.x; t < kPrepChunk + 2 * kPrepHalo; t += blockDim.x) {
+			const int x = v.s0 + t;
+			stage[t] = x >= 0 && x < P.nl? pc.nt4[v.ns[P.left? P.nl - 1 - x : x]] : 4;
 		}
-		const int d = v.donor(i + 1), ac = v.acceptor(i);
-		rec[P.rec_off + i] = (uint32_t)res | ((uint32_t)(d & 0xff) << 8) | ((uint32_t)(ac & 0xff) << 16);
+		__syncthreads();
+		const int i1 = min(P.nl, i0 + kPrepChunk);
+		for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+			// residue of the codon ending at DP position i: bases i-2, i-1, i in sequence order
+			// (left: the reversed sequence read backwards, so DP positions i, i-1, i-2)
+			int res = X;
+			if (i >= 2 && (!P.left || i <= P.nl - 1)) {
+				const int c0 = v.b(P.left? i : i - 2), c1 = v.b(i - 1), c2 = v.b(P.left? i - 2 : i);
+				if (c0 < 4 && c1 < 4 && c2 < 4) res = pc.codon[c0 << 4 | c1 << 2 | c2];
+			}
+			const int d = v.donor(i + 1), ac = v.acceptor(i);
+			rec[P.rec_off + i] = (uint32_t)res | ((uint32_t)(d & 0xff) << 8) | ((uint32_t)(ac & 0xff) << 16);
+		}
 	}
 }
 
