@@ -73,8 +73,26 @@ __global__ void __launch_bounds__(64) backtrack_kernel(const BtProb *bp, int32_t
 	BtRun b = { tmp + P.tmp_off, 0, 0xffffffffu };
 	int32_t i = P.nl - 1, j = P.al - 1, last = 0;
 	const int64_t al = P.al;
+	// The walk's words come from a window of kBtWin words fetched together along the last move's
+	// direction (di rows, dj columns per step): an intron is a straight run up one column and an
+	// exon a 3:1 diagonal, so one memory latency serves up to kBtWin steps instead of one.
+	constexpr int kBtWin = 16;
+	// win[q] holds the word at (wi - wdi * q, wj - wdj * q)
+	int32_t win[kBtWin], wi = -1, wj = -1, wdi = 1, wdj = 0, di = 1, dj = 0;
 	while (i >= 2 && j >= 0) {   // nasw-sse.c:44-93
-		int32_t x = tr[(int64_t)i * al + j], state, ext;
+		int32_t t = wdj? wj - j : (wi - i) / wdi;
+		if (wj < 0 || t < 0 || t >= kBtWin || i != wi - wdi * t || j != wj - wdj * t) {
+			wi = i, wj = j, wdi = di, wdj = dj, t = 0;
+#pragma unroll
+			for (int q = 0; q < kBtWin; ++q) {
+				const int32_t ii = i - di * q, jj = j - dj * q;
+				win[q] = ii >= 0 && jj >= 0? tr[(int64_t)ii * al + jj] : 0;
+			}
+		}
+		int32_t x = win[0], state, ext;
+#pragma unroll
+		for (int q = 1; q < kBtWin; ++q) if (t == q) x = win[q];
+		const int32_t i0 = i, j0 = j;
 		if (x >> 9 & 1) x = 1 | x >> 4 << 4;
 		state = last == 0? x & 0xf : last;
 		ext = state >= 1 && state <= 5? x >> (state + 3) & 1 : 0;
@@ -92,6 +110,7 @@ __global__ void __launch_bounds__(64) backtrack_kernel(const BtProb *bp, int32_t
 		default: i = -1; break;   // not produced by the kernels
 		}
 		last = state >= 1 && state <= 5 && ext? state : 0;
+		if (i0 - i > 0 || j0 - j > 0) di = i0 - i, dj = j0 - j;   // every step moves up, left or both
 	}
 	if (j > 0) b.push(NS_CIGAR_I, j);
 	if (i >= 0) {
@@ -230,58 +249,74 @@ int run_list(RunCtx &c, const GlobIn *P, int32_t n, const GlobTerms &T, int32_t 
 		int32_t *cig_n, const uint32_t **pk_out)
 {
 	const auto t0 = std::chrono::steady_clock::now();
+	nsh_step_clock clk("global DP");
 	// layout of the input block: ns | as | ss | pssm | PrepProb | GlobProb | BtProb | PrepConst | mat
+	// Pass 1, one thread: each problem's offsets into the packed arrays, its bucket and its
+	// backtrack slot. Buckets are (kernel variant, waves desc, columns per lane desc); a counting
+	// sort gives every problem its place in bucket order, problems in input order inside a bucket.
+	struct Off { int64_t ns, as, rec, pssm, tr, tmp; };
+	std::vector<Off> off(n);
+	std::vector<uint16_t> key(n);
+	std::vector<int32_t> pos(n), bt_slot(n, -1), bt_req;   // bt_req: problem index of each backtrack slot
 	int64_t l_raw = 0, l_rec = 0, l_as = 0, l_pssm = 0, l_tr = 0, l_tmp = 0;
 	int any_ss = 0;
-	std::vector<nsh_prep::PrepProb> pp(n);
-	std::vector<GlobProb> gp(n);
-	std::vector<int32_t> order(n);
+	constexpr int kKeys = 3 * (kNWMax + 1) * (kCMaxG + 1);
+	int32_t kstart[kKeys + 1] = {};
 	for (int32_t k = 0; k < n; ++k) {
 		const GlobIn &q = P[k];
-		nsh_prep::PrepProb &p = pp[k];
-		p = {};
-		if (q.g_off >= 0) p.from_genome = 1, p.ns_off = q.g_off;
-		else p.ns_off = l_raw, l_raw += q.nl + 1;
-		p.as_off = l_as, p.rec_off = l_rec, p.aa_off = l_as;
-		// the strand only matters to int32 score-only requests with an extension flag (nasw-sse.c:246-247)
-		p.nl = q.nl, p.al = q.al, p.left = q.left, p.io = q.io, p.has_ss = q.ss != nullptr;
-		GlobProb &g = gp[k];
-		g = {};
-		g.rec_off = l_rec, g.aa_off = l_as, g.pssm_off = q.pssm? l_pssm : -1;
-		g.tr_off = q.trace? l_tr : 0;
-		g.nl = q.nl, g.al = q.al, g.io = q.io, g.out = k, g.left = q.left;
-		shape(q.al, g.C, g.NW);
-		const int vs = q.is32? 4 : 8;
-		g.slen = (q.al + vs - 1) / vs, g.npad = q.al, g.mode = 0;
+		Off &o = off[k];
+		o.ns = q.g_off >= 0? q.g_off : l_raw;
+		if (q.g_off < 0) l_raw += q.nl + 1;
+		o.as = l_as, o.rec = l_rec, o.pssm = q.pssm? l_pssm : -1, o.tr = q.trace? l_tr : 0, o.tmp = l_tmp;
 		l_rec += q.nl + 1, l_as += q.al + 1;
 		if (q.pssm) l_pssm += (int64_t)q.al * 22;
-		if (q.trace) l_tr += (int64_t)q.nl * q.al, l_tmp += (int64_t)q.nl + q.al + 8;
-		any_ss |= q.ss != nullptr;
-		order[k] = k;
-	}
-	// buckets of equal kernel variant and shape
-	std::sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-		const int vx = kernel_variant(P[x]), vy = kernel_variant(P[y]);
-		if (vx != vy) return vx < vy;
-		if (gp[x].NW != gp[y].NW) return gp[x].NW > gp[y].NW;
-		if (gp[x].C != gp[y].C) return gp[x].C > gp[y].C;
-		return x < y; });
-	std::vector<GlobProb> gsorted(n);
-	for (int32_t k = 0; k < n; ++k) gsorted[k] = gp[order[k]];
-	std::vector<BtProb> bt;
-	std::vector<int32_t> bt_req;   // problem index of each backtrack slot
-	{
-		int64_t tmp = 0;
-		for (int32_t k = 0; k < n; ++k) {
-			if (!P[k].trace) continue;
-			bt.push_back({ gp[k].tr_off, tmp, P[k].nl, P[k].al });
-			bt_req.push_back(k);
-			tmp += (int64_t)P[k].nl + P[k].al + 8;
+		if (q.trace) {
+			bt_slot[k] = (int32_t)bt_req.size(), bt_req.push_back(k);
+			l_tr += (int64_t)q.nl * q.al, l_tmp += (int64_t)q.nl + q.al + 8;
 		}
+		any_ss |= q.ss != nullptr;
+		int C, NW;
+		shape(q.al, C, NW);
+		if (C > kCMaxG || NW > kNWMax) { nsh_set_error("global DP: protein length %d exceeds %d", q.al, 64 * kCMaxG * kNWMax); return 0; }
+		key[k] = (uint16_t)((kernel_variant(q) * (kNWMax + 1) + (kNWMax - NW)) * (kCMaxG + 1) + (kCMaxG - C));
+		++kstart[key[k] + 1];
 	}
+	for (int t = 0; t < kKeys; ++t) kstart[t + 1] += kstart[t];
+	{
+		int32_t next[kKeys];
+		std::memcpy(next, kstart, sizeof(next));
+		for (int32_t k = 0; k < n; ++k) pos[k] = next[key[k]]++;
+	}
+	// a bucket's kernel variant, waves and columns per lane from its key
+	auto key_v = [&](int t) { return t / ((kNWMax + 1) * (kCMaxG + 1)); };
+	auto key_nw = [&](int t) { return kNWMax - t / (kCMaxG + 1) % (kNWMax + 1); };
+	auto key_c = [&](int t) { return kCMaxG - t % (kCMaxG + 1); };
+	if (clk.on) {
+		// the batch's shape: per bucket (waves, columns per lane) its problems, rows and cells
+		int64_t rows = 0, cells = 0, nl_max = 0, hn[6] = {}, hr[6] = {}, br[kKeys] = {}, bc[kKeys] = {}, bm[kKeys] = {};
+		for (int32_t k = 0; k < n; ++k) {
+			const int64_t nl = P[k].nl;
+			rows += nl, cells += nl * P[k].al, nl_max = std::max(nl_max, nl);
+			const int b = nl <= 64? 0 : nl <= 256? 1 : nl <= 1024? 2 : nl <= 4096? 3 : nl <= 16384? 4 : 5;
+			++hn[b], hr[b] += nl;
+			br[key[k]] += nl, bc[key[k]] += nl * P[k].al, bm[key[k]] = std::max(bm[key[k]], nl);
+		}
+		std::fprintf(stderr, "[global DP]   %d problems, %lld rows, %.3g cells, longest %lld rows\n", n, (long long)rows,
+				(double)cells, (long long)nl_max);
+		std::fprintf(stderr, "[global DP]   rows <=64/256/1024/4096/16384/more: problems %lld %lld %lld %lld %lld %lld, rows %lld %lld %lld %lld %lld %lld\n",
+				(long long)hn[0], (long long)hn[1], (long long)hn[2], (long long)hn[3], (long long)hn[4], (long long)hn[5],
+				(long long)hr[0], (long long)hr[1], (long long)hr[2], (long long)hr[3], (long long)hr[4], (long long)hr[5]);
+		for (int t = 0; t < kKeys; ++t) {
+			const int32_t m = kstart[t + 1] - kstart[t];
+			if (m) std::fprintf(stderr, "[global DP]   bucket v%d NW %d C %d: %d problems, %lld rows (mean %.0f, max %lld), %.3g cells\n",
+					key_v(t), key_nw(t), key_c(t), m, (long long)br[t], (double)br[t] / m, (long long)bm[t], (double)bc[t]);
+		}
+		clk.lap("  offsets and buckets", n);
+	}
+	const size_t n_bt = bt_req.size();
 	const size_t o_ns = 0, o_as = al256(o_ns + l_raw), o_ss = al256(o_as + l_as), o_pssm = al256(o_ss + (any_ss? l_raw : 0));
 	const size_t o_pp = al256(o_pssm + l_pssm), o_gp = al256(o_pp + (size_t)n * sizeof(nsh_prep::PrepProb));
-	const size_t o_bt = al256(o_gp + (size_t)n * sizeof(GlobProb)), o_pc = al256(o_bt + bt.size() * sizeof(BtProb));
+	const size_t o_bt = al256(o_gp + (size_t)n * sizeof(GlobProb)), o_pc = al256(o_bt + n_bt * sizeof(BtProb));
 	const size_t o_mat = al256(o_pc + sizeof(nsh_prep::PrepConst)), o_pen = al256(o_mat + 484);
 	const size_t in_bytes = al256(o_pen + 16);
 	int ok = 0;
@@ -290,7 +325,7 @@ int run_list(RunCtx &c, const GlobIn *P, int32_t n, const GlobTerms &T, int32_t 
 	int64_t *bmeta = nullptr, *d_btmeta = nullptr;
 	unsigned long long total = 0, *d_total = nullptr;
 	// device result block: scores (n ints) | CIGAR total | (offset, count) per backtracked problem
-	const size_t meta_bytes = al256((size_t)n * 4) + 8 + bt.size() * 16;
+	const size_t meta_bytes = al256((size_t)n * 4) + 8 + n_bt * 16;
 	*pk_out = nullptr;
 	HIPOK(c.h_in.ensure(in_bytes));
 	HIPOK(c.d_in.ensure(in_bytes));
@@ -300,30 +335,50 @@ int run_list(RunCtx &c, const GlobIn *P, int32_t n, const GlobTerms &T, int32_t 
 	if (l_tmp) { HIPOK(c.d_tmp.ensure((size_t)l_tmp * 4 + 256)); HIPOK(c.d_packed.ensure((size_t)l_tmp * 4 + 256)); }
 	HIPOK(c.d_meta.ensure(meta_bytes + 256));
 	HIPOK(c.h_meta.ensure(meta_bytes + 256));
+	clk.lap("  pools", (int64_t)in_bytes);
 	h = (uint8_t*)c.h_in.p, d = (uint8_t*)c.d_in.p;
-	// pack the inputs
+	// pass 2, on the host threads: problem records and strings straight into the pinned block
 	{
 		nsh_prep::PrepConst pc;
 		std::memcpy(pc.nt4, T.nt4, 256), std::memcpy(pc.aa20, T.aa20, 256), std::memcpy(pc.codon, T.codon, 64);
 		for (int t = 0; t < 6; ++t) pc.sp[t] = T.sp[t];
 		pc.sp_null = T.sp_null;
 		pc.genome = T.genome;
-		for (int32_t k = 0; k < n; ++k) {
-			const GlobIn &q = P[k];
-			if (q.nl && !pp[k].from_genome) std::memcpy(h + o_ns + pp[k].ns_off, q.ns, q.nl);
-			std::memcpy(h + o_as + pp[k].as_off, q.as, q.al);
-			if (any_ss && !pp[k].from_genome) {
-				if (q.ss) std::memcpy(h + o_ss + pp[k].ns_off, q.ss, q.nl);
-				else std::memset(h + o_ss + pp[k].ns_off, 0xff, q.nl);
+		nsh_prep::PrepProb *hpp = (nsh_prep::PrepProb*)(h + o_pp);
+		GlobProb *hgp = (GlobProb*)(h + o_gp);
+		BtProb *hbt = (BtProb*)(h + o_bt);
+		const int32_t nth = (int32_t)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+		nsh_parallel_for((n + 4095) / 4096, nth, [&](int64_t b) {
+			for (int32_t k = (int32_t)(b * 4096); k < std::min<int64_t>(n, (b + 1) * 4096); ++k) {
+				const GlobIn &q = P[k];
+				const Off &o = off[k];
+				nsh_prep::PrepProb p = {};
+				p.from_genome = q.g_off >= 0, p.ns_off = o.ns;
+				p.as_off = o.as, p.rec_off = o.rec, p.aa_off = o.as;
+				// the strand only matters to int32 score-only requests with an extension flag (nasw-sse.c:246-247)
+				p.nl = q.nl, p.al = q.al, p.left = q.left, p.io = q.io, p.has_ss = q.ss != nullptr;
+				hpp[k] = p;
+				GlobProb g = {};
+				g.rec_off = o.rec, g.aa_off = o.as, g.pssm_off = o.pssm, g.tr_off = o.tr;
+				g.nl = q.nl, g.al = q.al, g.io = q.io, g.out = k, g.left = q.left;
+				g.C = key_c(key[k]), g.NW = key_nw(key[k]);
+				const int vs = q.is32? 4 : 8;
+				g.slen = (q.al + vs - 1) / vs, g.npad = q.al, g.mode = 0;
+				hgp[pos[k]] = g;
+				if (bt_slot[k] >= 0) hbt[bt_slot[k]] = { o.tr, o.tmp, q.nl, q.al };
+				if (q.nl && !p.from_genome) std::memcpy(h + o_ns + o.ns, q.ns, q.nl);
+				std::memcpy(h + o_as + o.as, q.as, q.al);
+				if (any_ss && !p.from_genome) {
+					if (q.ss) std::memcpy(h + o_ss + o.ns, q.ss, q.nl);
+					else std::memset(h + o_ss + o.ns, 0xff, q.nl);
+				}
+				if (q.pssm) std::memcpy(h + o_pssm + o.pssm, q.pssm, (size_t)q.al * 22);
 			}
-			if (q.pssm) std::memcpy(h + o_pssm + gp[k].pssm_off, q.pssm, (size_t)q.al * 22);
-		}
-		std::memcpy(h + o_pp, pp.data(), (size_t)n * sizeof(nsh_prep::PrepProb));
-		std::memcpy(h + o_gp, gsorted.data(), (size_t)n * sizeof(GlobProb));
-		if (!bt.empty()) std::memcpy(h + o_bt, bt.data(), bt.size() * sizeof(BtProb));
+		});
 		std::memcpy(h + o_pc, &pc, sizeof(pc));
 		std::memcpy(h + o_mat, T.mat, 484);
 	}
+	clk.lap("  pack", n);
 	{
 		const auto t1 = std::chrono::steady_clock::now();
 		HIPOK(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, c.st[0]));
@@ -334,6 +389,10 @@ int run_list(RunCtx &c, const GlobIn *P, int32_t n, const GlobTerms &T, int32_t 
 			c.d_rec.as<uint32_t>(), c.d_aas.as<uint8_t>(), (const nsh_prep::PrepConst*)(d + o_pc), c.st[0]);
 	HIPOK(hipGetLastError());
 	HIPOK(hipEventRecord(c.ev_prep, c.st[0]));
+	if (clk.on) {
+		HIPOK(hipStreamSynchronize(c.st[0]));
+		clk.lap("  upload, prep", n);
+	}
 	{
 		GlobArgs ga = {};
 		ga.rec = c.d_rec.as<uint32_t>(), ga.aas = c.d_aas.as<uint8_t>(), ga.pssm = (const int8_t*)(d + o_pssm);
@@ -343,31 +402,35 @@ int run_list(RunCtx &c, const GlobIn *P, int32_t n, const GlobTerms &T, int32_t 
 		ga.pen = (const int32_t*)(d + o_pen), ga.pen_n = 1;
 		const GlobProb *d_gp = (const GlobProb*)(d + o_gp);
 		int nb = 0;
-		for (int32_t k = 0; k < n;) {
-			int32_t e = k;
-			const int v = kernel_variant(P[order[k]]);
-			while (e < n && kernel_variant(P[order[e]]) == v && gsorted[e].NW == gsorted[k].NW && gsorted[e].C == gsorted[k].C) ++e;
+		for (int t = 0; t < kKeys; ++t) {
+			const int32_t k = kstart[t], m = kstart[t + 1] - k;
+			if (!m) continue;
 			hipStream_t s = c.st[nb & 3];
 			HIPOK(hipStreamWaitEvent(s, c.ev_prep, 0));
 			ga.prob = d_gp + k;
-			if (v == 0) launch_trace16(ga, e - k, gsorted[k].NW, gsorted[k].C, s);
-			else if (v == 1) launch_trace32(ga, e - k, gsorted[k].NW, gsorted[k].C, s);
-			else launch_score32(ga, e - k, gsorted[k].NW, gsorted[k].C, s);
+			const int v = key_v(t);
+			if (v == 0) launch_trace16(ga, m, key_nw(t), key_c(t), s);
+			else if (v == 1) launch_trace32(ga, m, key_nw(t), key_c(t), s);
+			else launch_score32(ga, m, key_nw(t), key_c(t), s);
 			HIPOK(hipGetLastError());
-			++nb, k = e;
+			++nb;
 		}
 		for (int b = 1; b < std::min(nb, 4); ++b) {
 			HIPOK(hipEventRecord(c.ev_b[b], c.st[b]));
 			HIPOK(hipStreamWaitEvent(c.st[0], c.ev_b[b], 0));
 		}
 		nsh_count_launch(1 + nb);
+		if (clk.on) {
+			HIPOK(hipStreamSynchronize(c.st[0]));
+			clk.lap("  DP kernels", nb);
+		}
 	}
 	d_btmeta = (int64_t*)((char*)c.d_meta.p + al256((size_t)n * 4) + 8);
 	d_total = (unsigned long long*)((char*)c.d_meta.p + al256((size_t)n * 4));
-	if (!bt.empty()) {
+	if (n_bt) {
 		hipLaunchKernelGGL(fill_u64_zero, dim3(1), dim3(64), 0, c.st[0], d_total);
-		hipLaunchKernelGGL(backtrack_kernel, dim3(((int)bt.size() + 63) / 64), dim3(64), 0, c.st[0], (const BtProb*)(d + o_bt),
-				(int32_t)bt.size(), c.d_tr.as<uint16_t>(), c.d_tmp.as<uint32_t>(), c.d_packed.as<uint32_t>(), d_total, d_btmeta);
+		hipLaunchKernelGGL(backtrack_kernel, dim3(((int)n_bt + 63) / 64), dim3(64), 0, c.st[0], (const BtProb*)(d + o_bt),
+				(int32_t)n_bt, c.d_tr.as<uint16_t>(), c.d_tmp.as<uint32_t>(), c.d_packed.as<uint32_t>(), d_total, d_btmeta);
 		HIPOK(hipGetLastError());
 		nsh_count_launch(2);
 	}
@@ -375,14 +438,16 @@ int run_list(RunCtx &c, const GlobIn *P, int32_t n, const GlobTerms &T, int32_t 
 	HIPOK(hipMemcpyAsync(c.h_meta.p, c.d_meta.p, meta_bytes, hipMemcpyDeviceToHost, c.st[0]));
 	HIPOK(hipStreamSynchronize(c.st[0]));
 	nsh_add_times(0, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(), 0);
+	clk.lap("  backtrack, results", (int64_t)n_bt);
 	bmeta = (int64_t*)((char*)c.h_meta.p + al256((size_t)n * 4) + 8);
-	total = bt.empty()? 0 : *(unsigned long long*)((char*)c.h_meta.p + al256((size_t)n * 4));
+	total = !n_bt? 0 : *(unsigned long long*)((char*)c.h_meta.p + al256((size_t)n * 4));
 	if (total) {
 		const auto t3 = std::chrono::steady_clock::now();
 		HIPOK(c.h_packed.ensure((size_t)total * 4));
 		HIPOK(hipMemcpyAsync(c.h_packed.p, c.d_packed.p, (size_t)total * 4, hipMemcpyDeviceToHost, c.st[0]));
 		HIPOK(hipStreamSynchronize(c.st[0]));
 		nsh_add_times(0, 0, std::chrono::duration<double>(std::chrono::steady_clock::now() - t3).count());
+		clk.lap("  CIGAR download", (int64_t)total);
 	}
 	{
 		const int32_t *hs = (const int32_t*)c.h_meta.p;
@@ -390,7 +455,7 @@ int run_list(RunCtx &c, const GlobIn *P, int32_t n, const GlobTerms &T, int32_t 
 			score[k] = P[k].nl > 2? hs[k] : (P[k].is32? -0x40000000 : -32768);
 			cig_off[k] = 0, cig_n[k] = 0;
 		}
-		for (size_t b = 0; b < bt.size(); ++b) cig_off[bt_req[b]] = bmeta[2 * b], cig_n[bt_req[b]] = (int32_t)bmeta[2 * b + 1];
+		for (size_t b = 0; b < n_bt; ++b) cig_off[bt_req[b]] = bmeta[2 * b], cig_n[bt_req[b]] = (int32_t)bmeta[2 * b + 1];
 		*pk_out = (const uint32_t*)c.h_packed.p;
 	}
 	ok = 1;
