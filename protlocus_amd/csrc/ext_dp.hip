@@ -278,6 +278,7 @@ struct nsh_ext_plan_s {
 	std::vector<ExtProb> prob_in;            // the narrow problems' descriptors in input order
 	bool pssm_plan = false;                  // some problem scores with a PSSM (no pairs then)
 	int32_t pair_rows = INT32_MAX;           // problems of more rows run one per wave (the autotuner's choice)
+	bool merged = false;                     // group 0 is the merged C <= 8 queue (launch_group01)
 	int32_t grp_first[kGroups] = {}, grp_n[kGroups] = {};
 	// problems wider than 128 * kCmax padded columns: the work-group-per-problem kernel in extension
 	// mode (global_dp_impl.h), in buckets of equal shape, on their own stream
@@ -311,7 +312,7 @@ struct nsh_ext_plan_s {
 		drop_graph();
 		n = 0, ran = 0, cells_window = 0, max_al = max_nl = 0;
 		al.clear(), out_h.clear(), prob_h.clear(), wide_h.clear(), wide_sub_h.clear(), sub_h.clear(), idx_h.clear();
-		prob_in.clear(), pssm_plan = false, pair_rows = INT32_MAX;
+		prob_in.clear(), pssm_plan = false, pair_rows = INT32_MAX, merged = false;
 		d_ns = d_ss = d_as = d_aas = nullptr, d_rec = nullptr, d_mat = d_pssm = nullptr;
 		d_pen = d_counter = d_out = nullptr, d_prob = nullptr, d_prep = nullptr, d_pc = nullptr;
 		d_wide = d_wide_sub = nullptr, d_prob_sub = nullptr, d_idx_sub = nullptr, d_feat = nullptr;
@@ -387,6 +388,8 @@ void shell_release(nsh_ext_plan_t *p)
 // queue item. Items are ordered small-C group first (its launch), then by the most wave time first
 // in each group (rows x per-row instructions, about 41 + 30 C for one problem and 49 + 30 C for a
 // pair). n_grp receives the items per column-count group.
+static int plan_group(const nsh_ext_plan_t *p, int32_t C) { return p->merged && C <= 8? 0 : group_of(C); }
+
 static void plan_order(nsh_ext_plan_t *p, int32_t pair_rows, int32_t n_grp[kGroups])
 {
 	const std::vector<ExtProb> &prob = p->prob_in;
@@ -411,9 +414,9 @@ static void plan_order(nsh_ext_plan_t *p, int32_t pair_rows, int32_t n_grp[kGrou
 		if (partner[k] < 0) return (int64_t)prob[k].nl * (41 + 30 * prob[k].C);
 		return (int64_t)std::max(prob[k].nl, prob[partner[k]].nl) * (49 + 30 * ((prob[k].npad + 63) / 64));
 	};
-	for (int32_t k = 0; k < n; ++k) if (!second[k]) order.push_back(k), ++n_grp[group_of(prob[k].C)];
+	for (int32_t k = 0; k < n; ++k) if (!second[k]) order.push_back(k), ++n_grp[plan_group(p, prob[k].C)];
 	std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-		const int bx = group_of(prob[x].C), by = group_of(prob[y].C);
+		const int bx = plan_group(p, prob[x].C), by = plan_group(p, prob[y].C);
 		if (bx != by) return bx < by;
 		return cost(x) > cost(y); });
 	const int32_t n_items = (int32_t)order.size();
@@ -483,7 +486,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 	}
 	int64_t l_ns = 0, l_raw = 0, l_as = 0, l_pssm = 0;
 	const uint8_t *genome = nullptr;
-	int32_t max_nl = 2, any_ss = 0, n_grp[kGroups] = {};
+	int32_t max_nl = 2, any_ss = 0, any_long = 0, n_grp[kGroups] = {};
 	std::vector<ExtProb> prob(n);
 	std::vector<PrepProb> prep(n);
 	std::vector<int32_t> order;
@@ -510,6 +513,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		if (q.nl > max_nl) max_nl = q.nl;
 		any_ss |= q.ss != nullptr;
 		if (!is_wide[k]) ++n_grp[group_of(C)];
+		any_long |= (int64_t)q.nl * C >= kLongWork;
 	}
 	nsh_ext_plan_t *p = shell_acquire();
 	if (!p) return nullptr;
@@ -536,6 +540,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		return x.NW != y.NW? x.NW > y.NW : x.C > y.C; });
 	for (int32_t k = 0; k < n; ++k) if (!is_wide[k]) p->prob_in.push_back(prob[k]);
 	p->pssm_plan = l_pssm > 0;
+	p->merged = !p->pssm_plan && any_long && n_grp[0] > 0 && n_grp[1] > 0;
 	plan_order(p, p->pair_rows, n_grp);
 	const int32_t n_narrow = (int32_t)p->prob_h.size();
 	p->out_h.assign((size_t)4 * n, 0);
@@ -622,7 +627,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 			a.go = opt->go, a.ge = opt->ge, a.fs = opt->fs, a.xdrop = opt->xdrop, a.eb = opt->end_bonus;
 			a.counter = p->d_counter + g, a.out = p->d_out;
 			// persistent waves: about as many as the CUs keep resident
-			p->wpc[g] = l_pssm? kGroupWavesPerCUPssm[g] : kGroupWavesPerCU[g];
+			p->wpc[g] = l_pssm? kGroupWavesPerCUPssm[g] : p->merged && g == 0? kMergedWavesPerCU : kGroupWavesPerCU[g];
 			p->grid[g] = (int32_t)std::min<int64_t>(a.n, (int64_t)ncu * p->wpc[g]);
 		}
 		nsh_glob::GlobArgs &w = p->wargs;
@@ -710,7 +715,8 @@ static int plan_launch(nsh_ext_plan_t *p, const ExtArgs *args, const int32_t *gr
 	for (int g = kGroups - 1; g >= 0; --g) {
 		if (grid[g] <= 0) continue;
 		HIPOK(hipStreamWaitEvent(p->gs[g], p->evf, 0));
-		launch_group(g, args[g], grid[g], p->gs[g]);
+		if (g == 0 && p->merged) launch_group01(args[g], grid[g], p->gs[g]);
+		else launch_group(g, args[g], grid[g], p->gs[g]);
 		HIPOK(hipGetLastError());
 		HIPOK(hipEventRecord(p->gev[g], p->gs[g]));
 		HIPOK(hipStreamWaitEvent(p->stream, p->gev[g], 0));

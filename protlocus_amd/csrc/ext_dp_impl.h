@@ -600,10 +600,9 @@ DEV void ext_solve2(const ExtArgs &a, const ExtProb &PA, const ExtProb &PB, int8
 	}
 }
 
-constexpr int64_t kLongWork = (int64_t)1 << 18;   // rows x 128-column chunks of a long problem
-
 // persistent waves over one group's queue; CLO..CHI columns per lane per half. The waves-per-SIMD
-// target trades registers for latency hiding (measured: 6 / 3 / 1 for C <= 4 / <= 8 / <= 16).
+// target trades registers for latency hiding (measured: 6 / 3 / 1 for C <= 4 / <= 8 / <= 16; the
+// merged queue of C 1..8 takes the C <= 8 budget).
 template <int CLO, int CHI, bool PROF>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CHI <= 4? NSH_EXT_WPE0 : CHI <= 8? NSH_EXT_WPE1 : 1)))
 ext_dp_kernel(ExtArgs a)
@@ -625,11 +624,11 @@ ext_dp_kernel(ExtArgs a)
 		const int p = __builtin_amdgcn_readfirstlane(atomicAdd(a.counter, L == 0? 1 : 0));
 		if (p >= a.n) break;
 		const ExtProb P = a.prob[p];
-		if (!PROF && CHI <= 4 && P.pair >= 0) {   // two problems of P.C columns per lane each
+		if (!PROF && CLO <= 4 && P.pair >= 0) {   // two problems of P.C columns per lane each
 			const ExtProb Q = a.pbase[P.pair];
 			if ((int64_t)(P.nl > Q.nl? P.nl : Q.nl) * P.C >= kLongWork) __builtin_amdgcn_s_setprio(2);
 			else __builtin_amdgcn_s_setprio(0);
-#define NSH_C2(K) if (!PROF && K <= CHI && P.C == K) { ext_solve2<(K <= CHI)? K : 1>(a, P, Q, prof, snap, dring, L); continue; }
+#define NSH_C2(K) if (!PROF && K >= CLO && K <= CHI && P.C == K) { ext_solve2<(K >= CLO && K <= CHI)? K : CLO>(a, P, Q, prof, snap, dring, L); continue; }
 			NSH_C2(1) NSH_C2(2) NSH_C2(3) NSH_C2(4)
 #undef NSH_C2
 			continue;

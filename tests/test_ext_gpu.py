@@ -184,6 +184,57 @@ def test_ext_plan_config1_batch(oracle, product, seed, n, lo, hi):
     assert not bad, [(k, len(pairs[k][0]), len(pairs[k][1]), got[k], exp[k]) for k in bad[:5]]
 
 
+@pytest.mark.parametrize("global_mode", [True, False])
+def test_ext_plan_merged_queue_with_a_long_problem(oracle, product, global_mode):
+    # a plan holding a long problem (rows x 128-column chunks >= 2^18: a 60-kb target, 600 aa)
+    # and problems of 1-4 and 5-8 columns per lane runs its C <= 8 problems as one merged queue,
+    # longest first, in one kernel; pairs, single problems of every column count, the C 9-12 group
+    # beside it and a subset run must all equal the oracle
+    from protlocus_amd import synth
+    rng = np.random.default_rng(11 + global_mode)
+    pairs = []
+    p = synth.rand_protein(rng, 600)
+    if global_mode:
+        pairs.append((synth.rand_dna(rng, 60000), p))
+    else:   # the protein's gene with a 60-kb intron: the extension crosses it
+        pairs.append((synth.gene_of(rng, p, 0.1, 1, lambda: 60000, 0, 200), p))
+    for lo, hi, m in [(1, 250, 40), (257, 512, 20), (520, 1000, 20), (1030, 1500, 4)]:
+        for _ in range(m):
+            al = int(rng.integers(lo, hi + 1))
+            q = synth.rand_protein(rng, al)
+            if rng.integers(0, 3) == 0:
+                d = synth.rand_dna(rng, int(rng.integers(3, 3 * al + 400)))
+            else:
+                d = synth.gene_of(rng, q, 0.2, int(rng.integers(0, 3)), lambda: int(rng.integers(40, 400)), 1, 200)
+            pairs.append((d, q))
+    assert len(pairs[0][0]) * ((600 + 127) // 128) >= 1 << 18
+    ns, ns_off, nl, as_, as_off, al = synth.pack_batch(pairs)
+    flag = 0 if global_mode else NS_F_EXT_RIGHT
+    plan = product.ext_plan(ns, ns_off, nl, as_, as_off, al, product.opt(flag=flag))
+    try:
+        plan.run()
+        got = plan.results()
+        keep = np.zeros(len(pairs), np.uint8)
+        keep[::2] = 1
+        assert product.lib.nsh_ext_plan_run_subset(plan.p, bytes(keep)), product.error()
+        assert product.lib.nsh_ext_plan_sync(plan.p)
+        sub = plan.results()
+    finally:
+        plan.close()
+
+    def one(k):
+        d, q = pairs[k]
+        if global_mode:
+            return (len(d), len(q), oracle.global_dp(d, q, oracle.opt(flag=0), mode=16)[2])
+        return _oracle_ext(oracle, product, d, q, dict(flag=NS_F_EXT_RIGHT), None)
+    with cf.ThreadPoolExecutor(8) as ex:
+        exp = list(ex.map(one, range(len(pairs))))
+    bad = [k for k in range(len(pairs)) if got[k] != exp[k]]
+    assert not bad, [(k, len(pairs[k][0]), len(pairs[k][1]), got[k], exp[k]) for k in bad[:5]]
+    bad = [k for k in range(0, len(pairs), 2) if sub[k] != exp[k]]
+    assert not bad, [(k, sub[k], exp[k]) for k in bad[:5]]
+
+
 @pytest.mark.parametrize("xdrop,global_mode", [(20, False), (60, False), (100, True)])
 def test_ext_plan_two_problems_per_wave(oracle, product, xdrop, global_mode):
     # problems of at most 256 padded columns share a wave two at a time (one per 16-bit half):
