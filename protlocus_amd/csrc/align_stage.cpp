@@ -65,6 +65,9 @@ struct Cand {
 	int64_t as = 0, ae = 0, vs0 = 0, vs1 = 0;
 	int32_t i0 = 0, ne0 = 0, ae0 = 0, score = 0;
 	int32_t as1 = 0, as1_win = 0, aa_win = 0, nt_len = 0, aa_len = 0;
+	int64_t ve1 = 0;             // end of the last trusted anchor: where the right extension starts
+	int32_t qe1 = 0;
+	int32_t rnt_len = 0, raa_len = 0;   // the right extension's result
 	bool alive = false, right = false;
 	std::vector<uint32_t> cigar;
 };
@@ -343,40 +346,68 @@ int nsh_align_stage_view(const nsh_genome_view &g, const char *const *aa, const 
 		live.push_back(&c);
 	}
 	clk.lap("start", (int64_t)live.size());
+	// MP_AS_LEFT and MP_AS_RIGHT in one extension plan: the right extension starts at the end of the
+	// last trusted anchor, which neither the left extension nor the gap DPs move (the gap steps end
+	// on the trusted anchors), so both directions run together and the batch has one tail instead
+	// of two; their io_end retries (MP_AS_LEFT_RETRY, MP_AS_RIGHT_RETRY) form a second plan
 	std::vector<nsh_ext_req> req;
 	std::vector<nsh_ext_result_t> res;
-	std::vector<Cand*> who;
-	// MP_AS_LEFT: all left extensions
+	std::vector<Cand*> rlive, who, rwho;
+	for (Cand *c : live) {
+		int32_t il = c->r->cnt - 1;
+		while (!(c->a[il] & kTrusted)) --il;
+		c->ve1 = c->vs0 + anchor_x(c->a[il]) + 1, c->qe1 = anchor_q(c->a[il]) + 1;
+		if (c->qe1 < c->len && c->ve1 < c->ae) c->aa_win = c->len - c->qe1, c->right = true, rlive.push_back(c);
+	}
 	for (Cand *c : live) req.push_back(ext_req(c, 0, (int32_t)(c->vs1 - c->as), 0, c->as1_win, 1, ao->io));
+	for (Cand *c : rlive) req.push_back(ext_req(c, c->ve1 - c->as, (int32_t)(c->ae - c->ve1), c->qe1, c->aa_win, 0, ao->io));
 	if (!ext_round(live, req, o, res)) return 0;
 	req.clear();
-	for (size_t k = 0; k < live.size(); ++k) {
+	const size_t n_live = live.size();
+	for (size_t k = 0; k < n_live; ++k) {
 		Cand *c = live[k];
 		c->nt_len = res[k].nt_len, c->aa_len = res[k].aa_len;
-		// MP_AS_LEFT_RETRY: a 5'-end exon missed by the extension gets another try at io_end
+		// a 5'-end exon missed by the extension gets another try at io_end
 		if (res[k].aa_len != c->as1_win && res[k].nt_len < ao->max_ext && ao->io > ao->io_end) {
 			const int64_t as_alt = c->vs1 - c->as > ao->end_retry_win? c->vs1 - ao->end_retry_win : c->as;
 			req.push_back(ext_req(c, as_alt - c->as, (int32_t)(c->vs1 - as_alt), 0, c->as1_win, 1, ao->io_end));
 			who.push_back(c);
 		}
 	}
+	for (size_t k = 0; k < rlive.size(); ++k) {
+		Cand *c = rlive[k];
+		c->rnt_len = res[n_live + k].nt_len, c->raa_len = res[n_live + k].aa_len;
+		// the same for a 3'-end exon
+		if (c->raa_len < c->aa_win && c->rnt_len < ao->max_ext && ao->io > ao->io_end) {
+			const int32_t l_ext = (int32_t)std::min<int64_t>(c->ae - c->ve1, ao->end_retry_win);
+			req.push_back(ext_req(c, c->ve1 - c->as, l_ext, c->qe1, c->aa_win, 0, ao->io_end));
+			rwho.push_back(c);
+		}
+	}
+	clk.lap("extensions", (int64_t)(n_live + rlive.size()));
 	if (!ext_round(who, req, o, res)) return 0;
 	for (size_t k = 0; k < who.size(); ++k)
 		if (res[k].aa_len == who[k]->as1_win) who[k]->nt_len = res[k].nt_len, who[k]->aa_len = res[k].aa_len;
-	clk.lap("left extensions", (int64_t)who.size());
-	// MP_AS_GAP: the DPs between consecutive trusted anchors (mp_align_gap, align.c:482-540)
+	for (size_t k = 0; k < rwho.size(); ++k) {
+		const nsh_ext_result_t &x = res[who.size() + k];
+		if (x.aa_len == rwho[k]->aa_win) rwho[k]->rnt_len = x.nt_len, rwho[k]->raa_len = x.aa_len;
+	}
+	clk.lap("extension retries", (int64_t)(who.size() + rwho.size()));
+	// MP_AS_GAP (the DPs between consecutive trusted anchors, mp_align_gap, align.c:482-540) and
+	// MP_AS_RIGHT_APPLY (the DP over the right extension) as one global-DP batch; a candidate's
+	// tasks are its gap steps, then its right DP
 	const int32_t nth = 16;
 	TaskSet ts;
-	ts.first.assign(live.size() + 1, 0);
-	nsh_parallel_for((int64_t)live.size(), nth, [&](int64_t k) {   // tasks per candidate
+	ts.first.assign(n_live + 1, 0);
+	nsh_parallel_for((int64_t)n_live, nth, [&](int64_t k) {   // tasks per candidate
 		const Cand *c = live[k];
-		int64_t m = 0;
+		int64_t m = c->right;
 		for (int32_t i = c->i0; i < c->r->cnt; ++i) m += (c->a[i] & kTrusted) != 0;
 		ts.first[k + 1] = m;
 	});
-	for (size_t k = 0; k < live.size(); ++k) ts.first[k + 1] += ts.first[k];
-	ts.t.resize((size_t)ts.first[live.size()]);
-	nsh_parallel_for((int64_t)live.size(), nth, [&](int64_t k) {
+	for (size_t k = 0; k < n_live; ++k) ts.first[k + 1] += ts.first[k];
+	ts.t.resize((size_t)ts.first[n_live]);
+	nsh_parallel_for((int64_t)n_live, nth, [&](int64_t k) {
 		Cand *c = live[k];
 		nsh_align_reg_t *r = c->r;
 		r->vs = c->vs1 - c->nt_len;
@@ -392,55 +423,17 @@ int nsh_align_stage_view(const nsh_genome_view &g, const char *const *aa, const 
 			t.score = 0, t.dp = false;
 			c->i0 = i, c->ne0 = ne1, c->ae0 = ae1;
 		}
-		r->ve = c->ne0 + c->vs0, r->qe = c->ae0;
+		r->ve = c->ne0 + c->vs0, r->qe = c->ae0;   // the last trusted anchor's end: ve1, qe1
+		if (c->right) {
+			Task &t = ts.t[w++];
+			t.c = c, t.nt_off = r->ve - c->as, t.nlen = c->rnt_len, t.aa_off = r->qe, t.alen = c->raa_len, t.score = 0, t.dp = false;
+		}
 	});
-	clk.lap("gap tasks", (int64_t)ts.t.size());
+	clk.lap("DP tasks", (int64_t)ts.t.size());
 	if (!dp_round(ts, o, ao, nth)) return 0;
 	merge_tasks(ts, live, nth);
-	clk.lap("gap DPs", (int64_t)ts.t.size());
-	// MP_AS_RIGHT: right extensions where protein and window remain
-	req.clear(), who.clear();
-	for (Cand *c : live) {
-		nsh_align_reg_t *r = c->r;
-		if (r->qe < c->len && r->ve < c->ae) {
-			c->aa_win = c->len - r->qe;
-			c->right = true;
-			req.push_back(ext_req(c, r->ve - c->as, (int32_t)(c->ae - r->ve), r->qe, c->aa_win, 0, ao->io));
-			who.push_back(c);
-		}
-	}
-	if (!ext_round(who, req, o, res)) return 0;
-	std::vector<Cand*> rwho;
-	std::vector<nsh_ext_req> rreq;
-	for (size_t k = 0; k < who.size(); ++k) {
-		Cand *c = who[k];
-		c->nt_len = res[k].nt_len, c->aa_len = res[k].aa_len;
-		// MP_AS_RIGHT_RETRY: the same for a 3'-end exon
-		if (c->aa_len < c->aa_win && c->nt_len < ao->max_ext && ao->io > ao->io_end) {
-			const nsh_align_reg_t *r = c->r;
-			const int32_t l_ext = (int32_t)std::min<int64_t>(c->ae - r->ve, ao->end_retry_win);
-			rreq.push_back(ext_req(c, r->ve - c->as, l_ext, r->qe, c->aa_win, 0, ao->io_end));
-			rwho.push_back(c);
-		}
-	}
-	if (!ext_round(rwho, rreq, o, res)) return 0;
-	for (size_t k = 0; k < rwho.size(); ++k)
-		if (res[k].aa_len == rwho[k]->aa_win) rwho[k]->nt_len = res[k].nt_len, rwho[k]->aa_len = res[k].aa_len;
-	clk.lap("right extensions", (int64_t)rwho.size());
-	// MP_AS_RIGHT_APPLY: the DP over the right extension
-	ts.t.assign(who.size(), Task{});
-	ts.first.assign(who.size() + 1, 0);
-	for (size_t k = 0; k < who.size(); ++k) {
-		Cand *c = who[k];
-		const nsh_align_reg_t *r = c->r;
-		Task &t = ts.t[k];
-		t.c = c, t.nt_off = r->ve - c->as, t.nlen = c->nt_len, t.aa_off = r->qe, t.alen = c->aa_len, t.score = 0, t.dp = false;
-		ts.first[k + 1] = (int64_t)k + 1;
-	}
-	if (!dp_round(ts, o, ao, nth)) return 0;
-	merge_tasks(ts, who, nth);
-	for (Cand *c : who) c->r->ve += c->nt_len, c->r->qe += c->aa_len;
-	clk.lap("right DPs", (int64_t)ts.t.size());
+	for (Cand *c : rlive) c->r->ve += c->rnt_len, c->r->qe += c->raa_len;
+	clk.lap("gap and right DPs", (int64_t)ts.t.size());
 	// MP_AS_FINISH: the CIGARs, one after another
 	int64_t tot = 0;
 	for (Cand *c : live) tot += (int64_t)c->cigar.size();

@@ -12,7 +12,7 @@
 //     writes the row records; problems are sorted into buckets of equal shape (waves per
 //     work-group, columns per lane; global_dp_impl.h) launched on four streams;
 //   * the trace (one uint16 word per cell, the reference's bit layout) stays in HBM: a backtrack
-//     kernel walks it, one lane per problem, writes each CIGAR (reversed, tiny U/V turned into G,
+//     kernel walks it, one wave per problem, writes each CIGAR (reversed, tiny U/V turned into G,
 //     nasw-sse.c:34-42) at an offset reserved with one atomic, and only the CIGARs cross PCIe;
 //   * device and pinned buffers are grow-only pools owned by the executor, so a batch allocates
 //     nothing once the pools have reached the workload's size.
@@ -38,60 +38,57 @@ using namespace nsh_glob;
 namespace {
 
 // ---------------------------------------------------------------------------------------
-// device backtrack: one lane per problem walks its trace words from (nl-1, al-1)
+// device backtrack: one wave per problem. The walk is wave-uniform; its trace words come from a
+// window the 64 lanes fetch together, one word each along the last move's direction (an intron is
+// a straight run up one column, an exon a 3:1 diagonal), so one memory round trip serves up to
+// 64 steps, and each step reads its word from the window lane.
 // ---------------------------------------------------------------------------------------
 struct BtProb {
 	int64_t tr_off, tmp_off;   // trace words; scratch for the ops in push order
 	int32_t nl, al;
 };
 
-// the ops in push order go to scratch; the op being extended stays in registers (cur, one
-// uint32 len << 4 | op) so that a step needs no read of the scratch
+// the ops in push order go to scratch (written by lane 0); the op being extended stays in
+// registers (cur, one uint32 len << 4 | op)
 struct BtRun {
 	uint32_t *t;
 	int32_t n;       // ops written to t
 	uint32_t cur;    // the open op (0xffffffff: none)
+	bool writer;     // lane 0
+	__device__ void put(uint32_t c) { if (writer) t[n] = c; ++n; }
 	__device__ void push(uint32_t op, int32_t len)
 	{
 		// ns_push_cigar (nasw.h:219-229): F and G are never merged
 		if (cur != 0xffffffffu && op == (cur & 0xf) && op != NS_CIGAR_F && op != NS_CIGAR_G) cur += (uint32_t)len << 4;
 		else {
-			if (cur != 0xffffffffu) t[n++] = cur;
+			if (cur != 0xffffffffu) put(cur);
 			cur = (uint32_t)len << 4 | op;
 		}
 	}
-	__device__ void close() { if (cur != 0xffffffffu) t[n++] = cur, cur = 0xffffffffu; }
+	__device__ void close() { if (cur != 0xffffffffu) put(cur), cur = 0xffffffffu; }
 };
 
+// grid = problems, 64 threads each
 __global__ void __launch_bounds__(64) backtrack_kernel(const BtProb *bp, int32_t n, const uint16_t *trace,
 		uint32_t *tmp, uint32_t *packed, unsigned long long *total, int64_t *meta)
 {
-	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	const int k = blockIdx.x, L = threadIdx.x;
 	if (k >= n) return;
 	const BtProb P = bp[k];
 	const uint16_t *tr = trace + P.tr_off;
-	BtRun b = { tmp + P.tmp_off, 0, 0xffffffffu };
+	BtRun b = { tmp + P.tmp_off, 0, 0xffffffffu, L == 0 };
 	int32_t i = P.nl - 1, j = P.al - 1, last = 0;
 	const int64_t al = P.al;
-	// The walk's words come from a window of kBtWin words fetched together along the last move's
-	// direction (di rows, dj columns per step): an intron is a straight run up one column and an
-	// exon a 3:1 diagonal, so one memory latency serves up to kBtWin steps instead of one.
-	constexpr int kBtWin = 16;
-	// win[q] holds the word at (wi - wdi * q, wj - wdj * q)
-	int32_t win[kBtWin], wi = -1, wj = -1, wdi = 1, wdj = 0, di = 1, dj = 0;
+	// the window: lane q holds the word at (wi - wdi * q, wj - wdj * q); (di, dj): the last move
+	int32_t win = 0, wi = -1, wj = -1, wdi = 1, wdj = 0, di = 1, dj = 0;
 	while (i >= 2 && j >= 0) {   // nasw-sse.c:44-93
 		int32_t t = wdj? wj - j : (wi - i) / wdi;
-		if (wj < 0 || t < 0 || t >= kBtWin || i != wi - wdi * t || j != wj - wdj * t) {
+		if (wj < 0 || t < 0 || t >= 64 || i != wi - wdi * t || j != wj - wdj * t) {
 			wi = i, wj = j, wdi = di, wdj = dj, t = 0;
-#pragma unroll
-			for (int q = 0; q < kBtWin; ++q) {
-				const int32_t ii = i - di * q, jj = j - dj * q;
-				win[q] = ii >= 0 && jj >= 0? tr[(int64_t)ii * al + jj] : 0;
-			}
+			const int32_t ii = i - di * L, jj = j - dj * L;
+			win = ii >= 0 && jj >= 0? tr[(int64_t)ii * al + jj] : 0;
 		}
-		int32_t x = win[0], state, ext;
-#pragma unroll
-		for (int q = 1; q < kBtWin; ++q) if (t == q) x = win[q];
+		int32_t x = __builtin_amdgcn_readlane(win, __builtin_amdgcn_readfirstlane(t)), state, ext;
 		const int32_t i0 = i, j0 = j;
 		if (x >> 9 & 1) x = 1 | x >> 4 << 4;
 		state = last == 0? x & 0xf : last;
@@ -120,15 +117,19 @@ __global__ void __launch_bounds__(64) backtrack_kernel(const BtProb *bp, int32_t
 	}
 	b.close();
 	const int32_t nc = b.n;
-	uint32_t *t = b.t;
-	const unsigned long long off = atomicAdd(total, (unsigned long long)nc);
-	for (int32_t q = 0; q < nc; ++q) {   // reversed, U/V shorter than 3 become G (nasw-sse.c:34-42)
+	unsigned long long off = 0;
+	if (L == 0) off = atomicAdd(total, (unsigned long long)nc);
+	off = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(off >> 32)) << 32) |
+		(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)off);
+	__syncthreads();   // lane 0's scratch writes are visible to the wave
+	const uint32_t *t = b.t;
+	for (int32_t q = L; q < nc; q += 64) {   // reversed, U/V shorter than 3 become G (nasw-sse.c:34-42)
 		uint32_t c = t[nc - 1 - q];
 		const uint32_t op = c & 0xf;
 		if ((op == NS_CIGAR_U || op == NS_CIGAR_V) && c >> 4 < 3) c = c >> 4 << 4 | NS_CIGAR_G;
 		packed[off + q] = c;
 	}
-	meta[2 * k] = (int64_t)off, meta[2 * k + 1] = nc;
+	if (L == 0) meta[2 * k] = (int64_t)off, meta[2 * k + 1] = nc;
 }
 
 __global__ void fill_u64_zero(unsigned long long *p) { if (threadIdx.x == 0) *p = 0; }
@@ -429,7 +430,7 @@ int run_list(RunCtx &c, const GlobIn *P, int32_t n, const GlobTerms &T, int32_t 
 	d_total = (unsigned long long*)((char*)c.d_meta.p + al256((size_t)n * 4));
 	if (n_bt) {
 		hipLaunchKernelGGL(fill_u64_zero, dim3(1), dim3(64), 0, c.st[0], d_total);
-		hipLaunchKernelGGL(backtrack_kernel, dim3(((int)n_bt + 63) / 64), dim3(64), 0, c.st[0], (const BtProb*)(d + o_bt),
+		hipLaunchKernelGGL(backtrack_kernel, dim3((int)n_bt), dim3(64), 0, c.st[0], (const BtProb*)(d + o_bt),
 				(int32_t)n_bt, c.d_tr.as<uint16_t>(), c.d_tmp.as<uint32_t>(), c.d_packed.as<uint32_t>(), d_total, d_btmeta);
 		HIPOK(hipGetLastError());
 		nsh_count_launch(2);
