@@ -10,6 +10,7 @@
 // what the reference does between the DPs: interval bookkeeping, the ungapped scores of anchor-to-
 // anchor steps and CIGAR concatenation.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -75,6 +76,7 @@ struct Cand {
 // one DP of mp_align_seq (align.c:103-145): an anchor-to-anchor step of exactly alen codons and at
 // most kmer2 residues is scored without a DP; anything else goes to the global DP with CIGAR
 struct Task {
+	Task() {}             // every field is written where a task is made (no zero fill of millions)
 	Cand *c;
 	int64_t nt_off;
 	int32_t nlen, aa_off, alen;
@@ -311,8 +313,12 @@ int nsh_align_stage_view(const nsh_genome_view &g, const char *const *aa, const 
 	StepClock clk;
 	std::vector<Cand> cand(n_reg);
 	std::vector<Cand*> live;
-	// MP_AS_START: trusted anchors, the search window, its bases (mp_align_start, align.c:431-480)
-	for (int32_t k = 0; k < n_reg; ++k) {
+	const int32_t nth = 16;
+	// MP_AS_START: trusted anchors, the search window, its bases (mp_align_start, align.c:431-480),
+	// every candidate on the host threads; the first region outside its contig fails the call
+	std::atomic<int32_t> bad_reg(INT32_MAX);
+	nsh_parallel_for(((int64_t)n_reg + 255) / 256, nth, [&](int64_t b) {
+	for (int32_t k = (int32_t)(b * 256); k < std::min<int64_t>(n_reg, (b + 1) * 256); ++k) {
 		nsh_align_reg_t *r = &reg[k];
 		Cand &c = cand[k];
 		c.r = r, r->status = 0, r->dp_score = 0, r->n_cigar = 0, r->cigar_off = 0;
@@ -334,8 +340,8 @@ int nsh_align_stage_view(const nsh_genome_view &g, const char *const *aa, const 
 		c.as = r->vs > extl? r->vs - extl : 0;
 		c.ae = r->ve + extr < clen? r->ve + extr : clen;
 		if (c.as >= clen || c.ae <= c.as) {
-			nsh_set_error("nsh_align_stage: region %d outside its contig", k);
-			return 0;
+			for (int32_t x = bad_reg.load(); k < x && !bad_reg.compare_exchange_weak(x, k);) {}
+			continue;
 		}
 		c.g_off = g.at(r->vid, c.as), c.nt = g.h_g + c.g_off, c.d_g = g.d_g;
 		c.vs0 = r->vs;
@@ -343,8 +349,13 @@ int nsh_align_stage_view(const nsh_genome_view &g, const char *const *aa, const 
 		c.as1 = anchor_q(c.a[i]) + 1;
 		c.as1_win = c.as1;
 		c.alive = true;
-		live.push_back(&c);
 	}
+	});
+	if (bad_reg.load() != INT32_MAX) {
+		nsh_set_error("nsh_align_stage: region %d outside its contig", bad_reg.load());
+		return 0;
+	}
+	for (Cand &c : cand) if (c.alive) live.push_back(&c);
 	clk.lap("start", (int64_t)live.size());
 	// MP_AS_LEFT and MP_AS_RIGHT in one extension plan: the right extension starts at the end of the
 	// last trusted anchor, which neither the left extension nor the gap DPs move (the gap steps end
@@ -396,7 +407,6 @@ int nsh_align_stage_view(const nsh_genome_view &g, const char *const *aa, const 
 	// MP_AS_GAP (the DPs between consecutive trusted anchors, mp_align_gap, align.c:482-540) and
 	// MP_AS_RIGHT_APPLY (the DP over the right extension) as one global-DP batch; a candidate's
 	// tasks are its gap steps, then its right DP
-	const int32_t nth = 16;
 	TaskSet ts;
 	ts.first.assign(n_live + 1, 0);
 	nsh_parallel_for((int64_t)n_live, nth, [&](int64_t k) {   // tasks per candidate

@@ -573,23 +573,29 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 	if (!cands.empty()) st[16] = (int64_t)std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t_start).count();
 
 	// ---- refine: every region's window against its query (map.c:141-258, 666-684)
+	nsh_step_clock hclk("nsh_map_batch");
+	hclk.lap("  (region selection done)");
 	struct RefJob { int32_t q, r; int64_t as; };
 	std::vector<RefJob> jobs;
 	std::vector<std::vector<uint64_t>> qsk(n_q);
 	std::vector<std::vector<uint64_t>> ext0(n_q);
-	for (int32_t q = 0; q < n_q; ++q) {
+	nsh_parallel_for(n_q, nth, [&](int64_t q) {
 		ext0[q] = cal_max_ext(pool[q], regs[q], false, nullptr, nullptr, ca + c_off[q], 100, o->dp.max_ext,
 				o->dp.max_ext);
+	});
+	for (int32_t q = 0; q < n_q; ++q)
 		for (int32_t i = 0; i < (int32_t)regs[q].size(); ++i) jobs.push_back({ q, i, 0 });
-	}
 	st[1] = (int64_t)jobs.size();
+	hclk.lap("  extension limits", (int64_t)jobs.size());
 	nsh_parallel_for(n_q, nth, [&](int64_t q) {
 		sketch_prot(seq[q], qlen[q], kmer2, 0, qsk[q]);
 		std::sort(qsk[q].begin(), qsk[q].end());
 	});
+	hclk.lap("  query sketches", n_q);
 	// the windows (mp_refine_reg, map.c:152-156): slices of the oriented genome in HBM
 	std::vector<nsh_refine::Job> rjobs(jobs.size());
-	for (size_t t = 0; t < jobs.size(); ++t) {
+	nsh_parallel_for(((int64_t)jobs.size() + 4095) / 4096, nth, [&](int64_t b) {
+	for (size_t t = (size_t)b * 4096; t < std::min(jobs.size(), (size_t)(b + 1) * 4096); ++t) {
 		RefJob &jb = jobs[t];
 		const Reg &r = pool[jb.q][regs[jb.q][jb.r]];
 		const int32_t extl = (int32_t)(ext0[jb.q][jb.r] >> 32), extr = (int32_t)ext0[jb.q][jb.r];
@@ -600,19 +606,22 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 		rjobs[t].len = (int32_t)std::max<int64_t>(0, ae - as);
 		rjobs[t].q = jb.q;
 	}
+	});
 	std::vector<int64_t> qoff(n_q + 1, 0);
 	for (int32_t q = 0; q < n_q; ++q) qoff[q + 1] = qoff[q] + (int64_t)qsk[q].size();
-	std::vector<uint64_t> qall((size_t)qoff[n_q]);
-	for (int32_t q = 0; q < n_q; ++q) std::copy(qsk[q].begin(), qsk[q].end(), qall.begin() + qoff[q]);
+	std::unique_ptr<uint64_t[]> qall_buf(new uint64_t[(size_t)qoff[n_q] + 1]);
+	nsh_parallel_for(n_q, nth, [&](int64_t q) { std::copy(qsk[q].begin(), qsk[q].end(), qall_buf.get() + qoff[q]); });
+	hclk.lap("  windows, sketches packed", (int64_t)qoff[n_q]);
 	lap(2);
 	// window k-mers shared with the query, bucket pairs and the window chains of every region (map.c:153-203)
 	std::vector<uint64_t> best_u, best_a;
 	std::vector<int64_t> best_off;
 	const nsh_chain::Params rcp = { o->dp.max_intron, o->max_gap, o->bw, o->max_chn_max_skip, o->max_chn_iter,
 		o->min_chn_cnt, o->min_chn_sc, o->chn_coef_log, 1, kmer2, 0 };
-	if (!nsh_refine::run(m->d_g, rjobs, qoff, qall, kmer2, m->min_aa_len, o->max_ava, rcp, best_u, best_off, best_a,
+	if (!nsh_refine::run(m->d_g, rjobs, qoff, qall_buf.get(), qoff[n_q], kmer2, m->min_aa_len, o->max_ava, rcp, best_u, best_off, best_a,
 			&st[2]))
 		return 0;
+	hclk.lap("  refine on the device", (int64_t)jobs.size());
 	nsh_parallel_for((int64_t)jobs.size(), nth, [&](int64_t t) {   // the best chain becomes the region (map.c:249-258)
 		const RefJob &jb = jobs[t];
 		Reg &r = pool[jb.q][regs[jb.q][jb.r]];
@@ -641,31 +650,43 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 		select_sub(pool[q], regs[q], sel_ratio * sel_ratio, m->kmer * 2, sel_n);
 	});
 
+	hclk.lap("  refined regions", n_q);
 	lap(3);
 	// ---- align: every candidate of every query through the device stages (map.c:701-739)
-	std::vector<nsh_align_reg_t> cand;
-	std::vector<uint64_t> canch;
-	for (int32_t q = 0; q < n_q; ++q) {
+	// candidates in query order: first candidate and first anchor of each query, then the fill
+	std::vector<int64_t> cfirst(n_q + 1, 0), afirst(n_q + 1, 0);
+	nsh_parallel_for(n_q, nth, [&](int64_t q) {
+		int64_t na = 0;
+		for (int32_t i : regs[q]) na += (int64_t)pool[q][i].a.size();
+		cfirst[q + 1] = (int64_t)regs[q].size(), afirst[q + 1] = na;
+	});
+	for (int32_t q = 0; q < n_q; ++q) cfirst[q + 1] += cfirst[q], afirst[q + 1] += afirst[q];
+	std::vector<nsh_align_reg_t> cand((size_t)cfirst[n_q]);
+	std::unique_ptr<uint64_t[]> canch(new uint64_t[(size_t)afirst[n_q] + 1]);
+	canch[afirst[n_q]] = 0;
+	nsh_parallel_for(n_q, nth, [&](int64_t q) {
 		const std::vector<uint64_t> ext = cal_max_ext(pool[q], regs[q], true, m->off.data(), m->len.data(), nullptr, 100,
 				o->dp.max_intron / 2, o->dp.max_intron / 2);
+		int64_t ao = afirst[q];
 		for (size_t i = 0; i < regs[q].size(); ++i) {
 			Reg &r = pool[q][regs[q][i]];
-			nsh_align_reg_t c;
+			nsh_align_reg_t &c = cand[(size_t)cfirst[q] + i];
 			std::memset(&c, 0, sizeof(c));
-			c.query = q, c.vid = r.vid, c.qs = r.qs, c.qe = r.qe, c.vs = r.vs, c.ve = r.ve, c.cnt = r.cnt;
-			c.chn_sc = r.chn_sc, c.chn_sc_ungap = r.chn_sc_ungap, c.a_off = (int64_t)canch.size(), c.ext = ext[i];
-			canch.insert(canch.end(), r.a.begin(), r.a.end());
-			r.slot = (int32_t)cand.size();
-			cand.push_back(c);
+			c.query = (int32_t)q, c.vid = r.vid, c.qs = r.qs, c.qe = r.qe, c.vs = r.vs, c.ve = r.ve, c.cnt = r.cnt;
+			c.chn_sc = r.chn_sc, c.chn_sc_ungap = r.chn_sc_ungap, c.a_off = ao, c.ext = ext[i];
+			std::copy(r.a.begin(), r.a.end(), canch.get() + ao);
+			ao += (int64_t)r.a.size();
+			r.slot = (int32_t)(cfirst[q] + (int64_t)i);
 		}
-	}
-	canch.push_back(0);
+	});
 	st[3] = (int64_t)cand.size();
 	uint32_t *cig = nullptr;
 	std::vector<const char*> aa_p(seq, seq + n_q);
-	if (!cand.empty() && !nsh_align_stage_view(m->view, aa_p.data(), qlen, (int32_t)cand.size(), cand.data(), canch.data(),
+	hclk.lap("  align candidates", (int64_t)cand.size());
+	if (!cand.empty() && !nsh_align_stage_view(m->view, aa_p.data(), qlen, (int32_t)cand.size(), cand.data(), canch.get(),
 			&o->dp, &cig))
 		return 0;
+	hclk.lap("  align stage", (int64_t)cand.size());
 	lap(4);
 	ns_opt_t so;
 	ns_opt_init(&so);
@@ -693,6 +714,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 		if (nf == 0) r.st.n_exon = -1;   // the CIGAR does not span the region: the reference drops the details
 	});
 	if (fail) { std::free(cig); nsh_set_error("nsh_map_batch: alignment statistics failed"); return 0; }
+	hclk.lap("  alignment statistics", (int64_t)cand.size());
 
 	// ---- after the DP: ranking and selection per query (map.c:840-866), then the records (map.c:1143-1166)
 	std::vector<std::string> text(n_q);
@@ -721,6 +743,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 		set_parent(pool[q], rq, o->mask_level, o->mask_len);
 		select_sub(pool[q], rq, sel_ratio, m->kmer * 2, sel_n);
 	});
+	hclk.lap("  ranking", n_q);
 	// the records: ids run over the batch in query order, as the output step numbers them
 	std::vector<int64_t> id0(n_q + 1, 0);
 	std::vector<std::vector<int32_t>> emit(n_q);
@@ -777,6 +800,7 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 	});
 	std::free(cig);
 	if (fail) { nsh_set_error("nsh_map_batch: writing a record failed"); return 0; }
+	hclk.lap("  records", n_q);
 	int64_t tot = 0;
 	for (const auto &s : text) tot += (int64_t)s.size();
 	char *o_buf = (char*)std::malloc((size_t)tot + 1);

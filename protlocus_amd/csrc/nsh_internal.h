@@ -178,7 +178,7 @@ int oriented_genome(const uint8_t *nt4, const int64_t *ctg_off, const int64_t *c
 // (window position << 32 | query position) are anchors[best_off[j] ..). *n_pairs: the anchor pairs
 // chained. 1 on success.
 int run(const uint8_t *d_g, const std::vector<Job> &jobs, const std::vector<int64_t> &qoff,
-		const std::vector<uint64_t> &qsk, int32_t kmer, int32_t min_aa_len, int32_t max_ava, const nsh_chain::Params &cp,
+		const uint64_t *qsk, int64_t n_qsk, int32_t kmer, int32_t min_aa_len, int32_t max_ava, const nsh_chain::Params &cp,
 		std::vector<uint64_t> &best_u, std::vector<int64_t> &best_off, std::vector<uint64_t> &anchors, int64_t *n_pairs);
 }
 

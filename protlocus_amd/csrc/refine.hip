@@ -245,7 +245,7 @@ done:
 }
 
 int nsh_refine::run(const uint8_t *d_g, const std::vector<Job> &jobs, const std::vector<int64_t> &qoff,
-		const std::vector<uint64_t> &qsk, int32_t kmer, int32_t min_aa_len, int32_t max_ava, const nsh_chain::Params &cp,
+		const uint64_t *qsk, int64_t n_qsk, int32_t kmer, int32_t min_aa_len, int32_t max_ava, const nsh_chain::Params &cp,
 		std::vector<uint64_t> &best_u, std::vector<int64_t> &best_off, std::vector<uint64_t> &anchors, int64_t *n_pairs)
 {
 	const int32_t n = (int32_t)jobs.size();
@@ -294,7 +294,7 @@ int nsh_refine::run(const uint8_t *d_g, const std::vector<Job> &jobs, const std:
 		RFOK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
 		RFOK(hipMalloc((void**)&d_jobs, (size_t)n * sizeof(Job)));
 		RFOK(hipMalloc((void**)&d_qoff, qoff.size() * 8));
-		RFOK(hipMalloc((void**)&d_qsk, (qsk.size() + 1) * 8));
+		RFOK(hipMalloc((void**)&d_qsk, ((size_t)n_qsk + 1) * 8));
 		RFOK(hipMalloc((void**)&d_cnt, (size_t)n * 8));
 		RFOK(hipMalloc((void**)&d_cap, (size_t)n * 8));
 		RFOK(hipMalloc((void**)&d_capoff, (size_t)n * 8));
@@ -303,7 +303,7 @@ int nsh_refine::run(const uint8_t *d_g, const std::vector<Job> &jobs, const std:
 		RFOK(hipMalloc((void**)&d_rc, sizeof(RConst)));
 		RFOK(hipMemcpyAsync(d_jobs, jobs.data(), (size_t)n * sizeof(Job), hipMemcpyHostToDevice, s));
 		RFOK(hipMemcpyAsync(d_qoff, qoff.data(), qoff.size() * 8, hipMemcpyHostToDevice, s));
-		if (!qsk.empty()) RFOK(hipMemcpyAsync(d_qsk, qsk.data(), qsk.size() * 8, hipMemcpyHostToDevice, s));
+		if (n_qsk > 0) RFOK(hipMemcpyAsync(d_qsk, qsk, (size_t)n_qsk * 8, hipMemcpyHostToDevice, s));
 		RFOK(hipMemcpyAsync(d_rc, &rc, sizeof(RConst), hipMemcpyHostToDevice, s));
 		// the shared window k-mers of every job, in one sweep: each job writes at most its room
 		// (its window length, at most 2,048, a few hundred being usual); a job with more is swept
