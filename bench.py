@@ -836,12 +836,37 @@ def main():
     ap.add_argument("--map-batch", type=int, default=20000, help="proteins per nsh_map_batch call")
     ap.add_argument("--only-map-prescreen", action="store_true",
                     help="run only the learned prescreen on the map path's candidates and print its JSON")
+    ap.add_argument("--only-prescreen", type=int, default=0, metavar="PROTEINS",
+                    help="run only the learned prescreen section with PROTEINS evaluation proteins (two candidates "
+                         "each; 524288 gives the workload's ~1M candidates) and print its JSON")
     ap.add_argument("--only-dropin", action="store_true",
                     help="run only the drop-in section (reference calling patterns from C threads) and print its JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device three times (a copy kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
+    ap.add_argument("--scalar-read-calibration", type=int, default=0, metavar="MIB",
+                    help="only read MIB MiB through the scalar data cache three times (nsh_diag_scalar_read: the "
+                         "path of the extension DP's row records) and exit")
     args = ap.parse_args()
+    if args.scalar_read_calibration:
+        hip = C.CDLL("libamdhip64.so")
+        hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        hip.hipFree.argtypes = [C.c_void_p]
+        from protlocus_amd import capi
+        lib = capi.lib()
+        lib.nsh_diag_scalar_read.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+        n = args.scalar_read_calibration << 20
+        src, out = C.c_void_p(), C.c_void_p()
+        if hip.hipMalloc(C.byref(src), n) or hip.hipMalloc(C.byref(out), n // 65536 * 4 + 4) or hip.hipMemset(src, 1, n):
+            raise SystemExit("hipMalloc/hipMemset failed")
+        hip.hipDeviceSynchronize()
+        for _ in range(3):
+            if not lib.nsh_diag_scalar_read(src, n, out):
+                raise SystemExit("nsh_diag_scalar_read failed")
+        hip.hipFree(src), hip.hipFree(out)
+        print(json.dumps({"scalar_read_calibration_bytes": n, "launches": 3}), flush=True)
+        return
     if args.copy_calibration:
         # HIP runtime device-to-device copies (the runtime's copy kernel): n bytes read, n written
         hip = C.CDLL("libamdhip64.so")
@@ -880,6 +905,13 @@ def main():
     if args.only_map_prescreen:
         from protlocus_amd import prescreen
         print(json.dumps(prescreen.map_evaluate(log=log)), flush=True)
+        return
+    if args.only_prescreen:
+        from protlocus_amd import prescreen
+        ps = prescreen.evaluate(n_train=32768, n_val=8192, n_eval=args.only_prescreen, epochs=20, seed=11, log=log)
+        ps["workload"] = ("%d proteins x (true locus + decoy window), protein 50-1000 aa; trained on 32,768 other "
+                          "proteins' candidates; DP = plain global spliced DP over the window" % args.only_prescreen)
+        print(json.dumps(ps), flush=True)
         return
 
     import nsffi

@@ -593,6 +593,12 @@ int64_t nsh_write_output(char *out, int64_t size, const nsh_aln_t *a, const nsh_
 int nsh_device_synchronize(void);
 const char *nsh_build_info(void);
 
+// Measurement aid, no counterpart in the reference: reads `bytes` (a multiple of 64 KiB) of device
+// memory at src through the scalar data cache, one wave per 64 KiB, and writes one folded word per
+// wave to out (device memory, bytes / 65536 words); synchronous. A read of known size on the path
+// the extension DP's row records take, for scaling hardware counter readings. 1 on success.
+int nsh_diag_scalar_read(const void *src, int64_t bytes, uint32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

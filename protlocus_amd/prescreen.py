@@ -28,13 +28,15 @@ def accept(scores, al):
     return scores >= ACCEPT_SLOPE * np.asarray(al, dtype=np.int64) + ACCEPT_OFFSET
 
 
-def candidates(n_prot, seed, min_len=50, max_len=1000):
+def candidates(n_prot, seed, min_len=50, max_len=1000, log=None):
     """two candidates per protein: its true locus (10-45% substitutions, 0-3 introns, 0-2
     frameshifts, 200-nt flanks) and a decoy window of the same length: 70% random genomic DNA
     (GC 0.45), 30% a distant homolog (55-80% substitutions) that scores like random k-mer hits"""
     rng = np.random.default_rng(seed)
     prots, dnas, kind = [], [], []
-    for _ in range(n_prot):
+    for k in range(n_prot):
+        if log and k and k % 65536 == 0:
+            log("[prescreen] %d of %d proteins' candidates made" % (k, n_prot))
         L = int(round(np.exp(rng.uniform(np.log(min_len), np.log(max_len)))))
         prot = synth.rand_protein(rng, L)
         ni, nfs = int(rng.integers(0, 4)), int(rng.integers(0, 3))
@@ -162,7 +164,9 @@ def evaluate(n_train=16384, n_val=4096, n_eval=16384, seed=11, epochs=12, pos_we
         if kf < best_keep:
             best_f, best_keep, best_th = f, kf, th
     va.close()
-    ev = Batch(*candidates(n_eval, seed + 2)[:2])
+    ev = Batch(*candidates(n_eval, seed + 2, log=log)[:2])
+    if log:
+        log("[prescreen] evaluation set: %d candidates on the device" % len(ev.prots))
     n = len(ev.prots)
     rng = np.random.default_rng(seed + 3)
     # plain pipeline: the DP of every candidate (mean of 3 timed runs after a warm-up)
