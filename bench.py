@@ -49,11 +49,24 @@ MFMA_BF16_TFLOPS = 2500.0                         # dense BF16 MFMA
 OPS_PER_CELL = 32                                 # extension DP: 31 recurrence ops + row max
 HMM_OPS_PER_CELL = 17                             # Viterbi: 8 adds, 4 max, 3 clamps, 2 running-best max
 MLP_FLOP_PER_SAMPLE = 2 * (64 * 256 + 256 * 256 + 256)
-# HBM bytes moved by one DP phase (both column-group kernels) of the default workload (65,536
-# pairs, seed 1): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of one step, each scaled by what the
-# counter reads for a 1-GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); the summary and the
-# commands are in profiles/ext_dp_kernel_65536pairs.md
-EXT_DP_TRAFFIC_BYTES = 1277.4e6
+# HBM bytes of one DP phase (both column-group kernels) of the default workload (65,536 pairs,
+# seed 1), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of one step (KiB as read, x 1024). The
+# counters are scaled per access path by what they read for reads of known size: FETCH_SIZE reads
+# 1.0 x a 1-GiB read through the scalar data cache (nsh_diag_scalar_read, the path of the row
+# records) and 0.5 x a 1-GiB runtime device copy (the vector path); WRITE_SIZE reads 1.0 x. The
+# scalar-path bytes are the row records the DP reads (4 bytes per evaluated row, the rows read
+# ahead, a half 64-byte line per problem); the rest of FETCH_SIZE is taken as vector-path reads.
+# The summary and the commands are in profiles/ext_dp_kernel_65536pairs.md
+EXT_DP_FETCH_RAW_BYTES = 585703.8 * 1024
+EXT_DP_WRITE_BYTES = 74018.2 * 1024
+FETCH_SCALE_SCALAR, FETCH_SCALE_VECTOR = 1.0, 2.0
+
+
+def ext_dp_traffic(rows_done, n_problems):
+    """HBM bytes per DP phase of the default workload (see the constants above)"""
+    scalar = min(EXT_DP_FETCH_RAW_BYTES, 4.0 * (rows_done + 4 * n_problems) + 32.0 * n_problems)
+    return (FETCH_SCALE_SCALAR * scalar + FETCH_SCALE_VECTOR * (EXT_DP_FETCH_RAW_BYTES - scalar) +
+            EXT_DP_WRITE_BYTES)
 
 
 def threads():
@@ -957,6 +970,7 @@ def main():
     gpu_res = plan.results()
     cells = plan.cells(1)          # evaluated cells of the last run
     win = plan.cells(0)
+    rows_done = plan.cells(2)      # evaluated rows (the row records the DP reads)
     # the same step replayed as one captured hipGraph (preparation, both group kernels): its wall
     # clock per step and whether every replay's results equal the stream-launched run's
     plan.run(graph=True)           # capture (untimed)
@@ -999,10 +1013,11 @@ def main():
                            "gcups": round(cells / ((ms + plan_build_s * 1e3) * 1e-3) / 1e9, 2)},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
                      "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
-                     "traffic": EXT_DP_TRAFFIC_BYTES if (args.pairs, args.seed) == (65536, 1) else None,
+                     "traffic": round(ext_dp_traffic(rows_done, args.pairs), -5)
+                     if (args.pairs, args.seed) == (65536, 1) else None,
                      "ops_per_cell": OPS_PER_CELL},
     }
-    log("[bench] gpu: %.2f GCUPS, %.1f ms/step, dp %.1f ms" % (gcups, ms, dp_avg))
+    log("[bench] gpu: %.2f GCUPS, %.1f ms/step, dp %.1f ms; %.0f rows evaluated" % (gcups, ms, dp_avg, rows_done))
     ref_res = None
     if not args.no_cpu:
         cb, ref_res = cpu_baseline(pairs, kw, seconds=args.cpu_seconds, packed=(ns, ns_off, nl, as_, as_off, al))

@@ -245,7 +245,7 @@ int nsh_ext_plan_run_subset(nsh_ext_plan_t *p, const uint8_t *keep);
  * pointer to n x 64 floats owned by the plan, or NULL (protein > 4096 aa or DNA > 24000 nt) */
 const float *nsh_ext_plan_features(nsh_ext_plan_t *p, float *host_out);
 int nsh_ext_plan_results(nsh_ext_plan_t *p, nsh_ext_result_t *out); /* copy results (input order) */
-double nsh_ext_plan_cells(const nsh_ext_plan_t *p, int32_t kind);   /* 0: sum nl*al, 1: sum (rows done)*al after a run */
+double nsh_ext_plan_cells(const nsh_ext_plan_t *p, int32_t kind);   /* 0: sum nl*al, 1: sum (rows done)*al after a run, 2: sum (rows done) */
 float nsh_ext_plan_last_ms(nsh_ext_plan_t *p, int32_t which);  /* 0: whole run, 1: DP kernels only (HIP events) */
 /* Autotuner: picks each column-count group's persistent wave count (waves per CU, one of the
  * compiled default's 1/2, 3/4, 1, 3/2 and 2 multiples), then the wavefronts per problem: whether

@@ -861,7 +861,7 @@ extern "C" double nsh_ext_plan_cells(const nsh_ext_plan_t *p, int32_t kind)
 	if (!p) return 0;
 	if (kind == 0) return p->cells_window;
 	double s = 0;
-	for (int32_t k = 0; k < p->n; ++k) s += (double)p->out_h[4 * k + 3] * p->al[k];
+	for (int32_t k = 0; k < p->n; ++k) s += (double)p->out_h[4 * k + 3] * (kind == 2? 1 : p->al[k]);
 	return s;
 }
 
