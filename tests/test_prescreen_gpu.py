@@ -98,6 +98,33 @@ def test_mlp_inference_matches_oracle(gpu):
     assert np.all(np.abs(got - exp) <= 2e-3 * (1 + np.abs(exp))), np.max(np.abs(got - exp))
 
 
+def test_mlp_inference_many_blocks_standardised(gpu):
+    # 40,001 samples: each work-group of the persistent inference kernel takes several 64-sample
+    # blocks in turn, the last one ragged; the device standardises the raw features. Against the
+    # oracle on the standardised inputs, and bit for bit against the same rows inferred alone
+    rng = np.random.default_rng(5)
+    p = prescreen.init_params(rng)
+    p[64 * 256:64 * 256 + 256] = rng.normal(0, 0.1, 256)
+    p[-257 - 256:-257] = rng.normal(0, 0.1, 256)
+    X = rng.normal(3, 2, size=(40001, 64)).astype(np.float32)
+    mu = rng.normal(3, 0.5, 64).astype(np.float32)
+    sd = rng.uniform(1, 3, 64).astype(np.float32)
+    net = gpu.MLP(p, mu, sd)
+    try:
+        got = net.infer(X)
+        head = net.infer(X[:1000])
+        tail = net.infer(X[-777:])
+    finally:
+        net.close()
+    Xn = (X - mu) * (np.float32(1) / sd)          # float32, as the device computes it
+    exp, _ = mlp_ref.forward(p, Xn)
+    # FP32 MFMA sums and the oracle's float64 sums can round a hidden activation to neighbouring
+    # BF16 values; over 40,001 samples a few such flips move a logit by a few 1e-3
+    err = np.abs(got - exp) / (1 + np.abs(exp))
+    assert np.mean(err > 2e-3) <= 1e-3 and err.max() <= 2e-2, (int(np.sum(err > 2e-3)), err.max())
+    assert np.array_equal(got[:1000], head) and np.array_equal(got[-777:], tail)
+
+
 def test_mlp_gradients_match_oracle(gpu):
     rng = np.random.default_rng(5)
     p = prescreen.init_params(rng)
