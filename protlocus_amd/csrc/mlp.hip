@@ -79,11 +79,32 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g)
 	const int m0 = blockIdx.x * 32, n0 = (blockIdx.y * 4 + w) * 32;
 	if (n0 >= g.N) return;   // whole wave; no barrier in this kernel
 	const int r = L & 31, kh = (L >> 5) * 8;
+	// an operand with unit K stride, 16-byte aligned rows and a row stride of whole 16-byte units
+	// is read with one vector load per fragment on every lane (a wave-uniform choice)
+	const bool va = g.sak == 1 && !((uintptr_t)g.A & 15) && !(g.sam & 7);
+	const bool vb = g.sbk == 1 && !((uintptr_t)g.B & 15) && !(g.sbn & 7);
+	auto frag = [&](const u16 *p, long sr, long sk, bool vec, int row, int k) -> bf16x8 {
+		if (vec) return *(const bf16x8*)(p + (long)row * sr + k);
+		V8 x;
+#pragma unroll
+		for (int e = 0; e < 8; ++e) x.s[e] = p[(long)row * sr + (long)(k + e) * sk];
+		return x.v;
+	};
+	// fragments of four K steps are loaded ahead of their products
+	constexpr int D = 4;
+	bf16x8 fa[D], fb[D];
+#pragma unroll
+	for (int d = 0; d < D; ++d)
+		if (d * 16 < g.K) fa[d] = frag(g.A, g.sam, g.sak, va, m0 + r, d * 16 + kh), fb[d] = frag(g.B, g.sbn, g.sbk, vb, n0 + r, d * 16 + kh);
 	f32x16 acc = {};
-	for (int k0 = 0; k0 < g.K; k0 += 16) {
-		const bf16x8 a = load8(g.A, g.sam, g.sak, m0 + r, k0 + kh);
-		const bf16x8 b = load8(g.B, g.sbn, g.sbk, n0 + r, k0 + kh);
-		acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+	for (int k0 = 0; k0 < g.K; k0 += 16 * D) {
+#pragma unroll
+		for (int d = 0; d < D; ++d) {
+			if (k0 + d * 16 >= g.K) break;
+			acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[d], fb[d], acc, 0, 0, 0);
+			const int kn = k0 + (D + d) * 16;
+			if (kn < g.K) fa[d] = frag(g.A, g.sam, g.sak, va, m0 + r, kn + kh), fb[d] = frag(g.B, g.sbn, g.sbk, vb, n0 + r, kn + kh);
+		}
 	}
 	const int col = n0 + r;
 #pragma unroll
