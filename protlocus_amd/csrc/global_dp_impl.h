@@ -454,13 +454,20 @@ DEV void glob_solve(const GlobArgs &a, const GlobProb &P, int8_t *prof, Xch &X)
 }
 
 // one work-group per problem. MAXT: the work-group size bound (512 keeps the column counts up to
-// 4 free of spills; 1024 serves proteins longer than 2048, up to 8 columns per lane).
-template <class AR, bool TRACE, bool EXT, int MAXT>
-__global__ void __launch_bounds__(MAXT) glob_kernel(GlobArgs a)
+// 4 free of spills; 1024 serves proteins longer than 2048, up to 8 columns per lane). CK > 0: a
+// kernel for one-wave problems of exactly CK columns per lane only, whose registers are sized for
+// that case alone (the general kernel's 178 VGPRs allow two waves per SIMD, and one-wave problems
+// of a few hundred cells are latency-bound)
+template <class AR, bool TRACE, bool EXT, int MAXT, int CK = 0>
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(CK == 1? 7 : CK == 2? 5 : 1))) glob_kernel(GlobArgs a)
 {
 	extern __shared__ int8_t prof[];
 	__shared__ Xch X;
 	const GlobProb P = a.prob[blockIdx.x];
+	if constexpr (CK > 0) {
+		glob_solve<AR, CK, TRACE, EXT>(a, P, prof, X);
+		return;
+	}
 	switch (P.C) {
 	case 1: glob_solve<AR, 1, TRACE, EXT>(a, P, prof, X); break;
 	case 2: glob_solve<AR, 2, TRACE, EXT>(a, P, prof, X); break;
@@ -484,7 +491,9 @@ template <class AR, bool TRACE, bool EXT>
 void launch_bucket(const GlobArgs &a, int n, int NW, int C, hipStream_t s)
 {
 	const size_t lds = C <= 4? (size_t)22 * 64 * C * NW : 512;   // the profile, or the matrix (C > 4)
-	if (NW <= 8 && C <= 4) hipLaunchKernelGGL((glob_kernel<AR, TRACE, EXT, 512>), dim3(n), dim3(64 * NW), lds, s, a);
+	if (NW == 1 && C == 1) hipLaunchKernelGGL((glob_kernel<AR, TRACE, EXT, 64, 1>), dim3(n), dim3(64), lds, s, a);
+	else if (NW == 1 && C == 2) hipLaunchKernelGGL((glob_kernel<AR, TRACE, EXT, 64, 2>), dim3(n), dim3(64), lds, s, a);
+	else if (NW <= 8 && C <= 4) hipLaunchKernelGGL((glob_kernel<AR, TRACE, EXT, 512>), dim3(n), dim3(64 * NW), lds, s, a);
 	else hipLaunchKernelGGL((glob_kernel<AR, TRACE, EXT, 1024>), dim3(n), dim3(64 * NW), lds, s, a);
 }
 
