@@ -604,7 +604,7 @@ DEV void ext_solve2(const ExtArgs &a, const ExtProb &PA, const ExtProb &PB, int8
 // target trades registers for latency hiding (measured: 6 / 3 / 1 for C <= 4 / <= 8 / <= 16; the
 // merged queue of C 1..8 takes the C <= 8 budget).
 template <int CLO, int CHI, bool PROF>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CHI <= 4? NSH_EXT_WPE0 : CHI <= 8? NSH_EXT_WPE1 : 1)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CHI <= 4? NSH_EXT_WPE0 : CLO == 1 && CHI == 8? NSH_EXT_WPE01 : CHI <= 8? NSH_EXT_WPE1 : 1)))
 ext_dp_kernel(ExtArgs a)
 {
 	__shared__ __attribute__((aligned(16))) int8_t prof[PROF? NRES * 128 * CHI : NRES * TAB_W * 2];

@@ -41,15 +41,19 @@ constexpr int kGroupLo[kGroups] = { 1, 5, 9 }, kGroupHi[kGroups] = { 4, 8, 16 };
 #ifndef NSH_EXT_WPE1
 #define NSH_EXT_WPE1 3
 #endif
+#ifndef NSH_EXT_WPE01
+#define NSH_EXT_WPE01 4
+#endif
 constexpr int kGroupWavesPerCU[kGroups] = { 4 * NSH_EXT_WPE0, 4 * NSH_EXT_WPE1, 3 };
 constexpr int kGroupWavesPerCUPssm[kGroups] = { 12, 7, 3 };
 // A long problem: rows x 128-column chunks at which one problem runs for a large part of a plan.
 constexpr int64_t kLongWork = (int64_t)1 << 18;
 // Merged queue: a matrix-scored plan that holds a long problem runs its C <= 8 problems as one
-// queue, longest first, in one kernel (1..8 columns per lane, 3 waves per SIMD). The groups'
+// queue, longest first, in one kernel (1..8 columns per lane, NSH_EXT_WPE01 waves per SIMD: on the
+// map path's plan of 39,510 extensions, 4 ran it in 134-144 ms, 3 in 158 and 5 in 149). The groups'
 // kernels run on separate streams, and two streams that share a hardware queue run one after the
 // other, so a long problem in the second would start only when the first has drained.
-constexpr int kMergedWavesPerCU = 12;
+constexpr int kMergedWavesPerCU = 4 * NSH_EXT_WPE01;
 void launch_group01(const ExtArgs &a, int grid, hipStream_t s);
 constexpr int kCmax = 16;
 inline int group_of(int C) { return C <= 4? 0 : C <= 8? 1 : 2; }
