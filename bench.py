@@ -945,6 +945,10 @@ def main():
     if not prod.lib.ns_gpu_init(0):
         raise SystemExit("GPU not usable: " + prod.error())
     kw = dict(flag=nsffi.NS_F_EXT_RIGHT)
+    # a 64-pair plan first: the process's first plan creates the HIP streams and hardware queues
+    # that later plans reuse, a one-time cost kept out of the timed build below
+    w = synth.pack_batch(pairs[:64])
+    prod.ext_plan(*w, prod.opt(**kw)).close()
     ns, ns_off, nl, as_, as_off, al = synth.pack_batch(pairs)
     t_up = time.perf_counter()
     plan = prod.ext_plan(ns, ns_off, nl, as_, as_off, al, prod.opt(**kw))   # host buffers -> HBM (PCIe), once

@@ -484,6 +484,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 	if (opt->ge < 0 || opt->go < 0 || (int64_t)opt->ge * CMAX * 64 > 32767) {
 		nsh_set_error("gap penalties go=%d ge=%d out of the supported range", opt->go, opt->ge); return nullptr;
 	}
+	nsh_step_clock clk("nsh_ext_plan_build");
 	int64_t l_ns = 0, l_raw = 0, l_as = 0, l_pssm = 0;
 	const uint8_t *genome = nullptr;
 	int32_t max_nl = 2, any_ss = 0, any_long = 0, n_grp[kGroups] = {};
@@ -541,7 +542,9 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 	for (int32_t k = 0; k < n; ++k) if (!is_wide[k]) p->prob_in.push_back(prob[k]);
 	p->pssm_plan = l_pssm > 0;
 	p->merged = !p->pssm_plan && any_long && n_grp[0] > 0 && n_grp[1] > 0;
+	clk.lap("  problems", n);
 	plan_order(p, p->pair_rows, n_grp);
+	clk.lap("  queue order", n);
 	const int32_t n_narrow = (int32_t)p->prob_h.size();
 	p->out_h.assign((size_t)4 * n, 0);
 	// inputs staged in pinned memory: ns | as | ss | pssm | pen | prob | prep | pc | mat | wide
@@ -564,16 +567,20 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		p->h_stage_cap = want;
 	}
 	hs = (uint8_t*)p->h_stage;
-	for (int32_t k = 0; k < n; ++k) {
-		const nsh_ext_req &q = req[k];
-		if (q.nl && !prep[k].from_genome) std::memcpy(hs + s_ns + prep[k].ns_off, q.ns, q.nl);
-		std::memcpy(hs + s_as + prep[k].as_off, q.as, q.al);
-		if (any_ss && !prep[k].from_genome) {
-			if (q.ss) std::memcpy(hs + s_ss + prep[k].ns_off, q.ss, q.nl);
-			else std::memset(hs + s_ss + prep[k].ns_off, 0xff, q.nl);
+	clk.lap("  pinned staging", (int64_t)s_end);
+	// the strings into the pinned block on the host threads, 1,024 problems at a time
+	nsh_parallel_for(((int64_t)n + 1023) / 1024, n >= 4096? 16 : 1, [&](int64_t b) {
+		for (int32_t k = (int32_t)(b * 1024); k < std::min<int64_t>(n, (b + 1) * 1024); ++k) {
+			const nsh_ext_req &q = req[k];
+			if (q.nl && !prep[k].from_genome) std::memcpy(hs + s_ns + prep[k].ns_off, q.ns, q.nl);
+			std::memcpy(hs + s_as + prep[k].as_off, q.as, q.al);
+			if (any_ss && !prep[k].from_genome) {
+				if (q.ss) std::memcpy(hs + s_ss + prep[k].ns_off, q.ss, q.nl);
+				else std::memset(hs + s_ss + prep[k].ns_off, 0xff, q.nl);
+			}
+			if (q.pssm) std::memcpy(hs + s_pssm + prob[k].pssm_off, q.pssm, (size_t)q.al * asize);
 		}
-		if (q.pssm) std::memcpy(hs + s_pssm + prob[k].pssm_off, q.pssm, (size_t)q.al * asize);
-	}
+	});
 	{
 		int32_t *pen = (int32_t*)(hs + s_pen);
 		for (int32_t d = 0; d < n_pen; ++d) pen[d] = nsh_ie_penalty(opt->ie_coef, d);
@@ -587,6 +594,7 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 		std::memcpy(hs + s_mat, opt->sc, (size_t)asize * asize);
 		if (!p->wide_h.empty()) std::memcpy(hs + s_wide, p->wide_h.data(), p->wide_h.size() * sizeof(nsh_glob::GlobProb));
 	}
+	clk.lap("  pack", n);
 	HIPOK(p->b_ns.ensure(std::max<int64_t>(l_raw, 1))); HIPOK(p->b_as.ensure(l_as)); HIPOK(p->b_aas.ensure(l_as));
 	HIPOK(p->b_rec.ensure(l_ns));
 	if (any_ss) HIPOK(p->b_ss.ensure(std::max<int64_t>(l_raw, 1)));
@@ -609,6 +617,10 @@ nsh_ext_plan_t *nsh_ext_plan_build(int32_t n, const nsh_ext_req *req, const ns_o
 			cp(p->d_pc, s_pc, sizeof(PrepConst)) && cp(p->d_mat, s_mat, (size_t)asize * asize) &&
 			(p->wide_h.empty() || cp(p->d_wide, s_wide, p->wide_h.size() * sizeof(nsh_glob::GlobProb)));
 		if (!ok) { nsh_set_error("extension plan: copy to the device failed"); goto fail; }
+	}
+	if (clk.on) {
+		(void)hipStreamSynchronize(p->stream);
+		clk.lap("  device buffers, upload", (int64_t)s_end);
 	}
 	{
 		int dev = 0, ncu = 0;
