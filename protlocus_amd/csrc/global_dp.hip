@@ -283,10 +283,16 @@ int run_list(RunCtx &c, const GlobIn *P, int32_t n, const GlobTerms &T, int32_t 
 		++kstart[key[k] + 1];
 	}
 	for (int t = 0; t < kKeys; ++t) kstart[t + 1] += kstart[t];
-	{
-		int32_t next[kKeys];
+	{   // inside a bucket, problems of kLongRowsG rows or more first, longest first (they set the
+		// bucket's end), the others in input order
+		int32_t next[kKeys], nlong[kKeys + 1] = {};
+		std::vector<int32_t> longs;
+		for (int32_t k = 0; k < n; ++k) if (P[k].nl >= kLongRowsG) ++nlong[key[k]], longs.push_back(k);
+		std::stable_sort(longs.begin(), longs.end(), [&](int32_t x, int32_t y) { return P[x].nl > P[y].nl; });
 		std::memcpy(next, kstart, sizeof(next));
-		for (int32_t k = 0; k < n; ++k) pos[k] = next[key[k]]++;
+		for (int32_t k : longs) pos[k] = next[key[k]]++;
+		for (int t = 0; t < kKeys; ++t) next[t] = kstart[t] + nlong[t];
+		for (int32_t k = 0; k < n; ++k) if (P[k].nl < kLongRowsG) pos[k] = next[key[k]]++;
 	}
 	// a bucket's kernel variant, waves and columns per lane from its key
 	auto key_v = [&](int t) { return t / ((kNWMax + 1) * (kCMaxG + 1)); };

@@ -464,6 +464,9 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(CK ==
 	extern __shared__ int8_t prof[];
 	__shared__ Xch X;
 	const GlobProb P = a.prob[blockIdx.x];
+	// a problem of many rows sets the end of its bucket's launch (its rows are dependent): it takes
+	// the issue priority over the short problems sharing its SIMD, which fill the cycles it leaves
+	if (P.nl >= kLongRowsG) __builtin_amdgcn_s_setprio(2);
 	if constexpr (CK > 0) {
 		glob_solve<AR, CK, TRACE, EXT>(a, P, prof, X);
 		return;

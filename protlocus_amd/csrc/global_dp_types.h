@@ -8,6 +8,7 @@
 
 namespace nsh_glob {
 constexpr int kCMaxG = 8, kNWMax = 16;   // up to 8 columns per lane, 16 waves: 8192 columns
+constexpr int kLongRowsG = 2048;         // rows from which a problem starts first in its bucket, at issue priority
 
 struct GlobProb {
 	int64_t rec_off, aa_off, tr_off, pssm_off;   // pssm_off < 0: matrix scores
