@@ -80,37 +80,6 @@ void radix_sort128(P128 *beg, P128 *end)
 	else rs_sort(beg, end, 8, 56);
 }
 
-// sketch.c:7-16 (Wang's integer hash restricted to mask); the index build hashes with the same
-inline uint32_t hash32_mask(uint32_t key, uint32_t mask)
-{
-	key = (key + ~(key << 15)) & mask;
-	key ^= key >> 10;
-	key = (key + (key << 3)) & mask;
-	key ^= key >> 6;
-	key = (key + ~(key << 11)) & mask;
-	key ^= key >> 16;
-	return key;
-}
-
-// mp_sketch_prot (sketch.c:18-38) with 4-bit residues: hash << 32 | position of the k-mer's last residue
-void sketch_prot(const char *seq, int32_t len, int32_t kmer, int32_t mod_bit, std::vector<uint64_t> &out)
-{
-	const uint32_t mask_k = (1U << kmer * 4) - 1, mask_mod = (1U << mod_bit) - 1;
-	uint32_t x = 0;
-	int32_t l = 0;
-	out.clear();
-	for (int32_t i = 0; i < len; ++i) {
-		const int32_t c = ns_tab_aa13[(uint8_t)seq[i]];
-		if (c < 14) {
-			x = (x << 4 | (uint32_t)c) & mask_k;
-			if (++l >= kmer) {
-				const uint32_t y = hash32_mask(x, mask_k);
-				if ((y & mask_mod) == 0) out.push_back((uint64_t)(y >> mod_bit) << 32 | (uint32_t)i);
-			}
-		} else x = 0, l = 0;
-	}
-}
-
 // one region of a query (the fields of mp_reg1_t the map path reads)
 struct Reg {
 	int32_t id = 0, parent = 0;
@@ -577,7 +546,6 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 	hclk.lap("  (region selection done)");
 	struct RefJob { int32_t q, r; int64_t as; };
 	std::vector<RefJob> jobs;
-	std::vector<std::vector<uint64_t>> qsk(n_q);
 	std::vector<std::vector<uint64_t>> ext0(n_q);
 	nsh_parallel_for(n_q, nth, [&](int64_t q) {
 		ext0[q] = cal_max_ext(pool[q], regs[q], false, nullptr, nullptr, ca + c_off[q], 100, o->dp.max_ext,
@@ -587,11 +555,6 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 		for (int32_t i = 0; i < (int32_t)regs[q].size(); ++i) jobs.push_back({ q, i, 0 });
 	st[1] = (int64_t)jobs.size();
 	hclk.lap("  extension limits", (int64_t)jobs.size());
-	nsh_parallel_for(n_q, nth, [&](int64_t q) {
-		sketch_prot(seq[q], qlen[q], kmer2, 0, qsk[q]);
-		std::sort(qsk[q].begin(), qsk[q].end());
-	});
-	hclk.lap("  query sketches", n_q);
 	// the windows (mp_refine_reg, map.c:152-156): slices of the oriented genome in HBM
 	std::vector<nsh_refine::Job> rjobs(jobs.size());
 	nsh_parallel_for(((int64_t)jobs.size() + 4095) / 4096, nth, [&](int64_t b) {
@@ -607,18 +570,15 @@ extern "C" int nsh_map_batch(nsh_mapper_t *m, int32_t n_q, const char *const *qn
 		rjobs[t].q = jb.q;
 	}
 	});
-	std::vector<int64_t> qoff(n_q + 1, 0);
-	for (int32_t q = 0; q < n_q; ++q) qoff[q + 1] = qoff[q] + (int64_t)qsk[q].size();
-	std::unique_ptr<uint64_t[]> qall_buf(new uint64_t[(size_t)qoff[n_q] + 1]);
-	nsh_parallel_for(n_q, nth, [&](int64_t q) { std::copy(qsk[q].begin(), qsk[q].end(), qall_buf.get() + qoff[q]); });
-	hclk.lap("  windows, sketches packed", (int64_t)qoff[n_q]);
+	hclk.lap("  windows", (int64_t)jobs.size());
 	lap(2);
 	// window k-mers shared with the query, bucket pairs and the window chains of every region (map.c:153-203)
 	std::vector<uint64_t> best_u, best_a;
 	std::vector<int64_t> best_off;
 	const nsh_chain::Params rcp = { o->dp.max_intron, o->max_gap, o->bw, o->max_chn_max_skip, o->max_chn_iter,
 		o->min_chn_cnt, o->min_chn_sc, o->chn_coef_log, 1, kmer2, 0 };
-	if (!nsh_refine::run(m->d_g, rjobs, qoff, qall_buf.get(), qoff[n_q], kmer2, m->min_aa_len, o->max_ava, rcp, best_u, best_off, best_a,
+	if (!nsh_refine::run(m->d_g, rjobs, qs_all.data(), q_off.data(), qlen, n_q, (int64_t)qs_all.size(), kmer2,
+			m->min_aa_len, o->max_ava, rcp, best_u, best_off, best_a,
 			&st[2]))
 		return 0;
 	hclk.lap("  refine on the device", (int64_t)jobs.size());

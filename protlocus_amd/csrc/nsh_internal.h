@@ -172,13 +172,14 @@ struct Job { int64_t g_off; int32_t len, q; };   // a window: its first base in 
 // followed by their reverse complements at total + ctg_off[c]; *d_g is hipFree'd by the caller
 int oriented_genome(const uint8_t *nt4, const int64_t *ctg_off, const int64_t *ctg_len, int32_t n_ctg, int64_t total,
 		uint8_t **d_g);
-// mp_refine_reg's sketch, bucket pairs, mp_chain and chain choice (map.c:153-247) for every job: qsk
-// holds each query's sorted k-mers (hash << 32 | position, mp_sketch_prot with mod_bit 0) at qoff[q].
+// mp_refine_reg's sketch, bucket pairs, mp_chain and chain choice (map.c:153-247) for every job. The
+// n_q queries' residues are qs[q_off[q] .. q_off[q] + qlen[q]) (n_res bytes in all); their k-mers
+// (mp_sketch_prot with kmer and mod_bit 0, sorted: hash << 32 | position) are made on the device.
 // Job j's chosen chain is best_u[j] (score << 32 | count; 0 when the job has no chain) and its anchors
 // (window position << 32 | query position) are anchors[best_off[j] ..). *n_pairs: the anchor pairs
 // chained. 1 on success.
-int run(const uint8_t *d_g, const std::vector<Job> &jobs, const std::vector<int64_t> &qoff,
-		const uint64_t *qsk, int64_t n_qsk, int32_t kmer, int32_t min_aa_len, int32_t max_ava, const nsh_chain::Params &cp,
+int run(const uint8_t *d_g, const std::vector<Job> &jobs, const char *qs, const int64_t *q_off, const int32_t *qlen,
+		int32_t n_q, int64_t n_res, int32_t kmer, int32_t min_aa_len, int32_t max_ava, const nsh_chain::Params &cp,
 		std::vector<uint64_t> &best_u, std::vector<int64_t> &best_off, std::vector<uint64_t> &anchors, int64_t *n_pairs);
 }
 

@@ -39,6 +39,50 @@ __device__ inline uint32_t hash32_mask(uint32_t key, uint32_t mask)   // sketch.
 	return key;
 }
 
+// The query k-mers of the refinement (mp_sketch_prot, sketch.c:18-38: every position that ends
+// kmer residues of the 13-letter alphabet, (hash >> mod_bit) << 32 | position when the hash's low
+// mod_bit bits are zero), one work-group per query, in position order: with out == nullptr it
+// counts them into cnt[q], else it writes them from out[out_off[q]]
+__global__ void __launch_bounds__(256) qsketch_kernel(const uint8_t *aa, const int64_t *q_off, const int32_t *qlen,
+		const uint8_t *aa13, int32_t kmer, int32_t mod_bit, const int64_t *out_off, int64_t *cnt, uint64_t *out)
+{
+	__shared__ int32_t wsum[4];
+	const int q = blockIdx.x, t = threadIdx.x, L = t & 63, w = t >> 6;
+	const uint8_t *sq = aa + q_off[q];
+	const int32_t len = qlen[q];
+	const uint32_t mask_k = (1u << kmer * 4) - 1, mask_mod = (1u << mod_bit) - 1;
+	const int64_t base = out? out_off[q] : 0;
+	int64_t total = 0;
+	for (int32_t i0 = 0; i0 < len; i0 += 256) {
+		const int32_t i = i0 + t;
+		bool v = false;
+		uint64_t e = 0;
+		if (i < len && i >= kmer - 1) {
+			uint32_t x = 0;
+			v = true;
+			for (int j = kmer - 1; j >= 0; --j) {   // residues i-kmer+1 .. i, the oldest in the high bits
+				const uint32_t c = aa13[sq[i - j]];
+				if (c >= 14) { v = false; break; }
+				x = x << 4 | c;
+			}
+			if (v) {
+				const uint32_t y = hash32_mask(x & mask_k, mask_k);
+				v = (y & mask_mod) == 0;
+				e = (uint64_t)(y >> mod_bit) << 32 | (uint32_t)i;
+			}
+		}
+		const unsigned long long b = __ballot(v);
+		if (L == 0) wsum[w] = __popcll(b);
+		__syncthreads();
+		int64_t before = total;
+		for (int u = 0; u < w; ++u) before += wsum[u];
+		if (out && v) out[base + before + __popcll(b & ((1ull << L) - 1))] = e;
+		total += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+		__syncthreads();
+	}
+	if (!out && t == 0) cnt[q] = total;
+}
+
 // the sense codon ending at window position e (its three bases valid and not a stop)
 __device__ inline bool sense_at(const uint8_t *w, int64_t e, const uint8_t *codon)
 {
@@ -244,9 +288,10 @@ done:
 	return ok;
 }
 
-int nsh_refine::run(const uint8_t *d_g, const std::vector<Job> &jobs, const std::vector<int64_t> &qoff,
-		const uint64_t *qsk, int64_t n_qsk, int32_t kmer, int32_t min_aa_len, int32_t max_ava, const nsh_chain::Params &cp,
-		std::vector<uint64_t> &best_u, std::vector<int64_t> &best_off, std::vector<uint64_t> &anchors, int64_t *n_pairs)
+int nsh_refine::run(const uint8_t *d_g, const std::vector<Job> &jobs, const char *qs, const int64_t *q_off,
+		const int32_t *qlen, int32_t n_q, int64_t n_res, int32_t kmer, int32_t min_aa_len, int32_t max_ava,
+		const nsh_chain::Params &cp, std::vector<uint64_t> &best_u, std::vector<int64_t> &best_off,
+		std::vector<uint64_t> &anchors, int64_t *n_pairs)
 {
 	const int32_t n = (int32_t)jobs.size();
 	std::vector<int64_t> p_off((size_t)n + 1, 0);
@@ -273,6 +318,12 @@ int nsh_refine::run(const uint8_t *d_g, const std::vector<Job> &jobs, const std:
 	RConst *d_rc = nullptr;
 	std::vector<int64_t> cnt(n), cap(n), cap_off(n), wend(n), pc(n);
 	int64_t *d_cap = nullptr, *d_capoff = nullptr, *d_wend = nullptr;
+	uint8_t *d_qs = nullptr, *d_aa13 = nullptr;
+	int64_t *d_qoffaa = nullptr, *d_qcnt = nullptr;
+	int32_t *d_qlen = nullptr;
+	uint64_t *d_qsk_raw = nullptr;
+	std::vector<int64_t> qoff((size_t)n_q + 1, 0);
+	int64_t n_qsk = 0;
 	for (int t = 0; t < 64; ++t) rc.codon[t] = ns_tab_codon[t], rc.codon13[t] = ns_tab_codon13[t];
 	// a segmented sort of the keys in [beg[j], end[j]) of each segment (others untouched)
 	auto sort_segments = [&](const uint64_t *in, uint64_t *outk, int64_t total, const int64_t *d_beg,
@@ -292,9 +343,47 @@ int nsh_refine::run(const uint8_t *d_g, const std::vector<Job> &jobs, const std:
 	};
 	{
 		RFOK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+		// the query k-mers on the device: counts, offsets, the k-mers in position order, then each
+		// query's k-mers sorted (the order of the reference's per-query sort: hash, then position)
+		RFOK(hipMalloc((void**)&d_qs, (size_t)n_res + 8));
+		RFOK(hipMalloc((void**)&d_qoffaa, (size_t)n_q * 8 + 8));
+		RFOK(hipMalloc((void**)&d_qlen, (size_t)n_q * 4 + 4));
+		RFOK(hipMalloc((void**)&d_qcnt, (size_t)n_q * 8 + 8));
+		RFOK(hipMalloc((void**)&d_aa13, 256));
+		if (n_res > 0) RFOK(hipMemcpyAsync(d_qs, qs, (size_t)n_res, hipMemcpyHostToDevice, s));
+		if (n_q > 0) {
+			RFOK(hipMemcpyAsync(d_qoffaa, q_off, (size_t)n_q * 8, hipMemcpyHostToDevice, s));
+			RFOK(hipMemcpyAsync(d_qlen, qlen, (size_t)n_q * 4, hipMemcpyHostToDevice, s));
+		}
+		RFOK(hipMemcpyAsync(d_aa13, ns_tab_aa13, 256, hipMemcpyHostToDevice, s));
+		if (n_q > 0) {
+			hipLaunchKernelGGL(qsketch_kernel, dim3(n_q), dim3(256), 0, s, d_qs, d_qoffaa, d_qlen, d_aa13, kmer, 0,
+					(const int64_t*)nullptr, d_qcnt, (uint64_t*)nullptr);
+			RFOK(hipGetLastError());
+			RFOK(hipMemcpyAsync(qoff.data() + 1, d_qcnt, (size_t)n_q * 8, hipMemcpyDeviceToHost, s));
+			RFOK(hipStreamSynchronize(s));
+		}
+		for (int32_t q = 0; q < n_q; ++q) qoff[q + 1] += qoff[q];
+		n_qsk = qoff[n_q];
 		RFOK(hipMalloc((void**)&d_jobs, (size_t)n * sizeof(Job)));
 		RFOK(hipMalloc((void**)&d_qoff, qoff.size() * 8));
 		RFOK(hipMalloc((void**)&d_qsk, ((size_t)n_qsk + 1) * 8));
+		RFOK(hipMalloc((void**)&d_qsk_raw, ((size_t)n_qsk + 1) * 8));
+		RFOK(hipMemcpyAsync(d_qoff, qoff.data(), qoff.size() * 8, hipMemcpyHostToDevice, s));
+		if (n_qsk > 0) {
+			hipLaunchKernelGGL(qsketch_kernel, dim3(n_q), dim3(256), 0, s, d_qs, d_qoffaa, d_qlen, d_aa13, kmer, 0,
+					(const int64_t*)d_qoff, (int64_t*)nullptr, d_qsk_raw);
+			RFOK(hipGetLastError());
+			size_t need = 0;
+			RFOK(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, need, d_qsk_raw, d_qsk, (int)n_qsk, n_q, d_qoff,
+					d_qoff + 1, 0, 64, s));
+			RFOK(hipMalloc(&d_tmp, need + 256));
+			tmp_cap = need + 256;
+			RFOK(hipcub::DeviceSegmentedRadixSort::SortKeys(d_tmp, need, d_qsk_raw, d_qsk, (int)n_qsk, n_q, d_qoff,
+					d_qoff + 1, 0, 64, s));
+		}
+		nsh_count_launch(3);
+		clk.lap("query k-mers", n_qsk);
 		RFOK(hipMalloc((void**)&d_cnt, (size_t)n * 8));
 		RFOK(hipMalloc((void**)&d_cap, (size_t)n * 8));
 		RFOK(hipMalloc((void**)&d_capoff, (size_t)n * 8));
@@ -302,8 +391,6 @@ int nsh_refine::run(const uint8_t *d_g, const std::vector<Job> &jobs, const std:
 		RFOK(hipMalloc((void**)&d_poff, ((size_t)n + 1) * 8));
 		RFOK(hipMalloc((void**)&d_rc, sizeof(RConst)));
 		RFOK(hipMemcpyAsync(d_jobs, jobs.data(), (size_t)n * sizeof(Job), hipMemcpyHostToDevice, s));
-		RFOK(hipMemcpyAsync(d_qoff, qoff.data(), qoff.size() * 8, hipMemcpyHostToDevice, s));
-		if (n_qsk > 0) RFOK(hipMemcpyAsync(d_qsk, qsk, (size_t)n_qsk * 8, hipMemcpyHostToDevice, s));
 		RFOK(hipMemcpyAsync(d_rc, &rc, sizeof(RConst), hipMemcpyHostToDevice, s));
 		// the shared window k-mers of every job, in one sweep: each job writes at most its room
 		// (its window length, at most 2,048, a few hundred being usual); a job with more is swept
@@ -414,7 +501,7 @@ done:
 	if (s) (void)hipStreamSynchronize(s);
 	{
 		void *bufs[] = { d_jobs, d_qoff, d_cnt, d_cap, d_capoff, d_wend, d_poff, d_qsk, d_w, d_ws, d_p, d_ps, d_u, d_ao, d_nu,
-			d_tmp, d_rc, d_best, d_compact, d_total, d_over };
+			d_tmp, d_rc, d_best, d_compact, d_total, d_over, d_qs, d_aa13, d_qoffaa, d_qcnt, d_qlen, d_qsk_raw };
 		for (void *b : bufs) if (b) (void)hipFree(b);
 	}
 	if (s) (void)hipStreamDestroy(s);
